@@ -1,0 +1,192 @@
+"""Benchmark: batched FP32 RK4 ensemble at 256^2 on one MI355X.
+
+Workload (synthetic, seeded): B = 1024 members at 256x256, nu = 5e-4, dt = 5e-4,
+alpha = 0, forcing 0.1 sin(4y) + white noise sigma = 0.05 on 1 <= |k| <= 4,
+GRF initial fields (E(k) ~ k^4 exp(-(k/4)^2), enstrophy 1), member seeds
+stream_seed(20261203, m).  A step is one RK4 step of every member; the state is
+resident in HBM before the timed region starts.
+
+Prints ONE JSON line with the metric, a roofline object for the dominant kernel
+(algorithmic bytes per launch / its event-timed average duration) and the CPU
+baseline: the reference's own member-parallel CPU ensemble (oracle/_ref, kind
+"reference") or, where that was not built, the FP64 oracle (kind "port"), timed
+on a bounded sample on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # bytes/s, MI355X
+SEED = 20261203
+
+
+def model_bytes(n: int, itemsize: int = 8) -> dict:
+    """Algorithmic bytes per member (complex FP32 = 8 bytes)."""
+    K = (n - 1) // 3
+    S = (K + 1) * (2 * K + 1) * itemsize  # retained half-plane state
+    M = (K + 1) * n * itemsize  # one mixed (kx spectral, y physical) field
+    return {
+        "S_r": S, "M": M,
+        # step model: 4 stages x (5 S_r + 10 M)
+        "member_step": 4 * (5 * S + 10 * M),
+        # row kernel: read 4 mixed fields, write the nonlinear mixed field
+        "row_launch": 5 * M,
+        # col kernel, stage average: read N~ + write 4 mixed fields; state
+        # traffic 3/5/5/4 S_r over stages 1..4 -> 4.25 S_r
+        "col_launch": 5 * M + 4.25 * S,
+    }
+
+
+class HipTimer:
+    """HIP events on a given stream (the stream the kernels are launched on)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def cpu_baseline(n: int, nu: float, dt: float) -> dict:
+    """Reference CPU ensemble (or the oracle port) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc_mod  # CPU baseline leg only
+
+    threads = len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, 16))
+    members, steps = 2 * threads, 8
+    if os.path.exists(orc_mod.REF_SO):
+        ref = orc_mod.Reference()
+        ref.ensemble(n, nu, 0.0, 0.05, 0.5, 1, 4, dt, 1, threads, threads, 1)  # warm
+        t0 = time.perf_counter()
+        ref.ensemble(n, nu, 0.0, 0.05, 0.5, 1, 4, dt, steps, members, threads, SEED)
+        sec = time.perf_counter() - t0
+        kind = "reference"
+        what = (f"reference hzl::ensemble (src/inference.cpp:555-590), {members} members x "
+                f"{steps} steps at {n}^2 FP64, {threads} worker threads")
+    else:
+        o = orc_mod.Oracle()
+        t0 = time.perf_counter()
+        o.ensemble(n, nu, 0.0, dt, steps, members, threads, SEED)
+        sec = time.perf_counter() - t0
+        kind = "port"
+        what = (f"FP64 oracle restatement, {members} members x {steps} steps at {n}^2, "
+                f"{threads} OpenMP threads")
+    return {"value": members * steps / sec, "unit": "member-steps/s", "cores": threads,
+            "kind": kind, "sample": what, "seconds": round(sec, 3)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--members", type=int, default=1024)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: members shard across GPUs with no "
+              "communication during forecasts, so multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        return 2
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    n, B = args.n, args.members
+    nu = dt = 5e-4
+    ens = pkg.Ensemble(n, B, "fp32", nu=nu, alpha=0.0, sigma=0.05, kmin=1, kmax=4, amp=0.1,
+                       kf=4, seed=SEED)
+    ens.init_grf()
+    ens.step(dt, args.warmup)
+    ens.synchronize()
+    timer = HipTimer()
+    stream = ens.stream
+    e0, e1 = timer.event(), timer.event()
+    wall0 = time.perf_counter()
+    timer.record(e0, stream)
+    ens.step(dt, args.steps)
+    timer.record(e1, stream)
+    ms = timer.elapsed_ms(e0, e1)
+    wall = time.perf_counter() - wall0
+    ens.synchronize()  # CFL / finiteness checks of every timed step
+    ms_per_step = ms / args.steps
+    value = B * args.steps / (ms / 1e3)
+
+    # per-kernel durations (events around each launch, outside the timed region)
+    prof = np.stack([ens.profile_step(dt) for _ in range(args.profile_steps)])
+    ens.synchronize()
+    row_ms = float(prof[:, 0::2].mean())
+    col_ms = float(prof[:, 1::2].mean())
+    mb = model_bytes(n)
+    dominant = "col" if col_ms >= row_ms else "row"
+    k_ms = col_ms if dominant == "col" else row_ms
+    k_bytes = B * (mb["col_launch"] if dominant == "col" else mb["row_launch"])
+    achieved = k_bytes / (k_ms / 1e3)
+    diag = ens.diagnostics()
+    ens.close()
+
+    line = {
+        "metric": "member RK4 steps/s @256^2 FP32",
+        "value": round(value, 1),
+        "unit": "member-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded GRF initial fields, Philox forcing noise)",
+        "config": {"workload": f"batched RK4 ensemble, {B} members at {n}x{n}, nu=dt=5e-4, "
+                               "0.1 sin(4y) + white noise 0.05 on 1<=|k|<=4",
+                   "members": B, "grid": n, "dt": dt, "nu": nu},
+        "step_model": {"bytes_per_member_step": mb["member_step"],
+                       "achieved_GBps": round(value * mb["member_step"] / 1e9, 1),
+                       "frac_of_hbm_peak": round(value * mb["member_step"] / HBM_PEAK, 4),
+                       "wall_seconds_timed": round(wall, 4)},
+        "roofline": {"bound": "hbm", "kernel": f"k_{dominant}<{n},float>",
+                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(k_bytes),
+                     "avg_launch_ms": round(k_ms, 4),
+                     "row_ms": round(row_ms, 4), "col_ms": round(col_ms, 4)},
+        "sanity": {"mean_enstrophy": float(diag[:, 1].mean()),
+                   "max_cfl_rate": float(diag[:, 4].max())},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, nu, dt)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,12 @@
+"""MI355X (gfx950) batched vorticity ensemble engine: Python host bindings.
+
+The product is the C-ABI library libhzl_amd.so (include/hzl_amd.h); this
+package only binds it.  The directory name is not a Python identifier, so it is
+loaded with `load_package()` from __graft_entry__.py (or importlib) under the
+module name `hzl_sfi_amd`.
+"""
+from .engine import (ALL, EXPORTS, FP32, FP64, Ensemble, HzlError, HzlInvalidArgument,
+                     Parameters, load_library)
+
+__all__ = ["ALL", "EXPORTS", "FP32", "FP64", "Ensemble", "HzlError", "HzlInvalidArgument",
+           "Parameters", "load_library"]

@@ -1,0 +1,74 @@
+"""Build the engine's native library and the test infrastructure.
+
+    python hzl-stochasticflowinference_amd/build.py        # everything
+
+Products (all in-tree, git-ignored, shipped to the GPU host with the tree):
+    hzl-stochasticflowinference_amd/libhzl_amd.so   C-ABI engine, HIP for gfx950
+Test infrastructure (never loaded by the product path):
+    oracle/liboracle.so          FP64 C oracle
+    oracle/_ref/libhzlref.so     unmodified reference CPU library (only where
+                                 the reference sources are present)
+    tests/emulator/libhzl_emu.so CPU lane emulator of the kernel bodies
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libhzl_amd.so")
+EMU_SRC = os.path.join(ROOT, "tests", "emulator", "emu.cpp")
+EMU_LIB = os.path.join(ROOT, "tests", "emulator", "libhzl_emu.so")
+REFERENCE = os.environ.get("HZL_REFERENCE", "/root/reference")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# Packed FP32 (v_pk_*_f32) is disabled for every HIP translation unit.
+HIP_FLAGS = ["-O3", "-std=c++20", "--offload-arch=gfx950", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function"]
+SOURCES = [os.path.join(CSRC, "engine.hip")]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, cwd=cwd, check=True)
+
+
+def build_engine() -> str:
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC, *SOURCES, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
+    if os.path.isdir(os.path.join(REFERENCE, "src")):
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref",
+              f"REFERENCE={REFERENCE}"])
+
+
+def build_emulator() -> str:
+    tmp = EMU_LIB + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++20", "--offload-host-only", "-x", "hip", "-fPIC", "-shared",
+          "-I", INCLUDE, "-I", CSRC, EMU_SRC, "-o", tmp])
+    os.replace(tmp, EMU_LIB)
+    return EMU_LIB
+
+
+def build_all() -> None:
+    build_engine()
+    build_oracle()
+    build_emulator()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "engine": build_engine, "oracle": build_oracle,
+         "emulator": build_emulator}[w]()

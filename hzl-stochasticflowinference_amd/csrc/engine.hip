@@ -1,0 +1,643 @@
+// engine.hip -- gfx950 kernels, the batched ensemble engine and its C-ABI.
+//
+// Behavioural spec: hzl::CudaSolver (src/cuda_solver.cu:86-235) and
+// hzl::Solver::step / rhs (src/solver.cpp:186-254), batched over members with
+// device-generated forcing.  See kernels.hpp for the data layout and the
+// split of one RHS into a column kernel and a row kernel.
+#include "hzl_amd.h"
+#include "tables.hpp"
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hzl_amd {
+
+struct DevCtx {
+    __host__ __device__ int tid() const {
+#if __HIP_DEVICE_COMPILE__
+        return (int)threadIdx.x;
+#else
+        return 0;
+#endif
+    }
+    __host__ __device__ void sync() const {
+#if __HIP_DEVICE_COMPILE__
+        __syncthreads();
+#endif
+    }
+};
+
+// ------------------------------------------------------------ kernels ----
+template <int N, typename R>
+__global__ void __launch_bounds__(Geometry<N>::COL_THREADS) k_col(StageArgs<R> a) {
+    using G = Geometry<N>;
+    __shared__ cx<R> lds[G::COL_G * GroupFFT<N, G::E, R>::LDS_ELEMS];
+    col_body<N, R>(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+
+template <int N, typename R>
+__global__ void __launch_bounds__(Geometry<N>::ROW_THREADS) k_row(RowArgs<R> a) {
+    using G = Geometry<N>;
+    __shared__ cx<R> lds[G::ROW_PPW * GroupFFT<N, G::E, R>::LDS_ELEMS];
+    row_body<N, R>(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+
+template <typename R> __global__ void __launch_bounds__(256) k_grf(GrfArgs<R> a) {
+    __shared__ double red[256];
+    grf_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
+}
+
+template <typename R> __global__ void __launch_bounds__(256) k_diag(DiagArgs<R> a) {
+    __shared__ double red[3 * 256];
+    diag_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
+}
+
+// ------------------------------------------------------------ helpers ----
+[[noreturn]] void hip_fail(hipError_t e, const char *what) {
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline void check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        hip_fail(e, what);
+}
+
+template <typename T> struct DeviceBuffer {
+    T *p = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t n) : count(n) {
+        if (n)
+            check(hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T)), "hipMalloc");
+    }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    DeviceBuffer(DeviceBuffer &&o) noexcept : p(o.p), count(o.count) {
+        o.p = nullptr;
+        o.count = 0;
+    }
+    DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
+        std::swap(p, o.p);
+        std::swap(count, o.count);
+        return *this;
+    }
+    ~DeviceBuffer() {
+        if (p)
+            (void)hipFree(p);
+    }
+};
+
+template <int N, typename R> void launch_col(const StageArgs<R> &a, hipStream_t s) {
+    using G = Geometry<N>;
+    const long cols = (long)a.B * G::KP;
+    const long blocks = (cols + G::COL_G - 1) / G::COL_G;
+    hipLaunchKernelGGL((k_col<N, R>), dim3((unsigned)blocks), dim3(G::COL_THREADS), 0, s, a);
+}
+
+template <int N, typename R> void launch_row(const RowArgs<R> &a, hipStream_t s) {
+    using G = Geometry<N>;
+    const long pairs = (long)a.B * (N / 2);
+    const long blocks = (pairs + G::ROW_PPW - 1) / G::ROW_PPW;
+    hipLaunchKernelGGL((k_row<N, R>), dim3((unsigned)blocks), dim3(G::ROW_THREADS), 0, s, a);
+}
+
+template <typename R> void dispatch_col(int n, const StageArgs<R> &a, hipStream_t s) {
+    switch (n) {
+    case 16:
+        return launch_col<16, R>(a, s);
+    case 32:
+        return launch_col<32, R>(a, s);
+    case 64:
+        return launch_col<64, R>(a, s);
+    case 128:
+        return launch_col<128, R>(a, s);
+    case 256:
+        return launch_col<256, R>(a, s);
+    case 512:
+        return launch_col<512, R>(a, s);
+    }
+    throw std::invalid_argument("unsupported grid");
+}
+
+template <typename R> void dispatch_row(int n, const RowArgs<R> &a, hipStream_t s) {
+    switch (n) {
+    case 16:
+        return launch_row<16, R>(a, s);
+    case 32:
+        return launch_row<32, R>(a, s);
+    case 64:
+        return launch_row<64, R>(a, s);
+    case 128:
+        return launch_row<128, R>(a, s);
+    case 256:
+        return launch_row<256, R>(a, s);
+    case 512:
+        return launch_row<512, R>(a, s);
+    }
+    throw std::invalid_argument("unsupported grid");
+}
+
+// ------------------------------------------------------------- engine ----
+class EngineBase {
+  public:
+    virtual ~EngineBase() = default;
+    virtual void set_member_parameters(size_t m, double nu, double alpha, double sigma,
+                                       double amp) = 0;
+    virtual void set_state(size_t m, const double *field) = 0;
+    virtual void get_state(size_t m, double *field) = 0;
+    virtual void init_grf() = 0;
+    virtual void step(double dt, size_t steps) = 0;
+    virtual void step_forced(double dt, const double *forcing, bool shared) = 0;
+    virtual void synchronize() = 0;
+    virtual void diagnostics(double *out) = 0;
+    virtual void profile_step(double dt, float *ms) = 0;
+    virtual uint64_t step_index() const = 0;
+    virtual void device_state(void **ptr, size_t *stride) = 0;
+    virtual void *stream() = 0;
+};
+
+template <typename R> class Engine final : public EngineBase {
+  public:
+    Engine(int n, size_t members, const hzl_parameters &p, uint64_t seed)
+        : n_(n), B_(members), K_((n - 1) / 3), KP_(K_ + 1), J_(2 * K_ + 1), p_(p) {
+        spec_ = (size_t)KP_ * J_;
+        mixf_ = (size_t)KP_ * n;
+        check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        w0_ = DeviceBuffer<cx<R>>(B_ * spec_);
+        w1_ = DeviceBuffer<cx<R>>(B_ * spec_);
+        acc_ = DeviceBuffer<cx<R>>(B_ * spec_);
+        mix_ = DeviceBuffer<cx<R>>(B_ * 4 * mixf_);
+        nl_ = DeviceBuffer<cx<R>>(B_ * mixf_);
+        ext_ = DeviceBuffer<cx<R>>(B_ * spec_);
+        cfl_ = DeviceBuffer<float>(B_ * (size_t)(n / 2));
+        cfl_diag_ = DeviceBuffer<float>(B_ * (size_t)(n / 2));
+        const auto tw = twiddles<R>(n);
+        tw_ = DeviceBuffer<cx<R>>(tw.size());
+        upload(tw_.p, tw.data(), tw.size());
+        int count = 0;
+        const auto pairs = pair_table(p.forcing_min, p.forcing_max, &count);
+        pairs_ = DeviceBuffer<int>(pairs.size());
+        upload(pairs_.p, pairs.data(), pairs.size());
+        nu_.assign(B_, p.viscosity);
+        alpha_.assign(B_, p.friction);
+        sigma_.assign(B_, p.forcing_sigma);
+        amp_.assign(B_, p.kolmogorov_amplitude);
+        keys_.resize(B_);
+        for (size_t m = 0; m < B_; ++m)
+            keys_[m] = stream_seed(seed, m);
+        d_nu_ = DeviceBuffer<double>(B_);
+        d_alpha_ = DeviceBuffer<double>(B_);
+        d_sigma_ = DeviceBuffer<double>(B_);
+        d_amp_ = DeviceBuffer<double>(B_);
+        d_keys_ = DeviceBuffer<uint64_t>(B_);
+        upload(d_keys_.p, keys_.data(), B_);
+        push_parameters();
+        check(hipMemsetAsync(w0_.p, 0, B_ * spec_ * sizeof(cx<R>), stream_), "clear state");
+        check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(float), stream_), "clear CFL");
+        check(hipStreamSynchronize(stream_), "engine setup");
+    }
+    ~Engine() override {
+        if (stream_) {
+            (void)hipStreamSynchronize(stream_);
+            (void)hipStreamDestroy(stream_);
+        }
+    }
+
+    void set_member_parameters(size_t m, double nu, double alpha, double sigma,
+                               double amp) override {
+        member_check(m);
+        if (!std::isfinite(nu) || nu < 0 || !std::isfinite(alpha) || alpha < 0 ||
+            !std::isfinite(sigma) || sigma < 0 || !std::isfinite(amp))
+            throw std::invalid_argument("nu, alpha, sigma must be finite and nonnegative");
+        nu_[m] = nu;
+        alpha_[m] = alpha;
+        sigma_[m] = sigma;
+        amp_[m] = amp;
+        params_dirty_ = true;
+    }
+
+    void set_state(size_t m, const double *field) override {
+        const size_t first = m == (size_t)-1 ? 0 : m, count = m == (size_t)-1 ? B_ : 1;
+        if (m != (size_t)-1)
+            member_check(m);
+        const size_t nn = (size_t)n_ * n_;
+        for (size_t i = 0; i < 2 * nn * count; ++i)
+            if (!std::isfinite(field[i]))
+                throw std::invalid_argument("nonfinite initial GPU state");
+        std::vector<cx<R>> half(spec_ * count);
+        for (size_t k = 0; k < count; ++k)
+            full_to_half<R>(n_, field + 2 * nn * k, half.data() + spec_ * k);
+        upload(w0_.p + first * spec_, half.data(), half.size());
+        mix_valid_ = false;
+    }
+
+    void get_state(size_t m, double *field) override {
+        const size_t first = m == (size_t)-1 ? 0 : m, count = m == (size_t)-1 ? B_ : 1;
+        if (m != (size_t)-1)
+            member_check(m);
+        std::vector<cx<R>> half(spec_ * count);
+        check(hipMemcpyAsync(half.data(), w0_.p + first * spec_, half.size() * sizeof(cx<R>),
+                             hipMemcpyDeviceToHost, stream_),
+              "state download");
+        check(hipStreamSynchronize(stream_), "state download");
+        for (const auto &v : half)
+            if (!std::isfinite((double)v.x) || !std::isfinite((double)v.y))
+                throw std::runtime_error("nonfinite GPU result");
+        const size_t nn = (size_t)n_ * n_;
+        for (size_t k = 0; k < count; ++k)
+            half_to_full<R>(n_, half.data() + spec_ * k, field + 2 * nn * k);
+    }
+
+    void init_grf() override {
+        GrfArgs<R> a{K_, w0_.p, d_keys_.p};
+        hipLaunchKernelGGL(k_grf<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
+        check(hipGetLastError(), "GRF kernel");
+        mix_valid_ = false;
+    }
+
+    void step(double dt, size_t steps) override {
+        validate_dt(dt);
+        for (size_t s = 0; s < steps; ++s)
+            advance(dt, nullptr, 0);
+    }
+
+    void step_forced(double dt, const double *forcing, bool shared) override {
+        validate_dt(dt);
+        const size_t nn = (size_t)n_ * n_, count = shared ? 1 : B_;
+        for (size_t k = 0; k < count; ++k) {
+            const double *F = forcing + 2 * nn * k;
+            for (size_t i = 0; i < nn; ++i) {
+                const double re = F[2 * i], im = F[2 * i + 1];
+                if (!std::isfinite(re) || !std::isfinite(im))
+                    throw std::invalid_argument("nonfinite GPU forcing");
+                const size_t x = i % n_, y = i / n_;
+                const size_t j = ((n_ - y) % n_) * n_ + (n_ - x) % n_;
+                const double dr = re - F[2 * j], di = im + F[2 * j + 1];
+                if (std::sqrt(dr * dr + di * di) > 1e-12 * (1 + std::hypot(re, im)))
+                    throw std::invalid_argument("GPU forcing must preserve conjugate symmetry");
+            }
+        }
+        std::vector<cx<R>> half(spec_ * count);
+        for (size_t k = 0; k < count; ++k)
+            full_to_half<R>(n_, forcing + 2 * nn * k, half.data() + spec_ * k);
+        upload(ext_.p, half.data(), half.size());
+        advance(dt, ext_.p, shared ? 0 : (long long)spec_);
+    }
+
+    void synchronize() override {
+        check(hipStreamSynchronize(stream_), "synchronize GPU");
+        check(hipGetLastError(), "GPU kernels");
+        std::vector<float> c(cfl_.count);
+        check(hipMemcpy(c.data(), cfl_.p, c.size() * sizeof(float), hipMemcpyDeviceToHost),
+              "CFL download");
+        float mx = 0;
+        bool bad = false;
+        for (float v : c) {
+            if (!std::isfinite(v))
+                bad = true;
+            else
+                mx = std::max(mx, v);
+        }
+        check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(float), stream_), "reset CFL");
+        check(hipStreamSynchronize(stream_), "reset CFL");
+        const double rate = (double)mx * n_ / (2 * PI);
+        last_cfl_rate_ = rate;
+        const double dt = max_dt_;
+        max_dt_ = 0;
+        if (bad)
+            throw std::runtime_error("nonfinite flow state");
+        if (dt * rate > 0.5)
+            throw std::runtime_error("GPU CFL limit exceeded");
+    }
+
+    void diagnostics(double *out) override {
+        flush_parameters();
+        // CFL rate of the current state (src/solver.cpp:392-396): one row pass
+        // over the mixed fields of the current state into a private buffer.
+        if (!mix_valid_) {
+            dispatch_col<R>(n_, stage_args(0, 1.0, nullptr, 0), stream_);
+            mix_valid_ = true;
+        }
+        check(hipMemsetAsync(cfl_diag_.p, 0, cfl_diag_.count * sizeof(float), stream_),
+              "clear CFL");
+        RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_.p, cfl_diag_.p};
+        dispatch_row<R>(n_, r, stream_);
+        DeviceBuffer<double> d(B_ * 6);
+        ForcingModel f = forcing_model(step_ > 0 ? last_dt_ : 1.0);
+        DiagArgs<R> a{K_, w0_.p, d_nu_.p, f, step_ > 0 ? step_ - 1 : 0, step_ > 0 ? 1 : 0,
+                      cfl_diag_.p, n_ / 2, n_, d.p};
+        hipLaunchKernelGGL(k_diag<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
+        check(hipGetLastError(), "diagnostics kernel");
+        check(hipMemcpyAsync(out, d.p, B_ * 6 * sizeof(double), hipMemcpyDeviceToHost, stream_),
+              "diagnostics download");
+        check(hipStreamSynchronize(stream_), "diagnostics");
+    }
+
+    void profile_step(double dt, float *ms) override {
+        validate_dt(dt);
+        hipEvent_t ev[9];
+        for (auto &e : ev)
+            check(hipEventCreate(&e), "hipEventCreate");
+        advance(dt, nullptr, 0, ev);
+        check(hipEventSynchronize(ev[8]), "profile step");
+        for (int i = 0; i < 8; ++i)
+            check(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]), "hipEventElapsedTime");
+        for (auto &e : ev)
+            (void)hipEventDestroy(e);
+    }
+
+    uint64_t step_index() const override {
+        return step_;
+    }
+
+    void device_state(void **ptr, size_t *stride) override {
+        *ptr = w0_.p;
+        *stride = spec_ * sizeof(cx<R>);
+    }
+
+    void *stream() override {
+        return stream_;
+    }
+
+  private:
+    void member_check(size_t m) const {
+        if (m >= B_)
+            throw std::invalid_argument("member index out of range");
+    }
+
+    template <typename T> void upload(T *dst, const T *src, size_t count) {
+        check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, stream_),
+              "upload");
+        check(hipStreamSynchronize(stream_), "upload");
+    }
+
+    void push_parameters() {
+        upload(d_nu_.p, nu_.data(), B_);
+        upload(d_alpha_.p, alpha_.data(), B_);
+        upload(d_sigma_.p, sigma_.data(), B_);
+        upload(d_amp_.p, amp_.data(), B_);
+        params_dirty_ = false;
+    }
+
+    void flush_parameters() {
+        if (params_dirty_)
+            push_parameters();
+    }
+
+    // Explicit-step guard of CudaSolver::advance (src/cuda_solver.cu:203-208):
+    // dt * max_retained(nu k^2 + alpha) <= 2.5 for every member.
+    void validate_dt(double dt) {
+        if (!std::isfinite(dt) || dt <= 0)
+            throw std::invalid_argument("invalid GPU time step or diffusion stability limit exceeded");
+        const double k2 = 2.0 * K_ * K_;
+        for (size_t m = 0; m < B_; ++m)
+            if (dt * (nu_[m] * k2 + alpha_[m]) > 2.5)
+                throw std::invalid_argument(
+                    "invalid GPU time step or diffusion stability limit exceeded");
+    }
+
+    ForcingModel forcing_model(double dt) const {
+        return ForcingModel{d_sigma_.p,        d_amp_.p,          d_keys_.p,
+                            pairs_.p,          p_.forcing_min,    p_.forcing_max,
+                            p_.kolmogorov_wavenumber, dt};
+    }
+
+    StageArgs<R> stage_args(int mode, double dt, const cx<R> *ext, long long ext_stride) const {
+        StageArgs<R> a;
+        a.B = (int)B_;
+        a.mode = mode;
+        a.w0 = w0_.p;
+        a.w1 = w1_.p;
+        a.acc = acc_.p;
+        a.nl = nl_.p;
+        a.mix = mix_.p;
+        a.tw = tw_.p;
+        a.nu = d_nu_.p;
+        a.alpha = d_alpha_.p;
+        a.forcing = forcing_model(dt);
+        a.ext = ext;
+        a.ext_stride = ext_stride;
+        a.h_half = (R)(dt / 2);
+        a.h_full = (R)dt;
+        a.h_sixth = (R)(dt / 6);
+        a.step = step_;
+        return a;
+    }
+
+    // One RK4 step: 4 x (row, col).  With `events` (9 entries) the launches
+    // are bracketed by events so their individual durations can be read.
+    void advance(double dt, const cx<R> *ext, long long ext_stride,
+                 hipEvent_t *events = nullptr) {
+        flush_parameters();
+        if (!mix_valid_) {
+            dispatch_col<R>(n_, stage_args(0, dt, ext, ext_stride), stream_);
+            mix_valid_ = true;
+        }
+        RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_.p, nullptr};
+        if (events)
+            check(hipEventRecord(events[0], stream_), "event");
+        for (int stage = 1; stage <= 4; ++stage) {
+            r.cfl = stage == 1 ? cfl_.p : nullptr;
+            dispatch_row<R>(n_, r, stream_);
+            if (events)
+                check(hipEventRecord(events[2 * stage - 1], stream_), "event");
+            dispatch_col<R>(n_, stage_args(stage, dt, ext, ext_stride), stream_);
+            if (events)
+                check(hipEventRecord(events[2 * stage], stream_), "event");
+        }
+        check(hipGetLastError(), "RK4 kernels");
+        ++step_;
+        last_dt_ = dt;
+        max_dt_ = std::max(max_dt_, dt);
+    }
+
+    int n_;
+    size_t B_;
+    int K_, KP_, J_;
+    hzl_parameters p_;
+    size_t spec_ = 0, mixf_ = 0;
+    hipStream_t stream_ = nullptr;
+    DeviceBuffer<cx<R>> w0_, w1_, acc_, mix_, nl_, ext_, tw_;
+    DeviceBuffer<float> cfl_, cfl_diag_;
+    DeviceBuffer<int> pairs_;
+    DeviceBuffer<double> d_nu_, d_alpha_, d_sigma_, d_amp_;
+    DeviceBuffer<uint64_t> d_keys_;
+    std::vector<double> nu_, alpha_, sigma_, amp_;
+    std::vector<uint64_t> keys_;
+    bool params_dirty_ = false, mix_valid_ = false;
+    uint64_t step_ = 0;
+    double last_dt_ = 0, max_dt_ = 0, last_cfl_rate_ = 0;
+};
+
+// Parameters::validate (src/solver.cpp:31-40) plus the engine's own limits.
+void validate(size_t n, size_t members, int precision, const hzl_parameters &p) {
+    (void)elems_per_thread((int)n);
+    if (members == 0)
+        throw std::invalid_argument("ensemble needs at least one member");
+    if (precision != HZL_FP32 && precision != HZL_FP64)
+        throw std::invalid_argument("precision must be HZL_FP32 or HZL_FP64");
+    if (!std::isfinite(p.viscosity) || p.viscosity < 0 || !std::isfinite(p.friction) ||
+        p.friction < 0 || !std::isfinite(p.forcing_sigma) || p.forcing_sigma < 0 ||
+        !std::isfinite(p.forcing_tau) || p.forcing_tau <= 0)
+        throw std::invalid_argument(
+            "nu, alpha, sigma must be finite and nonnegative; tau must be positive");
+    if (p.forcing_min < 1 || p.forcing_max < p.forcing_min ||
+        3 * static_cast<long long>(p.forcing_max) >= static_cast<long long>(n))
+        throw std::invalid_argument("forcing shell must lie strictly inside the dealiased grid");
+    if (!std::isfinite(p.kolmogorov_amplitude) || p.kolmogorov_wavenumber < 0 ||
+        3LL * p.kolmogorov_wavenumber >= (long long)n)
+        throw std::invalid_argument("Kolmogorov forcing must lie inside the dealiased grid");
+}
+
+} // namespace hzl_amd
+
+// ---------------------------------------------------------------- C-ABI --
+struct hzl_ensemble {
+    std::unique_ptr<hzl_amd::EngineBase> engine;
+};
+
+namespace {
+thread_local std::string g_error;
+
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        g_error.clear();
+        return HZL_OK;
+    } catch (const std::invalid_argument &e) {
+        g_error = e.what();
+        return HZL_EINVAL;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return HZL_ERUNTIME;
+    } catch (...) {
+        g_error = "unknown error";
+        return HZL_ERUNTIME;
+    }
+}
+
+hzl_amd::EngineBase &engine_of(hzl_ensemble_t h) {
+    if (!h || !h->engine)
+        throw std::invalid_argument("null ensemble handle");
+    return *h->engine;
+}
+} // namespace
+
+extern "C" {
+
+const char *hzl_last_error(void) {
+    return g_error.c_str();
+}
+
+int hzl_ensemble_create(hzl_ensemble_t *out, size_t n, size_t members, int precision,
+                        const hzl_parameters *params, uint64_t seed) {
+    return guarded([&] {
+        if (!out || !params)
+            throw std::invalid_argument("null argument");
+        *out = nullptr;
+        hzl_amd::validate(n, members, precision, *params);
+        auto h = std::make_unique<hzl_ensemble>();
+        if (precision == HZL_FP32)
+            h->engine = std::make_unique<hzl_amd::Engine<float>>((int)n, members, *params, seed);
+        else
+            h->engine =
+                std::make_unique<hzl_amd::Engine<double>>((int)n, members, *params, seed);
+        *out = h.release();
+    });
+}
+
+int hzl_ensemble_destroy(hzl_ensemble_t h) {
+    return guarded([&] { delete h; });
+}
+
+int hzl_ensemble_set_member_parameters(hzl_ensemble_t h, size_t member, double viscosity,
+                                       double friction, double forcing_sigma,
+                                       double kolmogorov_amplitude) {
+    return guarded([&] {
+        engine_of(h).set_member_parameters(member, viscosity, friction, forcing_sigma,
+                                           kolmogorov_amplitude);
+    });
+}
+
+int hzl_ensemble_set_state(hzl_ensemble_t h, size_t member, const double *field) {
+    return guarded([&] {
+        if (!field)
+            throw std::invalid_argument("null field");
+        engine_of(h).set_state(member, field);
+    });
+}
+
+int hzl_ensemble_get_state(hzl_ensemble_t h, size_t member, double *field) {
+    return guarded([&] {
+        if (!field)
+            throw std::invalid_argument("null field");
+        engine_of(h).get_state(member, field);
+    });
+}
+
+int hzl_ensemble_init_grf(hzl_ensemble_t h) {
+    return guarded([&] { engine_of(h).init_grf(); });
+}
+
+int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps) {
+    return guarded([&] { engine_of(h).step(dt, steps); });
+}
+
+int hzl_ensemble_step_forced(hzl_ensemble_t h, double dt, const double *forcing, int shared) {
+    return guarded([&] {
+        if (!forcing)
+            throw std::invalid_argument("null forcing");
+        engine_of(h).step_forced(dt, forcing, shared != 0);
+    });
+}
+
+int hzl_ensemble_synchronize(hzl_ensemble_t h) {
+    return guarded([&] { engine_of(h).synchronize(); });
+}
+
+int hzl_ensemble_diagnostics(hzl_ensemble_t h, double *out) {
+    return guarded([&] {
+        if (!out)
+            throw std::invalid_argument("null output");
+        engine_of(h).diagnostics(out);
+    });
+}
+
+int hzl_ensemble_profile_step(hzl_ensemble_t h, double dt, float *kernel_ms) {
+    return guarded([&] {
+        if (!kernel_ms)
+            throw std::invalid_argument("null output");
+        engine_of(h).profile_step(dt, kernel_ms);
+    });
+}
+
+int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out) {
+    return guarded([&] {
+        if (!out)
+            throw std::invalid_argument("null output");
+        *out = engine_of(h).step_index();
+    });
+}
+
+int hzl_ensemble_device_state(hzl_ensemble_t h, void **ptr, size_t *member_stride_bytes) {
+    return guarded([&] {
+        if (!ptr || !member_stride_bytes)
+            throw std::invalid_argument("null output");
+        engine_of(h).device_state(ptr, member_stride_bytes);
+    });
+}
+
+int hzl_ensemble_stream(hzl_ensemble_t h, void **stream) {
+    return guarded([&] {
+        if (!stream)
+            throw std::invalid_argument("null output");
+        *stream = engine_of(h).stream();
+    });
+}
+}

@@ -1,0 +1,213 @@
+// fft_lds.hpp -- register + LDS FFT building blocks for gfx950 (CDNA4).
+//
+// A length-N transform is owned by a group of T = N/E threads that each hold E
+// complex values in registers.  It runs as two passes (the "four-step"
+// factorisation N = T * E, Van Loan 1992 sec. 3.3; Stockham-style autosort so
+// no bit reversal is ever materialised):
+//
+//   forward  in : v[j]          = x[t + T*j]                (j < E)
+//            pass 1: E-point DFT in registers over j, twiddle W_N^{t*k2}
+//            LDS exchange (one write, one read, padded stride T+1)
+//            pass 2: E/T T-point DFTs in registers
+//            out: v[r*T + k1]  = X[(t + T*r) + E*k1]       (r < E/T, k1 < T)
+//   inverse  takes the forward output ordering and returns the forward input
+//            ordering, so spectral work between a forward and an inverse
+//            transform never touches LDS.
+//
+// Small DFTs (E, T <= 32) are unrolled radix-2/4 with compile-time twiddles;
+// the large twiddles W_N^{t*k2} come from a table computed on the host in
+// double precision and rounded once to the working precision.
+//
+// Everything here is __host__ __device__ and synchronises through a context
+// object (`ctx.sync()`), so the same code also runs in the CPU lane emulator
+// used by the tests.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define HZL_HD __host__ __device__ __forceinline__
+
+namespace hzl_amd {
+
+template <typename R> struct alignas(2 * sizeof(R)) cx {
+    R x, y;
+};
+
+template <typename R> HZL_HD cx<R> cadd(cx<R> a, cx<R> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+template <typename R> HZL_HD cx<R> csub(cx<R> a, cx<R> b) {
+    return {a.x - b.x, a.y - b.y};
+}
+template <typename R> HZL_HD cx<R> cmul(cx<R> a, cx<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename R> HZL_HD cx<R> cmulc(cx<R> a, cx<R> b) {
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename R> HZL_HD cx<R> cscale(cx<R> a, R s) {
+    return {a.x * s, a.y * s};
+}
+// i * a and -i * a
+template <typename R> HZL_HD cx<R> mul_i(cx<R> a) {
+    return {-a.y, a.x};
+}
+template <typename R> HZL_HD cx<R> mul_mi(cx<R> a) {
+    return {a.y, -a.x};
+}
+
+// cos(2 pi m / 64) for any integer m (quarter-wave table + symmetry).
+HZL_HD constexpr double cos64(int m) {
+    constexpr double q[17] = {1.0,
+                              0.995184726672196886244837,
+                              0.9807852804032304491261822,
+                              0.9569403357322088649357979,
+                              0.9238795325112867561281832,
+                              0.8819212643483550297127569,
+                              0.8314696123025452370787884,
+                              0.7730104533627369608109066,
+                              0.7071067811865475244008444,
+                              0.6343932841636454982151716,
+                              0.5555702330196022247428308,
+                              0.4713967368259976485563876,
+                              0.38268343236508977172846,
+                              0.2902846772544623676361924,
+                              0.1950903220161282678482849,
+                              0.09801714032956060199419556,
+                              0.0};
+    m = ((m % 64) + 64) % 64;
+    if (m <= 16)
+        return q[m];
+    if (m <= 32)
+        return -q[32 - m];
+    if (m <= 48)
+        return -q[m - 32];
+    return q[64 - m];
+}
+
+// a * W_L^k with W_L = exp(-2 pi i / L) (forward) or its conjugate (inverse).
+// k is a compile-time constant after unrolling, so the branches fold away.
+template <int L, bool INV, typename R> HZL_HD cx<R> twiddle_const(int k, cx<R> a) {
+    static_assert(L <= 64 && (64 % L) == 0, "twiddle_const supports L | 64");
+    if (k == 0)
+        return a;
+    if (4 * k == L)
+        return INV ? mul_i(a) : mul_mi(a);
+    const int m = k * (64 / L);
+    const R c = static_cast<R>(cos64(m));
+    const R s = static_cast<R>(cos64(m - 16)); // sin(2 pi m / 64)
+    if (8 * k == L || 8 * k == 3 * L) {
+        // |c| == |s| == 1/sqrt(2): sign flips fold, two multiplies instead of four
+        const R h = static_cast<R>(0.7071067811865475244008444);
+        const R sc = c > 0 ? R(1) : R(-1), ss = s > 0 ? R(1) : R(-1);
+        if (INV)
+            return {(sc * a.x - ss * a.y) * h, (sc * a.y + ss * a.x) * h};
+        return {(sc * a.x + ss * a.y) * h, (sc * a.y - ss * a.x) * h};
+    }
+    if (INV)
+        return {a.x * c - a.y * s, a.y * c + a.x * s};
+    return {a.x * c + a.y * s, a.y * c - a.x * s};
+}
+
+// In-register DFT of L values, natural order in and out.
+template <int L, bool INV, typename R> HZL_HD void dft(cx<R> *v) {
+    if constexpr (L == 1) {
+    } else if constexpr (L == 2) {
+        const cx<R> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (L == 4) {
+        const cx<R> s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+        const cx<R> s1 = cadd(v[1], v[3]), d1 = csub(v[1], v[3]);
+        const cx<R> t = INV ? mul_i(d1) : mul_mi(d1);
+        v[0] = cadd(s0, s1);
+        v[2] = csub(s0, s1);
+        v[1] = cadd(d0, t);
+        v[3] = csub(d0, t);
+    } else {
+        cx<R> e[L / 2], o[L / 2];
+#pragma unroll
+        for (int i = 0; i < L / 2; ++i) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        dft<L / 2, INV>(e);
+        dft<L / 2, INV>(o);
+#pragma unroll
+        for (int k = 0; k < L / 2; ++k) {
+            const cx<R> t = twiddle_const<L, INV>(k, o[k]);
+            v[k] = cadd(e[k], t);
+            v[k + L / 2] = csub(e[k], t);
+        }
+    }
+}
+
+// Two-pass transform of length N by a group of T = N/E threads.
+// `lds` is the group's private region of at least LDS_ELEMS entries.
+// `tw[t*E + k2]` = exp(-2 pi i t k2 / N) for t < T, k2 < E.
+// ctx.sync() is a workgroup-wide barrier: every thread of the workgroup must
+// call forward/inverse the same number of times (idle groups included).
+template <int N, int E, typename R> struct GroupFFT {
+    static constexpr int T = N / E;
+    static constexpr int Q = E / T;
+    static constexpr int LD = T + 1;
+    static constexpr int LDS_ELEMS = E * LD;
+    static_assert(T * E == N && T >= 1 && (E % T) == 0, "need N = T*E with T | E");
+    static_assert(LDS_ELEMS >= N, "LDS region must hold one natural-order line");
+
+    // Spectral index held in register slot idx by thread t (forward output order).
+    HZL_HD static constexpr int index(int t, int idx) {
+        return t + T * (idx / T) + E * (idx % T);
+    }
+
+    template <class Ctx>
+    HZL_HD static void forward(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const cx<R> *tw) {
+        dft<E, false>(v);
+#pragma unroll
+        for (int k2 = 1; k2 < E; ++k2)
+            v[k2] = cmul(v[k2], tw[t * E + k2]);
+#pragma unroll
+        for (int k2 = 0; k2 < E; ++k2)
+            lds[k2 * LD + t] = v[k2];
+        ctx.sync();
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            const int k2 = t + T * r;
+            cx<R> u[T];
+#pragma unroll
+            for (int n1 = 0; n1 < T; ++n1)
+                u[n1] = lds[k2 * LD + n1];
+            dft<T, false>(u);
+#pragma unroll
+            for (int k1 = 0; k1 < T; ++k1)
+                v[r * T + k1] = u[k1];
+        }
+        ctx.sync();
+    }
+
+    template <class Ctx>
+    HZL_HD static void inverse(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const cx<R> *tw) {
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            const int k2 = t + T * r;
+            cx<R> u[T];
+#pragma unroll
+            for (int k1 = 0; k1 < T; ++k1)
+                u[k1] = v[r * T + k1];
+            dft<T, true>(u);
+#pragma unroll
+            for (int n1 = 0; n1 < T; ++n1)
+                lds[k2 * LD + n1] = n1 == 0 ? u[n1] : cmulc(u[n1], tw[n1 * E + k2]);
+        }
+        ctx.sync();
+#pragma unroll
+        for (int k2 = 0; k2 < E; ++k2)
+            v[k2] = lds[k2 * LD + t];
+        dft<E, true>(v);
+        ctx.sync();
+    }
+};
+
+} // namespace hzl_amd

@@ -1,0 +1,113 @@
+// tables.hpp -- host-side tables and layout conversions shared by the engine
+// and its tests.
+#pragma once
+#include "kernels.hpp"
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hzl_amd {
+
+// SplitMix64-style logical stream identity (reference src/solver.cpp:61-66).
+inline uint64_t stream_seed(uint64_t base, uint64_t index) {
+    uint64_t x = base + 0x9e3779b97f4a7c15ULL * (index + 1);
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+inline int wave_number(long i, long n) {
+    return i < n / 2 ? (int)i : (int)(i - n);
+}
+
+inline int elems_per_thread(int n) {
+    switch (n) {
+    case 16:
+        return Shape<16>::E;
+    case 32:
+        return Shape<32>::E;
+    case 64:
+        return Shape<64>::E;
+    case 128:
+        return Shape<128>::E;
+    case 256:
+        return Shape<256>::E;
+    case 512:
+        return Shape<512>::E;
+    }
+    throw std::invalid_argument("grid must be a power of two between 16 and 512");
+}
+
+// tw[t*E + k2] = exp(-2 pi i t k2 / n), evaluated in double, rounded once.
+template <typename R> std::vector<cx<R>> twiddles(int n) {
+    const int E = elems_per_thread(n), T = n / E;
+    std::vector<cx<R>> tw((size_t)n);
+    for (int t = 0; t < T; ++t)
+        for (int k2 = 0; k2 < E; ++k2) {
+            const long p = ((long)t * k2) % n;
+            const double a = -2.0 * PI * (double)p / (double)n;
+            tw[(size_t)(t * E + k2)] = {(R)std::cos(a), (R)std::sin(a)};
+        }
+    return tw;
+}
+
+// Canonical shell pairs kmin^2 <= |k|^2 <= kmax^2 of the half plane (kx > 0,
+// or kx == 0 and ky > 0), numbered kx-major then ky; -1 elsewhere.  The pair
+// set is the reference's forcing_pairs (src/solver.cpp:67-83).
+inline std::vector<int> pair_table(int kmin, int kmax, int *count = nullptr) {
+    const int w = 2 * kmax + 1;
+    std::vector<int> table((size_t)((kmax + 1) * w), -1);
+    int p = 0;
+    for (int a = 0; a <= kmax; ++a)
+        for (int b = -kmax; b <= kmax; ++b) {
+            const int k2 = a * a + b * b;
+            if ((a > 0 || b > 0) && k2 >= kmin * kmin && k2 <= kmax * kmax)
+                table[(size_t)(a * w + b + kmax)] = p++;
+        }
+    if (count)
+        *count = p;
+    return table;
+}
+
+// Full reference Field (n*n complex, y*n + x) -> retained half plane [c][j].
+template <typename R>
+void full_to_half(int n, const double *full, cx<R> *half) {
+    const int K = (n - 1) / 3, J = 2 * K + 1;
+    for (int c = 0; c <= K; ++c)
+        for (int ky = -K; ky <= K; ++ky) {
+            const long y = ((ky % n) + n) % n;
+            const long i = y * n + c;
+            cx<R> v = {(R)full[2 * i], (R)full[2 * i + 1]};
+            if (c == 0 && ky == 0)
+                v = {0, 0};
+            half[c * J + ky + K] = v;
+        }
+}
+
+// Retained half plane -> full reference Field (conjugate mirror for kx > 0,
+// zero outside the mask).
+template <typename R>
+void half_to_full(int n, const cx<R> *half, double *full) {
+    const int K = (n - 1) / 3, J = 2 * K + 1;
+    for (long i = 0; i < 2L * n * n; ++i)
+        full[i] = 0;
+    for (int c = 0; c <= K; ++c)
+        for (int ky = -K; ky <= K; ++ky) {
+            const cx<R> v = half[c * J + ky + K];
+            const long y = ((ky % n) + n) % n;
+            const long i = y * n + c;
+            full[2 * i] = (double)v.x;
+            full[2 * i + 1] = (double)v.y;
+            if (c > 0) {
+                const long my = ((-ky % n) + n) % n;
+                const long j = my * n + (n - c);
+                full[2 * j] = (double)v.x;
+                full[2 * j + 1] = -(double)v.y;
+            }
+        }
+}
+
+} // namespace hzl_amd

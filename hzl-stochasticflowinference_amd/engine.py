@@ -1,0 +1,195 @@
+"""Python binding of the C-ABI (include/hzl_amd.h) over ctypes.
+
+Mirrors the reference's solver interfaces for tests and benchmarks:
+  Ensemble      -- a batch of hzl::Solver / hzl::CudaSolver members
+                   (include/hzl/solver.hpp:61-117, include/hzl/cuda_solver.hpp:7-22)
+There is no CPU fallback: constructing an Ensemble without the built HIP
+library, or without a GPU, raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libhzl_amd.so")
+
+HZL_OK, HZL_EINVAL, HZL_ERUNTIME = 0, 1, 2
+FP32, FP64 = 0, 1
+ALL = C.c_size_t(-1).value  # (size_t)-1: every member
+
+# Every entry point declared in include/hzl_amd.h.
+EXPORTS = (
+    "hzl_last_error", "hzl_ensemble_create", "hzl_ensemble_destroy",
+    "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
+    "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_step_forced",
+    "hzl_ensemble_synchronize", "hzl_ensemble_diagnostics", "hzl_ensemble_profile_step",
+    "hzl_ensemble_step_index",
+    "hzl_ensemble_device_state", "hzl_ensemble_stream",
+)
+
+
+class Parameters(C.Structure):
+    """hzl_parameters; first six fields mirror hzl::Parameters (include/hzl/solver.hpp:15-23)."""
+    _fields_ = [("viscosity", C.c_double), ("friction", C.c_double),
+                ("forcing_sigma", C.c_double), ("forcing_tau", C.c_double),
+                ("forcing_min", C.c_int), ("forcing_max", C.c_int),
+                ("kolmogorov_amplitude", C.c_double), ("kolmogorov_wavenumber", C.c_int)]
+
+
+class HzlError(RuntimeError):
+    pass
+
+
+class HzlInvalidArgument(HzlError, ValueError):
+    """std::invalid_argument of the reference."""
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libhzl_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} not built: run `python hzl-stochasticflowinference_amd/build.py engine`")
+    L = C.CDLL(path)
+    L.hzl_last_error.restype = C.c_char_p
+    vp = C.c_void_p
+    L.hzl_ensemble_create.argtypes = [C.POINTER(vp), C.c_size_t, C.c_size_t, C.c_int,
+                                      C.POINTER(Parameters), C.c_uint64]
+    L.hzl_ensemble_destroy.argtypes = [vp]
+    L.hzl_ensemble_set_member_parameters.argtypes = [vp, C.c_size_t, C.c_double, C.c_double,
+                                                     C.c_double, C.c_double]
+    L.hzl_ensemble_set_state.argtypes = [vp, C.c_size_t, _dp]
+    L.hzl_ensemble_get_state.argtypes = [vp, C.c_size_t, _dp]
+    L.hzl_ensemble_init_grf.argtypes = [vp]
+    L.hzl_ensemble_step.argtypes = [vp, C.c_double, C.c_size_t]
+    L.hzl_ensemble_step_forced.argtypes = [vp, C.c_double, _dp, C.c_int]
+    L.hzl_ensemble_synchronize.argtypes = [vp]
+    L.hzl_ensemble_diagnostics.argtypes = [vp, _dp]
+    L.hzl_ensemble_profile_step.argtypes = [vp, C.c_double, C.POINTER(C.c_float)]
+    L.hzl_ensemble_step_index.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.hzl_ensemble_device_state.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_size_t)]
+    L.hzl_ensemble_stream.argtypes = [vp, C.POINTER(vp)]
+    for name in EXPORTS[1:]:
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc == HZL_OK:
+        return
+    msg = _lib.hzl_last_error().decode()
+    if rc == HZL_EINVAL:
+        raise HzlInvalidArgument(msg)
+    raise HzlError(msg)
+
+
+def _fields(a, count: int, n: int) -> np.ndarray:
+    arr = np.ascontiguousarray(a, dtype=np.complex128)
+    if arr.size != count * n * n:
+        raise ValueError(f"expected {count} field(s) of {n}x{n}")
+    return arr
+
+
+class Ensemble:
+    """A batch of independent members on one n x n grid (one engine handle)."""
+
+    def __init__(self, n: int, members: int, precision: str = "fp32", *, nu: float = 1e-3,
+                 alpha: float = 0.0, sigma: float = 0.05, tau: float = 0.5, kmin: int = 1,
+                 kmax: int = 4, amp: float = 0.1, kf: int = 4, seed: int = 0):
+        L = load_library()
+        self.n, self.members = int(n), int(members)
+        self.precision = {"fp32": FP32, "fp64": FP64}[precision]
+        p = Parameters(nu, alpha, sigma, tau, kmin, kmax, amp, kf)
+        h = C.c_void_p()
+        _check(L.hzl_ensemble_create(C.byref(h), self.n, self.members, self.precision,
+                                     C.byref(p), seed))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.hzl_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_member_parameters(self, m, nu, alpha, sigma, amp):
+        _check(_lib.hzl_ensemble_set_member_parameters(self._h, m, nu, alpha, sigma, amp))
+
+    def set_state(self, fields, member=None):
+        if member is None:
+            a = _fields(fields, self.members, self.n)
+            _check(_lib.hzl_ensemble_set_state(self._h, ALL, a.ctypes.data_as(_dp)))
+        else:
+            a = _fields(fields, 1, self.n)
+            _check(_lib.hzl_ensemble_set_state(self._h, member, a.ctypes.data_as(_dp)))
+
+    def get_state(self, member=None) -> np.ndarray:
+        if member is None:
+            out = np.zeros((self.members, self.n, self.n), dtype=np.complex128)
+            _check(_lib.hzl_ensemble_get_state(self._h, ALL, out.ctypes.data_as(_dp)))
+        else:
+            out = np.zeros((self.n, self.n), dtype=np.complex128)
+            _check(_lib.hzl_ensemble_get_state(self._h, member, out.ctypes.data_as(_dp)))
+        return out
+
+    def init_grf(self):
+        _check(_lib.hzl_ensemble_init_grf(self._h))
+
+    def step(self, dt: float, steps: int = 1):
+        _check(_lib.hzl_ensemble_step(self._h, dt, steps))
+
+    def step_forced(self, dt: float, forcing, shared: bool = True):
+        a = _fields(forcing, 1 if shared else self.members, self.n)
+        _check(_lib.hzl_ensemble_step_forced(self._h, dt, a.ctypes.data_as(_dp), int(shared)))
+
+    def synchronize(self):
+        _check(_lib.hzl_ensemble_synchronize(self._h))
+
+    def diagnostics(self) -> np.ndarray:
+        out = np.zeros((self.members, 6))
+        _check(_lib.hzl_ensemble_diagnostics(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def profile_step(self, dt: float) -> np.ndarray:
+        """Durations (ms) of the 8 launches of one step: row1, col1, ..., row4, col4."""
+        out = np.zeros(8, dtype=np.float32)
+        _check(_lib.hzl_ensemble_profile_step(self._h, dt,
+                                              out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    @property
+    def step_index(self) -> int:
+        v = C.c_uint64()
+        _check(_lib.hzl_ensemble_step_index(self._h, C.byref(v)))
+        return v.value
+
+    def device_state(self):
+        p, s = C.c_void_p(), C.c_size_t()
+        _check(_lib.hzl_ensemble_device_state(self._h, C.byref(p), C.byref(s)))
+        return p.value, s.value
+
+    @property
+    def stream(self) -> int:
+        p = C.c_void_p()
+        _check(_lib.hzl_ensemble_stream(self._h, C.byref(p)))
+        return p.value

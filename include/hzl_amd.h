@@ -1,0 +1,145 @@
+/*
+ * hzl_amd.h -- C-ABI of the MI355X batched vorticity ensemble engine.
+ *
+ * Drop-in boundary for the solver hot path of pvasbob/HZL-stochasticFlowInference.
+ * The reference exposes that path as C++ classes (no FFI of its own):
+ *   hzl::CudaSolver  (include/hzl/cuda_solver.hpp:7-22, src/cuda_solver.cu:86-235)
+ *   hzl::Solver      (include/hzl/solver.hpp:61-117, src/solver.cpp:120-254)
+ *   hzl::ensemble    (include/hzl/inference.hpp:84-85, src/inference.cpp:555-590)
+ * Each entry point below names the reference member it replaces.  A C++ layer
+ * that restores those exact class signatures on top of this ABI lives in
+ * hzl-stochasticflowinference_amd/csrc/host/cuda_solver_dropin.cpp.
+ *
+ * Conventions
+ *   - every function returns HZL_OK (0) or an error code; the message of the
+ *     last failure on the calling thread is returned by hzl_last_error().
+ *     HZL_EINVAL corresponds to the reference's std::invalid_argument,
+ *     HZL_ERUNTIME to std::runtime_error (src/cuda_solver.cu:11-19).
+ *   - host pointers are borrowed for the duration of the call only.
+ *   - a handle owns its device memory and one HIP stream; it is not re-entrant
+ *     (one host thread at a time, like one CudaSolver object).
+ *   - "field" arguments use the reference Field layout: n*n complex values as
+ *     interleaved doubles (re, im), row-major index y*n + x, forward-normalised
+ *     coefficients (physical(x) = sum_k coef(k) exp(i k.x); include/hzl/fft.hpp:7-9).
+ *     The engine stores the retained 2/3 half plane internally in FP32 or FP64.
+ */
+#ifndef HZL_AMD_H
+#define HZL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HZL_OK 0
+#define HZL_EINVAL 1
+#define HZL_ERUNTIME 2
+
+#define HZL_FP32 0
+#define HZL_FP64 1
+
+typedef struct hzl_ensemble *hzl_ensemble_t;
+
+/* Model parameters.  The first six fields mirror hzl::Parameters
+ * (include/hzl/solver.hpp:15-23).  The engine's stochastic forcing is white in
+ * time on the shell forcing_min <= |k| <= forcing_max with per-component
+ * increment variance forcing_sigma^2 dt / 2 (forcing_tau is accepted and
+ * ignored), plus a deterministic Kolmogorov term
+ * kolmogorov_amplitude * sin(kolmogorov_wavenumber * y). */
+typedef struct hzl_parameters {
+    double viscosity;
+    double friction;
+    double forcing_sigma;
+    double forcing_tau;
+    int forcing_min;
+    int forcing_max;
+    double kolmogorov_amplitude;
+    int kolmogorov_wavenumber;
+} hzl_parameters;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *hzl_last_error(void);
+
+/* Ensemble of `members` independent members on an n x n grid, n a power of
+ * two in [16, 512].  Member m's random streams are keyed by
+ * stream_seed(seed, m) (src/solver.cpp:61-66).  State starts at zero.
+ * Replaces: CudaSolver::CudaSolver (src/cuda_solver.cu:126-171) batched over
+ * members; validation as Parameters::validate (src/solver.cpp:31-40). */
+int hzl_ensemble_create(hzl_ensemble_t *out, size_t n, size_t members, int precision,
+                        const hzl_parameters *params, uint64_t seed);
+
+/* Replaces: CudaSolver::~CudaSolver. */
+int hzl_ensemble_destroy(hzl_ensemble_t h);
+
+/* Per-member overrides of nu, alpha, sigma and Kolmogorov amplitude (used by
+ * parameter inference where every member carries its own parameters). */
+int hzl_ensemble_set_member_parameters(hzl_ensemble_t h, size_t member, double viscosity,
+                                       double friction, double forcing_sigma,
+                                       double kolmogorov_amplitude);
+
+/* Upload one member's spectral state (reference Field layout); non-retained
+ * modes are dropped as the CudaSolver constructor masks its upload
+ * (src/cuda_solver.cu:160-167).  member == (size_t)-1 uploads all members from
+ * `field` holding members consecutive fields. */
+int hzl_ensemble_set_state(hzl_ensemble_t h, size_t member, const double *field);
+
+/* Download one member's state (or all with member == (size_t)-1).
+ * Replaces: CudaSolver::coefficients (src/cuda_solver.cu:222-235), including
+ * its non-finite check. */
+int hzl_ensemble_get_state(hzl_ensemble_t h, size_t member, double *field);
+
+/* Seeded Gaussian random field initial state for every member:
+ * E(k) ~ k^4 exp(-(k/4)^2), enstrophy (A/2) sum |w|^2 = 1. */
+int hzl_ensemble_init_grf(hzl_ensemble_t h);
+
+/* `steps` RK4 steps of every member with the engine's own forcing (generated
+ * on the device from (seed, member, step) counters).  Replaces:
+ * CudaSolver::step(double dt) (src/cuda_solver.cu:172-183) and the per-member
+ * loop of hzl::ensemble (src/inference.cpp:572-575).  Non-finite or diffusion-
+ * unstable dt fails immediately (src/cuda_solver.cu:203-208); the CFL limit
+ * dt * max(|u|+|v|) n / 2 pi <= 0.5 (src/solver.cpp:204-216) is evaluated on
+ * the device at every step and reported by hzl_ensemble_synchronize. */
+int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps);
+
+/* One RK4 step with host-provided midpoint forcing held through the four
+ * stages: `forcing` holds one field (shared != 0) or one per member.  Checks
+ * finiteness and conjugate symmetry to 1e-12 as CudaSolver::step(dt, F)
+ * (src/cuda_solver.cu:184-202). */
+int hzl_ensemble_step_forced(hzl_ensemble_t h, double dt, const double *forcing, int shared);
+
+/* Wait for queued work, then report a CFL violation or a non-finite state
+ * seen by any step since the previous call.  Replaces: CudaSolver::synchronize
+ * (src/cuda_solver.cu:219-221). */
+int hzl_ensemble_synchronize(hzl_ensemble_t h);
+
+/* Per-member diagnostics, 6 doubles each: energy, enstrophy, dissipation,
+ * injection (against the forcing of the last step), cfl_rate of the current
+ * state (max over the grid of (|u|+|v|) n / 2 pi), 0.  FP64 reductions on the
+ * device.
+ * Replaces: Solver::diagnostics (src/solver.cpp:381-397). */
+int hzl_ensemble_diagnostics(hzl_ensemble_t h, double *out);
+
+/* One engine-forcing RK4 step (as hzl_ensemble_step with steps = 1) with HIP
+ * events around each launch; kernel_ms[8] receives the durations of
+ * row(stage 1), col(stage 1), ..., row(stage 4), col(stage 4) in milliseconds.
+ * Measurement-harness counterpart of src/cuda_ensemble_benchmark.cpp:73-79. */
+int hzl_ensemble_profile_step(hzl_ensemble_t h, double dt, float *kernel_ms);
+
+/* Number of steps taken so far (the noise counter of the next step). */
+int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out);
+
+/* Device pointer of member 0's retained half-plane state ([c][j], K = (n-1)/3,
+ * c in [0,K], j = ky + K in [0,2K], complex float or double) and the byte
+ * stride between members; for zero-copy consumers on the same device. */
+int hzl_ensemble_device_state(hzl_ensemble_t h, void **ptr, size_t *member_stride_bytes);
+
+/* The handle's HIP stream (hipStream_t) for callers that order their own work
+ * after the engine's or time it with events. */
+int hzl_ensemble_stream(hzl_ensemble_t h, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,47 @@
+"""ctypes binding of the CPU lane emulator (tests/emulator/emu.cpp), TEST ONLY."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU_SO = os.path.join(HERE, "emulator", "libhzl_emu.so")
+_dp = C.POINTER(C.c_double)
+
+
+class Emulator:
+    def __init__(self, path: str = EMU_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run __graft_entry__.build()")
+        L = self.lib = C.CDLL(path)
+        L.emu_run.restype = C.c_int
+        L.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_double,
+                              C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+        L.emu_fft_lines.restype = C.c_int
+        L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+
+    def fft_lines(self, x, inverse=False, precision="fp32"):
+        d = np.ascontiguousarray(x, dtype=np.complex128).copy()
+        lines, n = d.shape
+        rc = self.lib.emu_fft_lines(n, 0 if precision == "fp32" else 1, lines, int(inverse),
+                                    d.ctypes.data_as(_dp))
+        assert rc == 0
+        return d
+
+    def run(self, n, members, precision, dt, steps, nu=1e-3, alpha=0.0, sigma=0.05, amp=0.1,
+            kf=4, kmin=1, kmax=4, seed=0, init=None, forcing=None, shared=True):
+        """Replay the engine's launch sequence on the CPU; returns (states, diagnostics)."""
+        out = np.zeros((members, n, n), dtype=np.complex128)
+        diag = np.zeros((members, 6))
+        ia = None if init is None else np.ascontiguousarray(init, dtype=np.complex128)
+        fa = None if forcing is None else np.ascontiguousarray(forcing, dtype=np.complex128)
+        rc = self.lib.emu_run(n, 0 if precision == "fp32" else 1, members, nu, alpha, sigma, amp,
+                              kf, kmin, kmax, seed, dt, steps, 1 if ia is None else 0,
+                              None if ia is None else ia.ctypes.data_as(_dp),
+                              None if fa is None else fa.ctypes.data_as(_dp), int(shared),
+                              out.ctypes.data_as(_dp), diag.ctypes.data_as(_dp))
+        assert rc == 0
+        return out, diag

@@ -1,0 +1,111 @@
+"""Write the golden vectors of tests/golden/ from the UNMODIFIED reference CPU
+library (oracle/_ref/libhzlref.so, built by `make -C oracle ref` from the
+reference's own sources).  Run in the container that holds the reference:
+
+    python tests/golden/make_golden.py
+
+Each fixture stores inputs and reference outputs only (numpy .npz, no pickles)
+plus a JSON sidecar with the recipe and sha256.  The BASELINE physics is driven
+through the reference's public API (frozen OU forcing + update_latent_state
+before every step; see oracle/ref_shim.cpp).
+
+Fixtures
+  config0_64.npz   BASELINE config 0: 64x64, nu 1e-3, dt 1e-3, alpha 0,
+                   0.1 sin(4y) + white noise sigma 0.05 on 1<=|k|<=4, GRF start
+                   (member 0 of seed 20261203); reference states at steps
+                   0 / 100 / 1000 as retained half planes [kx 0..K][ky -K..K],
+                   reference diagnostics at the same steps.
+  drift_multimode_32.npz  reference Solver::drift of the multimode state, 32x32.
+  pf_deterministic_16.npz reference bootstrap_filter / negative_log_likelihood /
+                   ensemble_kalman_filter on the deterministic dataset of
+                   tests/verification.cpp:184-210 (16x16).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from oracle import Oracle, Reference  # noqa: E402
+
+
+def half(field: np.ndarray) -> np.ndarray:
+    n = field.shape[0]
+    K = (n - 1) // 3
+    ky = np.arange(-K, K + 1) % n
+    return np.ascontiguousarray(field[ky][:, : K + 1].T)  # [kx][ky + K]
+
+
+def save(name: str, recipe: dict, **arrays) -> None:
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
+    meta = dict(recipe, file=name + ".npz", sha256=digest,
+                arrays={k: [list(v.shape), str(v.dtype)] for k, v in arrays.items()})
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=2, sort_keys=True)
+        f.write("\n")
+    print("wrote", path, digest[:16])
+
+
+def config0(orc: Oracle, ref: Reference) -> None:
+    n, nu, dt, seed = 64, 1e-3, 1e-3, 20261203
+    key = orc.stream_seed(seed, 0)
+    w0 = orc.grf(n, key)
+    F = np.stack([orc.step_forcing(n, 0.1, 4, 0.05, 1, 4, dt, key, k) for k in range(1000)])
+    w100 = ref.run(w0, nu, 0.0, dt, 100, F[:100], per_step=True)
+    w1000 = ref.run(w100, nu, 0.0, dt, 900, F[100:], per_step=True)
+    d = [ref.diagnostics(w, nu, 0.0) for w in (w0, w100, w1000)]
+    diag = np.array([[x["energy"], x["enstrophy"], x["dissipation"], x["cfl_rate"]] for x in d])
+    save("config0_64",
+         dict(recipe="reference Solver, frozen OU + update_latent_state(F_step) per step",
+              n=n, nu=nu, alpha=0.0, dt=dt, seed=seed, member=0, amp=0.1, kf=4, sigma=0.05,
+              kmin=1, kmax=4, steps=[0, 100, 1000],
+              diag_columns=["energy", "enstrophy", "dissipation", "cfl_rate"]),
+         w0=half(w0), w100=half(w100), w1000=half(w1000), diag=diag)
+
+
+def drift32(ref: Reference) -> None:
+    n = 32
+    w = ref.initialize(n, "multimode")
+    d = ref.drift(w, 0.01, 0.1)
+    save("drift_multimode_32", dict(recipe="reference Solver::drift of initialize('multimode')",
+                                    n=n, nu=0.01, alpha=0.1),
+         state=half(w), drift=half(d))
+
+
+def pf16(ref: Reference) -> None:
+    nu, alpha = 0.03, 0.1
+    h, rows = ref.synthetic(16, 0.01, 30, 10, nu, alpha, 0.0, 0.5, 2, 3, 0.2, 987)
+    pf = ref.bootstrap_filter(h, nu, alpha, 0.0, 0.5, 8, 555, means=True, n=16)
+    nll = ref.negative_log_likelihood(h, np.log(nu), np.log(alpha))
+    enkf = ref.enkf(h, nu, alpha, 0.0, 0.5, 8, 555, n=16)
+    init = ref.initialize(16, "multimode")
+    ref.dataset_free(h)
+    save("pf_deterministic_16",
+         dict(recipe="synthetic_observations(16, 0.01, 30, 10, p, 0.2, 987) with nu 0.03, "
+                     "alpha 0.1, sigma 0; bootstrap_filter(data, p, 8, 555); "
+                     "ensemble_kalman_filter(data, p, 8, 555)",
+              n=16, dt=0.01, nu=nu, alpha=alpha, particles=8, seed=555,
+              log_likelihood=pf["log_likelihood"], resampling_events=pf["resampling_events"],
+              negative_log_likelihood=nll),
+         rows=rows, records=pf["records"], means=pf["means"], initial=half(init),
+         enkf_records=enkf["records"], enkf_means=enkf["means"])
+
+
+def main() -> None:
+    orc, ref = Oracle(), Reference()
+    config0(orc, ref)
+    drift32(ref)
+    pf16(ref)
+
+
+if __name__ == "__main__":
+    main()
