@@ -31,21 +31,43 @@ struct DevCtx {
         __syncthreads();
 #endif
     }
+    // Wave-local ordering of LDS traffic: a compiler barrier plus wavefront-
+    // scope fences (the hardware completes one wavefront's LDS ops in order).
+    __host__ __device__ void wsync() const {
+#if __HIP_DEVICE_COMPILE__
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+    }
 };
 
 // ------------------------------------------------------------ kernels ----
-template <int N, typename R>
-__global__ void __launch_bounds__(Geometry<N>::COL_THREADS) k_col(StageArgs<R> a) {
+// Register budgets (waves per SIMD) chosen by measurement on MI355X.
+#ifndef HZL_COL_WPE
+#define HZL_COL_WPE 1
+#endif
+#ifndef HZL_ROW_WPE
+#define HZL_ROW_WPE 3
+#endif
+
+template <int N, int MODE, typename R>
+__global__ void __launch_bounds__(Geometry<N>::COL_THREADS)
+    __attribute__((amdgpu_waves_per_eu(HZL_COL_WPE))) k_col(StageArgs<R> a) {
     using G = Geometry<N>;
     __shared__ cx<R> lds[G::COL_G * GroupFFT<N, G::E, R>::LDS_ELEMS];
-    col_body<N, R>(DevCtx{}, a, lds, (long)blockIdx.x);
+    col_body<N, MODE, R>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 
-template <int N, typename R>
-__global__ void __launch_bounds__(Geometry<N>::ROW_THREADS) k_row(RowArgs<R> a) {
-    using G = Geometry<N>;
-    __shared__ cx<R> lds[G::ROW_PPW * GroupFFT<N, G::E, R>::LDS_ELEMS];
-    row_body<N, R>(DevCtx{}, a, lds, (long)blockIdx.x);
+template <typename R> __global__ void __launch_bounds__(256) k_noise(NoiseArgs<R> a) {
+    noise_body<R>(a, (long)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <int N, typename R, bool CFL>
+__global__ void __launch_bounds__(Geometry<N>::ROW_THREADS)
+    __attribute__((amdgpu_waves_per_eu(HZL_ROW_WPE))) k_row(RowArgs<R> a) {
+    __shared__ cx<R> lds[RowShape<N>::template lds_elems<R>()];
+    row_body<N, R, CFL>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 
 template <typename R> __global__ void __launch_bounds__(256) k_grf(GrfArgs<R> a) {
@@ -93,34 +115,56 @@ template <typename T> struct DeviceBuffer {
     }
 };
 
-template <int N, typename R> void launch_col(const StageArgs<R> &a, hipStream_t s) {
+template <int N, typename R> void launch_col(int mode, const StageArgs<R> &a, hipStream_t s) {
     using G = Geometry<N>;
     const long cols = (long)a.B * G::KP;
-    const long blocks = (cols + G::COL_G - 1) / G::COL_G;
-    hipLaunchKernelGGL((k_col<N, R>), dim3((unsigned)blocks), dim3(G::COL_THREADS), 0, s, a);
+    const dim3 grid((unsigned)((cols + G::COL_G - 1) / G::COL_G)), block(G::COL_THREADS);
+    switch (mode) {
+    case 0:
+        hipLaunchKernelGGL((k_col<N, 0, R>), grid, block, 0, s, a);
+        return;
+    case 1:
+        hipLaunchKernelGGL((k_col<N, 1, R>), grid, block, 0, s, a);
+        return;
+    case 2:
+        hipLaunchKernelGGL((k_col<N, 2, R>), grid, block, 0, s, a);
+        return;
+    case 3:
+        hipLaunchKernelGGL((k_col<N, 3, R>), grid, block, 0, s, a);
+        return;
+    case 4:
+        hipLaunchKernelGGL((k_col<N, 4, R>), grid, block, 0, s, a);
+        return;
+    }
+    throw std::invalid_argument("bad stage");
 }
 
 template <int N, typename R> void launch_row(const RowArgs<R> &a, hipStream_t s) {
     using G = Geometry<N>;
     const long pairs = (long)a.B * (N / 2);
     const long blocks = (pairs + G::ROW_PPW - 1) / G::ROW_PPW;
-    hipLaunchKernelGGL((k_row<N, R>), dim3((unsigned)blocks), dim3(G::ROW_THREADS), 0, s, a);
+    if (a.cfl)
+        hipLaunchKernelGGL((k_row<N, R, true>), dim3((unsigned)blocks), dim3(G::ROW_THREADS), 0,
+                           s, a);
+    else
+        hipLaunchKernelGGL((k_row<N, R, false>), dim3((unsigned)blocks), dim3(G::ROW_THREADS), 0,
+                           s, a);
 }
 
-template <typename R> void dispatch_col(int n, const StageArgs<R> &a, hipStream_t s) {
+template <typename R> void dispatch_col(int n, int mode, const StageArgs<R> &a, hipStream_t s) {
     switch (n) {
     case 16:
-        return launch_col<16, R>(a, s);
+        return launch_col<16, R>(mode, a, s);
     case 32:
-        return launch_col<32, R>(a, s);
+        return launch_col<32, R>(mode, a, s);
     case 64:
-        return launch_col<64, R>(a, s);
+        return launch_col<64, R>(mode, a, s);
     case 128:
-        return launch_col<128, R>(a, s);
+        return launch_col<128, R>(mode, a, s);
     case 256:
-        return launch_col<256, R>(a, s);
+        return launch_col<256, R>(mode, a, s);
     case 512:
-        return launch_col<512, R>(a, s);
+        return launch_col<512, R>(mode, a, s);
     }
     throw std::invalid_argument("unsupported grid");
 }
@@ -175,15 +219,16 @@ template <typename R> class Engine final : public EngineBase {
         mix_ = DeviceBuffer<cx<R>>(B_ * 4 * mixf_);
         nl_ = DeviceBuffer<cx<R>>(B_ * mixf_);
         ext_ = DeviceBuffer<cx<R>>(B_ * spec_);
-        cfl_ = DeviceBuffer<float>(B_ * (size_t)(n / 2));
-        cfl_diag_ = DeviceBuffer<float>(B_ * (size_t)(n / 2));
+        cfl_ = DeviceBuffer<double>(B_ * (size_t)(n / 2));
+        cfl_diag_ = DeviceBuffer<double>(B_ * (size_t)(n / 2));
         const auto tw = twiddles<R>(n);
         tw_ = DeviceBuffer<cx<R>>(tw.size());
         upload(tw_.p, tw.data(), tw.size());
-        int count = 0;
-        const auto pairs = pair_table(p.forcing_min, p.forcing_max, &count);
+        const auto pairs = pair_table(p.forcing_min, p.forcing_max, &npairs_);
         pairs_ = DeviceBuffer<int>(pairs.size());
         upload(pairs_.p, pairs.data(), pairs.size());
+        noise_ = DeviceBuffer<cx<R>>(B_ * (size_t)npairs_);
+        check(hipMemsetAsync(noise_.p, 0, noise_.count * sizeof(cx<R>), stream_), "clear noise");
         nu_.assign(B_, p.viscosity);
         alpha_.assign(B_, p.friction);
         sigma_.assign(B_, p.forcing_sigma);
@@ -199,7 +244,7 @@ template <typename R> class Engine final : public EngineBase {
         upload(d_keys_.p, keys_.data(), B_);
         push_parameters();
         check(hipMemsetAsync(w0_.p, 0, B_ * spec_ * sizeof(cx<R>), stream_), "clear state");
-        check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(float), stream_), "clear CFL");
+        check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(double), stream_), "clear CFL");
         check(hipStreamSynchronize(stream_), "engine setup");
     }
     ~Engine() override {
@@ -293,18 +338,18 @@ template <typename R> class Engine final : public EngineBase {
     void synchronize() override {
         check(hipStreamSynchronize(stream_), "synchronize GPU");
         check(hipGetLastError(), "GPU kernels");
-        std::vector<float> c(cfl_.count);
-        check(hipMemcpy(c.data(), cfl_.p, c.size() * sizeof(float), hipMemcpyDeviceToHost),
+        std::vector<double> c(cfl_.count);
+        check(hipMemcpy(c.data(), cfl_.p, c.size() * sizeof(double), hipMemcpyDeviceToHost),
               "CFL download");
-        float mx = 0;
+        double mx = 0;
         bool bad = false;
-        for (float v : c) {
+        for (double v : c) {
             if (!std::isfinite(v))
                 bad = true;
             else
                 mx = std::max(mx, v);
         }
-        check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(float), stream_), "reset CFL");
+        check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(double), stream_), "reset CFL");
         check(hipStreamSynchronize(stream_), "reset CFL");
         const double rate = (double)mx * n_ / (2 * PI);
         last_cfl_rate_ = rate;
@@ -321,17 +366,17 @@ template <typename R> class Engine final : public EngineBase {
         // CFL rate of the current state (src/solver.cpp:392-396): one row pass
         // over the mixed fields of the current state into a private buffer.
         if (!mix_valid_) {
-            dispatch_col<R>(n_, stage_args(0, 1.0, nullptr, 0), stream_);
+            dispatch_col<R>(n_, 0, stage_args(1.0, nullptr, 0), stream_);
             mix_valid_ = true;
         }
-        check(hipMemsetAsync(cfl_diag_.p, 0, cfl_diag_.count * sizeof(float), stream_),
+        check(hipMemsetAsync(cfl_diag_.p, 0, cfl_diag_.count * sizeof(double), stream_),
               "clear CFL");
         RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_.p, cfl_diag_.p};
         dispatch_row<R>(n_, r, stream_);
         DeviceBuffer<double> d(B_ * 6);
-        ForcingModel f = forcing_model(step_ > 0 ? last_dt_ : 1.0);
-        DiagArgs<R> a{K_, w0_.p, d_nu_.p, f, step_ > 0 ? step_ - 1 : 0, step_ > 0 ? 1 : 0,
-                      cfl_diag_.p, n_ / 2, n_, d.p};
+        // the noise buffer still holds the last step's shell noise
+        DiagArgs<R> a{K_, w0_.p, d_nu_.p, forcing_model(), step_ > 0 ? 1 : 0, cfl_diag_.p,
+                      n_ / 2, n_, d.p};
         hipLaunchKernelGGL(k_diag<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
         check(hipGetLastError(), "diagnostics kernel");
         check(hipMemcpyAsync(out, d.p, B_ * 6 * sizeof(double), hipMemcpyDeviceToHost, stream_),
@@ -402,16 +447,15 @@ template <typename R> class Engine final : public EngineBase {
                     "invalid GPU time step or diffusion stability limit exceeded");
     }
 
-    ForcingModel forcing_model(double dt) const {
-        return ForcingModel{d_sigma_.p,        d_amp_.p,          d_keys_.p,
-                            pairs_.p,          p_.forcing_min,    p_.forcing_max,
-                            p_.kolmogorov_wavenumber, dt};
+    ForcingModel<R> forcing_model() const {
+        return ForcingModel<R>{noise_.p,       npairs_,        d_amp_.p,
+                               pairs_.p,       p_.forcing_min, p_.forcing_max,
+                               p_.kolmogorov_wavenumber};
     }
 
-    StageArgs<R> stage_args(int mode, double dt, const cx<R> *ext, long long ext_stride) const {
+    StageArgs<R> stage_args(double dt, const cx<R> *ext, long long ext_stride) const {
         StageArgs<R> a;
         a.B = (int)B_;
-        a.mode = mode;
         a.w0 = w0_.p;
         a.w1 = w1_.p;
         a.acc = acc_.p;
@@ -420,13 +464,12 @@ template <typename R> class Engine final : public EngineBase {
         a.tw = tw_.p;
         a.nu = d_nu_.p;
         a.alpha = d_alpha_.p;
-        a.forcing = forcing_model(dt);
+        a.forcing = forcing_model();
         a.ext = ext;
         a.ext_stride = ext_stride;
         a.h_half = (R)(dt / 2);
         a.h_full = (R)dt;
         a.h_sixth = (R)(dt / 6);
-        a.step = step_;
         return a;
     }
 
@@ -436,8 +479,14 @@ template <typename R> class Engine final : public EngineBase {
                  hipEvent_t *events = nullptr) {
         flush_parameters();
         if (!mix_valid_) {
-            dispatch_col<R>(n_, stage_args(0, dt, ext, ext_stride), stream_);
+            dispatch_col<R>(n_, 0, stage_args(dt, ext, ext_stride), stream_);
             mix_valid_ = true;
+        }
+        if (!ext) {
+            NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_, noise_.p};
+            const long count = (long)B_ * npairs_;
+            hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                               stream_, na);
         }
         RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_.p, nullptr};
         if (events)
@@ -447,7 +496,7 @@ template <typename R> class Engine final : public EngineBase {
             dispatch_row<R>(n_, r, stream_);
             if (events)
                 check(hipEventRecord(events[2 * stage - 1], stream_), "event");
-            dispatch_col<R>(n_, stage_args(stage, dt, ext, ext_stride), stream_);
+            dispatch_col<R>(n_, stage, stage_args(dt, ext, ext_stride), stream_);
             if (events)
                 check(hipEventRecord(events[2 * stage], stream_), "event");
         }
@@ -463,8 +512,9 @@ template <typename R> class Engine final : public EngineBase {
     hzl_parameters p_;
     size_t spec_ = 0, mixf_ = 0;
     hipStream_t stream_ = nullptr;
-    DeviceBuffer<cx<R>> w0_, w1_, acc_, mix_, nl_, ext_, tw_;
-    DeviceBuffer<float> cfl_, cfl_diag_;
+    DeviceBuffer<cx<R>> w0_, w1_, acc_, mix_, nl_, ext_, tw_, noise_;
+    int npairs_ = 0;
+    DeviceBuffer<double> cfl_, cfl_diag_;
     DeviceBuffer<int> pairs_;
     DeviceBuffer<double> d_nu_, d_alpha_, d_sigma_, d_amp_;
     DeviceBuffer<uint64_t> d_keys_;

@@ -19,8 +19,11 @@
 // double precision and rounded once to the working precision.
 //
 // Everything here is __host__ __device__ and synchronises through a context
-// object (`ctx.sync()`), so the same code also runs in the CPU lane emulator
-// used by the tests.
+// object, so the same code also runs in the CPU lane emulator used by the
+// tests.  A group's T lanes are contiguous and never straddle a wavefront, so
+// its LDS exchange needs only wave-local ordering (`ctx.wsync()`: LDS
+// instructions of one wavefront complete in issue order), not a workgroup
+// barrier.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -145,8 +148,9 @@ template <int L, bool INV, typename R> HZL_HD void dft(cx<R> *v) {
 // Two-pass transform of length N by a group of T = N/E threads.
 // `lds` is the group's private region of at least LDS_ELEMS entries.
 // `tw[t*E + k2]` = exp(-2 pi i t k2 / N) for t < T, k2 < E.
-// ctx.sync() is a workgroup-wide barrier: every thread of the workgroup must
-// call forward/inverse the same number of times (idle groups included).
+// ctx.wsync() orders the exchange within the wavefront; every lane of the
+// wavefront must call forward/inverse the same number of times (idle groups
+// included).
 template <int N, int E, typename R> struct GroupFFT {
     static constexpr int T = N / E;
     static constexpr int Q = E / T;
@@ -154,10 +158,19 @@ template <int N, int E, typename R> struct GroupFFT {
     static constexpr int LDS_ELEMS = E * LD;
     static_assert(T * E == N && T >= 1 && (E % T) == 0, "need N = T*E with T | E");
     static_assert(LDS_ELEMS >= N, "LDS region must hold one natural-order line");
+    static_assert(64 % T == 0, "a group must not straddle a wavefront");
 
     // Spectral index held in register slot idx by thread t (forward output order).
     HZL_HD static constexpr int index(int t, int idx) {
         return t + T * (idx / T) + E * (idx % T);
+    }
+
+    // True when slot idx holds, for every thread t < T, an index k with
+    // K < k < N - K (outside the retained band |k| <= K).  idx is a
+    // compile-time constant after unrolling, so such slots fold to zero.
+    HZL_HD static constexpr bool slot_outside(int idx, int K) {
+        const int lo = T * (idx / T) + E * (idx % T);
+        return lo > K && lo + T - 1 < N - K;
     }
 
     template <class Ctx>
@@ -170,7 +183,7 @@ template <int N, int E, typename R> struct GroupFFT {
 #pragma unroll
         for (int k2 = 0; k2 < E; ++k2)
             lds[k2 * LD + t] = v[k2];
-        ctx.sync();
+        ctx.wsync();
 #pragma unroll
         for (int r = 0; r < Q; ++r) {
             const int k2 = t + T * r;
@@ -183,7 +196,7 @@ template <int N, int E, typename R> struct GroupFFT {
             for (int k1 = 0; k1 < T; ++k1)
                 v[r * T + k1] = u[k1];
         }
-        ctx.sync();
+        ctx.wsync();
     }
 
     template <class Ctx>
@@ -201,12 +214,12 @@ template <int N, int E, typename R> struct GroupFFT {
             for (int n1 = 0; n1 < T; ++n1)
                 lds[k2 * LD + n1] = n1 == 0 ? u[n1] : cmulc(u[n1], tw[n1 * E + k2]);
         }
-        ctx.sync();
+        ctx.wsync();
 #pragma unroll
         for (int k2 = 0; k2 < E; ++k2)
             v[k2] = lds[k2 * LD + t];
         dft<E, true>(v);
-        ctx.sync();
+        ctx.wsync();
     }
 };
 

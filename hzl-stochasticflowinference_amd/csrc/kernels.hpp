@@ -69,6 +69,19 @@ template <int N> struct Geometry {
     static constexpr long MIXF = (long)KP * N; // complex entries per mixed field
 };
 
+// 1/x: correctly rounded division in FP64; the hardware reciprocal (1 ulp)
+// in FP32, where it feeds psi = w / |k|^2.
+HZL_HD double recip(double x) {
+    return 1.0 / x;
+}
+HZL_HD float recip(float x) {
+#if __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 HZL_HD long tiled(int KP, int c, int y) {
     return (long)(y >> 4) * (KP * 16) + c * 16 + (y & 15);
 }
@@ -109,39 +122,60 @@ HZL_HD void philox_normal2(uint64_t key, uint32_t c0, uint64_t c12, uint32_t tag
 }
 
 // ----------------------------------------------------------- forcing ----
-struct ForcingModel {
+// Shell white noise of one step, generated once per step for every member by
+// a small kernel (so the FP64 Philox/Box-Muller code stays out of the FFT
+// kernels): noise[m * npairs + p] = sigma_m / sqrt(2 dt) (g0 + i g1) with
+// (g0, g1) from Philox keyed by member m, counter (p, step, "NOIS").
+template <typename R> struct NoiseArgs {
+    int B, npairs;
     const double *sigma;  // per member white-noise amplitude
-    const double *amp;    // per member Kolmogorov amplitude
     const uint64_t *keys; // per member Philox key
-    const int *pair_of;   // (kmax+1) x (2 kmax + 1): canonical pair index or -1
-    int kmin, kmax, kf;
     double dt;
+    uint64_t step;
+    cx<R> *noise;
 };
 
-// Held forcing of mode (c, ky) for member m at step `step`:
+template <typename R> HZL_HD void noise_body(const NoiseArgs<R> &a, long i) {
+    if (i >= (long)a.B * a.npairs)
+        return;
+    const int m = (int)(i / a.npairs), p = (int)(i % a.npairs);
+    const double s = a.sigma[m];
+    cx<R> v = {0, 0};
+    if (s > 0) {
+        double g0, g1;
+        philox_normal2(a.keys[m], (uint32_t)p, a.step, TAG_NOISE, g0, g1);
+        const double scale = s / sqrt(2.0 * a.dt);
+        v = {(R)(scale * g0), (R)(scale * g1)};
+    }
+    a.noise[i] = v;
+}
+
+template <typename R> struct ForcingModel {
+    const cx<R> *noise;  // [B][npairs] shell noise of the current step, or null
+    int npairs;
+    const double *amp;   // per member Kolmogorov amplitude
+    const int *pair_of;  // (kmax+1) x (2 kmax + 1): canonical pair index or -1
+    int kmin, kmax, kf;
+};
+
+// Held forcing of mode (c, ky) for member m:
 //   amp sin(kf y) -> -i amp/2 at (0, kf), +i amp/2 at (0, -kf);
-//   shell pairs kmin <= |k| <= kmax -> sigma/sqrt(2 dt) (g0 + i g1), with the
-//   kx = 0, ky < 0 entries the conjugates of their ky > 0 partners.
-HZL_HD cx<double> model_forcing(const ForcingModel &f, int m, int c, int ky, uint64_t step) {
-    cx<double> F = {0.0, 0.0};
+//   shell pairs kmin <= |k| <= kmax -> the step's noise, with the kx = 0,
+//   ky < 0 entries the conjugates of their ky > 0 partners.
+template <typename R>
+HZL_HD cx<R> model_forcing(const ForcingModel<R> &f, int m, int c, int ky) {
+    cx<R> F = {0, 0};
     if (c == 0 && f.kf > 0 && (ky == f.kf || ky == -f.kf))
-        F.y += ky > 0 ? -0.5 * f.amp[m] : 0.5 * f.amp[m];
+        F.y = (R)(ky > 0 ? -0.5 * f.amp[m] : 0.5 * f.amp[m]);
     const int k2 = c * c + ky * ky;
-    const double s = f.sigma[m];
-    if (s > 0 && k2 >= f.kmin * f.kmin && k2 <= f.kmax * f.kmax) {
-        int b = ky;
-        bool flip = false;
-        if (c == 0 && b < 0) {
-            b = -b;
-            flip = true;
-        }
+    if (f.noise && k2 >= f.kmin * f.kmin && k2 <= f.kmax * f.kmax) {
+        const bool flip = c == 0 && ky < 0;
+        const int b = flip ? -ky : ky;
         const int p = f.pair_of[c * (2 * f.kmax + 1) + (b + f.kmax)];
         if (p >= 0) {
-            double g0, g1;
-            philox_normal2(f.keys[m], (uint32_t)p, step, TAG_NOISE, g0, g1);
-            const double scale = s / sqrt(2.0 * f.dt);
-            F.x += scale * g0;
-            F.y += flip ? -scale * g1 : scale * g1;
+            const cx<R> v = f.noise[(long)m * f.npairs + p];
+            F.x += v.x;
+            F.y += flip ? -v.y : v.y;
         }
     }
     return F;
@@ -149,21 +183,20 @@ HZL_HD cx<double> model_forcing(const ForcingModel &f, int m, int c, int ky, uin
 
 // -------------------------------------------------------- column pass ----
 template <typename R> struct StageArgs {
-    int B;    // members
-    int mode; // 0: pass A only (from w0); 1..4: RK4 stage
+    int B; // members
     cx<R> *w0, *w1, *acc;
     const cx<R> *nl;
     cx<R> *mix;
     const cx<R> *tw;
     const double *nu, *alpha; // per member
-    ForcingModel forcing;
+    ForcingModel<R> forcing;
     const cx<R> *ext;     // host-provided held forcing [.][KP][J] or null
     long long ext_stride; // complex entries between members (0: shared)
     R h_half, h_full, h_sixth;
-    uint64_t step;
 };
 
-template <int N, typename R, class Ctx>
+// MODE 0: pass A only (mixed fields of w0); MODE 1..4: RK4 stage MODE.
+template <int N, int MODE, typename R, class Ctx>
 HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long block) {
     using G = Geometry<N>;
     using FFT = GroupFFT<N, G::E, R>;
@@ -177,61 +210,93 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
     const long soff = (long)m * G::SPEC + (long)c * J;
     cx<R> v[E];
 
-    if (a.mode == 0) {
+    if constexpr (MODE == 0) {
 #pragma unroll
         for (int idx = 0; idx < E; ++idx) {
             const int k = FFT::index(t, idx);
             const int ky = k < N / 2 ? k : k - N;
-            v[idx] = (live && ky >= -K && ky <= K) ? a.w0[soff + ky + K] : cx<R>{0, 0};
+            v[idx] = (!FFT::slot_outside(idx, K) && live && ky >= -K && ky <= K)
+                         ? a.w0[soff + ky + K]
+                         : cx<R>{0, 0};
         }
     } else {
         const cx<R> *nl = a.nl + (long)m * G::MIXF;
 #pragma unroll
         for (int j = 0; j < E; ++j)
             v[j] = live ? nl[tiled(KP, c, t + T * j)] : cx<R>{0, 0};
+        // All state loads of the column are issued together (one memory round
+        // trip for the column, not one per slot).  Stage 2 rebuilds
+        // w2 = w_n + dt/2 k1 from acc = k1 exactly as stage 1 formed it, so
+        // stage 1 never stores w1.
+        cx<R> wn[E], ws[E], sa[E];
+        auto load_state = [&]() {
+#pragma unroll
+            for (int idx = 0; idx < E; ++idx) {
+                const int k = FFT::index(t, idx);
+                const int ky = k < N / 2 ? k : k - N;
+                wn[idx] = ws[idx] = sa[idx] = {0, 0};
+                if (!FFT::slot_outside(idx, K) && live && ky >= -K && ky <= K) {
+                    const long off = soff + ky + K;
+                    wn[idx] = a.w0[off];
+                    if constexpr (MODE >= 3)
+                        ws[idx] = a.w1[off];
+                    if constexpr (MODE >= 2)
+                        sa[idx] = a.acc[off];
+                }
+            }
+        };
+#ifdef HZL_COL_PREFETCH
+        load_state();
+#endif
         FFT::forward(ctx, v, L, t, a.tw);
+#ifndef HZL_COL_PREFETCH
+        load_state();
+#endif
         const R inv_nn = R(1) / (R(N) * R(N));
         const R nu = live ? (R)a.nu[m] : R(0), alpha = live ? (R)a.alpha[m] : R(0);
+        if constexpr (MODE == 2) {
+#pragma unroll
+            for (int idx = 0; idx < E; ++idx)
+                ws[idx] = cadd(wn[idx], cscale(sa[idx], a.h_half));
+        }
 #pragma unroll
         for (int idx = 0; idx < E; ++idx) {
+            if (FFT::slot_outside(idx, K)) {
+                v[idx] = {0, 0};
+                continue;
+            }
             const int k = FFT::index(t, idx);
             const int ky = k < N / 2 ? k : k - N;
             cx<R> next = {0, 0};
             if (live && ky >= -K && ky <= K) {
                 const long off = soff + ky + K;
                 const int k2 = c * c + ky * ky;
-                const cx<R> wn = a.w0[off];
-                const cx<R> ws = a.mode == 1 ? wn : a.w1[off];
+                const cx<R> w_n = wn[idx];
+                const cx<R> w_s = MODE == 1 ? w_n : ws[idx];
                 cx<R> rhs = {0, 0};
                 if (k2 != 0) {
                     cx<R> F;
-                    if (a.ext) {
+                    if (a.ext)
                         F = a.ext[(long)m * a.ext_stride + (long)c * J + ky + K];
-                    } else {
-                        const cx<double> f = model_forcing(a.forcing, m, c, ky, a.step);
-                        F = {(R)f.x, (R)f.y};
-                    }
+                    else
+                        F = model_forcing(a.forcing, m, c, ky);
                     const R rate = nu * (R)k2 + alpha;
                     const cx<R> nh = cscale(v[idx], inv_nn);
-                    rhs = {-nh.x - rate * ws.x + F.x, -nh.y - rate * ws.y + F.y};
+                    rhs = {-nh.x - rate * w_s.x + F.x, -nh.y - rate * w_s.y + F.y};
                 }
-                if (a.mode == 1) {
+                if constexpr (MODE == 1) {
                     a.acc[off] = rhs;
-                    next = cadd(wn, cscale(rhs, a.h_half));
+                    next = cadd(w_n, cscale(rhs, a.h_half));
+                } else if constexpr (MODE == 2) {
+                    a.acc[off] = cadd(sa[idx], cscale(rhs, R(2)));
+                    next = cadd(w_n, cscale(rhs, a.h_half));
                     a.w1[off] = next;
-                } else if (a.mode == 2) {
-                    const cx<R> s = a.acc[off];
-                    a.acc[off] = cadd(s, cscale(rhs, R(2)));
-                    next = cadd(wn, cscale(rhs, a.h_half));
-                    a.w1[off] = next;
-                } else if (a.mode == 3) {
-                    const cx<R> s = a.acc[off];
-                    a.acc[off] = cadd(s, cscale(rhs, R(2)));
-                    next = cadd(wn, cscale(rhs, a.h_full));
+                } else if constexpr (MODE == 3) {
+                    a.acc[off] = cadd(sa[idx], cscale(rhs, R(2)));
+                    next = cadd(w_n, cscale(rhs, a.h_full));
                     a.w1[off] = next;
                 } else {
-                    const cx<R> s = a.acc[off];
-                    next = cadd(wn, cscale(cadd(s, rhs), a.h_sixth));
+                    next = cadd(w_n, cscale(cadd(sa[idx], rhs), a.h_sixth));
                     a.w0[off] = next;
                 }
             }
@@ -249,7 +314,7 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
             const int k = FFT::index(t, idx);
             const int ky = k < N / 2 ? k : k - N;
             const int k2 = c * c + ky * ky;
-            const R inv = k2 > 0 ? R(1) / (R)k2 : R(0);
+            const R inv = k2 > 0 ? recip((R)k2) : R(0);
             R s = R(1);
             if (f == 0)
                 s = inv;
@@ -274,103 +339,239 @@ template <typename R> struct RowArgs {
     const cx<R> *mix;
     cx<R> *nl;
     const cx<R> *tw;
-    float *cfl; // per row pair running max of |u|+|v| (null: off)
+    double *cfl; // per row pair running max of |u|+|v| (CFL variant only)
 };
 
-template <int N, typename R, class Ctx>
+template <int N> struct RowShape {
+    using G = Geometry<N>;
+    static constexpr int ROWS = 2 * G::ROW_PPW;      // rows per workgroup
+    static constexpr int TILES = ROWS / 16;          // 16-row tiles per workgroup
+    static constexpr int PITCH = 17;                 // staged kx-row pitch (bank-conflict free)
+    static constexpr int CHUNK = TILES * G::KP * PITCH; // complex entries of one staged field
+    template <typename R> static constexpr int lds_elems() {
+        constexpr int fft = G::ROW_PPW * GroupFFT<N, G::E, R>::LDS_ELEMS;
+        return 2 * CHUNK > fft ? 2 * CHUNK : fft;
+    }
+};
+
+// Stage field f of the workgroup's rows into dst, layout [tile][kx][PITCH]
+// (the global [tile][kx][16] tile layout with padded rows), with 2-entry
+// vector loads by consecutive lanes.
+// A field's share of the workgroup's 16-row tile held in registers by one
+// thread: ITERS pairs of consecutive complex entries.
+template <int N, typename R> struct RowTile {
+    using G = Geometry<N>;
+    using RS = RowShape<N>;
+    struct alignas(4 * sizeof(R)) pair2 {
+        cx<R> u, v;
+    };
+    static constexpr int PER_TILE = G::KP * 16, ELEMS = RS::TILES * PER_TILE;
+    static constexpr int STEP = 2 * G::ROW_THREADS, ITERS = (ELEMS + STEP - 1) / STEP;
+    pair2 v[ITERS];
+
+    // Issue all loads of field f (global tile layout [tile][kx][16]).
+    template <class Ctx> HZL_HD void load(const Ctx &ctx, const RowArgs<R> &a, long row0, int f) {
+        const long total_rows = (long)a.B * N;
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int e = 2 * ctx.tid() + it * STEP;
+            const int ti = e / PER_TILE, rem = e % PER_TILE;
+            const long grow = row0 + 16 * ti;
+            if (e < ELEMS && grow < total_rows) {
+                const long mm = grow / N;
+                const int yy = (int)(grow % N);
+                v[it] = *reinterpret_cast<const pair2 *>(a.mix + (mm * 4 + f) * G::MIXF +
+                                                         (long)(yy >> 4) * PER_TILE + rem);
+            }
+        }
+    }
+
+    // Write to LDS with padded kx rows: layout [tile][kx][PITCH].
+    template <class Ctx> HZL_HD void store(const Ctx &ctx, const RowArgs<R> &a, long row0,
+                                           cx<R> *dst) const {
+        const long total_rows = (long)a.B * N;
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int e = 2 * ctx.tid() + it * STEP;
+            const int ti = e / PER_TILE, rem = e % PER_TILE;
+            if (e < ELEMS && row0 + 16 * ti < total_rows) {
+                cx<R> *d = dst + ti * (G::KP * RS::PITCH) + (rem >> 4) * RS::PITCH + (rem & 15);
+                d[0] = v[it].u;
+                d[1] = v[it].v;
+            }
+        }
+    }
+};
+
+// Offset of (kx, local row ly) in a staged field.
+template <int N> HZL_HD int staged(int kx, int ly) {
+    using RS = RowShape<N>;
+    return (ly >> 4) * (Geometry<N>::KP * RS::PITCH) + kx * RS::PITCH + (ly & 15);
+}
+
+// One row-pair group: build the packed spectra z1 = u + i wx (fields B, C) or
+// z2 = v + i wy (fields A, D) of local row ly from the staged fields P, Q.
+template <int N, typename R, bool FIRST>
+HZL_HD void row_spectrum(cx<R> (&z)[Geometry<N>::E], const cx<R> *P, const cx<R> *Q, int t,
+                         int ly) {
+    using G = Geometry<N>;
+    using FFT = GroupFFT<N, G::E, R>;
+    constexpr int K = G::K;
+#pragma unroll
+    for (int idx = 0; idx < G::E; ++idx) {
+        const int k = FFT::index(t, idx);
+        const bool pos = k <= K, neg = k >= N - K;
+        z[idx] = {0, 0};
+        if (!FFT::slot_outside(idx, K) && (pos || neg)) {
+            const int kx = pos ? k : N - k;
+            const int o = staged<N>(kx, ly);
+            const cx<R> p = P[o], q = Q[o];
+            const R fx = (R)kx;
+            // FIRST: P = (ky psi)~, Q = w~:   a = u = i P,        b = wx = i kx Q
+            // else : P = psi~,      Q = (ky w)~: a = v = -i kx P, b = wy = i Q
+            const cx<R> A = FIRST ? mul_i(p) : mul_mi(cscale(p, fx));
+            const cx<R> Bv = FIRST ? mul_i(cscale(q, fx)) : mul_i(q);
+            if (k == 0) // DC terms of real signals: real parts only
+                z[idx] = {A.x, Bv.x};
+            else if (pos)
+                z[idx] = {A.x - Bv.y, A.y + Bv.x};
+            else
+                z[idx] = {A.x + Bv.y, Bv.x - A.y};
+        }
+    }
+}
+
+template <int N, typename R, bool CFL, class Ctx>
 HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block) {
     using G = Geometry<N>;
     using FFT = GroupFFT<N, G::E, R>;
+    using RS = RowShape<N>;
     constexpr int E = G::E, T = G::T, K = G::K, KP = G::KP, PPW = G::ROW_PPW;
     const int tid = ctx.tid();
-    const int t = tid / PPW, g = tid % PPW; // pair-fastest lanes: coalesced tile rows
+    const int g = tid / T, t = tid % T; // group-contiguous lanes: conflict-free exchanges
+    const long row0 = block * RS::ROWS;
     const long gp = block * PPW + g;
     const bool live = gp < (long)a.B * (N / 2);
     const int m = live ? (int)((2 * gp) / N) : 0, y0 = live ? (int)((2 * gp) % N) : 0;
+    const int ly = 2 * g; // local row of y0
+    cx<R> *S0 = lds, *S1 = lds + RS::CHUNK;
     cx<R> *L = lds + g * FFT::LDS_ELEMS;
-    const cx<R> *mix = a.mix + (long)m * 4 * G::MIXF;
 
-    R p[2][E];
+    R pa[E], pb[E], ua[CFL ? E : 1], ub[CFL ? E : 1];
     R vmax = R(0);
-#pragma unroll
-    for (int row = 0; row < 2; ++row) {
-        const int y = y0 + row;
-        cx<R> z1[E], z2[E];
-#pragma unroll
-        for (int idx = 0; idx < E; ++idx) {
-            const int k = FFT::index(t, idx);
-            z1[idx] = {0, 0};
-            z2[idx] = {0, 0};
-            const bool pos = k <= K, neg = k >= N - K;
-            if (live && (pos || neg)) {
-                const int kx = pos ? k : N - k;
-                const long o = tiled(KP, kx, y);
-                const cx<R> A = mix[o], Bv = mix[G::MIXF + o], C = mix[2 * G::MIXF + o],
-                            D = mix[3 * G::MIXF + o];
-                const R fx = (R)kx;
-                const cx<R> U = mul_i(Bv), V = mul_mi(cscale(A, fx)), WX = mul_i(cscale(C, fx)),
-                            WY = mul_i(D);
-                if (k == 0) {
-                    // DC terms of real signals: keep the real parts only
-                    z1[idx] = {U.x, R(0)};
-                    z2[idx] = {R(0), WY.x};
-                } else if (pos) {
-                    z1[idx] = {U.x - WX.y, U.y + WX.x};
-                    z2[idx] = {V.x - WY.y, V.y + WY.x};
-                } else {
-                    z1[idx] = {U.x + WX.y, WX.x - U.y};
-                    z2[idx] = {V.x + WY.y, WY.x - V.y};
-                }
-            }
-        }
-        FFT::inverse(ctx, z1, L, t, a.tw);
-        FFT::inverse(ctx, z2, L, t, a.tw);
+
+    // u wx from (ky psi)~ and w~; the loads of psi~ and (ky w)~ are issued
+    // right after, so they arrive while the first pair is transformed.
+    {
+        RowTile<N, R> tb, tc;
+        tb.load(ctx, a, row0, 1);
+        tc.load(ctx, a, row0, 2);
+        tb.store(ctx, a, row0, S0);
+        tc.store(ctx, a, row0, S1);
+    }
+    RowTile<N, R> ta, td;
+#ifdef HZL_ROW_PREFETCH
+    ta.load(ctx, a, row0, 0);
+    td.load(ctx, a, row0, 3);
+#endif
+    ctx.sync();
+    {
+        cx<R> za[E], zb[E];
+        row_spectrum<N, R, true>(za, S0, S1, t, ly);
+        row_spectrum<N, R, true>(zb, S0, S1, t, ly + 1);
+        ctx.sync();
+        FFT::inverse(ctx, za, L, t, a.tw);
 #pragma unroll
         for (int j = 0; j < E; ++j) {
-            p[row][j] = z1[j].x * z1[j].y + z2[j].x * z2[j].y;
-            R s = (z1[j].x < 0 ? -z1[j].x : z1[j].x) + (z2[j].x < 0 ? -z2[j].x : z2[j].x);
-            if (!(s <= R(3.0e38)))
-                s = R(3.0e38) * R(10); // non-finite velocity poisons the maximum
-            vmax = s > vmax ? s : vmax;
+            pa[j] = za[j].x * za[j].y;
+            if constexpr (CFL)
+                ua[j] = za[j].x < 0 ? -za[j].x : za[j].x;
+        }
+        FFT::inverse(ctx, zb, L, t, a.tw);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pb[j] = zb[j].x * zb[j].y;
+            if constexpr (CFL)
+                ub[j] = zb[j].x < 0 ? -zb[j].x : zb[j].x;
         }
     }
+    // v wy from psi~ and (ky w)~ (the staging area doubles as the groups'
+    // exchange areas: every wavefront must be done with them first)
+    ctx.sync();
+#ifndef HZL_ROW_PREFETCH
+    ta.load(ctx, a, row0, 0);
+    td.load(ctx, a, row0, 3);
+#endif
+    ta.store(ctx, a, row0, S0);
+    td.store(ctx, a, row0, S1);
+    ctx.sync();
+    {
+        cx<R> za[E], zb[E];
+        row_spectrum<N, R, false>(za, S0, S1, t, ly);
+        row_spectrum<N, R, false>(zb, S0, S1, t, ly + 1);
+        ctx.sync();
+        FFT::inverse(ctx, za, L, t, a.tw);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pa[j] += za[j].x * za[j].y;
+            if constexpr (CFL) {
+                R s = ua[j] + (za[j].x < 0 ? -za[j].x : za[j].x);
+                if (!(s <= R(1e30)))
+                    s = (R)__builtin_huge_valf(); // non-finite velocity poisons the maximum
+                vmax = s > vmax ? s : vmax;
+            }
+        }
+        FFT::inverse(ctx, zb, L, t, a.tw);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pb[j] += zb[j].x * zb[j].y;
+            if constexpr (CFL) {
+                R s = ub[j] + (zb[j].x < 0 ? -zb[j].x : zb[j].x);
+                if (!(s <= R(1e30)))
+                    s = (R)__builtin_huge_valf();
+                vmax = s > vmax ? s : vmax;
+            }
+        }
+    }
+    // packed forward transform of the two rows' products, keep kx in [0, K]
     cx<R> q[E];
 #pragma unroll
     for (int j = 0; j < E; ++j)
-        q[j] = {p[0][j], p[1][j]};
+        q[j] = {pa[j], pb[j]};
     FFT::forward(ctx, q, L, t, a.tw);
 #pragma unroll
     for (int idx = 0; idx < E; ++idx)
-        L[FFT::index(t, idx)] = q[idx];
-    ctx.sync();
+        if (!FFT::slot_outside(idx, K)) // only |kx| <= K is unpacked
+            L[FFT::index(t, idx)] = q[idx];
+    ctx.wsync();
     cx<R> *nl = a.nl + (long)m * G::MIXF;
 #pragma unroll
     for (int idx = 0; idx < E; ++idx) {
         const int k = FFT::index(t, idx);
         if (live && k <= K) {
             const cx<R> qa = L[k], qb = L[(N - k) % N];
-            const cx<R> pa = {R(0.5) * (qa.x + qb.x), R(0.5) * (qa.y - qb.y)};
-            const cx<R> pb = {R(0.5) * (qa.y + qb.y), R(0.5) * (qb.x - qa.x)};
+            const cx<R> p0 = {R(0.5) * (qa.x + qb.x), R(0.5) * (qa.y - qb.y)};
+            const cx<R> p1 = {R(0.5) * (qa.y + qb.y), R(0.5) * (qb.x - qa.x)};
             const long o = tiled(KP, k, y0);
-            nl[o] = pa;
-            nl[o + 1] = pb;
+            nl[o] = p0;
+            nl[o + 1] = p1;
         }
     }
-    ctx.sync();
-    if (a.cfl) {
-        // reduce |u|+|v| over the T threads of this pair through LDS
-        R *red = reinterpret_cast<R *>(lds);
-        red[g * T + t] = vmax;
-        ctx.sync();
+    if constexpr (CFL) {
+        ctx.wsync();
+        R *red = reinterpret_cast<R *>(L);
+        red[t] = vmax;
+        ctx.wsync();
         if (live && t == 0) {
             R mx = R(0);
-            for (int i = 0; i < T; ++i)
-                mx = red[g * T + i] > mx ? red[g * T + i] : mx;
-            const float prev = a.cfl[gp];
-            const float cur = (float)mx;
+            for (int i = 0; i < T; ++i) {
+                const R v = red[i];
+                mx = (v > mx || v != v) ? v : mx;
+            }
+            const double prev = a.cfl[gp];
+            const double cur = (double)mx;
             a.cfl[gp] = (cur > prev || cur != cur) ? cur : prev;
         }
-        ctx.sync();
     }
 }
 
@@ -445,10 +646,9 @@ template <typename R> struct DiagArgs {
     int K;
     const cx<R> *w0;
     const double *nu;
-    ForcingModel forcing;
-    uint64_t step; // forcing of this step is used for the injection
+    ForcingModel<R> forcing; // forcing of the last step (injection)
     int with_forcing;
-    const float *cfl; // per pair maxima or null
+    const double *cfl; // per pair maxima or null
     int pairs;        // N/2
     int n;
     double *out; // [B][6]: energy, enstrophy, dissipation, injection, cfl_rate, unused
@@ -475,8 +675,8 @@ HZL_HD void diag_body(const Ctx &ctx, const DiagArgs<R> &a, double *red, int nt,
         e += mult * norm / k2;
         z += mult * norm;
         if (a.with_forcing) {
-            const cx<double> F = model_forcing(a.forcing, (int)member, c, ky, a.step);
-            inj += mult * (wr * F.x + wi * F.y) / k2;
+            const cx<R> F = model_forcing(a.forcing, (int)member, c, ky);
+            inj += mult * (wr * (double)F.x + wi * (double)F.y) / k2;
         }
     }
     red[tid] = e;

@@ -27,6 +27,11 @@ struct CpuCtx {
     void sync() const {
         bar->arrive_and_wait();
     }
+    // lanes are independent OS threads here, so wave-local order needs the
+    // full barrier
+    void wsync() const {
+        bar->arrive_and_wait();
+    }
 };
 
 // Launch `blocks` workgroups of `threads` lanes; body(ctx, block, lds).
@@ -51,8 +56,9 @@ template <int N, typename R> struct Sim {
     using G = Geometry<N>;
     using FFT = GroupFFT<N, G::E, R>;
     int B;
-    std::vector<cx<R>> w0, w1, acc, mix, nl, ext, tw;
-    std::vector<float> cfl;
+    std::vector<cx<R>> w0, w1, acc, mix, nl, ext, tw, noise;
+    int npairs = 0;
+    std::vector<double> cfl;
     std::vector<double> nu, alpha, sigma, amp;
     std::vector<uint64_t> keys;
     std::vector<int> pairs;
@@ -63,22 +69,36 @@ template <int N, typename R> struct Sim {
         int kmax_, uint64_t seed)
         : B(b), w0((size_t)b * G::SPEC), w1((size_t)b * G::SPEC), acc((size_t)b * G::SPEC),
           mix((size_t)b * 4 * G::MIXF), nl((size_t)b * G::MIXF), ext((size_t)b * G::SPEC),
-          tw(twiddles<R>(N)), cfl((size_t)b * (N / 2), 0.f), nu(b, nu0), alpha(b, alpha0),
+          tw(twiddles<R>(N)), cfl((size_t)b * (N / 2), 0.0), nu(b, nu0), alpha(b, alpha0),
           sigma(b, sigma0), amp(b, amp0), keys(b), pairs(pair_table(kmin_, kmax_)), kmin(kmin_),
           kmax(kmax_), kf(kf_) {
         for (int m = 0; m < b; ++m)
             keys[m] = stream_seed(seed, (uint64_t)m);
+        pair_table(kmin_, kmax_, &npairs);
+        noise.assign((size_t)b * npairs, cx<R>{0, 0});
     }
 
-    ForcingModel model(double dt) {
-        return ForcingModel{sigma.data(), amp.data(), keys.data(), pairs.data(),
-                            kmin,         kmax,       kf,          dt};
+    ForcingModel<R> model() {
+        return ForcingModel<R>{noise.data(), npairs, amp.data(), pairs.data(), kmin, kmax, kf};
+    }
+
+    void make_noise(double dt) {
+        NoiseArgs<R> a{B, npairs, sigma.data(), keys.data(), dt, step, noise.data()};
+        for (long i = 0; i < (long)B * npairs; ++i)
+            noise_body<R>(a, i);
+    }
+
+    template <int MODE> void col_mode(const StageArgs<R> &a) {
+        const long blocks = ((long)B * G::KP + G::COL_G - 1) / G::COL_G;
+        launch(blocks, G::COL_THREADS, sizeof(cx<R>) * G::COL_G * FFT::LDS_ELEMS,
+               [&](const CpuCtx &ctx, long b, unsigned char *lds) {
+                   col_body<N, MODE, R>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+               });
     }
 
     void col(int mode, double dt, const cx<R> *e, long long es) {
         StageArgs<R> a;
         a.B = B;
-        a.mode = mode;
         a.w0 = w0.data();
         a.w1 = w1.data();
         a.acc = acc.data();
@@ -87,26 +107,35 @@ template <int N, typename R> struct Sim {
         a.tw = tw.data();
         a.nu = nu.data();
         a.alpha = alpha.data();
-        a.forcing = model(dt);
+        a.forcing = model();
         a.ext = e;
         a.ext_stride = es;
         a.h_half = (R)(dt / 2);
         a.h_full = (R)dt;
         a.h_sixth = (R)(dt / 6);
-        a.step = step;
-        const long blocks = ((long)B * G::KP + G::COL_G - 1) / G::COL_G;
-        launch(blocks, G::COL_THREADS, sizeof(cx<R>) * G::COL_G * FFT::LDS_ELEMS,
-               [&](const CpuCtx &ctx, long b, unsigned char *lds) {
-                   col_body<N, R>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
-               });
+        switch (mode) {
+        case 0:
+            return col_mode<0>(a);
+        case 1:
+            return col_mode<1>(a);
+        case 2:
+            return col_mode<2>(a);
+        case 3:
+            return col_mode<3>(a);
+        default:
+            return col_mode<4>(a);
+        }
     }
 
     void row(bool with_cfl) {
         RowArgs<R> a{B, mix.data(), nl.data(), tw.data(), with_cfl ? cfl.data() : nullptr};
         const long blocks = ((long)B * (N / 2) + G::ROW_PPW - 1) / G::ROW_PPW;
-        launch(blocks, G::ROW_THREADS, sizeof(cx<R>) * G::ROW_PPW * FFT::LDS_ELEMS,
+        launch(blocks, G::ROW_THREADS, sizeof(cx<R>) * RowShape<N>::template lds_elems<R>(),
                [&](const CpuCtx &ctx, long b, unsigned char *lds) {
-                   row_body<N, R>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+                   if (with_cfl)
+                       row_body<N, R, true>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+                   else
+                       row_body<N, R, false>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
                });
     }
 
@@ -118,8 +147,9 @@ template <int N, typename R> struct Sim {
     }
 
     void diag(double dt, double *out) {
-        DiagArgs<R> a{G::K, w0.data(), nu.data(), model(dt), step > 0 ? step - 1 : 0,
-                      step > 0 ? 1 : 0, cfl.data(), N / 2, N, out};
+        (void)dt;
+        DiagArgs<R> a{G::K, w0.data(), nu.data(), model(), step > 0 ? 1 : 0, cfl.data(), N / 2,
+                      N, out};
         launch(B, 64, sizeof(double) * 3 * 64, [&](const CpuCtx &ctx, long b, unsigned char *lds) {
             diag_body<R>(ctx, a, reinterpret_cast<double *>(lds), 64, b);
         });
@@ -131,6 +161,8 @@ template <int N, typename R> struct Sim {
                 col(0, dt, e, es);
                 mix_valid = true;
             }
+            if (!e)
+                make_noise(dt);
             for (int stage = 1; stage <= 4; ++stage) {
                 row(stage == 1);
                 col(stage, dt, e, es);
