@@ -6,7 +6,7 @@ loaded with `load_package()` from __graft_entry__.py (or importlib) under the
 module name `hzl_sfi_amd`.
 """
 from .engine import (ALL, EXPORTS, FP32, FP64, Ensemble, HzlError, HzlInvalidArgument,
-                     Parameters, load_library)
+                     Parameters, bootstrap_filter, load_library)
 
 __all__ = ["ALL", "EXPORTS", "FP32", "FP64", "Ensemble", "HzlError", "HzlInvalidArgument",
-           "Parameters", "load_library"]
+           "Parameters", "bootstrap_filter", "load_library"]

@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
+#include <random>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -78,6 +80,30 @@ template <typename R> __global__ void __launch_bounds__(256) k_grf(GrfArgs<R> a)
 template <typename R> __global__ void __launch_bounds__(256) k_diag(DiagArgs<R> a) {
     __shared__ double red[3 * 256];
     diag_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
+}
+
+template <typename R> __global__ void __launch_bounds__(256) k_pf_loglik(PfArgs<R> a) {
+    __shared__ double red[256];
+    pf_loglik_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_pf_normalize(PfControl a) {
+    __shared__ double red[2 * 256];
+    pf_normalize_body(DevCtx{}, a, red, 256);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_pf_mean(int B, long spec, const cx<R> *w0,
+                                                 const double *wn, double *mean) {
+    pf_mean_body<R>(B, spec, w0, wn, mean, (long)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_pf_gather(int B, long spec, const cx<R> *src, cx<R> *dst,
+                                                   const int *anc) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < (long)B * spec)
+        dst[i] = src[(long)anc[i / spec] * spec + i % spec];
 }
 
 // ------------------------------------------------------------ helpers ----
@@ -201,6 +227,11 @@ class EngineBase {
     virtual void synchronize() = 0;
     virtual void diagnostics(double *out) = 0;
     virtual void profile_step(double dt, float *ms) = 0;
+    virtual void pf_begin(uint64_t seed, size_t max_groups) = 0;
+    virtual void pf_assimilate(double dt, size_t steps, size_t nobs, const int *x, const int *y,
+                               const double *value, const double *sd, bool is_last,
+                               double *record, double *mean_field) = 0;
+    virtual void pf_result(double *log_likelihood, size_t *resampling_events) = 0;
     virtual uint64_t step_index() const = 0;
     virtual void device_state(void **ptr, size_t *stride) = 0;
     virtual void *stream() = 0;
@@ -397,6 +428,131 @@ template <typename R> class Engine final : public EngineBase {
             (void)hipEventDestroy(e);
     }
 
+    // ------------------------------------------------ particle filter ----
+    void pf_begin(uint64_t seed, size_t max_groups) override {
+        if (B_ < 2)
+            throw std::invalid_argument("particle filter needs at least two particles");
+        pf_seed_ = seed;
+        pf_group_ = 0;
+        pf_wstate_ = DeviceBuffer<double>(B_);
+        pf_wnorm_ = DeviceBuffer<double>(B_);
+        pf_logw_ = DeviceBuffer<double>(B_);
+        pf_stats_ = DeviceBuffer<double>(8);
+        pf_anc_ = DeviceBuffer<int>(B_);
+        pf_mean_ = DeviceBuffer<double>(2 * spec_);
+        std::vector<double> w(B_, 1.0 / (double)B_), st(8, 0.0), cs(n_), sn(n_);
+        upload(pf_wstate_.p, w.data(), B_);
+        upload(pf_stats_.p, st.data(), 8);
+        // resampling uniforms: mt19937_64(stream_seed(seed, UINT64_MAX)), one per
+        // resampling event, (x >> 11) 2^-53 (src/filter.cpp:11-13,54)
+        std::mt19937_64 rng(stream_seed(seed, std::numeric_limits<uint64_t>::max()));
+        std::vector<double> u(std::max<size_t>(max_groups, 1));
+        for (auto &x : u)
+            x = (double)(rng() >> 11) * 0x1.0p-53;
+        pf_uniforms_ = DeviceBuffer<double>(u.size());
+        upload(pf_uniforms_.p, u.data(), u.size());
+        for (int j = 0; j < n_; ++j) {
+            cs[j] = std::cos(2 * PI * j / n_);
+            sn[j] = std::sin(2 * PI * j / n_);
+        }
+        pf_cs_ = DeviceBuffer<double>(n_);
+        pf_sn_ = DeviceBuffer<double>(n_);
+        upload(pf_cs_.p, cs.data(), n_);
+        upload(pf_sn_.p, sn.data(), n_);
+        pf_on_ = true;
+    }
+
+    void pf_assimilate(double dt, size_t steps, size_t nobs, const int *x, const int *y,
+                       const double *value, const double *sd, bool is_last, double *record,
+                       double *mean_field) override {
+        if (!pf_on_)
+            throw std::invalid_argument("particle filter not started");
+        if (nobs == 0)
+            throw std::invalid_argument("observation group is empty");
+        for (size_t o = 0; o < nobs; ++o)
+            if (x[o] < 0 || y[o] < 0 || x[o] >= n_ || y[o] >= n_ || !std::isfinite(value[o]) ||
+                !std::isfinite(sd[o]) || sd[o] <= 0)
+                throw std::invalid_argument("observations must be finite, in bounds, and have "
+                                            "positive uncertainty");
+        if (steps)
+            step(dt, steps);
+        if (!mix_valid_) {
+            dispatch_col<R>(n_, 0, stage_args(dt, nullptr, 0), stream_);
+            mix_valid_ = true;
+        }
+        if (pf_obs_cap_ < nobs) {
+            pf_ox_ = DeviceBuffer<int>(nobs);
+            pf_oy_ = DeviceBuffer<int>(nobs);
+            pf_ov_ = DeviceBuffer<double>(nobs);
+            pf_osd_ = DeviceBuffer<double>(nobs);
+            pf_obs_cap_ = nobs;
+        }
+        upload(pf_ox_.p, x, nobs);
+        upload(pf_oy_.p, y, nobs);
+        upload(pf_ov_.p, value, nobs);
+        upload(pf_osd_.p, sd, nobs);
+        PfArgs<R> a{(int)B_,   n_,        K_,           mix_.p,       (int)nobs,
+                    pf_ox_.p,  pf_oy_.p,  pf_ov_.p,     pf_osd_.p,    pf_cs_.p,
+                    pf_sn_.p,  pf_wstate_.p, pf_wnorm_.p, pf_logw_.p, nullptr};
+        hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
+        PfControl c{(int)B_,         pf_logw_.p,     pf_wstate_.p,  pf_wnorm_.p,
+                    pf_stats_.p,     pf_anc_.p,      pf_uniforms_.p, (int)pf_uniforms_.count,
+                    d_keys_.p,       pf_seed_,       pf_group_,     is_last ? 1 : 0};
+        hipLaunchKernelGGL(k_pf_normalize, dim3(1), dim3(256), 0, stream_, c);
+        if (mean_field) {
+            const long blocks = ((long)spec_ + 255) / 256;
+            hipLaunchKernelGGL(k_pf_mean<R>, dim3((unsigned)blocks), dim3(256), 0, stream_,
+                               (int)B_, (long)spec_, (const cx<R> *)w0_.p, pf_wnorm_.p,
+                               pf_mean_.p);
+        }
+        const long total = (long)B_ * (long)spec_;
+        hipLaunchKernelGGL(k_pf_gather<R>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                           stream_, (int)B_, (long)spec_, (const cx<R> *)w0_.p, w1_.p,
+                           (const int *)pf_anc_.p);
+        check(hipMemcpyAsync(w0_.p, w1_.p, total * sizeof(cx<R>), hipMemcpyDeviceToDevice,
+                             stream_),
+              "ancestor copy");
+        check(hipGetLastError(), "particle filter kernels");
+        mix_valid_ = false;
+        double st[8];
+        check(hipMemcpyAsync(st, pf_stats_.p, sizeof(st), hipMemcpyDeviceToHost, stream_),
+              "filter statistics");
+        std::vector<double> mean;
+        if (mean_field) {
+            mean.resize(2 * spec_);
+            check(hipMemcpyAsync(mean.data(), pf_mean_.p, mean.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, stream_),
+                  "filter mean");
+        }
+        check(hipStreamSynchronize(stream_), "particle filter");
+        if (record) {
+            record[0] = st[0];
+            record[1] = st[1];
+            record[2] = st[2];
+            record[3] = st[3];
+        }
+        if (mean_field)
+            half_to_full<double>(n_, reinterpret_cast<const cx<double> *>(mean.data()),
+                                 mean_field);
+        ++pf_group_;
+        // refresh the host copy of the member keys (children were reseeded)
+        if (st[3] != 0)
+            check(hipMemcpy(keys_.data(), d_keys_.p, B_ * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost),
+                  "keys download");
+    }
+
+    void pf_result(double *log_likelihood, size_t *resampling_events) override {
+        if (!pf_on_)
+            throw std::invalid_argument("particle filter not started");
+        double st[8];
+        check(hipMemcpy(st, pf_stats_.p, sizeof(st), hipMemcpyDeviceToHost), "filter result");
+        if (log_likelihood)
+            *log_likelihood = st[4];
+        if (resampling_events)
+            *resampling_events = (size_t)st[5];
+    }
+
     uint64_t step_index() const override {
         return step_;
     }
@@ -513,6 +669,13 @@ template <typename R> class Engine final : public EngineBase {
     size_t spec_ = 0, mixf_ = 0;
     hipStream_t stream_ = nullptr;
     DeviceBuffer<cx<R>> w0_, w1_, acc_, mix_, nl_, ext_, tw_, noise_;
+    // particle filter state
+    bool pf_on_ = false;
+    uint64_t pf_seed_ = 0, pf_group_ = 0;
+    size_t pf_obs_cap_ = 0;
+    DeviceBuffer<double> pf_wstate_, pf_wnorm_, pf_logw_, pf_stats_, pf_mean_, pf_uniforms_,
+        pf_cs_, pf_sn_, pf_ov_, pf_osd_;
+    DeviceBuffer<int> pf_anc_, pf_ox_, pf_oy_;
     int npairs_ = 0;
     DeviceBuffer<double> cfl_, cfl_diag_;
     DeviceBuffer<int> pairs_;
@@ -665,6 +828,25 @@ int hzl_ensemble_profile_step(hzl_ensemble_t h, double dt, float *kernel_ms) {
             throw std::invalid_argument("null output");
         engine_of(h).profile_step(dt, kernel_ms);
     });
+}
+
+int hzl_pf_begin(hzl_ensemble_t h, uint64_t seed, size_t max_groups) {
+    return guarded([&] { engine_of(h).pf_begin(seed, max_groups); });
+}
+
+int hzl_pf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, const int *x,
+                      const int *y, const double *value, const double *sd, int is_last,
+                      double *record, double *mean_field) {
+    return guarded([&] {
+        if (!x || !y || !value || !sd)
+            throw std::invalid_argument("null observation array");
+        engine_of(h).pf_assimilate(dt, steps, nobs, x, y, value, sd, is_last != 0, record,
+                                   mean_field);
+    });
+}
+
+int hzl_pf_result(hzl_ensemble_t h, double *log_likelihood, size_t *resampling_events) {
+    return guarded([&] { engine_of(h).pf_result(log_likelihood, resampling_events); });
 }
 
 int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out) {

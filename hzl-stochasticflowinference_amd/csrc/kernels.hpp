@@ -103,6 +103,14 @@ HZL_HD void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1) {
     }
 }
 
+// SplitMix64-style logical stream identity (reference src/solver.cpp:61-66).
+HZL_HD uint64_t stream_seed(uint64_t base, uint64_t index) {
+    uint64_t x = base + 0x9e3779b97f4a7c15ULL * (index + 1);
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
 constexpr uint32_t TAG_NOISE = 0x4E4F4953u; // "NOIS"
 constexpr uint32_t TAG_GRF = 0x47524631u;   // "GRF1"
 constexpr double PI = 3.14159265358979323846;
@@ -708,6 +716,195 @@ HZL_HD void diag_body(const Ctx &ctx, const DiagArgs<R> &a, double *red, int nt,
         o[5] = 0;
     }
     ctx.sync();
+}
+
+} // namespace hzl_amd
+
+namespace hzl_amd {
+
+// ------------------------------------------------------ particle filter ----
+// Bootstrap particle filter pieces (reference src/filter.cpp:38-126), FP64.
+template <typename R> struct PfArgs {
+    int B, N, K;
+    const cx<R> *mix;      // mixed fields of the current states (field 2 = w~)
+    int nobs;
+    const int *ox, *oy;    // observation grid indices
+    const double *ov, *osd;
+    const double *cs, *sn; // cos / sin (2 pi j / N), j < N
+    double *wstate;        // running weights (1/B after resampling)
+    double *wnorm;         // normalised weights of this group
+    double *logw;
+    double *pred;          // [B][nobs] predictions (optional)
+};
+
+// Point value of member m's physical vorticity at grid (x, y) from the mixed
+// field: w(x, y) = Re w~(0, y) + 2 sum_{kx=1..K} Re(w~(kx, y) e^{2 pi i kx x / N}).
+template <typename R>
+HZL_HD double point_value(const PfArgs<R> &a, const cx<R> *w, int x, int y) {
+    const int KP = a.K + 1;
+    double s = 0;
+    for (int kx = 0; kx <= a.K; ++kx) {
+        const cx<R> c = w[tiled(KP, kx, y)];
+        const int j = (int)(((long)kx * x) % a.N);
+        const double t = (double)c.x * a.cs[j] - (double)c.y * a.sn[j];
+        s += kx == 0 ? t : 2 * t;
+    }
+    return s;
+}
+
+// One workgroup of nt threads per particle m: Gaussian log-weight increment
+// log w_prev - sum_o [r^2/2 + log sd + log(2 pi)/2] (src/filter.cpp:66-74).
+template <typename R, class Ctx>
+HZL_HD void pf_loglik_body(const Ctx &ctx, const PfArgs<R> &a, double *red, int nt, long m) {
+    const int tid = ctx.tid();
+    const cx<R> *w = a.mix + m * 4 * (long)(a.K + 1) * a.N + 2 * (long)(a.K + 1) * a.N;
+    const double half_log_2pi = 0.5 * log(2 * PI);
+    double acc = 0;
+    for (int o = tid; o < a.nobs; o += nt) {
+        const double v = point_value(a, w, a.ox[o], a.oy[o]);
+        if (a.pred)
+            a.pred[m * a.nobs + o] = v;
+        const double r = (v - a.ov[o]) / a.osd[o];
+        acc += 0.5 * r * r + log(a.osd[o]) + half_log_2pi;
+    }
+    red[tid] = acc;
+    ctx.sync();
+    for (int s = nt / 2; s > 0; s /= 2) {
+        if (tid < s)
+            red[tid] += red[tid + s];
+        ctx.sync();
+    }
+    if (tid == 0) {
+        const double wp = a.wstate[m];
+        a.logw[m] = (wp > 0 ? log(wp) : -__builtin_huge_val()) - red[0];
+    }
+    ctx.sync();
+}
+
+struct PfControl {
+    int B;
+    const double *logw;
+    double *wstate, *wnorm;
+    double *stats;           // [0] increment [1] ESS [2] max weight [3] resampled [4] total log-lik
+                             // [5] resampling events [6] uniforms used
+    int *ancestors;
+    const double *uniforms;  // pre-drawn uniforms of the resampling stream
+    int max_uniforms;
+    uint64_t *keys;          // member Philox keys (children are reseeded)
+    uint64_t seed;
+    uint64_t group;
+    int is_last;
+};
+
+// Single workgroup: log-sum-exp normalisation, ESS, systematic resampling
+// below B/2 unless this is the last group (src/filter.cpp:76-120).
+template <class Ctx>
+HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, int nt) {
+    const int tid = ctx.tid();
+    double mx = -__builtin_huge_val();
+    for (int i = tid; i < a.B; i += nt)
+        mx = a.logw[i] > mx ? a.logw[i] : mx;
+    red[tid] = mx;
+    ctx.sync();
+    for (int s = nt / 2; s > 0; s /= 2) {
+        if (tid < s)
+            red[tid] = red[tid + s] > red[tid] ? red[tid + s] : red[tid];
+        ctx.sync();
+    }
+    mx = red[0];
+    ctx.sync();
+    const bool finite = mx > -__builtin_huge_val() && mx < __builtin_huge_val();
+    double tot = 0;
+    if (finite)
+        for (int i = tid; i < a.B; i += nt)
+            tot += exp(a.logw[i] - mx);
+    red[tid] = tot;
+    ctx.sync();
+    for (int s = nt / 2; s > 0; s /= 2) {
+        if (tid < s)
+            red[tid] += red[tid + s];
+        ctx.sync();
+    }
+    const double inc = finite ? mx + log(red[0]) : -__builtin_huge_val();
+    ctx.sync();
+    double sq = 0, mw = 0;
+    for (int i = tid; i < a.B; i += nt) {
+        const double w = finite ? exp(a.logw[i] - inc) : 0.0;
+        a.wnorm[i] = w;
+        sq += w * w;
+        mw = w > mw ? w : mw;
+    }
+    red[tid] = sq;
+    red[nt + tid] = mw;
+    ctx.sync();
+    for (int s = nt / 2; s > 0; s /= 2) {
+        if (tid < s) {
+            red[tid] += red[tid + s];
+            red[nt + tid] = red[nt + tid + s] > red[nt + tid] ? red[nt + tid + s] : red[nt + tid];
+        }
+        ctx.sync();
+    }
+    const double ess = 1 / red[0];
+    const bool resample = finite && ess < (double)a.B / 2 && !a.is_last &&
+                          (int)a.stats[6] < a.max_uniforms;
+    if (tid == 0) {
+        a.stats[0] = inc;
+        a.stats[1] = ess;
+        a.stats[2] = red[nt];
+        a.stats[3] = resample ? 1 : 0;
+        a.stats[4] += inc;
+        if (resample) {
+            // one uniform per resampling event, strict `position > cumulative`
+            const double u = a.uniforms[(int)a.stats[6]];
+            a.stats[6] += 1;
+            a.stats[5] += 1;
+            int anc = 0;
+            double cum = a.wnorm[0];
+            const double start = u / (double)a.B;
+            for (int i = 0; i < a.B; ++i) {
+                const double pos = start + (double)i / (double)a.B;
+                while (pos > cum && anc + 1 < a.B)
+                    cum += a.wnorm[++anc];
+                a.ancestors[i] = anc;
+            }
+        }
+    }
+    ctx.sync();
+    for (int i = tid; i < a.B; i += nt) {
+        a.wstate[i] = resample ? 1.0 / (double)a.B : a.wnorm[i];
+        if (resample)
+            a.keys[i] = stream_seed(a.seed, (a.group + 1) * (uint64_t)a.B + (uint64_t)i);
+        else
+            a.ancestors[i] = i;
+    }
+    ctx.sync();
+}
+
+// Weighted mean of the retained states with the group's normalised weights,
+// accumulated over particles in order (src/filter.cpp:94-100).
+template <typename R>
+HZL_HD void pf_mean_body(int B, long spec, const cx<R> *w0, const double *wn, double *mean,
+                         long e) {
+    if (e >= spec)
+        return;
+    double re = 0, im = 0;
+    for (int m = 0; m < B; ++m) {
+        const cx<R> v = w0[m * spec + e];
+        re += wn[m] * (double)v.x;
+        im += wn[m] * (double)v.y;
+    }
+    mean[2 * e] = re;
+    mean[2 * e + 1] = im;
+}
+
+// Ancestor gather dst[m] = src[anc[m]] when the group resampled.
+template <typename R>
+HZL_HD void pf_gather_body(int B, long spec, const cx<R> *src, cx<R> *dst, const int *anc,
+                           const double *stats, long i) {
+    if (stats[3] == 0 || i >= (long)B * spec)
+        return;
+    const long m = i / spec, e = i % spec;
+    dst[i] = src[(long)anc[m] * spec + e];
 }
 
 } // namespace hzl_amd

@@ -11,14 +11,6 @@
 
 namespace hzl_amd {
 
-// SplitMix64-style logical stream identity (reference src/solver.cpp:61-66).
-inline uint64_t stream_seed(uint64_t base, uint64_t index) {
-    uint64_t x = base + 0x9e3779b97f4a7c15ULL * (index + 1);
-    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
-    return x ^ (x >> 31);
-}
-
 inline int wave_number(long i, long n) {
     return i < n / 2 ? (int)i : (int)(i - n);
 }

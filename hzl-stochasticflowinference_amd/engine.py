@@ -28,6 +28,7 @@ EXPORTS = (
     "hzl_ensemble_synchronize", "hzl_ensemble_diagnostics", "hzl_ensemble_profile_step",
     "hzl_ensemble_step_index",
     "hzl_ensemble_device_state", "hzl_ensemble_stream",
+    "hzl_pf_begin", "hzl_pf_assimilate", "hzl_pf_result",
 )
 
 
@@ -78,6 +79,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_step_index.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.hzl_ensemble_device_state.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_size_t)]
     L.hzl_ensemble_stream.argtypes = [vp, C.POINTER(vp)]
+    ip = C.POINTER(C.c_int)
+    L.hzl_pf_begin.argtypes = [vp, C.c_uint64, C.c_size_t]
+    L.hzl_pf_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp, _dp,
+                                    C.c_int, _dp, _dp]
+    L.hzl_pf_result.argtypes = [vp, _dp, C.POINTER(C.c_size_t)]
     for name in EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -193,3 +199,60 @@ class Ensemble:
         p = C.c_void_p()
         _check(_lib.hzl_ensemble_stream(self._h, C.byref(p)))
         return p.value
+
+    # -- particle filter (members are particles) ---------------------------
+    def pf_begin(self, seed: int, max_groups: int = 4096):
+        _check(_lib.hzl_pf_begin(self._h, seed, max_groups))
+
+    def pf_assimilate(self, dt, steps, x, y, value, sd, is_last, mean=False):
+        x = np.ascontiguousarray(x, dtype=np.int32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        value = np.ascontiguousarray(value, dtype=np.float64)
+        sd = np.ascontiguousarray(sd, dtype=np.float64)
+        rec = np.zeros(4)
+        mf = np.zeros((self.n, self.n), dtype=np.complex128) if mean else None
+        ip = C.POINTER(C.c_int)
+        _check(_lib.hzl_pf_assimilate(self._h, dt, steps, len(x), x.ctypes.data_as(ip),
+                                      y.ctypes.data_as(ip), value.ctypes.data_as(_dp),
+                                      sd.ctypes.data_as(_dp), int(is_last),
+                                      rec.ctypes.data_as(_dp),
+                                      mf.ctypes.data_as(_dp) if mean else None))
+        return rec, mf
+
+    def pf_result(self):
+        ll = C.c_double()
+        ev = C.c_size_t()
+        _check(_lib.hzl_pf_result(self._h, C.byref(ll), C.byref(ev)))
+        return ll.value, ev.value
+
+
+def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,
+                     retain_states: bool = True):
+    """Bootstrap particle filter over an initialised ensemble.
+
+    Mirrors hzl::bootstrap_filter (src/filter.cpp:38-126): `observations` rows are
+    (step, x, y, value, sd), time ordered; rows sharing a step form one group.
+    Returns dict(log_likelihood, resampling_events, history) with history
+    records (step, increment, ess, max_weight, resampled, mean_field).
+    """
+    obs = np.asarray(observations, dtype=np.float64)
+    if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
+        raise ValueError("observations must be rows of (step, x, y, value, sd)")
+    if np.any(np.diff(obs[:, 0]) < 0):
+        raise ValueError("observations must be time ordered")
+    steps = obs[:, 0].astype(np.int64)
+    groups = np.unique(steps)
+    particles.pf_begin(seed, max_groups=len(groups) + 1)
+    history, current = [], 0
+    for gi, target in enumerate(groups):
+        sel = obs[steps == target]
+        rec, mean = particles.pf_assimilate(dt, int(target - current), sel[:, 1], sel[:, 2],
+                                            sel[:, 3], sel[:, 4], gi == len(groups) - 1,
+                                            mean=retain_states)
+        history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
+                            max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean))
+        current = int(target)
+        if not np.isfinite(rec[0]):
+            break
+    ll, events = particles.pf_result()
+    return dict(log_likelihood=ll, resampling_events=events, history=history)

@@ -127,6 +127,33 @@ int hzl_ensemble_diagnostics(hzl_ensemble_t h, double *out);
  * Measurement-harness counterpart of src/cuda_ensemble_benchmark.cpp:73-79. */
 int hzl_ensemble_profile_step(hzl_ensemble_t h, double dt, float *kernel_ms);
 
+/* ------------------------------------------------------- particle filter --
+ * Bootstrap particle filter over the ensemble's members as particles
+ * (reference bootstrap_filter, src/filter.cpp:38-126; include/hzl/filter.hpp:17-23).
+ * Members keep the keys stream_seed(seed_of_create, i); after a resampling
+ * event at observation group g, child i is reseeded with
+ * stream_seed(seed, (g + 1) * particles + i) (src/filter.cpp:112-115).
+ * Systematic resampling uses one uniform per event from
+ * mt19937_64(stream_seed(seed, UINT64_MAX)) (src/filter.cpp:54,102-110). */
+
+/* Start a filter: weights 1/N, log-likelihood 0; at most max_groups
+ * resampling events are supported. */
+int hzl_pf_begin(hzl_ensemble_t h, uint64_t seed, size_t max_groups);
+
+/* Advance every particle `steps` RK4 steps (engine forcing), then assimilate
+ * one observation group (grid points x, y; values; standard deviations):
+ * FP64 Gaussian log-weights, log-sum-exp increment, ESS, maximum weight, and
+ * systematic resampling when ESS < N/2 unless is_last.  record[4] receives
+ * (increment, ESS, max weight, resampled 0/1); mean_field (optional, reference
+ * Field layout) the weighted mean state with this group's weights. */
+int hzl_pf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, const int *x,
+                      const int *y, const double *value, const double *sd, int is_last,
+                      double *record, double *mean_field);
+
+/* Accumulated log-likelihood (sum of increments; -inf once every weight was
+ * -inf, src/filter.cpp:76-80) and number of resampling events. */
+int hzl_pf_result(hzl_ensemble_t h, double *log_likelihood, size_t *resampling_events);
+
 /* Number of steps taken so far (the noise counter of the next step). */
 int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out);
 
