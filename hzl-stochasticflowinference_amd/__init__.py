@@ -5,8 +5,9 @@ package only binds it.  The directory name is not a Python identifier, so it is
 loaded with `load_package()` from __graft_entry__.py (or importlib) under the
 module name `hzl_sfi_amd`.
 """
-from .engine import (ALL, EXPORTS, FP32, FP64, Ensemble, HzlError, HzlInvalidArgument,
-                     Parameters, bootstrap_filter, load_library)
+from .engine import (ALL, BF16, EXPORTS, FP32, FP64, Ensemble, HzlError, HzlInvalidArgument,
+                     Parameters, Surrogate, bootstrap_filter, load_library, surrogate_filter)
 
-__all__ = ["ALL", "EXPORTS", "FP32", "FP64", "Ensemble", "HzlError", "HzlInvalidArgument",
-           "Parameters", "bootstrap_filter", "load_library"]
+__all__ = ["ALL", "BF16", "EXPORTS", "FP32", "FP64", "Ensemble", "HzlError",
+           "HzlInvalidArgument", "Parameters", "Surrogate", "bootstrap_filter", "load_library",
+           "surrogate_filter"]

@@ -5,6 +5,7 @@
 // device-generated forcing.  See kernels.hpp for the data layout and the
 // split of one RHS into a column kernel and a row kernel.
 #include "hzl_amd.h"
+#include "engine.hpp"
 #include "tables.hpp"
 #include <hip/hip_runtime.h>
 
@@ -19,30 +20,6 @@
 #include <vector>
 
 namespace hzl_amd {
-
-struct DevCtx {
-    __host__ __device__ int tid() const {
-#if __HIP_DEVICE_COMPILE__
-        return (int)threadIdx.x;
-#else
-        return 0;
-#endif
-    }
-    __host__ __device__ void sync() const {
-#if __HIP_DEVICE_COMPILE__
-        __syncthreads();
-#endif
-    }
-    // Wave-local ordering of LDS traffic: a compiler barrier plus wavefront-
-    // scope fences (the hardware completes one wavefront's LDS ops in order).
-    __host__ __device__ void wsync() const {
-#if __HIP_DEVICE_COMPILE__
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#endif
-    }
-};
 
 // ------------------------------------------------------------ kernels ----
 // Register budgets (waves per SIMD) chosen by measurement on MI355X.
@@ -214,29 +191,6 @@ template <typename R> void dispatch_row(int n, const RowArgs<R> &a, hipStream_t 
 }
 
 // ------------------------------------------------------------- engine ----
-class EngineBase {
-  public:
-    virtual ~EngineBase() = default;
-    virtual void set_member_parameters(size_t m, double nu, double alpha, double sigma,
-                                       double amp) = 0;
-    virtual void set_state(size_t m, const double *field) = 0;
-    virtual void get_state(size_t m, double *field) = 0;
-    virtual void init_grf() = 0;
-    virtual void step(double dt, size_t steps) = 0;
-    virtual void step_forced(double dt, const double *forcing, bool shared) = 0;
-    virtual void synchronize() = 0;
-    virtual void diagnostics(double *out) = 0;
-    virtual void profile_step(double dt, float *ms) = 0;
-    virtual void pf_begin(uint64_t seed, size_t max_groups) = 0;
-    virtual void pf_assimilate(double dt, size_t steps, size_t nobs, const int *x, const int *y,
-                               const double *value, const double *sd, bool is_last,
-                               double *record, double *mean_field) = 0;
-    virtual void pf_result(double *log_likelihood, size_t *resampling_events) = 0;
-    virtual uint64_t step_index() const = 0;
-    virtual void device_state(void **ptr, size_t *stride) = 0;
-    virtual void *stream() = 0;
-};
-
 template <typename R> class Engine final : public EngineBase {
   public:
     Engine(int n, size_t members, const hzl_parameters &p, uint64_t seed)
@@ -566,6 +520,22 @@ template <typename R> class Engine final : public EngineBase {
         return stream_;
     }
 
+    EngineViews views() override {
+        flush_parameters();
+        return EngineViews{n_,        K_,        (int)B_,        sizeof(R) == 4 ? 0 : 1,
+                           w0_.p,     d_nu_.p,   d_alpha_.p,     d_sigma_.p,
+                           d_amp_.p,  d_keys_.p, pairs_.p,       p_.forcing_min,
+                           p_.forcing_max, p_.kolmogorov_wavenumber, stream_};
+    }
+
+    void invalidate_mixed() override {
+        mix_valid_ = false;
+    }
+
+    void advance_counter(uint64_t steps) override {
+        step_ += steps;
+    }
+
   private:
     void member_check(size_t m) const {
         if (m >= B_)
@@ -715,37 +685,27 @@ struct hzl_ensemble {
     std::unique_ptr<hzl_amd::EngineBase> engine;
 };
 
-namespace {
-thread_local std::string g_error;
-
-template <class F> int guarded(F &&f) {
-    try {
-        f();
-        g_error.clear();
-        return HZL_OK;
-    } catch (const std::invalid_argument &e) {
-        g_error = e.what();
-        return HZL_EINVAL;
-    } catch (const std::exception &e) {
-        g_error = e.what();
-        return HZL_ERUNTIME;
-    } catch (...) {
-        g_error = "unknown error";
-        return HZL_ERUNTIME;
-    }
+namespace hzl_amd {
+std::string &error_slot() {
+    thread_local std::string message;
+    return message;
 }
+} // namespace hzl_amd
+using hzl_amd::guarded;
 
-hzl_amd::EngineBase &engine_of(hzl_ensemble_t h) {
+namespace hzl_amd {
+EngineBase &engine_of(hzl_ensemble_t h) {
     if (!h || !h->engine)
         throw std::invalid_argument("null ensemble handle");
     return *h->engine;
 }
-} // namespace
+} // namespace hzl_amd
+using hzl_amd::engine_of;
 
 extern "C" {
 
 const char *hzl_last_error(void) {
-    return g_error.c_str();
+    return hzl_amd::error_slot().c_str();
 }
 
 int hzl_ensemble_create(hzl_ensemble_t *out, size_t n, size_t members, int precision,

@@ -1,0 +1,108 @@
+// engine.hpp -- internal interface between the engine's translation units.
+#pragma once
+#include "kernels.hpp"
+#include <cstddef>
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace hzl_amd {
+
+// Execution context of device kernels (the CPU lane emulator has its own).
+struct DevCtx {
+    __host__ __device__ int tid() const {
+#if __HIP_DEVICE_COMPILE__
+        return (int)threadIdx.x;
+#else
+        return 0;
+#endif
+    }
+    __host__ __device__ void sync() const {
+#if __HIP_DEVICE_COMPILE__
+        __syncthreads();
+#endif
+    }
+    // Wave-local ordering of LDS traffic: a compiler barrier plus wavefront-
+    // scope fences (the hardware completes one wavefront's LDS ops in order).
+    __host__ __device__ void wsync() const {
+#if __HIP_DEVICE_COMPILE__
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+    }
+    // host-only placeholders so host-side template checks of HD bodies that
+    // emulate matrix instructions on the CPU compile; never called
+    template <class A, class B, class C> void mfma_bf16(const A &, const B &, C &, int) const {}
+    template <class A, class B, class C> void mfma_f32(const A &, const B &, C &, int) const {}
+};
+
+// Raw device pointers and sizes of an ensemble, for modules that operate on
+// its members in place (FP32 engines only where noted).
+struct EngineViews {
+    int n, K, B, precision; // precision: 0 FP32, 1 FP64
+    void *w0;               // retained half-plane states [B][K+1][2K+1] complex
+    double *nu, *alpha, *sigma, *amp;
+    uint64_t *keys;
+    const int *pair_of;
+    int kmin, kmax, kf;
+    hipStream_t stream;
+};
+
+class EngineBase {
+  public:
+    virtual ~EngineBase() = default;
+    virtual void set_member_parameters(size_t m, double nu, double alpha, double sigma,
+                                       double amp) = 0;
+    virtual void set_state(size_t m, const double *field) = 0;
+    virtual void get_state(size_t m, double *field) = 0;
+    virtual void init_grf() = 0;
+    virtual void step(double dt, size_t steps) = 0;
+    virtual void step_forced(double dt, const double *forcing, bool shared) = 0;
+    virtual void synchronize() = 0;
+    virtual void diagnostics(double *out) = 0;
+    virtual void profile_step(double dt, float *ms) = 0;
+    virtual void pf_begin(uint64_t seed, size_t max_groups) = 0;
+    virtual void pf_assimilate(double dt, size_t steps, size_t nobs, const int *x, const int *y,
+                               const double *value, const double *sd, bool is_last,
+                               double *record, double *mean_field) = 0;
+    virtual void pf_result(double *log_likelihood, size_t *resampling_events) = 0;
+    virtual uint64_t step_index() const = 0;
+    virtual void device_state(void **ptr, size_t *stride) = 0;
+    virtual void *stream() = 0;
+    // device views for other engine modules (surrogate forecast, EnKF)
+    virtual EngineViews views() = 0;
+    virtual void invalidate_mixed() = 0;
+    virtual void advance_counter(uint64_t steps) = 0;
+};
+
+} // namespace hzl_amd
+
+#include "hzl_amd.h"
+#include <stdexcept>
+#include <string>
+namespace hzl_amd {
+// Engine behind a C-ABI handle (throws std::invalid_argument on null).
+EngineBase &engine_of(hzl_ensemble_t h);
+
+// Message of the last failed C-ABI call on the calling thread.
+std::string &error_slot();
+
+// Run f, mapping exceptions to C-ABI codes: std::invalid_argument ->
+// HZL_EINVAL, anything else -> HZL_ERUNTIME (message kept in error_slot()).
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        error_slot().clear();
+        return HZL_OK;
+    } catch (const std::invalid_argument &e) {
+        error_slot() = e.what();
+        return HZL_EINVAL;
+    } catch (const std::exception &e) {
+        error_slot() = e.what();
+        return HZL_ERUNTIME;
+    } catch (...) {
+        error_slot() = "unknown error";
+        return HZL_ERUNTIME;
+    }
+}
+} // namespace hzl_amd

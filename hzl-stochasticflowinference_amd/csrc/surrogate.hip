@@ -1,0 +1,1299 @@
+// surrogate.hip -- learned forecast surrogate: a spectral neural operator
+// (Li et al., "Fourier Neural Operator for Parametric PDEs", ICLR 2021) that
+// maps an ensemble member w(t) to w(t + h) for a fixed horizon h of many
+// solver steps, trained and evaluated on gfx950 by this library's own kernels.
+// The reference has no surrogate; its role here is to replace the particle
+// forecast of the filter loops (src/filter.cpp:46-48, src/inference.cpp:555-590).
+//
+// Network on an internal G x G grid (G = 64 for a 128^2 state):
+//   features [s][y][x][3] = (w * in_scale, log(nu / 1e-3), sin(kf y))
+//   lift   v0 = feat Lw^T + Lb                                    (3 -> C)
+//   layer  v' = act(IDFT(R . DFT(v)) + v W^T + b)   act = GELU, none on the last
+//          DFT keeps kx in [0, KX), ky in [-KY/2, KY/2); R[mode] is C x C complex
+//   head   out = GELU(v Q1^T + q1) Q2 + q2                       (C -> H -> 1)
+// `out` is the grid increment w(t + h) - w(t) scaled by out_scale.  State and
+// grid are related by exact truncated DFTs: the state's modes |kx|, |ky| <= KS
+// (KS = G/2 - 1) are sampled on the grid; the increment's grid DFT on the same
+// band is added back; modes outside it decay linearly (exp(-(nu k^2+alpha) h)).
+//
+// Every product is a batched C = A B^T on MFMA (gemm_mfma.hpp): DFTs as dense
+// cos/sin matrices, per-mode complex mixing in real 2C x 2C block form,
+// pointwise channel mixing, and all products of the backward pass (transposes
+// are stride swaps).  Training runs on FP32 operands with FP32 master weights
+// (Adam); inference runs on BF16 or FP32 operands, FP32 accumulation.
+#include "engine.hpp"
+#include "surrogate_kernels.hpp"
+#include "tables.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hzl_amd {
+
+// ----------------------------------------------------------- kernels -------
+// Four waves per block, each an independent 32 x 32 output tile.
+template <typename T> __global__ void __launch_bounds__(256) k_gemm(GemmArgs<T> a) {
+    extern __shared__ unsigned char smem[];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    T *lds = reinterpret_cast<T *>(smem) + wave * GemmTile<T>::LDS_PER_WAVE;
+    gemm_wave<T>(DevCtx{}, a, (long)blockIdx.x * 4 + wave, lane, lds);
+}
+
+#define HZL_GID ((long)blockIdx.x * 256 + threadIdx.x)
+
+template <typename T>
+__global__ void k_feature(T *f, const double *nu, int S, int G, int CIN, int kf) {
+    feature_body<T>(f, nu, S, G, CIN, kf, HZL_GID);
+}
+template <typename T> __global__ void k_gelu_back(const T *go, const T *pre, T *g, long n) {
+    gelu_back_body<T>(go, pre, g, n, HZL_GID);
+}
+template <typename T>
+__global__ void k_head(const T *h, const float *q2, const float *b2, float *out, long M, int H) {
+    head_body<T>(h, q2, b2[0], out, M, H, HZL_GID);
+}
+template <typename T>
+__global__ void k_head_back(const float *dout, const float *q2, const T *hpre, T *dh, long M,
+                            int H) {
+    head_back_body<T>(dout, q2, hpre, dh, M, H, HZL_GID);
+}
+__global__ void k_mse(const float *p, const float *t, float *g, double *part, long n, double sc) {
+    mse_body(p, t, g, part, n, sc, HZL_GID);
+}
+template <typename T>
+__global__ void k_colsum(const T *x, long M, int N, long ld, int chunks, float *out) {
+    colsum_body<T>(x, M, N, ld, chunks, out, HZL_GID);
+}
+template <typename T>
+__global__ void k_wcolsum(const T *x, const float *w, long M, int N, long ld, int chunks,
+                          float *out) {
+    wcolsum_body<T>(x, w, M, N, ld, chunks, out, HZL_GID);
+}
+__global__ void k_sum_chunks(const float *part, int chunks, long count, float *out, int add) {
+    sum_chunks_body(part, chunks, count, out, add, HZL_GID);
+}
+template <typename T>
+__global__ void k_rcat(const float *Rr, const float *Ri, T *Rcat, int modes, int C) {
+    rcat_body<T>(Rr, Ri, Rcat, modes, C, HZL_GID);
+}
+__global__ void k_rcat_back(const float *g, float *dRr, float *dRi, int modes, int C) {
+    rcat_back_body(g, dRr, dRi, modes, C, HZL_GID);
+}
+__global__ void k_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+    adam_body(w, g, m, v, n, lr, b1, b2, eps, c1, c2, HZL_GID);
+}
+template <typename T> __global__ void k_cast(const float *src, T *dst, long n) {
+    cast_body<T>(src, dst, n, HZL_GID);
+}
+__global__ void k_gather_states(const cx<float> *src, const int *idx, long spec, cx<float> *dst,
+                                int S) {
+    gather_states_body(src, idx, spec, dst, S, HZL_GID);
+}
+__global__ void k_diff_states(const cx<float> *a, const cx<float> *b, cx<float> *d, long n) {
+    diff_states_body(a, b, d, n, HZL_GID);
+}
+__global__ void k_gather_nu(const double *nu, const int *idx, double *out, int S) {
+    const long i = HZL_GID;
+    if (i < S)
+        out[i] = nu[idx[i]];
+}
+__global__ void k_dsum(const double *x, long n, int chunks, double *out) {
+    dsum_body(x, n, chunks, out, HZL_GID);
+}
+__global__ void k_fill_f32(float *p, long n, float v) {
+    const long i = HZL_GID;
+    if (i < n)
+        p[i] = v;
+}
+__global__ void k_apply_increment(cx<float> *w0, const float *incr, int S, int K, int KS,
+                                  float scale, const double *nu, const double *alpha,
+                                  const double *sigma, const uint64_t *keys, const int *pair_of,
+                                  int kmin, int kmax, double horizon, uint64_t counter) {
+    apply_increment_body(w0, incr, S, K, KS, scale, nu, alpha, sigma, keys, pair_of, kmin, kmax,
+                         horizon, counter, HZL_GID);
+}
+
+namespace {
+
+void hcheck(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+dim3 grid1d(long count) {
+    return dim3((unsigned)((count + 255) / 256));
+}
+
+// Device buffer that only grows.  Growing waits for the device first: work
+// still queued on any stream may read the old allocation.
+template <typename T> struct Buf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (count <= n)
+            return;
+        if (p) {
+            hcheck(hipDeviceSynchronize(), "buffer growth");
+            (void)hipFree(p);
+        }
+        p = nullptr;
+        n = 0;
+        hcheck(hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    Buf() = default;
+    Buf(const Buf &) = delete;
+    Buf &operator=(const Buf &) = delete;
+    ~Buf() {
+        if (p)
+            (void)hipFree(p);
+    }
+};
+
+template <typename T> constexpr int type_of() {
+    return sizeof(T) == 2 ? OUT_BF16 : OUT_F32;
+}
+
+// Operand with its addressable extent (elements), for the host-side bounds
+// check of every product before launch.
+template <typename P> struct Span {
+    P *p;
+    size_t n;
+};
+template <typename T> Span<const T> cs(Buf<T> &b, size_t off = 0) {
+    if (off > b.n)
+        throw std::logic_error("operand offset beyond buffer");
+    return {b.p + off, b.n - off};
+}
+template <typename T> Span<void> vs(Buf<T> &b, size_t off = 0) {
+    if (off > b.n)
+        throw std::logic_error("operand offset beyond buffer");
+    return {b.p + off, b.n - off};
+}
+
+template <typename T> struct Gemm {
+    GemmArgs<T> a{};
+    size_t na = 0, nb = 0, nc = 0, nx = 0;
+    Gemm(int M, int N, int K, Span<const T> A, long am, long ak, Span<const T> B, long bn,
+         long bk, Span<void> C, long cm, long cn, int ctype) {
+        a.M = M;
+        a.N = N;
+        a.K = K;
+        a.nb1 = a.nb2 = 1;
+        a.A = A.p;
+        a.a_m = am;
+        a.a_k = ak;
+        a.B = B.p;
+        a.b_n = bn;
+        a.b_k = bk;
+        a.C = C.p;
+        a.c_m = cm;
+        a.c_n = cn;
+        a.c_type = ctype;
+        a.alpha = 1.f;
+        a.beta = 0.f;
+        a.aux_type = type_of<T>();
+        a.act = ACT_NONE;
+        na = A.n;
+        nb = B.n;
+        nc = C.n;
+    }
+    Gemm &batch(int nb1, long ab1, long bb1, long cb1, int nb2 = 1, long ab2 = 0, long bb2 = 0,
+                long cb2 = 0) {
+        a.nb1 = nb1;
+        a.a_b1 = ab1;
+        a.b_b1 = bb1;
+        a.c_b1 = cb1;
+        a.nb2 = nb2;
+        a.a_b2 = ab2;
+        a.b_b2 = bb2;
+        a.c_b2 = cb2;
+        return *this;
+    }
+    Gemm &aux(Span<const T> X, long xm, long xn, long xb1, long xb2) {
+        a.aux = X.p;
+        nx = X.n;
+        a.x_m = xm;
+        a.x_n = xn;
+        a.x_b1 = xb1;
+        a.x_b2 = xb2;
+        return *this;
+    }
+    // Largest element index an operand touches (all strides are >= 0).
+    static size_t extent(long b1, long s1, long b2, long s2, long r, long sr, long k, long sk) {
+        return (size_t)((b1 - 1) * s1 + (b2 - 1) * s2 + (r - 1) * sr + (k - 1) * sk);
+    }
+    void check() const {
+        const long strides[] = {a.a_m, a.a_k, a.a_b1, a.a_b2, a.b_n, a.b_k, a.b_b1, a.b_b2,
+                                a.c_m, a.c_n, a.c_b1, a.c_b2, a.x_m, a.x_n, a.x_b1, a.x_b2};
+        for (long s : strides)
+            if (s < 0)
+                throw std::logic_error("negative GEMM stride");
+        if (a.M <= 0 || a.N <= 0 || a.K <= 0 || a.nb1 <= 0 || a.nb2 <= 0)
+            throw std::logic_error("empty GEMM");
+        if (extent(a.nb1, a.a_b1, a.nb2, a.a_b2, a.M, a.a_m, a.K, a.a_k) >= na ||
+            extent(a.nb1, a.b_b1, a.nb2, a.b_b2, a.N, a.b_n, a.K, a.b_k) >= nb ||
+            extent(a.nb1, a.c_b1, a.nb2, a.c_b2, a.M, a.c_m, a.N, a.c_n) >= nc ||
+            (a.aux && extent(a.nb1, a.x_b1, a.nb2, a.x_b2, a.M, a.x_m, a.N, a.x_n) >= nx))
+            throw std::logic_error("GEMM operand out of bounds");
+    }
+    void run(hipStream_t s) const {
+        check();
+        const long tiles = (long)a.nb1 * a.nb2 * ((a.M + 31) / 32) * ((a.N + 31) / 32);
+        const size_t lds = 4 * GemmTile<T>::LDS_PER_WAVE * sizeof(T);
+        hipLaunchKernelGGL(k_gemm<T>, dim3((unsigned)((tiles + 3) / 4)), dim3(256), lds, s, a);
+    }
+};
+
+} // namespace
+
+// ------------------------------------------------------------- model ------
+struct FnoShape {
+    int G = 64, C = 32, L = 4, KX = 16, KY = 32, H = 128, CIN = 3, kf = 4;
+    int N = 128, K = 42; // engine grid and its retained band
+    int KS = 31;         // band of the state sampled on the grid: |kx|, |ky| <= KS
+    static constexpr int KXS = 32; // padded kx extent of the state <-> grid products
+    int modes() const {
+        return KX * KY;
+    }
+    long P() const {
+        return (long)G * G;
+    }
+    long SA() const { // x-transformed activations per sample: [2KX][G][C]
+        return 2L * KX * G * C;
+    }
+    long SV() const { // spectral activations per sample: [KX][KY][2][C]
+        return (long)modes() * 2 * C;
+    }
+    long spec() const {
+        return (long)(K + 1) * (2 * K + 1);
+    }
+};
+
+// Offsets of the parameter groups inside the one FP32 parameter vector.
+struct FnoParams {
+    long Lw = 0, Lb = 0, Q1 = 0, q1 = 0, Q2 = 0, q2 = 0, total = 0;
+    std::vector<long> W, b, Rr, Ri;
+    explicit FnoParams(const FnoShape &s) {
+        long o = 0;
+        auto take = [&](long n) {
+            const long at = o;
+            o += n;
+            return at;
+        };
+        Lw = take((long)s.C * s.CIN); // [C][CIN]
+        Lb = take(s.C);
+        for (int l = 0; l < s.L; ++l) {
+            W.push_back(take((long)s.C * s.C)); // [C out][C in]
+            b.push_back(take(s.C));
+            Rr.push_back(take((long)s.modes() * s.C * s.C)); // [kx][kyi][C out][C in]
+            Ri.push_back(take((long)s.modes() * s.C * s.C));
+        }
+        Q1 = take((long)s.H * s.C); // [H][C]
+        q1 = take(s.H);
+        Q2 = take(s.H); // [H]
+        q2 = take(1);
+        total = o;
+    }
+};
+
+// Constant matrices of the truncated transforms, in operand type T.
+//   Fx  [2kx+ri][x]         forward x:  (cos, -sin)(2 pi kx x / G)
+//   Fy2 [2kyi+ro][ri*G+y]   forward y:  [[c, s], [-s, c]]
+//   Gy2 [ro*G+y][2kyi+ri]   inverse y:  [[c, -s], [s, c]]
+//   Gx  [x][2kx+ri]         inverse x:  w / G^2 (cos, -sin), w = 1 (kx = 0) else 2
+// and their transposes for the backward products.
+template <typename T> struct FnoConstants {
+    Buf<T> Fx, FxT, Fy2, Fy2T, Gy2, Gy2T, Gx, GxT;
+    void build(const FnoShape &s) {
+        const int G = s.G, KX = s.KX, KY = s.KY;
+        std::vector<double> fx(2 * KX * G), fy(2 * KY * 2 * G), gy(2 * G * 2 * KY), gx(G * 2 * KX);
+        for (int kx = 0; kx < KX; ++kx)
+            for (int x = 0; x < G; ++x) {
+                const double ph = 2 * PI * kx * x / G, w = (kx == 0 ? 1.0 : 2.0) / ((double)G * G);
+                fx[(2 * kx) * G + x] = std::cos(ph);
+                fx[(2 * kx + 1) * G + x] = -std::sin(ph);
+                gx[x * 2 * KX + 2 * kx] = w * std::cos(ph);
+                gx[x * 2 * KX + 2 * kx + 1] = -w * std::sin(ph);
+            }
+        for (int kyi = 0; kyi < KY; ++kyi) {
+            const int ky = kyi < KY / 2 ? kyi : kyi - KY;
+            for (int y = 0; y < G; ++y) {
+                const double th = 2 * PI * ky * y / G, c = std::cos(th), sn = std::sin(th);
+                fy[(2 * kyi) * 2 * G + y] = c;
+                fy[(2 * kyi) * 2 * G + G + y] = sn;
+                fy[(2 * kyi + 1) * 2 * G + y] = -sn;
+                fy[(2 * kyi + 1) * 2 * G + G + y] = c;
+                gy[y * 2 * KY + 2 * kyi] = c;
+                gy[y * 2 * KY + 2 * kyi + 1] = -sn;
+                gy[(G + y) * 2 * KY + 2 * kyi] = sn;
+                gy[(G + y) * 2 * KY + 2 * kyi + 1] = c;
+            }
+        }
+        auto up = [&](Buf<T> &b, const std::vector<double> &v, int rows, int cols, bool tr) {
+            std::vector<T> h(v.size());
+            for (int r = 0; r < rows; ++r)
+                for (int c = 0; c < cols; ++c)
+                    h[tr ? (size_t)c * rows + r : (size_t)r * cols + c] =
+                        (T)(float)v[(size_t)r * cols + c];
+            b.alloc(h.size());
+            hcheck(hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice),
+                   "transform constants");
+        };
+        up(Fx, fx, 2 * KX, G, false);
+        up(FxT, fx, 2 * KX, G, true);
+        up(Fy2, fy, 2 * KY, 2 * G, false);
+        up(Fy2T, fy, 2 * KY, 2 * G, true);
+        up(Gy2, gy, 2 * G, 2 * KY, false);
+        up(Gy2T, gy, 2 * G, 2 * KY, true);
+        up(Gx, gx, G, 2 * KX, false);
+        up(GxT, gx, G, 2 * KX, true);
+    }
+};
+
+// Activations of one batch of S samples in operand type T, plus the weights
+// cast to T.  Layouts: pixel tensors [s][y][x][channel]; A/Bi [s][2kx+ri][y][c];
+// V/U [s][kx][kyi][ri][c].
+template <typename T> struct FnoWork {
+    int S = 0;
+    bool built = false;
+    FnoConstants<T> cst;
+    Buf<T> feat, h, hpre, dh, dv0, dv1, g, dBi, dU, dV, dA, w_T, rcat;
+    std::vector<std::unique_ptr<Buf<T>>> v, pre, A, V, U, Bi, Z;
+    Buf<float> out, dout;
+    Buf<double> loss_part;
+    uint64_t version = ~0ULL; // parameter version w_T / rcat were cast from
+
+    void ensure(const FnoShape &s, int S_, long nparams) {
+        if (!built) {
+            cst.build(s);
+            for (int l = 0; l <= s.L; ++l)
+                v.emplace_back(new Buf<T>);
+            for (int l = 0; l < s.L; ++l) {
+                pre.emplace_back(new Buf<T>);
+                A.emplace_back(new Buf<T>);
+                V.emplace_back(new Buf<T>);
+                U.emplace_back(new Buf<T>);
+                Bi.emplace_back(new Buf<T>);
+                Z.emplace_back(new Buf<T>);
+            }
+            w_T.alloc(nparams);
+            rcat.alloc((size_t)s.L * s.modes() * 4 * s.C * s.C);
+            built = true;
+        }
+        if (S_ <= S)
+            return;
+        S = S_;
+        const size_t SP = (size_t)S * s.P();
+        feat.alloc(SP * s.CIN);
+        for (auto &b : v)
+            b->alloc(SP * s.C);
+        for (int l = 0; l < s.L; ++l) {
+            pre[l]->alloc(SP * s.C);
+            A[l]->alloc(S * s.SA());
+            V[l]->alloc(S * s.SV());
+            U[l]->alloc(S * s.SV());
+            Bi[l]->alloc(S * s.SA());
+            Z[l]->alloc(SP * s.C);
+        }
+        h.alloc(SP * s.H);
+        hpre.alloc(SP * s.H);
+        dh.alloc(SP * s.H);
+        dv0.alloc(SP * s.C);
+        dv1.alloc(SP * s.C);
+        g.alloc(SP * s.C);
+        dBi.alloc(S * s.SA());
+        dA.alloc(S * s.SA());
+        dU.alloc(S * s.SV());
+        dV.alloc(S * s.SV());
+        out.alloc(SP);
+        dout.alloc(SP);
+        loss_part.alloc(SP);
+    }
+};
+
+class Surrogate {
+  public:
+    Surrogate(int state_n, int grid, int width, int layers, int modes, uint64_t seed)
+        : s_(make_shape(state_n, grid, width, layers, modes)), prm_(s_) {
+        hcheck(hipStreamCreateWithFlags(&own_, hipStreamNonBlocking), "hipStreamCreate");
+        st_ = own_;
+        master_.alloc(prm_.total);
+        grad_.alloc(prm_.total);
+        m_.alloc(prm_.total);
+        v_.alloc(prm_.total);
+        init_params(seed);
+        conv_build();
+    }
+    ~Surrogate() {
+        if (own_) {
+            (void)hipStreamSynchronize(own_);
+            (void)hipStreamDestroy(own_);
+        }
+    }
+    Surrogate(const Surrogate &) = delete;
+    Surrogate &operator=(const Surrogate &) = delete;
+
+    long param_count() const {
+        return prm_.total;
+    }
+    void get_params(float *out) {
+        hcheck(hipMemcpy(out, master_.p, prm_.total * sizeof(float), hipMemcpyDeviceToHost),
+               "parameter download");
+    }
+    void set_params(const float *in) {
+        for (long i = 0; i < prm_.total; ++i)
+            if (!std::isfinite(in[i]))
+                throw std::invalid_argument("nonfinite surrogate parameter");
+        hcheck(hipMemcpy(master_.p, in, prm_.total * sizeof(float), hipMemcpyHostToDevice),
+               "parameter upload");
+        reset_adam();
+    }
+    void set_scales(double in_scale, double out_scale) {
+        if (!(in_scale > 0) || !(out_scale > 0) || !std::isfinite(in_scale) ||
+            !std::isfinite(out_scale))
+            throw std::invalid_argument("surrogate scales must be positive and finite");
+        in_scale_ = in_scale;
+        out_scale_ = out_scale;
+        scales_set_ = true;
+    }
+    void get_scales(double *s) const {
+        s[0] = in_scale_;
+        s[1] = out_scale_;
+    }
+    size_t dataset_size() const {
+        return pairs_;
+    }
+
+    // ---- host-array entry points (reference Field layout) -----------------
+    void forward_host(int precision, size_t S, const double *fields, const double *nu,
+                      double *grid) {
+        check_count(S);
+        check_precision(precision);
+        st_ = own_;
+        upload_fields(S, fields, hx_);
+        upload_nu(S, nu);
+        if (precision == HZL_BF16) {
+            forward(wb_, hx_.p, hnu_.p, (int)S);
+            download_out(wb_, S, grid);
+        } else {
+            forward(wf_, hx_.p, hnu_.p, (int)S);
+            download_out(wf_, S, grid);
+        }
+    }
+    void loss_grad_host(size_t S, const double *fields, const double *next, const double *nu,
+                        double *loss, float *grad) {
+        check_count(S);
+        st_ = own_;
+        upload_fields(S, fields, hx_);
+        upload_fields(S, next, hy_);
+        upload_nu(S, nu);
+        *loss = loss_and_grad(wf_, hx_.p, hy_.p, hnu_.p, (int)S, true);
+        hcheck(hipMemcpy(grad, grad_.p, prm_.total * sizeof(float), hipMemcpyDeviceToHost),
+               "gradient download");
+    }
+
+    // ---- training data from an FP32 ensemble ------------------------------
+    // windows x members pairs (w(t), w(t + steps dt)); the ensemble advances
+    // by windows * steps solver steps with its own forcing.  With noise_free,
+    // each target is the same start state advanced without the white shell
+    // forcing (a scratch ensemble with the members' nu, alpha and amplitude):
+    // the network then learns the deterministic propagator, the conditional
+    // mean of the forced flow map, and the forecast adds the noise itself.
+    void collect(EngineBase &eng, double dt, size_t steps, size_t windows, bool noise_free) {
+        const EngineViews ev = check_engine(eng);
+        if (steps == 0 || windows == 0 || !(dt > 0) || !std::isfinite(dt))
+            throw std::invalid_argument("collect needs dt > 0, steps > 0 and windows > 0");
+        if (horizon_ && (steps != horizon_ || dt != dt_))
+            throw std::invalid_argument("collect horizon differs from the dataset's");
+        const size_t spec = (size_t)s_.spec(), B = (size_t)ev.B;
+        std::unique_ptr<hzl_ensemble, int (*)(hzl_ensemble_t)> scratch(nullptr,
+                                                                       hzl_ensemble_destroy);
+        if (noise_free) {
+            std::vector<double> nu(B), alpha(B), amp(B);
+            eng.synchronize();
+            hcheck(hipMemcpy(nu.data(), ev.nu, B * sizeof(double), hipMemcpyDeviceToHost), "nu");
+            hcheck(hipMemcpy(alpha.data(), ev.alpha, B * sizeof(double), hipMemcpyDeviceToHost),
+                   "alpha");
+            hcheck(hipMemcpy(amp.data(), ev.amp, B * sizeof(double), hipMemcpyDeviceToHost), "amp");
+            hzl_parameters p{nu[0], alpha[0], 0.0, 1.0, ev.kmin, ev.kmax, amp[0], ev.kf};
+            hzl_ensemble_t h = nullptr;
+            if (hzl_ensemble_create(&h, (size_t)s_.N, B, HZL_FP32, &p, 0) != HZL_OK)
+                throw std::runtime_error("scratch ensemble: " + error_slot());
+            scratch.reset(h);
+            EngineBase &se = engine_of(h);
+            for (size_t m = 0; m < B; ++m)
+                se.set_member_parameters(m, nu[m], alpha[m], 0.0, amp[m]);
+        }
+        horizon_ = steps;
+        dt_ = dt;
+        for (size_t wdw = 0; wdw < windows; ++wdw) {
+            const EngineViews v = eng.views();
+            grow_dataset(pairs_ + B);
+            const size_t bytes = B * spec * sizeof(cx<float>);
+            hcheck(hipMemcpyAsync(ds_x_.p + pairs_ * spec, v.w0, bytes, hipMemcpyDeviceToDevice,
+                                  v.stream),
+                   "dataset copy");
+            hcheck(hipMemcpyAsync(ds_nu_.p + pairs_, v.nu, B * sizeof(double),
+                                  hipMemcpyDeviceToDevice, v.stream),
+                   "dataset copy");
+            if (scratch) {
+                EngineBase &se = engine_of(scratch.get());
+                const EngineViews sv = se.views();
+                eng.synchronize();
+                hcheck(hipMemcpyAsync(sv.w0, v.w0, bytes, hipMemcpyDeviceToDevice, sv.stream),
+                       "scratch state");
+                se.invalidate_mixed();
+                se.step(dt, steps);
+                hcheck(hipMemcpyAsync(ds_y_.p + pairs_ * spec, sv.w0, bytes,
+                                      hipMemcpyDeviceToDevice, sv.stream),
+                       "dataset copy");
+                se.synchronize();
+                eng.step(dt, steps);
+            } else {
+                eng.step(dt, steps);
+                hcheck(hipMemcpyAsync(ds_y_.p + pairs_ * spec, v.w0, bytes,
+                                      hipMemcpyDeviceToDevice, v.stream),
+                       "dataset copy");
+            }
+            eng.synchronize();
+            if (!scales_set_)
+                auto_scales(pairs_, B);
+            pairs_ += B;
+        }
+    }
+
+    // Minibatch Adam over pairs [first, first + count) with a cosine learning
+    // rate from lr down to lr / 20.
+    void fit(size_t first, size_t count, size_t epochs, size_t batch, double lr, uint64_t seed,
+             double *epoch_loss) {
+        if (count == 0 || first + count > pairs_)
+            throw std::invalid_argument("fit range outside the dataset");
+        if (batch == 0 || batch > count || batch > 4096 || !(lr > 0) || epochs == 0)
+            throw std::invalid_argument(
+                "fit needs epochs > 0, 0 < batch <= min(count, 4096) and lr > 0");
+        st_ = own_;
+        const size_t nb = count / batch, total = nb * epochs;
+        std::mt19937_64 rng(seed);
+        std::vector<int> order(count), idx(batch);
+        size_t t = 0;
+        for (size_t e = 0; e < epochs; ++e) {
+            for (size_t i = 0; i < count; ++i)
+                order[i] = (int)(first + i);
+            std::shuffle(order.begin(), order.end(), rng);
+            double sum = 0;
+            for (size_t b = 0; b < nb; ++b, ++t) {
+                std::copy(order.begin() + b * batch, order.begin() + (b + 1) * batch, idx.begin());
+                gather_batch(idx);
+                sum += loss_and_grad(wf_, bx_.p, by_.p, bnu_.p, (int)batch, true);
+                const double f = 0.5 * (1 + std::cos(PI * (double)t / (double)total));
+                adam((float)(lr * (0.05 + 0.95 * f)));
+            }
+            if (epoch_loss)
+                epoch_loss[e] = sum / (double)nb;
+        }
+        hcheck(hipStreamSynchronize(st_), "fit");
+    }
+
+    // Mean loss over pairs [first, first + count) (forward only).
+    double evaluate(int precision, size_t first, size_t count) {
+        if (count == 0 || first + count > pairs_)
+            throw std::invalid_argument("evaluate range outside the dataset");
+        check_precision(precision);
+        st_ = own_;
+        const size_t batch = std::min<size_t>(count, 256);
+        std::vector<int> idx;
+        double sum = 0;
+        for (size_t done = 0; done < count;) {
+            const size_t S = std::min(batch, count - done);
+            idx.resize(S);
+            for (size_t i = 0; i < S; ++i)
+                idx[i] = (int)(first + done + i);
+            gather_batch(idx);
+            const double l = precision == HZL_BF16
+                                 ? loss_and_grad(wb_, bx_.p, by_.p, bnu_.p, (int)S, false)
+                                 : loss_and_grad(wf_, bx_.p, by_.p, bnu_.p, (int)S, false);
+            sum += l * (double)S;
+            done += S;
+        }
+        return sum / (double)count;
+    }
+
+    // Advance every member of an FP32 ensemble by steps * dt with the network.
+    void forecast(EngineBase &eng, double dt, size_t steps, int precision) {
+        const EngineViews ev = check_engine(eng);
+        check_precision(precision);
+        if (steps == 0 || !(dt > 0) || !std::isfinite(dt))
+            throw std::invalid_argument("forecast needs dt > 0 and steps > 0");
+        if (horizon_ && (steps != horizon_ || std::fabs(dt - dt_) > 1e-12 * dt_))
+            throw std::invalid_argument("forecast horizon differs from the trained horizon");
+        // run on the engine's stream, in order with its own work
+        st_ = ev.stream;
+        const int chunk = std::min(ev.B, 1024);
+        for (int s0 = 0; s0 < ev.B; s0 += chunk) {
+            const int S = std::min(chunk, ev.B - s0);
+            cx<float> *w0 = reinterpret_cast<cx<float> *>(ev.w0) + (long)s0 * s_.spec();
+            if (precision == HZL_BF16) {
+                forward(wb_, w0, ev.nu + s0, S);
+                grid_to_increment(wb_.out.p, S);
+            } else {
+                forward(wf_, w0, ev.nu + s0, S);
+                grid_to_increment(wf_.out.p, S);
+            }
+            const long cnt = (long)S * s_.spec();
+            hipLaunchKernelGGL(k_apply_increment, grid1d(cnt), dim3(256), 0, st_, w0, incr_.p, S,
+                               s_.K, s_.KS, (float)(1.0 / out_scale_), ev.nu + s0, ev.alpha + s0,
+                               ev.sigma + s0, ev.keys + s0, ev.pair_of, ev.kmin, ev.kmax,
+                               (double)steps * dt, counter_);
+        }
+        hcheck(hipGetLastError(), "surrogate forecast");
+        st_ = own_;
+        ++counter_;
+        eng.invalidate_mixed();
+        eng.advance_counter(steps);
+    }
+
+  private:
+    static FnoShape make_shape(int state_n, int grid, int width, int layers, int modes) {
+        FnoShape s;
+        if (state_n != 64 && state_n != 128 && state_n != 256)
+            throw std::invalid_argument("surrogate state grid must be 64, 128 or 256");
+        if (grid != 32 && grid != 64)
+            throw std::invalid_argument("surrogate grid must be 32 or 64");
+        if (width < 8 || width > 64 || width % 8 || layers < 1 || layers > 8 || modes < 2 ||
+            2 * modes > grid / 2)
+            throw std::invalid_argument("surrogate needs width in [8, 64] (multiple of 8), "
+                                        "1..8 layers and 2 <= modes <= grid / 4");
+        s.N = state_n;
+        s.K = (state_n - 1) / 3;
+        s.G = grid;
+        s.C = width;
+        s.L = layers;
+        s.KX = modes;
+        s.KY = 2 * modes;
+        s.KS = std::min(grid / 2 - 1, s.K);
+        return s;
+    }
+
+    static void check_count(size_t S) {
+        if (S == 0 || S > 4096)
+            throw std::invalid_argument("sample count must be in [1, 4096]");
+    }
+    static void check_precision(int precision) {
+        if (precision != HZL_FP32 && precision != HZL_BF16)
+            throw std::invalid_argument("surrogate precision must be HZL_FP32 or HZL_BF16");
+    }
+
+    EngineViews check_engine(EngineBase &eng) {
+        const EngineViews ev = eng.views();
+        if (ev.precision != 0)
+            throw std::invalid_argument("the surrogate works on FP32 ensembles");
+        if (ev.n != s_.N)
+            throw std::invalid_argument("ensemble grid does not match the surrogate");
+        return ev;
+    }
+
+    void init_params(uint64_t seed) {
+        std::mt19937_64 rng(seed);
+        std::uniform_real_distribution<double> u(-1.0, 1.0);
+        std::vector<float> h(prm_.total, 0.f);
+        auto fill = [&](long off, long n, double a) {
+            for (long i = 0; i < n; ++i)
+                h[off + i] = (float)(a * u(rng));
+        };
+        fill(prm_.Lw, (long)s_.C * s_.CIN, 1.0 / std::sqrt((double)s_.CIN));
+        for (int l = 0; l < s_.L; ++l) {
+            fill(prm_.W[l], (long)s_.C * s_.C, 1.0 / std::sqrt((double)s_.C));
+            fill(prm_.Rr[l], (long)s_.modes() * s_.C * s_.C, 1.0 / s_.C);
+            fill(prm_.Ri[l], (long)s_.modes() * s_.C * s_.C, 1.0 / s_.C);
+        }
+        fill(prm_.Q1, (long)s_.H * s_.C, 1.0 / std::sqrt((double)s_.C));
+        fill(prm_.Q2, s_.H, 1.0 / std::sqrt((double)s_.H));
+        hcheck(hipMemcpy(master_.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice),
+               "parameter init");
+        reset_adam();
+    }
+
+    void reset_adam() {
+        hipLaunchKernelGGL(k_fill_f32, grid1d(prm_.total), dim3(256), 0, own_, m_.p, prm_.total,
+                           0.f);
+        hipLaunchKernelGGL(k_fill_f32, grid1d(prm_.total), dim3(256), 0, own_, v_.p, prm_.total,
+                           0.f);
+        hcheck(hipStreamSynchronize(own_), "optimizer reset");
+        t_ = 0;
+        ++version_;
+    }
+
+    // State <-> grid matrices (FP32):
+    //   g1 [2y+ro][2jj+ri]  inverse y of the band (ky = jj - KS): [[c, -s], [s, c]]
+    //   g2 [x][ri*KXS+kx]   real inverse x: w (cos, -sin), w = 1 (kx = 0) else 2
+    //   f1 [2kx+ri][x]      forward x: (cos, -sin)
+    //   f2 [2jj+ro][ri*G+y] forward y / G^2: [[c, s], [-s, c]] / G^2
+    void conv_build() {
+        const int G = s_.G, KS = s_.KS, JS = 2 * KS + 1, KXS = FnoShape::KXS;
+        std::vector<float> g1((size_t)2 * G * 2 * JS), g2((size_t)G * 2 * KXS, 0.f),
+            f1((size_t)2 * KXS * G, 0.f), f2((size_t)2 * JS * 2 * G);
+        const double z = 1.0 / ((double)G * G);
+        for (int y = 0; y < G; ++y)
+            for (int jj = 0; jj < JS; ++jj) {
+                const double th = 2 * PI * (jj - KS) * y / G, c = std::cos(th), sn = std::sin(th);
+                g1[(size_t)(2 * y) * 2 * JS + 2 * jj] = (float)c;
+                g1[(size_t)(2 * y) * 2 * JS + 2 * jj + 1] = (float)-sn;
+                g1[(size_t)(2 * y + 1) * 2 * JS + 2 * jj] = (float)sn;
+                g1[(size_t)(2 * y + 1) * 2 * JS + 2 * jj + 1] = (float)c;
+                f2[(size_t)(2 * jj) * 2 * G + y] = (float)(z * c);
+                f2[(size_t)(2 * jj) * 2 * G + G + y] = (float)(z * sn);
+                f2[(size_t)(2 * jj + 1) * 2 * G + y] = (float)(-z * sn);
+                f2[(size_t)(2 * jj + 1) * 2 * G + G + y] = (float)(z * c);
+            }
+        for (int kx = 0; kx <= KS; ++kx)
+            for (int x = 0; x < G; ++x) {
+                const double ph = 2 * PI * kx * x / G, w = kx == 0 ? 1.0 : 2.0;
+                g2[(size_t)x * 2 * KXS + kx] = (float)(w * std::cos(ph));
+                g2[(size_t)x * 2 * KXS + KXS + kx] = (float)(-w * std::sin(ph));
+                f1[(size_t)(2 * kx) * G + x] = (float)std::cos(ph);
+                f1[(size_t)(2 * kx + 1) * G + x] = (float)-std::sin(ph);
+            }
+        auto up = [&](Buf<float> &b, const std::vector<float> &v) {
+            b.alloc(v.size());
+            hcheck(hipMemcpy(b.p, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice),
+                   "conversion matrices");
+        };
+        up(g1_, g1);
+        up(g2_, g2);
+        up(f1_, f1);
+        up(f2_, f2);
+    }
+
+    // Grid values of S states (FP32 half planes): out[s*cb1 + y*cm + x*cn] =
+    // alpha * w_s(x, y) over the band |kx|, |ky| <= KS.
+    void state_to_grid(const cx<float> *states, int S, Span<void> out, long cm, long cn,
+                       long cb1, int out_type, float alpha) {
+        const int G = s_.G, KS = s_.KS, JS = 2 * KS + 1, KXS = FnoShape::KXS, J = 2 * s_.K + 1;
+        const size_t nbm = (size_t)S * 2 * G * KXS;
+        bm_.alloc(nbm);
+        hipLaunchKernelGGL(k_fill_f32, grid1d((long)nbm), dim3(256), 0, st_, bm_.p, (long)nbm,
+                           0.f);
+        const float *sf = reinterpret_cast<const float *>(states);
+        const size_t skip = 2 * (size_t)(s_.K - KS);
+        const Span<const float> st{sf + skip, (size_t)S * 2 * s_.spec() - skip};
+        // (i) inverse y of columns kx <= KS: Bm[s][2y+ro][kx]
+        Gemm<float> a(2 * G, KS + 1, 2 * JS, cs(g1_), 2 * JS, 1, st, 2L * J, 1, vs(bm_), KXS, 1,
+                      OUT_F32);
+        a.batch(S, 0, 2 * s_.spec(), 2L * G * KXS);
+        a.run(st_);
+        // (ii) real inverse x: out[y][x] = sum_k Bm[s][y][k] g2[x][k]
+        Gemm<float> b(G, G, 2 * KXS, cs(bm_), 2 * KXS, 1, cs(g2_), 2 * KXS, 1, out, cm, cn,
+                      out_type);
+        b.batch(S, 2L * G * KXS, 0, cb1);
+        b.a.alpha = alpha;
+        b.run(st_);
+    }
+
+    // Grid increment (S, G, G) -> band modes into incr_ [s][kx][ky+KS][re, im].
+    void grid_to_increment(const float *grid, int S) {
+        const int G = s_.G, KS = s_.KS, JS = 2 * KS + 1, KXS = FnoShape::KXS;
+        dgrid_.alloc((size_t)S * 2 * KXS * G);
+        const Span<const float> gs{grid, (size_t)S * s_.P()};
+        Gemm<float> a(2 * (KS + 1), G, G, cs(f1_), G, 1, gs, G, 1, vs(dgrid_), G, 1, OUT_F32);
+        a.batch(S, 0, s_.P(), 2L * KXS * G);
+        a.run(st_);
+        incr_.alloc((size_t)S * (KS + 1) * 2 * JS);
+        Gemm<float> b(KS + 1, 2 * JS, 2 * G, cs(dgrid_), 2 * G, 1, cs(f2_), 2 * G, 1, vs(incr_),
+                      2 * JS, 1, OUT_F32);
+        b.batch(S, 2L * KXS * G, 0, (long)(KS + 1) * 2 * JS);
+        b.run(st_);
+    }
+
+    template <typename T> void prepare_weights(FnoWork<T> &w) {
+        if (w.version == version_)
+            return;
+        hipLaunchKernelGGL(k_cast<T>, grid1d(prm_.total), dim3(256), 0, st_, master_.p, w.w_T.p,
+                           prm_.total);
+        const long per = 4L * s_.C * s_.C * s_.modes();
+        for (int l = 0; l < s_.L; ++l)
+            hipLaunchKernelGGL(k_rcat<T>, grid1d(per), dim3(256), 0, st_, master_.p + prm_.Rr[l],
+                               master_.p + prm_.Ri[l], w.rcat.p + l * per, s_.modes(), s_.C);
+        w.version = version_;
+    }
+
+    // Network output for S FP32 half-plane states into w.out (S x G x G).
+    template <typename T>
+    void forward(FnoWork<T> &w, const cx<float> *states, const double *nu, int S) {
+        w.ensure(s_, S, prm_.total);
+        prepare_weights(w);
+        const FnoShape &s = s_;
+        const int G = s.G, C = s.C, KX = s.KX, KY = s.KY, TY = type_of<T>();
+        const long P = s.P(), SP = (long)S * P, SA = s.SA(), SV = s.SV();
+        const long rper = 4L * C * C * s.modes();
+        state_to_grid(states, S, vs(w.feat), (long)G * s.CIN, s.CIN, P * s.CIN, TY,
+                      (float)in_scale_);
+        hipLaunchKernelGGL(k_feature<T>, grid1d(SP), dim3(256), 0, st_, w.feat.p, nu, S, G, s.CIN,
+                           s.kf);
+        {
+            Gemm<T> a((int)SP, C, s.CIN, cs(w.feat), s.CIN, 1, cs(w.w_T, prm_.Lw), s.CIN, 1,
+                      vs(*w.v[0]), C, 1, TY);
+            a.a.bias = master_.p + prm_.Lb;
+            a.run(st_);
+        }
+        for (int l = 0; l < s.L; ++l) {
+            Buf<T> &vl = *w.v[l];
+            // x-DFT: A[s][kx'][y][c] = sum_x Fx[kx'][x] v[s][y][x][c]
+            Gemm<T> x(2 * KX, C, G, cs(w.cst.Fx), G, 1, cs(vl), 1, C, vs(*w.A[l]), (long)G * C, 1,
+                      TY);
+            x.batch(S, 0, P * C, SA, G, 0, (long)G * C, C);
+            x.run(st_);
+            // y-DFT: V[s][kx][k][c] = sum_m Fy2[k][m] A[s][kx][m][c]
+            Gemm<T> y(2 * KY, C, 2 * G, cs(w.cst.Fy2), 2 * G, 1, cs(*w.A[l]), 1, C, vs(*w.V[l]), C,
+                      1, TY);
+            y.batch(S, 0, SA, SV, KX, 0, 2L * G * C, (long)KY * 2 * C);
+            y.run(st_);
+            // mixing per mode: U[s][mode] = Rcat[mode] V[s][mode]
+            Gemm<T> r(S, 2 * C, 2 * C, cs(*w.V[l]), SV, 1, cs(w.rcat, l * rper), 2 * C, 1,
+                      vs(*w.U[l]), SV, 1, TY);
+            r.batch(s.modes(), 2 * C, 4L * C * C, 2 * C);
+            r.run(st_);
+            // inverse y: Bi[s][kx][m][c] = sum_k Gy2[m][k] U[s][kx][k][c]
+            Gemm<T> iy(2 * G, C, 2 * KY, cs(w.cst.Gy2), 2 * KY, 1, cs(*w.U[l]), 1, C, vs(*w.Bi[l]),
+                       C, 1, TY);
+            iy.batch(S, 0, SV, SA, KX, 0, (long)KY * 2 * C, 2L * G * C);
+            iy.run(st_);
+            // pointwise: Z = v W^T + b
+            Gemm<T> pw((int)SP, C, C, cs(vl), C, 1, cs(w.w_T, prm_.W[l]), C, 1, vs(*w.Z[l]), C, 1,
+                       TY);
+            pw.a.bias = master_.p + prm_.b[l];
+            pw.run(st_);
+            // inverse x + Z, activation: v'[s][y][x][c]
+            Gemm<T> ix(G, C, 2 * KX, cs(w.cst.Gx), 2 * KX, 1, cs(*w.Bi[l]), 1, (long)G * C,
+                       vs(*w.v[l + 1]), C, 1, TY);
+            ix.batch(S, 0, SA, P * C, G, 0, C, (long)G * C);
+            ix.aux(cs(*w.Z[l]), C, 1, P * C, (long)G * C);
+            if (l + 1 < s.L) {
+                ix.a.pre = w.pre[l]->p;
+                ix.a.act = ACT_GELU;
+            }
+            ix.run(st_);
+        }
+        Gemm<T> q((int)SP, s.H, C, cs(*w.v[s.L]), C, 1, cs(w.w_T, prm_.Q1), C, 1, vs(w.h), s.H, 1,
+                  TY);
+        q.a.bias = master_.p + prm_.q1;
+        q.a.pre = w.hpre.p;
+        q.a.act = ACT_GELU;
+        q.run(st_);
+        hipLaunchKernelGGL(k_head<T>, grid1d(SP), dim3(256), 0, st_, w.h.p, master_.p + prm_.Q2,
+                           master_.p + prm_.q2, w.out.p, SP, s.H);
+    }
+
+    // Mean squared error against the scaled true grid increment; with
+    // want_grad, every parameter gradient (FP32) into grad_.
+    template <typename T>
+    double loss_and_grad(FnoWork<T> &w, const cx<float> *x, const cx<float> *y, const double *nu,
+                         int S, bool want_grad) {
+        forward(w, x, nu, S);
+        const long n = (long)S * s_.spec(), SP = (long)S * s_.P();
+        diff_.alloc((size_t)n);
+        hipLaunchKernelGGL(k_diff_states, grid1d(n), dim3(256), 0, st_, y, x, diff_.p, n);
+        target_.alloc((size_t)SP);
+        state_to_grid(diff_.p, S, vs(target_), s_.G, 1, s_.P(), OUT_F32, (float)out_scale_);
+        hipLaunchKernelGGL(k_mse, grid1d(SP), dim3(256), 0, st_, w.out.p, target_.p, w.dout.p,
+                           w.loss_part.p, SP, 1.0 / (double)SP);
+        if (want_grad)
+            backward(w, S);
+        const int chunks = 256;
+        lsum_.alloc(chunks);
+        hipLaunchKernelGGL(k_dsum, grid1d(chunks), dim3(256), 0, st_, w.loss_part.p, SP, chunks,
+                           lsum_.p);
+        double h[chunks];
+        hcheck(hipMemcpyAsync(h, lsum_.p, sizeof(h), hipMemcpyDeviceToHost, st_), "loss download");
+        hcheck(hipStreamSynchronize(st_), "loss");
+        double sum = 0;
+        for (double v : h)
+            sum += v;
+        return sum / (double)SP;
+    }
+
+    // grad[off + m*N + n] = sum over `rows` rows r of X1[r*ld1 + m] X2[r*ld2 + n]
+    // (split over row chunks of 1024, then summed).
+    template <typename T>
+    void weight_grad(Span<const T> X1, int M, long ld1, Span<const T> X2, int N, long ld2,
+                     long rows, long off) {
+        const long kc = 1024;
+        if (rows % kc != 0)
+            throw std::logic_error("weight gradient rows must be a multiple of 1024");
+        const int chunks = (int)(rows / kc);
+        part_.alloc((size_t)chunks * M * N);
+        Gemm<T> g(M, N, (int)kc, X1, 1, ld1, X2, 1, ld2, vs(part_), N, 1, OUT_F32);
+        g.batch(chunks, kc * ld1, kc * ld2, (long)M * N);
+        g.run(st_);
+        hipLaunchKernelGGL(k_sum_chunks, grid1d((long)M * N), dim3(256), 0, st_, part_.p, chunks,
+                           (long)M * N, grad_.p + off, 0);
+    }
+
+    template <typename T> void bias_grad(const T *X, long rows, int N, long ld, long off) {
+        const int chunks = 256;
+        part_.alloc((size_t)chunks * N);
+        hipLaunchKernelGGL(k_colsum<T>, grid1d((long)chunks * N), dim3(256), 0, st_, X, rows, N,
+                           ld, chunks, part_.p);
+        hipLaunchKernelGGL(k_sum_chunks, grid1d(N), dim3(256), 0, st_, part_.p, chunks, (long)N,
+                           grad_.p + off, 0);
+    }
+
+    template <typename T> void backward(FnoWork<T> &w, int S) {
+        const FnoShape &s = s_;
+        const int G = s.G, C = s.C, KX = s.KX, KY = s.KY, H = s.H, TY = type_of<T>();
+        const long P = s.P(), SP = (long)S * P, SA = s.SA(), SV = s.SV();
+        const long rper = 4L * C * C * s.modes();
+        hipLaunchKernelGGL(k_fill_f32, grid1d(prm_.total), dim3(256), 0, st_, grad_.p, prm_.total,
+                           0.f);
+        // head: dq2 = sum dout; dQ2[j] = sum dout h[.][j]; dhpre = dout Q2 gelu'(hpre)
+        {
+            const int chunks = 256;
+            part_.alloc((size_t)chunks * H);
+            hipLaunchKernelGGL(k_colsum<float>, grid1d(chunks), dim3(256), 0, st_, w.dout.p, SP, 1,
+                               1L, chunks, part_.p);
+            hipLaunchKernelGGL(k_sum_chunks, grid1d(1), dim3(256), 0, st_, part_.p, chunks, 1L,
+                               grad_.p + prm_.q2, 0);
+            hipLaunchKernelGGL(k_wcolsum<T>, grid1d((long)chunks * H), dim3(256), 0, st_, w.h.p,
+                               w.dout.p, SP, H, (long)H, chunks, part_.p);
+            hipLaunchKernelGGL(k_sum_chunks, grid1d(H), dim3(256), 0, st_, part_.p, chunks,
+                               (long)H, grad_.p + prm_.Q2, 0);
+            hipLaunchKernelGGL(k_head_back<T>, grid1d(SP * H), dim3(256), 0, st_, w.dout.p,
+                               master_.p + prm_.Q2, w.hpre.p, w.dh.p, SP, H);
+        }
+        // projection: dv = dh Q1; dQ1 = dh^T v_L; dq1 = colsum dh
+        Buf<T> *dv = &w.dv0, *dvn = &w.dv1;
+        {
+            Gemm<T> a((int)SP, C, H, cs(w.dh), H, 1, cs(w.w_T, prm_.Q1), 1, C, vs(*dv), C, 1, TY);
+            a.run(st_);
+            weight_grad<T>(cs(w.dh), H, H, cs(*w.v[s.L]), C, C, SP, prm_.Q1);
+            bias_grad<T>(w.dh.p, SP, H, H, prm_.q1);
+        }
+        for (int l = s.L - 1; l >= 0; --l) {
+            Buf<T> *dpre = dv;
+            if (l + 1 < s.L) {
+                hipLaunchKernelGGL(k_gelu_back<T>, grid1d(SP * C), dim3(256), 0, st_, dv->p,
+                                   w.pre[l]->p, w.g.p, SP * C);
+                dpre = &w.g;
+            }
+            // inverse x: dBi[s][kx'][y][c] = sum_x Gx[x][kx'] dpre[s][y][x][c]
+            Gemm<T> a(2 * KX, C, G, cs(w.cst.GxT), G, 1, cs(*dpre), 1, C, vs(w.dBi), (long)G * C,
+                      1, TY);
+            a.batch(S, 0, P * C, SA, G, 0, (long)G * C, C);
+            a.run(st_);
+            // pointwise: dv_l = dpre W; dW = dpre^T v_l; db = colsum dpre
+            Gemm<T> b((int)SP, C, C, cs(*dpre), C, 1, cs(w.w_T, prm_.W[l]), 1, C, vs(*dvn), C, 1,
+                      TY);
+            b.run(st_);
+            weight_grad<T>(cs(*dpre), C, C, cs(*w.v[l]), C, C, SP, prm_.W[l]);
+            bias_grad<T>(dpre->p, SP, C, C, prm_.b[l]);
+            // inverse y: dU[s][kx][k][c] = sum_m Gy2[m][k] dBi[s][kx][m][c]
+            Gemm<T> c(2 * KY, C, 2 * G, cs(w.cst.Gy2T), 2 * G, 1, cs(w.dBi), 1, C, vs(w.dU), C, 1,
+                      TY);
+            c.batch(S, 0, SA, SV, KX, 0, 2L * G * C, (long)KY * 2 * C);
+            c.run(st_);
+            // mixing: dV = dU Rcat; dRcat[mode] = sum_s dU[s]^T V[s]
+            Gemm<T> d(S, 2 * C, 2 * C, cs(w.dU), SV, 1, cs(w.rcat, l * rper), 1, 2 * C, vs(w.dV),
+                      SV, 1, TY);
+            d.batch(s.modes(), 2 * C, 4L * C * C, 2 * C);
+            d.run(st_);
+            rgrad_.alloc((size_t)rper);
+            Gemm<T> e(2 * C, 2 * C, S, cs(w.dU), 1, SV, cs(*w.V[l]), 1, SV, vs(rgrad_), 2 * C, 1,
+                      OUT_F32);
+            e.batch(s.modes(), 2 * C, 2 * C, 4L * C * C);
+            e.run(st_);
+            hipLaunchKernelGGL(k_rcat_back, grid1d((long)s.modes() * C * C), dim3(256), 0, st_,
+                               rgrad_.p, grad_.p + prm_.Rr[l], grad_.p + prm_.Ri[l], s.modes(), C);
+            // forward y: dA[s][kx][m][c] = sum_k Fy2[k][m] dV[s][kx][k][c]
+            Gemm<T> f(2 * G, C, 2 * KY, cs(w.cst.Fy2T), 2 * KY, 1, cs(w.dV), 1, C, vs(w.dA), C, 1,
+                      TY);
+            f.batch(S, 0, SV, SA, KX, 0, (long)KY * 2 * C, 2L * G * C);
+            f.run(st_);
+            // forward x, accumulated onto the pointwise part
+            Gemm<T> x(G, C, 2 * KX, cs(w.cst.FxT), 2 * KX, 1, cs(w.dA), 1, (long)G * C, vs(*dvn),
+                      C, 1, TY);
+            x.batch(S, 0, SA, P * C, G, 0, C, (long)G * C);
+            x.a.beta = 1.f;
+            x.run(st_);
+            std::swap(dv, dvn);
+        }
+        // lift: dLw = dv^T feat; dLb = colsum dv
+        weight_grad<T>(cs(*dv), C, C, cs(w.feat), s.CIN, s.CIN, SP, prm_.Lw);
+        bias_grad<T>(dv->p, SP, C, C, prm_.Lb);
+        hcheck(hipGetLastError(), "surrogate backward");
+    }
+
+    void adam(float lr) {
+        ++t_;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1 - std::pow(b1, (float)t_), c2 = 1 - std::pow(b2, (float)t_);
+        hipLaunchKernelGGL(k_adam, grid1d(prm_.total), dim3(256), 0, st_, master_.p, grad_.p, m_.p,
+                           v_.p, prm_.total, lr, b1, b2, 1e-8f, c1, c2);
+        ++version_;
+    }
+
+    // ---- data movement ----------------------------------------------------
+    void upload_fields(size_t S, const double *fields, Buf<cx<float>> &dst) {
+        const size_t nn = (size_t)s_.N * s_.N, spec = (size_t)s_.spec();
+        for (size_t i = 0; i < 2 * nn * S; ++i)
+            if (!std::isfinite(fields[i]))
+                throw std::invalid_argument("nonfinite surrogate input");
+        std::vector<cx<float>> half(spec * S);
+        for (size_t k = 0; k < S; ++k)
+            full_to_half<float>(s_.N, fields + 2 * nn * k, half.data() + spec * k);
+        dst.alloc(half.size());
+        hcheck(hipMemcpy(dst.p, half.data(), half.size() * sizeof(cx<float>),
+                         hipMemcpyHostToDevice),
+               "input upload");
+    }
+    void upload_nu(size_t S, const double *nu) {
+        for (size_t i = 0; i < S; ++i)
+            if (!(nu[i] > 0) || !std::isfinite(nu[i]))
+                throw std::invalid_argument("viscosities must be positive and finite");
+        hnu_.alloc(S);
+        hcheck(hipMemcpy(hnu_.p, nu, S * sizeof(double), hipMemcpyHostToDevice), "nu upload");
+    }
+    template <typename T> void download_out(FnoWork<T> &w, size_t S, double *grid) {
+        std::vector<float> h((size_t)S * s_.P());
+        hcheck(hipStreamSynchronize(st_), "surrogate forward");
+        hcheck(hipMemcpy(h.data(), w.out.p, h.size() * sizeof(float), hipMemcpyDeviceToHost),
+               "output download");
+        for (size_t i = 0; i < h.size(); ++i)
+            grid[i] = h[i];
+    }
+
+    void grow_dataset(size_t pairs) {
+        if (pairs <= cap_)
+            return;
+        const size_t cap = std::max(pairs, 2 * cap_), spec = (size_t)s_.spec();
+        Buf<cx<float>> x, y;
+        Buf<double> nu;
+        x.alloc(cap * spec);
+        y.alloc(cap * spec);
+        nu.alloc(cap);
+        if (pairs_) {
+            hcheck(hipDeviceSynchronize(), "dataset grow");
+            hcheck(hipMemcpy(x.p, ds_x_.p, pairs_ * spec * sizeof(cx<float>),
+                             hipMemcpyDeviceToDevice),
+                   "dataset grow");
+            hcheck(hipMemcpy(y.p, ds_y_.p, pairs_ * spec * sizeof(cx<float>),
+                             hipMemcpyDeviceToDevice),
+                   "dataset grow");
+            hcheck(hipMemcpy(nu.p, ds_nu_.p, pairs_ * sizeof(double), hipMemcpyDeviceToDevice),
+                   "dataset grow");
+        }
+        std::swap(x.p, ds_x_.p);
+        std::swap(x.n, ds_x_.n);
+        std::swap(y.p, ds_y_.p);
+        std::swap(y.n, ds_y_.n);
+        std::swap(nu.p, ds_nu_.p);
+        std::swap(nu.n, ds_nu_.n);
+        cap_ = cap;
+    }
+
+    // Input / output scales from the first collected window: the inverse grid
+    // RMS of w and of its increment over the horizon (Parseval on the band).
+    void auto_scales(size_t first, size_t count) {
+        const size_t spec = (size_t)s_.spec();
+        std::vector<cx<float>> x(count * spec), y(count * spec);
+        hcheck(hipMemcpy(x.data(), ds_x_.p + first * spec, x.size() * sizeof(cx<float>),
+                         hipMemcpyDeviceToHost),
+               "dataset statistics");
+        hcheck(hipMemcpy(y.data(), ds_y_.p + first * spec, y.size() * sizeof(cx<float>),
+                         hipMemcpyDeviceToHost),
+               "dataset statistics");
+        const int K = s_.K, J = 2 * K + 1, KS = s_.KS;
+        double sw = 0, sd = 0;
+        for (size_t m = 0; m < count; ++m)
+            for (int c = 0; c <= KS; ++c)
+                for (int ky = -KS; ky <= KS; ++ky) {
+                    const size_t e = m * spec + (size_t)c * J + ky + K;
+                    const double wgt = c > 0 ? 2.0 : 1.0;
+                    const double xr = x[e].x, xi = x[e].y, dr = y[e].x - xr, di = y[e].y - xi;
+                    sw += wgt * (xr * xr + xi * xi);
+                    sd += wgt * (dr * dr + di * di);
+                }
+        sw /= (double)count;
+        sd /= (double)count;
+        if (sw > 0 && sd > 0) {
+            in_scale_ = 1.0 / std::sqrt(sw);
+            out_scale_ = 1.0 / std::sqrt(sd);
+            scales_set_ = true;
+        }
+    }
+
+    void gather_batch(const std::vector<int> &idx) {
+        const int S = (int)idx.size();
+        const long spec = s_.spec();
+        bidx_.alloc(S);
+        hcheck(hipMemcpy(bidx_.p, idx.data(), S * sizeof(int), hipMemcpyHostToDevice),
+               "batch indices");
+        bx_.alloc((size_t)S * spec);
+        by_.alloc((size_t)S * spec);
+        bnu_.alloc(S);
+        hipLaunchKernelGGL(k_gather_states, grid1d((long)S * spec), dim3(256), 0, st_, ds_x_.p,
+                           bidx_.p, spec, bx_.p, S);
+        hipLaunchKernelGGL(k_gather_states, grid1d((long)S * spec), dim3(256), 0, st_, ds_y_.p,
+                           bidx_.p, spec, by_.p, S);
+        hipLaunchKernelGGL(k_gather_nu, grid1d(S), dim3(256), 0, st_, ds_nu_.p, bidx_.p, bnu_.p, S);
+    }
+
+    FnoShape s_;
+    FnoParams prm_;
+    hipStream_t own_ = nullptr, st_ = nullptr;
+    Buf<float> master_, grad_, m_, v_;
+    Buf<float> g1_, g2_, f1_, f2_, bm_, dgrid_, incr_, target_, part_, rgrad_;
+    Buf<cx<float>> diff_, hx_, hy_, ds_x_, ds_y_, bx_, by_;
+    Buf<double> hnu_, ds_nu_, bnu_, lsum_;
+    Buf<int> bidx_;
+    FnoWork<float> wf_;
+    FnoWork<__bf16> wb_;
+    size_t pairs_ = 0, cap_ = 0, horizon_ = 0;
+    double dt_ = 0;
+    long t_ = 0;
+    uint64_t version_ = 0;
+    double in_scale_ = 1.0, out_scale_ = 1.0;
+    bool scales_set_ = false;
+    uint64_t counter_ = 0;
+};
+
+} // namespace hzl_amd
+
+// ---------------------------------------------------------------- C-ABI --
+struct hzl_surrogate {
+    std::unique_ptr<hzl_amd::Surrogate> net;
+};
+
+using hzl_amd::guarded;
+
+namespace {
+hzl_amd::Surrogate &net_of(hzl_surrogate_t h) {
+    if (!h || !h->net)
+        throw std::invalid_argument("null surrogate handle");
+    return *h->net;
+}
+void need(const void *p) {
+    if (!p)
+        throw std::invalid_argument("null argument");
+}
+} // namespace
+
+extern "C" {
+
+int hzl_surrogate_create(hzl_surrogate_t *out, size_t state_n, int grid, int width, int layers,
+                         int modes, uint64_t seed) {
+    return guarded([&] {
+        need(out);
+        *out = nullptr;
+        if (state_n > 4096)
+            throw std::invalid_argument("surrogate state grid must be 64, 128 or 256");
+        auto h = std::make_unique<hzl_surrogate>();
+        h->net = std::make_unique<hzl_amd::Surrogate>((int)state_n, grid, width, layers, modes,
+                                                      seed);
+        *out = h.release();
+    });
+}
+
+int hzl_surrogate_destroy(hzl_surrogate_t h) {
+    return guarded([&] { delete h; });
+}
+
+int hzl_surrogate_param_count(hzl_surrogate_t h, size_t *count) {
+    return guarded([&] {
+        need(count);
+        *count = (size_t)net_of(h).param_count();
+    });
+}
+
+int hzl_surrogate_get_params(hzl_surrogate_t h, float *params) {
+    return guarded([&] {
+        need(params);
+        net_of(h).get_params(params);
+    });
+}
+
+int hzl_surrogate_set_params(hzl_surrogate_t h, const float *params) {
+    return guarded([&] {
+        need(params);
+        net_of(h).set_params(params);
+    });
+}
+
+int hzl_surrogate_set_scales(hzl_surrogate_t h, double input_scale, double output_scale) {
+    return guarded([&] { net_of(h).set_scales(input_scale, output_scale); });
+}
+
+int hzl_surrogate_get_scales(hzl_surrogate_t h, double *scales) {
+    return guarded([&] {
+        need(scales);
+        net_of(h).get_scales(scales);
+    });
+}
+
+int hzl_surrogate_forward(hzl_surrogate_t h, int precision, size_t count, const double *fields,
+                          const double *viscosity, double *grid_out) {
+    return guarded([&] {
+        need(fields);
+        need(viscosity);
+        need(grid_out);
+        net_of(h).forward_host(precision, count, fields, viscosity, grid_out);
+    });
+}
+
+int hzl_surrogate_loss_grad(hzl_surrogate_t h, size_t count, const double *fields,
+                            const double *next_fields, const double *viscosity, double *loss,
+                            float *grad) {
+    return guarded([&] {
+        need(fields);
+        need(next_fields);
+        need(viscosity);
+        need(loss);
+        need(grad);
+        net_of(h).loss_grad_host(count, fields, next_fields, viscosity, loss, grad);
+    });
+}
+
+int hzl_surrogate_collect(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt,
+                          size_t horizon_steps, size_t windows, int noise_free_targets) {
+    return guarded([&] {
+        net_of(h).collect(hzl_amd::engine_of(ensemble), dt, horizon_steps, windows,
+                          noise_free_targets != 0);
+    });
+}
+
+int hzl_surrogate_dataset_size(hzl_surrogate_t h, size_t *pairs) {
+    return guarded([&] {
+        need(pairs);
+        *pairs = net_of(h).dataset_size();
+    });
+}
+
+int hzl_surrogate_fit(hzl_surrogate_t h, size_t first, size_t count, size_t epochs, size_t batch,
+                      double learning_rate, uint64_t seed, double *epoch_loss) {
+    return guarded([&] {
+        net_of(h).fit(first, count, epochs, batch, learning_rate, seed, epoch_loss);
+    });
+}
+
+int hzl_surrogate_evaluate(hzl_surrogate_t h, int precision, size_t first, size_t count,
+                           double *loss) {
+    return guarded([&] {
+        need(loss);
+        *loss = net_of(h).evaluate(precision, first, count);
+    });
+}
+
+int hzl_surrogate_forecast(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt, size_t steps,
+                           int precision) {
+    return guarded([&] {
+        net_of(h).forecast(hzl_amd::engine_of(ensemble), dt, steps, precision);
+    });
+}
+
+} // extern "C"

@@ -17,7 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libhzl_amd.so")
 
 HZL_OK, HZL_EINVAL, HZL_ERUNTIME = 0, 1, 2
-FP32, FP64 = 0, 1
+FP32, FP64, BF16 = 0, 1, 2
 ALL = C.c_size_t(-1).value  # (size_t)-1: every member
 
 # Every entry point declared in include/hzl_amd.h.
@@ -29,6 +29,11 @@ EXPORTS = (
     "hzl_ensemble_step_index",
     "hzl_ensemble_device_state", "hzl_ensemble_stream",
     "hzl_pf_begin", "hzl_pf_assimilate", "hzl_pf_result",
+    "hzl_surrogate_create", "hzl_surrogate_destroy", "hzl_surrogate_param_count",
+    "hzl_surrogate_get_params", "hzl_surrogate_set_params", "hzl_surrogate_set_scales",
+    "hzl_surrogate_get_scales", "hzl_surrogate_forward", "hzl_surrogate_loss_grad",
+    "hzl_surrogate_collect", "hzl_surrogate_dataset_size", "hzl_surrogate_fit",
+    "hzl_surrogate_evaluate", "hzl_surrogate_forecast",
 )
 
 
@@ -84,6 +89,22 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_pf_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp, _dp,
                                     C.c_int, _dp, _dp]
     L.hzl_pf_result.argtypes = [vp, _dp, C.POINTER(C.c_size_t)]
+    fp, sz = C.POINTER(C.c_float), C.c_size_t
+    L.hzl_surrogate_create.argtypes = [C.POINTER(vp), sz, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_uint64]
+    L.hzl_surrogate_destroy.argtypes = [vp]
+    L.hzl_surrogate_param_count.argtypes = [vp, C.POINTER(sz)]
+    L.hzl_surrogate_get_params.argtypes = [vp, fp]
+    L.hzl_surrogate_set_params.argtypes = [vp, fp]
+    L.hzl_surrogate_set_scales.argtypes = [vp, C.c_double, C.c_double]
+    L.hzl_surrogate_get_scales.argtypes = [vp, _dp]
+    L.hzl_surrogate_forward.argtypes = [vp, C.c_int, sz, _dp, _dp, _dp]
+    L.hzl_surrogate_loss_grad.argtypes = [vp, sz, _dp, _dp, _dp, _dp, fp]
+    L.hzl_surrogate_collect.argtypes = [vp, vp, C.c_double, sz, sz, C.c_int]
+    L.hzl_surrogate_dataset_size.argtypes = [vp, C.POINTER(sz)]
+    L.hzl_surrogate_fit.argtypes = [vp, sz, sz, sz, sz, C.c_double, C.c_uint64, _dp]
+    L.hzl_surrogate_evaluate.argtypes = [vp, C.c_int, sz, sz, _dp]
+    L.hzl_surrogate_forecast.argtypes = [vp, vp, C.c_double, sz, C.c_int]
     for name in EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -249,6 +270,149 @@ def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,
         rec, mean = particles.pf_assimilate(dt, int(target - current), sel[:, 1], sel[:, 2],
                                             sel[:, 3], sel[:, 4], gi == len(groups) - 1,
                                             mean=retain_states)
+        history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
+                            max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean))
+        current = int(target)
+        if not np.isfinite(rec[0]):
+            break
+    ll, events = particles.pf_result()
+    return dict(log_likelihood=ll, resampling_events=events, history=history)
+
+
+_SUR_PRECISION = {"fp32": FP32, "bf16": BF16}
+
+
+class Surrogate:
+    """Learned forecast surrogate (spectral neural operator) on one handle.
+
+    No reference counterpart: it replaces the particle forecast of the filter
+    loops (src/filter.cpp:46-48).  See include/hzl_amd.h for every call.
+    """
+
+    def __init__(self, state_n: int = 128, grid: int = 64, width: int = 32, layers: int = 4,
+                 modes: int = 16, seed: int = 0):
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.hzl_surrogate_create(C.byref(h), state_n, grid, width, layers, modes, seed))
+        self._h = h
+        self.state_n, self.grid = int(state_n), int(grid)
+        self.width, self.layers, self.modes = int(width), int(layers), int(modes)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.hzl_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def param_count(self) -> int:
+        v = C.c_size_t()
+        _check(_lib.hzl_surrogate_param_count(self._h, C.byref(v)))
+        return v.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.param_count, dtype=np.float32)
+        _check(_lib.hzl_surrogate_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        if a.size != self.param_count:
+            raise ValueError("parameter vector has the wrong length")
+        _check(_lib.hzl_surrogate_set_params(self._h, a.ctypes.data_as(C.POINTER(C.c_float))))
+
+    @property
+    def scales(self):
+        out = np.zeros(2)
+        _check(_lib.hzl_surrogate_get_scales(self._h, out.ctypes.data_as(_dp)))
+        return float(out[0]), float(out[1])
+
+    def set_scales(self, input_scale: float, output_scale: float):
+        _check(_lib.hzl_surrogate_set_scales(self._h, input_scale, output_scale))
+
+    def forward(self, fields, nu, precision: str = "fp32") -> np.ndarray:
+        f = np.ascontiguousarray(fields, dtype=np.complex128)
+        S = f.shape[0]
+        nu = np.ascontiguousarray(np.broadcast_to(nu, (S,)), dtype=np.float64)
+        _fields(f, S, self.state_n)
+        out = np.zeros((S, self.grid, self.grid))
+        _check(_lib.hzl_surrogate_forward(self._h, _SUR_PRECISION[precision], S,
+                                          f.ctypes.data_as(_dp), nu.ctypes.data_as(_dp),
+                                          out.ctypes.data_as(_dp)))
+        return out
+
+    def loss_grad(self, fields, next_fields, nu):
+        f = np.ascontiguousarray(fields, dtype=np.complex128)
+        g = np.ascontiguousarray(next_fields, dtype=np.complex128)
+        S = f.shape[0]
+        _fields(f, S, self.state_n)
+        _fields(g, S, self.state_n)
+        nu = np.ascontiguousarray(np.broadcast_to(nu, (S,)), dtype=np.float64)
+        loss = C.c_double()
+        grad = np.zeros(self.param_count, dtype=np.float32)
+        _check(_lib.hzl_surrogate_loss_grad(self._h, S, f.ctypes.data_as(_dp),
+                                            g.ctypes.data_as(_dp), nu.ctypes.data_as(_dp),
+                                            C.byref(loss), grad.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, grad
+
+    def collect(self, ensemble: Ensemble, dt: float, horizon_steps: int, windows: int,
+                noise_free_targets: bool = True):
+        _check(_lib.hzl_surrogate_collect(self._h, ensemble._h, dt, horizon_steps, windows,
+                                          int(noise_free_targets)))
+
+    @property
+    def dataset_size(self) -> int:
+        v = C.c_size_t()
+        _check(_lib.hzl_surrogate_dataset_size(self._h, C.byref(v)))
+        return v.value
+
+    def fit(self, first: int, count: int, epochs: int, batch: int, lr: float, seed: int = 0):
+        out = np.zeros(epochs)
+        _check(_lib.hzl_surrogate_fit(self._h, first, count, epochs, batch, lr, seed,
+                                      out.ctypes.data_as(_dp)))
+        return out
+
+    def evaluate(self, first: int, count: int, precision: str = "fp32") -> float:
+        v = C.c_double()
+        _check(_lib.hzl_surrogate_evaluate(self._h, _SUR_PRECISION[precision], first, count,
+                                           C.byref(v)))
+        return v.value
+
+    def forecast(self, ensemble: Ensemble, dt: float, steps: int, precision: str = "bf16"):
+        _check(_lib.hzl_surrogate_forecast(self._h, ensemble._h, dt, steps,
+                                           _SUR_PRECISION[precision]))
+
+
+def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt: float,
+                     seed: int, precision: str = "bf16", retain_states: bool = True):
+    """Bootstrap particle filter whose forecast between observation groups is
+    one surrogate evaluation (every group interval must equal the surrogate's
+    horizon); weighting and resampling as in bootstrap_filter."""
+    obs = np.asarray(observations, dtype=np.float64)
+    if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
+        raise ValueError("observations must be rows of (step, x, y, value, sd)")
+    if np.any(np.diff(obs[:, 0]) < 0):
+        raise ValueError("observations must be time ordered")
+    steps = obs[:, 0].astype(np.int64)
+    groups = np.unique(steps)
+    particles.pf_begin(seed, max_groups=len(groups) + 1)
+    history, current = [], 0
+    for gi, target in enumerate(groups):
+        sel = obs[steps == target]
+        surrogate.forecast(particles, dt, int(target - current), precision)
+        rec, mean = particles.pf_assimilate(dt, 0, sel[:, 1], sel[:, 2], sel[:, 3], sel[:, 4],
+                                            gi == len(groups) - 1, mean=retain_states)
         history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
                             max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean))
         current = int(target)

@@ -39,8 +39,10 @@ extern "C" {
 
 #define HZL_FP32 0
 #define HZL_FP64 1
+#define HZL_BF16 2 /* surrogate inference operands only */
 
 typedef struct hzl_ensemble *hzl_ensemble_t;
+typedef struct hzl_surrogate *hzl_surrogate_t;
 
 /* Model parameters.  The first six fields mirror hzl::Parameters
  * (include/hzl/solver.hpp:15-23).  The engine's stochastic forcing is white in
@@ -165,6 +167,76 @@ int hzl_ensemble_device_state(hzl_ensemble_t h, void **ptr, size_t *member_strid
 /* The handle's HIP stream (hipStream_t) for callers that order their own work
  * after the engine's or time it with events. */
 int hzl_ensemble_stream(hzl_ensemble_t h, void **stream);
+
+/* ------------------------------------------------------------------------
+ * Learned forecast surrogate (spectral neural operator).  No counterpart in
+ * the reference: it replaces the particle forecast of the filter loops
+ * (src/filter.cpp:46-48 advances every particle with the solver) by one
+ * network evaluation per observation interval.  The network sees the state
+ * on an internal grid x grid lattice (exact truncated DFT of the band
+ * |kx|, |ky| <= grid/2 - 1), log viscosity and the Kolmogorov pattern, and
+ * predicts the increment over a fixed horizon of `horizon_steps` solver
+ * steps; modes outside the band decay linearly, and the shell forcing adds
+ * white noise of variance forcing_sigma^2 h / 2 per component.
+ * ------------------------------------------------------------------------ */
+
+/* state_n: the ensemble grid it will serve (64, 128 or 256); grid 32 or 64;
+ * width (channels, multiple of 8, <= 64); layers 1..8; modes kept per axis
+ * (kx in [0, modes), ky in [-modes, modes)), modes <= grid / 4.  Weights are
+ * drawn from mt19937_64(seed). */
+int hzl_surrogate_create(hzl_surrogate_t *out, size_t state_n, int grid, int width, int layers,
+                         int modes, uint64_t seed);
+int hzl_surrogate_destroy(hzl_surrogate_t h);
+
+/* The FP32 parameter vector (layout documented in csrc/surrogate.hip). */
+int hzl_surrogate_param_count(hzl_surrogate_t h, size_t *count);
+int hzl_surrogate_get_params(hzl_surrogate_t h, float *params);
+int hzl_surrogate_set_params(hzl_surrogate_t h, const float *params);
+
+/* Input scale (grid w is multiplied by it) and output scale (the network
+ * predicts increment * output_scale).  Set from the first collected window
+ * unless given here. */
+int hzl_surrogate_set_scales(hzl_surrogate_t h, double input_scale, double output_scale);
+int hzl_surrogate_get_scales(hzl_surrogate_t h, double *scales);
+
+/* Network output (count x grid x grid doubles) for `count` states in the
+ * reference Field layout with per-sample viscosities; precision HZL_FP32 or
+ * HZL_BF16 operands. */
+int hzl_surrogate_forward(hzl_surrogate_t h, int precision, size_t count, const double *fields,
+                          const double *viscosity, double *grid_out);
+
+/* Mean squared error between the network output and the scaled grid
+ * increment next - fields, and its gradient with respect to every parameter
+ * (FP32 operands). */
+int hzl_surrogate_loss_grad(hzl_surrogate_t h, size_t count, const double *fields,
+                            const double *next_fields, const double *viscosity, double *loss,
+                            float *grad);
+
+/* Append windows x members training pairs from an FP32 ensemble of the
+ * surrogate's state_n: each window copies the states as inputs and advances
+ * the ensemble horizon_steps steps of dt with its own forcing.  Targets are
+ * the advanced states, or with noise_free_targets the same inputs advanced
+ * without the white shell forcing (the deterministic propagator, which the
+ * forecast then complements with the forcing noise). */
+int hzl_surrogate_collect(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt,
+                          size_t horizon_steps, size_t windows, int noise_free_targets);
+int hzl_surrogate_dataset_size(hzl_surrogate_t h, size_t *pairs);
+
+/* Adam on pairs [first, first + count): `epochs` passes of shuffled
+ * minibatches (mt19937_64(seed)), cosine learning rate from learning_rate to
+ * learning_rate / 20; epoch_loss (optional) receives each epoch's mean loss. */
+int hzl_surrogate_fit(hzl_surrogate_t h, size_t first, size_t count, size_t epochs, size_t batch,
+                      double learning_rate, uint64_t seed, double *epoch_loss);
+
+/* Mean loss over pairs [first, first + count), forward only. */
+int hzl_surrogate_evaluate(hzl_surrogate_t h, int precision, size_t first, size_t count,
+                           double *loss);
+
+/* Advance every member of an FP32 ensemble by steps * dt with the network
+ * (steps and dt must equal the trained horizon).  The ensemble's step counter
+ * advances by `steps`; filter calls may follow with steps = 0. */
+int hzl_surrogate_forecast(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt, size_t steps,
+                           int precision);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,8 @@
 // LDS, and the engine's launch sequence (kernels.hpp header comment) replayed
 // over host arrays.  Used by the CPU test suite to check indexing, layouts and
 // numerics against the oracle without a GPU.  Never linked into the product.
+#include "gemm_mfma.hpp"
+#include "surrogate_kernels.hpp"
 #include "tables.hpp"
 
 #include <barrier>
@@ -18,9 +20,15 @@ using namespace hzl_amd;
 
 namespace {
 
+// Operands of one emulated matrix instruction, exchanged between lanes.
+struct MfmaScratch {
+    float a[64][8], b[64][8];
+};
+
 struct CpuCtx {
     int lane;
     std::barrier<> *bar;
+    MfmaScratch *mx = nullptr;
     int tid() const {
         return lane;
     }
@@ -32,6 +40,35 @@ struct CpuCtx {
     void wsync() const {
         bar->arrive_and_wait();
     }
+    // v_mfma_f32_32x32x16_bf16: lane l supplies A row l%32, k (l/32)*8 + j and
+    // B column l%32, same k; accumulator i is D[8(i/4) + 4(l/32) + i%4][l%32].
+    void mfma_bf16(const bf16x8_t &fa, const bf16x8_t &fb, f32x16_t &acc, int l) const {
+        for (int j = 0; j < 8; ++j) {
+            mx->a[l][j] = (float)fa[j];
+            mx->b[l][j] = (float)fb[j];
+        }
+        bar->arrive_and_wait();
+        for (int i = 0; i < 16; ++i) {
+            const int row = 8 * (i / 4) + 4 * (l / 32) + i % 4, col = l % 32;
+            float s = 0;
+            for (int kb = 0; kb < 2; ++kb)
+                for (int j = 0; j < 8; ++j)
+                    s += mx->a[row + 32 * kb][j] * mx->b[col + 32 * kb][j];
+            acc[i] += s;
+        }
+        bar->arrive_and_wait();
+    }
+    // v_mfma_f32_32x32x2_f32: lane l supplies A[l%32][l/32] and B[l/32][l%32].
+    void mfma_f32(float fa, float fb, f32x16_t &acc, int l) const {
+        mx->a[l][0] = fa;
+        mx->b[l][0] = fb;
+        bar->arrive_and_wait();
+        for (int i = 0; i < 16; ++i) {
+            const int row = 8 * (i / 4) + 4 * (l / 32) + i % 4, col = l % 32;
+            acc[i] += mx->a[row][0] * mx->b[col][0] + mx->a[row + 32][0] * mx->b[col + 32][0];
+        }
+        bar->arrive_and_wait();
+    }
 };
 
 // Launch `blocks` workgroups of `threads` lanes; body(ctx, block, lds).
@@ -39,10 +76,11 @@ template <class Body> void launch(long blocks, int threads, size_t lds_bytes, Bo
     std::barrier<> bar(threads);
     std::vector<unsigned char> lds(lds_bytes + 64);
     std::vector<std::thread> pool;
+    MfmaScratch mx;
     pool.reserve((size_t)threads);
     for (int lane = 0; lane < threads; ++lane)
         pool.emplace_back([&, lane] {
-            CpuCtx ctx{lane, &bar};
+            CpuCtx ctx{lane, &bar, &mx};
             for (long b = 0; b < blocks; ++b) {
                 body(ctx, b, lds.data());
                 bar.arrive_and_wait();
@@ -264,9 +302,89 @@ template <int N, typename R> int fft_lines(int lines, int inverse, double *data)
     return 0;
 }
 
+// One product through gemm_wave, one wavefront per 32 x 32 tile.  Operands
+// arrive as FP32 host arrays and are rounded to T; C (and pre) are returned
+// as FP32 after rounding to the output type.
+template <typename T>
+int gemm_any(const int *dims, const long *st, const float *A, size_t na, const float *B,
+             size_t nb, float *C, size_t nc, const float *bias, const float *aux, size_t nx,
+             float *pre, float alpha, float beta, int act, int c_bf16) {
+    std::vector<T> a(na), b(nb), x(nx);
+    for (size_t i = 0; i < na; ++i)
+        a[i] = (T)A[i];
+    for (size_t i = 0; i < nb; ++i)
+        b[i] = (T)B[i];
+    for (size_t i = 0; i < nx; ++i)
+        x[i] = (T)aux[i];
+    std::vector<unsigned char> c(nc * 4), p(nc * 4);
+    for (size_t i = 0; i < nc; ++i) {
+        if (c_bf16) {
+            reinterpret_cast<__bf16 *>(c.data())[i] = (__bf16)C[i];
+        } else {
+            reinterpret_cast<float *>(c.data())[i] = C[i];
+        }
+    }
+    GemmArgs<T> g{};
+    g.M = dims[0];
+    g.N = dims[1];
+    g.K = dims[2];
+    g.nb1 = dims[3];
+    g.nb2 = dims[4];
+    g.A = a.data();
+    g.a_m = st[0];
+    g.a_k = st[1];
+    g.a_b1 = st[2];
+    g.a_b2 = st[3];
+    g.B = b.data();
+    g.b_n = st[4];
+    g.b_k = st[5];
+    g.b_b1 = st[6];
+    g.b_b2 = st[7];
+    g.C = c.data();
+    g.c_m = st[8];
+    g.c_n = st[9];
+    g.c_b1 = st[10];
+    g.c_b2 = st[11];
+    g.c_type = c_bf16 ? OUT_BF16 : OUT_F32;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.bias = bias;
+    g.aux = nx ? x.data() : nullptr;
+    g.x_m = st[12];
+    g.x_n = st[13];
+    g.x_b1 = st[14];
+    g.x_b2 = st[15];
+    g.aux_type = sizeof(T) == 2 ? OUT_BF16 : OUT_F32;
+    g.pre = pre ? p.data() : nullptr;
+    g.act = act;
+    const long tiles = (long)g.nb1 * g.nb2 * ((g.M + 31) / 32) * ((g.N + 31) / 32);
+    launch(tiles, 64, GemmTile<T>::LDS_PER_WAVE * sizeof(T),
+           [&](const CpuCtx &ctx, long tile, unsigned char *lds) {
+               gemm_wave<T>(ctx, g, tile, ctx.lane, reinterpret_cast<T *>(lds));
+           });
+    for (size_t i = 0; i < nc; ++i) {
+        C[i] = c_bf16 ? (float)reinterpret_cast<__bf16 *>(c.data())[i]
+                      : reinterpret_cast<float *>(c.data())[i];
+        if (pre)
+            pre[i] = c_bf16 ? (float)reinterpret_cast<__bf16 *>(p.data())[i]
+                            : reinterpret_cast<float *>(p.data())[i];
+    }
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
+
+int emu_gemm(int bf16, const int *dims, const long *strides, const float *A, size_t na,
+             const float *B, size_t nb, float *C, size_t nc, const float *bias, const float *aux,
+             size_t nx, float *pre, float alpha, float beta, int act, int c_bf16) {
+    if (bf16)
+        return gemm_any<__bf16>(dims, strides, A, na, B, nb, C, nc, bias, aux, nx, pre, alpha,
+                                beta, act, c_bf16);
+    return gemm_any<float>(dims, strides, A, na, B, nb, C, nc, bias, aux, nx, pre, alpha, beta,
+                           act, c_bf16);
+}
 
 int emu_run(int n, int precision, int B, double nu, double alpha, double sigma, double amp,
             int kf, int kmin, int kmax, uint64_t seed, double dt, int steps, int grf,

@@ -20,8 +20,34 @@ class Emulator:
         L.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                               C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_double,
                               C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+        _fp, _ip, _lp = C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_long)
+        L.emu_gemm.restype = C.c_int
+        L.emu_gemm.argtypes = [C.c_int, _ip, _lp, _fp, C.c_size_t, _fp, C.c_size_t, _fp,
+                               C.c_size_t, _fp, _fp, C.c_size_t, _fp, C.c_float, C.c_float,
+                               C.c_int, C.c_int]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+
+    def gemm(self, bf16, dims, strides, A, B, C0, bias=None, aux=None, want_pre=False,
+             alpha=1.0, beta=0.0, act=0, c_bf16=False):
+        """gemm_wave on the CPU lanes.  dims (M, N, K, nb1, nb2); strides the 16
+        element strides (a_m, a_k, a_b1, a_b2, b_n, b_k, b_b1, b_b2, c_m, c_n,
+        c_b1, c_b2, x_m, x_n, x_b1, x_b2).  Returns (C, pre)."""
+        fp = C.POINTER(C.c_float)
+        f = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float32).ravel()
+        A, B, aux, bias = f(A), f(B), f(aux), f(bias)
+        out = np.ascontiguousarray(C0, dtype=np.float32).ravel().copy()
+        pre = np.zeros_like(out) if want_pre else None
+        d = np.array(dims, dtype=np.int32)
+        st = np.array(strides, dtype=np.int64)
+        ptr = lambda x: None if x is None else x.ctypes.data_as(fp)
+        rc = self.lib.emu_gemm(int(bf16), d.ctypes.data_as(C.POINTER(C.c_int)),
+                               st.ctypes.data_as(C.POINTER(C.c_long)), ptr(A), A.size, ptr(B),
+                               B.size, ptr(out), out.size, ptr(bias), ptr(aux),
+                               0 if aux is None else aux.size, ptr(pre), alpha, beta, act,
+                               int(c_bf16))
+        assert rc == 0
+        return out, pre
 
     def fft_lines(self, x, inverse=False, precision="fp32"):
         d = np.ascontiguousarray(x, dtype=np.complex128).copy()

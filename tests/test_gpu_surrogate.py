@@ -1,0 +1,185 @@
+"""GPU checks of the forecast surrogate (csrc/surrogate.hip) through the C-ABI.
+
+  * forward output (FP32 and BF16 operands) against the FP64 network reference
+    (tests/oracle_fno.py) on the same parameters and inputs;
+  * every parameter gradient of the training loss against the reference's
+    autograd gradients (themselves pinned by finite differences);
+  * the state <-> grid transforms and the forecast update against the
+    reference's band transforms, and the forecast noise variance;
+  * training: Adam on pairs collected from the engine lowers the held-out loss
+    far below persistence (predicting no change), i.e. the network has learned
+    the flow map; and a surrogate-driven particle filter runs end to end.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_fno as F
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(state_n=64, grid=32, width=16, layers=2, modes=4)
+TRAIN = dict(state_n=64, grid=32, width=32, layers=4, modes=8)
+FULL = dict(state_n=128, grid=64, width=32, layers=4, modes=16)
+
+
+def shape_of(cfg):
+    return F.Shape(cfg["state_n"], cfg["grid"], cfg["width"], cfg["layers"], cfg["modes"])
+
+
+def grf_fields(orc, n, count, base):
+    return np.stack([orc.grf(n, orc.stream_seed(base, i)) for i in range(count)])
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-4), ("bf16", 4e-2)])
+@pytest.mark.parametrize("cfg", [SMALL, FULL], ids=["small", "full"])
+def test_forward_matches_reference(pkg, orc, cfg, precision, tol):
+    s = shape_of(cfg)
+    f = grf_fields(orc, s.n, 3, 11)
+    nu = np.array([5e-4, 1e-3, 2e-3])
+    scale = 1.0 / np.sqrt(np.mean(F.sample_grid(s, f).numpy() ** 2))
+    with pkg.Surrogate(**cfg, seed=7) as sur:
+        sur.set_scales(scale, 1.0)
+        got = sur.forward(f, nu, precision)
+        p = torch.as_tensor(sur.get_params().astype(np.float64))
+    want = F.forward(s, p, f, nu, scale).numpy()
+    print(cfg, precision, rel(got, want))
+    assert rel(got, want) < tol
+
+
+@pytest.mark.parametrize("cfg,S", [(SMALL, 2), (FULL, 1)], ids=["small", "full"])
+def test_gradients_match_autograd(pkg, orc, cfg, S):
+    s = shape_of(cfg)
+    f = grf_fields(orc, s.n, S, 21)
+    nxt = f + 0.2 * grf_fields(orc, s.n, S, 22)
+    nu = np.linspace(6e-4, 1.8e-3, S)
+    ins, outs = 2.0, 3.0
+    with pkg.Surrogate(**cfg, seed=8) as sur:
+        sur.set_scales(ins, outs)
+        loss, grad = sur.loss_grad(f, nxt, nu)
+        p = sur.get_params().astype(np.float64)
+    want_loss, want = F.loss_and_grad(s, p, f, nxt, nu, ins, outs)
+    print(cfg, loss, want_loss, rel(grad, want))
+    assert abs(loss - want_loss) < 1e-5 * want_loss
+    assert rel(grad, want) < 1e-4
+    # every parameter group separately (small groups would hide in the total)
+    o = 0
+    for name, sh in s.sizes():
+        k = int(np.prod(sh))
+        g, w = grad[o:o + k], want[o:o + k]
+        assert np.linalg.norm(g - w) <= 1e-3 * np.linalg.norm(w) + 1e-9, name
+        o += k
+
+
+def test_forecast_applies_band_increment_and_decay(pkg, orc):
+    cfg = SMALL
+    s = shape_of(cfg)
+    B, dt, steps, nu, alpha = 8, 2e-3, 25, 1e-3, 0.01
+    f = grf_fields(orc, s.n, B, 31)
+    with pkg.Surrogate(**cfg, seed=9) as sur, \
+            pkg.Ensemble(s.n, B, "fp32", nu=nu, alpha=alpha, sigma=0.0, amp=0.1, seed=4) as ens:
+        sur.set_scales(1.5, 4.0)
+        ens.set_state(f)
+        f32 = ens.get_state()  # FP32-rounded retained states
+        out = sur.forward(f32, np.full(B, nu), "fp32")
+        sur.forecast(ens, dt, steps, "fp32")
+        got = ens.get_state()
+        assert ens.step_index == steps
+    h = steps * dt
+    want = f32 + F.band_increment(s, out / 4.0)
+    k = np.fft.fftfreq(s.n, 1.0 / s.n)
+    kx, ky = np.abs(k)[None, :], np.abs(k)[:, None]
+    outside = (kx > s.KS) | (ky > s.KS)
+    decay = np.exp(-(nu * (kx ** 2 + ky ** 2) + alpha) * h)
+    want = np.where(outside[None], f32 * decay[None], want)
+    assert rel(got, want) < 1e-5
+
+
+def test_forecast_noise_has_the_forcing_variance(pkg, orc):
+    cfg = SMALL
+    s = shape_of(cfg)
+    B, dt, steps, sigma = 512, 2e-3, 25, 0.5
+    f = grf_fields(orc, s.n, 1, 41)
+    with pkg.Surrogate(**cfg, seed=9) as sur:
+        sur.set_scales(1.0, 1.0)
+        runs = []
+        for sg in (0.0, sigma):
+            with pkg.Ensemble(s.n, B, "fp32", nu=1e-3, sigma=sg, kmin=1, kmax=4, seed=5) as ens:
+                ens.set_state(np.repeat(f, B, axis=0))
+                sur.forecast(ens, dt, steps, "fp32")
+                runs.append(ens.get_state())
+    d = runs[1] - runs[0]
+    k = np.fft.fftfreq(s.n, 1.0 / s.n)
+    k2 = k[None, :] ** 2 + k[:, None] ** 2
+    shell = (k2 >= 1) & (k2 <= 16) & (k[None, :] > 0)
+    var = np.mean(np.concatenate([d[:, shell].real.ravel(), d[:, shell].imag.ravel()]) ** 2)
+    want = sigma ** 2 * steps * dt / 2
+    print(var, want)
+    assert abs(var / want - 1) < 0.1
+
+
+def collect_and_train(pkg, cfg, epochs=20, windows=6, B=256, seed=3):
+    """Pairs from a forced 64^2 ensemble with log-uniform viscosities; targets
+    advanced without the shell noise (the deterministic propagator)."""
+    n, dt, steps, sigma = cfg["state_n"], 4e-3, 25, 0.02
+    rng = np.random.default_rng(seed)
+    ens = pkg.Ensemble(n, B, "fp32", nu=1e-3, sigma=sigma, amp=0.1, seed=seed)
+    for m in range(B):
+        ens.set_member_parameters(m, float(np.exp(rng.uniform(np.log(5e-4), np.log(2e-3)))),
+                                  0.0, sigma, 0.1)
+    ens.init_grf()
+    ens.step(dt, 50)
+    sur = pkg.Surrogate(**cfg, seed=seed)
+    sur.collect(ens, dt, steps, windows, noise_free_targets=True)
+    assert sur.dataset_size == windows * B
+    train = (windows - 1) * B
+    losses = sur.fit(0, train, epochs, 32, 3e-3, seed=seed)
+    return ens, sur, losses, train, B, dt, steps
+
+
+def test_training_learns_the_flow_map(pkg):
+    """What the network gains: on held-out pairs its squared error is a small
+    fraction of persistence's (forecasting no change)."""
+    ens, sur, losses, train, B, dt, steps = collect_and_train(pkg, TRAIN)
+    held = sur.evaluate(train, B, "fp32")
+    held_bf16 = sur.evaluate(train, B, "bf16")
+    # persistence on the same held-out pairs: the all-zero network
+    trained = sur.get_params()
+    sur.set_params(np.zeros_like(trained))
+    persistence = sur.evaluate(train, B, "fp32")
+    sur.set_params(trained)
+    print("epoch losses", losses, "held-out", held, held_bf16, "persistence", persistence)
+    assert np.all(np.isfinite(losses)) and losses[-1] < 0.1 * losses[0]
+    assert held < 0.1 * persistence
+    assert held_bf16 < 1.5 * held + 0.005
+    ens.close()
+    sur.close()
+
+
+def test_surrogate_filter_runs_and_tracks(pkg, orc):
+    ens, sur, _, _, _, dt, steps = collect_and_train(pkg, TRAIN, epochs=8)
+    ens.close()
+    n = TRAIN["state_n"]
+    rng = np.random.default_rng(9)
+    truth = orc.grf(n, 777)
+    rows, t = [], 0
+    for g in range(4):
+        truth = orc.trajectory(n, 1e-3, 0.0, dt, steps, 777, sigma=0.02, step0=t,
+                               initial=truth)
+        t += steps
+        ph = orc.fft2d(truth, inverse=True).real
+        xs, ys = rng.integers(0, n, 64), rng.integers(0, n, 64)
+        rows += [(t, x, y, ph[y, x] + 0.05 * rng.standard_normal(), 0.05) for x, y in zip(xs, ys)]
+    rows = np.array(rows)
+    with pkg.Ensemble(n, 128, "fp32", nu=1e-3, sigma=0.02, amp=0.1, seed=12) as pf:
+        pf.init_grf()
+        res = pkg.surrogate_filter(pf, sur, rows, dt, 5, precision="bf16")
+    sur.close()
+    assert np.isfinite(res["log_likelihood"])
+    assert len(res["history"]) == 4
