@@ -129,6 +129,114 @@ def cpu_baseline(n: int, nu: float, dt: float) -> dict:
             "kind": kind, "sample": what, "seconds": round(sec, 3)}
 
 
+def rms_error(mean_field, truth) -> float:
+    """Grid RMS of (mean - truth) by Parseval on full spectra whose physical
+    field is sum_k c_k e^{ik.x}."""
+    return float(np.sqrt(np.sum(np.abs(mean_field - truth) ** 2)))
+
+
+def surrogate_pf(pkg, cycles: int = 20, particles: int = 256, train_windows: int = 20,
+                 epochs: int = 4) -> dict:
+    """Particle filter at 128^2 with the plain solver vs the trained surrogate
+    as the forecast between observation groups.
+
+    Twin experiment (synthetic, seeded): truth = one FP32 member, nu = 1e-3,
+    dt = 1e-3, 0.1 sin(4y) + white noise 0.05 on 1 <= |k| <= 4; every 50 steps
+    (t + 0.05) 128 point vorticity observations with noise sd 0.02.  Particles
+    start at truth + 0.5 x an independent GRF.  The surrogate (4 layers, width
+    32, 16 modes, 64^2 grid) is first trained on 20 x 1024 solver pairs
+    (log-uniform nu in [5e-4, 2e-3], noise-free targets) and runs in BF16.
+    cycles/s counts the whole filter loop (forecast + weighting + resampling).
+    """
+    n, dt, H, sigma, amp = 128, 1e-3, 50, 0.05, 0.1
+    nobs, sd = 128, 0.02
+    out = {"config": f"{particles} particles at {n}x{n}, {cycles} cycles of {H} steps "
+                     f"(dt={dt}), {nobs} obs/cycle sd={sd}; surrogate 4x32ch, 16 modes, "
+                     f"trained on {train_windows}x1024 pairs for {epochs} epochs"}
+    # -- training data and fit
+    rng = np.random.default_rng(SEED)
+    t0 = time.perf_counter()
+    data = pkg.Ensemble(n, 1024, "fp32", nu=1e-3, sigma=sigma, amp=amp, seed=SEED + 1)
+    for m in range(1024):
+        data.set_member_parameters(m, float(np.exp(rng.uniform(np.log(5e-4), np.log(2e-3)))),
+                                   0.0, sigma, amp)
+    data.init_grf()
+    data.step(dt, 50)
+    sur = pkg.Surrogate(128, 64, 32, 4, 16, seed=SEED)
+    sur.collect(data, dt, H, train_windows, noise_free_targets=True)
+    data.close()
+    t1 = time.perf_counter()
+    train = (train_windows - 1) * 1024
+    losses = sur.fit(0, train, epochs, 32, 3e-3, seed=SEED)
+    t2 = time.perf_counter()
+    trained = sur.get_params()
+    sur.set_params(np.zeros_like(trained))
+    persistence = sur.evaluate(train, 1024, "fp32")
+    sur.set_params(trained)
+    held32 = sur.evaluate(train, 1024, "fp32")
+    held16 = sur.evaluate(train, 1024, "bf16")  # sets the error estimate used below
+    out["training"] = {"pairs": sur.dataset_size, "collect_s": round(t1 - t0, 2),
+                       "fit_s": round(t2 - t1, 2), "epoch_loss": [round(x, 5) for x in losses],
+                       "held_out_loss_fp32": round(held32, 6),
+                       "held_out_loss_bf16": round(held16, 6),
+                       "persistence_loss": round(persistence, 6),
+                       "explained_increment_variance": round(1 - held16 / persistence, 5),
+                       "surrogate_vs_solver_rms": sur.error_estimate}
+    # -- truth and observations
+    truth_ens = pkg.Ensemble(n, 1, "fp32", nu=1e-3, sigma=sigma, amp=amp, seed=SEED + 2)
+    truth_ens.init_grf()
+    truth0 = truth_ens.get_state()
+    truths, rows = [], []
+    for c in range(cycles):
+        truth_ens.step(dt, H)
+        w = truth_ens.get_state()[0]
+        truths.append(w)
+        ph = (np.fft.ifft2(w) * n * n).real
+        xs, ys = rng.integers(0, n, nobs), rng.integers(0, n, nobs)
+        vals = ph[ys, xs] + sd * rng.standard_normal(nobs)
+        rows += [((c + 1) * H, x, y, v, sd) for x, y, v in zip(xs, ys, vals)]
+    truth_ens.close()
+    rows = np.array(rows, dtype=np.float64)
+    pert = pkg.Ensemble(n, particles, "fp32", seed=SEED + 3)
+    pert.init_grf()
+    init = truth0 + 0.5 * pert.get_state()
+    pert.close()
+
+    def run(kind, retain):
+        with pkg.Ensemble(n, particles, "fp32", nu=1e-3, sigma=sigma, amp=amp,
+                          seed=SEED + 4) as pf:
+            pf.set_state(init)
+            pf.synchronize()
+            t = time.perf_counter()
+            if kind == "plain":
+                res = pkg.bootstrap_filter(pf, rows, dt, SEED, retain_states=retain)
+            elif kind == "surrogate":
+                res = pkg.surrogate_filter(pf, sur, rows, dt, SEED, "bf16", retain_states=retain)
+            else:
+                res = pkg.surrogate_filter(pf, sur, rows, dt, SEED, "bf16", retain_states=retain,
+                                           fallback_threshold=0.1)
+            pf.synchronize()
+            sec = time.perf_counter() - t
+        return res, sec
+
+    for kind in ("plain", "surrogate", "surrogate_fallback"):
+        run(kind, False)  # warm
+        _, sec = run(kind, False)
+        res, _ = run(kind, True)
+        err = [rms_error(h["mean_field"], tr) for h, tr in zip(res["history"], truths)]
+        out[kind] = {"cycles_per_s": round(cycles / sec, 2), "ms_per_cycle": round(sec / cycles * 1e3, 3),
+                     "analysis_rmse": round(float(np.mean(err)), 6),
+                     "log_likelihood": round(res["log_likelihood"], 3),
+                     "resampling_events": res["resampling_events"]}
+        if kind == "surrogate_fallback":
+            out[kind]["solver_cycles"] = int(sum(h["solver"] for h in res["history"]))
+    out["speedup"] = round(out["surrogate"]["cycles_per_s"] / out["plain"]["cycles_per_s"], 2)
+    out["rmse_ratio"] = round(out["surrogate"]["analysis_rmse"] / out["plain"]["analysis_rmse"], 4)
+    out["truth_rms"] = round(float(np.mean([np.sqrt(np.sum(np.abs(t) ** 2)) for t in truths])), 5)
+    sur.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +246,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-pf", action="store_true", help="skip the surrogate PF comparison")
+    ap.add_argument("--pf-only", action="store_true", help="only the surrogate PF comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -151,6 +261,9 @@ def main() -> int:
 
     import __graft_entry__ as ge
     pkg = ge.load_package()
+    if args.pf_only:
+        print(json.dumps({"surrogate_pf": surrogate_pf(pkg)}), flush=True)
+        return 0
     n, B = args.n, args.members
     nu = dt = 5e-4
     ens = pkg.Ensemble(n, B, "fp32", nu=nu, alpha=0.0, sigma=0.05, kmin=1, kmax=4, amp=0.1,
@@ -216,6 +329,8 @@ def main() -> int:
         "sanity": {"mean_enstrophy": float(diag[:, 1].mean()),
                    "max_cfl_rate": float(diag[:, 4].max())},
     }
+    if not args.no_pf:
+        line["surrogate_pf"] = surrogate_pf(pkg)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, nu, dt)
     print(json.dumps(line), flush=True)

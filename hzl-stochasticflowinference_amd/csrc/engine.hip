@@ -536,6 +536,12 @@ template <typename R> class Engine final : public EngineBase {
         step_ += steps;
     }
 
+    void rewind_counter(uint64_t steps) override {
+        if (steps > step_)
+            throw std::logic_error("step counter rewound past zero");
+        step_ -= steps;
+    }
+
   private:
     void member_check(size_t m) const {
         if (m >= B_)

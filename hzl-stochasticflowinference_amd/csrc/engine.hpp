@@ -30,6 +30,15 @@ struct DevCtx {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #endif
     }
+    // value of v held by lane (lane ^ mask) of the wavefront
+    __host__ __device__ float shfl_xor(float v, int mask, int) const {
+#if __HIP_DEVICE_COMPILE__
+        return __shfl_xor(v, mask);
+#else
+        (void)mask;
+        return v;
+#endif
+    }
     // host-only placeholders so host-side template checks of HD bodies that
     // emulate matrix instructions on the CPU compile; never called
     template <class A, class B, class C> void mfma_bf16(const A &, const B &, C &, int) const {}
@@ -73,6 +82,7 @@ class EngineBase {
     virtual EngineViews views() = 0;
     virtual void invalidate_mixed() = 0;
     virtual void advance_counter(uint64_t steps) = 0;
+    virtual void rewind_counter(uint64_t steps) = 0;
 };
 
 } // namespace hzl_amd

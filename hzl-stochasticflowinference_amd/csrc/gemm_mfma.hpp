@@ -13,8 +13,10 @@
 // wave's private LDS tile: operands whose k or m (n) index is contiguous are
 // read with 16-byte vector loads by consecutive lanes and written to LDS in
 // [row][k] order, which the fragment reads need.  Batches have two levels
-// (b1, b2) with independent strides.  Epilogue (FP32): alpha * acc
-// + beta * C_old + bias[n] + aux, optional pre-activation store, optional GELU.
+// (b1, b2) with independent strides.  An optional second product A2 B2^T
+// (its own strides and K2) accumulates into the same tile, which fuses two
+// layer terms into one pass.  Epilogue (FP32): alpha * acc + beta * C_old +
+// bias[n] + aux, optional pre-activation store, optional GELU.
 #pragma once
 #include "fft_lds.hpp"
 #include <cstdint>
@@ -43,6 +45,12 @@ template <typename T> struct GemmArgs {
     int aux_type;
     void *pre;          // pre-activation (C strides and type), or null
     int act;
+    // optional second product accumulated into the same tile (A2 null: none)
+    const T *A2;
+    long a2_m, a2_k, a2_b1, a2_b2;
+    const T *B2;
+    long b2_n, b2_k, b2_b1, b2_b2;
+    int K2;
 };
 
 HZL_HD float gelu(float x) {
@@ -140,31 +148,17 @@ HZL_HD void stage_operand(const T *base, long s_row, long s_k, int r0, int R, in
     }
 }
 
-// One wavefront: one 32x32 tile.  `wave` is the global wave index; `lds` the
-// wave's private LDS area (GemmTile<T>::LDS_PER_WAVE elements).
+// acc += A[m0..m0+31][0..K) B[n0..n0+31][0..K)^T through the wave's LDS tiles.
 template <typename T, class Ctx>
-HZL_HD void gemm_wave(const Ctx &ctx, const GemmArgs<T> &a, long wave, int lane, T *lds) {
+HZL_HD void mfma_accumulate(const Ctx &ctx, const T *A, long a_m, long a_k, int M, const T *B,
+                            long b_n, long b_k, int N, int K, int m0, int n0, T *lds, int lane,
+                            f32x16_t &acc) {
     using GT = GemmTile<T>;
     constexpr int KC = GT::KC, LD = GT::LD;
-    const int tm_count = (a.M + 31) / 32, tn_count = (a.N + 31) / 32;
-    const long tiles = (long)a.nb1 * a.nb2 * tm_count * tn_count;
-    if (wave >= tiles)
-        return;
-    const int tn = (int)(wave % tn_count);
-    const int tm = (int)((wave / tn_count) % tm_count);
-    const long b = wave / ((long)tn_count * tm_count);
-    const long b1 = b / a.nb2, b2 = b % a.nb2;
-    const T *A = a.A + b1 * a.a_b1 + b2 * a.a_b2;
-    const T *B = a.B + b1 * a.b_b1 + b2 * a.b_b2;
     T *sA = lds, *sB = lds + 32 * LD;
-    const int m0 = tm * 32, n0 = tn * 32;
-    f32x16_t acc;
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-        acc[i] = 0.f;
-    for (int k0 = 0; k0 < a.K; k0 += KC) {
-        stage_operand<T>(A, a.a_m, a.a_k, m0, a.M, k0, a.K, sA, lane);
-        stage_operand<T>(B, a.b_n, a.b_k, n0, a.N, k0, a.K, sB, lane);
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        stage_operand<T>(A, a_m, a_k, m0, M, k0, K, sA, lane);
+        stage_operand<T>(B, b_n, b_k, n0, N, k0, K, sB, lane);
         ctx.wsync();
         if constexpr (sizeof(T) == 2) {
 #pragma unroll
@@ -198,6 +192,32 @@ HZL_HD void gemm_wave(const Ctx &ctx, const GemmArgs<T> &a, long wave, int lane,
         }
         ctx.wsync();
     }
+}
+
+// One wavefront: one 32x32 tile.  `wave` is the global wave index; `lds` the
+// wave's private LDS area (GemmTile<T>::LDS_PER_WAVE elements).
+template <typename T, class Ctx>
+HZL_HD void gemm_wave(const Ctx &ctx, const GemmArgs<T> &a, long wave, int lane, T *lds) {
+    const int tm_count = (a.M + 31) / 32, tn_count = (a.N + 31) / 32;
+    const long tiles = (long)a.nb1 * a.nb2 * tm_count * tn_count;
+    if (wave >= tiles)
+        return;
+    const int tn = (int)(wave % tn_count);
+    const int tm = (int)((wave / tn_count) % tm_count);
+    const long b = wave / ((long)tn_count * tm_count);
+    const long b1 = b / a.nb2, b2 = b % a.nb2;
+    const int m0 = tm * 32, n0 = tn * 32;
+    f32x16_t acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        acc[i] = 0.f;
+    mfma_accumulate<T>(ctx, a.A + b1 * a.a_b1 + b2 * a.a_b2, a.a_m, a.a_k, a.M,
+                       a.B + b1 * a.b_b1 + b2 * a.b_b2, a.b_n, a.b_k, a.N, a.K, m0, n0, lds, lane,
+                       acc);
+    if (a.A2)
+        mfma_accumulate<T>(ctx, a.A2 + b1 * a.a2_b1 + b2 * a.a2_b2, a.a2_m, a.a2_k, a.M,
+                           a.B2 + b1 * a.b2_b1 + b2 * a.b2_b2, a.b2_n, a.b2_k, a.N, a.K2, m0, n0,
+                           lds, lane, acc);
     // epilogue
     char *C = reinterpret_cast<char *>(a.C);
     const int csize = a.c_type == OUT_BF16 ? 2 : 4;
@@ -225,6 +245,62 @@ HZL_HD void gemm_wave(const Ctx &ctx, const GemmArgs<T> &a, long wave, int lane,
             if (a.act == ACT_GELU)
                 v = gelu(v);
             store_any(Cb, ci, a.c_type, v);
+        }
+    }
+}
+
+// Projection and head of the surrogate in one pass (inference):
+//   out[m] = q2 + sum_j Q2[j] gelu(q1[j] + sum_c X[m][c] Q1[j][c])
+// for the 32 rows m0..m0+31 of one wavefront; X rows have C contiguous
+// channels, Q1 is [H][C] (H a multiple of 32, C <= 64).  The hidden
+// activations stay in registers: each 32-column block of them is reduced
+// into per-row partial sums, then summed across the 32 lanes of a half-wave.
+template <typename T> struct HeadArgs {
+    const T *X;       // [M][C]
+    const T *Q1;      // [H][C]
+    const float *q1;  // [H]
+    const float *Q2;  // [H]
+    const float *q2;  // [1]
+    float *out;       // [M]
+    long M;
+    int C, H;
+};
+
+template <typename T, class Ctx>
+HZL_HD void proj_head_wave(const Ctx &ctx, const HeadArgs<T> &a, long wave, int lane, T *lds) {
+    const long m0 = wave * 32;
+    if (m0 >= a.M)
+        return;
+    const int rows = a.M - m0 < 32 ? (int)(a.M - m0) : 32;
+    float rs[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        rs[i] = 0.f;
+    for (int n0 = 0; n0 < a.H; n0 += 32) {
+        f32x16_t acc;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            acc[i] = 0.f;
+        mfma_accumulate<T>(ctx, a.X + m0 * a.C, a.C, 1, rows, a.Q1, a.C, 1, a.H, a.C, 0, n0, lds,
+                           lane, acc);
+        const int n = n0 + lane % 32;
+        const float bias = a.q1[n], w = a.Q2[n];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            rs[i] += w * gelu(acc[i] + bias);
+    }
+    // sum over the 32 columns held by lanes with the same lane / 32
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        for (int off = 1; off < 32; off *= 2)
+            rs[i] += ctx.shfl_xor(rs[i], off, lane);
+    if (lane % 32 == 0) {
+        const float q2 = a.q2[0];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = 8 * (i / 4) + 4 * (lane / 32) + (i % 4);
+            if (r < rows)
+                a.out[m0 + r] = rs[i] + q2;
         }
     }
 }

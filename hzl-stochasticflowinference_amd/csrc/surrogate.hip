@@ -45,6 +45,14 @@ template <typename T> __global__ void __launch_bounds__(256) k_gemm(GemmArgs<T> 
     gemm_wave<T>(DevCtx{}, a, (long)blockIdx.x * 4 + wave, lane, lds);
 }
 
+// Fused projection + head (inference), 32 rows per wave.
+template <typename T> __global__ void __launch_bounds__(256) k_proj_head(HeadArgs<T> a) {
+    extern __shared__ unsigned char smem[];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    T *lds = reinterpret_cast<T *>(smem) + wave * GemmTile<T>::LDS_PER_WAVE;
+    proj_head_wave<T>(DevCtx{}, a, (long)blockIdx.x * 4 + wave, lane, lds);
+}
+
 #define HZL_GID ((long)blockIdx.x * 256 + threadIdx.x)
 
 template <typename T>
@@ -106,6 +114,9 @@ __global__ void k_gather_nu(const double *nu, const int *idx, double *out, int S
 }
 __global__ void k_dsum(const double *x, long n, int chunks, double *out) {
     dsum_body(x, n, chunks, out, HZL_GID);
+}
+__global__ void k_spread(const cx<float> *w0, int B, int K, double *out) {
+    spread_body(w0, B, K, out, HZL_GID);
 }
 __global__ void k_fill_f32(float *p, long n, float v) {
     const long i = HZL_GID;
@@ -180,7 +191,7 @@ template <typename T> Span<void> vs(Buf<T> &b, size_t off = 0) {
 
 template <typename T> struct Gemm {
     GemmArgs<T> a{};
-    size_t na = 0, nb = 0, nc = 0, nx = 0;
+    size_t na = 0, nb = 0, nc = 0, nx = 0, na2 = 0, nb2 = 0;
     Gemm(int M, int N, int K, Span<const T> A, long am, long ak, Span<const T> B, long bn,
          long bk, Span<void> C, long cm, long cn, int ctype) {
         a.M = M;
@@ -226,13 +237,32 @@ template <typename T> struct Gemm {
         a.x_b2 = xb2;
         return *this;
     }
+    Gemm &second(int K2, Span<const T> A2, long am, long ak, long ab1, long ab2, Span<const T> B2,
+                 long bn, long bk, long bb1, long bb2) {
+        a.K2 = K2;
+        a.A2 = A2.p;
+        a.a2_m = am;
+        a.a2_k = ak;
+        a.a2_b1 = ab1;
+        a.a2_b2 = ab2;
+        a.B2 = B2.p;
+        a.b2_n = bn;
+        a.b2_k = bk;
+        a.b2_b1 = bb1;
+        a.b2_b2 = bb2;
+        na2 = A2.n;
+        nb2 = B2.n;
+        return *this;
+    }
     // Largest element index an operand touches (all strides are >= 0).
     static size_t extent(long b1, long s1, long b2, long s2, long r, long sr, long k, long sk) {
         return (size_t)((b1 - 1) * s1 + (b2 - 1) * s2 + (r - 1) * sr + (k - 1) * sk);
     }
     void check() const {
-        const long strides[] = {a.a_m, a.a_k, a.a_b1, a.a_b2, a.b_n, a.b_k, a.b_b1, a.b_b2,
-                                a.c_m, a.c_n, a.c_b1, a.c_b2, a.x_m, a.x_n, a.x_b1, a.x_b2};
+        const long strides[] = {a.a_m,  a.a_k,  a.a_b1, a.a_b2, a.b_n,  a.b_k,  a.b_b1,
+                                a.b_b2, a.c_m,  a.c_n,  a.c_b1, a.c_b2, a.x_m,  a.x_n,
+                                a.x_b1, a.x_b2, a.a2_m, a.a2_k, a.a2_b1, a.a2_b2, a.b2_n,
+                                a.b2_k, a.b2_b1, a.b2_b2};
         for (long s : strides)
             if (s < 0)
                 throw std::logic_error("negative GEMM stride");
@@ -243,6 +273,10 @@ template <typename T> struct Gemm {
             extent(a.nb1, a.c_b1, a.nb2, a.c_b2, a.M, a.c_m, a.N, a.c_n) >= nc ||
             (a.aux && extent(a.nb1, a.x_b1, a.nb2, a.x_b2, a.M, a.x_m, a.N, a.x_n) >= nx))
             throw std::logic_error("GEMM operand out of bounds");
+        if (a.A2 && (a.K2 <= 0 || !a.B2 ||
+                     extent(a.nb1, a.a2_b1, a.nb2, a.a2_b2, a.M, a.a2_m, a.K2, a.a2_k) >= na2 ||
+                     extent(a.nb1, a.b2_b1, a.nb2, a.b2_b2, a.N, a.b2_n, a.K2, a.b2_k) >= nb2))
+            throw std::logic_error("second GEMM operand out of bounds");
     }
     void run(hipStream_t s) const {
         check();
@@ -366,12 +400,14 @@ template <typename T> struct FnoWork {
     bool built = false;
     FnoConstants<T> cst;
     Buf<T> feat, h, hpre, dh, dv0, dv1, g, dBi, dU, dV, dA, w_T, rcat;
-    std::vector<std::unique_ptr<Buf<T>>> v, pre, A, V, U, Bi, Z;
+    std::vector<std::unique_ptr<Buf<T>>> v, pre, A, V, U, Bi;
     Buf<float> out, dout;
     Buf<double> loss_part;
     uint64_t version = ~0ULL; // parameter version w_T / rcat were cast from
 
-    void ensure(const FnoShape &s, int S_, long nparams) {
+    int S_train = 0;
+    // Buffers for S samples; training (backward) buffers only when `train`.
+    void ensure(const FnoShape &s, int S_, long nparams, bool train) {
         if (!built) {
             cst.build(s);
             for (int l = 0; l <= s.L; ++l)
@@ -382,40 +418,43 @@ template <typename T> struct FnoWork {
                 V.emplace_back(new Buf<T>);
                 U.emplace_back(new Buf<T>);
                 Bi.emplace_back(new Buf<T>);
-                Z.emplace_back(new Buf<T>);
             }
             w_T.alloc(nparams);
             rcat.alloc((size_t)s.L * s.modes() * 4 * s.C * s.C);
             built = true;
         }
-        if (S_ <= S)
-            return;
-        S = S_;
-        const size_t SP = (size_t)S * s.P();
-        feat.alloc(SP * s.CIN);
-        for (auto &b : v)
-            b->alloc(SP * s.C);
-        for (int l = 0; l < s.L; ++l) {
-            pre[l]->alloc(SP * s.C);
-            A[l]->alloc(S * s.SA());
-            V[l]->alloc(S * s.SV());
-            U[l]->alloc(S * s.SV());
-            Bi[l]->alloc(S * s.SA());
-            Z[l]->alloc(SP * s.C);
+        if (S_ > S) {
+            S = S_;
+            const size_t SP = (size_t)S * s.P();
+            feat.alloc(SP * s.CIN);
+            for (auto &b : v)
+                b->alloc(SP * s.C);
+            for (int l = 0; l < s.L; ++l) {
+                A[l]->alloc(S * s.SA());
+                V[l]->alloc(S * s.SV());
+                U[l]->alloc(S * s.SV());
+                Bi[l]->alloc(S * s.SA());
+            }
+            out.alloc(SP);
+            dout.alloc(SP);
+            loss_part.alloc(SP);
         }
-        h.alloc(SP * s.H);
-        hpre.alloc(SP * s.H);
-        dh.alloc(SP * s.H);
-        dv0.alloc(SP * s.C);
-        dv1.alloc(SP * s.C);
-        g.alloc(SP * s.C);
-        dBi.alloc(S * s.SA());
-        dA.alloc(S * s.SA());
-        dU.alloc(S * s.SV());
-        dV.alloc(S * s.SV());
-        out.alloc(SP);
-        dout.alloc(SP);
-        loss_part.alloc(SP);
+        if (train && S_ > S_train) {
+            S_train = S_;
+            const size_t SP = (size_t)S_ * s.P();
+            for (int l = 0; l < s.L; ++l)
+                pre[l]->alloc(SP * s.C);
+            h.alloc(SP * s.H);
+            hpre.alloc(SP * s.H);
+            dh.alloc(SP * s.H);
+            dv0.alloc(SP * s.C);
+            dv1.alloc(SP * s.C);
+            g.alloc(SP * s.C);
+            dBi.alloc(S_ * s.SA());
+            dA.alloc(S_ * s.SA());
+            dU.alloc(S_ * s.SV());
+            dV.alloc(S_ * s.SV());
+        }
     }
 };
 
@@ -481,10 +520,10 @@ class Surrogate {
         upload_fields(S, fields, hx_);
         upload_nu(S, nu);
         if (precision == HZL_BF16) {
-            forward(wb_, hx_.p, hnu_.p, (int)S);
+            forward(wb_, hx_.p, hnu_.p, (int)S, false);
             download_out(wb_, S, grid);
         } else {
-            forward(wf_, hx_.p, hnu_.p, (int)S);
+            forward(wf_, hx_.p, hnu_.p, (int)S, false);
             download_out(wf_, S, grid);
         }
     }
@@ -623,7 +662,68 @@ class Surrogate {
             sum += l * (double)S;
             done += S;
         }
-        return sum / (double)count;
+        const double loss = sum / (double)count;
+        // grid RMS error of the predicted increment, in vorticity units
+        error_rms_ = std::sqrt(loss) / out_scale_;
+        return loss;
+    }
+
+    double error_estimate() const {
+        return error_rms_;
+    }
+    void set_error_estimate(double e) {
+        if (!(e >= 0) || !std::isfinite(e))
+            throw std::invalid_argument("error estimate must be finite and nonnegative");
+        error_rms_ = e;
+    }
+
+    // Grid RMS deviation of the ensemble members from their mean.
+    double spread(EngineBase &eng) {
+        const EngineViews ev = check_engine(eng);
+        const long spec = s_.spec();
+        spread_.alloc((size_t)spec);
+        hipLaunchKernelGGL(k_spread, grid1d(spec), dim3(256), 0, ev.stream,
+                           reinterpret_cast<const cx<float> *>(ev.w0), ev.B, s_.K, spread_.p);
+        std::vector<double> h((size_t)spec);
+        hcheck(hipMemcpyAsync(h.data(), spread_.p, spec * sizeof(double), hipMemcpyDeviceToHost,
+                              ev.stream),
+               "spread download");
+        hcheck(hipStreamSynchronize(ev.stream), "spread");
+        double sum = 0;
+        for (double v : h)
+            sum += v;
+        return std::sqrt(sum);
+    }
+
+    // Forecast with the network, then compare its error estimate with the
+    // spread of the forecast ensemble: when the error exceeds threshold x
+    // spread, the states are restored and the plain solver advances the
+    // ensemble instead (its own forcing, same noise counters as a plain step).
+    // Returns 1 when the solver ran.
+    int forecast_or_solve(EngineBase &eng, double dt, size_t steps, int precision,
+                          double threshold, double *spread_out) {
+        if (!(threshold > 0) || !std::isfinite(threshold))
+            throw std::invalid_argument("fallback threshold must be positive");
+        if (!(error_rms_ >= 0))
+            throw std::invalid_argument(
+                "no error estimate: evaluate the surrogate or set one first");
+        const EngineViews ev = check_engine(eng);
+        const size_t bytes = (size_t)ev.B * s_.spec() * sizeof(cx<float>);
+        backup_.alloc((size_t)ev.B * s_.spec());
+        hcheck(hipMemcpyAsync(backup_.p, ev.w0, bytes, hipMemcpyDeviceToDevice, ev.stream),
+               "state backup");
+        forecast(eng, dt, steps, precision);
+        const double sp = spread(eng);
+        if (spread_out)
+            *spread_out = sp;
+        if (error_rms_ <= threshold * sp)
+            return 0;
+        hcheck(hipMemcpyAsync(ev.w0, backup_.p, bytes, hipMemcpyDeviceToDevice, ev.stream),
+               "state restore");
+        eng.rewind_counter(steps);
+        eng.invalidate_mixed();
+        eng.step(dt, steps);
+        return 1;
     }
 
     // Advance every member of an FP32 ensemble by steps * dt with the network.
@@ -634,28 +734,30 @@ class Surrogate {
             throw std::invalid_argument("forecast needs dt > 0 and steps > 0");
         if (horizon_ && (steps != horizon_ || std::fabs(dt - dt_) > 1e-12 * dt_))
             throw std::invalid_argument("forecast horizon differs from the trained horizon");
-        // run on the engine's stream, in order with its own work
+        // run on the engine's stream, in order with its own work; the noise
+        // counter is the ensemble's step index, so a forecast is a function of
+        // the ensemble state alone
         st_ = ev.stream;
+        const uint64_t counter = eng.step_index();
         const int chunk = std::min(ev.B, 1024);
         for (int s0 = 0; s0 < ev.B; s0 += chunk) {
             const int S = std::min(chunk, ev.B - s0);
             cx<float> *w0 = reinterpret_cast<cx<float> *>(ev.w0) + (long)s0 * s_.spec();
             if (precision == HZL_BF16) {
-                forward(wb_, w0, ev.nu + s0, S);
+                forward(wb_, w0, ev.nu + s0, S, false);
                 grid_to_increment(wb_.out.p, S);
             } else {
-                forward(wf_, w0, ev.nu + s0, S);
+                forward(wf_, w0, ev.nu + s0, S, false);
                 grid_to_increment(wf_.out.p, S);
             }
             const long cnt = (long)S * s_.spec();
             hipLaunchKernelGGL(k_apply_increment, grid1d(cnt), dim3(256), 0, st_, w0, incr_.p, S,
                                s_.K, s_.KS, (float)(1.0 / out_scale_), ev.nu + s0, ev.alpha + s0,
                                ev.sigma + s0, ev.keys + s0, ev.pair_of, ev.kmin, ev.kmax,
-                               (double)steps * dt, counter_);
+                               (double)steps * dt, counter);
         }
         hcheck(hipGetLastError(), "surrogate forecast");
         st_ = own_;
-        ++counter_;
         eng.invalidate_mixed();
         eng.advance_counter(steps);
     }
@@ -825,9 +927,10 @@ class Surrogate {
     }
 
     // Network output for S FP32 half-plane states into w.out (S x G x G).
+    // With `train` the pre-activations and hidden layer are kept for backward.
     template <typename T>
-    void forward(FnoWork<T> &w, const cx<float> *states, const double *nu, int S) {
-        w.ensure(s_, S, prm_.total);
+    void forward(FnoWork<T> &w, const cx<float> *states, const double *nu, int S, bool train) {
+        w.ensure(s_, S, prm_.total, train);
         prepare_weights(w);
         const FnoShape &s = s_;
         const int G = s.G, C = s.C, KX = s.KX, KY = s.KY, TY = type_of<T>();
@@ -865,30 +968,40 @@ class Surrogate {
                        C, 1, TY);
             iy.batch(S, 0, SV, SA, KX, 0, (long)KY * 2 * C, 2L * G * C);
             iy.run(st_);
-            // pointwise: Z = v W^T + b
-            Gemm<T> pw((int)SP, C, C, cs(vl), C, 1, cs(w.w_T, prm_.W[l]), C, 1, vs(*w.Z[l]), C, 1,
-                       TY);
-            pw.a.bias = master_.p + prm_.b[l];
-            pw.run(st_);
-            // inverse x + Z, activation: v'[s][y][x][c]
+            // inverse x plus the pointwise term v W^T + b, activation:
+            //   v'[s][y][x][c] = act(sum_k Gx[x][k] Bi[s][k][y][c] + sum_c' v[s][y][x][c'] W[c][c'] + b[c])
             Gemm<T> ix(G, C, 2 * KX, cs(w.cst.Gx), 2 * KX, 1, cs(*w.Bi[l]), 1, (long)G * C,
                        vs(*w.v[l + 1]), C, 1, TY);
             ix.batch(S, 0, SA, P * C, G, 0, C, (long)G * C);
-            ix.aux(cs(*w.Z[l]), C, 1, P * C, (long)G * C);
+            ix.second(C, cs(vl), C, 1, P * C, (long)G * C, cs(w.w_T, prm_.W[l]), C, 1, 0, 0);
+            ix.a.bias = master_.p + prm_.b[l];
             if (l + 1 < s.L) {
-                ix.a.pre = w.pre[l]->p;
+                if (train)
+                    ix.a.pre = w.pre[l]->p;
                 ix.a.act = ACT_GELU;
             }
             ix.run(st_);
         }
-        Gemm<T> q((int)SP, s.H, C, cs(*w.v[s.L]), C, 1, cs(w.w_T, prm_.Q1), C, 1, vs(w.h), s.H, 1,
-                  TY);
-        q.a.bias = master_.p + prm_.q1;
-        q.a.pre = w.hpre.p;
-        q.a.act = ACT_GELU;
-        q.run(st_);
-        hipLaunchKernelGGL(k_head<T>, grid1d(SP), dim3(256), 0, st_, w.h.p, master_.p + prm_.Q2,
-                           master_.p + prm_.q2, w.out.p, SP, s.H);
+        if (train) {
+            Gemm<T> q((int)SP, s.H, C, cs(*w.v[s.L]), C, 1, cs(w.w_T, prm_.Q1), C, 1, vs(w.h), s.H,
+                      1, TY);
+            q.a.bias = master_.p + prm_.q1;
+            q.a.pre = w.hpre.p;
+            q.a.act = ACT_GELU;
+            q.run(st_);
+            hipLaunchKernelGGL(k_head<T>, grid1d(SP), dim3(256), 0, st_, w.h.p,
+                               master_.p + prm_.Q2, master_.p + prm_.q2, w.out.p, SP, s.H);
+        } else {
+            if (w.v[s.L]->n < (size_t)SP * C || w.w_T.n < (size_t)(prm_.Q1 + s.H * C) ||
+                w.out.n < (size_t)SP || s.H % 32 != 0 || C > 64)
+                throw std::logic_error("projection operands out of bounds");
+            HeadArgs<T> ha{w.v[s.L]->p,         w.w_T.p + prm_.Q1, master_.p + prm_.q1,
+                           master_.p + prm_.Q2, master_.p + prm_.q2, w.out.p,
+                           SP,                  C,                 s.H};
+            const long waves = (SP + 31) / 32;
+            hipLaunchKernelGGL(k_proj_head<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256),
+                               4 * GemmTile<T>::LDS_PER_WAVE * sizeof(T), st_, ha);
+        }
     }
 
     // Mean squared error against the scaled true grid increment; with
@@ -896,7 +1009,7 @@ class Surrogate {
     template <typename T>
     double loss_and_grad(FnoWork<T> &w, const cx<float> *x, const cx<float> *y, const double *nu,
                          int S, bool want_grad) {
-        forward(w, x, nu, S);
+        forward(w, x, nu, S, want_grad);
         const long n = (long)S * s_.spec(), SP = (long)S * s_.P();
         diff_.alloc((size_t)n);
         hipLaunchKernelGGL(k_diff_states, grid1d(n), dim3(256), 0, st_, y, x, diff_.p, n);
@@ -1149,8 +1262,9 @@ class Surrogate {
     hipStream_t own_ = nullptr, st_ = nullptr;
     Buf<float> master_, grad_, m_, v_;
     Buf<float> g1_, g2_, f1_, f2_, bm_, dgrid_, incr_, target_, part_, rgrad_;
-    Buf<cx<float>> diff_, hx_, hy_, ds_x_, ds_y_, bx_, by_;
-    Buf<double> hnu_, ds_nu_, bnu_, lsum_;
+    Buf<cx<float>> diff_, hx_, hy_, ds_x_, ds_y_, bx_, by_, backup_;
+    Buf<double> hnu_, ds_nu_, bnu_, lsum_, spread_;
+    double error_rms_ = -1;
     Buf<int> bidx_;
     FnoWork<float> wf_;
     FnoWork<__bf16> wb_;
@@ -1160,7 +1274,6 @@ class Surrogate {
     uint64_t version_ = 0;
     double in_scale_ = 1.0, out_scale_ = 1.0;
     bool scales_set_ = false;
-    uint64_t counter_ = 0;
 };
 
 } // namespace hzl_amd
@@ -1286,6 +1399,35 @@ int hzl_surrogate_evaluate(hzl_surrogate_t h, int precision, size_t first, size_
     return guarded([&] {
         need(loss);
         *loss = net_of(h).evaluate(precision, first, count);
+    });
+}
+
+int hzl_surrogate_error_estimate(hzl_surrogate_t h, double *rms) {
+    return guarded([&] {
+        need(rms);
+        *rms = net_of(h).error_estimate();
+    });
+}
+
+int hzl_surrogate_set_error_estimate(hzl_surrogate_t h, double rms) {
+    return guarded([&] { net_of(h).set_error_estimate(rms); });
+}
+
+int hzl_surrogate_spread(hzl_surrogate_t h, hzl_ensemble_t ensemble, double *rms) {
+    return guarded([&] {
+        need(rms);
+        *rms = net_of(h).spread(hzl_amd::engine_of(ensemble));
+    });
+}
+
+int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt,
+                                    size_t steps, int precision, double threshold,
+                                    int *used_solver, double *spread) {
+    return guarded([&] {
+        const int r = net_of(h).forecast_or_solve(hzl_amd::engine_of(ensemble), dt, steps,
+                                                  precision, threshold, spread);
+        if (used_solver)
+            *used_solver = r;
     });
 }
 

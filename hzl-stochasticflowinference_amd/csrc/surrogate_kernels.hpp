@@ -206,6 +206,30 @@ HZL_HD void apply_increment_body(cx<float> *w0, const float *incr, int S, int K,
     w.y += im;
 }
 
+// Ensemble spread of one retained mode e: (kx > 0 ? 2 : 1) x the member
+// variance of w_e, so that the sum over modes is the grid mean square
+// deviation from the ensemble mean (Parseval).
+HZL_HD void spread_body(const cx<float> *w0, int B, int K, double *out, long e) {
+    const int J = 2 * K + 1;
+    const long spec = (long)(K + 1) * J;
+    if (e >= spec)
+        return;
+    double mr = 0, mi = 0;
+    for (int m = 0; m < B; ++m) {
+        mr += w0[m * spec + e].x;
+        mi += w0[m * spec + e].y;
+    }
+    mr /= B;
+    mi /= B;
+    double s = 0;
+    for (int m = 0; m < B; ++m) {
+        const double dr = w0[m * spec + e].x - mr, di = w0[m * spec + e].y - mi;
+        s += dr * dr + di * di;
+    }
+    const int c = (int)(e / J);
+    out[e] = (c > 0 ? 2.0 : 1.0) * s / B;
+}
+
 // Weighted column sums: out[c][n] = sum over rows m of chunk c of w[m] x[m][n].
 template <typename T>
 HZL_HD void wcolsum_body(const T *x, const float *w, long M, int N, long ld, int chunks,

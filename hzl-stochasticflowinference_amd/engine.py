@@ -33,7 +33,9 @@ EXPORTS = (
     "hzl_surrogate_get_params", "hzl_surrogate_set_params", "hzl_surrogate_set_scales",
     "hzl_surrogate_get_scales", "hzl_surrogate_forward", "hzl_surrogate_loss_grad",
     "hzl_surrogate_collect", "hzl_surrogate_dataset_size", "hzl_surrogate_fit",
-    "hzl_surrogate_evaluate", "hzl_surrogate_forecast",
+    "hzl_surrogate_evaluate", "hzl_surrogate_error_estimate",
+    "hzl_surrogate_set_error_estimate", "hzl_surrogate_spread",
+    "hzl_surrogate_forecast_or_solve", "hzl_surrogate_forecast",
 )
 
 
@@ -105,6 +107,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_surrogate_fit.argtypes = [vp, sz, sz, sz, sz, C.c_double, C.c_uint64, _dp]
     L.hzl_surrogate_evaluate.argtypes = [vp, C.c_int, sz, sz, _dp]
     L.hzl_surrogate_forecast.argtypes = [vp, vp, C.c_double, sz, C.c_int]
+    L.hzl_surrogate_error_estimate.argtypes = [vp, _dp]
+    L.hzl_surrogate_set_error_estimate.argtypes = [vp, C.c_double]
+    L.hzl_surrogate_spread.argtypes = [vp, vp, _dp]
+    L.hzl_surrogate_forecast_or_solve.argtypes = [vp, vp, C.c_double, sz, C.c_int, C.c_double,
+                                                  C.POINTER(C.c_int), _dp]
     for name in EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -393,12 +400,41 @@ class Surrogate:
         _check(_lib.hzl_surrogate_forecast(self._h, ensemble._h, dt, steps,
                                            _SUR_PRECISION[precision]))
 
+    @property
+    def error_estimate(self) -> float:
+        v = C.c_double()
+        _check(_lib.hzl_surrogate_error_estimate(self._h, C.byref(v)))
+        return v.value
+
+    @error_estimate.setter
+    def error_estimate(self, rms: float):
+        _check(_lib.hzl_surrogate_set_error_estimate(self._h, rms))
+
+    def spread(self, ensemble: Ensemble) -> float:
+        v = C.c_double()
+        _check(_lib.hzl_surrogate_spread(self._h, ensemble._h, C.byref(v)))
+        return v.value
+
+    def forecast_or_solve(self, ensemble: Ensemble, dt: float, steps: int, threshold: float,
+                          precision: str = "bf16"):
+        """Surrogate forecast, or the plain solver when error > threshold x spread.
+        Returns (used_solver, spread)."""
+        used, sp = C.c_int(), C.c_double()
+        _check(_lib.hzl_surrogate_forecast_or_solve(self._h, ensemble._h, dt, steps,
+                                                    _SUR_PRECISION[precision], threshold,
+                                                    C.byref(used), C.byref(sp)))
+        return bool(used.value), sp.value
+
 
 def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt: float,
-                     seed: int, precision: str = "bf16", retain_states: bool = True):
+                     seed: int, precision: str = "bf16", retain_states: bool = True,
+                     fallback_threshold: float | None = None):
     """Bootstrap particle filter whose forecast between observation groups is
     one surrogate evaluation (every group interval must equal the surrogate's
-    horizon); weighting and resampling as in bootstrap_filter."""
+    horizon); weighting and resampling as in bootstrap_filter.  With
+    fallback_threshold, a group whose surrogate error estimate exceeds
+    threshold x the ensemble spread is forecast by the plain solver instead
+    (history records `solver` and `spread`)."""
     obs = np.asarray(observations, dtype=np.float64)
     if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
         raise ValueError("observations must be rows of (step, x, y, value, sd)")
@@ -410,11 +446,17 @@ def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt
     history, current = [], 0
     for gi, target in enumerate(groups):
         sel = obs[steps == target]
-        surrogate.forecast(particles, dt, int(target - current), precision)
+        solver, spread = False, None
+        if fallback_threshold is None:
+            surrogate.forecast(particles, dt, int(target - current), precision)
+        else:
+            solver, spread = surrogate.forecast_or_solve(particles, dt, int(target - current),
+                                                         fallback_threshold, precision)
         rec, mean = particles.pf_assimilate(dt, 0, sel[:, 1], sel[:, 2], sel[:, 3], sel[:, 4],
                                             gi == len(groups) - 1, mean=retain_states)
         history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
-                            max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean))
+                            max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean,
+                            solver=solver, spread=spread))
         current = int(target)
         if not np.isfinite(rec[0]):
             break

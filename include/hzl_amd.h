@@ -228,9 +228,27 @@ int hzl_surrogate_dataset_size(hzl_surrogate_t h, size_t *pairs);
 int hzl_surrogate_fit(hzl_surrogate_t h, size_t first, size_t count, size_t epochs, size_t batch,
                       double learning_rate, uint64_t seed, double *epoch_loss);
 
-/* Mean loss over pairs [first, first + count), forward only. */
+/* Mean loss over pairs [first, first + count), forward only.  Also sets the
+ * error estimate: the grid RMS error of the predicted increment against the
+ * solver's, sqrt(loss) / output_scale, in vorticity units. */
 int hzl_surrogate_evaluate(hzl_surrogate_t h, int precision, size_t first, size_t count,
                            double *loss);
+
+/* The error estimate (negative until evaluated or set). */
+int hzl_surrogate_error_estimate(hzl_surrogate_t h, double *rms);
+int hzl_surrogate_set_error_estimate(hzl_surrogate_t h, double rms);
+
+/* Grid RMS deviation of an FP32 ensemble's members from their mean. */
+int hzl_surrogate_spread(hzl_surrogate_t h, hzl_ensemble_t ensemble, double *rms);
+
+/* Surrogate forecast with plain-solver fallback: after the network forecast,
+ * when the error estimate exceeds threshold x the spread of the forecast
+ * ensemble, the states are restored and the engine advances the ensemble
+ * `steps` RK4 steps with its own forcing instead (used_solver = 1; identical
+ * to hzl_ensemble_step).  spread receives the forecast ensemble's spread. */
+int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt,
+                                    size_t steps, int precision, double threshold,
+                                    int *used_solver, double *spread);
 
 /* Advance every member of an FP32 ensemble by steps * dt with the network
  * (steps and dt must equal the trained horizon).  The ensemble's step counter

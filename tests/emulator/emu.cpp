@@ -58,6 +58,14 @@ struct CpuCtx {
         }
         bar->arrive_and_wait();
     }
+    // value of v held by lane (l ^ mask)
+    float shfl_xor(float v, int mask, int l) const {
+        mx->a[l][0] = v;
+        bar->arrive_and_wait();
+        const float r = mx->a[l ^ mask][0];
+        bar->arrive_and_wait();
+        return r;
+    }
     // v_mfma_f32_32x32x2_f32: lane l supplies A[l%32][l/32] and B[l/32][l%32].
     void mfma_f32(float fa, float fb, f32x16_t &acc, int l) const {
         mx->a[l][0] = fa;
@@ -372,9 +380,32 @@ int gemm_any(const int *dims, const long *st, const float *A, size_t na, const f
     return 0;
 }
 
+template <typename T>
+int proj_head_any(long M, int C, int H, const float *X, const float *Q1, const float *q1,
+                  const float *Q2, const float *q2, float *out) {
+    std::vector<T> x((size_t)M * C), q((size_t)H * C);
+    for (size_t i = 0; i < x.size(); ++i)
+        x[i] = (T)X[i];
+    for (size_t i = 0; i < q.size(); ++i)
+        q[i] = (T)Q1[i];
+    HeadArgs<T> a{x.data(), q.data(), q1, Q2, q2, out, M, C, H};
+    launch((M + 31) / 32, 64, GemmTile<T>::LDS_PER_WAVE * sizeof(T),
+           [&](const CpuCtx &ctx, long wave, unsigned char *lds) {
+               proj_head_wave<T>(ctx, a, wave, ctx.lane, reinterpret_cast<T *>(lds));
+           });
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
+
+int emu_proj_head(int bf16, long M, int C, int H, const float *X, const float *Q1,
+                  const float *q1, const float *Q2, const float *q2, float *out) {
+    if (bf16)
+        return proj_head_any<__bf16>(M, C, H, X, Q1, q1, Q2, q2, out);
+    return proj_head_any<float>(M, C, H, X, Q1, q1, Q2, q2, out);
+}
 
 int emu_gemm(int bf16, const int *dims, const long *strides, const float *A, size_t na,
              const float *B, size_t nb, float *C, size_t nc, const float *bias, const float *aux,

@@ -25,6 +25,9 @@ class Emulator:
         L.emu_gemm.argtypes = [C.c_int, _ip, _lp, _fp, C.c_size_t, _fp, C.c_size_t, _fp,
                                C.c_size_t, _fp, _fp, C.c_size_t, _fp, C.c_float, C.c_float,
                                C.c_int, C.c_int]
+        L.emu_proj_head.restype = C.c_int
+        L.emu_proj_head.argtypes = [C.c_int, C.c_long, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp,
+                                    _fp]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
 
@@ -48,6 +51,20 @@ class Emulator:
                                int(c_bf16))
         assert rc == 0
         return out, pre
+
+    def proj_head(self, bf16, X, Q1, q1, Q2, q2):
+        """proj_head_wave on the CPU lanes: X [M][C], Q1 [H][C]; returns out [M]."""
+        fp = C.POINTER(C.c_float)
+        f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+        X, Q1, q1, Q2, q2 = f(X), f(Q1), f(q1), f(Q2), f(np.atleast_1d(q2))
+        M, Cc = X.shape
+        out = np.zeros(M, dtype=np.float32)
+        rc = self.lib.emu_proj_head(int(bf16), M, Cc, Q1.shape[0], X.ctypes.data_as(fp),
+                                    Q1.ctypes.data_as(fp), q1.ctypes.data_as(fp),
+                                    Q2.ctypes.data_as(fp), q2.ctypes.data_as(fp),
+                                    out.ctypes.data_as(fp))
+        assert rc == 0
+        return out
 
     def fft_lines(self, x, inverse=False, precision="fp32"):
         d = np.ascontiguousarray(x, dtype=np.complex128).copy()

@@ -108,3 +108,22 @@ def test_gemm_wave_matches_numpy(emu, case, bf):
     mask = np.ones(nc, dtype=bool)
     mask[c_idx.ravel()] = False
     np.testing.assert_array_equal(got[mask], (bf16(C0) if c_bf16 else C0)[mask])
+
+
+@pytest.mark.parametrize("bf", [False, True], ids=["f32", "bf16"])
+@pytest.mark.parametrize("M,C", [(96, 32), (45, 64), (32, 8)])
+def test_fused_projection_head_matches_numpy(emu, bf, M, C):
+    """out[m] = q2 + sum_j Q2[j] gelu(q1[j] + X[m] . Q1[j]) with the hidden layer
+    kept in registers and summed across lanes (ragged last wave included)."""
+    H = 128
+    rng = np.random.default_rng(M * 100 + C)
+    X = rng.standard_normal((M, C)).astype(np.float32)
+    Q1 = (rng.standard_normal((H, C)) / np.sqrt(C)).astype(np.float32)
+    q1 = rng.standard_normal(H).astype(np.float32)
+    Q2 = (rng.standard_normal(H) / np.sqrt(H)).astype(np.float32)
+    q2 = np.float32(0.25)
+    got = emu.proj_head(bf, X, Q1, q1, Q2, q2)
+    r = bf16 if bf else (lambda x: x)
+    hid = r(X).astype(np.float64) @ r(Q1).astype(np.float64).T + q1
+    want = gelu(hid) @ Q2 + q2
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5 * np.sqrt(H))

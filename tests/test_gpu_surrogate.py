@@ -183,3 +183,36 @@ def test_surrogate_filter_runs_and_tracks(pkg, orc):
     sur.close()
     assert np.isfinite(res["log_likelihood"])
     assert len(res["history"]) == 4
+
+
+def test_spread_and_solver_fallback(pkg, orc):
+    """The spread is the grid RMS deviation from the ensemble mean; the
+    fallback runs the plain solver (bit-identical to stepping the engine)
+    when the error estimate exceeds threshold x spread, else the network."""
+    cfg = SMALL
+    s = shape_of(cfg)
+    B, dt, steps = 16, 2e-3, 25
+    f = grf_fields(orc, s.n, B, 51)
+    with pkg.Surrogate(**cfg, seed=9) as sur, \
+            pkg.Ensemble(s.n, B, "fp32", sigma=0.05, seed=6) as a, \
+            pkg.Ensemble(s.n, B, "fp32", sigma=0.05, seed=6) as b:
+        sur.set_scales(1.0, 1.0)
+        a.set_state(f)
+        b.set_state(f)
+        x = a.get_state()
+        phys = np.stack([orc.fft2d(m, inverse=True).real for m in x])
+        want = np.sqrt(np.mean((phys - phys.mean(0)) ** 2) * 1.0)
+        assert abs(sur.spread(a) - want) < 1e-5 * want
+        with pytest.raises(pkg.HzlInvalidArgument):
+            sur.forecast_or_solve(a, dt, steps, 0.5)  # no error estimate yet
+        sur.error_estimate = 1e3
+        used, sp = sur.forecast_or_solve(a, dt, steps, 0.5)
+        b.step(dt, steps)
+        assert used and sp > 0
+        np.testing.assert_array_equal(a.get_state(), b.get_state())
+        assert a.step_index == steps
+        sur.error_estimate = 0.0
+        used, sp = sur.forecast_or_solve(a, dt, steps, 0.5, precision="fp32")
+        assert not used
+        assert abs(sp - sur.spread(a)) < 1e-12 * sp
+        assert a.step_index == 2 * steps
