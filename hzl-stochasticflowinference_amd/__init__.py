@@ -6,8 +6,9 @@ loaded with `load_package()` from __graft_entry__.py (or importlib) under the
 module name `hzl_sfi_amd`.
 """
 from .engine import (ALL, BF16, EXPORTS, FP32, FP64, Ensemble, HzlError, HzlInvalidArgument,
-                     Parameters, Surrogate, bootstrap_filter, load_library, surrogate_filter)
+                     Parameters, Surrogate, bootstrap_filter, ensemble_kalman_filter,
+                     load_library, surrogate_filter)
 
 __all__ = ["ALL", "BF16", "EXPORTS", "FP32", "FP64", "Ensemble", "HzlError",
-           "HzlInvalidArgument", "Parameters", "Surrogate", "bootstrap_filter", "load_library",
-           "surrogate_filter"]
+           "HzlInvalidArgument", "Parameters", "Surrogate", "bootstrap_filter",
+           "ensemble_kalman_filter", "load_library", "surrogate_filter"]

@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <hip/hip_runtime.h>
+#include <memory>
 
 namespace hzl_amd {
 
@@ -57,9 +58,15 @@ struct EngineViews {
     hipStream_t stream;
 };
 
+// State that another module keeps per ensemble handle (owned by the handle).
+struct ModuleState {
+    virtual ~ModuleState() = default;
+};
+
 class EngineBase {
   public:
     virtual ~EngineBase() = default;
+    std::unique_ptr<ModuleState> enkf; // ensemble Kalman filter state, if started
     virtual void set_member_parameters(size_t m, double nu, double alpha, double sigma,
                                        double amp) = 0;
     virtual void set_state(size_t m, const double *field) = 0;

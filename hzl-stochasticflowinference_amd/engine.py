@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG, "libhzl_amd.so")
 
 HZL_OK, HZL_EINVAL, HZL_ERUNTIME = 0, 1, 2
 FP32, FP64, BF16 = 0, 1, 2
+ENKF_SERIAL, ENKF_BATCH = 0, 1
 ALL = C.c_size_t(-1).value  # (size_t)-1: every member
 
 # Every entry point declared in include/hzl_amd.h.
@@ -29,6 +30,7 @@ EXPORTS = (
     "hzl_ensemble_step_index",
     "hzl_ensemble_device_state", "hzl_ensemble_stream",
     "hzl_pf_begin", "hzl_pf_assimilate", "hzl_pf_result",
+    "hzl_enkf_begin", "hzl_enkf_assimilate", "hzl_ensemble_mean_spread",
     "hzl_surrogate_create", "hzl_surrogate_destroy", "hzl_surrogate_param_count",
     "hzl_surrogate_get_params", "hzl_surrogate_set_params", "hzl_surrogate_set_scales",
     "hzl_surrogate_get_scales", "hzl_surrogate_forward", "hzl_surrogate_loss_grad",
@@ -91,6 +93,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_pf_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp, _dp,
                                     C.c_int, _dp, _dp]
     L.hzl_pf_result.argtypes = [vp, _dp, C.POINTER(C.c_size_t)]
+    L.hzl_enkf_begin.argtypes = [vp, C.c_uint64, C.c_int]
+    L.hzl_enkf_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp, _dp,
+                                      C.c_double, C.c_double, _dp, _dp]
+    L.hzl_ensemble_mean_spread.argtypes = [vp, _dp, _dp]
     fp, sz = C.POINTER(C.c_float), C.c_size_t
     L.hzl_surrogate_create.argtypes = [C.POINTER(vp), sz, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_uint64]
@@ -247,6 +253,34 @@ class Ensemble:
                                       mf.ctypes.data_as(_dp) if mean else None))
         return rec, mf
 
+    # -- ensemble Kalman filter (members are the ensemble) ----------------
+    def enkf_begin(self, seed: int, mode: str = "serial"):
+        _check(_lib.hzl_enkf_begin(self._h, seed, {"serial": ENKF_SERIAL,
+                                                   "batch": ENKF_BATCH}[mode]))
+
+    def enkf_assimilate(self, dt, steps, x, y, value, sd, inflation=1.0, radius=0.0,
+                        mean=False):
+        x = np.ascontiguousarray(x, dtype=np.int32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        value = np.ascontiguousarray(value, dtype=np.float64)
+        sd = np.ascontiguousarray(sd, dtype=np.float64)
+        rec = np.zeros(2)
+        mf = np.zeros((self.n, self.n), dtype=np.complex128) if mean else None
+        ip = C.POINTER(C.c_int)
+        _check(_lib.hzl_enkf_assimilate(self._h, dt, steps, len(x), x.ctypes.data_as(ip),
+                                        y.ctypes.data_as(ip), value.ctypes.data_as(_dp),
+                                        sd.ctypes.data_as(_dp), inflation, radius,
+                                        rec.ctypes.data_as(_dp),
+                                        mf.ctypes.data_as(_dp) if mean else None))
+        return rec, mf
+
+    def mean_spread(self, mean=True):
+        sp = C.c_double()
+        mf = np.zeros((self.n, self.n), dtype=np.complex128) if mean else None
+        _check(_lib.hzl_ensemble_mean_spread(self._h, C.byref(sp),
+                                             mf.ctypes.data_as(_dp) if mean else None))
+        return sp.value, mf
+
     def pf_result(self):
         ll = C.c_double()
         ev = C.c_size_t()
@@ -284,6 +318,47 @@ def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,
             break
     ll, events = particles.pf_result()
     return dict(log_likelihood=ll, resampling_events=events, history=history)
+
+
+def ensemble_kalman_filter(members: Ensemble, observations, dt: float, seed: int,
+                           mode: str = "serial", inflation: float = 1.0,
+                           localization_radius: float = 0.0, forecast_steps: int = 0,
+                           forecast_every: int = 1, retain_states: bool = True):
+    """Perturbed-observation EnKF over an initialised ensemble.
+
+    Mirrors hzl::ensemble_kalman_filter (src/filter.cpp:204-356): rows
+    (step, x, y, value, sd) grouped by step; per group a record dict(step,
+    forecast_sensor_rmse, posterior_spread, mean_field, is_forecast), then
+    `forecast_steps` free steps recorded every `forecast_every` (and the last).
+    mode "serial" follows the reference's scalar order; "batch" assimilates
+    each group at once on MFMA (FP32 ensembles).
+    """
+    obs = np.asarray(observations, dtype=np.float64)
+    if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
+        raise ValueError("observations must be rows of (step, x, y, value, sd)")
+    if np.any(np.diff(obs[:, 0]) < 0):
+        raise ValueError("observations must be time ordered")
+    if forecast_every <= 0:
+        raise HzlInvalidArgument("EnKF forecast output interval must be positive")
+    members.enkf_begin(seed, mode)
+    steps = obs[:, 0].astype(np.int64)
+    history, current = [], 0
+    for target in np.unique(steps):
+        sel = obs[steps == target]
+        rec, mean = members.enkf_assimilate(dt, int(target - current), sel[:, 1], sel[:, 2],
+                                            sel[:, 3], sel[:, 4], inflation,
+                                            localization_radius, mean=retain_states)
+        history.append(dict(step=int(target), forecast_sensor_rmse=rec[0],
+                            posterior_spread=rec[1], mean_field=mean, is_forecast=False))
+        current = int(target)
+    for horizon in range(1, forecast_steps + 1):
+        members.step(dt, 1)
+        if horizon % forecast_every != 0 and horizon != forecast_steps:
+            continue
+        sp, mean = members.mean_spread(mean=retain_states)
+        history.append(dict(step=current + horizon, forecast_sensor_rmse=float("nan"),
+                            posterior_spread=sp, mean_field=mean, is_forecast=True))
+    return history
 
 
 _SUR_PRECISION = {"fp32": FP32, "bf16": BF16}

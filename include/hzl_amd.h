@@ -156,6 +156,32 @@ int hzl_pf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, co
  * -inf, src/filter.cpp:76-80) and number of resampling events. */
 int hzl_pf_result(hzl_ensemble_t h, double *log_likelihood, size_t *resampling_events);
 
+/* ------------------------------------------------------------------------
+ * Ensemble Kalman filter (members are the ensemble), reference
+ * ensemble_kalman_filter (src/filter.cpp:204-356).
+ * ------------------------------------------------------------------------ */
+#define HZL_ENKF_SERIAL 0 /* scalar observations in order, as the reference */
+#define HZL_ENKF_BATCH 1  /* all observations of a group at once (MFMA, FP32) */
+
+/* Start a filter: observation perturbations from the reference NormalStream
+ * seeded stream_seed(seed, members) (src/filter.cpp:224). */
+int hzl_enkf_begin(hzl_ensemble_t h, uint64_t seed, int mode);
+
+/* Advance every member `steps` RK4 steps (engine forcing), apply the
+ * multiplicative anomaly inflation (>= 1, src/filter.cpp:237-253), then
+ * assimilate one observation group.  record[2] receives (forecast sensor
+ * RMSE of the ensemble mean before the analysis, posterior spread
+ * sqrt(sum |w - mean|^2 / (M - 1)) over the full field); mean_field
+ * (optional, reference Field layout) the analysis mean.  Localization
+ * (radius > 0) is rejected: the analysis has no covariance taper. */
+int hzl_enkf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, const int *x,
+                        const int *y, const double *value, const double *sd, double inflation,
+                        double localization_radius, double *record, double *mean_field);
+
+/* Ensemble mean (reference Field layout, optional) and spread, as recorded
+ * by the reference's free-forecast phase (src/filter.cpp:333-355). */
+int hzl_ensemble_mean_spread(hzl_ensemble_t h, double *spread, double *mean_field);
+
 /* Number of steps taken so far (the noise counter of the next step). */
 int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out);
 
