@@ -46,6 +46,28 @@ def build_engine() -> str:
     return LIB
 
 
+DROPIN_SRC = os.path.join(CSRC, "host", "hzl_gpu.cpp")
+DROPIN_LIB = os.path.join(PKG, "libhzl_gpu.so")
+DROPIN_CHECK_SRC = os.path.join(ROOT, "tests", "dropin", "dropin_check.cpp")
+DROPIN_CHECK = os.path.join(ROOT, "tests", "dropin", "dropin_check")
+
+
+def build_dropin() -> str:
+    """C++ drop-in layer (reference class signatures over the C-ABI) and the
+    test driver that exercises it."""
+    flags = ["-O2", "-std=c++20", "-Wall", "-I", INCLUDE, "-I", os.path.join(CSRC, "host")]
+    tmp = DROPIN_LIB + ".tmp"
+    _run(["g++", *flags, "-fPIC", "-shared", DROPIN_SRC, "-L", PKG, "-lhzl_amd",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, DROPIN_LIB)
+    tmp = DROPIN_CHECK + ".tmp"
+    _run(["g++", *flags, DROPIN_CHECK_SRC, "-L", PKG, "-lhzl_gpu", "-lhzl_amd",
+          f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../hzl-stochasticflowinference_amd",
+          "-o", tmp])
+    os.replace(tmp, DROPIN_CHECK)
+    return DROPIN_LIB
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
     if os.path.isdir(os.path.join(REFERENCE, "src")):
@@ -63,6 +85,7 @@ def build_emulator() -> str:
 
 def build_all() -> None:
     build_engine()
+    build_dropin()
     build_oracle()
     build_emulator()
 
@@ -70,5 +93,5 @@ def build_all() -> None:
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "engine": build_engine, "oracle": build_oracle,
-         "emulator": build_emulator}[w]()
+        {"all": build_all, "engine": build_engine, "dropin": build_dropin,
+         "oracle": build_oracle, "emulator": build_emulator}[w]()

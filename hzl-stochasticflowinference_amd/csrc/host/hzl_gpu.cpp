@@ -1,0 +1,485 @@
+// hzl_gpu.cpp -- C++ drop-in layer over the C-ABI (see hzl_gpu.hpp).
+#include "hzl_gpu.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+
+namespace hzl {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// C-ABI status -> the reference's exception types (src/cuda_solver.cu:11-19).
+void check(int rc) {
+    if (rc == HZL_OK)
+        return;
+    const std::string msg = hzl_last_error();
+    if (rc == HZL_EINVAL)
+        throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+const double *raw(const Field &f) {
+    return reinterpret_cast<const double *>(f.data());
+}
+double *raw(Field &f) {
+    return reinterpret_cast<double *>(f.data());
+}
+
+hzl_parameters abi_parameters(const Parameters &p, double amp, int kf) {
+    return hzl_parameters{p.viscosity,   p.friction,    p.forcing_sigma, p.forcing_tau,
+                          p.forcing_min, p.forcing_max, amp,             kf};
+}
+
+Field physical(std::size_t n, Field spectral) {
+    Fourier2D(n).inverse(spectral);
+    return spectral;
+}
+
+} // namespace
+
+// ------------------------------------------------------------ Fourier2D --
+Fourier2D::Fourier2D(std::size_t n) : n_(n), bitrev_(n), twiddle_(n / 2), line_(n) {
+    if (n < 8 || n > 4096 || (n & (n - 1)) != 0)
+        throw std::invalid_argument("grid must be a power of two between 8 and 4096");
+    int bits = 0;
+    while ((std::size_t{1} << bits) < n)
+        ++bits;
+    for (std::size_t i = 0; i < n; ++i) {
+        std::size_t r = 0;
+        for (int b = 0; b < bits; ++b)
+            if (i & (std::size_t{1} << b))
+                r |= std::size_t{1} << (bits - 1 - b);
+        bitrev_[i] = r;
+    }
+    for (std::size_t k = 0; k < n / 2; ++k)
+        twiddle_[k] = std::polar(1.0, -2 * kPi * (double)k / (double)n);
+}
+
+// In-place radix-2 decimation-in-time transform of every row (or column).
+void Fourier2D::lines(Field &data, bool columns, bool inverse) {
+    const std::size_t n = n_;
+    for (std::size_t l = 0; l < n; ++l) {
+        for (std::size_t i = 0; i < n; ++i)
+            line_[bitrev_[i]] = columns ? data[i * n + l] : data[l * n + i];
+        for (std::size_t len = 2; len <= n; len <<= 1) {
+            const std::size_t half = len / 2, stride = n / len;
+            for (std::size_t s = 0; s < n; s += len)
+                for (std::size_t k = 0; k < half; ++k) {
+                    const Complex w = inverse ? std::conj(twiddle_[k * stride]) : twiddle_[k * stride];
+                    const Complex a = line_[s + k], b = line_[s + k + half] * w;
+                    line_[s + k] = a + b;
+                    line_[s + k + half] = a - b;
+                }
+        }
+        for (std::size_t i = 0; i < n; ++i)
+            (columns ? data[i * n + l] : data[l * n + i]) = line_[i];
+    }
+}
+
+void Fourier2D::forward(Field &data) {
+    if (data.size() != n_ * n_)
+        throw std::invalid_argument("FFT field size mismatch");
+    lines(data, false, false);
+    lines(data, true, false);
+    const double scale = 1.0 / (double)(n_ * n_);
+    for (auto &v : data)
+        v *= scale;
+}
+
+void Fourier2D::inverse(Field &data) {
+    if (data.size() != n_ * n_)
+        throw std::invalid_argument("FFT field size mismatch");
+    lines(data, false, true);
+    lines(data, true, true);
+}
+
+int wave_number(std::size_t index, std::size_t n) {
+    return index < n / 2 ? (int)index : (int)index - (int)n;
+}
+
+// ----------------------------------------------------------- parameters --
+void Parameters::validate(std::size_t n) const {
+    if (!std::isfinite(viscosity) || viscosity < 0 || !std::isfinite(friction) || friction < 0 ||
+        !std::isfinite(forcing_sigma) || forcing_sigma < 0 || !std::isfinite(forcing_tau) ||
+        forcing_tau <= 0)
+        throw std::invalid_argument(
+            "nu, alpha, sigma must be finite and nonnegative; tau must be positive");
+    if (forcing_min < 1 || forcing_max < forcing_min ||
+        3 * static_cast<long long>(forcing_max) >= static_cast<long long>(n))
+        throw std::invalid_argument("forcing shell must lie strictly inside the dealiased grid");
+}
+
+std::uint64_t stream_seed(std::uint64_t base, std::uint64_t index) {
+    // SplitMix64 finaliser of base + golden-ratio increments (src/solver.cpp:61-66)
+    std::uint64_t x = base + 0x9e3779b97f4a7c15ULL * (index + 1);
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+namespace {
+void mask(std::size_t n, Field &f) {
+    for (std::size_t y = 0; y < n; ++y)
+        for (std::size_t x = 0; x < n; ++x) {
+            const int a = wave_number(x, n), b = wave_number(y, n);
+            const bool active = 3 * std::abs(a) < (int)n && 3 * std::abs(b) < (int)n &&
+                                (x != 0 || y != 0);
+            if (!active)
+                f[y * n + x] = 0;
+        }
+}
+} // namespace
+
+Field initial_condition(std::size_t n, const std::string &name) {
+    if (name != "taylor-green" && name != "multimode" && name != "zero")
+        throw std::invalid_argument("unknown initial condition");
+    Field w(n * n);
+    for (std::size_t y = 0; y < n; ++y)
+        for (std::size_t x = 0; x < n; ++x) {
+            const double a = 2 * kPi * (double)x / (double)n, b = 2 * kPi * (double)y / (double)n;
+            double v = name == "zero" ? 0.0 : 2 * std::cos(a) * std::cos(b);
+            if (name == "multimode")
+                v += 0.6 * std::sin(2 * a + b) + 0.4 * std::cos(a - 3 * b) +
+                     0.3 * std::sin(3 * a + 2 * b);
+            w[y * n + x] = v;
+        }
+    Fourier2D(n).forward(w);
+    mask(n, w);
+    return w;
+}
+
+Field physical_to_spectral(std::size_t n, const Field &vorticity) {
+    if (vorticity.size() != n * n)
+        throw std::invalid_argument("physical initial field size mismatch");
+    for (const auto &v : vorticity)
+        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || v.imag() != 0)
+            throw std::invalid_argument("physical initial vorticity must be finite and real");
+    Field w = vorticity;
+    Fourier2D(n).forward(w);
+    mask(n, w);
+    return w;
+}
+
+// ----------------------------------------------------------- CudaSolver --
+struct CudaSolver::Implementation {
+    std::size_t n;
+    Parameters p;
+    hzl_ensemble_t h = nullptr;
+    ~Implementation() {
+        if (h)
+            hzl_ensemble_destroy(h);
+    }
+};
+
+CudaSolver::CudaSolver(std::size_t n, Parameters parameters, const Field &initial)
+    : implementation_(std::make_unique<Implementation>()) {
+    parameters.validate(n);
+    if (initial.size() != n * n)
+        throw std::invalid_argument("CUDA initial coefficient size mismatch");
+    auto &s = *implementation_;
+    s.n = n;
+    s.p = parameters;
+    const hzl_parameters ap = abi_parameters(parameters, 0.0, 4);
+    check(hzl_ensemble_create(&s.h, n, 1, HZL_FP64, &ap, 0));
+    check(hzl_ensemble_set_state(s.h, 0, raw(initial)));
+}
+
+CudaSolver::~CudaSolver() = default;
+
+void CudaSolver::step(double dt) {
+    // the reference GPU solver has no forcing generator (src/cuda_solver.cu:172-183)
+    if (implementation_->p.forcing_sigma != 0)
+        throw std::invalid_argument("stochastic GPU step requires midpoint forcing coefficients");
+    // sigma = 0 and no Kolmogorov term: the engine step adds no forcing
+    check(hzl_ensemble_step(implementation_->h, dt, 1));
+}
+
+void CudaSolver::step(double dt, const Field &midpoint_forcing) {
+    if (midpoint_forcing.size() != implementation_->n * implementation_->n)
+        throw std::invalid_argument("GPU forcing size mismatch");
+    check(hzl_ensemble_step_forced(implementation_->h, dt, raw(midpoint_forcing), 1));
+}
+
+void CudaSolver::synchronize() {
+    check(hzl_ensemble_synchronize(implementation_->h));
+}
+
+Field CudaSolver::coefficients() const {
+    const std::size_t n = implementation_->n;
+    check(hzl_ensemble_synchronize(implementation_->h));
+    Field out(n * n);
+    check(hzl_ensemble_get_state(implementation_->h, 0, raw(out)));
+    return out;
+}
+
+// ------------------------------------------------------------- Ensemble --
+Ensemble::Ensemble(std::size_t n, std::size_t members, Parameters parameters,
+                   Precision precision, std::uint64_t seed, double kolmogorov_amplitude,
+                   int kolmogorov_wavenumber)
+    : n_(n), members_(members) {
+    parameters.validate(n);
+    const hzl_parameters ap = abi_parameters(parameters, kolmogorov_amplitude,
+                                             kolmogorov_wavenumber);
+    check(hzl_ensemble_create(&handle_, n, members,
+                              precision == Precision::fp32 ? HZL_FP32 : HZL_FP64, &ap, seed));
+}
+
+Ensemble::~Ensemble() {
+    if (handle_)
+        hzl_ensemble_destroy(handle_);
+}
+
+void Ensemble::set_member_parameters(std::size_t member, double viscosity, double friction,
+                                     double forcing_sigma, double kolmogorov_amplitude) {
+    check(hzl_ensemble_set_member_parameters(handle_, member, viscosity, friction,
+                                             forcing_sigma, kolmogorov_amplitude));
+}
+
+void Ensemble::set_state(std::size_t member, const Field &coefficients) {
+    if (coefficients.size() != n_ * n_)
+        throw std::invalid_argument("state size mismatch");
+    check(hzl_ensemble_set_state(handle_, member, raw(coefficients)));
+}
+
+void Ensemble::set_all(const Field &coefficients) {
+    if (coefficients.size() != n_ * n_)
+        throw std::invalid_argument("state size mismatch");
+    for (std::size_t m = 0; m < members_; ++m)
+        check(hzl_ensemble_set_state(handle_, m, raw(coefficients)));
+}
+
+Field Ensemble::state(std::size_t member) const {
+    Field out(n_ * n_);
+    check(hzl_ensemble_get_state(handle_, member, raw(out)));
+    return out;
+}
+
+void Ensemble::step(double dt, std::size_t steps) {
+    check(hzl_ensemble_step(handle_, dt, steps));
+}
+
+void Ensemble::step(double dt, const Field &shared_midpoint_forcing) {
+    if (shared_midpoint_forcing.size() != n_ * n_)
+        throw std::invalid_argument("GPU forcing size mismatch");
+    check(hzl_ensemble_step_forced(handle_, dt, raw(shared_midpoint_forcing), 1));
+}
+
+void Ensemble::synchronize() {
+    check(hzl_ensemble_synchronize(handle_));
+}
+
+std::vector<Diagnostics> Ensemble::diagnostics() {
+    std::vector<double> d(6 * members_);
+    check(hzl_ensemble_diagnostics(handle_, d.data()));
+    std::vector<Diagnostics> out(members_);
+    for (std::size_t m = 0; m < members_; ++m)
+        out[m] = Diagnostics{0.0, d[6 * m], d[6 * m + 1], d[6 * m + 2], d[6 * m + 3],
+                             d[6 * m + 4]};
+    return out;
+}
+
+// -------------------------------------------------------------- Dataset --
+void Dataset::validate() const {
+    Parameters p;
+    p.forcing_min = forcing_min;
+    p.forcing_max = forcing_max;
+    (void)Fourier2D(n);
+    p.validate(n);
+    if (n < 16 || !std::isfinite(dt) || dt <= 0 || observations.empty())
+        throw std::invalid_argument("invalid observation dataset");
+    std::size_t previous = 0;
+    for (const auto &o : observations) {
+        if (o.step < previous || o.x >= n || o.y >= n || !std::isfinite(o.value) ||
+            !std::isfinite(o.standard_deviation) || o.standard_deviation <= 0)
+            throw std::invalid_argument("observations must be time ordered, finite, in bounds, and "
+                                        "have positive uncertainty");
+        previous = o.step;
+    }
+}
+
+void Dataset::save(const std::filesystem::path &path) const {
+    validate();
+    std::ofstream out(path);
+    out << std::setprecision(17);
+    out << "HZL_OBSERVATIONS_V2," << n << ',' << dt << ',' << stochastic << ',' << forcing_min
+        << ',' << forcing_max << "\nstep,x,y,vorticity,standard_deviation\n";
+    for (const auto &o : observations)
+        out << o.step << ',' << o.x << ',' << o.y << ',' << o.value << ',' << o.standard_deviation
+            << '\n';
+    if (!out)
+        throw std::runtime_error("observation write failed");
+}
+
+Dataset Dataset::load(const std::filesystem::path &path) {
+    std::ifstream in(path);
+    std::string line, magic;
+    Dataset data;
+    if (!std::getline(in, line))
+        throw std::runtime_error("cannot read observation dataset");
+    std::replace(line.begin(), line.end(), ',', ' ');
+    std::istringstream header(line);
+    if (!(header >> magic >> data.n >> data.dt) ||
+        (magic != "HZL_OBSERVATIONS_V1" && magic != "HZL_OBSERVATIONS_V2"))
+        throw std::runtime_error("invalid dataset header");
+    if (magic == "HZL_OBSERVATIONS_V2" &&
+        !(header >> data.stochastic >> data.forcing_min >> data.forcing_max))
+        throw std::runtime_error("invalid stochastic dataset metadata");
+    if (!std::getline(in, line) || line != "step,x,y,vorticity,standard_deviation")
+        throw std::runtime_error("invalid dataset columns");
+    while (std::getline(in, line)) {
+        if (line.empty())
+            continue;
+        std::replace(line.begin(), line.end(), ',', ' ');
+        std::istringstream row(line);
+        Observation o;
+        std::string extra;
+        if (!(row >> o.step >> o.x >> o.y >> o.value >> o.standard_deviation) || (row >> extra))
+            throw std::runtime_error("invalid observation row");
+        data.observations.push_back(o);
+    }
+    data.validate();
+    return data;
+}
+
+// -------------------------------------------------------------- filters --
+namespace {
+
+struct Group {
+    std::size_t step;
+    std::vector<int> x, y;
+    std::vector<double> value, sd;
+};
+
+std::vector<Group> groups_of(const Dataset &data) {
+    std::vector<Group> out;
+    for (const auto &o : data.observations) {
+        if (out.empty() || out.back().step != o.step)
+            out.push_back(Group{o.step, {}, {}, {}, {}});
+        out.back().x.push_back((int)o.x);
+        out.back().y.push_back((int)o.y);
+        out.back().value.push_back(o.value);
+        out.back().sd.push_back(o.standard_deviation);
+    }
+    return out;
+}
+
+Parameters with_data_shell(Parameters p, const Dataset &data) {
+    p.forcing_min = data.forcing_min;
+    p.forcing_max = data.forcing_max;
+    return p;
+}
+
+} // namespace
+
+FilterResult bootstrap_filter(Ensemble &particles, const Dataset &data, std::uint64_t seed,
+                              bool retain_states) {
+    data.validate();
+    if (particles.size() != data.n)
+        throw std::invalid_argument("particle grid does not match the dataset");
+    const auto groups = groups_of(data);
+    check(hzl_pf_begin(particles.handle(), seed, groups.size() + 1));
+    FilterResult result;
+    std::size_t current = 0;
+    const std::size_t n = data.n;
+    for (std::size_t g = 0; g < groups.size(); ++g) {
+        const Group &gr = groups[g];
+        double rec[4];
+        Field mean(retain_states ? n * n : 0);
+        check(hzl_pf_assimilate(particles.handle(), data.dt, gr.step - current, gr.x.size(),
+                                gr.x.data(), gr.y.data(), gr.value.data(), gr.sd.data(),
+                                g + 1 == groups.size(), rec, retain_states ? raw(mean) : nullptr));
+        FilterRecord r{gr.step, rec[0], rec[1], rec[2], {}};
+        if (retain_states)
+            r.mean_vorticity = physical(n, std::move(mean));
+        result.history.push_back(std::move(r));
+        current = gr.step;
+        if (!std::isfinite(rec[0]))
+            break;
+    }
+    check(hzl_pf_result(particles.handle(), &result.log_likelihood, &result.resampling_events));
+    return result;
+}
+
+FilterResult bootstrap_filter(const Dataset &data, Parameters p, std::size_t particles,
+                              std::uint64_t seed, bool retain_states) {
+    data.validate();
+    if (particles == 0)
+        throw std::invalid_argument("particle filter requires at least one particle");
+    p = with_data_shell(p, data);
+    Ensemble e(data.n, particles, p, Precision::fp64, seed);
+    e.set_all(initial_condition(data.n, "multimode"));
+    return bootstrap_filter(e, data, seed, retain_states);
+}
+
+std::vector<EnsembleFilterRecord> ensemble_kalman_filter(Ensemble &members, const Dataset &data,
+                                                         std::uint64_t seed, EnKFOptions options,
+                                                         bool batch) {
+    data.validate();
+    if (members.members() < 2)
+        throw std::invalid_argument("EnKF requires at least two ensemble members");
+    if (!std::isfinite(options.inflation) || options.inflation < 1 ||
+        !std::isfinite(options.localization_radius) || options.localization_radius < 0)
+        throw std::invalid_argument(
+            "EnKF inflation must be >= 1 and localization radius nonnegative");
+    if (options.forecast_every == 0)
+        throw std::invalid_argument("EnKF forecast output interval must be positive");
+    const std::size_t n = data.n;
+    check(hzl_enkf_begin(members.handle(), seed, batch ? HZL_ENKF_BATCH : HZL_ENKF_SERIAL));
+    std::vector<EnsembleFilterRecord> history;
+    std::size_t current = 0;
+    for (const Group &gr : groups_of(data)) {
+        double rec[2];
+        Field mean(n * n);
+        check(hzl_enkf_assimilate(members.handle(), data.dt, gr.step - current, gr.x.size(),
+                                  gr.x.data(), gr.y.data(), gr.value.data(), gr.sd.data(),
+                                  options.inflation, options.localization_radius, rec,
+                                  raw(mean)));
+        history.push_back({gr.step, rec[0], rec[1], physical(n, std::move(mean)), false});
+        current = gr.step;
+    }
+    for (std::size_t horizon = 1; horizon <= options.forecast_steps; ++horizon) {
+        members.step(data.dt, 1);
+        if (horizon % options.forecast_every != 0 && horizon != options.forecast_steps)
+            continue;
+        double spread = 0;
+        Field mean(n * n);
+        check(hzl_ensemble_mean_spread(members.handle(), &spread, raw(mean)));
+        history.push_back({current + horizon, std::numeric_limits<double>::quiet_NaN(), spread,
+                           physical(n, std::move(mean)), true});
+    }
+    return history;
+}
+
+std::vector<EnsembleFilterRecord> ensemble_kalman_filter(const Dataset &data, Parameters p,
+                                                         std::size_t count, std::uint64_t seed,
+                                                         EnKFOptions options) {
+    data.validate();
+    if (count < 2)
+        throw std::invalid_argument("EnKF requires at least two ensemble members");
+    p = with_data_shell(p, data);
+    Ensemble e(data.n, count, p, Precision::fp64, seed);
+    e.set_all(initial_condition(data.n, "multimode"));
+    return ensemble_kalman_filter(e, data, seed, options, false);
+}
+
+std::vector<Diagnostics> ensemble(std::size_t n, Parameters p, double dt, std::size_t steps,
+                                  std::size_t count, std::size_t workers, std::uint64_t seed) {
+    if (count == 0 || workers == 0)
+        throw std::invalid_argument("ensemble count and workers must be positive");
+    Ensemble e(n, count, p, Precision::fp64, seed);
+    e.step(dt, steps);
+    e.synchronize();
+    auto d = e.diagnostics();
+    for (auto &x : d)
+        x.time = dt * (double)steps;
+    return d;
+}
+
+} // namespace hzl

@@ -1,0 +1,196 @@
+// hzl_gpu.hpp -- C++ drop-in for the solver / filter path of
+// pvasbob/HZL-stochasticFlowInference, implemented over the C-ABI in
+// include/hzl_amd.h (libhzl_amd.so, gfx950 kernels).
+//
+// The class and function signatures are those of the reference headers, so a
+// caller of
+//   hzl::CudaSolver             (include/hzl/cuda_solver.hpp:7-22)
+//   hzl::Parameters, Diagnostics, stream_seed        (include/hzl/solver.hpp:15-40)
+//   hzl::Fourier2D, wave_number (include/hzl/fft.hpp:10-25)
+//   hzl::Observation, Dataset   (include/hzl/inference.hpp:33-49)
+//   hzl::ensemble               (include/hzl/inference.hpp:84-85)
+//   hzl::bootstrap_filter, ensemble_kalman_filter and their records
+//                               (include/hzl/filter.hpp:5-54)
+// compiles and links against libhzl_gpu.so instead of the reference library.
+// Differences of behaviour, all on the stochastic-forcing side:
+//   * the engine's forcing is white in time (variance sigma^2 dt / 2 per real
+//     component on the shell) instead of the reference's Ornstein-Uhlenbeck
+//     process; with forcing_sigma = 0 every function here reproduces the
+//     reference's deterministic results (filters, EnKF, ensemble diagnostics);
+//   * the EnKF analysis updates the vorticity only (white forcing carries no
+//     latent state) and rejects localization_radius > 0;
+//   * CudaSolver's CFL check is read when the step's results are read
+//     (synchronize / coefficients) instead of after every step.
+// hzl::Ensemble (GPU batches of members) has no reference counterpart: it
+// replaces the reference's std::vector<Solver> loops.
+#pragma once
+#include "hzl_amd.h"
+
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <filesystem>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace hzl {
+
+using Complex = std::complex<double>;
+using Field = std::vector<Complex>;
+
+// Host 2D FFT with the reference normalisation: physical(x) = sum_k
+// coefficient(k) exp(ikx); forward divides by n^2, inverse does not.
+class Fourier2D {
+  public:
+    explicit Fourier2D(std::size_t n);
+    [[nodiscard]] std::size_t size() const noexcept {
+        return n_;
+    }
+    void forward(Field &data);
+    void inverse(Field &data);
+
+  private:
+    void lines(Field &data, bool columns, bool inverse);
+    std::size_t n_;
+    std::vector<std::size_t> bitrev_;
+    Field twiddle_, line_;
+};
+[[nodiscard]] int wave_number(std::size_t index, std::size_t n);
+
+struct Parameters {
+    double viscosity = 0.01;
+    double friction = 0.1;
+    double forcing_sigma = 0.2;
+    double forcing_tau = 0.5;
+    int forcing_min = 2;
+    int forcing_max = 3;
+    void validate(std::size_t n) const;
+};
+struct Diagnostics {
+    double time{}, energy{}, enstrophy{}, dissipation{}, injection{}, cfl_rate{};
+};
+std::uint64_t stream_seed(std::uint64_t base, std::uint64_t index);
+
+// Named initial conditions of Solver::initialize (taylor-green, multimode,
+// zero) as masked spectral coefficients, and Solver::initialize_physical.
+Field initial_condition(std::size_t n, const std::string &name);
+Field physical_to_spectral(std::size_t n, const Field &vorticity);
+
+class CudaSolver {
+  public:
+    CudaSolver(std::size_t n, Parameters parameters, const Field &initial);
+    ~CudaSolver();
+    CudaSolver(const CudaSolver &) = delete;
+    CudaSolver &operator=(const CudaSolver &) = delete;
+    void step(double dt);
+    void step(double dt, const Field &midpoint_forcing);
+    void synchronize();
+    [[nodiscard]] Field coefficients() const;
+
+  private:
+    struct Implementation;
+    std::unique_ptr<Implementation> implementation_;
+};
+
+enum class Precision { fp32, fp64 };
+
+// A batch of members on one grid in one engine handle.  Members start at
+// zero; member m's noise stream is keyed by stream_seed(seed, m).
+class Ensemble {
+  public:
+    Ensemble(std::size_t n, std::size_t members, Parameters parameters,
+             Precision precision = Precision::fp64, std::uint64_t seed = 0,
+             double kolmogorov_amplitude = 0.0, int kolmogorov_wavenumber = 4);
+    ~Ensemble();
+    Ensemble(const Ensemble &) = delete;
+    Ensemble &operator=(const Ensemble &) = delete;
+    void set_member_parameters(std::size_t member, double viscosity, double friction,
+                               double forcing_sigma, double kolmogorov_amplitude);
+    void set_state(std::size_t member, const Field &coefficients);
+    void set_all(const Field &coefficients); // every member
+    [[nodiscard]] Field state(std::size_t member) const;
+    void step(double dt, std::size_t steps = 1);
+    void step(double dt, const Field &shared_midpoint_forcing);
+    void synchronize();
+    [[nodiscard]] std::vector<Diagnostics> diagnostics();
+    [[nodiscard]] std::size_t size() const noexcept {
+        return n_;
+    }
+    [[nodiscard]] std::size_t members() const noexcept {
+        return members_;
+    }
+    [[nodiscard]] hzl_ensemble_t handle() const noexcept {
+        return handle_;
+    }
+
+  private:
+    std::size_t n_, members_;
+    hzl_ensemble_t handle_ = nullptr;
+};
+
+struct Observation {
+    std::size_t step{}, x{}, y{};
+    double value{}, standard_deviation{};
+};
+// Time-ordered point observations; HZL_OBSERVATIONS_V2 CSV files
+// (src/inference.cpp:197-255; V1 files load too).
+struct Dataset {
+    std::size_t n = 16;
+    double dt = 0.01;
+    bool stochastic = false;
+    int forcing_min = 2, forcing_max = 3;
+    std::vector<Observation> observations;
+    void validate() const;
+    void save(const std::filesystem::path &path) const;
+    static Dataset load(const std::filesystem::path &path);
+};
+
+struct FilterRecord {
+    std::size_t step{};
+    double log_likelihood_increment{}, effective_particles{}, maximum_weight{};
+    Field mean_vorticity; // physical, as the reference records it
+};
+struct FilterResult {
+    double log_likelihood = 0;
+    std::size_t resampling_events = 0;
+    std::vector<FilterRecord> history;
+};
+// Bootstrap particle filter with systematic resampling below N/2 ESS;
+// particles start from initialize("multimode") (src/filter.cpp:38-126).
+FilterResult bootstrap_filter(const Dataset &data, Parameters p, std::size_t particles,
+                              std::uint64_t seed, bool retain_states = true);
+// The same filter over an initialised GPU ensemble (members are particles).
+FilterResult bootstrap_filter(Ensemble &particles, const Dataset &data, std::uint64_t seed,
+                              bool retain_states = true);
+
+struct EnsembleFilterRecord {
+    std::size_t step{};
+    double forecast_sensor_rmse{}, posterior_spread{};
+    Field mean_vorticity; // physical
+    bool is_forecast = false;
+};
+struct EnKFOptions {
+    double inflation = 1.0;
+    double localization_radius = 0.0; // > 0 is rejected by the GPU analysis
+    std::size_t forecast_steps = 0;
+    std::size_t forecast_every = 1;
+};
+// Serial perturbed-observation EnKF; members start from initialize("multimode")
+// (src/filter.cpp:204-356).
+std::vector<EnsembleFilterRecord> ensemble_kalman_filter(const Dataset &data, Parameters p,
+                                                         std::size_t members, std::uint64_t seed,
+                                                         EnKFOptions options = {});
+// The same filter over an initialised GPU ensemble; batch = true assimilates
+// each observation group at once (MFMA analysis, FP32 ensembles).
+std::vector<EnsembleFilterRecord> ensemble_kalman_filter(Ensemble &members, const Dataset &data,
+                                                         std::uint64_t seed,
+                                                         EnKFOptions options = {},
+                                                         bool batch = false);
+
+// Independent members from the zero state, final diagnostics in member order
+// (src/inference.cpp:555-590); `workers` is accepted and unused.
+std::vector<Diagnostics> ensemble(std::size_t n, Parameters p, double dt, std::size_t steps,
+                                  std::size_t count, std::size_t workers, std::uint64_t seed);
+
+} // namespace hzl

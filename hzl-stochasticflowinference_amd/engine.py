@@ -288,6 +288,67 @@ class Ensemble:
         return ll.value, ev.value
 
 
+OBSERVATIONS_COLUMNS = "step,x,y,vorticity,standard_deviation"
+
+
+def load_observations(path: str):
+    """Read an HZL_OBSERVATIONS_V2 (or V1) CSV file (src/inference.cpp:226-255).
+    Returns (meta, rows): meta dict(n, dt, stochastic, forcing_min,
+    forcing_max), rows an (m, 5) array of (step, x, y, value, sd)."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    head = lines[0].split(",")
+    if head[0] not in ("HZL_OBSERVATIONS_V1", "HZL_OBSERVATIONS_V2") or len(head) < 3:
+        raise HzlError("invalid dataset header")
+    meta = dict(n=int(head[1]), dt=float(head[2]), stochastic=False, forcing_min=2,
+                forcing_max=3)
+    if head[0] == "HZL_OBSERVATIONS_V2":
+        if len(head) != 6:
+            raise HzlError("invalid stochastic dataset metadata")
+        meta.update(stochastic=bool(int(head[3])), forcing_min=int(head[4]),
+                    forcing_max=int(head[5]))
+    if len(lines) < 2 or lines[1] != OBSERVATIONS_COLUMNS:
+        raise HzlError("invalid dataset columns")
+    rows = []
+    for line in lines[2:]:
+        if not line:
+            continue
+        parts = line.split(",")
+        if len(parts) != 5:
+            raise HzlError("invalid observation row")
+        rows.append((int(parts[0]), int(parts[1]), int(parts[2]), float(parts[3]),
+                     float(parts[4])))
+    rows = np.array(rows, dtype=np.float64).reshape(-1, 5)
+    validate_observations(meta, rows)
+    return meta, rows
+
+
+def validate_observations(meta, rows):
+    """Dataset::validate (src/inference.cpp:197-213)."""
+    n = meta["n"]
+    if (n < 16 or n & (n - 1) or not np.isfinite(meta["dt"]) or meta["dt"] <= 0
+            or len(rows) == 0 or meta["forcing_min"] < 1
+            or meta["forcing_max"] < meta["forcing_min"] or 3 * meta["forcing_max"] >= n):
+        raise HzlInvalidArgument("invalid observation dataset")
+    if (np.any(np.diff(rows[:, 0]) < 0) or np.any(rows[:, 1] >= n) or np.any(rows[:, 2] >= n)
+            or not np.all(np.isfinite(rows[:, 3:])) or np.any(rows[:, 4] <= 0)):
+        raise HzlInvalidArgument("observations must be time ordered, finite, in bounds, and "
+                                 "have positive uncertainty")
+
+
+def save_observations(path: str, meta, rows):
+    """Write an HZL_OBSERVATIONS_V2 CSV file with 17 significant digits
+    (src/inference.cpp:214-225)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    validate_observations(meta, rows)
+    g = lambda v: "%.17g" % v
+    out = [f"HZL_OBSERVATIONS_V2,{meta['n']},{g(meta['dt'])},{int(meta['stochastic'])},"
+           f"{meta['forcing_min']},{meta['forcing_max']}", OBSERVATIONS_COLUMNS]
+    out += [f"{int(r[0])},{int(r[1])},{int(r[2])},{g(r[3])},{g(r[4])}" for r in rows]
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
 def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,
                      retain_states: bool = True):
     """Bootstrap particle filter over an initialised ensemble.

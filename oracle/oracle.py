@@ -308,6 +308,8 @@ class Reference:
                                            C.c_size_t, _dp]
         L.ref_negative_log_likelihood.restype = C.c_int
         L.ref_negative_log_likelihood.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+        L.ref_dataset_save.restype = C.c_int
+        L.ref_dataset_save.argtypes = [C.c_void_p, C.c_char_p]
         L.ref_enkf.restype = C.c_int
         L.ref_enkf.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_size_t, C.c_uint64, C.c_double, C.c_double, _dp, C.c_size_t,
@@ -405,6 +407,9 @@ class Reference:
         out = np.zeros(1)
         self._check(self.lib.ref_negative_log_likelihood(h, log_nu, log_alpha, _ptr(out)))
         return float(out[0])
+
+    def dataset_save(self, h, path):
+        self._check(self.lib.ref_dataset_save(h, str(path).encode()))
 
     def enkf(self, h, nu, alpha, sigma, tau, members, seed, inflation=1.0, radius=0.0, n=16,
              max_records=4096):

@@ -263,6 +263,11 @@ int ref_bootstrap_filter(void *d, double nu, double alpha, double sigma, double 
     });
 }
 
+// Dataset::save (src/inference.cpp:214-225): the reference's own CSV writer.
+int ref_dataset_save(void *d, const char *path) {
+    return guarded([&] { static_cast<RefDataset *>(d)->data.save(path); });
+}
+
 // Deterministic negative log-likelihood over (log nu, log alpha) (src/inference.cpp:321-339).
 int ref_negative_log_likelihood(void *d, double log_nu, double log_alpha, double *out) {
     return guarded([&] {

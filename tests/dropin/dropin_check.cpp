@@ -1,0 +1,148 @@
+// dropin_check.cpp -- drives the C++ drop-in layer (libhzl_gpu.so) through the
+// reference's own class and function signatures (TEST ONLY).  Each mode writes
+// its results as raw little-endian doubles for tests/test_dropin.py.
+//   dataset <in.csv> <out.csv>      Dataset::load then Dataset::save
+//   fft <n> <out>                   Fourier2D forward of a fixed pattern, then inverse
+//   initial <n> <name> <out>        initial_condition
+//   solver <n> <steps> <out>        CudaSolver from multimode, deterministic steps
+//   forced <n> <steps> <out>        CudaSolver::step(dt, F) with a fixed forcing
+//   pf <csv> <out>                  bootstrap_filter(data, p, 8, 555)
+//   enkf <csv> <out>                ensemble_kalman_filter(data, p, 8, 555)
+//   errors                          invalid arguments raise std::invalid_argument
+#include "hzl_gpu.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace hzl;
+
+static void write(const std::string &path, const std::vector<double> &v) {
+    std::ofstream out(path, std::ios::binary);
+    out.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(double)));
+}
+static void append(std::vector<double> &v, const Field &f) {
+    for (const auto &c : f) {
+        v.push_back(c.real());
+        v.push_back(c.imag());
+    }
+}
+
+template <class F> static bool throws_invalid(F &&f) {
+    try {
+        f();
+    } catch (const std::invalid_argument &) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2)
+        return 2;
+    const std::string mode = argv[1];
+    try {
+        if (mode == "dataset" && argc == 4) {
+            Dataset::load(argv[2]).save(argv[3]);
+        } else if (mode == "fft" && argc == 4) {
+            const std::size_t n = std::stoul(argv[2]);
+            Field f(n * n);
+            for (std::size_t i = 0; i < f.size(); ++i)
+                f[i] = Complex(std::sin(0.37 * (double)i) + 0.25, std::cos(1.3 * (double)i));
+            std::vector<double> out;
+            Fourier2D t(n);
+            Field g = f;
+            t.forward(g);
+            append(out, g);
+            t.inverse(g);
+            append(out, g);
+            write(argv[3], out);
+        } else if (mode == "initial" && argc == 5) {
+            std::vector<double> out;
+            append(out, initial_condition(std::stoul(argv[2]), argv[3]));
+            write(argv[4], out);
+        } else if ((mode == "solver" || mode == "forced") && argc == 5) {
+            const std::size_t n = std::stoul(argv[2]), steps = std::stoul(argv[3]);
+            Parameters p;
+            p.viscosity = 1e-2;
+            p.friction = 0.1;
+            p.forcing_sigma = 0;
+            CudaSolver s(n, p, initial_condition(n, "multimode"));
+            Field F(n * n);
+            if (mode == "forced") {
+                // 0.3 sin(2x) + 0.2 cos(3y) as coefficients
+                F[2] = Complex(0, -0.15);
+                F[n - 2] = Complex(0, 0.15);
+                F[3 * n] = Complex(0.1, 0);
+                F[(n - 3) * n] = Complex(0.1, 0);
+            }
+            for (std::size_t k = 0; k < steps; ++k) {
+                if (mode == "forced")
+                    s.step(1e-3, F);
+                else
+                    s.step(1e-3);
+            }
+            s.synchronize();
+            std::vector<double> out;
+            append(out, s.coefficients());
+            write(argv[4], out);
+        } else if (mode == "pf" && argc == 4) {
+            const Dataset data = Dataset::load(argv[2]);
+            Parameters p;
+            p.viscosity = 0.03;
+            p.friction = 0.1;
+            p.forcing_sigma = 0;
+            const FilterResult r = bootstrap_filter(data, p, 8, 555);
+            std::vector<double> out{r.log_likelihood, (double)r.resampling_events};
+            for (const auto &h : r.history) {
+                out.insert(out.end(), {(double)h.step, h.log_likelihood_increment,
+                                       h.effective_particles, h.maximum_weight});
+                append(out, h.mean_vorticity);
+            }
+            write(argv[3], out);
+        } else if (mode == "enkf" && argc == 4) {
+            const Dataset data = Dataset::load(argv[2]);
+            Parameters p;
+            p.viscosity = 0.03;
+            p.friction = 0.1;
+            p.forcing_sigma = 0;
+            const auto h = ensemble_kalman_filter(data, p, 8, 555);
+            std::vector<double> out;
+            for (const auto &r : h) {
+                out.insert(out.end(),
+                           {(double)r.step, r.forecast_sensor_rmse, r.posterior_spread});
+                append(out, r.mean_vorticity);
+            }
+            write(argv[3], out);
+        } else if (mode == "errors") {
+            Parameters p;
+            const bool ok =
+                throws_invalid([&] { CudaSolver(16, p, Field(15)); }) &&
+                throws_invalid([&] { CudaSolver(16, p, initial_condition(16, "zero")).step(1e-3); }) &&
+                throws_invalid([&] { Parameters q; q.forcing_max = 6; q.validate(16); }) &&
+                throws_invalid([&] { (void)initial_condition(16, "vortex"); }) &&
+                throws_invalid([&] {
+                    Parameters q = p;
+                    q.forcing_sigma = 0;
+                    Field F(16 * 16);
+                    F[1] = Complex(1, 0); // no conjugate partner
+                    CudaSolver(16, q, initial_condition(16, "zero")).step(1e-3, F);
+                });
+            std::cout << (ok ? "ok" : "fail") << "\n";
+            return ok ? 0 : 1;
+        } else {
+            return 2;
+        }
+    } catch (const std::exception &e) {
+        std::cerr << "error: " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}

@@ -88,6 +88,8 @@ def pf16(ref: Reference) -> None:
     nll = ref.negative_log_likelihood(h, np.log(nu), np.log(alpha))
     enkf = ref.enkf(h, nu, alpha, 0.0, 0.5, 8, 555, n=16)
     init = ref.initialize(16, "multimode")
+    # the same dataset written by the reference's Dataset::save
+    ref.dataset_save(h, os.path.join(HERE, "observations_v2_16.csv"))
     ref.dataset_free(h)
     save("pf_deterministic_16",
          dict(recipe="synthetic_observations(16, 0.01, 30, 10, p, 0.2, 987) with nu 0.03, "

@@ -1,0 +1,144 @@
+"""The C++ drop-in layer (csrc/host/hzl_gpu.hpp: the reference's class and
+function signatures over the C-ABI) and the observation-file formats.
+
+CPU: the reference-written HZL_OBSERVATIONS_V2 fixture round-trips byte for
+byte through Dataset::load/save and through the Python reader/writer; the host
+FFT matches numpy; initial_condition matches the reference's own
+Solver::initialize fixtures.  GPU: CudaSolver against the FP64 oracle, and the
+reference-signature bootstrap_filter / ensemble_kalman_filter against the
+reference golden records.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+CHECK = os.path.join(HERE, "dropin", "dropin_check")
+CSV = os.path.join(GOLDEN, "observations_v2_16.csv")
+
+
+def run(*args, timeout=120):
+    if not os.path.exists(CHECK):
+        pytest.fail(f"{CHECK} not built: run __graft_entry__.build()")
+    p = subprocess.run([CHECK, *map(str, args)], capture_output=True, text=True,
+                       timeout=timeout)
+    assert p.returncode == 0, p.stderr + p.stdout
+    return p.stdout
+
+
+def field(raw, n, count=1):
+    c = raw[0::2] + 1j * raw[1::2]
+    return c.reshape(count, n, n) if count > 1 else c.reshape(n, n)
+
+
+def full_from_half(h, n):
+    K = (n - 1) // 3
+    out = np.zeros((n, n), dtype=np.complex128)
+    for c in range(K + 1):
+        for j, ky in enumerate(range(-K, K + 1)):
+            out[ky % n, c] = h[c, j]
+            if c > 0:
+                out[(-ky) % n, n - c] = np.conj(h[c, j])
+    return out
+
+
+def test_dataset_csv_round_trip_is_byte_identical(tmp_path, pkg):
+    out = tmp_path / "out.csv"
+    run("dataset", CSV, out)
+    assert out.read_bytes() == open(CSV, "rb").read()
+    meta, rows = pkg.load_observations(CSV)
+    g = np.load(os.path.join(GOLDEN, "pf_deterministic_16.npz"))
+    np.testing.assert_array_equal(rows, g["rows"])
+    assert meta == dict(n=16, dt=0.01, stochastic=False, forcing_min=2, forcing_max=3)
+    py = tmp_path / "py.csv"
+    pkg.save_observations(py, meta, rows)
+    assert py.read_bytes() == open(CSV, "rb").read()
+
+
+def test_dataset_validation_errors(tmp_path, pkg):
+    meta, rows = pkg.load_observations(CSV)
+    bad = rows.copy()
+    bad[0, 4] = 0.0
+    with pytest.raises(pkg.HzlInvalidArgument):
+        pkg.save_observations(tmp_path / "x.csv", meta, bad)
+    bad = rows[::-1].copy()
+    with pytest.raises(pkg.HzlInvalidArgument):
+        pkg.validate_observations(meta, bad)
+    (tmp_path / "h.csv").write_text("HZL_OBSERVATIONS_V3,16,0.01\n")
+    with pytest.raises(pkg.HzlError):
+        pkg.load_observations(tmp_path / "h.csv")
+    p = subprocess.run([CHECK, "dataset", tmp_path / "h.csv", tmp_path / "o.csv"],
+                       capture_output=True, text=True)
+    assert p.returncode == 1 and "invalid dataset header" in p.stderr
+
+
+@pytest.mark.parametrize("n", [16, 64])
+def test_host_fft_matches_numpy(tmp_path, n):
+    run("fft", n, tmp_path / "f.bin")
+    raw = np.fromfile(tmp_path / "f.bin")
+    fwd, back = field(raw[:2 * n * n], n), field(raw[2 * n * n:], n)
+    i = np.arange(n * n)
+    x = (np.sin(0.37 * i) + 0.25 + 1j * np.cos(1.3 * i)).reshape(n, n)
+    np.testing.assert_allclose(fwd, np.fft.fft2(x) / (n * n), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(back, x, rtol=0, atol=1e-13)
+
+
+def test_initial_condition_matches_reference_fixtures(tmp_path):
+    g = np.load(os.path.join(GOLDEN, "pf_deterministic_16.npz"))
+    run("initial", 16, "multimode", tmp_path / "a.bin")
+    np.testing.assert_allclose(field(np.fromfile(tmp_path / "a.bin"), 16),
+                               full_from_half(g["initial"], 16), rtol=0, atol=1e-15)
+    d = np.load(os.path.join(GOLDEN, "drift_multimode_32.npz"))
+    run("initial", 32, "multimode", tmp_path / "b.bin")
+    np.testing.assert_allclose(field(np.fromfile(tmp_path / "b.bin"), 32),
+                               full_from_half(d["state"], 32), rtol=0, atol=1e-15)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["solver", "forced"])
+def test_cudasolver_matches_oracle(tmp_path, orc, mode):
+    n, steps = 64, 100
+    run(mode, n, steps, tmp_path / "s.bin")
+    got = field(np.fromfile(tmp_path / "s.bin"), n)
+    run("initial", n, "multimode", tmp_path / "i.bin")
+    w0 = field(np.fromfile(tmp_path / "i.bin"), n)
+    F = None
+    if mode == "forced":
+        F = np.zeros((n, n), dtype=np.complex128)
+        F[0, 2], F[0, n - 2], F[3, 0], F[n - 3, 0] = -0.15j, 0.15j, 0.1, 0.1
+    want = orc.rk4(w0, 1e-2, 0.1, 1e-3, steps, forcing=F)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+
+
+@pytest.mark.gpu
+def test_reference_signature_filters_match_golden(tmp_path):
+    g = np.load(os.path.join(GOLDEN, "pf_deterministic_16.npz"))
+    meta = json.load(open(os.path.join(GOLDEN, "pf_deterministic_16.json")))
+    n = 16
+    run("pf", CSV, tmp_path / "pf.bin")
+    raw = np.fromfile(tmp_path / "pf.bin")
+    assert abs(raw[0] - meta["log_likelihood"]) < 1e-10 and raw[1] == 0
+    rec = 4 + 2 * n * n
+    for k, (want, mean) in enumerate(zip(g["records"], g["means"])):
+        r = raw[2 + k * rec: 2 + (k + 1) * rec]
+        assert abs(r[1] - want[0]) < 1e-10 and abs(r[2] - want[1]) < 1e-9
+        assert abs(r[3] - want[2]) < 1e-12
+        np.testing.assert_allclose(field(r[4:], n).real, mean, rtol=0, atol=1e-10)
+    run("enkf", CSV, tmp_path / "enkf.bin")
+    raw = np.fromfile(tmp_path / "enkf.bin")
+    rec = 3 + 2 * n * n
+    for k, (want, mean) in enumerate(zip(g["enkf_records"], g["enkf_means"])):
+        r = raw[k * rec:(k + 1) * rec]
+        assert r[0] == want[0] and abs(r[1] - want[1]) < 1e-10 and abs(r[2] - want[2]) < 1e-12
+        np.testing.assert_allclose(field(r[3:], n).real, mean, rtol=0, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_reference_error_contract():
+    assert run("errors").strip() == "ok"
