@@ -7,10 +7,11 @@ module name `hzl_sfi_amd`.
 """
 from .engine import (ALL, BF16, EXPORTS, FP32, FP64, Ensemble, HzlError, HzlInvalidArgument,
                      Parameters, Surrogate, bootstrap_filter, ensemble_kalman_filter,
-                     load_library, load_observations, save_observations, surrogate_filter,
-                     validate_observations)
+                     load_library, load_observations, particle_metropolis, save_observations,
+                     surrogate_filter, validate_observations)
 
 __all__ = ["ALL", "BF16", "EXPORTS", "FP32", "FP64", "Ensemble", "HzlError",
            "HzlInvalidArgument", "Parameters", "Surrogate", "bootstrap_filter",
-           "ensemble_kalman_filter", "load_library", "load_observations", "save_observations",
+           "ensemble_kalman_filter", "load_library", "load_observations", "particle_metropolis",
+           "save_observations",
            "surrogate_filter", "validate_observations"]

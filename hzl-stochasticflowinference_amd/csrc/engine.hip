@@ -64,9 +64,10 @@ template <typename R> __global__ void __launch_bounds__(256) k_pf_loglik(PfArgs<
     pf_loglik_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
 }
 
+// one workgroup per filter
 __global__ void __launch_bounds__(256) k_pf_normalize(PfControl a) {
     __shared__ double red[2 * 256];
-    pf_normalize_body(DevCtx{}, a, red, 256);
+    pf_normalize_body(DevCtx{}, a, red, 256, (int)blockIdx.x);
 }
 
 template <typename R>
@@ -384,27 +385,53 @@ template <typename R> class Engine final : public EngineBase {
 
     // ------------------------------------------------ particle filter ----
     void pf_begin(uint64_t seed, size_t max_groups) override {
-        if (B_ < 2)
+        pf_begin_filters(&seed, 1, max_groups, false);
+    }
+
+    // F independent filters over contiguous member segments of B / F
+    // particles; filter f draws its resampling uniforms from
+    // mt19937_64(stream_seed(seeds[f], UINT64_MAX)) and reseeds children with
+    // stream_seed(seeds[f], (g+1) P + i) (src/filter.cpp:54,115).  With
+    // reset_keys, particle i of filter f starts on key stream_seed(seeds[f], i)
+    // (the reference's Solver(n, p, stream_seed(seed, i)) per particle).
+    void pf_begin_filters(const uint64_t *seeds, size_t filters, size_t max_groups,
+                          bool reset_keys) override {
+        if (filters == 0 || B_ % filters != 0)
+            throw std::invalid_argument("members must split evenly into the filters");
+        const size_t P = B_ / filters;
+        if (P < 2)
             throw std::invalid_argument("particle filter needs at least two particles");
-        pf_seed_ = seed;
+        pf_filters_ = filters;
         pf_group_ = 0;
         pf_wstate_ = DeviceBuffer<double>(B_);
         pf_wnorm_ = DeviceBuffer<double>(B_);
         pf_logw_ = DeviceBuffer<double>(B_);
-        pf_stats_ = DeviceBuffer<double>(8);
+        pf_stats_ = DeviceBuffer<double>(8 * filters);
         pf_anc_ = DeviceBuffer<int>(B_);
         pf_mean_ = DeviceBuffer<double>(2 * spec_);
-        std::vector<double> w(B_, 1.0 / (double)B_), st(8, 0.0), cs(n_), sn(n_);
+        std::vector<double> w(B_, 1.0 / (double)P), st(8 * filters, 0.0), cs(n_), sn(n_);
         upload(pf_wstate_.p, w.data(), B_);
-        upload(pf_stats_.p, st.data(), 8);
-        // resampling uniforms: mt19937_64(stream_seed(seed, UINT64_MAX)), one per
-        // resampling event, (x >> 11) 2^-53 (src/filter.cpp:11-13,54)
-        std::mt19937_64 rng(stream_seed(seed, std::numeric_limits<uint64_t>::max()));
-        std::vector<double> u(std::max<size_t>(max_groups, 1));
-        for (auto &x : u)
-            x = (double)(rng() >> 11) * 0x1.0p-53;
+        upload(pf_stats_.p, st.data(), 8 * filters);
+        // resampling uniforms, one per resampling event, (x >> 11) 2^-53
+        // (src/filter.cpp:11-13,54)
+        const size_t U = std::max<size_t>(max_groups, 1);
+        std::vector<double> u(U * filters);
+        for (size_t f = 0; f < filters; ++f) {
+            std::mt19937_64 rng(stream_seed(seeds[f], std::numeric_limits<uint64_t>::max()));
+            for (size_t k = 0; k < U; ++k)
+                u[f * U + k] = (double)(rng() >> 11) * 0x1.0p-53;
+        }
         pf_uniforms_ = DeviceBuffer<double>(u.size());
         upload(pf_uniforms_.p, u.data(), u.size());
+        pf_max_uniforms_ = (int)U;
+        pf_seeds_ = DeviceBuffer<uint64_t>(filters);
+        upload(pf_seeds_.p, seeds, filters);
+        if (reset_keys) {
+            for (size_t f = 0; f < filters; ++f)
+                for (size_t i = 0; i < P; ++i)
+                    keys_[f * P + i] = stream_seed(seeds[f], i);
+            upload(d_keys_.p, keys_.data(), B_);
+        }
         for (int j = 0; j < n_; ++j) {
             cs[j] = std::cos(2 * PI * j / n_);
             sn[j] = std::sin(2 * PI * j / n_);
@@ -423,6 +450,8 @@ template <typename R> class Engine final : public EngineBase {
             throw std::invalid_argument("particle filter not started");
         if (nobs == 0)
             throw std::invalid_argument("observation group is empty");
+        if (mean_field && pf_filters_ != 1)
+            throw std::invalid_argument("mean fields are recorded for a single filter only");
         for (size_t o = 0; o < nobs; ++o)
             if (x[o] < 0 || y[o] < 0 || x[o] >= n_ || y[o] >= n_ || !std::isfinite(value[o]) ||
                 !std::isfinite(sd[o]) || sd[o] <= 0)
@@ -449,10 +478,11 @@ template <typename R> class Engine final : public EngineBase {
                     pf_ox_.p,  pf_oy_.p,  pf_ov_.p,     pf_osd_.p,    pf_cs_.p,
                     pf_sn_.p,  pf_wstate_.p, pf_wnorm_.p, pf_logw_.p, nullptr};
         hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
-        PfControl c{(int)B_,         pf_logw_.p,     pf_wstate_.p,  pf_wnorm_.p,
-                    pf_stats_.p,     pf_anc_.p,      pf_uniforms_.p, (int)pf_uniforms_.count,
-                    d_keys_.p,       pf_seed_,       pf_group_,     is_last ? 1 : 0};
-        hipLaunchKernelGGL(k_pf_normalize, dim3(1), dim3(256), 0, stream_, c);
+        const int P = (int)(B_ / pf_filters_);
+        PfControl c{P,           pf_logw_.p, pf_wstate_.p,      pf_wnorm_.p,
+                    pf_stats_.p, pf_anc_.p,  pf_uniforms_.p,    pf_max_uniforms_,
+                    d_keys_.p,   pf_seeds_.p, pf_group_,        is_last ? 1 : 0};
+        hipLaunchKernelGGL(k_pf_normalize, dim3((unsigned)pf_filters_), dim3(256), 0, stream_, c);
         if (mean_field) {
             const long blocks = ((long)spec_ + 255) / 256;
             hipLaunchKernelGGL(k_pf_mean<R>, dim3((unsigned)blocks), dim3(256), 0, stream_,
@@ -468,8 +498,9 @@ template <typename R> class Engine final : public EngineBase {
               "ancestor copy");
         check(hipGetLastError(), "particle filter kernels");
         mix_valid_ = false;
-        double st[8];
-        check(hipMemcpyAsync(st, pf_stats_.p, sizeof(st), hipMemcpyDeviceToHost, stream_),
+        std::vector<double> st(8 * pf_filters_);
+        check(hipMemcpyAsync(st.data(), pf_stats_.p, st.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_),
               "filter statistics");
         std::vector<double> mean;
         if (mean_field) {
@@ -479,18 +510,19 @@ template <typename R> class Engine final : public EngineBase {
                   "filter mean");
         }
         check(hipStreamSynchronize(stream_), "particle filter");
-        if (record) {
-            record[0] = st[0];
-            record[1] = st[1];
-            record[2] = st[2];
-            record[3] = st[3];
+        bool any_resampled = false;
+        for (size_t f = 0; f < pf_filters_; ++f) {
+            if (record)
+                for (int k = 0; k < 4; ++k)
+                    record[4 * f + k] = st[8 * f + k];
+            any_resampled = any_resampled || st[8 * f + 3] != 0;
         }
         if (mean_field)
             half_to_full<double>(n_, reinterpret_cast<const cx<double> *>(mean.data()),
                                  mean_field);
         ++pf_group_;
         // refresh the host copy of the member keys (children were reseeded)
-        if (st[3] != 0)
+        if (any_resampled)
             check(hipMemcpy(keys_.data(), d_keys_.p, B_ * sizeof(uint64_t),
                             hipMemcpyDeviceToHost),
                   "keys download");
@@ -499,12 +531,35 @@ template <typename R> class Engine final : public EngineBase {
     void pf_result(double *log_likelihood, size_t *resampling_events) override {
         if (!pf_on_)
             throw std::invalid_argument("particle filter not started");
-        double st[8];
-        check(hipMemcpy(st, pf_stats_.p, sizeof(st), hipMemcpyDeviceToHost), "filter result");
-        if (log_likelihood)
-            *log_likelihood = st[4];
-        if (resampling_events)
-            *resampling_events = (size_t)st[5];
+        std::vector<double> st(8 * pf_filters_);
+        check(hipStreamSynchronize(stream_), "filter result");
+        check(hipMemcpy(st.data(), pf_stats_.p, st.size() * sizeof(double),
+                        hipMemcpyDeviceToHost),
+              "filter result");
+        for (size_t f = 0; f < pf_filters_; ++f) {
+            if (log_likelihood)
+                log_likelihood[f] = st[8 * f + 4];
+            if (resampling_events)
+                resampling_events[f] = (size_t)st[8 * f + 5];
+        }
+    }
+
+    size_t pf_filters() const override {
+        return pf_on_ ? pf_filters_ : 0;
+    }
+
+    // Upload one field and replicate it into every member on the device.
+    void fill_state(const double *field) override {
+        set_state(0, field);
+        for (size_t m = 1; m < B_; ++m)
+            check(hipMemcpyAsync(w0_.p + m * spec_, w0_.p, spec_ * sizeof(cx<R>),
+                                 hipMemcpyDeviceToDevice, stream_),
+                  "state fill");
+        mix_valid_ = false;
+    }
+
+    void set_step_index(uint64_t step) override {
+        step_ = step;
     }
 
     uint64_t step_index() const override {
@@ -647,7 +702,10 @@ template <typename R> class Engine final : public EngineBase {
     DeviceBuffer<cx<R>> w0_, w1_, acc_, mix_, nl_, ext_, tw_, noise_;
     // particle filter state
     bool pf_on_ = false;
-    uint64_t pf_seed_ = 0, pf_group_ = 0;
+    uint64_t pf_group_ = 0;
+    size_t pf_filters_ = 1;
+    int pf_max_uniforms_ = 0;
+    DeviceBuffer<uint64_t> pf_seeds_;
     size_t pf_obs_cap_ = 0;
     DeviceBuffer<double> pf_wstate_, pf_wnorm_, pf_logw_, pf_stats_, pf_mean_, pf_uniforms_,
         pf_cs_, pf_sn_, pf_ov_, pf_osd_;
@@ -813,6 +871,27 @@ int hzl_pf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, co
 
 int hzl_pf_result(hzl_ensemble_t h, double *log_likelihood, size_t *resampling_events) {
     return guarded([&] { engine_of(h).pf_result(log_likelihood, resampling_events); });
+}
+
+int hzl_pf_begin_filters(hzl_ensemble_t h, const uint64_t *seeds, size_t filters,
+                         size_t max_groups) {
+    return guarded([&] {
+        if (!seeds)
+            throw std::invalid_argument("null seeds");
+        engine_of(h).pf_begin_filters(seeds, filters, max_groups, true);
+    });
+}
+
+int hzl_ensemble_fill_state(hzl_ensemble_t h, const double *field) {
+    return guarded([&] {
+        if (!field)
+            throw std::invalid_argument("null field");
+        engine_of(h).fill_state(field);
+    });
+}
+
+int hzl_ensemble_set_step_index(hzl_ensemble_t h, uint64_t step) {
+    return guarded([&] { engine_of(h).set_step_index(step); });
 }
 
 int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out) {

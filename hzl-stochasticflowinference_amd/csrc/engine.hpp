@@ -82,6 +82,11 @@ class EngineBase {
                                const double *value, const double *sd, bool is_last,
                                double *record, double *mean_field) = 0;
     virtual void pf_result(double *log_likelihood, size_t *resampling_events) = 0;
+    virtual void pf_begin_filters(const uint64_t *seeds, size_t filters, size_t max_groups,
+                                  bool reset_keys) = 0;
+    virtual size_t pf_filters() const = 0;
+    virtual void fill_state(const double *field) = 0;
+    virtual void set_step_index(uint64_t step) = 0;
     virtual uint64_t step_index() const = 0;
     virtual void device_state(void **ptr, size_t *stride) = 0;
     virtual void *stream() = 0;

@@ -782,28 +782,35 @@ HZL_HD void pf_loglik_body(const Ctx &ctx, const PfArgs<R> &a, double *red, int 
 }
 
 struct PfControl {
-    int B;
+    int P;                   // particles per filter; filter f owns members [f P, (f+1) P)
     const double *logw;
     double *wstate, *wnorm;
-    double *stats;           // [0] increment [1] ESS [2] max weight [3] resampled [4] total log-lik
-                             // [5] resampling events [6] uniforms used
-    int *ancestors;
-    const double *uniforms;  // pre-drawn uniforms of the resampling stream
+    double *stats;           // per filter, 8 each: [0] increment [1] ESS [2] max weight
+                             // [3] resampled [4] total log-lik [5] resampling events
+                             // [6] uniforms used
+    int *ancestors;          // global member indices
+    const double *uniforms;  // per filter, max_uniforms pre-drawn resampling uniforms
     int max_uniforms;
     uint64_t *keys;          // member Philox keys (children are reseeded)
-    uint64_t seed;
+    const uint64_t *seeds;   // per filter seed
     uint64_t group;
     int is_last;
 };
 
-// Single workgroup: log-sum-exp normalisation, ESS, systematic resampling
-// below B/2 unless this is the last group (src/filter.cpp:76-120).
+// One workgroup per filter: log-sum-exp normalisation, ESS, systematic
+// resampling below P/2 unless this is the last group (src/filter.cpp:76-120).
 template <class Ctx>
-HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, int nt) {
-    const int tid = ctx.tid();
+HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, int nt, int f) {
+    const int tid = ctx.tid(), P = a.P;
+    const long base = (long)f * P;
+    const double *logw = a.logw + base;
+    double *wnorm = a.wnorm + base, *wstate = a.wstate + base, *stats = a.stats + 8L * f;
+    int *ancestors = a.ancestors + base;
+    uint64_t *keys = a.keys + base;
+    const double *uniforms = a.uniforms + (long)f * a.max_uniforms;
     double mx = -__builtin_huge_val();
-    for (int i = tid; i < a.B; i += nt)
-        mx = a.logw[i] > mx ? a.logw[i] : mx;
+    for (int i = tid; i < P; i += nt)
+        mx = logw[i] > mx ? logw[i] : mx;
     red[tid] = mx;
     ctx.sync();
     for (int s = nt / 2; s > 0; s /= 2) {
@@ -816,8 +823,8 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
     const bool finite = mx > -__builtin_huge_val() && mx < __builtin_huge_val();
     double tot = 0;
     if (finite)
-        for (int i = tid; i < a.B; i += nt)
-            tot += exp(a.logw[i] - mx);
+        for (int i = tid; i < P; i += nt)
+            tot += exp(logw[i] - mx);
     red[tid] = tot;
     ctx.sync();
     for (int s = nt / 2; s > 0; s /= 2) {
@@ -828,9 +835,9 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
     const double inc = finite ? mx + log(red[0]) : -__builtin_huge_val();
     ctx.sync();
     double sq = 0, mw = 0;
-    for (int i = tid; i < a.B; i += nt) {
-        const double w = finite ? exp(a.logw[i] - inc) : 0.0;
-        a.wnorm[i] = w;
+    for (int i = tid; i < P; i += nt) {
+        const double w = finite ? exp(logw[i] - inc) : 0.0;
+        wnorm[i] = w;
         sq += w * w;
         mw = w > mw ? w : mw;
     }
@@ -845,37 +852,37 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
         ctx.sync();
     }
     const double ess = 1 / red[0];
-    const bool resample = finite && ess < (double)a.B / 2 && !a.is_last &&
-                          (int)a.stats[6] < a.max_uniforms;
+    const bool resample = finite && ess < (double)P / 2 && !a.is_last &&
+                          (int)stats[6] < a.max_uniforms;
     if (tid == 0) {
-        a.stats[0] = inc;
-        a.stats[1] = ess;
-        a.stats[2] = red[nt];
-        a.stats[3] = resample ? 1 : 0;
-        a.stats[4] += inc;
+        stats[0] = inc;
+        stats[1] = ess;
+        stats[2] = red[nt];
+        stats[3] = resample ? 1 : 0;
+        stats[4] += inc;
         if (resample) {
             // one uniform per resampling event, strict `position > cumulative`
-            const double u = a.uniforms[(int)a.stats[6]];
-            a.stats[6] += 1;
-            a.stats[5] += 1;
+            const double u = uniforms[(int)stats[6]];
+            stats[6] += 1;
+            stats[5] += 1;
             int anc = 0;
-            double cum = a.wnorm[0];
-            const double start = u / (double)a.B;
-            for (int i = 0; i < a.B; ++i) {
-                const double pos = start + (double)i / (double)a.B;
-                while (pos > cum && anc + 1 < a.B)
-                    cum += a.wnorm[++anc];
-                a.ancestors[i] = anc;
+            double cum = wnorm[0];
+            const double start = u / (double)P;
+            for (int i = 0; i < P; ++i) {
+                const double pos = start + (double)i / (double)P;
+                while (pos > cum && anc + 1 < P)
+                    cum += wnorm[++anc];
+                ancestors[i] = (int)base + anc;
             }
         }
     }
     ctx.sync();
-    for (int i = tid; i < a.B; i += nt) {
-        a.wstate[i] = resample ? 1.0 / (double)a.B : a.wnorm[i];
+    for (int i = tid; i < P; i += nt) {
+        wstate[i] = resample ? 1.0 / (double)P : wnorm[i];
         if (resample)
-            a.keys[i] = stream_seed(a.seed, (a.group + 1) * (uint64_t)a.B + (uint64_t)i);
+            keys[i] = stream_seed(a.seeds[f], (a.group + 1) * (uint64_t)P + (uint64_t)i);
         else
-            a.ancestors[i] = i;
+            ancestors[i] = (int)base + i;
     }
     ctx.sync();
 }
@@ -895,16 +902,6 @@ HZL_HD void pf_mean_body(int B, long spec, const cx<R> *w0, const double *wn, do
     }
     mean[2 * e] = re;
     mean[2 * e + 1] = im;
-}
-
-// Ancestor gather dst[m] = src[anc[m]] when the group resampled.
-template <typename R>
-HZL_HD void pf_gather_body(int B, long spec, const cx<R> *src, cx<R> *dst, const int *anc,
-                           const double *stats, long i) {
-    if (stats[3] == 0 || i >= (long)B * spec)
-        return;
-    const long m = i / spec, e = i % spec;
-    dst[i] = src[(long)anc[m] * spec + e];
 }
 
 } // namespace hzl_amd

@@ -30,7 +30,8 @@ EXPORTS = (
     "hzl_ensemble_step_index",
     "hzl_ensemble_device_state", "hzl_ensemble_stream",
     "hzl_pf_begin", "hzl_pf_assimilate", "hzl_pf_result",
-    "hzl_enkf_begin", "hzl_enkf_assimilate", "hzl_ensemble_mean_spread",
+    "hzl_pf_begin_filters", "hzl_ensemble_fill_state", "hzl_ensemble_set_step_index",
+    "hzl_enkf_begin", "hzl_enkf_assimilate", "hzl_ensemble_mean_spread", "hzl_pmcmc_run",
     "hzl_surrogate_create", "hzl_surrogate_destroy", "hzl_surrogate_param_count",
     "hzl_surrogate_get_params", "hzl_surrogate_set_params", "hzl_surrogate_set_scales",
     "hzl_surrogate_get_scales", "hzl_surrogate_forward", "hzl_surrogate_loss_grad",
@@ -47,6 +48,14 @@ class Parameters(C.Structure):
                 ("forcing_sigma", C.c_double), ("forcing_tau", C.c_double),
                 ("forcing_min", C.c_int), ("forcing_max", C.c_int),
                 ("kolmogorov_amplitude", C.c_double), ("kolmogorov_wavenumber", C.c_int)]
+
+
+class PmcmcOptions(C.Structure):
+    """hzl_pmcmc_options (include/hzl_amd.h)."""
+    _fields_ = [("chains", C.c_size_t), ("samples", C.c_size_t),
+                ("proposal_scale", C.c_double), ("log_lower", C.c_double * 2),
+                ("log_upper", C.c_double * 2), ("friction", C.c_double),
+                ("forcing_sigma", C.c_double), ("seed", C.c_uint64)]
 
 
 class HzlError(RuntimeError):
@@ -93,10 +102,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_pf_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp, _dp,
                                     C.c_int, _dp, _dp]
     L.hzl_pf_result.argtypes = [vp, _dp, C.POINTER(C.c_size_t)]
+    L.hzl_pf_begin_filters.argtypes = [vp, C.POINTER(C.c_uint64), C.c_size_t, C.c_size_t]
+    L.hzl_ensemble_fill_state.argtypes = [vp, _dp]
+    L.hzl_ensemble_set_step_index.argtypes = [vp, C.c_uint64]
     L.hzl_enkf_begin.argtypes = [vp, C.c_uint64, C.c_int]
     L.hzl_enkf_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp, _dp,
                                       C.c_double, C.c_double, _dp, _dp]
     L.hzl_ensemble_mean_spread.argtypes = [vp, _dp, _dp]
+    L.hzl_pmcmc_run.argtypes = [vp, C.POINTER(PmcmcOptions), C.c_double, C.c_size_t, ip, ip, ip,
+                                _dp, _dp, _dp, _dp, _dp, _dp, _dp, ip]
     fp, sz = C.POINTER(C.c_float), C.c_size_t
     L.hzl_surrogate_create.argtypes = [C.POINTER(vp), sz, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_uint64]
@@ -237,13 +251,29 @@ class Ensemble:
     # -- particle filter (members are particles) ---------------------------
     def pf_begin(self, seed: int, max_groups: int = 4096):
         _check(_lib.hzl_pf_begin(self._h, seed, max_groups))
+        self._filters = 1
+
+    def pf_begin_filters(self, seeds, max_groups: int = 4096):
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        _check(_lib.hzl_pf_begin_filters(self._h, s.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         len(s), max_groups))
+        self._filters = len(s)
+
+    def fill_state(self, field):
+        a = _fields(field, 1, self.n)
+        _check(_lib.hzl_ensemble_fill_state(self._h, a.ctypes.data_as(_dp)))
+
+    @step_index.setter
+    def step_index(self, step: int):
+        _check(_lib.hzl_ensemble_set_step_index(self._h, step))
 
     def pf_assimilate(self, dt, steps, x, y, value, sd, is_last, mean=False):
+        """Returns (records, mean); records has 4 entries per filter."""
         x = np.ascontiguousarray(x, dtype=np.int32)
         y = np.ascontiguousarray(y, dtype=np.int32)
         value = np.ascontiguousarray(value, dtype=np.float64)
         sd = np.ascontiguousarray(sd, dtype=np.float64)
-        rec = np.zeros(4)
+        rec = np.zeros(4 * getattr(self, "_filters", 1))
         mf = np.zeros((self.n, self.n), dtype=np.complex128) if mean else None
         ip = C.POINTER(C.c_int)
         _check(_lib.hzl_pf_assimilate(self._h, dt, steps, len(x), x.ctypes.data_as(ip),
@@ -282,10 +312,15 @@ class Ensemble:
         return sp.value, mf
 
     def pf_result(self):
-        ll = C.c_double()
-        ev = C.c_size_t()
-        _check(_lib.hzl_pf_result(self._h, C.byref(ll), C.byref(ev)))
-        return ll.value, ev.value
+        """(log-likelihood, resampling events); arrays with several filters."""
+        F = getattr(self, "_filters", 1)
+        ll = np.zeros(F)
+        ev = np.zeros(F, dtype=np.uint64)
+        _check(_lib.hzl_pf_result(self._h, ll.ctypes.data_as(_dp),
+                                  ev.ctypes.data_as(C.POINTER(C.c_size_t))))
+        if F == 1:
+            return float(ll[0]), int(ev[0])
+        return ll, ev.astype(np.int64)
 
 
 OBSERVATIONS_COLUMNS = "step,x,y,vorticity,standard_deviation"
@@ -420,6 +455,43 @@ def ensemble_kalman_filter(members: Ensemble, observations, dt: float, seed: int
         history.append(dict(step=current + horizon, forecast_sensor_rmse=float("nan"),
                             posterior_spread=sp, mean_field=mean, is_forecast=True))
     return history
+
+
+def particle_metropolis(particles: Ensemble, observations, dt: float, initial_field,
+                        initial_q, samples: int, proposal_scale: float, seed: int,
+                        nu_bounds=(2e-4, 5e-3), amp_bounds=(0.02, 0.5), friction: float = 0.0,
+                        forcing_sigma: float = 0.05):
+    """Batched pseudo-marginal random-walk Metropolis over q = (log nu,
+    log amplitude), one chain per filter of members / chains particles
+    (reference particle_metropolis, src/filter.cpp:166-203, with a
+    log-uniform prior).  initial_q: (chains, 2).  Returns dict(q (chains,
+    samples, 2), log_likelihood, log_density, accepted (chains, samples))."""
+    obs = np.asarray(observations, dtype=np.float64)
+    q0 = np.ascontiguousarray(initial_q, dtype=np.float64).reshape(-1, 2)
+    chains = q0.shape[0]
+    o = PmcmcOptions(chains, samples, proposal_scale,
+                     (C.c_double * 2)(np.log(nu_bounds[0]), np.log(amp_bounds[0])),
+                     (C.c_double * 2)(np.log(nu_bounds[1]), np.log(amp_bounds[1])),
+                     friction, forcing_sigma, seed)
+    step = np.ascontiguousarray(obs[:, 0], dtype=np.int32)
+    x = np.ascontiguousarray(obs[:, 1], dtype=np.int32)
+    y = np.ascontiguousarray(obs[:, 2], dtype=np.int32)
+    val = np.ascontiguousarray(obs[:, 3])
+    sd = np.ascontiguousarray(obs[:, 4])
+    f = _fields(initial_field, 1, particles.n)
+    q = np.zeros((chains, samples, 2))
+    ll = np.zeros((chains, samples))
+    dens = np.zeros((chains, samples))
+    acc = np.zeros((chains, samples), dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    _check(_lib.hzl_pmcmc_run(particles._h, C.byref(o), dt, len(obs), step.ctypes.data_as(ip),
+                              x.ctypes.data_as(ip), y.ctypes.data_as(ip),
+                              val.ctypes.data_as(_dp), sd.ctypes.data_as(_dp),
+                              f.ctypes.data_as(_dp), q0.ctypes.data_as(_dp),
+                              q.ctypes.data_as(_dp), ll.ctypes.data_as(_dp),
+                              dens.ctypes.data_as(_dp), acc.ctypes.data_as(ip)))
+    particles._filters = chains
+    return dict(q=q, log_likelihood=ll, log_density=dens, accepted=acc.astype(bool))
 
 
 _SUR_PRECISION = {"fp32": FP32, "bf16": BF16}

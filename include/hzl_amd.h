@@ -145,16 +145,34 @@ int hzl_pf_begin(hzl_ensemble_t h, uint64_t seed, size_t max_groups);
 /* Advance every particle `steps` RK4 steps (engine forcing), then assimilate
  * one observation group (grid points x, y; values; standard deviations):
  * FP64 Gaussian log-weights, log-sum-exp increment, ESS, maximum weight, and
- * systematic resampling when ESS < N/2 unless is_last.  record[4] receives
- * (increment, ESS, max weight, resampled 0/1); mean_field (optional, reference
- * Field layout) the weighted mean state with this group's weights. */
+ * systematic resampling when ESS < N/2 unless is_last.  record[4 x filters]
+ * receives per filter (increment, ESS, max weight, resampled 0/1);
+ * mean_field (optional, single filter only, reference Field layout) the
+ * weighted mean state with this group's weights. */
 int hzl_pf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, const int *x,
                       const int *y, const double *value, const double *sd, int is_last,
                       double *record, double *mean_field);
 
 /* Accumulated log-likelihood (sum of increments; -inf once every weight was
- * -inf, src/filter.cpp:76-80) and number of resampling events. */
+ * -inf, src/filter.cpp:76-80) and number of resampling events, one entry per
+ * filter. */
 int hzl_pf_result(hzl_ensemble_t h, double *log_likelihood, size_t *resampling_events);
+
+/* `filters` independent particle filters over contiguous member segments of
+ * members / filters particles (one filter per chain of a batched particle
+ * MCMC, src/filter.cpp:166-203).  Filter f behaves as hzl_pf_begin(seeds[f])
+ * on its own particles, whose noise keys restart at stream_seed(seeds[f], i)
+ * (the reference's per-particle Solver(n, p, stream_seed(seed, i)),
+ * src/filter.cpp:44-48). */
+int hzl_pf_begin_filters(hzl_ensemble_t h, const uint64_t *seeds, size_t filters,
+                         size_t max_groups);
+
+/* Upload one state (reference Field layout) into every member. */
+int hzl_ensemble_fill_state(hzl_ensemble_t h, const double *field);
+
+/* Set the step counter (the noise counter of the next step), e.g. back to 0
+ * when a likelihood evaluation restarts from the initial state. */
+int hzl_ensemble_set_step_index(hzl_ensemble_t h, uint64_t step);
 
 /* ------------------------------------------------------------------------
  * Ensemble Kalman filter (members are the ensemble), reference
@@ -181,6 +199,34 @@ int hzl_enkf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, 
 /* Ensemble mean (reference Field layout, optional) and spread, as recorded
  * by the reference's free-forecast phase (src/filter.cpp:333-355). */
 int hzl_ensemble_mean_spread(hzl_ensemble_t h, double *spread, double *mean_field);
+
+/* ------------------------------------------------------------------------
+ * Batched particle-marginal Metropolis-Hastings, reference
+ * particle_metropolis (src/filter.cpp:166-203), one chain per filter segment.
+ * ------------------------------------------------------------------------ */
+typedef struct hzl_pmcmc_options {
+    size_t chains;         /* members / chains particles per chain's filter */
+    size_t samples;        /* Metropolis steps per chain */
+    double proposal_scale; /* random-walk step on (log nu, log amplitude) */
+    double log_lower[2];   /* log-uniform prior bounds of nu and amplitude */
+    double log_upper[2];
+    double friction;       /* fixed alpha and forcing sigma of every member */
+    double forcing_sigma;
+    uint64_t seed;         /* chain c uses stream_seed(seed, c) */
+} hzl_pmcmc_options;
+
+/* Run every chain `samples` steps over one window of observations (rows with
+ * step >= 1, time ordered), every likelihood evaluation restarting all
+ * particles from initial_field at step 0.  initial_q[2 x chains] holds each
+ * chain's starting (log nu, log amplitude).  Outputs per chain c and sample s
+ * at index c * samples + s: samples_q[2 x ...] the chain state after the step,
+ * samples_ll its retained likelihood estimate, samples_density (optional)
+ * log-likelihood + log-prior, samples_accepted (optional) 0/1. */
+int hzl_pmcmc_run(hzl_ensemble_t particles, const hzl_pmcmc_options *options, double dt,
+                  size_t nobs, const int *step, const int *x, const int *y, const double *value,
+                  const double *sd, const double *initial_field, const double *initial_q,
+                  double *samples_q, double *samples_ll, double *samples_density,
+                  int *samples_accepted);
 
 /* Number of steps taken so far (the noise counter of the next step). */
 int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out);

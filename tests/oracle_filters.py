@@ -119,3 +119,48 @@ def oracle_enkf(orc, initial, obs, dt, ens_seed, filter_seed, nu, alpha=0.0, amp
                             mean_field=mu, states=states.copy()))
         current = int(target)
     return history
+
+
+def oracle_pmcmc(orc, initial, obs, dt, q0, samples, scale, seed, lower, upper, particles,
+                 alpha=0.0, sigma=0.05, kf=4, kmin=1, kmax=4):
+    """Reference particle_metropolis (src/filter.cpp:166-203) per chain with the
+    engine's forcing model and a log-uniform prior on (nu, amplitude): chain c
+    seeds NormalStream(stream_seed(seed, c)) for proposals and
+    mt19937_64(stream_seed(stream_seed(seed, c), 1)) for likelihood seeds and
+    acceptance uniforms."""
+    n = initial.shape[0]
+    K = (n - 1) // 3
+    init = np.stack([initial] * particles)
+
+    def log_prior(q):
+        ok = all(np.isfinite(q[i]) and lower[i] <= q[i] <= upper[i] for i in range(2))
+        return 0.0 if ok else -np.inf
+
+    def like(q, lseed):
+        if log_prior(q) == -np.inf or dt * (2 * np.exp(q[0]) * K * K + alpha) > 2.5:
+            return -np.inf
+        return oracle_bootstrap_filter(orc, init, obs, dt, lseed, np.exp(q[0]), alpha=alpha,
+                                       amp=np.exp(q[1]), kf=kf, sigma=sigma, kmin=kmin,
+                                       kmax=kmax)["log_likelihood"]
+
+    out = []
+    for c, qc in enumerate(np.asarray(q0, dtype=np.float64).reshape(-1, 2)):
+        s = orc.stream_seed(seed, c)
+        normal, random = orc.mt(s), orc.mt(orc.stream_seed(s, 1))
+        cur = qc.copy()
+        lseed = orc.mt_next(random)
+        ll = like(cur, lseed)
+        dens = ll + log_prior(cur)
+        rows = []
+        for _ in range(samples):
+            prop = cur + scale * np.array([orc.normal(normal), orc.normal(normal)])
+            pseed = orc.mt_next(random)
+            pll = like(prop, pseed)
+            pd = pll + log_prior(prop)
+            u = orc.mt_uniform(random)
+            acc = bool(np.isfinite(pd) and np.log(max(u, np.finfo(float).tiny)) < min(0.0, pd - dens))
+            if acc:
+                cur, ll, dens, lseed = prop, pll, pd, pseed
+            rows.append((cur.copy(), ll, dens, acc))
+        out.append(rows)
+    return out
