@@ -165,18 +165,23 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Member mean per mode, summed in member order (src/filter.cpp:254-258).
+// Member mean per mode in two passes: partial[c][e] = sum over the members
+// of chunk c of w_i[e] / B, then mean[e] = sum_c partial[c][e]
+// (src/filter.cpp:254-258 sums the same terms in member order).
 template <typename R>
-__global__ void k_enkf_mean(const cx<R> *w, int B, long spec, cx<double> *mean) {
+__global__ void k_enkf_mean_partial(const cx<R> *w, int B, long spec, int chunks,
+                                    cx<double> *partial) {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
     if (e >= spec)
         return;
+    const int per = (B + chunks - 1) / chunks, lo = c * per, hi = lo + per < B ? lo + per : B;
     double re = 0, im = 0;
-    for (int i = 0; i < B; ++i) {
+    for (int i = lo; i < hi; ++i) {
         re += (double)w[(long)i * spec + e].x / B;
         im += (double)w[(long)i * spec + e].y / B;
     }
-    mean[e] = {re, im};
+    partial[(long)c * spec + e] = {re, im};
 }
 
 // Multiplicative anomaly inflation w = mean + lambda (w - mean).
@@ -192,21 +197,35 @@ __global__ void k_enkf_inflate(cx<R> *w, int B, long spec, const cx<double> *mea
             (R)(mu.y + lambda * ((double)w[i].y - mu.y))};
 }
 
-// (c > 0 ? 2 : 1) sum_i |w_i[e] - mean[e]|^2: the full-field sum of the
-// reference's posterior variance, per retained half-plane mode.
+// partial[c][e] = (kx > 0 ? 2 : 1) sum over the members of chunk c of
+// |w_i[e] - mean[e]|^2: summed over chunks and modes, the full-field sum of
+// the reference's posterior variance (src/filter.cpp:343-347).
 template <typename R>
-__global__ void k_enkf_spread(const cx<R> *w, int B, long spec, int K, const cx<double> *mean,
-                              double *out) {
+__global__ void k_enkf_spread_partial(const cx<R> *w, int B, long spec, int K,
+                                      const cx<double> *mean, int chunks, double *partial) {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
     if (e >= spec)
         return;
+    const int per = (B + chunks - 1) / chunks, lo = c * per, hi = lo + per < B ? lo + per : B;
     double s = 0;
-    for (int i = 0; i < B; ++i) {
+    for (int i = lo; i < hi; ++i) {
         const double dr = (double)w[(long)i * spec + e].x - mean[e].x;
         const double di = (double)w[(long)i * spec + e].y - mean[e].y;
         s += dr * dr + di * di;
     }
-    out[e] = (e / (2 * K + 1) > 0 ? 2.0 : 1.0) * s;
+    partial[(long)c * spec + e] = (e / (2 * K + 1) > 0 ? 2.0 : 1.0) * s;
+}
+
+// out[e] = sum_s part[s * count + e]
+__global__ void k_sum_slices(const float *part, int slices, long count, float *out) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= count)
+        return;
+    float s = 0;
+    for (int k = 0; k < slices; ++k)
+        s += part[(long)k * count + e];
+    out[e] = s;
 }
 
 // Batch mode: per observation column o (block), HA[i][o] = HX[i][o] - mean_o
@@ -270,9 +289,9 @@ struct EnkfState : ModuleState {
     int mode;
     NormalStream normals;
     Buf<cx<double>> ex, ey, partial, cov, mean;
-    Buf<double> pred, pred2, delta, gain, eps, spread, value, sd;
+    Buf<double> pred, pred2, delta, gain, eps, spread, value, sd, spart;
     // batch mode (FP32)
-    Buf<float> H, HX, HA, P, S, Sinv, Dm, Z;
+    Buf<float> H, HX, HA, P, S, Sinv, Dm, Z, HXpart;
     EnkfState(int mode_, uint64_t seed) : mode(mode_), normals(seed) {}
 };
 
@@ -336,26 +355,32 @@ template <typename R> struct Enkf {
         w = reinterpret_cast<cx<R> *>(ev.w0);
     }
 
+    static constexpr int CHUNKS = 64; // member chunks of the two-pass reductions
+
     void mean() {
         st.mean.alloc((size_t)spec);
-        hipLaunchKernelGGL(k_enkf_mean<R>, grid1d(spec), dim3(256), 0, ev.stream, w, ev.B, spec,
-                           st.mean.p);
+        st.partial.alloc((size_t)CHUNKS * spec);
+        hipLaunchKernelGGL(k_enkf_mean_partial<R>, dim3((unsigned)((spec + 255) / 256), CHUNKS),
+                           dim3(256), 0, ev.stream, w, ev.B, spec, CHUNKS, st.partial.p);
+        hipLaunchKernelGGL(k_enkf_cov_final, grid1d(spec), dim3(256), 0, ev.stream, st.partial.p,
+                           CHUNKS, spec, st.mean.p);
     }
 
     // posterior spread (after mean()), src/filter.cpp:343-347
     double spread() {
-        st.spread.alloc((size_t)spec);
-        hipLaunchKernelGGL(k_enkf_spread<R>, grid1d(spec), dim3(256), 0, ev.stream, w, ev.B, spec,
-                           K, st.mean.p, st.spread.p);
-        std::vector<double> h((size_t)spec);
-        hcheck(hipMemcpyAsync(h.data(), st.spread.p, spec * sizeof(double), hipMemcpyDeviceToHost,
-                              ev.stream),
+        st.spart.alloc((size_t)CHUNKS * spec);
+        hipLaunchKernelGGL(k_enkf_spread_partial<R>, dim3((unsigned)((spec + 255) / 256), CHUNKS),
+                           dim3(256), 0, ev.stream, w, ev.B, spec, K, st.mean.p, CHUNKS,
+                           st.spart.p);
+        std::vector<double> h((size_t)CHUNKS * spec);
+        hcheck(hipMemcpyAsync(h.data(), st.spart.p, h.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, ev.stream),
                "spread download");
         hcheck(hipStreamSynchronize(ev.stream), "spread");
         double s = 0;
         for (double v : h)
-            s += v / (ev.B - 1);
-        return std::sqrt(s);
+            s += v;
+        return std::sqrt(s / (ev.B - 1));
     }
 
     void mean_field(double *field) {
@@ -423,7 +448,7 @@ template <typename R> struct Enkf {
     }
 
     void serial(size_t nobs, const double *value, const double *sd) {
-        const int B = ev.B, chunks = 16;
+        const int B = ev.B, chunks = CHUNKS;
         st.pred.alloc((size_t)B);
         st.pred2.alloc((size_t)B);
         st.delta.alloc((size_t)B);
@@ -490,9 +515,26 @@ template <> void Enkf<float>::batch(size_t nobs, const double *value, const doub
     st.Z.alloc((size_t)m * B);
     const Span<const float> X{reinterpret_cast<const float *>(w), (size_t)B * D};
     const Span<void> Xw{reinterpret_cast<float *>(w), (size_t)B * D};
-    // HX = X H^T
-    Gemm<float> g1(B, m, (int)D, X, D, 1, cs(st.H), D, 1, vs(st.HX), m, 1, OUT_F32);
-    g1.run(ev.stream);
+    // HX = X H^T, split along the long K = 2 spec axis: full slices of KC, a
+    // remainder slice, then a sum over slices
+    {
+        const long KC = 256, full = D / KC, rest = D - full * KC, slices = full + (rest ? 1 : 0);
+        const long count = (long)B * m;
+        st.HXpart.alloc((size_t)(slices * count));
+        if (full) {
+            Gemm<float> g1(B, m, (int)KC, X, D, 1, cs(st.H), D, 1, vs(st.HXpart), m, 1, OUT_F32);
+            g1.batch((int)full, KC, KC, count);
+            g1.run(ev.stream);
+        }
+        if (rest) {
+            const Span<const float> Xr{X.p + full * KC, X.n - full * KC};
+            Gemm<float> g1r(B, m, (int)rest, Xr, D, 1, cs(st.H, full * KC), D, 1,
+                            vs(st.HXpart, full * count), m, 1, OUT_F32);
+            g1r.run(ev.stream);
+        }
+        hipLaunchKernelGGL(k_sum_slices, grid1d(count), dim3(256), 0, ev.stream, st.HXpart.p,
+                           (int)slices, count, st.HX.p);
+    }
     hipLaunchKernelGGL(k_enkf_batch_anomalies, dim3((unsigned)m), dim3(256), 0, ev.stream,
                        st.HX.p, B, m, st.value.p, st.sd.p, st.eps.p, st.HA.p, st.Dm.p);
     // P = X^T HA / (M - 1)
