@@ -14,6 +14,9 @@
 //     S = (HX)'^T (HX)' / (M - 1) + diag(sd^2),
 //     with the four dense products on MFMA (FP32 operands, FP32 accumulation)
 //     and the m x m Cholesky factorisation and inverse in FP64 on the host.
+// Serial mode optionally tapers each covariance with the Gaspari-Cohn
+// function of the periodic grid distance to the observation, in physical
+// space (src/filter.cpp:290-308).
 // Observation perturbations eps come from the reference NormalStream
 // (mt19937_64 seeded stream_seed(seed, M), Box-Muller with a cached second
 // draw; src/solver.cpp:41-54), consumed observation-major, member-minor as in
@@ -228,6 +231,94 @@ __global__ void k_sum_slices(const float *part, int slices, long count, float *o
     out[e] = s;
 }
 
+// Gaspari-Cohn fifth-order compactly supported correlation (Gaspari & Cohn
+// 1999, eq. 4.10) of half-support `radius`, the taper of src/filter.cpp:25-36.
+HZL_HD double gaspari_cohn(double d, double radius) {
+    if (radius <= 0)
+        return 1;
+    const double x = d / radius;
+    if (x >= 2)
+        return 0;
+    if (x <= 1)
+        return 1 - 5.0 / 3 * x * x + 5.0 / 8 * x * x * x + 0.5 * x * x * x * x -
+               0.25 * x * x * x * x * x;
+    return 4 - 5 * x + 5.0 / 3 * x * x + 5.0 / 8 * x * x * x - 0.5 * x * x * x * x +
+           x * x * x * x * x / 12 - 2.0 / (3 * x);
+}
+
+// Localisation of the covariance (retained half plane, FP64) around the
+// observation (xo, yo), as src/filter.cpp:290-308: to physical space,
+// multiply by the taper of the periodic grid distance, back to spectral.
+// Four separable DFT passes over the (K+1) x n mixed field; cs/sn are
+// cos/sin(2 pi j / n).
+__global__ void k_loc_inv_y(const cx<double> *cov, int K, int n, const double *cs,
+                            const double *sn, cx<double> *mixed) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)(K + 1) * n)
+        return;
+    const int c = (int)(i / n), y = (int)(i % n), J = 2 * K + 1;
+    double re = 0, im = 0;
+    for (int j = 0; j < J; ++j) {
+        const int t = (int)((((long)(j - K) * y) % n + n) % n);
+        const cx<double> v = cov[(long)c * J + j];
+        re += v.x * cs[t] - v.y * sn[t];
+        im += v.x * sn[t] + v.y * cs[t];
+    }
+    mixed[i] = {re, im};
+}
+
+__global__ void k_loc_inv_x_taper(const cx<double> *mixed, int K, int n, const double *cs,
+                                  const double *sn, int xo, int yo, double radius,
+                                  double *phys) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)n * n)
+        return;
+    const int y = (int)(i / n), x = (int)(i % n);
+    double v = 0;
+    for (int c = 0; c <= K; ++c) {
+        const int t = (int)(((long)c * x) % n);
+        const cx<double> m = mixed[(long)c * n + y];
+        v += (c ? 2.0 : 1.0) * (m.x * cs[t] - m.y * sn[t]);
+    }
+    const int ax = x > xo ? x - xo : xo - x, ay = y > yo ? y - yo : yo - y;
+    const double dx = ax < n - ax ? ax : n - ax, dy = ay < n - ay ? ay : n - ay;
+    phys[i] = v * gaspari_cohn(sqrt(dx * dx + dy * dy), radius);
+}
+
+__global__ void k_loc_fwd_x(const double *phys, int K, int n, const double *cs, const double *sn,
+                            cx<double> *mixed) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)(K + 1) * n)
+        return;
+    const int c = (int)(i / n), y = (int)(i % n);
+    double re = 0, im = 0;
+    for (int x = 0; x < n; ++x) {
+        const int t = (int)(((long)c * x) % n);
+        const double v = phys[(long)y * n + x];
+        re += v * cs[t];
+        im -= v * sn[t];
+    }
+    mixed[i] = {re, im};
+}
+
+__global__ void k_loc_fwd_y(const cx<double> *mixed, int K, int n, const double *cs,
+                            const double *sn, cx<double> *cov) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int J = 2 * K + 1;
+    if (e >= (long)(K + 1) * J)
+        return;
+    const int c = (int)(e / J), j = (int)(e % J);
+    double re = 0, im = 0;
+    for (int y = 0; y < n; ++y) {
+        const int t = (int)((((long)(j - K) * y) % n + n) % n);
+        const cx<double> v = mixed[(long)c * n + y];
+        re += v.x * cs[t] + v.y * sn[t];
+        im += v.y * cs[t] - v.x * sn[t];
+    }
+    const double z = 1.0 / ((double)n * n);
+    cov[e] = (c == 0 && j == K) ? cx<double>{0, 0} : cx<double>{re * z, im * z};
+}
+
 // Batch mode: per observation column o (block), HA[i][o] = HX[i][o] - mean_o
 // and Dm[o][i] = y_o + sd_o eps[o][i] - HX[i][o].
 __global__ void __launch_bounds__(256)
@@ -289,7 +380,8 @@ struct EnkfState : ModuleState {
     int mode;
     NormalStream normals;
     Buf<cx<double>> ex, ey, partial, cov, mean;
-    Buf<double> pred, pred2, delta, gain, eps, spread, value, sd, spart;
+    Buf<double> pred, pred2, delta, gain, eps, spread, value, sd, spart, cs, sn, phys;
+    Buf<cx<double>> mixed;
     // batch mode (FP32)
     Buf<float> H, HX, HA, P, S, Sinv, Dm, Z, HXpart;
     EnkfState(int mode_, uint64_t seed) : mode(mode_), normals(seed) {}
@@ -393,7 +485,8 @@ template <typename R> struct Enkf {
     }
 
     void assimilate(size_t nobs, const int *x, const int *y, const double *value,
-                    const double *sd, double inflation, double *record, double *mean_out) {
+                    const double *sd, double inflation, double radius, double *record,
+                    double *mean_out) {
         const int B = ev.B, n = ev.n;
         // point-evaluation tables of every observation
         std::vector<cx<double>> ex(nobs * (K + 1)), ey(nobs * J);
@@ -436,7 +529,7 @@ template <typename R> struct Enkf {
             e = st.normals();
         upload(st.eps, eps.data(), eps.size(), ev.stream);
         if (st.mode == HZL_ENKF_SERIAL)
-            serial(nobs, value, sd);
+            serial(nobs, x, y, value, sd, radius);
         else
             batch(nobs, value, sd);
         hcheck(hipGetLastError(), "EnKF kernels");
@@ -447,7 +540,32 @@ template <typename R> struct Enkf {
             mean_field(mean_out);
     }
 
-    void serial(size_t nobs, const double *value, const double *sd) {
+    void localize(int xo, int yo, double radius) {
+        const int n = ev.n;
+        if (!st.cs.p || st.cs.n < (size_t)n) {
+            std::vector<double> c(n), sv(n);
+            for (int j = 0; j < n; ++j) {
+                c[j] = std::cos(2 * PI * j / n);
+                sv[j] = std::sin(2 * PI * j / n);
+            }
+            upload(st.cs, c.data(), n, ev.stream);
+            upload(st.sn, sv.data(), n, ev.stream);
+        }
+        const long mix = (long)(K + 1) * n;
+        st.mixed.alloc((size_t)mix);
+        st.phys.alloc((size_t)n * n);
+        hipLaunchKernelGGL(k_loc_inv_y, grid1d(mix), dim3(256), 0, ev.stream, st.cov.p, K, n,
+                           st.cs.p, st.sn.p, st.mixed.p);
+        hipLaunchKernelGGL(k_loc_inv_x_taper, grid1d((long)n * n), dim3(256), 0, ev.stream,
+                           st.mixed.p, K, n, st.cs.p, st.sn.p, xo, yo, radius, st.phys.p);
+        hipLaunchKernelGGL(k_loc_fwd_x, grid1d(mix), dim3(256), 0, ev.stream, st.phys.p, K, n,
+                           st.cs.p, st.sn.p, st.mixed.p);
+        hipLaunchKernelGGL(k_loc_fwd_y, grid1d(spec), dim3(256), 0, ev.stream, st.mixed.p, K, n,
+                           st.cs.p, st.sn.p, st.cov.p);
+    }
+
+    void serial(size_t nobs, const int *x, const int *y, const double *value, const double *sd,
+                double radius) {
         const int B = ev.B, chunks = CHUNKS;
         st.pred.alloc((size_t)B);
         st.pred2.alloc((size_t)B);
@@ -466,6 +584,8 @@ template <typename R> struct Enkf {
                                st.partial.p);
             hipLaunchKernelGGL(k_enkf_cov_final, grid1d(spec), dim3(256), 0, ev.stream,
                                st.partial.p, chunks, spec, st.cov.p);
+            if (radius > 0)
+                localize(x[o], y[o], radius);
             const bool more = o + 1 < nobs;
             hipLaunchKernelGGL(k_enkf_update<R>, dim3((unsigned)B), dim3(256), 0, ev.stream, w,
                                spec, K, st.cov.p, st.gain.p,
@@ -616,8 +736,8 @@ int hzl_enkf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, 
             localization_radius < 0)
             throw std::invalid_argument(
                 "EnKF inflation must be >= 1 and localization radius nonnegative");
-        if (localization_radius > 0)
-            throw std::invalid_argument("EnKF localization is not supported by the GPU analysis");
+        if (localization_radius > 0 && st.mode == HZL_ENKF_BATCH)
+            throw std::invalid_argument("EnKF localization needs the serial analysis");
         const hzl_amd::EngineViews v = e.views();
         if (st.mode == HZL_ENKF_BATCH && nobs > 1024)
             throw std::invalid_argument("batch EnKF analysis takes at most 1024 observations");
@@ -629,11 +749,11 @@ int hzl_enkf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, 
         if (steps)
             e.step(dt, steps);
         if (v.precision == HZL_FP32)
-            hzl_amd::Enkf<float>(e, st).assimilate(nobs, x, y, value, sd, inflation, record,
-                                                   mean_field);
+            hzl_amd::Enkf<float>(e, st).assimilate(nobs, x, y, value, sd, inflation,
+                                                   localization_radius, record, mean_field);
         else
-            hzl_amd::Enkf<double>(e, st).assimilate(nobs, x, y, value, sd, inflation, record,
-                                                    mean_field);
+            hzl_amd::Enkf<double>(e, st).assimilate(nobs, x, y, value, sd, inflation,
+                                                    localization_radius, record, mean_field);
     });
 }
 

@@ -18,7 +18,7 @@
 //     process; with forcing_sigma = 0 every function here reproduces the
 //     reference's deterministic results (filters, EnKF, ensemble diagnostics);
 //   * the EnKF analysis updates the vorticity only (white forcing carries no
-//     latent state) and rejects localization_radius > 0;
+//     latent state);
 //   * CudaSolver's CFL check is read when the step's results are read
 //     (synchronize / coefficients) instead of after every step.
 // hzl::Ensemble (GPU batches of members) has no reference counterpart: it
@@ -172,7 +172,7 @@ struct EnsembleFilterRecord {
 };
 struct EnKFOptions {
     double inflation = 1.0;
-    double localization_radius = 0.0; // > 0 is rejected by the GPU analysis
+    double localization_radius = 0.0; // Gaspari-Cohn half support, grid cells (serial)
     std::size_t forecast_steps = 0;
     std::size_t forecast_every = 1;
 };

@@ -190,8 +190,10 @@ int hzl_enkf_begin(hzl_ensemble_t h, uint64_t seed, int mode);
  * assimilate one observation group.  record[2] receives (forecast sensor
  * RMSE of the ensemble mean before the analysis, posterior spread
  * sqrt(sum |w - mean|^2 / (M - 1)) over the full field); mean_field
- * (optional, reference Field layout) the analysis mean.  Localization
- * (radius > 0) is rejected: the analysis has no covariance taper. */
+ * (optional, reference Field layout) the analysis mean.  With
+ * localization_radius > 0 (serial mode) every covariance is tapered by the
+ * Gaspari-Cohn function of half-support radius grid cells around its
+ * observation (src/filter.cpp:25-36,290-308). */
 int hzl_enkf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, const int *x,
                         const int *y, const double *value, const double *sd, double inflation,
                         double localization_radius, double *record, double *mean_field);

@@ -72,8 +72,38 @@ def symmetrise(field):
     return out
 
 
+def gaspari_cohn(d, radius):
+    """Gaspari & Cohn (1999) eq. 4.10, half support `radius` (src/filter.cpp:25-36)."""
+    x = np.asarray(d, dtype=np.float64) / radius
+    a = 1 - 5 / 3 * x**2 + 5 / 8 * x**3 + 0.5 * x**4 - 0.25 * x**5
+    with np.errstate(divide="ignore", invalid="ignore"):
+        b = 4 - 5 * x + 5 / 3 * x**2 + 5 / 8 * x**3 - 0.5 * x**4 + x**5 / 12 - 2 / (3 * x)
+    return np.where(x <= 1, a, np.where(x < 2, b, 0.0))
+
+
+def taper_field(n, xo, yo, radius):
+    g = np.arange(n)
+    dx = np.minimum(np.abs(g - xo), n - np.abs(g - xo))
+    dy = np.minimum(np.abs(g - yo), n - np.abs(g - yo))
+    return gaspari_cohn(np.hypot(dx[None, :], dy[:, None]), radius)
+
+
+def localized_scalar_update(orc, states, h, y, sd, eps, xo, yo, radius):
+    """Reference scalar update with the covariance tapered in physical space
+    (src/filter.cpp:276-319)."""
+    P, n = states.shape[0], states.shape[1]
+    X = states.reshape(P, -1)
+    d = h - h.mean()
+    var = np.sum(d * d) / (P - 1)
+    cov = np.sum((X - X.mean(0)) * (d / (P - 1))[:, None], axis=0).reshape(n, n)
+    phys = orc.fft2d(cov, inverse=True) * taper_field(n, xo, yo, radius)
+    cov = orc.fft2d(phys) * orc.active_mask(n)
+    g = (y + sd * eps - h) / (var + sd * sd)
+    return (X + g[:, None] * cov.ravel()[None, :]).reshape(P, n, n)
+
+
 def oracle_enkf(orc, initial, obs, dt, ens_seed, filter_seed, nu, alpha=0.0, amp=0.1, kf=4,
-                sigma=0.05, kmin=1, kmax=4, inflation=1.0, mode="serial"):
+                sigma=0.05, kmin=1, kmax=4, inflation=1.0, mode="serial", radius=0.0):
     """Perturbed-observation EnKF in FP64 on full spectra, restating reference
     ensemble_kalman_filter (src/filter.cpp:204-356) with the engine's forcing
     model; mode "batch" applies the dense all-observation analysis instead of
@@ -100,8 +130,13 @@ def oracle_enkf(orc, initial, obs, dt, ens_seed, filter_seed, nu, alpha=0.0, amp
         if mode == "serial":
             for o, row in enumerate(sel):
                 h = np.array([physical(orc, s)[int(row[2]), int(row[1])] for s in states])
-                flat = orc.enkf_scalar_update(states.reshape(P, -1), h, row[3], row[4], eps[o])
-                states = np.stack([symmetrise(f) for f in flat.reshape(P, n, n)])
+                if radius > 0:
+                    upd = localized_scalar_update(orc, states, h, row[3], row[4], eps[o],
+                                                  int(row[1]), int(row[2]), radius)
+                else:
+                    upd = orc.enkf_scalar_update(states.reshape(P, -1), h, row[3], row[4],
+                                                 eps[o]).reshape(P, n, n)
+                states = np.stack([symmetrise(f) for f in upd])
         else:
             X = np.concatenate([states.reshape(P, -1).real, states.reshape(P, -1).imag], 1)
             HX = np.stack([physical(orc, s)[ys, xs] for s in states])  # P x m

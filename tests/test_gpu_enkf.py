@@ -76,6 +76,25 @@ def test_enkf_serial_matches_oracle(pkg, orc, precision, tol):
     assert want[-1]["posterior_spread"] > 0
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
+def test_enkf_localized_matches_oracle(pkg, orc, precision, tol):
+    """Gaspari-Cohn tapered covariances (src/filter.cpp:290-308), radius 5 cells."""
+    n, P, nu, dt, ens_seed, seed = 32, 16, 2e-3, 2e-3, 31, 77
+    rows, init = setup(orc, n, P, nu, dt)
+    want = oracle_enkf(orc, init, rows, dt, ens_seed, seed, nu, sigma=0.2, amp=0.1, radius=5.0)
+    plain = oracle_enkf(orc, init, rows, dt, ens_seed, seed, nu, sigma=0.2, amp=0.1)
+    with pkg.Ensemble(n, P, precision, nu=nu, sigma=0.2, amp=0.1, kf=4, seed=ens_seed) as ens:
+        ens.set_state(init)
+        got = pkg.ensemble_kalman_filter(ens, rows, dt, seed, localization_radius=5.0)
+        final = ens.get_state()
+    for a, b in zip(got, want):
+        assert abs(a["posterior_spread"] - b["posterior_spread"]) <= tol * b["posterior_spread"]
+        assert rel(a["mean_field"], b["mean_field"]) < tol
+    assert rel(final, want[-1]["states"]) < tol
+    # the taper changed the analysis
+    assert rel(want[-1]["mean_field"], plain[-1]["mean_field"]) > 1e-3
+
+
 def test_enkf_batch_matches_dense_closed_form(pkg, orc):
     n, P, nu, dt, ens_seed, seed = 32, 16, 2e-3, 2e-3, 31, 77
     rows, init = setup(orc, n, P, nu, dt)
@@ -105,8 +124,6 @@ def test_enkf_free_forecast_records_and_errors(pkg, orc):
         assert all(h["is_forecast"] for h in hist[1:]) and np.isnan(hist[1]["forecast_sensor_rmse"])
         sp, mean = ens.mean_spread()
         assert abs(sp - hist[-1]["posterior_spread"]) < 1e-12 * sp
-        with pytest.raises(pkg.HzlInvalidArgument):
-            ens.enkf_assimilate(2e-3, 0, [1], [1], [0.0], [0.1], radius=2.0)
         with pytest.raises(pkg.HzlInvalidArgument):
             ens.enkf_assimilate(2e-3, 0, [1], [1], [0.0], [0.1], inflation=0.5)
         with pytest.raises(pkg.HzlInvalidArgument):
