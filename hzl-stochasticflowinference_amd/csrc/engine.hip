@@ -544,6 +544,52 @@ template <typename R> class Engine final : public EngineBase {
         }
     }
 
+    // Log-weights of every member for one observation group, with running
+    // weights `wprev` and output `logw` in caller-owned device memory (a
+    // filter spanning several ensembles), on this ensemble's stream.
+    void pf_logweights(size_t nobs, const int *x, const int *y, const double *value,
+                       const double *sd, const double *wprev, double *logw) override {
+        if (nobs == 0)
+            throw std::invalid_argument("observation group is empty");
+        for (size_t o = 0; o < nobs; ++o)
+            if (x[o] < 0 || y[o] < 0 || x[o] >= n_ || y[o] >= n_ || !std::isfinite(value[o]) ||
+                !std::isfinite(sd[o]) || sd[o] <= 0)
+                throw std::invalid_argument("observations must be finite, in bounds, and have "
+                                            "positive uncertainty");
+        flush_parameters();
+        if (!mix_valid_) {
+            dispatch_col<R>(n_, 0, stage_args(1.0, nullptr, 0), stream_);
+            mix_valid_ = true;
+        }
+        if (pf_cs_.count != (size_t)n_) {
+            std::vector<double> cs(n_), sn(n_);
+            for (int j = 0; j < n_; ++j) {
+                cs[j] = std::cos(2 * PI * j / n_);
+                sn[j] = std::sin(2 * PI * j / n_);
+            }
+            pf_cs_ = DeviceBuffer<double>(n_);
+            pf_sn_ = DeviceBuffer<double>(n_);
+            upload(pf_cs_.p, cs.data(), n_);
+            upload(pf_sn_.p, sn.data(), n_);
+        }
+        if (pf_obs_cap_ < nobs) {
+            pf_ox_ = DeviceBuffer<int>(nobs);
+            pf_oy_ = DeviceBuffer<int>(nobs);
+            pf_ov_ = DeviceBuffer<double>(nobs);
+            pf_osd_ = DeviceBuffer<double>(nobs);
+            pf_obs_cap_ = nobs;
+        }
+        upload(pf_ox_.p, x, nobs);
+        upload(pf_oy_.p, y, nobs);
+        upload(pf_ov_.p, value, nobs);
+        upload(pf_osd_.p, sd, nobs);
+        PfArgs<R> a{(int)B_,  n_,       K_,       mix_.p,    (int)nobs,
+                    pf_ox_.p, pf_oy_.p, pf_ov_.p, pf_osd_.p, pf_cs_.p,
+                    pf_sn_.p, const_cast<double *>(wprev), nullptr, logw, nullptr};
+        hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
+        check(hipGetLastError(), "particle log-weights");
+    }
+
     size_t pf_filters() const override {
         return pf_on_ ? pf_filters_ : 0;
     }

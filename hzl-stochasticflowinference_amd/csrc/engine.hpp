@@ -85,6 +85,8 @@ class EngineBase {
     virtual void pf_begin_filters(const uint64_t *seeds, size_t filters, size_t max_groups,
                                   bool reset_keys) = 0;
     virtual size_t pf_filters() const = 0;
+    virtual void pf_logweights(size_t nobs, const int *x, const int *y, const double *value,
+                               const double *sd, const double *wprev, double *logw) = 0;
     virtual void fill_state(const double *field) = 0;
     virtual void set_step_index(uint64_t step) = 0;
     virtual uint64_t step_index() const = 0;

@@ -32,6 +32,8 @@ EXPORTS = (
     "hzl_pf_begin", "hzl_pf_assimilate", "hzl_pf_result",
     "hzl_pf_begin_filters", "hzl_ensemble_fill_state", "hzl_ensemble_set_step_index",
     "hzl_enkf_begin", "hzl_enkf_assimilate", "hzl_ensemble_mean_spread", "hzl_pmcmc_run",
+    "hzl_pf_joint_create", "hzl_pf_joint_destroy", "hzl_pf_joint_begin",
+    "hzl_pf_joint_assimilate", "hzl_pf_joint_result",
     "hzl_surrogate_create", "hzl_surrogate_destroy", "hzl_surrogate_param_count",
     "hzl_surrogate_get_params", "hzl_surrogate_set_params", "hzl_surrogate_set_scales",
     "hzl_surrogate_get_scales", "hzl_surrogate_forward", "hzl_surrogate_loss_grad",
@@ -109,6 +111,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_enkf_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp, _dp,
                                       C.c_double, C.c_double, _dp, _dp]
     L.hzl_ensemble_mean_spread.argtypes = [vp, _dp, _dp]
+    L.hzl_pf_joint_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_size_t]
+    L.hzl_pf_joint_destroy.argtypes = [vp]
+    L.hzl_pf_joint_begin.argtypes = [vp, C.c_uint64, C.c_size_t]
+    L.hzl_pf_joint_assimilate.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, ip, ip, _dp,
+                                          _dp, C.c_int, _dp]
+    L.hzl_pf_joint_result.argtypes = [vp, _dp, C.POINTER(C.c_size_t)]
     L.hzl_pmcmc_run.argtypes = [vp, C.POINTER(PmcmcOptions), C.c_double, C.c_size_t, ip, ip, ip,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, ip]
     fp, sz = C.POINTER(C.c_float), C.c_size_t
@@ -414,6 +422,45 @@ def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,
             break
     ll, events = particles.pf_result()
     return dict(log_likelihood=ll, resampling_events=events, history=history)
+
+
+def joint_bootstrap_filter(ensembles, observations, dt: float, seed: int):
+    """One bootstrap particle filter over the members of several ensembles of
+    different grid sizes (hzl_pf_joint_*); observation grid indices on the
+    coarsest grid.  Returns dict(log_likelihood, resampling_events, history)."""
+    L = load_library()
+    obs = np.asarray(observations, dtype=np.float64)
+    if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
+        raise ValueError("observations must be rows of (step, x, y, value, sd)")
+    handles = (C.c_void_p * len(ensembles))(*[e._h.value for e in ensembles])
+    h = C.c_void_p()
+    _check(L.hzl_pf_joint_create(C.byref(h), handles, len(ensembles)))
+    try:
+        steps = obs[:, 0].astype(np.int64)
+        groups = np.unique(steps)
+        _check(L.hzl_pf_joint_begin(h, seed, len(groups) + 1))
+        ip = C.POINTER(C.c_int)
+        history, current = [], 0
+        for gi, target in enumerate(groups):
+            sel = obs[steps == target]
+            x = np.ascontiguousarray(sel[:, 1], dtype=np.int32)
+            y = np.ascontiguousarray(sel[:, 2], dtype=np.int32)
+            v = np.ascontiguousarray(sel[:, 3])
+            sd = np.ascontiguousarray(sel[:, 4])
+            rec = np.zeros(4)
+            _check(L.hzl_pf_joint_assimilate(h, dt, int(target - current), len(sel),
+                                             x.ctypes.data_as(ip), y.ctypes.data_as(ip),
+                                             v.ctypes.data_as(_dp), sd.ctypes.data_as(_dp),
+                                             int(gi == len(groups) - 1),
+                                             rec.ctypes.data_as(_dp)))
+            history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
+                                max_weight=rec[2], resampled=bool(rec[3])))
+            current = int(target)
+        ll, ev = C.c_double(), C.c_size_t()
+        _check(L.hzl_pf_joint_result(h, C.byref(ll), C.byref(ev)))
+    finally:
+        L.hzl_pf_joint_destroy(h)
+    return dict(log_likelihood=ll.value, resampling_events=ev.value, history=history)
 
 
 def ensemble_kalman_filter(members: Ensemble, observations, dt: float, seed: int,

@@ -43,6 +43,7 @@ extern "C" {
 
 typedef struct hzl_ensemble *hzl_ensemble_t;
 typedef struct hzl_surrogate *hzl_surrogate_t;
+typedef struct hzl_pf_joint *hzl_pf_joint_t;
 
 /* Model parameters.  The first six fields mirror hzl::Parameters
  * (include/hzl/solver.hpp:15-23).  The engine's stochastic forcing is white in
@@ -166,6 +167,23 @@ int hzl_pf_result(hzl_ensemble_t h, double *log_likelihood, size_t *resampling_e
  * src/filter.cpp:44-48). */
 int hzl_pf_begin_filters(hzl_ensemble_t h, const uint64_t *seeds, size_t filters,
                          size_t max_groups);
+
+/* One bootstrap particle filter over the members of up to 4 ensembles of
+ * different grid sizes and one precision (e.g. 256 particles at 256^2 plus
+ * 32 at 512^2).  Each ensemble advances on its own stream; weights are
+ * normalised over the union; a child copies the retained modes its grid
+ * shares with its ancestor's (truncation / zero padding).  Observation grid
+ * indices refer to the coarsest grid.  Particle i of the union starts on
+ * noise key stream_seed(seed, i); everything else as hzl_pf_begin /
+ * hzl_pf_assimilate (record[4]) / hzl_pf_result.  The ensembles must not be
+ * used for other work while the filter runs. */
+int hzl_pf_joint_create(hzl_pf_joint_t *out, const hzl_ensemble_t *ensembles, size_t count);
+int hzl_pf_joint_destroy(hzl_pf_joint_t h);
+int hzl_pf_joint_begin(hzl_pf_joint_t h, uint64_t seed, size_t max_groups);
+int hzl_pf_joint_assimilate(hzl_pf_joint_t h, double dt, size_t steps, size_t nobs, const int *x,
+                            const int *y, const double *value, const double *sd, int is_last,
+                            double *record);
+int hzl_pf_joint_result(hzl_pf_joint_t h, double *log_likelihood, size_t *resampling_events);
 
 /* Upload one state (reference Field layout) into every member. */
 int hzl_ensemble_fill_state(hzl_ensemble_t h, const double *field);

@@ -199,3 +199,58 @@ def oracle_pmcmc(orc, initial, obs, dt, q0, samples, scale, seed, lower, upper, 
             rows.append((cur.copy(), ll, dens, acc))
         out.append(rows)
     return out
+
+
+def change_grid(field, n):
+    """Retained modes of `field` on an n x n grid: modes both grids retain are
+    copied, the others are zero (truncation / zero padding)."""
+    ns = field.shape[0]
+    K = min((n - 1) // 3, (ns - 1) // 3)
+    out = np.zeros((n, n), dtype=np.complex128)
+    for ky in range(-K, K + 1):
+        for kx in range(-K, K + 1):
+            if kx or ky:
+                out[ky % n, kx % n] = field[ky % ns, kx % ns]
+    return out
+
+
+def oracle_joint_filter(orc, initial, obs, dt, seed, nu, alpha=0.0, amp=0.1, kf=4, sigma=0.05,
+                        kmin=1, kmax=4):
+    """Bootstrap filter over particles of several grid sizes; `initial` is a
+    list of (P_e, n_e, n_e) arrays, observation indices on the coarsest grid."""
+    states = [s.copy() for part in initial for s in np.asarray(part, dtype=np.complex128)]
+    N = len(states)
+    coarse = min(s.shape[0] for s in states)
+    grid = [s.shape[0] for s in states]
+    keys = [orc.stream_seed(seed, i) for i in range(N)]
+    weights = np.full(N, 1.0 / N)
+    rng = orc.mt(orc.stream_seed(seed, UINT64_MAX))
+    steps = obs[:, 0].astype(np.int64)
+    groups = np.unique(steps)
+    current, total, events, history = 0, 0.0, 0, []
+    for g, target in enumerate(groups):
+        sel = obs[steps == target]
+        logs = np.zeros(N)
+        for i in range(N):
+            states[i] = orc.trajectory(grid[i], nu, alpha, dt, int(target - current), keys[i],
+                                       amp=amp, kf=kf, sigma=sigma, kmin=kmin, kmax=kmax,
+                                       step0=current, initial=states[i])
+            r = grid[i] // coarse
+            ph = physical(orc, states[i])
+            pred = ph[(sel[:, 2] * r).astype(int), (sel[:, 1] * r).astype(int)]
+            logs[i] = orc.pf_logweight(weights[i], pred, sel[:, 3], sel[:, 4])
+        inc, w, ess, mw = orc.pf_normalize(logs)
+        total += inc
+        resampled = False
+        if ess < N / 2 and g + 1 < len(groups):
+            anc = orc.systematic_resample(w, orc.mt_uniform(rng))
+            states = [change_grid(states[a], grid[i]) for i, a in enumerate(anc)]
+            keys = [orc.stream_seed(seed, (g + 1) * N + i) for i in range(N)]
+            w = np.full(N, 1.0 / N)
+            events += 1
+            resampled = True
+        weights = w
+        history.append(dict(step=int(target), increment=inc, ess=ess, max_weight=mw,
+                            resampled=resampled))
+        current = int(target)
+    return dict(log_likelihood=total, resampling_events=events, history=history, states=states)

@@ -16,7 +16,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle_filters import oracle_bootstrap_filter
+from oracle_filters import change_grid, oracle_bootstrap_filter, oracle_joint_filter
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -98,3 +98,31 @@ def test_pf_all_minus_infinity_gives_minus_infinity(pkg):
         rec, _ = ens.pf_assimilate(0.01, 1, [1], [2], [1e300], [1e-300], False)
         ll, _ = ens.pf_result()
     assert rec[0] == -np.inf and ll == -np.inf
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_joint_filter_over_two_grids_matches_oracle(pkg, orc, precision):
+    """One filter over 8 particles at 32^2 and 4 at 64^2 (the uneven-size
+    configuration in miniature): joint weights, cross-grid resampling."""
+    nu, dt, seed = 2e-3, 2e-3, 91
+    rows, _ = synthetic(orc, 32, nu, dt, groups=4, every=10, nobs=24, truth_key=999, seed=3,
+                        sd=1.0)
+    base = orc.grf(32, 999)
+    coarse = np.stack([base + 0.05 * orc.grf(32, orc.stream_seed(5, i)) for i in range(8)])
+    fine = np.stack([change_grid(base + 0.05 * orc.grf(32, orc.stream_seed(6, i)), 64)
+                     for i in range(4)])
+    want = oracle_joint_filter(orc, [coarse, fine], rows, dt, seed, nu, sigma=0.2)
+    with pkg.Ensemble(32, 8, precision, nu=nu, sigma=0.2, amp=0.1, seed=1) as a, \
+            pkg.Ensemble(64, 4, precision, nu=nu, sigma=0.2, amp=0.1, seed=2) as b:
+        a.set_state(coarse)
+        b.set_state(fine)
+        got = pkg.joint_bootstrap_filter([a, b], rows, dt, seed)
+        states = list(a.get_state()) + list(b.get_state())
+    print(precision, got["log_likelihood"], want["log_likelihood"],
+          [h["resampled"] for h in want["history"]])
+    assert [h["resampled"] for h in got["history"]] == [h["resampled"] for h in want["history"]]
+    assert any(h["resampled"] for h in want["history"])
+    tol = 1e-10 * abs(want["log_likelihood"]) if precision == "fp64" else 1e-4
+    assert abs(got["log_likelihood"] - want["log_likelihood"]) < max(tol, 1e-12)
+    for g, w in zip(states, want["states"]):
+        assert np.linalg.norm(g - w) / np.linalg.norm(w) < (1e-11 if precision == "fp64" else 1e-5)
