@@ -332,6 +332,65 @@ def pmcmc_window(pkg, chains: int = 16, particles: int = 128, samples: int = 3) 
             "posterior_amp_mean": float(np.exp(res["q"][:, -1, 1]).mean())}
 
 
+def uneven_pf(pkg, cycles: int = 5) -> dict:
+    """One particle filter over 256 particles at 256^2 and 32 at 512^2 (FP32),
+    nu = dt = 5e-4, 256 point observations (sd 0.02, on the 256 grid) every
+    100 steps from a 512^2 truth run; the two grids forecast on their own
+    streams and are weighted and resampled jointly.  Also times each grid's
+    100-step forecast alone, to show the overlap."""
+    nu = dt = 5e-4
+    H, nobs, sd = 100, 256, 0.02
+    rng = np.random.default_rng(SEED + 21)
+    truth = pkg.Ensemble(512, 1, "fp32", nu=nu, sigma=0.05, amp=0.1, seed=SEED + 22)
+    truth.init_grf()
+    t0_fine = truth.get_state()[0]
+    rows = []
+    for c in range(cycles):
+        truth.step(dt, H)
+        ph = (np.fft.ifft2(truth.get_state()[0]) * 512 * 512).real
+        xs, ys = rng.integers(0, 256, nobs), rng.integers(0, 256, nobs)
+        rows += [((c + 1) * H, x, y, ph[2 * y, 2 * x] + sd * rng.standard_normal(), sd)
+                 for x, y in zip(xs, ys)]
+    truth.close()
+    # the coarse particles start from the truth's modes that 256^2 retains
+    K = 85
+    kc, kf = np.fft.fftfreq(256, 1 / 256), np.fft.fftfreq(512, 1 / 512)
+    coarse0 = np.zeros((256, 256), dtype=np.complex128)
+    coarse0[np.ix_(np.abs(kc) <= K, np.abs(kc) <= K)] = \
+        t0_fine[np.ix_(np.abs(kf) <= K, np.abs(kf) <= K)]
+    out = {"config": f"256 particles at 256x256 + 32 at 512x512 (FP32), {cycles} cycles of {H} "
+                     f"steps, nu=dt={nu}, {nobs} obs/cycle sd={sd}"}
+    with pkg.Ensemble(256, 256, "fp32", nu=nu, sigma=0.05, amp=0.1, seed=SEED + 23) as a, \
+            pkg.Ensemble(512, 32, "fp32", nu=nu, sigma=0.05, amp=0.1, seed=SEED + 24) as b:
+        a.fill_state(coarse0)
+        b.fill_state(t0_fine)
+        for e, name in ((a, "alone_256_ms"), (b, "alone_512_ms")):
+            e.step(dt, 2)
+            e.synchronize()
+            t = time.perf_counter()
+            e.step(dt, H)
+            e.synchronize()
+            out[name] = round((time.perf_counter() - t) * 1e3, 3)
+        a.fill_state(coarse0)
+        b.fill_state(t0_fine)
+        a.step_index = 0
+        b.step_index = 0
+        a.synchronize()
+        b.synchronize()
+        t = time.perf_counter()
+        res = pkg.joint_bootstrap_filter([a, b], np.array(rows), dt, SEED)
+        a.synchronize()
+        b.synchronize()
+        sec = time.perf_counter() - t
+    steps = cycles * H * (256 + 32)
+    out.update({"cycles_per_s": round(cycles / sec, 3), "ms_per_cycle": round(sec / cycles * 1e3, 2),
+                "member_steps_per_s": round(steps / sec, 1),
+                "log_likelihood": round(res["log_likelihood"], 3),
+                "resampling_events": res["resampling_events"],
+                "ess_last": round(res["history"][-1]["ess"], 3)})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,6 +404,8 @@ def main() -> int:
     ap.add_argument("--pf-only", action="store_true", help="only the surrogate PF comparison")
     ap.add_argument("--no-enkf", action="store_true", help="skip the EnKF cycle")
     ap.add_argument("--no-pmcmc", action="store_true", help="skip the pMCMC window")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the two-grid filter")
+    ap.add_argument("--uneven-only", action="store_true", help="only the two-grid filter")
     ap.add_argument("--pmcmc-only", action="store_true", help="only the pMCMC window")
     ap.add_argument("--enkf-only", action="store_true", help="only the EnKF cycle")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -365,6 +426,9 @@ def main() -> int:
         return 0
     if args.enkf_only:
         print(json.dumps({"enkf": enkf_cycle(pkg)}), flush=True)
+        return 0
+    if args.uneven_only:
+        print(json.dumps({"uneven_pf": uneven_pf(pkg)}), flush=True)
         return 0
     if args.pmcmc_only:
         print(json.dumps({"pmcmc": pmcmc_window(pkg)}), flush=True)
@@ -440,6 +504,8 @@ def main() -> int:
         line["enkf"] = enkf_cycle(pkg)
     if not args.no_pmcmc:
         line["pmcmc"] = pmcmc_window(pkg)
+    if not args.no_uneven:
+        line["uneven_pf"] = uneven_pf(pkg)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, nu, dt)
     print(json.dumps(line), flush=True)
