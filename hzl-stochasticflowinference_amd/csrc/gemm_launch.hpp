@@ -29,6 +29,13 @@ inline dim3 grid1d(long count) {
     return dim3((unsigned)((count + 255) / 256));
 }
 
+// Number of device allocations made by Buf so far: a captured graph that
+// holds buffer addresses is valid while this has not changed.
+inline uint64_t &alloc_generation() {
+    static uint64_t g = 0;
+    return g;
+}
+
 // Device buffer that only grows.  Growing waits for the device first: work
 // still queued on any stream may read the old allocation.
 template <typename T> struct Buf {
@@ -45,6 +52,7 @@ template <typename T> struct Buf {
         n = 0;
         hcheck(hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)), "hipMalloc");
         n = count;
+        ++alloc_generation();
     }
     Buf() = default;
     Buf(const Buf &) = delete;

@@ -117,12 +117,18 @@ __global__ void k_fill_f32(float *p, long n, float v) {
     if (i < n)
         p[i] = v;
 }
+// The noise counter is read from device memory so that a captured forecast
+// graph can be replayed with a new counter.
 __global__ void k_apply_increment(cx<float> *w0, const float *incr, int S, int K, int KS,
                                   float scale, const double *nu, const double *alpha,
                                   const double *sigma, const uint64_t *keys, const int *pair_of,
-                                  int kmin, int kmax, double horizon, uint64_t counter) {
+                                  int kmin, int kmax, double horizon, const uint64_t *counter) {
     apply_increment_body(w0, incr, S, K, KS, scale, nu, alpha, sigma, keys, pair_of, kmin, kmax,
-                         horizon, counter, HZL_GID);
+                         horizon, counter[0], HZL_GID);
+}
+__global__ void k_set_u64(uint64_t *p, uint64_t v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        p[0] = v;
 }
 
 
@@ -310,6 +316,10 @@ class Surrogate {
         conv_build();
     }
     ~Surrogate() {
+        if (graph_) {
+            (void)hipDeviceSynchronize();
+            (void)hipGraphExecDestroy(graph_);
+        }
         if (own_) {
             (void)hipStreamSynchronize(own_);
             (void)hipStreamDestroy(own_);
@@ -576,7 +586,60 @@ class Surrogate {
         // counter is the ensemble's step index, so a forecast is a function of
         // the ensemble state alone
         st_ = ev.stream;
-        const uint64_t counter = eng.step_index();
+        counter_dev_.alloc(1);
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st_, counter_dev_.p,
+                           eng.step_index());
+        if (precision == HZL_BF16)
+            prepare_weights(wb_);
+        else
+            prepare_weights(wf_);
+        const GraphKey key{ev.w0,    ev.nu,       ev.alpha,  ev.sigma,   ev.keys,
+                           ev.pair_of, ev.stream, ev.B,      ev.kmin,    ev.kmax,
+                           precision, in_scale_,  out_scale_, (double)steps * dt,
+                           alloc_generation()};
+        if (!use_graph_ || ev.B > 1024) {
+            launch_forecast(ev, steps, dt, precision);
+        } else if (graph_ && key == graph_key_) {
+            hcheck(hipGraphLaunch(graph_, st_), "forecast graph launch");
+        } else if (warm_ && key == warm_key_) {
+            // second call with this key: every buffer exists, capture the sequence
+            reset_graph();
+            hipGraph_t g = nullptr;
+            hcheck(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
+            try {
+                launch_forecast(ev, steps, dt, precision);
+            } catch (...) {
+                (void)hipStreamEndCapture(st_, &g);
+                if (g)
+                    (void)hipGraphDestroy(g);
+                throw;
+            }
+            hcheck(hipStreamEndCapture(st_, &g), "capture end");
+            const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hcheck(e, "forecast graph instantiate");
+            graph_key_ = key;
+            hcheck(hipGraphLaunch(graph_, st_), "forecast graph launch");
+        } else {
+            launch_forecast(ev, steps, dt, precision);
+            warm_ = true;
+            warm_key_ = key;
+        }
+        hcheck(hipGetLastError(), "surrogate forecast");
+        st_ = own_;
+        eng.invalidate_mixed();
+        eng.advance_counter(steps);
+    }
+
+    void set_graph(bool on) {
+        use_graph_ = on;
+        reset_graph();
+    }
+
+  private:
+    // The launch sequence of one forecast (no allocation or host sync once
+    // the buffers exist, so it can be captured).
+    void launch_forecast(const EngineViews &ev, size_t steps, double dt, int precision) {
         const int chunk = std::min(ev.B, 1024);
         for (int s0 = 0; s0 < ev.B; s0 += chunk) {
             const int S = std::min(chunk, ev.B - s0);
@@ -592,15 +655,37 @@ class Surrogate {
             hipLaunchKernelGGL(k_apply_increment, grid1d(cnt), dim3(256), 0, st_, w0, incr_.p, S,
                                s_.K, s_.KS, (float)(1.0 / out_scale_), ev.nu + s0, ev.alpha + s0,
                                ev.sigma + s0, ev.keys + s0, ev.pair_of, ev.kmin, ev.kmax,
-                               (double)steps * dt, counter);
+                               (double)steps * dt, counter_dev_.p);
         }
-        hcheck(hipGetLastError(), "surrogate forecast");
-        st_ = own_;
-        eng.invalidate_mixed();
-        eng.advance_counter(steps);
     }
 
-  private:
+    // Everything a captured forecast holds: the ensemble's device pointers
+    // and stream, its sizes, the scales and horizon, and (through the
+    // allocation generation) the surrogate's own buffer addresses.
+    struct GraphKey {
+        const void *w0, *nu, *alpha, *sigma, *keys, *pair_of;
+        hipStream_t stream;
+        int B, kmin, kmax, precision;
+        double in_scale, out_scale, horizon;
+        uint64_t generation;
+        bool operator==(const GraphKey &o) const {
+            return w0 == o.w0 && nu == o.nu && alpha == o.alpha && sigma == o.sigma &&
+                   keys == o.keys && pair_of == o.pair_of && stream == o.stream && B == o.B &&
+                   kmin == o.kmin && kmax == o.kmax && precision == o.precision &&
+                   in_scale == o.in_scale && out_scale == o.out_scale &&
+                   horizon == o.horizon && generation == o.generation;
+        }
+    };
+
+    void reset_graph() {
+        if (graph_) {
+            (void)hipStreamSynchronize(st_);
+            (void)hipGraphExecDestroy(graph_);
+        }
+        graph_ = nullptr;
+        warm_ = false;
+    }
+
     static FnoShape make_shape(int state_n, int grid, int width, int layers, int modes) {
         FnoShape s;
         if (state_n != 64 && state_n != 128 && state_n != 256)
@@ -752,9 +837,15 @@ class Surrogate {
         b.run(st_);
     }
 
+    // Cast the FP32 master weights into w's operand type (after a change).
     template <typename T> void prepare_weights(FnoWork<T> &w) {
+        if (!w.built)
+            w.ensure(s_, 1, prm_.total, false); // allocates w_T and rcat
         if (w.version == version_)
             return;
+        const size_t rcat = (size_t)s_.L * 4 * s_.C * s_.C * s_.modes();
+        if (!w.w_T.p || w.w_T.n < (size_t)prm_.total || !w.rcat.p || w.rcat.n < rcat)
+            throw std::logic_error("surrogate weight buffers missing");
         hipLaunchKernelGGL(k_cast<T>, grid1d(prm_.total), dim3(256), 0, st_, master_.p, w.w_T.p,
                            prm_.total);
         const long per = 4L * s_.C * s_.C * s_.modes();
@@ -1103,6 +1194,10 @@ class Surrogate {
     Buf<cx<float>> diff_, hx_, hy_, ds_x_, ds_y_, bx_, by_, backup_;
     Buf<double> hnu_, ds_nu_, bnu_, lsum_, spread_;
     double error_rms_ = -1;
+    Buf<uint64_t> counter_dev_;
+    bool use_graph_ = true, warm_ = false;
+    GraphKey graph_key_{}, warm_key_{};
+    hipGraphExec_t graph_ = nullptr;
     Buf<int> bidx_;
     FnoWork<float> wf_;
     FnoWork<__bf16> wb_;
@@ -1267,6 +1362,10 @@ int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, 
         if (used_solver)
             *used_solver = r;
     });
+}
+
+int hzl_surrogate_set_graph(hzl_surrogate_t h, int enabled) {
+    return guarded([&] { net_of(h).set_graph(enabled != 0); });
 }
 
 int hzl_surrogate_forecast(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt, size_t steps,

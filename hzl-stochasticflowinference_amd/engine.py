@@ -40,7 +40,7 @@ EXPORTS = (
     "hzl_surrogate_collect", "hzl_surrogate_dataset_size", "hzl_surrogate_fit",
     "hzl_surrogate_evaluate", "hzl_surrogate_error_estimate",
     "hzl_surrogate_set_error_estimate", "hzl_surrogate_spread",
-    "hzl_surrogate_forecast_or_solve", "hzl_surrogate_forecast",
+    "hzl_surrogate_forecast_or_solve", "hzl_surrogate_set_graph", "hzl_surrogate_forecast",
 )
 
 
@@ -136,6 +136,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_surrogate_evaluate.argtypes = [vp, C.c_int, sz, sz, _dp]
     L.hzl_surrogate_forecast.argtypes = [vp, vp, C.c_double, sz, C.c_int]
     L.hzl_surrogate_error_estimate.argtypes = [vp, _dp]
+    L.hzl_surrogate_set_graph.argtypes = [vp, C.c_int]
     L.hzl_surrogate_set_error_estimate.argtypes = [vp, C.c_double]
     L.hzl_surrogate_spread.argtypes = [vp, vp, _dp]
     L.hzl_surrogate_forecast_or_solve.argtypes = [vp, vp, C.c_double, sz, C.c_int, C.c_double,
@@ -654,6 +655,10 @@ class Surrogate:
     def forecast(self, ensemble: Ensemble, dt: float, steps: int, precision: str = "bf16"):
         _check(_lib.hzl_surrogate_forecast(self._h, ensemble._h, dt, steps,
                                            _SUR_PRECISION[precision]))
+
+    def set_graph(self, enabled: bool):
+        """Replay forecasts from a captured HIP graph (default) or launch each kernel."""
+        _check(_lib.hzl_surrogate_set_graph(self._h, int(enabled)))
 
     @property
     def error_estimate(self) -> float:

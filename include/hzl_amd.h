@@ -326,6 +326,12 @@ int hzl_surrogate_fit(hzl_surrogate_t h, size_t first, size_t count, size_t epoc
 int hzl_surrogate_evaluate(hzl_surrogate_t h, int precision, size_t first, size_t count,
                            double *loss);
 
+/* Replay forecasts from a captured HIP graph (default on): the second
+ * forecast of an ensemble with unchanged scales, horizon and buffers records
+ * the launch sequence, later ones replay it with the current step counter.
+ * Off: every forecast launches its kernels one by one. */
+int hzl_surrogate_set_graph(hzl_surrogate_t h, int enabled);
+
 /* The error estimate (negative until evaluated or set). */
 int hzl_surrogate_error_estimate(hzl_surrogate_t h, double *rms);
 int hzl_surrogate_set_error_estimate(hzl_surrogate_t h, double rms);

@@ -216,3 +216,30 @@ def test_spread_and_solver_fallback(pkg, orc):
         assert not used
         assert abs(sp - sur.spread(a)) < 1e-12 * sp
         assert a.step_index == 2 * steps
+
+
+def test_graph_replays_match_eager_forecasts(pkg, orc):
+    """Forecasts replayed from the captured graph equal kernel-by-kernel
+    forecasts bit for bit, over several replays with eager engine work and
+    synchronisations between them (and a new noise counter each time).  Call
+    1 runs eagerly, call 2 captures, calls 3-5 replay."""
+    cfg = SMALL
+    s = shape_of(cfg)
+    B, dt, steps = 16, 2e-3, 25
+    f = grf_fields(orc, s.n, B, 61)
+    with pkg.Surrogate(**cfg, seed=9) as g, pkg.Surrogate(**cfg, seed=9) as e, \
+            pkg.Ensemble(s.n, B, "fp32", sigma=0.3, seed=6) as a, \
+            pkg.Ensemble(s.n, B, "fp32", sigma=0.3, seed=6) as b:
+        g.set_scales(1.5, 4.0)
+        e.set_scales(1.5, 4.0)
+        e.set_graph(False)
+        a.set_state(f)
+        b.set_state(f)
+        for k in range(5):
+            g.forecast(a, dt, steps, "bf16")
+            e.forecast(b, dt, steps, "bf16")
+            a.step(dt, 2)
+            b.step(dt, 2)
+            a.synchronize()
+            b.synchronize()
+            np.testing.assert_array_equal(a.get_state(), b.get_state())
