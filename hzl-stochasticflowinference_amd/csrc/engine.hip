@@ -238,6 +238,44 @@ template <typename R> class Engine final : public EngineBase {
             (void)hipStreamSynchronize(stream_);
             (void)hipStreamDestroy(stream_);
         }
+        if (obs_event_)
+            (void)hipEventDestroy(obs_event_);
+        if (obs_host_)
+            (void)hipHostFree(obs_host_);
+    }
+
+    // One observation group into device memory with a single asynchronous
+    // copy from pinned staging: [values | sds | x, y as int pairs].
+    void stage_observations(size_t nobs, const int *x, const int *y, const double *value,
+                            const double *sd) {
+        const size_t words = 3 * nobs + 1;
+        if (!obs_event_)
+            check(hipEventCreateWithFlags(&obs_event_, hipEventDisableTiming), "event");
+        else
+            check(hipEventSynchronize(obs_event_), "staging reuse"); // last copy done
+        if (obs_cap_ < words) {
+            check(hipStreamSynchronize(stream_), "staging growth"); // kernels may read it
+            if (obs_host_)
+                (void)hipHostFree(obs_host_);
+            obs_host_ = nullptr;
+            check(hipHostMalloc(reinterpret_cast<void **>(&obs_host_), words * sizeof(double)),
+                  "hipHostMalloc");
+            obs_dev_ = DeviceBuffer<double>(words);
+            obs_cap_ = words;
+        }
+        std::memcpy(obs_host_, value, nobs * sizeof(double));
+        std::memcpy(obs_host_ + nobs, sd, nobs * sizeof(double));
+        int *xy = reinterpret_cast<int *>(obs_host_ + 2 * nobs);
+        std::memcpy(xy, x, nobs * sizeof(int));
+        std::memcpy(xy + nobs, y, nobs * sizeof(int));
+        check(hipMemcpyAsync(obs_dev_.p, obs_host_, 3 * nobs * sizeof(double),
+                             hipMemcpyHostToDevice, stream_),
+              "observation upload");
+        check(hipEventRecord(obs_event_, stream_), "event");
+        obs_v_ = obs_dev_.p;
+        obs_sd_ = obs_dev_.p + nobs;
+        obs_x_ = reinterpret_cast<const int *>(obs_dev_.p + 2 * nobs);
+        obs_y_ = obs_x_ + nobs;
     }
 
     void set_member_parameters(size_t m, double nu, double alpha, double sigma,
@@ -463,19 +501,9 @@ template <typename R> class Engine final : public EngineBase {
             dispatch_col<R>(n_, 0, stage_args(dt, nullptr, 0), stream_);
             mix_valid_ = true;
         }
-        if (pf_obs_cap_ < nobs) {
-            pf_ox_ = DeviceBuffer<int>(nobs);
-            pf_oy_ = DeviceBuffer<int>(nobs);
-            pf_ov_ = DeviceBuffer<double>(nobs);
-            pf_osd_ = DeviceBuffer<double>(nobs);
-            pf_obs_cap_ = nobs;
-        }
-        upload(pf_ox_.p, x, nobs);
-        upload(pf_oy_.p, y, nobs);
-        upload(pf_ov_.p, value, nobs);
-        upload(pf_osd_.p, sd, nobs);
+        stage_observations(nobs, x, y, value, sd);
         PfArgs<R> a{(int)B_,   n_,        K_,           mix_.p,       (int)nobs,
-                    pf_ox_.p,  pf_oy_.p,  pf_ov_.p,     pf_osd_.p,    pf_cs_.p,
+                    obs_x_,    obs_y_,    obs_v_,       obs_sd_,      pf_cs_.p,
                     pf_sn_.p,  pf_wstate_.p, pf_wnorm_.p, pf_logw_.p, nullptr};
         hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
         const int P = (int)(B_ / pf_filters_);
@@ -572,19 +600,9 @@ template <typename R> class Engine final : public EngineBase {
             upload(pf_cs_.p, cs.data(), n_);
             upload(pf_sn_.p, sn.data(), n_);
         }
-        if (pf_obs_cap_ < nobs) {
-            pf_ox_ = DeviceBuffer<int>(nobs);
-            pf_oy_ = DeviceBuffer<int>(nobs);
-            pf_ov_ = DeviceBuffer<double>(nobs);
-            pf_osd_ = DeviceBuffer<double>(nobs);
-            pf_obs_cap_ = nobs;
-        }
-        upload(pf_ox_.p, x, nobs);
-        upload(pf_oy_.p, y, nobs);
-        upload(pf_ov_.p, value, nobs);
-        upload(pf_osd_.p, sd, nobs);
+        stage_observations(nobs, x, y, value, sd);
         PfArgs<R> a{(int)B_,  n_,       K_,       mix_.p,    (int)nobs,
-                    pf_ox_.p, pf_oy_.p, pf_ov_.p, pf_osd_.p, pf_cs_.p,
+                    obs_x_,   obs_y_,   obs_v_,   obs_sd_,   pf_cs_.p,
                     pf_sn_.p, const_cast<double *>(wprev), nullptr, logw, nullptr};
         hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
         check(hipGetLastError(), "particle log-weights");
@@ -752,10 +770,15 @@ template <typename R> class Engine final : public EngineBase {
     size_t pf_filters_ = 1;
     int pf_max_uniforms_ = 0;
     DeviceBuffer<uint64_t> pf_seeds_;
-    size_t pf_obs_cap_ = 0;
+    size_t obs_cap_ = 0;
+    double *obs_host_ = nullptr; // pinned staging of one observation group
+    DeviceBuffer<double> obs_dev_;
+    hipEvent_t obs_event_ = nullptr;
+    const double *obs_v_ = nullptr, *obs_sd_ = nullptr;
+    const int *obs_x_ = nullptr, *obs_y_ = nullptr;
     DeviceBuffer<double> pf_wstate_, pf_wnorm_, pf_logw_, pf_stats_, pf_mean_, pf_uniforms_,
-        pf_cs_, pf_sn_, pf_ov_, pf_osd_;
-    DeviceBuffer<int> pf_anc_, pf_ox_, pf_oy_;
+        pf_cs_, pf_sn_;
+    DeviceBuffer<int> pf_anc_;
     int npairs_ = 0;
     DeviceBuffer<double> cfl_, cfl_diag_;
     DeviceBuffer<int> pairs_;

@@ -854,6 +854,7 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
     const double ess = 1 / red[0];
     const bool resample = finite && ess < (double)P / 2 && !a.is_last &&
                           (int)stats[6] < a.max_uniforms;
+    ctx.sync();
     if (tid == 0) {
         stats[0] = inc;
         stats[1] = ess;
@@ -861,19 +862,36 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
         stats[3] = resample ? 1 : 0;
         stats[4] += inc;
         if (resample) {
-            // one uniform per resampling event, strict `position > cumulative`
-            const double u = uniforms[(int)stats[6]];
+            // one uniform per resampling event; the running sums of the
+            // reference's scan (cum_0 = w_0, cum_k = cum_{k-1} + w_k), kept in
+            // wstate until it is rewritten below
+            red[0] = uniforms[(int)stats[6]];
             stats[6] += 1;
             stats[5] += 1;
-            int anc = 0;
-            double cum = wnorm[0];
-            const double start = u / (double)P;
+            double cum = 0;
             for (int i = 0; i < P; ++i) {
-                const double pos = start + (double)i / (double)P;
-                while (pos > cum && anc + 1 < P)
-                    cum += wnorm[++anc];
-                ancestors[i] = (int)base + anc;
+                cum += wnorm[i];
+                wstate[i] = cum;
             }
+        }
+    }
+    ctx.sync();
+    if (resample) {
+        // child i descends from the first particle whose running sum reaches
+        // u/P + i/P: the strict `position > cumulative` scan of
+        // src/filter.cpp:102-120, by binary search
+        const double start = red[0] / (double)P;
+        for (int i = tid; i < P; i += nt) {
+            const double pos = start + (double)i / (double)P;
+            int lo = 0, hi = P - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (pos > wstate[mid])
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            ancestors[i] = (int)base + lo;
         }
     }
     ctx.sync();
