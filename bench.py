@@ -208,7 +208,8 @@ def surrogate_pf(pkg, cycles: int = 20, particles: int = 256, train_windows: int
             pf.set_state(init)
             pf.synchronize()
             t = time.perf_counter()
-            if kind == "plain":
+            if kind in ("plain", "plain_no_graph"):
+                pf.set_graph(kind == "plain")
                 res = pkg.bootstrap_filter(pf, rows, dt, SEED, retain_states=retain)
             elif kind == "surrogate":
                 res = pkg.surrogate_filter(pf, sur, rows, dt, SEED, "bf16", retain_states=retain)
@@ -219,7 +220,7 @@ def surrogate_pf(pkg, cycles: int = 20, particles: int = 256, train_windows: int
             sec = time.perf_counter() - t
         return res, sec
 
-    for kind in ("plain", "surrogate", "surrogate_fallback"):
+    for kind in ("plain", "plain_no_graph", "surrogate", "surrogate_fallback"):
         run(kind, False)  # warm
         _, sec = run(kind, False)
         res, _ = run(kind, True)

@@ -41,9 +41,21 @@ def _run(cmd, cwd=None):
 
 
 def build_engine() -> str:
+    """Compile every translation unit in parallel (one hipcc each), then link."""
+    objs = [os.path.join(CSRC, os.path.basename(src) + ".o") for src in SOURCES]
+    procs = []
+    for src, obj in zip(SOURCES, objs):
+        cmd = [HIPCC, *HIP_FLAGS, "-c", "-I", INCLUDE, "-I", CSRC, src, "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    failed = [src for src, p in zip(SOURCES, procs) if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, f"hipcc {' '.join(failed)}")
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC, *SOURCES, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    for obj in objs:
+        os.remove(obj)
     return LIB
 
 

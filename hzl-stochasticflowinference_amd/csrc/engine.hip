@@ -76,12 +76,38 @@ __global__ void __launch_bounds__(256) k_pf_mean(int B, long spec, const cx<R> *
     pf_mean_body<R>(B, spec, w0, wn, mean, (long)blockIdx.x * 256 + threadIdx.x);
 }
 
+// Resampling copies, a device-side no-op for every filter of P particles that
+// kept its particles this group (stats[8 f + 3] == 0): gather the ancestors'
+// states into dst, then commit dst back to the live states.
 template <typename R>
 __global__ void __launch_bounds__(256) k_pf_gather(int B, long spec, const cx<R> *src, cx<R> *dst,
-                                                   const int *anc) {
+                                                   const int *anc, const double *stats, int P) {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < (long)B * spec)
-        dst[i] = src[(long)anc[i / spec] * spec + i % spec];
+    if (i >= (long)B * spec)
+        return;
+    const long m = i / spec;
+    if (stats[8 * (m / P) + 3] != 0)
+        dst[i] = src[(long)anc[m] * spec + i % spec];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_pf_commit(int B, long spec, const cx<R> *src, cx<R> *dst,
+                                                   const double *stats, int P) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)B * spec)
+        return;
+    if (stats[8 * ((i / spec) / P) + 3] != 0)
+        dst[i] = src[i];
+}
+
+// Device counters read by replayed graphs: [0] step index, [1] filter group,
+// [2] last-group flag.
+__global__ void k_set_counters(uint64_t *c, uint64_t step, uint64_t group, uint64_t is_last) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c[0] = step;
+        c[1] = group;
+        c[2] = is_last;
+    }
 }
 
 // ------------------------------------------------------------ helpers ----
@@ -231,11 +257,13 @@ template <typename R> class Engine final : public EngineBase {
         push_parameters();
         check(hipMemsetAsync(w0_.p, 0, B_ * spec_ * sizeof(cx<R>), stream_), "clear state");
         check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(double), stream_), "clear CFL");
+        ctr_ = DeviceBuffer<uint64_t>(3);
         check(hipStreamSynchronize(stream_), "engine setup");
     }
     ~Engine() override {
         if (stream_) {
             (void)hipStreamSynchronize(stream_);
+            clear_graphs();
             (void)hipStreamDestroy(stream_);
         }
         if (obs_event_)
@@ -255,6 +283,7 @@ template <typename R> class Engine final : public EngineBase {
             check(hipEventSynchronize(obs_event_), "staging reuse"); // last copy done
         if (obs_cap_ < words) {
             check(hipStreamSynchronize(stream_), "staging growth"); // kernels may read it
+            clear_graphs(); // captured cycles hold the old device address
             if (obs_host_)
                 (void)hipHostFree(obs_host_);
             obs_host_ = nullptr;
@@ -332,8 +361,36 @@ template <typename R> class Engine final : public EngineBase {
 
     void step(double dt, size_t steps) override {
         validate_dt(dt);
-        for (size_t s = 0; s < steps; ++s)
+        if (!use_graph_ || steps < 2) {
+            for (size_t s = 0; s < steps; ++s)
+                advance(dt, nullptr, 0);
+            return;
+        }
+        // Whole chunks of up to kGraphSteps steps replay one captured launch
+        // sequence whose noise kernels read the step index from ctr_[0].
+        flush_parameters();
+        ensure_mixed(dt);
+        const size_t chunk = std::min(steps, kGraphSteps);
+        size_t done = 0;
+        for (; steps - done >= chunk; done += chunk) {
+            set_counters(0, 0);
+            const GraphKey key{1, dt, chunk, 0, 0, nullptr, false};
+            run_captured(key, [&]() {
+                for (size_t s = 0; s < chunk; ++s)
+                    launch_step(dt, nullptr, 0, ctr_.p, s, nullptr);
+            });
+            step_ += chunk;
+        }
+        last_dt_ = dt;
+        max_dt_ = std::max(max_dt_, dt);
+        for (; done < steps; ++done)
             advance(dt, nullptr, 0);
+    }
+
+    void set_graph(bool on) override {
+        check(hipStreamSynchronize(stream_), "graph switch");
+        use_graph_ = on;
+        clear_graphs();
     }
 
     void step_forced(double dt, const double *forcing, bool shared) override {
@@ -439,6 +496,8 @@ template <typename R> class Engine final : public EngineBase {
         const size_t P = B_ / filters;
         if (P < 2)
             throw std::invalid_argument("particle filter needs at least two particles");
+        check(hipStreamSynchronize(stream_), "filter start");
+        clear_graphs(); // captured cycles hold the previous filter's buffers
         pf_filters_ = filters;
         pf_group_ = 0;
         pf_wstate_ = DeviceBuffer<double>(B_);
@@ -496,34 +555,31 @@ template <typename R> class Engine final : public EngineBase {
                 throw std::invalid_argument("observations must be finite, in bounds, and have "
                                             "positive uncertainty");
         if (steps)
-            step(dt, steps);
-        if (!mix_valid_) {
-            dispatch_col<R>(n_, 0, stage_args(dt, nullptr, 0), stream_);
-            mix_valid_ = true;
-        }
+            validate_dt(dt);
         stage_observations(nobs, x, y, value, sd);
-        PfArgs<R> a{(int)B_,   n_,        K_,           mix_.p,       (int)nobs,
-                    obs_x_,    obs_y_,    obs_v_,       obs_sd_,      pf_cs_.p,
-                    pf_sn_.p,  pf_wstate_.p, pf_wnorm_.p, pf_logw_.p, nullptr};
-        hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
-        const int P = (int)(B_ / pf_filters_);
-        PfControl c{P,           pf_logw_.p, pf_wstate_.p,      pf_wnorm_.p,
-                    pf_stats_.p, pf_anc_.p,  pf_uniforms_.p,    pf_max_uniforms_,
-                    d_keys_.p,   pf_seeds_.p, pf_group_,        is_last ? 1 : 0};
-        hipLaunchKernelGGL(k_pf_normalize, dim3((unsigned)pf_filters_), dim3(256), 0, stream_, c);
-        if (mean_field) {
-            const long blocks = ((long)spec_ + 255) / 256;
-            hipLaunchKernelGGL(k_pf_mean<R>, dim3((unsigned)blocks), dim3(256), 0, stream_,
-                               (int)B_, (long)spec_, (const cx<R> *)w0_.p, pf_wnorm_.p,
-                               pf_mean_.p);
+        if (use_graph_ && steps >= 1 && steps <= kGraphSteps) {
+            // the whole cycle (forecast, weights, normalisation, resampling
+            // copies) replays one captured launch sequence; the step index,
+            // group and last-group flag come from ctr_
+            flush_parameters();
+            ensure_mixed(dt);
+            set_counters(pf_group_, is_last ? 1 : 0);
+            const GraphKey key{2, dt, steps, nobs, pf_filters_, obs_dev_.p, mean_field != nullptr};
+            run_captured(key, [&]() {
+                for (size_t s = 0; s < steps; ++s)
+                    launch_step(dt, nullptr, 0, ctr_.p, s, nullptr);
+                launch_filter(nobs, true, is_last, mean_field != nullptr);
+            });
+            step_ += steps;
+            last_dt_ = dt;
+            max_dt_ = std::max(max_dt_, dt);
+        } else {
+            if (steps)
+                step(dt, steps);
+            flush_parameters();
+            ensure_mixed(dt);
+            launch_filter(nobs, false, is_last, mean_field != nullptr);
         }
-        const long total = (long)B_ * (long)spec_;
-        hipLaunchKernelGGL(k_pf_gather<R>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                           stream_, (int)B_, (long)spec_, (const cx<R> *)w0_.p, w1_.p,
-                           (const int *)pf_anc_.p);
-        check(hipMemcpyAsync(w0_.p, w1_.p, total * sizeof(cx<R>), hipMemcpyDeviceToDevice,
-                             stream_),
-              "ancestor copy");
         check(hipGetLastError(), "particle filter kernels");
         mix_valid_ = false;
         std::vector<double> st(8 * pf_filters_);
@@ -590,6 +646,8 @@ template <typename R> class Engine final : public EngineBase {
             mix_valid_ = true;
         }
         if (pf_cs_.count != (size_t)n_) {
+            check(hipStreamSynchronize(stream_), "table growth");
+            clear_graphs();
             std::vector<double> cs(n_), sn(n_);
             for (int j = 0; j < n_; ++j) {
                 cs[j] = std::cos(2 * PI * j / n_);
@@ -724,17 +782,23 @@ template <typename R> class Engine final : public EngineBase {
         return a;
     }
 
-    // One RK4 step: 4 x (row, col).  With `events` (9 entries) the launches
-    // are bracketed by events so their individual durations can be read.
-    void advance(double dt, const cx<R> *ext, long long ext_stride,
-                 hipEvent_t *events = nullptr) {
-        flush_parameters();
+    // Mixed fields of the current states (pass A), once after any change.
+    void ensure_mixed(double dt) {
         if (!mix_valid_) {
-            dispatch_col<R>(n_, 0, stage_args(dt, ext, ext_stride), stream_);
+            dispatch_col<R>(n_, 0, stage_args(dt, nullptr, 0), stream_);
             mix_valid_ = true;
         }
+    }
+
+    // The launches of one RK4 step: shell noise, then 4 x (row, col).  The
+    // noise uses step index step_off, or *step_base + step_off when
+    // step_base is given (captured sequences).  With `events` (9 entries) the
+    // launches are bracketed by events so their durations can be read.
+    void launch_step(double dt, const cx<R> *ext, long long ext_stride,
+                     const uint64_t *step_base, uint64_t step_off, hipEvent_t *events) {
         if (!ext) {
-            NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_, noise_.p};
+            NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_off, noise_.p,
+                            step_base};
             const long count = (long)B_ * npairs_;
             hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
                                stream_, na);
@@ -752,9 +816,130 @@ template <typename R> class Engine final : public EngineBase {
                 check(hipEventRecord(events[2 * stage], stream_), "event");
         }
         check(hipGetLastError(), "RK4 kernels");
+    }
+
+    // One RK4 step launched eagerly.
+    void advance(double dt, const cx<R> *ext, long long ext_stride,
+                 hipEvent_t *events = nullptr) {
+        flush_parameters();
+        if (!mix_valid_) {
+            dispatch_col<R>(n_, 0, stage_args(dt, ext, ext_stride), stream_);
+            mix_valid_ = true;
+        }
+        launch_step(dt, ext, ext_stride, nullptr, step_, events);
         ++step_;
         last_dt_ = dt;
         max_dt_ = std::max(max_dt_, dt);
+    }
+
+    // The filter launches of one observation group (observations staged):
+    // log-weights, normalisation and resampling decision, optional weighted
+    // mean, resampling copies (no-ops on the device without resampling).
+    // `device_cycle`: group and last-group flag from ctr_ (captured cycles).
+    void launch_filter(size_t nobs, bool device_cycle, bool is_last, bool with_mean) {
+        PfArgs<R> a{(int)B_,   n_,        K_,           mix_.p,       (int)nobs,
+                    obs_x_,    obs_y_,    obs_v_,       obs_sd_,      pf_cs_.p,
+                    pf_sn_.p,  pf_wstate_.p, pf_wnorm_.p, pf_logw_.p, nullptr};
+        hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
+        const int P = (int)(B_ / pf_filters_);
+        PfControl c{P,           pf_logw_.p, pf_wstate_.p,      pf_wnorm_.p,
+                    pf_stats_.p, pf_anc_.p,  pf_uniforms_.p,    pf_max_uniforms_,
+                    d_keys_.p,   pf_seeds_.p, pf_group_,        is_last ? 1 : 0,
+                    device_cycle ? ctr_.p + 1 : nullptr};
+        hipLaunchKernelGGL(k_pf_normalize, dim3((unsigned)pf_filters_), dim3(256), 0, stream_, c);
+        if (with_mean) {
+            const long blocks = ((long)spec_ + 255) / 256;
+            hipLaunchKernelGGL(k_pf_mean<R>, dim3((unsigned)blocks), dim3(256), 0, stream_,
+                               (int)B_, (long)spec_, (const cx<R> *)w0_.p, pf_wnorm_.p,
+                               pf_mean_.p);
+        }
+        const long total = (long)B_ * (long)spec_;
+        const dim3 grid((unsigned)((total + 255) / 256));
+        hipLaunchKernelGGL(k_pf_gather<R>, grid, dim3(256), 0, stream_, (int)B_, (long)spec_,
+                           (const cx<R> *)w0_.p, w1_.p, (const int *)pf_anc_.p,
+                           (const double *)pf_stats_.p, P);
+        hipLaunchKernelGGL(k_pf_commit<R>, grid, dim3(256), 0, stream_, (int)B_, (long)spec_,
+                           (const cx<R> *)w1_.p, w0_.p, (const double *)pf_stats_.p, P);
+    }
+
+    // ------------------------------------------------ HIP graphs ----
+    // A launch sequence is captured on the second consecutive call with the
+    // same key (the first runs eagerly, so every lazy initialisation happens
+    // outside capture) and replayed afterwards.  Every device address a
+    // sequence holds is fixed for the engine's lifetime, except the filter
+    // and observation buffers, whose reallocation clears the cache.
+    struct GraphKey {
+        int kind; // 1: RK4 steps, 2: filter cycle
+        double dt;
+        size_t steps, nobs, filters;
+        const void *obs;
+        bool mean;
+        bool operator==(const GraphKey &o) const {
+            return kind == o.kind && dt == o.dt && steps == o.steps && nobs == o.nobs &&
+                   filters == o.filters && obs == o.obs && mean == o.mean;
+        }
+    };
+    struct CachedGraph {
+        GraphKey key;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+
+    void set_counters(uint64_t group, uint64_t is_last) {
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, stream_, ctr_.p, step_, group,
+                           is_last);
+    }
+
+    template <class F> void run_captured(const GraphKey &key, F &&launch) {
+        ++graph_clock_;
+        for (auto &g : graphs_)
+            if (g.key == key) {
+                g.used = graph_clock_;
+                check(hipGraphLaunch(g.exec, stream_), "graph launch");
+                return;
+            }
+        if (!(warm_ && warm_key_ == key)) {
+            launch();
+            warm_ = true;
+            warm_key_ = key;
+            return;
+        }
+        if (graphs_.size() >= kGraphCache) { // evict the least recently used
+            size_t lru = 0;
+            for (size_t i = 1; i < graphs_.size(); ++i)
+                if (graphs_[i].used < graphs_[lru].used)
+                    lru = i;
+            check(hipStreamSynchronize(stream_), "graph eviction");
+            (void)hipGraphExecDestroy(graphs_[lru].exec);
+            graphs_.erase(graphs_.begin() + (long)lru);
+        }
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+        try {
+            launch();
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g)
+                (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(stream_, &g), "capture end");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check(e, "graph instantiate");
+        graphs_.push_back(CachedGraph{key, exec, graph_clock_});
+        warm_ = false;
+        check(hipGraphLaunch(exec, stream_), "graph launch");
+    }
+
+    void clear_graphs() {
+        if (!graphs_.empty())
+            (void)hipStreamSynchronize(stream_);
+        for (auto &g : graphs_)
+            (void)hipGraphExecDestroy(g.exec);
+        graphs_.clear();
+        warm_ = false;
     }
 
     int n_;
@@ -789,6 +974,13 @@ template <typename R> class Engine final : public EngineBase {
     bool params_dirty_ = false, mix_valid_ = false;
     uint64_t step_ = 0;
     double last_dt_ = 0, max_dt_ = 0, last_cfl_rate_ = 0;
+    // captured launch sequences
+    static constexpr size_t kGraphSteps = 200, kGraphCache = 4;
+    DeviceBuffer<uint64_t> ctr_;
+    std::vector<CachedGraph> graphs_;
+    GraphKey warm_key_{};
+    bool warm_ = false, use_graph_ = true;
+    uint64_t graph_clock_ = 0;
 };
 
 // Parameters::validate (src/solver.cpp:31-40) plus the engine's own limits.
@@ -893,6 +1085,10 @@ int hzl_ensemble_init_grf(hzl_ensemble_t h) {
 
 int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps) {
     return guarded([&] { engine_of(h).step(dt, steps); });
+}
+
+int hzl_ensemble_set_graph(hzl_ensemble_t h, int enabled) {
+    return guarded([&] { engine_of(h).set_graph(enabled != 0); });
 }
 
 int hzl_ensemble_step_forced(hzl_ensemble_t h, double dt, const double *forcing, int shared) {

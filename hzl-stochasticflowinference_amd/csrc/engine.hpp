@@ -73,6 +73,7 @@ class EngineBase {
     virtual void get_state(size_t m, double *field) = 0;
     virtual void init_grf() = 0;
     virtual void step(double dt, size_t steps) = 0;
+    virtual void set_graph(bool on) = 0;
     virtual void step_forced(double dt, const double *forcing, bool shared) = 0;
     virtual void synchronize() = 0;
     virtual void diagnostics(double *out) = 0;

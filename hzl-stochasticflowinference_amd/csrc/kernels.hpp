@@ -139,8 +139,9 @@ template <typename R> struct NoiseArgs {
     const double *sigma;  // per member white-noise amplitude
     const uint64_t *keys; // per member Philox key
     double dt;
-    uint64_t step;
+    uint64_t step;        // step index, or the offset added to *step_base
     cx<R> *noise;
+    const uint64_t *step_base = nullptr; // device step counter (graph replays)
 };
 
 template <typename R> HZL_HD void noise_body(const NoiseArgs<R> &a, long i) {
@@ -151,7 +152,8 @@ template <typename R> HZL_HD void noise_body(const NoiseArgs<R> &a, long i) {
     cx<R> v = {0, 0};
     if (s > 0) {
         double g0, g1;
-        philox_normal2(a.keys[m], (uint32_t)p, a.step, TAG_NOISE, g0, g1);
+        const uint64_t step = a.step_base ? *a.step_base + a.step : a.step;
+        philox_normal2(a.keys[m], (uint32_t)p, step, TAG_NOISE, g0, g1);
         const double scale = s / sqrt(2.0 * a.dt);
         v = {(R)(scale * g0), (R)(scale * g1)};
     }
@@ -795,6 +797,8 @@ struct PfControl {
     const uint64_t *seeds;   // per filter seed
     uint64_t group;
     int is_last;
+    const uint64_t *cycle = nullptr; // device (group, is_last) replacing the two
+                                     // fields above (graph replays)
 };
 
 // One workgroup per filter: log-sum-exp normalisation, ESS, systematic
@@ -807,6 +811,8 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
     double *wnorm = a.wnorm + base, *wstate = a.wstate + base, *stats = a.stats + 8L * f;
     int *ancestors = a.ancestors + base;
     uint64_t *keys = a.keys + base;
+    const uint64_t group = a.cycle ? a.cycle[0] : a.group;
+    const bool is_last = a.cycle ? a.cycle[1] != 0 : a.is_last != 0;
     const double *uniforms = a.uniforms + (long)f * a.max_uniforms;
     double mx = -__builtin_huge_val();
     for (int i = tid; i < P; i += nt)
@@ -852,7 +858,7 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
         ctx.sync();
     }
     const double ess = 1 / red[0];
-    const bool resample = finite && ess < (double)P / 2 && !a.is_last &&
+    const bool resample = finite && ess < (double)P / 2 && !is_last &&
                           (int)stats[6] < a.max_uniforms;
     ctx.sync();
     if (tid == 0) {
@@ -898,7 +904,7 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
     for (int i = tid; i < P; i += nt) {
         wstate[i] = resample ? 1.0 / (double)P : wnorm[i];
         if (resample)
-            keys[i] = stream_seed(a.seeds[f], (a.group + 1) * (uint64_t)P + (uint64_t)i);
+            keys[i] = stream_seed(a.seeds[f], (group + 1) * (uint64_t)P + (uint64_t)i);
         else
             ancestors[i] = (int)base + i;
     }

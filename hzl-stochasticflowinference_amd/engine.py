@@ -25,7 +25,8 @@ ALL = C.c_size_t(-1).value  # (size_t)-1: every member
 EXPORTS = (
     "hzl_last_error", "hzl_ensemble_create", "hzl_ensemble_destroy",
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
-    "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_step_forced",
+    "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
+    "hzl_ensemble_step_forced",
     "hzl_ensemble_synchronize", "hzl_ensemble_diagnostics", "hzl_ensemble_profile_step",
     "hzl_ensemble_step_index",
     "hzl_ensemble_device_state", "hzl_ensemble_stream",
@@ -92,6 +93,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_get_state.argtypes = [vp, C.c_size_t, _dp]
     L.hzl_ensemble_init_grf.argtypes = [vp]
     L.hzl_ensemble_step.argtypes = [vp, C.c_double, C.c_size_t]
+    L.hzl_ensemble_set_graph.argtypes = [vp, C.c_int]
     L.hzl_ensemble_step_forced.argtypes = [vp, C.c_double, _dp, C.c_int]
     L.hzl_ensemble_synchronize.argtypes = [vp]
     L.hzl_ensemble_diagnostics.argtypes = [vp, _dp]
@@ -220,6 +222,10 @@ class Ensemble:
 
     def step(self, dt: float, steps: int = 1):
         _check(_lib.hzl_ensemble_step(self._h, dt, steps))
+
+    def set_graph(self, enabled: bool):
+        """Replay steps and filter cycles from captured HIP graphs (default on)."""
+        _check(_lib.hzl_ensemble_set_graph(self._h, int(bool(enabled))))
 
     def step_forced(self, dt: float, forcing, shared: bool = True):
         a = _fields(forcing, 1 if shared else self.members, self.n)

@@ -106,6 +106,18 @@ int hzl_ensemble_init_grf(hzl_ensemble_t h);
  * the device at every step and reported by hzl_ensemble_synchronize. */
 int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps);
 
+/* Replay launch sequences from captured HIP graphs (default on).  A
+ * hzl_ensemble_step of 2..200 steps (longer calls in chunks of 200), and a
+ * whole particle-filter cycle of hzl_pf_assimilate with 1..200 steps
+ * (forecast, log-weights, normalisation, resampling copies, which are
+ * device-side no-ops when the filter keeps its particles), is recorded on
+ * its second consecutive call with the same dt and sizes and replayed
+ * afterwards, with the step index, group and last-group flag read from
+ * device memory.  Results are identical to the launch-by-launch path (off).
+ * No counterpart in the reference, which launches every kernel from the host
+ * (src/cuda_solver.cu:172-218). */
+int hzl_ensemble_set_graph(hzl_ensemble_t h, int enabled);
+
 /* One RK4 step with host-provided midpoint forcing held through the four
  * stages: `forcing` holds one field (shared != 0) or one per member.  Checks
  * finiteness and conjugate symmetry to 1e-12 as CudaSolver::step(dt, F)
