@@ -438,6 +438,10 @@ def main() -> int:
     nu = dt = 5e-4
     ens = pkg.Ensemble(n, B, "fp32", nu=nu, alpha=0.0, sigma=0.05, kmin=1, kmax=4, amp=0.1,
                        kf=4, seed=SEED)
+    # launch shapes: time every variant on this workload (the engine already
+    # applied a remembered or freshly timed choice at creation)
+    tune_col, tune_row = ens.autotune()
+    cv, rv = ens.tuning
     ens.init_grf()
     ens.step(dt, args.warmup)
     ens.synchronize()
@@ -464,7 +468,9 @@ def main() -> int:
     k_ms = col_ms if dominant == "col" else row_ms
     k_bytes = B * (mb["col_launch"] if dominant == "col" else mb["row_launch"])
     achieved = k_bytes / (k_ms / 1e3)
-    traffic = measured_traffic(f"k_{dominant}<{n},float>", n, B)
+    variant = cv if dominant == "col" else rv
+    label = f"k_{dominant}<{n},float>" + (f" variant {variant}" if variant else "")
+    traffic = measured_traffic(label, n, B)
     diag = ens.diagnostics()
     ens.close()
 
@@ -488,7 +494,10 @@ def main() -> int:
                        "achieved_GBps": round(value * mb["member_step"] / 1e9, 1),
                        "frac_of_hbm_peak": round(value * mb["member_step"] / HBM_PEAK, 4),
                        "wall_seconds_timed": round(wall, 4)},
-        "roofline": {"bound": "hbm", "kernel": f"k_{dominant}<{n},float>",
+        "tuning": {"col_variant": cv, "row_variant": rv,
+                   "col_ms_per_step": [round(x, 4) for x in tune_col],
+                   "row_ms_per_step": [round(x, 4) for x in tune_row]},
+        "roofline": {"bound": "hbm", "kernel": label,
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "traffic_source": "profiles/ rocprofv3 FETCH_SIZE x2 + WRITE_SIZE"

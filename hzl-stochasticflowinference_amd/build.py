@@ -32,7 +32,8 @@ HIP_FLAGS = ["-O3", "-std=c++20", "--offload-arch=gfx950", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function"]
 SOURCES = [os.path.join(CSRC, f)
-           for f in ("engine.hip", "surrogate.hip", "enkf.hip", "pmcmc.hip", "pf_multi.hip")]
+           for f in ("engine.hip", "variants.hip", "surrogate.hip", "enkf.hip", "pmcmc.hip",
+                     "pf_multi.hip")]
 
 
 def _run(cmd, cwd=None):

@@ -7,11 +7,19 @@
 #include "hzl_amd.h"
 #include "engine.hpp"
 #include "tables.hpp"
+#include "variants.hpp"
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <sys/stat.h>
+#include <tuple>
 #include <limits>
 #include <random>
 #include <memory>
@@ -45,7 +53,7 @@ template <typename R> __global__ void __launch_bounds__(256) k_noise(NoiseArgs<R
 template <int N, typename R, bool CFL>
 __global__ void __launch_bounds__(Geometry<N>::ROW_THREADS)
     __attribute__((amdgpu_waves_per_eu(HZL_ROW_WPE))) k_row(RowArgs<R> a) {
-    __shared__ cx<R> lds[RowShape<N>::template lds_elems<R>()];
+    __shared__ cx<R> lds[RowShape<Geometry<N>>::template lds_elems<R>()];
     row_body<N, R, CFL>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 
@@ -217,6 +225,90 @@ template <typename R> void dispatch_row(int n, const RowArgs<R> &a, hipStream_t 
     throw std::invalid_argument("unsupported grid");
 }
 
+// ------------------------------------------------------ tuning cache ----
+// Chosen launch shapes per (device architecture, variant table, grid,
+// precision, member-count bucket): remembered for the process and in a text
+// file ($HZL_TUNE_CACHE, else $HOME/.cache/hzl_amd/launch_shapes.txt), one
+// "key col_variant row_variant" line per entry, the last line for a key wins.
+std::mutex &tune_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+std::map<std::string, std::pair<int, int>> &tune_memo() {
+    static std::map<std::string, std::pair<int, int>> m;
+    return m;
+}
+
+std::string tune_file() {
+    if (const char *p = std::getenv("HZL_TUNE_CACHE"))
+        return p;
+    if (const char *h = std::getenv("HOME"))
+        return std::string(h) + "/.cache/hzl_amd/launch_shapes.txt";
+    return "";
+}
+
+std::string tune_key(int n, int precision, size_t members) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    size_t bucket = 1;
+    while (bucket < members)
+        bucket *= 2;
+    std::ostringstream k;
+    std::string arch = prop.gcnArchName;
+    for (char &c : arch)
+        if (c == ' ')
+            c = '_';
+    k << arch << "/" << std::hex << variant_table_id() << std::dec << "/n" << n << "/"
+      << (precision == HZL_FP32 ? "fp32" : "fp64") << "/b" << bucket;
+    return k.str();
+}
+
+bool tune_lookup(const std::string &key, int *cv, int *rv) {
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    auto it = tune_memo().find(key);
+    if (it == tune_memo().end()) {
+        const std::string path = tune_file();
+        if (path.empty())
+            return false;
+        std::ifstream in(path);
+        std::string line;
+        bool found = false;
+        std::pair<int, int> v;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string k;
+            int c = -1, r = -1;
+            if (ls >> k >> c >> r && k == key && c >= 0 && r >= 0) {
+                v = {c, r};
+                found = true;
+            }
+        }
+        if (!found)
+            return false;
+        it = tune_memo().emplace(key, v).first;
+    }
+    *cv = it->second.first;
+    *rv = it->second.second;
+    return true;
+}
+
+void tune_store(const std::string &key, int cv, int rv) {
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    tune_memo()[key] = {cv, rv};
+    const std::string path = tune_file();
+    if (path.empty())
+        return;
+    // best effort: create the directory chain, ignore a read-only home
+    for (size_t i = path.find('/', 1); i != std::string::npos; i = path.find('/', i + 1))
+        (void)::mkdir(path.substr(0, i).c_str(), 0755);
+    std::ofstream out(path, std::ios::app);
+    if (out)
+        out << key << " " << cv << " " << rv << "\n";
+}
+
 // ------------------------------------------------------------- engine ----
 template <typename R> class Engine final : public EngineBase {
   public:
@@ -236,6 +328,7 @@ template <typename R> class Engine final : public EngineBase {
         const auto tw = twiddles<R>(n);
         tw_ = DeviceBuffer<cx<R>>(tw.size());
         upload(tw_.p, tw.data(), tw.size());
+        tw_col_ = tw_row_ = tw_.p;
         const auto pairs = pair_table(p.forcing_min, p.forcing_max, &npairs_);
         pairs_ = DeviceBuffer<int>(pairs.size());
         upload(pairs_.p, pairs.data(), pairs.size());
@@ -393,6 +486,113 @@ template <typename R> class Engine final : public EngineBase {
         clear_graphs();
     }
 
+    // ------------------------------------------------ launch shapes ----
+    int col_variants() const override {
+        return std::is_same_v<R, float> ? col_variant_count(n_) : 1;
+    }
+    int row_variants() const override {
+        return std::is_same_v<R, float> ? row_variant_count(n_) : 1;
+    }
+
+    void set_tuning(int cv, int rv) override {
+        if (cv < 0 || rv < 0 || cv >= col_variants() || rv >= row_variants())
+            throw std::invalid_argument("no such launch-shape variant for this grid and precision");
+        check(hipStreamSynchronize(stream_), "tuning switch");
+        clear_graphs(); // captured launches name the previous kernels
+        const int ec = cv ? col_variant_elems(n_, cv) : elems_per_thread(n_);
+        const int er = rv ? row_variant_elems(n_, rv) : elems_per_thread(n_);
+        tw_col_ = twiddle_table(ec);
+        tw_row_ = twiddle_table(er);
+        col_var_ = cv;
+        row_var_ = rv;
+        // mixed and nonlinear fields are stored in natural y order, so they
+        // stay valid across variants
+    }
+
+    void get_tuning(int *cv, int *rv) const override {
+        *cv = col_var_;
+        *rv = row_var_;
+    }
+
+    // Time every column variant (with the current row variant), keep the
+    // fastest, then every row variant with it.  A variant replaces the
+    // default only when it is more than 1% faster.  The states are restored
+    // afterwards.  col_ms / row_ms (optional) receive the per-step kernel
+    // times of each variant.  With use_cache, a remembered choice is applied
+    // without timing (the outputs are then left unchanged).
+    void tune(bool use_cache, double *col_ms, double *row_ms) override {
+        const int nc = col_variants(), nr = row_variants();
+        if (nc == 1 && nr == 1)
+            return;
+        const std::string key = tune_key(n_, sizeof(R) == 4 ? HZL_FP32 : HZL_FP64, B_);
+        int cv = 0, rv = 0;
+        if (use_cache && tune_lookup(key, &cv, &rv) && cv < nc && rv < nr) {
+            set_tuning(cv, rv);
+            return;
+        }
+        flush_parameters();
+        DeviceBuffer<cx<R>> backup(B_ * spec_);
+        check(hipMemcpyAsync(backup.p, w0_.p, B_ * spec_ * sizeof(cx<R>),
+                             hipMemcpyDeviceToDevice, stream_),
+              "tuning backup");
+        const bool mix_was_valid = mix_valid_;
+        const double dt = 1e-6;
+        // per-step (col, row) kernel milliseconds of one variant pair: the
+        // minimum over 3 timed steps after one untimed step
+        auto measure = [&](int c, int r) {
+            set_tuning(c, r);
+            run_col(0, stage_args(dt, nullptr, 0));
+            hipEvent_t ev[9];
+            for (auto &e : ev)
+                check(hipEventCreate(&e), "hipEventCreate");
+            double best_c = 1e300, best_r = 1e300;
+            for (int rep = 0; rep < 4; ++rep) {
+                launch_step(dt, nullptr, 0, nullptr, step_, ev);
+                check(hipEventSynchronize(ev[8]), "tuning step");
+                double tc = 0, tr = 0;
+                for (int i = 0; i < 8; ++i) {
+                    float ms = 0;
+                    check(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "hipEventElapsedTime");
+                    (i % 2 == 0 ? tr : tc) += ms;
+                }
+                if (rep > 0) {
+                    best_c = std::min(best_c, tc);
+                    best_r = std::min(best_r, tr);
+                }
+            }
+            for (auto &e : ev)
+                (void)hipEventDestroy(e);
+            return std::make_pair(best_c, best_r);
+        };
+        std::vector<double> ct(nc), rt(nr);
+        for (int c = 0; c < nc; ++c)
+            ct[c] = measure(c, 0).first;
+        int bc = 0;
+        for (int c = 1; c < nc; ++c)
+            if (ct[c] < 0.99 * ct[0] && ct[c] < ct[bc])
+                bc = c;
+        for (int r = 0; r < nr; ++r)
+            rt[r] = measure(bc, r).second;
+        int br = 0;
+        for (int r = 1; r < nr; ++r)
+            if (rt[r] < 0.99 * rt[0] && rt[r] < rt[br])
+                br = r;
+        check(hipMemcpyAsync(w0_.p, backup.p, B_ * spec_ * sizeof(cx<R>),
+                             hipMemcpyDeviceToDevice, stream_),
+              "tuning restore");
+        check(hipMemsetAsync(cfl_.p, 0, cfl_.count * sizeof(double), stream_), "reset CFL");
+        check(hipStreamSynchronize(stream_), "tuning restore");
+        mix_valid_ = false;
+        if (mix_was_valid)
+            ensure_mixed(1.0);
+        set_tuning(bc, br);
+        tune_store(key, bc, br);
+        if (col_ms)
+            std::copy(ct.begin(), ct.end(), col_ms);
+        if (row_ms)
+            std::copy(rt.begin(), rt.end(), row_ms);
+    }
+
     void step_forced(double dt, const double *forcing, bool shared) override {
         validate_dt(dt);
         const size_t nn = (size_t)n_ * n_, count = shared ? 1 : B_;
@@ -447,13 +647,13 @@ template <typename R> class Engine final : public EngineBase {
         // CFL rate of the current state (src/solver.cpp:392-396): one row pass
         // over the mixed fields of the current state into a private buffer.
         if (!mix_valid_) {
-            dispatch_col<R>(n_, 0, stage_args(1.0, nullptr, 0), stream_);
+            run_col(0, stage_args(1.0, nullptr, 0));
             mix_valid_ = true;
         }
         check(hipMemsetAsync(cfl_diag_.p, 0, cfl_diag_.count * sizeof(double), stream_),
               "clear CFL");
-        RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_.p, cfl_diag_.p};
-        dispatch_row<R>(n_, r, stream_);
+        RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_row_, cfl_diag_.p};
+        run_row(r);
         DeviceBuffer<double> d(B_ * 6);
         // the noise buffer still holds the last step's shell noise
         DiagArgs<R> a{K_, w0_.p, d_nu_.p, forcing_model(), step_ > 0 ? 1 : 0, cfl_diag_.p,
@@ -642,7 +842,7 @@ template <typename R> class Engine final : public EngineBase {
                                             "positive uncertainty");
         flush_parameters();
         if (!mix_valid_) {
-            dispatch_col<R>(n_, 0, stage_args(1.0, nullptr, 0), stream_);
+            run_col(0, stage_args(1.0, nullptr, 0));
             mix_valid_ = true;
         }
         if (pf_cs_.count != (size_t)n_) {
@@ -770,7 +970,7 @@ template <typename R> class Engine final : public EngineBase {
         a.acc = acc_.p;
         a.nl = nl_.p;
         a.mix = mix_.p;
-        a.tw = tw_.p;
+        a.tw = tw_col_;
         a.nu = d_nu_.p;
         a.alpha = d_alpha_.p;
         a.forcing = forcing_model();
@@ -782,10 +982,46 @@ template <typename R> class Engine final : public EngineBase {
         return a;
     }
 
+    // Column / row launches through the selected launch-shape variant.
+    void run_col(int mode, const StageArgs<R> &a) {
+        if constexpr (std::is_same_v<R, float>) {
+            if (col_var_ > 0) {
+                if (!launch_col_variant(n_, col_var_, mode, a, stream_))
+                    throw std::logic_error("unknown column variant");
+                return;
+            }
+        }
+        dispatch_col<R>(n_, mode, a, stream_);
+    }
+
+    void run_row(const RowArgs<R> &a) {
+        if constexpr (std::is_same_v<R, float>) {
+            if (row_var_ > 0) {
+                if (!launch_row_variant(n_, row_var_, a, stream_))
+                    throw std::logic_error("unknown row variant");
+                return;
+            }
+        }
+        dispatch_row<R>(n_, a, stream_);
+    }
+
+    // Twiddle table of the T x E split (the default one is tw_).
+    const cx<R> *twiddle_table(int E) {
+        if (E == elems_per_thread(n_))
+            return tw_.p;
+        for (auto &t : tw_extra_)
+            if (t.first == E)
+                return t.second.p;
+        const auto tw = twiddles<R>(n_, E);
+        tw_extra_.emplace_back(E, DeviceBuffer<cx<R>>(tw.size()));
+        upload(tw_extra_.back().second.p, tw.data(), tw.size());
+        return tw_extra_.back().second.p;
+    }
+
     // Mixed fields of the current states (pass A), once after any change.
     void ensure_mixed(double dt) {
         if (!mix_valid_) {
-            dispatch_col<R>(n_, 0, stage_args(dt, nullptr, 0), stream_);
+            run_col(0, stage_args(dt, nullptr, 0));
             mix_valid_ = true;
         }
     }
@@ -803,15 +1039,15 @@ template <typename R> class Engine final : public EngineBase {
             hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
                                stream_, na);
         }
-        RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_.p, nullptr};
+        RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_row_, nullptr};
         if (events)
             check(hipEventRecord(events[0], stream_), "event");
         for (int stage = 1; stage <= 4; ++stage) {
             r.cfl = stage == 1 ? cfl_.p : nullptr;
-            dispatch_row<R>(n_, r, stream_);
+            run_row(r);
             if (events)
                 check(hipEventRecord(events[2 * stage - 1], stream_), "event");
-            dispatch_col<R>(n_, stage, stage_args(dt, ext, ext_stride), stream_);
+            run_col(stage, stage_args(dt, ext, ext_stride));
             if (events)
                 check(hipEventRecord(events[2 * stage], stream_), "event");
         }
@@ -823,7 +1059,7 @@ template <typename R> class Engine final : public EngineBase {
                  hipEvent_t *events = nullptr) {
         flush_parameters();
         if (!mix_valid_) {
-            dispatch_col<R>(n_, 0, stage_args(dt, ext, ext_stride), stream_);
+            run_col(0, stage_args(dt, ext, ext_stride));
             mix_valid_ = true;
         }
         launch_step(dt, ext, ext_stride, nullptr, step_, events);
@@ -974,6 +1210,10 @@ template <typename R> class Engine final : public EngineBase {
     bool params_dirty_ = false, mix_valid_ = false;
     uint64_t step_ = 0;
     double last_dt_ = 0, max_dt_ = 0, last_cfl_rate_ = 0;
+    // launch shapes (variants.hpp) and their twiddle tables
+    int col_var_ = 0, row_var_ = 0;
+    const cx<R> *tw_col_ = nullptr, *tw_row_ = nullptr;
+    std::vector<std::pair<int, DeviceBuffer<cx<R>>>> tw_extra_;
     // captured launch sequences
     static constexpr size_t kGraphSteps = 200, kGraphCache = 4;
     DeviceBuffer<uint64_t> ctr_;
@@ -1046,8 +1286,40 @@ int hzl_ensemble_create(hzl_ensemble_t *out, size_t n, size_t members, int preci
         else
             h->engine =
                 std::make_unique<hzl_amd::Engine<double>>((int)n, members, *params, seed);
+        // large FP32 ensembles pick their launch shapes (remembered choice,
+        // else timed now, on the zero initial state); HZL_AUTOTUNE=0 keeps
+        // the defaults
+        const char *env = std::getenv("HZL_AUTOTUNE");
+        const bool autotune = !(env && std::strcmp(env, "0") == 0);
+        if (autotune && (double)members * (double)n * (double)n >= (double)(1 << 22))
+            h->engine->tune(true, nullptr, nullptr);
         *out = h.release();
     });
+}
+
+int hzl_ensemble_variant_count(hzl_ensemble_t h, int *col_variants, int *row_variants) {
+    return guarded([&] {
+        if (!col_variants || !row_variants)
+            throw std::invalid_argument("null argument");
+        *col_variants = engine_of(h).col_variants();
+        *row_variants = engine_of(h).row_variants();
+    });
+}
+
+int hzl_ensemble_set_tuning(hzl_ensemble_t h, int col_variant, int row_variant) {
+    return guarded([&] { engine_of(h).set_tuning(col_variant, row_variant); });
+}
+
+int hzl_ensemble_get_tuning(hzl_ensemble_t h, int *col_variant, int *row_variant) {
+    return guarded([&] {
+        if (!col_variant || !row_variant)
+            throw std::invalid_argument("null argument");
+        engine_of(h).get_tuning(col_variant, row_variant);
+    });
+}
+
+int hzl_ensemble_autotune(hzl_ensemble_t h, int use_cache, double *col_ms, double *row_ms) {
+    return guarded([&] { engine_of(h).tune(use_cache != 0, col_ms, row_ms); });
 }
 
 int hzl_ensemble_destroy(hzl_ensemble_t h) {

@@ -74,6 +74,12 @@ class EngineBase {
     virtual void init_grf() = 0;
     virtual void step(double dt, size_t steps) = 0;
     virtual void set_graph(bool on) = 0;
+    // launch-shape variants of the RK4 kernels (variant 0: the default)
+    virtual int col_variants() const = 0;
+    virtual int row_variants() const = 0;
+    virtual void set_tuning(int col_variant, int row_variant) = 0;
+    virtual void get_tuning(int *col_variant, int *row_variant) const = 0;
+    virtual void tune(bool use_cache, double *col_ms, double *row_ms) = 0;
     virtual void step_forced(double dt, const double *forcing, bool shared) = 0;
     virtual void synchronize() = 0;
     virtual void diagnostics(double *out) = 0;

@@ -52,18 +52,23 @@ template <> struct Shape<512> {
     static constexpr int E = 32;
 };
 
-template <int N> struct Geometry {
-    static constexpr int E = Shape<N>::E;
+// Launch shape of the column and row kernels at grid N: EV values per lane
+// (an N = T x EV two-pass transform), CT threads per column workgroup and PPW
+// row pairs per row workgroup (0: the default).  Shape<N> gives the default;
+// other shapes are variants the autotuner times.
+template <int N, int EV = Shape<N>::E, int CT = 64, int PPW = 0> struct Geometry {
+    static constexpr int NN = N;
+    static constexpr int E = EV;
     static constexpr int T = N / E;
     static constexpr int K = (N - 1) / 3;
     static constexpr int KP = K + 1;
     static constexpr int J = 2 * K + 1;
     static constexpr int TILES = N / 16;
-    // column kernel: one 64-lane workgroup, G columns
-    static constexpr int COL_THREADS = 64;
+    // column kernel: CT lanes per workgroup, G columns
+    static constexpr int COL_THREADS = CT;
     static constexpr int COL_G = COL_THREADS / T;
     // row kernel: PPW row pairs per workgroup
-    static constexpr int ROW_PPW = (64 / T) > 8 ? (64 / T) : 8;
+    static constexpr int ROW_PPW = PPW ? PPW : ((64 / T) > 8 ? (64 / T) : 8);
     static constexpr int ROW_THREADS = ROW_PPW * T;
     static constexpr long SPEC = (long)KP * J; // complex entries per member state
     static constexpr long MIXF = (long)KP * N; // complex entries per mixed field
@@ -206,9 +211,9 @@ template <typename R> struct StageArgs {
 };
 
 // MODE 0: pass A only (mixed fields of w0); MODE 1..4: RK4 stage MODE.
-template <int N, int MODE, typename R, class Ctx>
+template <int N, int MODE, typename R, class G = Geometry<N>, class Ctx>
 HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long block) {
-    using G = Geometry<N>;
+    static_assert(G::NN == N, "launch shape of another grid");
     using FFT = GroupFFT<N, G::E, R>;
     constexpr int E = G::E, T = G::T, K = G::K, KP = G::KP, J = G::J;
     const int tid = ctx.tid();
@@ -352,8 +357,9 @@ template <typename R> struct RowArgs {
     double *cfl; // per row pair running max of |u|+|v| (CFL variant only)
 };
 
-template <int N> struct RowShape {
-    using G = Geometry<N>;
+template <class GEO> struct RowShape {
+    using G = GEO;
+    static constexpr int N = G::NN;
     static constexpr int ROWS = 2 * G::ROW_PPW;      // rows per workgroup
     static constexpr int TILES = ROWS / 16;          // 16-row tiles per workgroup
     static constexpr int PITCH = 17;                 // staged kx-row pitch (bank-conflict free)
@@ -369,9 +375,10 @@ template <int N> struct RowShape {
 // vector loads by consecutive lanes.
 // A field's share of the workgroup's 16-row tile held in registers by one
 // thread: ITERS pairs of consecutive complex entries.
-template <int N, typename R> struct RowTile {
-    using G = Geometry<N>;
-    using RS = RowShape<N>;
+template <class GEO, typename R> struct RowTile {
+    using G = GEO;
+    using RS = RowShape<G>;
+    static constexpr int N = G::NN;
     struct alignas(4 * sizeof(R)) pair2 {
         cx<R> u, v;
     };
@@ -414,17 +421,17 @@ template <int N, typename R> struct RowTile {
 };
 
 // Offset of (kx, local row ly) in a staged field.
-template <int N> HZL_HD int staged(int kx, int ly) {
-    using RS = RowShape<N>;
-    return (ly >> 4) * (Geometry<N>::KP * RS::PITCH) + kx * RS::PITCH + (ly & 15);
+template <class G> HZL_HD int staged(int kx, int ly) {
+    using RS = RowShape<G>;
+    return (ly >> 4) * (G::KP * RS::PITCH) + kx * RS::PITCH + (ly & 15);
 }
 
 // One row-pair group: build the packed spectra z1 = u + i wx (fields B, C) or
 // z2 = v + i wy (fields A, D) of local row ly from the staged fields P, Q.
-template <int N, typename R, bool FIRST>
-HZL_HD void row_spectrum(cx<R> (&z)[Geometry<N>::E], const cx<R> *P, const cx<R> *Q, int t,
+template <class G, typename R, bool FIRST>
+HZL_HD void row_spectrum(cx<R> (&z)[G::E], const cx<R> *P, const cx<R> *Q, int t,
                          int ly) {
-    using G = Geometry<N>;
+    constexpr int N = G::NN;
     using FFT = GroupFFT<N, G::E, R>;
     constexpr int K = G::K;
 #pragma unroll
@@ -434,7 +441,7 @@ HZL_HD void row_spectrum(cx<R> (&z)[Geometry<N>::E], const cx<R> *P, const cx<R>
         z[idx] = {0, 0};
         if (!FFT::slot_outside(idx, K) && (pos || neg)) {
             const int kx = pos ? k : N - k;
-            const int o = staged<N>(kx, ly);
+            const int o = staged<G>(kx, ly);
             const cx<R> p = P[o], q = Q[o];
             const R fx = (R)kx;
             // FIRST: P = (ky psi)~, Q = w~:   a = u = i P,        b = wx = i kx Q
@@ -451,11 +458,11 @@ HZL_HD void row_spectrum(cx<R> (&z)[Geometry<N>::E], const cx<R> *P, const cx<R>
     }
 }
 
-template <int N, typename R, bool CFL, class Ctx>
+template <int N, typename R, bool CFL, class G = Geometry<N>, class Ctx>
 HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block) {
-    using G = Geometry<N>;
+    static_assert(G::NN == N, "launch shape of another grid");
     using FFT = GroupFFT<N, G::E, R>;
-    using RS = RowShape<N>;
+    using RS = RowShape<G>;
     constexpr int E = G::E, T = G::T, K = G::K, KP = G::KP, PPW = G::ROW_PPW;
     const int tid = ctx.tid();
     const int g = tid / T, t = tid % T; // group-contiguous lanes: conflict-free exchanges
@@ -473,13 +480,13 @@ HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block
     // u wx from (ky psi)~ and w~; the loads of psi~ and (ky w)~ are issued
     // right after, so they arrive while the first pair is transformed.
     {
-        RowTile<N, R> tb, tc;
+        RowTile<G, R> tb, tc;
         tb.load(ctx, a, row0, 1);
         tc.load(ctx, a, row0, 2);
         tb.store(ctx, a, row0, S0);
         tc.store(ctx, a, row0, S1);
     }
-    RowTile<N, R> ta, td;
+    RowTile<G, R> ta, td;
 #ifdef HZL_ROW_PREFETCH
     ta.load(ctx, a, row0, 0);
     td.load(ctx, a, row0, 3);
@@ -487,8 +494,8 @@ HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block
     ctx.sync();
     {
         cx<R> za[E], zb[E];
-        row_spectrum<N, R, true>(za, S0, S1, t, ly);
-        row_spectrum<N, R, true>(zb, S0, S1, t, ly + 1);
+        row_spectrum<G, R, true>(za, S0, S1, t, ly);
+        row_spectrum<G, R, true>(zb, S0, S1, t, ly + 1);
         ctx.sync();
         FFT::inverse(ctx, za, L, t, a.tw);
 #pragma unroll
@@ -517,8 +524,8 @@ HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block
     ctx.sync();
     {
         cx<R> za[E], zb[E];
-        row_spectrum<N, R, false>(za, S0, S1, t, ly);
-        row_spectrum<N, R, false>(zb, S0, S1, t, ly + 1);
+        row_spectrum<G, R, false>(za, S0, S1, t, ly);
+        row_spectrum<G, R, false>(zb, S0, S1, t, ly + 1);
         ctx.sync();
         FFT::inverse(ctx, za, L, t, a.tw);
 #pragma unroll

@@ -33,9 +33,12 @@ inline int elems_per_thread(int n) {
     throw std::invalid_argument("grid must be a power of two between 16 and 512");
 }
 
-// tw[t*E + k2] = exp(-2 pi i t k2 / n), evaluated in double, rounded once.
-template <typename R> std::vector<cx<R>> twiddles(int n) {
-    const int E = elems_per_thread(n), T = n / E;
+// tw[t*E + k2] = exp(-2 pi i t k2 / n), evaluated in double, rounded once,
+// for the T x E split with E values per lane (default: the grid's Shape).
+template <typename R> std::vector<cx<R>> twiddles(int n, int E = 0) {
+    if (E == 0)
+        E = elems_per_thread(n);
+    const int T = n / E;
     std::vector<cx<R>> tw((size_t)n);
     for (int t = 0; t < T; ++t)
         for (int k2 = 0; k2 < E; ++k2) {

@@ -1,0 +1,52 @@
+// variants.hpp -- launch-shape variants of the FP32 RK4 kernels that the
+// autotuner times against the default shape (variant 0, Shape<N> with the
+// register budgets of engine.hip).
+//
+// A column variant is (values per lane E, i.e. the T x E split of the y
+// transform; lanes per workgroup; waves-per-EU hint).  A row variant is (E of
+// the x transforms; row pairs per workgroup; waves-per-EU hint).  Every
+// variant computes the same stage; only the FFT factorisation (and so the
+// FP32 rounding) and the scheduling differ.
+#pragma once
+#include "kernels.hpp"
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+// X(N, index, E, lanes per workgroup, waves per EU)
+#define HZL_COL_VARIANTS(X)                                                                    \
+    X(128, 1, 16, 128, 1)                                                                       \
+    X(128, 2, 32, 64, 1)                                                                        \
+    X(128, 3, 16, 64, 2)                                                                        \
+    X(256, 1, 16, 128, 1)                                                                       \
+    X(256, 2, 32, 64, 1)                                                                        \
+    X(256, 3, 16, 64, 2)                                                                        \
+    X(512, 1, 32, 128, 1)                                                                       \
+    X(512, 2, 32, 64, 2)
+
+// X(N, index, E, row pairs per workgroup, waves per EU)
+#define HZL_ROW_VARIANTS(X)                                                                    \
+    X(128, 1, 16, 16, 2)                                                                        \
+    X(128, 2, 32, 16, 2)                                                                        \
+    X(128, 3, 16, 8, 2)                                                                         \
+    X(256, 1, 16, 8, 2)                                                                         \
+    X(256, 2, 16, 8, 4)                                                                         \
+    X(256, 3, 16, 16, 2)                                                                        \
+    X(256, 4, 32, 8, 2)                                                                         \
+    X(512, 1, 32, 8, 2)
+
+namespace hzl_amd {
+
+// FNV-1a hash of the variant lists above (keys the tuning cache, so a cached
+// choice never names a variant of another build's table).
+uint64_t variant_table_id();
+// Number of column / row variants at grid n for FP32, default included.
+int col_variant_count(int n);
+int row_variant_count(int n);
+// Values per lane of a variant (selects the twiddle table).
+int col_variant_elems(int n, int v);
+int row_variant_elems(int n, int v);
+// Launch variant v >= 1 (FP32); false when (n, v) names none.
+bool launch_col_variant(int n, int v, int mode, const StageArgs<float> &a, hipStream_t s);
+bool launch_row_variant(int n, int v, const RowArgs<float> &a, hipStream_t s);
+
+} // namespace hzl_amd

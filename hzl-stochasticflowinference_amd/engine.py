@@ -26,6 +26,8 @@ EXPORTS = (
     "hzl_last_error", "hzl_ensemble_create", "hzl_ensemble_destroy",
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
     "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
+    "hzl_ensemble_variant_count", "hzl_ensemble_set_tuning", "hzl_ensemble_get_tuning",
+    "hzl_ensemble_autotune",
     "hzl_ensemble_step_forced",
     "hzl_ensemble_synchronize", "hzl_ensemble_diagnostics", "hzl_ensemble_profile_step",
     "hzl_ensemble_step_index",
@@ -94,6 +96,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_init_grf.argtypes = [vp]
     L.hzl_ensemble_step.argtypes = [vp, C.c_double, C.c_size_t]
     L.hzl_ensemble_set_graph.argtypes = [vp, C.c_int]
+    ipp = C.POINTER(C.c_int)
+    L.hzl_ensemble_variant_count.argtypes = [vp, ipp, ipp]
+    L.hzl_ensemble_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.hzl_ensemble_get_tuning.argtypes = [vp, ipp, ipp]
+    L.hzl_ensemble_autotune.argtypes = [vp, C.c_int, _dp, _dp]
     L.hzl_ensemble_step_forced.argtypes = [vp, C.c_double, _dp, C.c_int]
     L.hzl_ensemble_synchronize.argtypes = [vp]
     L.hzl_ensemble_diagnostics.argtypes = [vp, _dp]
@@ -226,6 +233,32 @@ class Ensemble:
     def set_graph(self, enabled: bool):
         """Replay steps and filter cycles from captured HIP graphs (default on)."""
         _check(_lib.hzl_ensemble_set_graph(self._h, int(bool(enabled))))
+
+    # -- launch-shape variants (autotuner) ----------------------------------
+    def variant_count(self):
+        c, r = C.c_int(), C.c_int()
+        _check(_lib.hzl_ensemble_variant_count(self._h, C.byref(c), C.byref(r)))
+        return c.value, r.value
+
+    @property
+    def tuning(self):
+        c, r = C.c_int(), C.c_int()
+        _check(_lib.hzl_ensemble_get_tuning(self._h, C.byref(c), C.byref(r)))
+        return c.value, r.value
+
+    @tuning.setter
+    def tuning(self, cr):
+        _check(_lib.hzl_ensemble_set_tuning(self._h, int(cr[0]), int(cr[1])))
+
+    def autotune(self, use_cache: bool = False):
+        """Time every launch-shape variant now and keep the fastest; returns
+        (column ms per step per variant, row ms per step per variant).  With
+        use_cache a remembered choice is applied instead (timings all zero)."""
+        nc, nr = self.variant_count()
+        col, row = np.zeros(nc), np.zeros(nr)
+        _check(_lib.hzl_ensemble_autotune(self._h, int(use_cache), col.ctypes.data_as(_dp),
+                                          row.ctypes.data_as(_dp)))
+        return col, row
 
     def step_forced(self, dt: float, forcing, shared: bool = True):
         a = _fields(forcing, 1 if shared else self.members, self.n)

@@ -118,6 +118,27 @@ int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps);
  * (src/cuda_solver.cu:172-218). */
 int hzl_ensemble_set_graph(hzl_ensemble_t h, int enabled);
 
+/* Launch-shape autotuner.  FP32 ensembles at 128^2, 256^2 and 512^2 can run
+ * their column and row kernels in several compiled shapes (values per lane,
+ * i.e. the T x E split of each transform; workgroup size; register budget);
+ * variant 0 is the default shape and the only one in FP64 and at other
+ * grids.  hzl_ensemble_create of an FP32 ensemble with members * n^2 >= 2^22
+ * applies the shapes remembered for (device architecture, variant table,
+ * n, member-count bucket) in the process or in the cache file
+ * ($HZL_TUNE_CACHE, else $HOME/.cache/hzl_amd/launch_shapes.txt), else times
+ * the variants on the zero initial state and remembers the fastest
+ * (environment HZL_AUTOTUNE=0: keep the defaults).  hzl_ensemble_autotune
+ * times them now and restores the states (use_cache != 0: apply a
+ * remembered choice instead, if there is one); col_ms / row_ms (optional,
+ * one entry per variant) receive the per-step kernel milliseconds of a
+ * timing run and are left unchanged otherwise.  Variants
+ * differ only in FP32 rounding.  No counterpart in the reference (cuFFT picks
+ * its own plans, src/cuda_solver.cu:153-157). */
+int hzl_ensemble_variant_count(hzl_ensemble_t h, int *col_variants, int *row_variants);
+int hzl_ensemble_set_tuning(hzl_ensemble_t h, int col_variant, int row_variant);
+int hzl_ensemble_get_tuning(hzl_ensemble_t h, int *col_variant, int *row_variant);
+int hzl_ensemble_autotune(hzl_ensemble_t h, int use_cache, double *col_ms, double *row_ms);
+
 /* One RK4 step with host-provided midpoint forcing held through the four
  * stages: `forcing` holds one field (shared != 0) or one per member.  Checks
  * finiteness and conjugate symmetry to 1e-12 as CudaSolver::step(dt, F)

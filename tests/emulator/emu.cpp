@@ -9,6 +9,7 @@
 #include "gemm_mfma.hpp"
 #include "surrogate_kernels.hpp"
 #include "tables.hpp"
+#include "variants.hpp"
 
 #include <barrier>
 #include <cstring>
@@ -98,11 +99,12 @@ template <class Body> void launch(long blocks, int threads, size_t lds_bytes, Bo
         t.join();
 }
 
-template <int N, typename R> struct Sim {
+// GC / GR: launch shapes of the column and row kernels (variants.hpp).
+template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>> struct Sim {
     using G = Geometry<N>;
-    using FFT = GroupFFT<N, G::E, R>;
+    using FFT = GroupFFT<N, GC::E, R>;
     int B;
-    std::vector<cx<R>> w0, w1, acc, mix, nl, ext, tw, noise;
+    std::vector<cx<R>> w0, w1, acc, mix, nl, ext, tw, tw_row, noise;
     int npairs = 0;
     std::vector<double> cfl;
     std::vector<double> nu, alpha, sigma, amp;
@@ -115,7 +117,8 @@ template <int N, typename R> struct Sim {
         int kmax_, uint64_t seed)
         : B(b), w0((size_t)b * G::SPEC), w1((size_t)b * G::SPEC), acc((size_t)b * G::SPEC),
           mix((size_t)b * 4 * G::MIXF), nl((size_t)b * G::MIXF), ext((size_t)b * G::SPEC),
-          tw(twiddles<R>(N)), cfl((size_t)b * (N / 2), 0.0), nu(b, nu0), alpha(b, alpha0),
+          tw(twiddles<R>(N, GC::E)), tw_row(twiddles<R>(N, GR::E)), cfl((size_t)b * (N / 2), 0.0),
+          nu(b, nu0), alpha(b, alpha0),
           sigma(b, sigma0), amp(b, amp0), keys(b), pairs(pair_table(kmin_, kmax_)), kmin(kmin_),
           kmax(kmax_), kf(kf_) {
         for (int m = 0; m < b; ++m)
@@ -135,10 +138,10 @@ template <int N, typename R> struct Sim {
     }
 
     template <int MODE> void col_mode(const StageArgs<R> &a) {
-        const long blocks = ((long)B * G::KP + G::COL_G - 1) / G::COL_G;
-        launch(blocks, G::COL_THREADS, sizeof(cx<R>) * G::COL_G * FFT::LDS_ELEMS,
+        const long blocks = ((long)B * G::KP + GC::COL_G - 1) / GC::COL_G;
+        launch(blocks, GC::COL_THREADS, sizeof(cx<R>) * GC::COL_G * FFT::LDS_ELEMS,
                [&](const CpuCtx &ctx, long b, unsigned char *lds) {
-                   col_body<N, MODE, R>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+                   col_body<N, MODE, R, GC>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
                });
     }
 
@@ -174,14 +177,14 @@ template <int N, typename R> struct Sim {
     }
 
     void row(bool with_cfl) {
-        RowArgs<R> a{B, mix.data(), nl.data(), tw.data(), with_cfl ? cfl.data() : nullptr};
-        const long blocks = ((long)B * (N / 2) + G::ROW_PPW - 1) / G::ROW_PPW;
-        launch(blocks, G::ROW_THREADS, sizeof(cx<R>) * RowShape<N>::template lds_elems<R>(),
+        RowArgs<R> a{B, mix.data(), nl.data(), tw_row.data(), with_cfl ? cfl.data() : nullptr};
+        const long blocks = ((long)B * (N / 2) + GR::ROW_PPW - 1) / GR::ROW_PPW;
+        launch(blocks, GR::ROW_THREADS, sizeof(cx<R>) * RowShape<GR>::template lds_elems<R>(),
                [&](const CpuCtx &ctx, long b, unsigned char *lds) {
                    if (with_cfl)
-                       row_body<N, R, true>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+                       row_body<N, R, true, GR>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
                    else
-                       row_body<N, R, false>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+                       row_body<N, R, false, GR>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
                });
     }
 
@@ -218,11 +221,11 @@ template <int N, typename R> struct Sim {
     }
 };
 
-template <int N, typename R>
+template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>>
 int run(int B, double nu, double alpha, double sigma, double amp, int kf, int kmin, int kmax,
         uint64_t seed, double dt, int steps, int grf, const double *init, const double *forcing,
         int forcing_shared, double *out, double *diag) {
-    Sim<N, R> s(B, nu, alpha, sigma, amp, kf, kmin, kmax, seed);
+    Sim<N, R, GC, GR> s(B, nu, alpha, sigma, amp, kf, kmin, kmax, seed);
     const size_t nn = (size_t)N * N;
     if (grf)
         s.grf();
@@ -426,6 +429,32 @@ int emu_run(int n, int precision, int B, double nu, double alpha, double sigma, 
                               init, forcing, forcing_shared, out, diag);
     return run_any<double>(n, B, nu, alpha, sigma, amp, kf, kmin, kmax, seed, dt, steps, grf,
                            init, forcing, forcing_shared, out, diag);
+}
+
+// FP32 engine with column variant cv and the default row shape (rv == 0), or
+// the default column shape and row variant rv (cv == 0); variants.hpp.
+int emu_run_variant(int n, int cv, int rv, int B, double nu, double alpha, double sigma,
+                    double amp, int kf, int kmin, int kmax, uint64_t seed, double dt, int steps,
+                    int grf, const double *init, double *out, double *diag) {
+    if (rv == 0) {
+#define HZL_X(NN, IDX, E, CT, W)                                                               \
+    if (n == NN && cv == IDX)                                                                  \
+        return run<NN, float, Geometry<NN, E, CT>, Geometry<NN>>(                              \
+            B, nu, alpha, sigma, amp, kf, kmin, kmax, seed, dt, steps, grf, init, nullptr, 1, out, \
+            diag);
+        HZL_COL_VARIANTS(HZL_X)
+#undef HZL_X
+    }
+    if (cv == 0) {
+#define HZL_X(NN, IDX, E, PPW, W)                                                              \
+    if (n == NN && rv == IDX)                                                                  \
+        return run<NN, float, Geometry<NN>, Geometry<NN, E, 64, PPW>>(                         \
+            B, nu, alpha, sigma, amp, kf, kmin, kmax, seed, dt, steps, grf, init, nullptr, 1, out, \
+            diag);
+        HZL_ROW_VARIANTS(HZL_X)
+#undef HZL_X
+    }
+    return -1;
 }
 
 int emu_fft_lines(int n, int precision, int lines, int inverse, double *data) {

@@ -28,6 +28,11 @@ class Emulator:
         L.emu_proj_head.restype = C.c_int
         L.emu_proj_head.argtypes = [C.c_int, C.c_long, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp,
                                     _fp]
+        L.emu_run_variant.restype = C.c_int
+        L.emu_run_variant.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_int, C.c_uint64, C.c_double, C.c_int, C.c_int, _dp,
+                                      _dp, _dp]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
 
@@ -87,4 +92,15 @@ class Emulator:
                               None if fa is None else fa.ctypes.data_as(_dp), int(shared),
                               out.ctypes.data_as(_dp), diag.ctypes.data_as(_dp))
         assert rc == 0
+        return out, diag
+
+    def run_variant(self, n, members, col_variant, row_variant, dt, steps, nu=1e-3, alpha=0.0,
+                    sigma=0.05, amp=0.1, kf=4, kmin=1, kmax=4, seed=0):
+        """FP32 launch-shape variant (one of col / row nonzero) from GRF fields."""
+        out = np.zeros((members, n, n), dtype=np.complex128)
+        diag = np.zeros((members, 6))
+        rc = self.lib.emu_run_variant(n, col_variant, row_variant, members, nu, alpha, sigma,
+                                      amp, kf, kmin, kmax, seed, dt, steps, 1, None,
+                                      out.ctypes.data_as(_dp), diag.ctypes.data_as(_dp))
+        assert rc == 0, "no such variant"
         return out, diag

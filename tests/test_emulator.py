@@ -48,3 +48,14 @@ def test_forced_pipeline_matches_oracle(emu, orc):
     got, _ = emu.run(n, 1, "fp64", dt, 3, nu=nu, alpha=alpha, sigma=0.0, amp=0.0, init=w0,
                      forcing=F)
     assert rel(got[0], orc.rk4(w0, nu, alpha, dt, 3, F)) < 1e-13
+
+
+@pytest.mark.parametrize("n,cv,rv", [(128, 1, 0), (128, 2, 0), (128, 3, 0), (128, 0, 1),
+                                     (128, 0, 2), (128, 0, 3), (256, 2, 0), (256, 0, 3),
+                                     (256, 0, 4)])
+def test_launch_shape_variants_match_oracle(emu, orc, n, cv, rv):
+    """Autotuner variants (other T x E splits, workgroup sizes, row tiles)
+    compute the same RK4 step as the FP64 oracle."""
+    got, _ = emu.run_variant(n, 1, cv, rv, 1e-3, 2, seed=SEED)
+    want = orc.trajectories(n, 1e-3, 0.0, 1e-3, 2, SEED, 0, 1, kmax=4)
+    assert rel(got[0], want[0]) < 2e-7
