@@ -317,6 +317,100 @@ void Dataset::save(const std::filesystem::path &path) const {
         throw std::runtime_error("observation write failed");
 }
 
+namespace {
+// Retained strictly-inside-2/3 mask without the mean (src/solver.cpp:126-135).
+bool retained(std::size_t n, std::size_t i) {
+    const int a = wave_number(i % n, n), b = wave_number(i / n, n);
+    return 3 * std::abs(a) < (int)n && 3 * std::abs(b) < (int)n && i != 0;
+}
+// Indices of the forcing shell pairs, both members (src/solver.cpp:67-83).
+std::vector<bool> shell_indices(std::size_t n, const Parameters &p) {
+    std::vector<bool> forced(n * n, false);
+    bool any = false;
+    for (std::size_t i = 0; i < n * n; ++i) {
+        const int a = wave_number(i % n, n), b = wave_number(i / n, n);
+        const int k2 = a * a + b * b;
+        if (retained(n, i) && k2 >= p.forcing_min * p.forcing_min &&
+            k2 <= p.forcing_max * p.forcing_max) {
+            forced[i] = true;
+            any = true;
+        }
+    }
+    if (!any)
+        throw std::invalid_argument("empty forcing shell");
+    return forced;
+}
+constexpr std::size_t kEngineWords = 312;
+} // namespace
+
+void Checkpoint::save(const std::filesystem::path &path) const {
+    if (vorticity.size() != n * n || forcing.size() != n * n ||
+        random_engine.size() != kEngineWords + 1)
+        throw std::invalid_argument("checkpoint fields do not match the grid");
+    std::ofstream out(path);
+    if (!out)
+        throw std::runtime_error("cannot open checkpoint for writing");
+    out << std::setprecision(17);
+    const auto &p = parameters;
+    out << "HZL_CHECKPOINT_V1\n"
+        << n << ' ' << time << '\n'
+        << p.viscosity << ' ' << p.friction << ' ' << p.forcing_sigma << ' ' << p.forcing_tau
+        << ' ' << p.forcing_min << ' ' << p.forcing_max << '\n';
+    for (std::size_t k = 0; k < kEngineWords; ++k)
+        out << random_engine[k] << ' ';
+    out << random_engine[kEngineWords] << '\n' << (random_has_spare ? 1 : 0) << ' ' << random_spare
+        << '\n';
+    for (std::size_t i = 0; i < n * n; ++i)
+        out << vorticity[i].real() << ' ' << vorticity[i].imag() << ' ' << forcing[i].real() << ' '
+            << forcing[i].imag() << '\n';
+    if (!out)
+        throw std::runtime_error("checkpoint write failed");
+}
+
+Checkpoint Checkpoint::load(const std::filesystem::path &path) {
+    std::ifstream in(path);
+    std::string magic;
+    Checkpoint c;
+    Parameters &p = c.parameters;
+    if (!(in >> magic >> c.n >> c.time >> p.viscosity >> p.friction >> p.forcing_sigma >>
+          p.forcing_tau >> p.forcing_min >> p.forcing_max) ||
+        magic != "HZL_CHECKPOINT_V1" || !std::isfinite(c.time) || c.time < 0)
+        throw std::runtime_error("invalid checkpoint header");
+    Fourier2D grid_check(c.n); // power-of-two grid, as the reference's Solver(n, p, 0)
+    p.validate(c.n);
+    const auto forced = shell_indices(c.n, p);
+    c.random_engine.resize(kEngineWords + 1);
+    for (auto &w : c.random_engine)
+        if (!(in >> w))
+            throw std::runtime_error("invalid checkpoint random stream");
+    if (c.random_engine[kEngineWords] > kEngineWords ||
+        !(in >> c.random_has_spare >> c.random_spare) || !std::isfinite(c.random_spare))
+        throw std::runtime_error("invalid checkpoint random stream");
+    const std::size_t n = c.n;
+    c.vorticity.assign(n * n, Complex{});
+    c.forcing.assign(n * n, Complex{});
+    for (std::size_t i = 0; i < n * n; ++i) {
+        double a = 0, b = 0, f = 0, g = 0;
+        if (!(in >> a >> b >> f >> g) || !std::isfinite(a) || !std::isfinite(b) ||
+            !std::isfinite(f) || !std::isfinite(g))
+            throw std::runtime_error("invalid checkpoint field");
+        c.vorticity[i] = {a, b};
+        c.forcing[i] = {f, g};
+    }
+    for (std::size_t i = 0; i < n * n; ++i) {
+        const std::size_t y = i / n, x = i % n, partner = ((n - y) % n) * n + (n - x) % n;
+        const Complex w = c.vorticity[i], F = c.forcing[i];
+        if ((!retained(n, i) && w != Complex{}) || (!forced[i] && F != Complex{}) ||
+            std::abs(w - std::conj(c.vorticity[partner])) > 1e-10 * (1 + std::abs(w)) ||
+            F != std::conj(c.forcing[partner]))
+            throw std::runtime_error("checkpoint violates spectral constraints");
+    }
+    std::string trailing;
+    if (in >> trailing)
+        throw std::runtime_error("unexpected checkpoint data");
+    return c;
+}
+
 Dataset Dataset::load(const std::filesystem::path &path) {
     std::ifstream in(path);
     std::string line, magic;

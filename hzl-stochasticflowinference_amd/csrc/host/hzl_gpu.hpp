@@ -129,6 +129,27 @@ class Ensemble {
     hzl_ensemble_t handle_ = nullptr;
 };
 
+// HZL_CHECKPOINT_V1 text files, the format of Solver::save / Solver::load
+// (src/solver.cpp:434-491): grid, time, the first six Parameters fields, the
+// solver's NormalStream (mt19937_64 state as std::mt19937_64's text form: 312
+// words and the position, then has_spare and spare; src/solver.cpp:54-60),
+// and per spectral index y * n + x the vorticity and the OU forcing.  load()
+// applies the reference's checks (finite values, dealiasing mask, forcing only
+// on the shell pairs, conjugate symmetry) and errors; save() writes the same
+// bytes the reference would.  The random stream is carried verbatim: the GPU
+// engine's forcing noise is counter-based and does not consume it.
+struct Checkpoint {
+    std::size_t n = 0;
+    double time = 0;
+    Parameters parameters;
+    std::vector<std::uint64_t> random_engine; // 312 state words + position
+    bool random_has_spare = false;
+    double random_spare = 0;
+    Field vorticity, forcing; // spectral, n * n each
+    void save(const std::filesystem::path &path) const;
+    static Checkpoint load(const std::filesystem::path &path);
+};
+
 struct Observation {
     std::size_t step{}, x{}, y{};
     double value{}, standard_deviation{};

@@ -432,6 +432,104 @@ def save_observations(path: str, meta, rows):
         f.write("\n".join(out) + "\n")
 
 
+CHECKPOINT_MAGIC = "HZL_CHECKPOINT_V1"
+_ENGINE_WORDS = 312  # std::mt19937_64 state words (then its position)
+
+
+def _retained_mask(n: int) -> np.ndarray:
+    k = np.fft.fftfreq(n, 1.0 / n).astype(np.int64)
+    kx, ky = np.meshgrid(k, k)  # [y][x]
+    m = (3 * np.abs(kx) < n) & (3 * np.abs(ky) < n)
+    m[0, 0] = False
+    return m
+
+
+def load_checkpoint(path: str) -> dict:
+    """Read an HZL_CHECKPOINT_V1 file of Solver::save (src/solver.cpp:434-491)
+    with Solver::load's checks.  Returns dict(n, time, parameters (the six
+    Parameters fields), random_engine (313 words), random_has_spare,
+    random_spare, vorticity, forcing (n x n spectral, [y][x]))."""
+    with open(path) as f:
+        tok = f.read().split()
+    try:
+        if tok[0] != CHECKPOINT_MAGIC:
+            raise ValueError
+        n, t = int(tok[1]), float(tok[2])
+        par = dict(viscosity=float(tok[3]), friction=float(tok[4]), forcing_sigma=float(tok[5]),
+                   forcing_tau=float(tok[6]), forcing_min=int(tok[7]), forcing_max=int(tok[8]))
+        if not np.isfinite(t) or t < 0:
+            raise ValueError
+    except (ValueError, IndexError):
+        raise HzlError("invalid checkpoint header") from None
+    if n < 8 or n > 4096 or n & (n - 1):
+        raise HzlInvalidArgument("grid must be a power of two between 8 and 4096")
+    if (not all(np.isfinite([par["viscosity"], par["friction"], par["forcing_sigma"],
+                             par["forcing_tau"]]))
+            or par["viscosity"] < 0 or par["friction"] < 0 or par["forcing_sigma"] < 0
+            or par["forcing_tau"] <= 0):
+        raise HzlInvalidArgument("nu, alpha, sigma must be finite and nonnegative; "
+                                 "tau must be positive")
+    if par["forcing_min"] < 1 or par["forcing_max"] < par["forcing_min"] or \
+            3 * par["forcing_max"] >= n:
+        raise HzlInvalidArgument("forcing shell must lie strictly inside the dealiased grid")
+    at = 9
+    try:
+        engine = [int(w) for w in tok[at:at + _ENGINE_WORDS + 1]]
+        spare_flag, spare = int(tok[at + _ENGINE_WORDS + 1]), float(tok[at + _ENGINE_WORDS + 2])
+        if (len(engine) != _ENGINE_WORDS + 1 or engine[-1] > _ENGINE_WORDS
+                or any(w < 0 or w >= 1 << 64 for w in engine) or spare_flag not in (0, 1)
+                or not np.isfinite(spare)):
+            raise ValueError
+    except (ValueError, IndexError):
+        raise HzlError("invalid checkpoint random stream") from None
+    at += _ENGINE_WORDS + 3
+    body = tok[at:]
+    if len(body) < 4 * n * n:
+        raise HzlError("invalid checkpoint field")
+    try:
+        vals = np.array([float(x) for x in body[:4 * n * n]]).reshape(n * n, 4)
+    except ValueError:
+        raise HzlError("invalid checkpoint field") from None
+    if not np.all(np.isfinite(vals)):
+        raise HzlError("invalid checkpoint field")
+    w = (vals[:, 0] + 1j * vals[:, 1]).reshape(n, n)
+    F = (vals[:, 2] + 1j * vals[:, 3]).reshape(n, n)
+    mask = _retained_mask(n)
+    k = np.fft.fftfreq(n, 1.0 / n).astype(np.int64)
+    kx, ky = np.meshgrid(k, k)
+    k2 = kx * kx + ky * ky
+    shell = mask & (k2 >= par["forcing_min"] ** 2) & (k2 <= par["forcing_max"] ** 2)
+    partner = lambda a: np.roll(a[::-1, ::-1], 1, axis=(0, 1))  # a[(-y) % n, (-x) % n]
+    if (np.any(w[~mask] != 0) or np.any(F[~shell] != 0)
+            or np.any(np.abs(w - np.conj(partner(w))) > 1e-10 * (1 + np.abs(w)))
+            or np.any(F != np.conj(partner(F)))):
+        raise HzlError("checkpoint violates spectral constraints")
+    if len(body) > 4 * n * n:
+        raise HzlError("unexpected checkpoint data")
+    return dict(n=n, time=t, parameters=par, random_engine=engine,
+                random_has_spare=bool(spare_flag), random_spare=spare, vorticity=w, forcing=F)
+
+
+def save_checkpoint(path: str, c: dict) -> None:
+    """Write an HZL_CHECKPOINT_V1 file byte for byte as Solver::save does
+    (17 significant digits, src/solver.cpp:434-449)."""
+    g = lambda v: "%.17g" % v
+    n, p = int(c["n"]), c["parameters"]
+    w = np.asarray(c["vorticity"], dtype=np.complex128).reshape(n * n)
+    F = np.asarray(c["forcing"], dtype=np.complex128).reshape(n * n)
+    engine = [int(x) for x in c["random_engine"]]
+    if len(engine) != _ENGINE_WORDS + 1:
+        raise HzlInvalidArgument("checkpoint random stream needs 313 words")
+    out = [CHECKPOINT_MAGIC, f"{n} {g(c['time'])}",
+           f"{g(p['viscosity'])} {g(p['friction'])} {g(p['forcing_sigma'])} "
+           f"{g(p['forcing_tau'])} {int(p['forcing_min'])} {int(p['forcing_max'])}",
+           " ".join(str(x) for x in engine),
+           f"{int(bool(c['random_has_spare']))} {g(c['random_spare'])}"]
+    out += [f"{g(a.real)} {g(a.imag)} {g(b.real)} {g(b.imag)}" for a, b in zip(w, F)]
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
 def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,
                      retain_states: bool = True):
     """Bootstrap particle filter over an initialised ensemble.

@@ -411,6 +411,38 @@ class Reference:
     def dataset_save(self, h, path):
         self._check(self.lib.ref_dataset_save(h, str(path).encode()))
 
+    def checkpoint_write(self, path, n, nu, alpha, sigma, tau, kmin, kmax, seed, initial, dt,
+                         steps):
+        """Solver(n, p, seed).initialize(initial), `steps` steps, Solver::save."""
+        L = self.lib
+        L.ref_checkpoint_write.restype = C.c_int
+        L.ref_checkpoint_write.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_int, C.c_int, C.c_uint64,
+                                           C.c_char_p, C.c_double, C.c_size_t, C.c_char_p]
+        self._check(L.ref_checkpoint_write(n, nu, alpha, sigma, tau, kmin, kmax, seed,
+                                           initial.encode(), dt, steps, str(path).encode()))
+
+    def checkpoint_fields(self, path, n):
+        """Solver::load(path): (coefficients(), forcing()) as n x n arrays."""
+        L = self.lib
+        L.ref_checkpoint_fields.restype = C.c_int
+        L.ref_checkpoint_fields.argtypes = [C.c_char_p, _dp, _dp]
+        w = np.zeros((n, n), dtype=np.complex128)
+        f = np.zeros((n, n), dtype=np.complex128)
+        self._check(L.ref_checkpoint_fields(str(path).encode(), _ptr(w), _ptr(f)))
+        return w, f
+
+    def checkpoint_diagnostics(self, path):
+        """Solver::load(path).diagnostics(): dict(time, energy, enstrophy,
+        dissipation, injection, cfl_rate)."""
+        L = self.lib
+        L.ref_checkpoint_diagnostics.restype = C.c_int
+        L.ref_checkpoint_diagnostics.argtypes = [C.c_char_p, _dp]
+        out = np.zeros(6)
+        self._check(L.ref_checkpoint_diagnostics(str(path).encode(), _ptr(out)))
+        return dict(zip(("time", "energy", "enstrophy", "dissipation", "injection",
+                         "cfl_rate"), map(float, out)))
+
     def enkf(self, h, nu, alpha, sigma, tau, members, seed, inflation=1.0, radius=0.0, n=16,
              max_records=4096):
         records = np.zeros((max_records, 3))

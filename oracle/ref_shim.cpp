@@ -161,6 +161,52 @@ int ref_diagnostics(std::size_t n, double nu, double alpha, int kmin, int kmax,
     });
 }
 
+// A reference Solver with OU shell forcing (seed `seed`) started from a named
+// initial condition, advanced `steps` RK4 steps of dt, written by Solver::save.
+int ref_checkpoint_write(std::size_t n, double nu, double alpha, double sigma, double tau,
+                         int kmin, int kmax, std::uint64_t seed, const char *initial, double dt,
+                         std::size_t steps, const char *path) {
+    return guarded([&] {
+        hzl::Parameters p;
+        p.viscosity = nu;
+        p.friction = alpha;
+        p.forcing_sigma = sigma;
+        p.forcing_tau = tau;
+        p.forcing_min = kmin;
+        p.forcing_max = kmax;
+        hzl::Solver s(n, p, seed);
+        s.initialize(initial);
+        for (std::size_t k = 0; k < steps; ++k)
+            s.step(dt);
+        s.save(path);
+    });
+}
+
+// Solver::load of a checkpoint, then its time and diagnostics:
+// out = time, energy, enstrophy, dissipation, injection, cfl_rate.
+int ref_checkpoint_diagnostics(const char *path, double *out) {
+    return guarded([&] {
+        hzl::Solver s = hzl::Solver::load(path);
+        const auto d = s.diagnostics();
+        out[0] = d.time;
+        out[1] = d.energy;
+        out[2] = d.enstrophy;
+        out[3] = d.dissipation;
+        out[4] = d.injection;
+        out[5] = d.cfl_rate;
+    });
+}
+
+// Solver::load of a checkpoint: its vorticity and forcing coefficients
+// (n * n each, interleaved complex).
+int ref_checkpoint_fields(const char *path, double *vorticity, double *forcing) {
+    return guarded([&] {
+        const hzl::Solver s = hzl::Solver::load(path);
+        from_field(s.coefficients(), vorticity);
+        from_field(s.forcing(), forcing);
+    });
+}
+
 int ref_energy_spectrum(std::size_t n, const double *state, double *bins) {
     return guarded([&] {
         hzl::Solver s(n, frozen(0, 0, 1, 1), 0);

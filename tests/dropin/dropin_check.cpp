@@ -9,6 +9,11 @@
 //   pf <csv> <out>                  bootstrap_filter(data, p, 8, 555)
 //   enkf <csv> <out>                ensemble_kalman_filter(data, p, 8, 555)
 //   errors                          invalid arguments raise std::invalid_argument
+//   checkpoint <in> <out>           Checkpoint::load then Checkpoint::save
+//   resume <in> <steps> <out>       CudaSolver from a checkpoint's vorticity, `steps`
+//                                   steps of 1e-3 holding its forcing
+//                                   (CudaSolver::step(dt, F)); writes the loaded
+//                                   vorticity, forcing and the final state
 #include "hzl_gpu.hpp"
 
 #include <cmath>
@@ -121,6 +126,20 @@ int main(int argc, char **argv) {
                 append(out, r.mean_vorticity);
             }
             write(argv[3], out);
+        } else if (mode == "checkpoint" && argc == 4) {
+            Checkpoint::load(argv[2]).save(argv[3]);
+        } else if (mode == "resume" && argc == 5) {
+            const Checkpoint c = Checkpoint::load(argv[2]);
+            const std::size_t steps = std::stoul(argv[3]);
+            CudaSolver s(c.n, c.parameters, c.vorticity);
+            for (std::size_t k = 0; k < steps; ++k)
+                s.step(1e-3, c.forcing);
+            s.synchronize();
+            std::vector<double> out;
+            append(out, c.vorticity);
+            append(out, c.forcing);
+            append(out, s.coefficients());
+            write(argv[4], out);
         } else if (mode == "errors") {
             Parameters p;
             const bool ok =

@@ -19,6 +19,12 @@ Fixtures
   pf_deterministic_16.npz reference bootstrap_filter / negative_log_likelihood /
                    ensemble_kalman_filter on the deterministic dataset of
                    tests/verification.cpp:184-210 (16x16).
+  checkpoint_v1_16.txt  a reference Solver::save file (16x16, OU forcing on,
+                   7 steps from multimode); checkpoint_v1_16.npz holds the
+                   reference's Solver::load fields of it and the JSON sidecar its
+                   diagnostics.
+
+Usage: python tests/golden/make_golden.py [config0 drift32 pf16 checkpoint16]
 """
 from __future__ import annotations
 
@@ -102,11 +108,37 @@ def pf16(ref: Reference) -> None:
          enkf_records=enkf["records"], enkf_means=enkf["means"])
 
 
+def checkpoint16(ref: Reference) -> None:
+    """A reference Solver::save file (OU forcing active, random stream
+    advanced) and the reference's own Solver::load diagnostics of it."""
+    path = os.path.join(HERE, "checkpoint_v1_16.txt")
+    recipe = dict(n=16, nu=0.02, alpha=0.1, sigma=0.3, tau=0.5, kmin=2, kmax=3, seed=4242,
+                  initial="multimode", dt=0.01, steps=7)
+    ref.checkpoint_write(path, recipe["n"], recipe["nu"], recipe["alpha"], recipe["sigma"],
+                         recipe["tau"], recipe["kmin"], recipe["kmax"], recipe["seed"],
+                         recipe["initial"], recipe["dt"], recipe["steps"])
+    digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
+    w, F = ref.checkpoint_fields(path, recipe["n"])
+    save("checkpoint_v1_16",
+         dict(recipe, recipe_text="hzl::Solver(n, p, seed); initialize(initial); steps x "
+                                  "step(dt); save(path); arrays: Solver::load(path)."
+                                  "coefficients() / forcing()",
+              checkpoint_file="checkpoint_v1_16.txt", checkpoint_sha256=digest,
+              reference_load_diagnostics=ref.checkpoint_diagnostics(path)),
+         vorticity=w, forcing=F)
+
+
 def main() -> None:
     orc, ref = Oracle(), Reference()
-    config0(orc, ref)
-    drift32(ref)
-    pf16(ref)
+    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16"]
+    if "config0" in which:
+        config0(orc, ref)
+    if "drift32" in which:
+        drift32(ref)
+    if "pf16" in which:
+        pf16(ref)
+    if "checkpoint16" in which:
+        checkpoint16(ref)
 
 
 if __name__ == "__main__":

@@ -142,3 +142,70 @@ def test_reference_signature_filters_match_golden(tmp_path):
 @pytest.mark.gpu
 def test_reference_error_contract():
     assert run("errors").strip() == "ok"
+
+
+CKPT = os.path.join(GOLDEN, "checkpoint_v1_16.txt")
+
+
+def test_checkpoint_round_trip_is_byte_identical(tmp_path, pkg):
+    """A reference-written HZL_CHECKPOINT_V1 file (Solver::save) loads into
+    exactly the fields the reference's own Solver::load produced, and writes
+    back byte for byte, from C++ and from Python."""
+    g = np.load(os.path.join(GOLDEN, "checkpoint_v1_16.npz"))
+    meta = json.load(open(os.path.join(GOLDEN, "checkpoint_v1_16.json")))
+    out = tmp_path / "c.txt"
+    run("checkpoint", CKPT, out)
+    assert out.read_bytes() == open(CKPT, "rb").read()
+    c = pkg.load_checkpoint(CKPT)
+    assert c["n"] == 16 and c["time"] == meta["reference_load_diagnostics"]["time"]
+    assert c["parameters"] == dict(viscosity=0.02, friction=0.1, forcing_sigma=0.3,
+                                   forcing_tau=0.5, forcing_min=2, forcing_max=3)
+    assert len(c["random_engine"]) == 313 and c["random_engine"][-1] <= 312
+    np.testing.assert_array_equal(c["vorticity"], g["vorticity"])
+    np.testing.assert_array_equal(c["forcing"], g["forcing"])
+    assert np.any(c["forcing"] != 0)  # the OU forcing of the reference run is carried
+    py = tmp_path / "p.txt"
+    pkg.save_checkpoint(py, c)
+    assert py.read_bytes() == open(CKPT, "rb").read()
+
+
+def test_checkpoint_diagnostics_match_reference_load(orc):
+    """The loaded fields give the reference's own diagnostics of the loaded
+    solver (energy, enstrophy, dissipation, injection)."""
+    meta = json.load(open(os.path.join(GOLDEN, "checkpoint_v1_16.json")))
+    import __graft_entry__
+    c = __graft_entry__.load_package().load_checkpoint(CKPT)
+    want = meta["reference_load_diagnostics"]
+    d = orc.diagnostics(c["vorticity"], meta["nu"], forcing=c["forcing"])
+    for k in ("energy", "enstrophy", "dissipation", "injection"):
+        assert abs(d[k] - want[k]) < 1e-12 * max(1.0, abs(want[k])), k
+
+
+@pytest.mark.parametrize("edit,message", [
+    (lambda t: t.replace("HZL_CHECKPOINT_V1", "HZL_CHECKPOINT_V2", 1), "invalid checkpoint header"),
+    (lambda t: t + "1\n", "unexpected checkpoint data"),
+    (lambda t: "\n".join(t.split("\n")[:-40]) + "\n", "invalid checkpoint field"),
+    # a vorticity value on the highest (masked) wave number, y = 0, x = n/2
+    (lambda t: "\n".join(line if i != 5 + 8 else "1 0 0 0"
+                         for i, line in enumerate(t.split("\n"))), "spectral constraints"),
+])
+def test_checkpoint_errors_match_reference(tmp_path, pkg, edit, message):
+    bad = tmp_path / "bad.txt"
+    bad.write_text(edit(open(CKPT).read()))
+    with pytest.raises(pkg.HzlError, match=message):
+        pkg.load_checkpoint(bad)
+    p = subprocess.run([CHECK, "checkpoint", bad, tmp_path / "o.txt"], capture_output=True,
+                       text=True)
+    assert p.returncode == 1 and message in p.stderr
+
+
+@pytest.mark.gpu
+def test_cudasolver_resumes_from_checkpoint(tmp_path, orc):
+    """CudaSolver built from a reference checkpoint, advanced holding its
+    forcing, matches the FP64 oracle's held-forcing RK4 from the loaded state."""
+    run("resume", CKPT, 50, tmp_path / "r.bin")
+    raw = np.fromfile(tmp_path / "r.bin")
+    n = 16
+    w0, F, got = (field(raw[k * 2 * n * n:(k + 1) * 2 * n * n], n) for k in range(3))
+    want = orc.rk4(w0, 0.02, 0.1, 1e-3, 50, forcing=F)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
