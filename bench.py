@@ -409,11 +409,17 @@ def main() -> int:
     ap.add_argument("--uneven-only", action="store_true", help="only the two-grid filter")
     ap.add_argument("--pmcmc-only", action="store_true", help="only the pMCMC window")
     ap.add_argument("--enkf-only", action="store_true", help="only the EnKF cycle")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch shapes only (no autotuning; for profiling)")
+    ap.add_argument("--tuning", default=None,
+                    help="force launch-shape variants 'col,row' (no timing; for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.no_tune:
+        os.environ["HZL_AUTOTUNE"] = "0"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: members shard across GPUs with no "
               "communication during forecasts, so multi-GPU runs are not part of this build",
@@ -440,7 +446,13 @@ def main() -> int:
                        kf=4, seed=SEED)
     # launch shapes: time every variant on this workload (the engine already
     # applied a remembered or freshly timed choice at creation)
-    tune_col, tune_row = ens.autotune()
+    if args.tuning:
+        ens.tuning = tuple(int(x) for x in args.tuning.split(","))
+        tune_col, tune_row = [], []
+    elif args.no_tune:
+        tune_col, tune_row = [], []
+    else:
+        tune_col, tune_row = ens.autotune()
     cv, rv = ens.tuning
     ens.init_grf()
     ens.step(dt, args.warmup)

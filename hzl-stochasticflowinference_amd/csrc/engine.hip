@@ -35,7 +35,7 @@ namespace hzl_amd {
 #define HZL_COL_WPE 1
 #endif
 #ifndef HZL_ROW_WPE
-#define HZL_ROW_WPE 3
+#define HZL_ROW_WPE 2
 #endif
 
 template <int N, int MODE, typename R>

@@ -173,6 +173,51 @@ template <int N, int E, typename R> struct GroupFFT {
         return lo > K && lo + T - 1 < N - K;
     }
 
+    // Twiddles of lane t held in registers: w[j] = W_N^{t j}, j < E.  With
+    // E == T (one column of the T x E split per lane) the same E values serve
+    // the forward and the inverse transform, since W^{t j} is symmetric in
+    // (t, j); a group that transforms several lines loads them once.
+    struct Twiddles {
+        cx<R> w[E];
+    };
+    HZL_HD static Twiddles load_twiddles(const cx<R> *tw, int t) {
+        Twiddles r;
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            r.w[j] = tw[t * E + j];
+        return r;
+    }
+
+    template <class Ctx>
+    HZL_HD static void forward(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const Twiddles &tr) {
+        dft<E, false>(v);
+#pragma unroll
+        for (int k2 = 1; k2 < E; ++k2)
+            v[k2] = cmul(v[k2], tr.w[k2]);
+        exchange_forward(ctx, v, lds, t);
+    }
+
+    template <class Ctx>
+    HZL_HD static void inverse(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const Twiddles &tr) {
+        static_assert(Q == 1, "register twiddles of the inverse need E == T");
+        cx<R> u[T];
+#pragma unroll
+        for (int k1 = 0; k1 < T; ++k1)
+            u[k1] = v[k1];
+        dft<T, true>(u);
+#pragma unroll
+        for (int n1 = 0; n1 < T; ++n1)
+            lds[t * LD + n1] = n1 == 0 ? u[n1] : cmulc(u[n1], tr.w[n1]);
+        ctx.wsync();
+#pragma unroll
+        for (int k2 = 0; k2 < E; ++k2)
+            v[k2] = lds[k2 * LD + t];
+        dft<E, true>(v);
+        ctx.wsync();
+    }
+
     template <class Ctx>
     HZL_HD static void forward(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
                                const cx<R> *tw) {
@@ -180,6 +225,12 @@ template <int N, int E, typename R> struct GroupFFT {
 #pragma unroll
         for (int k2 = 1; k2 < E; ++k2)
             v[k2] = cmul(v[k2], tw[t * E + k2]);
+        exchange_forward(ctx, v, lds, t);
+    }
+
+    // LDS exchange and second pass of the forward transform.
+    template <class Ctx>
+    HZL_HD static void exchange_forward(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t) {
 #pragma unroll
         for (int k2 = 0; k2 < E; ++k2)
             lds[k2 * LD + t] = v[k2];

@@ -28,6 +28,7 @@
 #pragma once
 #include "fft_lds.hpp"
 #include <cstdint>
+#include <type_traits>
 
 namespace hzl_amd {
 
@@ -85,6 +86,19 @@ HZL_HD float recip(float x) {
 #else
     return 1.0f / x;
 #endif
+}
+
+// Twiddle argument of a group's transforms: the lane's twiddles in registers
+// when the split is square (E == T: one set serves forward and inverse), else
+// the table itself.
+template <class FFT, typename R>
+using TwArg = std::conditional_t<FFT::Q == 1, typename FFT::Twiddles, const cx<R> *>;
+
+template <class FFT, typename R> HZL_HD TwArg<FFT, R> lane_twiddles(const cx<R> *tw, int t) {
+    if constexpr (FFT::Q == 1)
+        return FFT::load_twiddles(tw, t);
+    else
+        return tw;
 }
 
 HZL_HD long tiled(int KP, int c, int y) {
@@ -223,6 +237,7 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
     const int m = live ? (int)(gc / KP) : 0, c = live ? (int)(gc % KP) : 0;
     cx<R> *L = lds + g * FFT::LDS_ELEMS;
     const long soff = (long)m * G::SPEC + (long)c * J;
+    const TwArg<FFT, R> twa = lane_twiddles<FFT>(a.tw, t);
     cx<R> v[E];
 
     if constexpr (MODE == 0) {
@@ -263,7 +278,7 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
 #ifdef HZL_COL_PREFETCH
         load_state();
 #endif
-        FFT::forward(ctx, v, L, t, a.tw);
+        FFT::forward(ctx, v, L, t, twa);
 #ifndef HZL_COL_PREFETCH
         load_state();
 #endif
@@ -339,7 +354,7 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
                 s = (R)ky;
             u[idx] = cscale(v[idx], s);
         }
-        FFT::inverse(ctx, u, L, t, a.tw);
+        FFT::inverse(ctx, u, L, t, twa);
         if (live) {
 #pragma unroll
             for (int n2 = 0; n2 < E; ++n2)
@@ -426,6 +441,23 @@ template <class G> HZL_HD int staged(int kx, int ly) {
     return (ly >> 4) * (G::KP * RS::PITCH) + kx * RS::PITCH + (ly & 15);
 }
 
+// Entry k of the packed row spectrum z1 = u + i wx (FIRST: p = (ky psi)~,
+// q = w~) or z2 = v + i wy (p = psi~, q = (ky w)~) at x wave number kx = k
+// (pos) or kx = N - k (the conjugate half).
+template <typename R, bool FIRST>
+HZL_HD cx<R> z_entry(cx<R> p, cx<R> q, int k, int kx, bool pos) {
+    const R fx = (R)kx;
+    // FIRST: a = u = i P,        b = wx = i kx Q
+    // else : a = v = -i kx P,    b = wy = i Q
+    const cx<R> A = FIRST ? mul_i(p) : mul_mi(cscale(p, fx));
+    const cx<R> Bv = FIRST ? mul_i(cscale(q, fx)) : mul_i(q);
+    if (k == 0) // DC terms of real signals: real parts only
+        return {A.x, Bv.x};
+    if (pos)
+        return {A.x - Bv.y, A.y + Bv.x};
+    return {A.x + Bv.y, Bv.x - A.y};
+}
+
 // One row-pair group: build the packed spectra z1 = u + i wx (fields B, C) or
 // z2 = v + i wy (fields A, D) of local row ly from the staged fields P, Q.
 template <class G, typename R, bool FIRST>
@@ -442,120 +474,53 @@ HZL_HD void row_spectrum(cx<R> (&z)[G::E], const cx<R> *P, const cx<R> *Q, int t
         if (!FFT::slot_outside(idx, K) && (pos || neg)) {
             const int kx = pos ? k : N - k;
             const int o = staged<G>(kx, ly);
-            const cx<R> p = P[o], q = Q[o];
-            const R fx = (R)kx;
-            // FIRST: P = (ky psi)~, Q = w~:   a = u = i P,        b = wx = i kx Q
-            // else : P = psi~,      Q = (ky w)~: a = v = -i kx P, b = wy = i Q
-            const cx<R> A = FIRST ? mul_i(p) : mul_mi(cscale(p, fx));
-            const cx<R> Bv = FIRST ? mul_i(cscale(q, fx)) : mul_i(q);
-            if (k == 0) // DC terms of real signals: real parts only
-                z[idx] = {A.x, Bv.x};
-            else if (pos)
-                z[idx] = {A.x - Bv.y, A.y + Bv.x};
-            else
-                z[idx] = {A.x + Bv.y, Bv.x - A.y};
+            z[idx] = z_entry<R, FIRST>(P[o], Q[o], k, kx, pos);
         }
     }
 }
 
-template <int N, typename R, bool CFL, class G = Geometry<N>, class Ctx>
-HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block) {
-    static_assert(G::NN == N, "launch shape of another grid");
+// The same spectra for both rows y0, y0 + 1 of a pair straight from the tiled
+// mixed fields in HBM: (kx, y0) and (kx, y0 + 1) are adjacent, so one 16-byte
+// load per field and slot serves both rows.
+template <class G, typename R, bool FIRST>
+HZL_HD void direct_spectra(cx<R> (&za)[G::E], cx<R> (&zb)[G::E], const cx<R> *P,
+                           const cx<R> *Q, int t, int y0) {
+    constexpr int N = G::NN;
     using FFT = GroupFFT<N, G::E, R>;
-    using RS = RowShape<G>;
-    constexpr int E = G::E, T = G::T, K = G::K, KP = G::KP, PPW = G::ROW_PPW;
-    const int tid = ctx.tid();
-    const int g = tid / T, t = tid % T; // group-contiguous lanes: conflict-free exchanges
-    const long row0 = block * RS::ROWS;
-    const long gp = block * PPW + g;
-    const bool live = gp < (long)a.B * (N / 2);
-    const int m = live ? (int)((2 * gp) / N) : 0, y0 = live ? (int)((2 * gp) % N) : 0;
-    const int ly = 2 * g; // local row of y0
-    cx<R> *S0 = lds, *S1 = lds + RS::CHUNK;
-    cx<R> *L = lds + g * FFT::LDS_ELEMS;
+    constexpr int K = G::K;
+    struct alignas(4 * sizeof(R)) pair2 {
+        cx<R> u, v;
+    };
+#pragma unroll
+    for (int idx = 0; idx < G::E; ++idx) {
+        const int k = FFT::index(t, idx);
+        const bool pos = k <= K, neg = k >= N - K;
+        za[idx] = zb[idx] = {0, 0};
+        if (!FFT::slot_outside(idx, K) && (pos || neg)) {
+            const int kx = pos ? k : N - k;
+            const long o = tiled(G::KP, kx, y0);
+            const pair2 p = *reinterpret_cast<const pair2 *>(P + o);
+            const pair2 q = *reinterpret_cast<const pair2 *>(Q + o);
+            za[idx] = z_entry<R, FIRST>(p.u, q.u, k, kx, pos);
+            zb[idx] = z_entry<R, FIRST>(p.v, q.v, k, kx, pos);
+        }
+    }
+}
 
-    R pa[E], pb[E], ua[CFL ? E : 1], ub[CFL ? E : 1];
-    R vmax = R(0);
-
-    // u wx from (ky psi)~ and w~; the loads of psi~ and (ky w)~ are issued
-    // right after, so they arrive while the first pair is transformed.
-    {
-        RowTile<G, R> tb, tc;
-        tb.load(ctx, a, row0, 1);
-        tc.load(ctx, a, row0, 2);
-        tb.store(ctx, a, row0, S0);
-        tc.store(ctx, a, row0, S1);
-    }
-    RowTile<G, R> ta, td;
-#ifdef HZL_ROW_PREFETCH
-    ta.load(ctx, a, row0, 0);
-    td.load(ctx, a, row0, 3);
-#endif
-    ctx.sync();
-    {
-        cx<R> za[E], zb[E];
-        row_spectrum<G, R, true>(za, S0, S1, t, ly);
-        row_spectrum<G, R, true>(zb, S0, S1, t, ly + 1);
-        ctx.sync();
-        FFT::inverse(ctx, za, L, t, a.tw);
-#pragma unroll
-        for (int j = 0; j < E; ++j) {
-            pa[j] = za[j].x * za[j].y;
-            if constexpr (CFL)
-                ua[j] = za[j].x < 0 ? -za[j].x : za[j].x;
-        }
-        FFT::inverse(ctx, zb, L, t, a.tw);
-#pragma unroll
-        for (int j = 0; j < E; ++j) {
-            pb[j] = zb[j].x * zb[j].y;
-            if constexpr (CFL)
-                ub[j] = zb[j].x < 0 ? -zb[j].x : zb[j].x;
-        }
-    }
-    // v wy from psi~ and (ky w)~ (the staging area doubles as the groups'
-    // exchange areas: every wavefront must be done with them first)
-    ctx.sync();
-#ifndef HZL_ROW_PREFETCH
-    ta.load(ctx, a, row0, 0);
-    td.load(ctx, a, row0, 3);
-#endif
-    ta.store(ctx, a, row0, S0);
-    td.store(ctx, a, row0, S1);
-    ctx.sync();
-    {
-        cx<R> za[E], zb[E];
-        row_spectrum<G, R, false>(za, S0, S1, t, ly);
-        row_spectrum<G, R, false>(zb, S0, S1, t, ly + 1);
-        ctx.sync();
-        FFT::inverse(ctx, za, L, t, a.tw);
-#pragma unroll
-        for (int j = 0; j < E; ++j) {
-            pa[j] += za[j].x * za[j].y;
-            if constexpr (CFL) {
-                R s = ua[j] + (za[j].x < 0 ? -za[j].x : za[j].x);
-                if (!(s <= R(1e30)))
-                    s = (R)__builtin_huge_valf(); // non-finite velocity poisons the maximum
-                vmax = s > vmax ? s : vmax;
-            }
-        }
-        FFT::inverse(ctx, zb, L, t, a.tw);
-#pragma unroll
-        for (int j = 0; j < E; ++j) {
-            pb[j] += zb[j].x * zb[j].y;
-            if constexpr (CFL) {
-                R s = ub[j] + (zb[j].x < 0 ? -zb[j].x : zb[j].x);
-                if (!(s <= R(1e30)))
-                    s = (R)__builtin_huge_valf();
-                vmax = s > vmax ? s : vmax;
-            }
-        }
-    }
-    // packed forward transform of the two rows' products, keep kx in [0, K]
+// Packed forward transform of a row pair's products pa, pb, unpacked to
+// kx in [0, K] and stored as the nonlinear mixed field; with CFL, the group's
+// maximum of |u| + |v| joins the pair's running maximum.
+template <int N, typename R, bool CFL, class G, class Ctx, class TW>
+HZL_HD void row_finish(const Ctx &ctx, const RowArgs<R> &a, cx<R> *L, int t, const TW &twa,
+                       const R (&pa)[G::E], const R (&pb)[G::E], R vmax, bool live, int m,
+                       int y0, long gp) {
+    using FFT = GroupFFT<N, G::E, R>;
+    constexpr int E = G::E, T = G::T, K = G::K, KP = G::KP;
     cx<R> q[E];
 #pragma unroll
     for (int j = 0; j < E; ++j)
         q[j] = {pa[j], pb[j]};
-    FFT::forward(ctx, q, L, t, a.tw);
+    FFT::forward(ctx, q, L, t, twa);
 #pragma unroll
     for (int idx = 0; idx < E; ++idx)
         if (!FFT::slot_outside(idx, K)) // only |kx| <= K is unpacked
@@ -590,6 +555,167 @@ HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block
             a.cfl[gp] = (cur > prev || cur != cur) ? cur : prev;
         }
     }
+}
+
+template <int N, typename R, bool CFL, class G = Geometry<N>, class Ctx>
+HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block) {
+    static_assert(G::NN == N, "launch shape of another grid");
+    using FFT = GroupFFT<N, G::E, R>;
+    using RS = RowShape<G>;
+    constexpr int E = G::E, T = G::T, K = G::K, KP = G::KP, PPW = G::ROW_PPW;
+    const int tid = ctx.tid();
+    const int g = tid / T, t = tid % T; // group-contiguous lanes: conflict-free exchanges
+    const long row0 = block * RS::ROWS;
+    const long gp = block * PPW + g;
+    const bool live = gp < (long)a.B * (N / 2);
+    const int m = live ? (int)((2 * gp) / N) : 0, y0 = live ? (int)((2 * gp) % N) : 0;
+    const int ly = 2 * g; // local row of y0
+    cx<R> *S0 = lds, *S1 = lds + RS::CHUNK;
+    cx<R> *L = lds + g * FFT::LDS_ELEMS;
+    const TwArg<FFT, R> twa = lane_twiddles<FFT>(a.tw, t);
+
+    R pa[E], pb[E], ua[CFL ? E : 1], ub[CFL ? E : 1];
+    R vmax = R(0);
+
+    // u wx from (ky psi)~ and w~; the loads of psi~ and (ky w)~ are issued
+    // right after, so they arrive while the first pair is transformed.
+    {
+        RowTile<G, R> tb, tc;
+        tb.load(ctx, a, row0, 1);
+        tc.load(ctx, a, row0, 2);
+        tb.store(ctx, a, row0, S0);
+        tc.store(ctx, a, row0, S1);
+    }
+    RowTile<G, R> ta, td;
+#ifdef HZL_ROW_PREFETCH
+    ta.load(ctx, a, row0, 0);
+    td.load(ctx, a, row0, 3);
+#endif
+    ctx.sync();
+    {
+        cx<R> za[E], zb[E];
+        row_spectrum<G, R, true>(za, S0, S1, t, ly);
+        row_spectrum<G, R, true>(zb, S0, S1, t, ly + 1);
+        ctx.sync();
+        FFT::inverse(ctx, za, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pa[j] = za[j].x * za[j].y;
+            if constexpr (CFL)
+                ua[j] = za[j].x < 0 ? -za[j].x : za[j].x;
+        }
+        FFT::inverse(ctx, zb, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pb[j] = zb[j].x * zb[j].y;
+            if constexpr (CFL)
+                ub[j] = zb[j].x < 0 ? -zb[j].x : zb[j].x;
+        }
+    }
+    // v wy from psi~ and (ky w)~ (the staging area doubles as the groups'
+    // exchange areas: every wavefront must be done with them first)
+    ctx.sync();
+#ifndef HZL_ROW_PREFETCH
+    ta.load(ctx, a, row0, 0);
+    td.load(ctx, a, row0, 3);
+#endif
+    ta.store(ctx, a, row0, S0);
+    td.store(ctx, a, row0, S1);
+    ctx.sync();
+    {
+        cx<R> za[E], zb[E];
+        row_spectrum<G, R, false>(za, S0, S1, t, ly);
+        row_spectrum<G, R, false>(zb, S0, S1, t, ly + 1);
+        ctx.sync();
+        FFT::inverse(ctx, za, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pa[j] += za[j].x * za[j].y;
+            if constexpr (CFL) {
+                R s = ua[j] + (za[j].x < 0 ? -za[j].x : za[j].x);
+                if (!(s <= R(1e30)))
+                    s = (R)__builtin_huge_valf(); // non-finite velocity poisons the maximum
+                vmax = s > vmax ? s : vmax;
+            }
+        }
+        FFT::inverse(ctx, zb, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pb[j] += zb[j].x * zb[j].y;
+            if constexpr (CFL) {
+                R s = ub[j] + (zb[j].x < 0 ? -zb[j].x : zb[j].x);
+                if (!(s <= R(1e30)))
+                    s = (R)__builtin_huge_valf();
+                vmax = s > vmax ? s : vmax;
+            }
+        }
+    }
+    row_finish<N, R, CFL, G>(ctx, a, L, t, twa, pa, pb, vmax, live, m, y0, gp);
+}
+
+// Row pass without staging: each row-pair group loads its spectra straight
+// from HBM (direct_spectra), so the workgroup needs no barrier and LDS only
+// for the groups' transform exchanges.  Same arithmetic as row_body.
+template <int N, typename R, bool CFL, class G, class Ctx>
+HZL_HD void row_body_direct(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block) {
+    static_assert(G::NN == N, "launch shape of another grid");
+    using FFT = GroupFFT<N, G::E, R>;
+    constexpr int E = G::E, T = G::T, PPW = G::ROW_PPW;
+    const int tid = ctx.tid();
+    const int g = tid / T, t = tid % T;
+    const long gp = block * PPW + g;
+    const bool live = gp < (long)a.B * (N / 2);
+    const int m = live ? (int)((2 * gp) / N) : 0, y0 = live ? (int)((2 * gp) % N) : 0;
+    cx<R> *L = lds + g * FFT::LDS_ELEMS;
+    const TwArg<FFT, R> twa = lane_twiddles<FFT>(a.tw, t);
+    const cx<R> *mix = a.mix + (long)m * 4 * G::MIXF;
+    R pa[E], pb[E], ua[CFL ? E : 1], ub[CFL ? E : 1];
+    R vmax = R(0);
+    {
+        cx<R> za[E], zb[E];
+        direct_spectra<G, R, true>(za, zb, mix + G::MIXF, mix + 2 * G::MIXF, t, y0);
+        FFT::inverse(ctx, za, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pa[j] = za[j].x * za[j].y;
+            if constexpr (CFL)
+                ua[j] = za[j].x < 0 ? -za[j].x : za[j].x;
+        }
+        FFT::inverse(ctx, zb, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pb[j] = zb[j].x * zb[j].y;
+            if constexpr (CFL)
+                ub[j] = zb[j].x < 0 ? -zb[j].x : zb[j].x;
+        }
+    }
+    {
+        cx<R> za[E], zb[E];
+        direct_spectra<G, R, false>(za, zb, mix, mix + 3 * G::MIXF, t, y0);
+        FFT::inverse(ctx, za, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pa[j] += za[j].x * za[j].y;
+            if constexpr (CFL) {
+                R s = ua[j] + (za[j].x < 0 ? -za[j].x : za[j].x);
+                if (!(s <= R(1e30)))
+                    s = (R)__builtin_huge_valf();
+                vmax = s > vmax ? s : vmax;
+            }
+        }
+        FFT::inverse(ctx, zb, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            pb[j] += zb[j].x * zb[j].y;
+            if constexpr (CFL) {
+                R s = ub[j] + (zb[j].x < 0 ? -zb[j].x : zb[j].x);
+                if (!(s <= R(1e30)))
+                    s = (R)__builtin_huge_valf();
+                vmax = s > vmax ? s : vmax;
+            }
+        }
+    }
+    row_finish<N, R, CFL, G>(ctx, a, L, t, twa, pa, pb, vmax, live, m, y0, gp);
 }
 
 // ---------------------------------------------------------------- GRF ----

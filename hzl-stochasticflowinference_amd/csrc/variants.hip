@@ -19,6 +19,13 @@ __global__ void __launch_bounds__(G::ROW_THREADS)
     row_body<N, float, CFL, G>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 
+template <int N, bool CFL, class G, int WPE>
+__global__ void __launch_bounds__(G::ROW_THREADS)
+    __attribute__((amdgpu_waves_per_eu(WPE))) k_row_direct_v(RowArgs<float> a) {
+    __shared__ cx<float> lds[G::ROW_PPW * GroupFFT<N, G::E, float>::LDS_ELEMS];
+    row_body_direct<N, float, CFL, G>(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+
 namespace {
 
 template <int N, class G, int WPE>
@@ -45,13 +52,21 @@ void col_v(int mode, const StageArgs<float> &a, hipStream_t s) {
     throw std::invalid_argument("bad stage");
 }
 
-template <int N, class G, int WPE> void row_v(const RowArgs<float> &a, hipStream_t s) {
+template <int N, class G, int WPE, int DIRECT>
+void row_v(const RowArgs<float> &a, hipStream_t s) {
     const long pairs = (long)a.B * (N / 2);
     const dim3 grid((unsigned)((pairs + G::ROW_PPW - 1) / G::ROW_PPW)), block(G::ROW_THREADS);
-    if (a.cfl)
-        hipLaunchKernelGGL((k_row_v<N, true, G, WPE>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((k_row_v<N, false, G, WPE>), grid, block, 0, s, a);
+    if constexpr (DIRECT) {
+        if (a.cfl)
+            hipLaunchKernelGGL((k_row_direct_v<N, true, G, WPE>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_row_direct_v<N, false, G, WPE>), grid, block, 0, s, a);
+    } else {
+        if (a.cfl)
+            hipLaunchKernelGGL((k_row_v<N, true, G, WPE>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_row_v<N, false, G, WPE>), grid, block, 0, s, a);
+    }
 }
 
 int default_elems(int n) {
@@ -70,8 +85,10 @@ int default_elems(int n) {
 
 uint64_t variant_table_id() {
 #define HZL_X(NN, IDX, E, A, W) #NN "," #IDX "," #E "," #A "," #W ";"
-    static const char table[] = "col:" HZL_COL_VARIANTS(HZL_X) "row:" HZL_ROW_VARIANTS(HZL_X);
+#define HZL_Y(NN, IDX, E, A, W, D) #NN "," #IDX "," #E "," #A "," #W "," #D ";"
+    static const char table[] = "col:" HZL_COL_VARIANTS(HZL_X) "row:" HZL_ROW_VARIANTS(HZL_Y);
 #undef HZL_X
+#undef HZL_Y
     uint64_t h = 0xcbf29ce484222325ULL;
     for (const char *c = table; *c; ++c)
         h = (h ^ (uint64_t)(unsigned char)*c) * 0x100000001b3ULL;
@@ -88,7 +105,7 @@ int col_variant_count(int n) {
 
 int row_variant_count(int n) {
     int c = 1;
-#define HZL_X(NN, IDX, E, PPW, W) c += n == NN;
+#define HZL_X(NN, IDX, E, PPW, W, D) c += n == NN;
     HZL_ROW_VARIANTS(HZL_X)
 #undef HZL_X
     return c;
@@ -108,7 +125,7 @@ int col_variant_elems(int n, int v) {
 int row_variant_elems(int n, int v) {
     if (v == 0)
         return default_elems(n);
-#define HZL_X(NN, IDX, E, PPW, W)                                                              \
+#define HZL_X(NN, IDX, E, PPW, W, D)                                                           \
     if (n == NN && v == IDX)                                                                   \
         return E;
     HZL_ROW_VARIANTS(HZL_X)
@@ -128,9 +145,9 @@ bool launch_col_variant(int n, int v, int mode, const StageArgs<float> &a, hipSt
 }
 
 bool launch_row_variant(int n, int v, const RowArgs<float> &a, hipStream_t s) {
-#define HZL_X(NN, IDX, E, PPW, W)                                                              \
+#define HZL_X(NN, IDX, E, PPW, W, D)                                                           \
     if (n == NN && v == IDX) {                                                                 \
-        row_v<NN, Geometry<NN, E, 64, PPW>, W>(a, s);                                          \
+        row_v<NN, Geometry<NN, E, 64, PPW>, W, D>(a, s);                                       \
         return true;                                                                           \
     }
     HZL_ROW_VARIANTS(HZL_X)

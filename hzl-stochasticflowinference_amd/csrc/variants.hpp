@@ -4,7 +4,8 @@
 //
 // A column variant is (values per lane E, i.e. the T x E split of the y
 // transform; lanes per workgroup; waves-per-EU hint).  A row variant is (E of
-// the x transforms; row pairs per workgroup; waves-per-EU hint).  Every
+// the x transforms; row pairs per workgroup; waves-per-EU hint; staged or
+// direct spectra).  Every
 // variant computes the same stage; only the FFT factorisation (and so the
 // FP32 rounding) and the scheduling differ.
 #pragma once
@@ -23,16 +24,22 @@
     X(512, 1, 32, 128, 1)                                                                       \
     X(512, 2, 32, 64, 2)
 
-// X(N, index, E, row pairs per workgroup, waves per EU)
+// X(N, index, E, row pairs per workgroup, waves per EU, direct): direct = 1
+// builds the row spectra straight from HBM (row_body_direct) instead of
+// staging the tiles through LDS.
 #define HZL_ROW_VARIANTS(X)                                                                    \
-    X(128, 1, 16, 16, 2)                                                                        \
-    X(128, 2, 32, 16, 2)                                                                        \
-    X(128, 3, 16, 8, 2)                                                                         \
-    X(256, 1, 16, 8, 2)                                                                         \
-    X(256, 2, 16, 8, 4)                                                                         \
-    X(256, 3, 16, 16, 2)                                                                        \
-    X(256, 4, 32, 8, 2)                                                                         \
-    X(512, 1, 32, 8, 2)
+    X(128, 1, 16, 16, 2, 0)                                                                     \
+    X(128, 2, 32, 16, 2, 0)                                                                     \
+    X(128, 3, 16, 8, 3, 0)                                                                      \
+    X(128, 4, 16, 8, 1, 1)                                                                      \
+    X(256, 1, 16, 8, 3, 0)                                                                      \
+    X(256, 2, 16, 8, 4, 0)                                                                      \
+    X(256, 3, 16, 16, 2, 0)                                                                     \
+    X(256, 4, 32, 8, 2, 0)                                                                      \
+    X(256, 5, 16, 8, 1, 1)                                                                      \
+    X(256, 6, 16, 8, 2, 1)                                                                      \
+    X(512, 1, 32, 8, 3, 0)                                                                      \
+    X(512, 2, 32, 8, 1, 1)
 
 namespace hzl_amd {
 

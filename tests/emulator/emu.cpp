@@ -99,8 +99,10 @@ template <class Body> void launch(long blocks, int threads, size_t lds_bytes, Bo
         t.join();
 }
 
-// GC / GR: launch shapes of the column and row kernels (variants.hpp).
-template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>> struct Sim {
+// GC / GR: launch shapes of the column and row kernels (variants.hpp);
+// DIRECT: row spectra straight from the mixed fields (row_body_direct).
+template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>, bool DIRECT = false>
+struct Sim {
     using G = Geometry<N>;
     using FFT = GroupFFT<N, GC::E, R>;
     int B;
@@ -179,6 +181,17 @@ template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>> str
     void row(bool with_cfl) {
         RowArgs<R> a{B, mix.data(), nl.data(), tw_row.data(), with_cfl ? cfl.data() : nullptr};
         const long blocks = ((long)B * (N / 2) + GR::ROW_PPW - 1) / GR::ROW_PPW;
+        if constexpr (DIRECT) {
+            launch(blocks, GR::ROW_THREADS,
+                   sizeof(cx<R>) * GR::ROW_PPW * GroupFFT<N, GR::E, R>::LDS_ELEMS,
+                   [&](const CpuCtx &ctx, long b, unsigned char *lds) {
+                       if (with_cfl)
+                           row_body_direct<N, R, true, GR>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+                       else
+                           row_body_direct<N, R, false, GR>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+                   });
+            return;
+        }
         launch(blocks, GR::ROW_THREADS, sizeof(cx<R>) * RowShape<GR>::template lds_elems<R>(),
                [&](const CpuCtx &ctx, long b, unsigned char *lds) {
                    if (with_cfl)
@@ -221,11 +234,11 @@ template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>> str
     }
 };
 
-template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>>
+template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>, bool DIRECT = false>
 int run(int B, double nu, double alpha, double sigma, double amp, int kf, int kmin, int kmax,
         uint64_t seed, double dt, int steps, int grf, const double *init, const double *forcing,
         int forcing_shared, double *out, double *diag) {
-    Sim<N, R, GC, GR> s(B, nu, alpha, sigma, amp, kf, kmin, kmax, seed);
+    Sim<N, R, GC, GR, DIRECT> s(B, nu, alpha, sigma, amp, kf, kmin, kmax, seed);
     const size_t nn = (size_t)N * N;
     if (grf)
         s.grf();
@@ -446,9 +459,9 @@ int emu_run_variant(int n, int cv, int rv, int B, double nu, double alpha, doubl
 #undef HZL_X
     }
     if (cv == 0) {
-#define HZL_X(NN, IDX, E, PPW, W)                                                              \
+#define HZL_X(NN, IDX, E, PPW, W, D)                                                           \
     if (n == NN && rv == IDX)                                                                  \
-        return run<NN, float, Geometry<NN>, Geometry<NN, E, 64, PPW>>(                         \
+        return run<NN, float, Geometry<NN>, Geometry<NN, E, 64, PPW>, (D != 0)>(               \
             B, nu, alpha, sigma, amp, kf, kmin, kmax, seed, dt, steps, grf, init, nullptr, 1, out, \
             diag);
         HZL_ROW_VARIANTS(HZL_X)
