@@ -57,6 +57,13 @@ __global__ void __launch_bounds__(Geometry<N>::ROW_THREADS)
     row_body<N, R, CFL>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 
+template <int N, typename R>
+__global__ void __launch_bounds__(Geometry<N>::ROW_THREADS) k_row_products(ProductArgs<R> a) {
+    using G = Geometry<N>;
+    __shared__ cx<R> lds[G::ROW_PPW * GroupFFT<N, G::E, R>::LDS_ELEMS];
+    row_body_products<N, R, G>(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+
 template <typename R> __global__ void __launch_bounds__(256) k_grf(GrfArgs<R> a) {
     __shared__ double red[256];
     grf_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
@@ -173,8 +180,52 @@ template <int N, typename R> void launch_col(int mode, const StageArgs<R> &a, hi
     case 4:
         hipLaunchKernelGGL((k_col<N, 4, R>), grid, block, 0, s, a);
         return;
+    case 5:
+        hipLaunchKernelGGL((k_col<N, 5, R>), grid, block, 0, s, a);
+        return;
+    case 6:
+        hipLaunchKernelGGL((k_col<N, 6, R>), grid, block, 0, s, a);
+        return;
+    case 7:
+        hipLaunchKernelGGL((k_col<N, 7, R>), grid, block, 0, s, a);
+        return;
+    case 8:
+        hipLaunchKernelGGL((k_col<N, 8, R>), grid, block, 0, s, a);
+        return;
+    case 9:
+        hipLaunchKernelGGL((k_col<N, 9, R>), grid, block, 0, s, a);
+        return;
+    case 10:
+        hipLaunchKernelGGL((k_col<N, 10, R>), grid, block, 0, s, a);
+        return;
     }
     throw std::invalid_argument("bad stage");
+}
+
+template <int N, typename R> void launch_products(const ProductArgs<R> &a, hipStream_t s) {
+    using G = Geometry<N>;
+    const long pairs = (long)a.B * (N / 2);
+    hipLaunchKernelGGL((k_row_products<N, R>),
+                       dim3((unsigned)((pairs + G::ROW_PPW - 1) / G::ROW_PPW)),
+                       dim3(G::ROW_THREADS), 0, s, a);
+}
+
+template <typename R> void dispatch_products(int n, const ProductArgs<R> &a, hipStream_t s) {
+    switch (n) {
+    case 16:
+        return launch_products<16, R>(a, s);
+    case 32:
+        return launch_products<32, R>(a, s);
+    case 64:
+        return launch_products<64, R>(a, s);
+    case 128:
+        return launch_products<128, R>(a, s);
+    case 256:
+        return launch_products<256, R>(a, s);
+    case 512:
+        return launch_products<512, R>(a, s);
+    }
+    throw std::invalid_argument("unsupported grid");
 }
 
 template <int N, typename R> void launch_row(const RowArgs<R> &a, hipStream_t s) {
@@ -484,6 +535,91 @@ template <typename R> class Engine final : public EngineBase {
         check(hipStreamSynchronize(stream_), "graph switch");
         use_graph_ = on;
         clear_graphs();
+    }
+
+    // ------------------------------------- SSPRK3 and calibration drifts ----
+    // Solver::step(dt, TimeIntegrator::ssprk3) (src/solver.cpp:235-246) for
+    // every member: device noise held through 3 x (row, col) launches.
+    void step_integrator(double dt, size_t steps, int integrator) override {
+        if (integrator == HZL_RK4)
+            return step(dt, steps);
+        if (integrator != HZL_SSPRK3)
+            throw std::invalid_argument("integrator must be HZL_RK4 or HZL_SSPRK3");
+        validate_dt(dt);
+        flush_parameters();
+        for (size_t s = 0; s < steps; ++s) {
+            ensure_mixed(dt);
+            NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_, noise_.p};
+            const long count = (long)B_ * npairs_;
+            hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                               stream_, na);
+            RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_row_, nullptr};
+            for (int stage = 1; stage <= 3; ++stage) {
+                r.cfl = stage == 1 ? cfl_.p : nullptr;
+                run_row(r);
+                run_col(4 + stage, stage_args(dt, nullptr, 0));
+            }
+            check(hipGetLastError(), "SSPRK3 kernels");
+            ++step_;
+            last_dt_ = dt;
+            max_dt_ = std::max(max_dt_, dt);
+        }
+    }
+
+    // Solver::drift (src/solver.cpp:255-263) of every member's state: the
+    // right-hand side without forcing.  out: members reference Fields.
+    void drift(double *out) override {
+        flush_parameters();
+        calib_buffers();
+        jacobian(w0_.p, w0_.p, calA_.p, nlA_.p); // J(w, w)
+        StageArgs<R> a = stage_args(1.0, nullptr, 0);
+        a.nl = nlA_.p;
+        a.w1 = w0_.p;
+        a.out = cout_.p;
+        run_col(9, a);
+        download_fields(cout_.p, out);
+    }
+
+    // Solver::linearized_drift (src/solver.cpp:264-290): the tangent of the
+    // drift at each member's state along directions[m],
+    // -(J(d, w) + J(w, d)) - (nu k^2 + alpha) d, with J(a, b) = u(a) . grad b
+    // formed by the row pass from a two-state column pass.
+    void linearized_drift(const double *directions, double *out) override {
+        flush_parameters();
+        calib_buffers();
+        upload_fields(directions, caux_.p);
+        jacobian(caux_.p, w0_.p, calA_.p, nlA_.p); // J(d, w)
+        jacobian(w0_.p, caux_.p, calB_.p, nlB_.p); // J(w, d)
+        StageArgs<R> a = stage_args(1.0, nullptr, 0);
+        a.nl = nlA_.p;
+        a.nl2 = nlB_.p;
+        a.w1 = caux_.p;
+        a.out = cout_.p;
+        run_col(9, a);
+        download_fields(cout_.p, out);
+    }
+
+    // Solver::adjoint_drift (src/solver.cpp:291-323) at each member's state
+    // applied to cotangents[m]: J(w, lam) + i ky/k^2 F[lam wx]
+    // - i kx/k^2 F[lam wy] - (nu k^2 + alpha) lam.
+    void adjoint_drift(const double *cotangents, double *out) override {
+        flush_parameters();
+        calib_buffers();
+        upload_fields(cotangents, caux_.p);
+        jacobian(w0_.p, caux_.p, calA_.p, nlA_.p); // J(w, lam); calA holds lam~
+        StageArgs<R> m0 = stage_args(1.0, nullptr, 0);
+        m0.mix = calB_.p; // mixed fields of the state: w~, (ky w)~
+        run_col(0, m0);
+        ProductArgs<R> pa{(int)B_, calA_.p, calB_.p, nlB_.p, nlC_.p, tw_.p};
+        dispatch_products<R>(n_, pa, stream_);
+        StageArgs<R> a = stage_args(1.0, nullptr, 0);
+        a.nl = nlA_.p;
+        a.nl2 = nlB_.p;
+        a.nl3 = nlC_.p;
+        a.w1 = caux_.p;
+        a.out = cout_.p;
+        run_col(10, a);
+        download_fields(cout_.p, out);
     }
 
     // ------------------------------------------------ launch shapes ----
@@ -984,6 +1120,14 @@ template <typename R> class Engine final : public EngineBase {
 
     // Column / row launches through the selected launch-shape variant.
     void run_col(int mode, const StageArgs<R> &a) {
+        if (mode > 4 && col_var_ > 0) {
+            // SSPRK3 stages and drift passes run in the default shape (the
+            // mixed and nonlinear fields are shape independent)
+            StageArgs<R> d = a;
+            d.tw = tw_.p;
+            dispatch_col<R>(n_, mode, d, stream_);
+            return;
+        }
         if constexpr (std::is_same_v<R, float>) {
             if (col_var_ > 0) {
                 if (!launch_col_variant(n_, col_var_, mode, a, stream_))
@@ -1016,6 +1160,56 @@ template <typename R> class Engine final : public EngineBase {
         tw_extra_.emplace_back(E, DeviceBuffer<cx<R>>(tw.size()));
         upload(tw_extra_.back().second.p, tw.data(), tw.size());
         return tw_extra_.back().second.p;
+    }
+
+    // Buffers of the calibration drifts, allocated on first use.
+    void calib_buffers() {
+        if (caux_.count)
+            return;
+        caux_ = DeviceBuffer<cx<R>>(B_ * spec_);
+        cout_ = DeviceBuffer<cx<R>>(B_ * spec_);
+        calA_ = DeviceBuffer<cx<R>>(B_ * 4 * mixf_);
+        calB_ = DeviceBuffer<cx<R>>(B_ * 4 * mixf_);
+        nlA_ = DeviceBuffer<cx<R>>(B_ * mixf_);
+        nlB_ = DeviceBuffer<cx<R>>(B_ * mixf_);
+        nlC_ = DeviceBuffer<cx<R>>(B_ * mixf_);
+    }
+
+    // J(a, b) = u(a) . grad(b) of every member into the nonlinear mixed field
+    // nl: two-state column pass (psi of a, w of b) into mix, then a row pass.
+    void jacobian(const cx<R> *a_state, const cx<R> *b_state, cx<R> *mix, cx<R> *nl) {
+        StageArgs<R> c = stage_args(1.0, nullptr, 0);
+        c.w0 = const_cast<cx<R> *>(a_state);
+        c.w1 = const_cast<cx<R> *>(b_state);
+        c.mix = mix;
+        run_col(8, c);
+        RowArgs<R> r{(int)B_, mix, nl, tw_row_, nullptr};
+        run_row(r);
+    }
+
+    // members reference Fields (n*n interleaved doubles each) -> retained
+    // half planes on the device, dropping masked modes as set_state does.
+    void upload_fields(const double *fields, cx<R> *dst) {
+        const size_t nn = (size_t)n_ * n_;
+        for (size_t i = 0; i < 2 * nn * B_; ++i)
+            if (!std::isfinite(fields[i]))
+                throw std::invalid_argument("nonfinite GPU field");
+        std::vector<cx<R>> half(spec_ * B_);
+        for (size_t k = 0; k < B_; ++k)
+            full_to_half<R>(n_, fields + 2 * nn * k, half.data() + spec_ * k);
+        upload(dst, half.data(), half.size());
+    }
+
+    void download_fields(const cx<R> *src, double *fields) {
+        check(hipGetLastError(), "drift kernels");
+        std::vector<cx<R>> half(spec_ * B_);
+        check(hipMemcpyAsync(half.data(), src, half.size() * sizeof(cx<R>),
+                             hipMemcpyDeviceToHost, stream_),
+              "drift download");
+        check(hipStreamSynchronize(stream_), "drift download");
+        const size_t nn = (size_t)n_ * n_;
+        for (size_t k = 0; k < B_; ++k)
+            half_to_full<R>(n_, half.data() + spec_ * k, fields + 2 * nn * k);
     }
 
     // Mixed fields of the current states (pass A), once after any change.
@@ -1210,6 +1404,8 @@ template <typename R> class Engine final : public EngineBase {
     bool params_dirty_ = false, mix_valid_ = false;
     uint64_t step_ = 0;
     double last_dt_ = 0, max_dt_ = 0, last_cfl_rate_ = 0;
+    // calibration drifts: input fields, result, mixed and nonlinear fields
+    DeviceBuffer<cx<R>> caux_, cout_, calA_, calB_, nlA_, nlB_, nlC_;
     // launch shapes (variants.hpp) and their twiddle tables
     int col_var_ = 0, row_var_ = 0;
     const cx<R> *tw_col_ = nullptr, *tw_row_ = nullptr;
@@ -1357,6 +1553,34 @@ int hzl_ensemble_init_grf(hzl_ensemble_t h) {
 
 int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps) {
     return guarded([&] { engine_of(h).step(dt, steps); });
+}
+
+int hzl_ensemble_step_integrator(hzl_ensemble_t h, double dt, size_t steps, int integrator) {
+    return guarded([&] { engine_of(h).step_integrator(dt, steps, integrator); });
+}
+
+int hzl_ensemble_drift(hzl_ensemble_t h, double *out) {
+    return guarded([&] {
+        if (!out)
+            throw std::invalid_argument("null argument");
+        engine_of(h).drift(out);
+    });
+}
+
+int hzl_ensemble_linearized_drift(hzl_ensemble_t h, const double *directions, double *out) {
+    return guarded([&] {
+        if (!directions || !out)
+            throw std::invalid_argument("null argument");
+        engine_of(h).linearized_drift(directions, out);
+    });
+}
+
+int hzl_ensemble_adjoint_drift(hzl_ensemble_t h, const double *cotangents, double *out) {
+    return guarded([&] {
+        if (!cotangents || !out)
+            throw std::invalid_argument("null argument");
+        engine_of(h).adjoint_drift(cotangents, out);
+    });
 }
 
 int hzl_ensemble_set_graph(hzl_ensemble_t h, int enabled) {

@@ -74,6 +74,11 @@ class EngineBase {
     virtual void init_grf() = 0;
     virtual void step(double dt, size_t steps) = 0;
     virtual void set_graph(bool on) = 0;
+    // SSPRK3 stepping and the calibration drifts (batched over members)
+    virtual void step_integrator(double dt, size_t steps, int integrator) = 0;
+    virtual void drift(double *out) = 0;
+    virtual void linearized_drift(const double *directions, double *out) = 0;
+    virtual void adjoint_drift(const double *cotangents, double *out) = 0;
     // launch-shape variants of the RK4 kernels (variant 0: the default)
     virtual int col_variants() const = 0;
     virtual int row_variants() const = 0;

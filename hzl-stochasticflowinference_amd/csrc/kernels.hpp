@@ -222,9 +222,24 @@ template <typename R> struct StageArgs {
     const cx<R> *ext;     // host-provided held forcing [.][KP][J] or null
     long long ext_stride; // complex entries between members (0: shared)
     R h_half, h_full, h_sixth;
+    // drift outputs (modes 9, 10): further nonlinear mixed fields and the
+    // retained-half-plane result
+    const cx<R> *nl2 = nullptr, *nl3 = nullptr;
+    cx<R> *out = nullptr;
 };
 
-// MODE 0: pass A only (mixed fields of w0); MODE 1..4: RK4 stage MODE.
+// Column pass modes.
+//   0      pass A only: mixed fields of w0
+//   1..4   RK4 stage MODE (src/solver.cpp:223-234)
+//   5..7   SSPRK(3,3) stage MODE - 4 (src/solver.cpp:235-246): u1 -> w1,
+//          u2 -> acc, the new state -> w0
+//   8      pass A of two states: psi~, (ky psi)~ of w0 and w~, (ky w)~ of w1,
+//          so the row pass forms J(w0, w1) = u(w0) . grad(w1)
+//   9      out = -FFT_y(nl [+ nl2]) / N^2 - (nu k^2 + alpha) w1 on the mask
+//          (drift and its tangent, src/solver.cpp:255-290); no pass A
+//   10     out = FFT_y(nl) / N^2 + i ky / k^2 FFT_y(nl2) / N^2
+//          - i kx / k^2 FFT_y(nl3) / N^2 - (nu k^2 + alpha) w1 on the mask
+//          (the adjoint drift, src/solver.cpp:291-323); no pass A
 template <int N, int MODE, typename R, class G = Geometry<N>, class Ctx>
 HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long block) {
     static_assert(G::NN == N, "launch shape of another grid");
@@ -239,16 +254,67 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
     const long soff = (long)m * G::SPEC + (long)c * J;
     const TwArg<FFT, R> twa = lane_twiddles<FFT>(a.tw, t);
     cx<R> v[E];
+    cx<R> vb[MODE == 8 ? E : 1]; // the w-source state of mode 8
 
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 || MODE == 8) {
 #pragma unroll
         for (int idx = 0; idx < E; ++idx) {
             const int k = FFT::index(t, idx);
             const int ky = k < N / 2 ? k : k - N;
-            v[idx] = (!FFT::slot_outside(idx, K) && live && ky >= -K && ky <= K)
-                         ? a.w0[soff + ky + K]
-                         : cx<R>{0, 0};
+            const bool in = !FFT::slot_outside(idx, K) && live && ky >= -K && ky <= K;
+            v[idx] = in ? a.w0[soff + ky + K] : cx<R>{0, 0};
+            if constexpr (MODE == 8)
+                vb[idx] = in ? a.w1[soff + ky + K] : cx<R>{0, 0};
         }
+    } else if constexpr (MODE == 9 || MODE == 10) {
+        constexpr int NF = MODE == 10 ? 3 : 1;
+        const cx<R> *src[3] = {a.nl, a.nl2, a.nl3};
+        const R inv_nn = R(1) / (R(N) * R(N));
+        const R nu = live ? (R)a.nu[m] : R(0), alpha = live ? (R)a.alpha[m] : R(0);
+        cx<R> f[NF][E];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            const cx<R> *nl = src[q] + (long)m * G::MIXF;
+            const cx<R> *nl2 = MODE == 9 && a.nl2 ? a.nl2 + (long)m * G::MIXF : nullptr;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                cx<R> x = live ? nl[tiled(KP, c, t + T * j)] : cx<R>{0, 0};
+                if (nl2 && live)
+                    x = cadd(x, nl2[tiled(KP, c, t + T * j)]);
+                f[q][j] = x;
+            }
+            FFT::forward(ctx, f[q], L, t, twa);
+        }
+#pragma unroll
+        for (int idx = 0; idx < E; ++idx) {
+            if (FFT::slot_outside(idx, K))
+                continue;
+            const int k = FFT::index(t, idx);
+            const int ky = k < N / 2 ? k : k - N;
+            if (live && ky >= -K && ky <= K) {
+                const long off = soff + ky + K;
+                const int k2 = c * c + ky * ky;
+                cx<R> r = {0, 0};
+                if (k2 != 0) {
+                    const cx<R> x = a.w1[off];
+                    const R rate = nu * (R)k2 + alpha;
+                    const cx<R> n1 = cscale(f[0][idx], inv_nn);
+                    if constexpr (MODE == 9) {
+                        r = {-n1.x - rate * x.x, -n1.y - rate * x.y};
+                    } else {
+                        const R q = R(1) / (R)k2;
+                        const cx<R> n2 = cscale(f[1][idx], inv_nn);
+                        const cx<R> n3 = cscale(f[NF - 1][idx], inv_nn);
+                        const R sy = (R)ky * q, sx = (R)c * q;
+                        // + i sy n2 - i sx n3
+                        r = {n1.x - sy * n2.y + sx * n3.y - rate * x.x,
+                             n1.y + sy * n2.x - sx * n3.x - rate * x.y};
+                    }
+                }
+                a.out[off] = r;
+            }
+        }
+        return;
     } else {
         const cx<R> *nl = a.nl + (long)m * G::MIXF;
 #pragma unroll
@@ -268,9 +334,11 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
                 if (!FFT::slot_outside(idx, K) && live && ky >= -K && ky <= K) {
                     const long off = soff + ky + K;
                     wn[idx] = a.w0[off];
-                    if constexpr (MODE >= 3)
+                    if constexpr (MODE == 3 || MODE == 4 || MODE == 6)
                         ws[idx] = a.w1[off];
-                    if constexpr (MODE >= 2)
+                    if constexpr (MODE == 7)
+                        ws[idx] = a.acc[off];
+                    if constexpr (MODE >= 2 && MODE <= 4)
                         sa[idx] = a.acc[off];
                 }
             }
@@ -302,7 +370,7 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
                 const long off = soff + ky + K;
                 const int k2 = c * c + ky * ky;
                 const cx<R> w_n = wn[idx];
-                const cx<R> w_s = MODE == 1 ? w_n : ws[idx];
+                const cx<R> w_s = (MODE == 1 || MODE == 5) ? w_n : ws[idx];
                 cx<R> rhs = {0, 0};
                 if (k2 != 0) {
                     cx<R> F;
@@ -325,8 +393,22 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
                     a.acc[off] = cadd(sa[idx], cscale(rhs, R(2)));
                     next = cadd(w_n, cscale(rhs, a.h_full));
                     a.w1[off] = next;
-                } else {
+                } else if constexpr (MODE == 4) {
                     next = cadd(w_n, cscale(cadd(sa[idx], rhs), a.h_sixth));
+                    a.w0[off] = next;
+                } else if constexpr (MODE == 5) {
+                    next = cadd(w_n, cscale(rhs, a.h_full)); // u1
+                    a.w1[off] = next;
+                } else if constexpr (MODE == 6) {
+                    const cx<R> u1 = ws[idx];
+                    next = cadd(cscale(w_n, R(0.75)),
+                                cscale(cadd(u1, cscale(rhs, a.h_full)), R(0.25))); // u2
+                    a.acc[off] = next;
+                } else {
+                    const cx<R> u2 = ws[idx];
+                    const R third = R(3), two_thirds = (R)(2.0 / 3.0);
+                    const cx<R> e = cadd(u2, cscale(rhs, a.h_full));
+                    next = {w_n.x / third + two_thirds * e.x, w_n.y / third + two_thirds * e.y};
                     a.w0[off] = next;
                 }
             }
@@ -352,7 +434,11 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
                 s = inv * (R)ky;
             else if (f == 3)
                 s = (R)ky;
-            u[idx] = cscale(v[idx], s);
+            cx<R> src = v[idx];
+            if constexpr (MODE == 8)
+                if (f >= 2)
+                    src = vb[idx];
+            u[idx] = cscale(src, s);
         }
         FFT::inverse(ctx, u, L, t, twa);
         if (live) {
@@ -441,6 +527,16 @@ template <class G> HZL_HD int staged(int kx, int ly) {
     return (ly >> 4) * (G::KP * RS::PITCH) + kx * RS::PITCH + (ly & 15);
 }
 
+// Entry k of the packed spectrum z = a + i b of two real rows with x spectra
+// A and B at x wave number kx = k (pos) or kx = N - k (the conjugate half).
+template <typename R> HZL_HD cx<R> pack_entry(cx<R> A, cx<R> Bv, int k, bool pos) {
+    if (k == 0) // DC terms of real signals: real parts only
+        return {A.x, Bv.x};
+    if (pos)
+        return {A.x - Bv.y, A.y + Bv.x};
+    return {A.x + Bv.y, Bv.x - A.y};
+}
+
 // Entry k of the packed row spectrum z1 = u + i wx (FIRST: p = (ky psi)~,
 // q = w~) or z2 = v + i wy (p = psi~, q = (ky w)~) at x wave number kx = k
 // (pos) or kx = N - k (the conjugate half).
@@ -451,11 +547,7 @@ HZL_HD cx<R> z_entry(cx<R> p, cx<R> q, int k, int kx, bool pos) {
     // else : a = v = -i kx P,    b = wy = i Q
     const cx<R> A = FIRST ? mul_i(p) : mul_mi(cscale(p, fx));
     const cx<R> Bv = FIRST ? mul_i(cscale(q, fx)) : mul_i(q);
-    if (k == 0) // DC terms of real signals: real parts only
-        return {A.x, Bv.x};
-    if (pos)
-        return {A.x - Bv.y, A.y + Bv.x};
-    return {A.x + Bv.y, Bv.x - A.y};
+    return pack_entry(A, Bv, k, pos);
 }
 
 // One row-pair group: build the packed spectra z1 = u + i wx (fields B, C) or
@@ -539,8 +631,9 @@ HZL_HD void row_finish(const Ctx &ctx, const RowArgs<R> &a, cx<R> *L, int t, con
             nl[o + 1] = p1;
         }
     }
+    // the group's region is free again only after every lane's unpack reads
+    ctx.wsync();
     if constexpr (CFL) {
-        ctx.wsync();
         R *red = reinterpret_cast<R *>(L);
         red[t] = vmax;
         ctx.wsync();
@@ -716,6 +809,64 @@ HZL_HD void row_body_direct(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, lon
         }
     }
     row_finish<N, R, CFL, G>(ctx, a, L, t, twa, pa, pb, vmax, live, m, y0, gp);
+}
+
+// Products of the adjoint drift (src/solver.cpp:291-323).  For each row pair:
+// the physical cotangent lam (field 2 of mixA) and the physical gradient
+// (wx, wy) of the state (fields 2, 3 of mixB, the state's w~ and (ky w)~),
+// then lam wx and lam wy, forward x-transformed into the mixed fields out2,
+// out3.  Spectra are loaded straight from HBM, one row at a time.
+template <typename R> struct ProductArgs {
+    int B;
+    const cx<R> *mixA, *mixB; // [member][4][mixed field]
+    cx<R> *out2, *out3;       // [member][mixed field]
+    const cx<R> *tw;
+};
+
+template <int N, typename R, class G, class Ctx>
+HZL_HD void row_body_products(const Ctx &ctx, const ProductArgs<R> &a, cx<R> *lds, long block) {
+    static_assert(G::NN == N, "launch shape of another grid");
+    using FFT = GroupFFT<N, G::E, R>;
+    constexpr int E = G::E, T = G::T, K = G::K, KP = G::KP, PPW = G::ROW_PPW;
+    const int tid = ctx.tid();
+    const int g = tid / T, t = tid % T;
+    const long gp = block * PPW + g;
+    const bool live = gp < (long)a.B * (N / 2);
+    const int m = live ? (int)((2 * gp) / N) : 0, y0 = live ? (int)((2 * gp) % N) : 0;
+    cx<R> *L = lds + g * FFT::LDS_ELEMS;
+    const TwArg<FFT, R> twa = lane_twiddles<FFT>(a.tw, t);
+    const cx<R> *lam = a.mixA + (long)m * 4 * G::MIXF + 2 * G::MIXF;
+    const cx<R> *w = a.mixB + (long)m * 4 * G::MIXF + 2 * G::MIXF, *kw = w + G::MIXF;
+    R p2[2][E], p3[2][E];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        // zA = lam + i wx, zB = wy (+ i 0) of row y0 + r
+        cx<R> zA[E], zB[E];
+#pragma unroll
+        for (int idx = 0; idx < E; ++idx) {
+            const int k = FFT::index(t, idx);
+            const bool pos = k <= K, neg = k >= N - K;
+            zA[idx] = zB[idx] = {0, 0};
+            if (!FFT::slot_outside(idx, K) && (pos || neg)) {
+                const int kx = pos ? k : N - k;
+                const long o = tiled(KP, kx, y0 + r);
+                const cx<R> wx = mul_i(cscale(w[o], (R)kx)), wy = mul_i(kw[o]);
+                zA[idx] = pack_entry(lam[o], wx, k, pos);
+                zB[idx] = pack_entry(wy, cx<R>{0, 0}, k, pos);
+            }
+        }
+        FFT::inverse(ctx, zA, L, t, twa);
+        FFT::inverse(ctx, zB, L, t, twa);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            p2[r][j] = zA[j].x * zA[j].y;
+            p3[r][j] = zA[j].x * zB[j].x;
+        }
+    }
+    const RowArgs<R> r2{a.B, nullptr, a.out2, a.tw, nullptr};
+    const RowArgs<R> r3{a.B, nullptr, a.out3, a.tw, nullptr};
+    row_finish<N, R, false, G>(ctx, r2, L, t, twa, p2[0], p2[1], R(0), live, m, y0, gp);
+    row_finish<N, R, false, G>(ctx, r3, L, t, twa, p3[0], p3[1], R(0), live, m, y0, gp);
 }
 
 // ---------------------------------------------------------------- GRF ----

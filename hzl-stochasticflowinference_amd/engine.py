@@ -26,6 +26,8 @@ EXPORTS = (
     "hzl_last_error", "hzl_ensemble_create", "hzl_ensemble_destroy",
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
     "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
+    "hzl_ensemble_step_integrator", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
+    "hzl_ensemble_adjoint_drift",
     "hzl_ensemble_variant_count", "hzl_ensemble_set_tuning", "hzl_ensemble_get_tuning",
     "hzl_ensemble_autotune",
     "hzl_ensemble_step_forced",
@@ -96,6 +98,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_init_grf.argtypes = [vp]
     L.hzl_ensemble_step.argtypes = [vp, C.c_double, C.c_size_t]
     L.hzl_ensemble_set_graph.argtypes = [vp, C.c_int]
+    L.hzl_ensemble_step_integrator.argtypes = [vp, C.c_double, C.c_size_t, C.c_int]
+    L.hzl_ensemble_drift.argtypes = [vp, _dp]
+    L.hzl_ensemble_linearized_drift.argtypes = [vp, _dp, _dp]
+    L.hzl_ensemble_adjoint_drift.argtypes = [vp, _dp, _dp]
     ipp = C.POINTER(C.c_int)
     L.hzl_ensemble_variant_count.argtypes = [vp, ipp, ipp]
     L.hzl_ensemble_set_tuning.argtypes = [vp, C.c_int, C.c_int]
@@ -227,8 +233,37 @@ class Ensemble:
     def init_grf(self):
         _check(_lib.hzl_ensemble_init_grf(self._h))
 
-    def step(self, dt: float, steps: int = 1):
-        _check(_lib.hzl_ensemble_step(self._h, dt, steps))
+    def step(self, dt: float, steps: int = 1, integrator: str = "rk4"):
+        """`steps` steps of every member; integrator "rk4" or "ssprk3"."""
+        if integrator == "rk4":
+            _check(_lib.hzl_ensemble_step(self._h, dt, steps))
+        else:
+            code = {"ssprk3": 1}.get(integrator)
+            if code is None:
+                raise HzlInvalidArgument("integrator must be rk4 or ssprk3")
+            _check(_lib.hzl_ensemble_step_integrator(self._h, dt, steps, code))
+
+    def drift(self) -> np.ndarray:
+        """Solver::drift of every member's state: (members, n, n) spectral."""
+        out = np.zeros((self.members, self.n, self.n), dtype=np.complex128)
+        _check(_lib.hzl_ensemble_drift(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def linearized_drift(self, directions) -> np.ndarray:
+        """Solver::linearized_drift of every member along directions[m]."""
+        d = _fields(directions, self.members, self.n)
+        out = np.zeros((self.members, self.n, self.n), dtype=np.complex128)
+        _check(_lib.hzl_ensemble_linearized_drift(self._h, d.ctypes.data_as(_dp),
+                                                  out.ctypes.data_as(_dp)))
+        return out
+
+    def adjoint_drift(self, cotangents) -> np.ndarray:
+        """Solver::adjoint_drift of every member applied to cotangents[m]."""
+        c = _fields(cotangents, self.members, self.n)
+        out = np.zeros((self.members, self.n, self.n), dtype=np.complex128)
+        _check(_lib.hzl_ensemble_adjoint_drift(self._h, c.ctypes.data_as(_dp),
+                                               out.ctypes.data_as(_dp)))
+        return out
 
     def set_graph(self, enabled: bool):
         """Replay steps and filter cycles from captured HIP graphs (default on)."""

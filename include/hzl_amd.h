@@ -41,6 +41,10 @@ extern "C" {
 #define HZL_FP64 1
 #define HZL_BF16 2 /* surrogate inference operands only */
 
+/* time integrators (reference hzl::TimeIntegrator, include/hzl/solver.hpp:12) */
+#define HZL_RK4 0
+#define HZL_SSPRK3 1
+
 typedef struct hzl_ensemble *hzl_ensemble_t;
 typedef struct hzl_surrogate *hzl_surrogate_t;
 typedef struct hzl_pf_joint *hzl_pf_joint_t;
@@ -105,6 +109,29 @@ int hzl_ensemble_init_grf(hzl_ensemble_t h);
  * dt * max(|u|+|v|) n / 2 pi <= 0.5 (src/solver.cpp:204-216) is evaluated on
  * the device at every step and reported by hzl_ensemble_synchronize. */
 int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps);
+
+/* `steps` steps of every member with the given integrator: HZL_RK4 as
+ * hzl_ensemble_step, or HZL_SSPRK3, the Shu-Osher SSPRK(3,3) scheme with the
+ * forcing held through its three stages.  Replaces: Solver::step(dt,
+ * TimeIntegrator) (src/solver.cpp:217-254), batched; same time-step checks. */
+int hzl_ensemble_step_integrator(hzl_ensemble_t h, double dt, size_t steps, int integrator);
+
+/* Calibration drifts of every member's current state, batched: out,
+ * directions and cotangents hold one reference Field per member (members
+ * n*n interleaved doubles; input modes outside the dealiasing mask are
+ * dropped, as hzl_ensemble_set_state does).  Each member uses its own
+ * viscosity and friction.
+ *   hzl_ensemble_drift             Solver::drift (src/solver.cpp:255-263): the
+ *                                  right-hand side without forcing
+ *   hzl_ensemble_linearized_drift  Solver::linearized_drift
+ *                                  (src/solver.cpp:264-290): its tangent along
+ *                                  directions[m]
+ *   hzl_ensemble_adjoint_drift     Solver::adjoint_drift (src/solver.cpp:291-323):
+ *                                  the tangent's adjoint applied to
+ *                                  cotangents[m], for Re sum conj(a) b */
+int hzl_ensemble_drift(hzl_ensemble_t h, double *out);
+int hzl_ensemble_linearized_drift(hzl_ensemble_t h, const double *directions, double *out);
+int hzl_ensemble_adjoint_drift(hzl_ensemble_t h, const double *cotangents, double *out);
 
 /* Replay launch sequences from captured HIP graphs (default on).  A
  * hzl_ensemble_step of 2..200 steps (longer calls in chunks of 200), and a

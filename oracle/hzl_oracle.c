@@ -18,6 +18,8 @@
  *   - bootstrap-PF weighting, log-sum-exp, ESS, systematic
  *     resampling                                                  (src/filter.cpp:38-126)
  *   - serial perturbed-observation EnKF scalar update             (src/filter.cpp:265-321)
+ *   - SSPRK(3,3) step, tangent and adjoint drifts                  (src/solver.cpp:235-246,
+ *                                                                  264-323)
  * plus the engine's own input definitions that have no reference counterpart
  * (Philox4x32-10 counter RNG, shell white noise, Kolmogorov forcing, Gaussian
  * random-field initial condition), written from their published definitions.
@@ -37,6 +39,10 @@
  *                 long-double LSE, brute-force systematic resampling
  *   EnKF scalar-- dense closed form on tiny ensembles, reference deterministic limit
  *   grf        -- enstrophy 1, Hermitian symmetry, spectrum slope (statistical)
+ *   ssprk3     -- single-shell closed form (stability polynomial, fixed point),
+ *                 reference build
+ *   tangent    -- central difference of the (quadratic) drift, reference build
+ *   adjoint    -- adjoint identity <lambda, L d> = <L* lambda, d>, reference build
  */
 #include <math.h>
 #include <stdint.h>
@@ -302,6 +308,149 @@ static void solver_step(orc_solver *s, double nu, double alpha, double dt, cpx *
             w[i].im = 0;
         }
     }
+}
+
+/* One Shu-Osher SSPRK(3,3) step with F held through the three stages
+ * (src/solver.cpp:235-246): u1 = w + dt k1, u2 = 3/4 w + 1/4 (u1 + dt k2),
+ * w+ = w / 3 + 2/3 (u2 + dt k3). */
+static void solver_step_ssprk3(orc_solver *s, double nu, double alpha, double dt, cpx *w,
+                               const cpx *F) {
+    const long c = s->n * s->n;
+    cpx *u1 = s->tmp, *u2 = s->k4;
+    solver_rhs(s, nu, alpha, w, F, s->k1);
+    for (long i = 0; i < c; ++i) {
+        u1[i].re = w[i].re + dt * s->k1[i].re;
+        u1[i].im = w[i].im + dt * s->k1[i].im;
+    }
+    solver_rhs(s, nu, alpha, u1, F, s->k2);
+    for (long i = 0; i < c; ++i) {
+        u2[i].re = 0.75 * w[i].re + 0.25 * (u1[i].re + dt * s->k2[i].re);
+        u2[i].im = 0.75 * w[i].im + 0.25 * (u1[i].im + dt * s->k2[i].im);
+    }
+    solver_rhs(s, nu, alpha, u2, F, s->k3);
+    for (long i = 0; i < c; ++i) {
+        w[i].re = w[i].re / 3.0 + 2.0 / 3.0 * (u2[i].re + dt * s->k3[i].re);
+        w[i].im = w[i].im / 3.0 + 2.0 / 3.0 * (u2[i].im + dt * s->k3[i].im);
+        if (!s->active[i]) {
+            w[i].re = 0;
+            w[i].im = 0;
+        }
+    }
+}
+
+/* `steps` SSPRK(3,3) steps in place, forcing as orc_rk4_steps. */
+int orc_ssprk3_steps(long n, double nu, double alpha, double dt, long steps, double *w,
+                     const double *forcing, long forcing_stride) {
+    orc_solver s;
+    if (solver_make(&s, n)) {
+        solver_free(&s);
+        return -1;
+    }
+    for (long k = 0; k < steps; ++k) {
+        const cpx *F = forcing ? (const cpx *)(forcing + 2 * forcing_stride * k) : NULL;
+        solver_step_ssprk3(&s, nu, alpha, dt, (cpx *)w, F);
+    }
+    solver_free(&s);
+    return 0;
+}
+
+/* i kx w and i ky w of a spectral field, transformed to physical space. */
+static void solver_gradient(orc_solver *s, const cpx *w, cpx *gx, cpx *gy) {
+    const long c = s->n * s->n;
+    for (long i = 0; i < c; ++i) {
+        gx[i].re = -s->kx[i] * w[i].im;
+        gx[i].im = s->kx[i] * w[i].re;
+        gy[i].re = -s->ky[i] * w[i].im;
+        gy[i].im = s->ky[i] * w[i].re;
+    }
+    fft2d_plan(&s->plan, gx, 1);
+    fft2d_plan(&s->plan, gy, 1);
+}
+
+/* Tangent of the drift along `direction` at `state` (src/solver.cpp:264-290):
+ * -FFT[u(d) . grad(state) + u(state) . grad(d)] - (nu k^2 + alpha) d on the
+ * mask, with u(x) the velocity of x's streamfunction. */
+int orc_linearized_drift(long n, double nu, double alpha, const double *state,
+                         const double *direction, double *out) {
+    orc_solver s;
+    if (solver_make(&s, n)) {
+        solver_free(&s);
+        return -1;
+    }
+    const long c = n * n;
+    const cpx *w = (const cpx *)state, *d = (const cpx *)direction;
+    cpx *o = (cpx *)out;
+    cpx *bu = s.k1, *bv = s.k2; /* velocity of the state */
+    solver_velocity(&s, w);
+    memcpy(bu, s.u, sizeof(cpx) * (size_t)c);
+    memcpy(bv, s.v, sizeof(cpx) * (size_t)c);
+    solver_gradient(&s, w, s.wx, s.wy);
+    solver_velocity(&s, d);
+    for (long i = 0; i < c; ++i) {
+        s.nl[i].re = s.u[i].re * s.wx[i].re + s.v[i].re * s.wy[i].re;
+        s.nl[i].im = 0;
+    }
+    solver_gradient(&s, d, s.wx, s.wy);
+    for (long i = 0; i < c; ++i)
+        s.nl[i].re += bu[i].re * s.wx[i].re + bv[i].re * s.wy[i].re;
+    fft2d_plan(&s.plan, s.nl, 0);
+    for (long i = 0; i < c; ++i) {
+        o[i].re = o[i].im = 0;
+        if (s.active[i]) {
+            const double rate = nu * s.ksq[i] + alpha;
+            o[i].re = -s.nl[i].re - rate * d[i].re;
+            o[i].im = -s.nl[i].im - rate * d[i].im;
+        }
+    }
+    solver_free(&s);
+    return 0;
+}
+
+/* Adjoint of the drift's tangent at `state` applied to `cotangent`
+ * (src/solver.cpp:291-323): FFT[u(state) . grad(lambda)]
+ * + i ky / k^2 FFT[lambda wx] - i kx / k^2 FFT[lambda wy]
+ * - (nu k^2 + alpha) lambda on the mask, lambda the physical cotangent and
+ * (wx, wy) the physical gradient of the state. */
+int orc_adjoint_drift(long n, double nu, double alpha, const double *state,
+                      const double *cotangent, double *out) {
+    orc_solver s;
+    if (solver_make(&s, n)) {
+        solver_free(&s);
+        return -1;
+    }
+    const long c = n * n;
+    const cpx *w = (const cpx *)state, *lam = (const cpx *)cotangent;
+    cpx *o = (cpx *)out;
+    cpx *phys = s.tmp, *lx = s.k1, *ly = s.k2;
+    solver_velocity(&s, w);
+    solver_gradient(&s, w, s.wx, s.wy);
+    memcpy(phys, lam, sizeof(cpx) * (size_t)c);
+    fft2d_plan(&s.plan, phys, 1);
+    solver_gradient(&s, lam, lx, ly);
+    for (long i = 0; i < c; ++i) {
+        s.nl[i].re = s.u[i].re * lx[i].re + s.v[i].re * ly[i].re;
+        s.nl[i].im = 0;
+        s.wx[i].re = phys[i].re * s.wx[i].re;
+        s.wx[i].im = 0;
+        s.wy[i].re = phys[i].re * s.wy[i].re;
+        s.wy[i].im = 0;
+    }
+    fft2d_plan(&s.plan, s.nl, 0);
+    fft2d_plan(&s.plan, s.wx, 0);
+    fft2d_plan(&s.plan, s.wy, 0);
+    for (long i = 0; i < c; ++i) {
+        o[i].re = o[i].im = 0;
+        if (s.active[i]) {
+            const double rate = nu * s.ksq[i] + alpha, q = 1.0 / s.ksq[i];
+            /* i ky/k^2 a = (-ky/k^2 a.im, ky/k^2 a.re) */
+            o[i].re = s.nl[i].re - s.ky[i] * q * s.wx[i].im + s.kx[i] * q * s.wy[i].im -
+                      rate * lam[i].re;
+            o[i].im = s.nl[i].im + s.ky[i] * q * s.wx[i].re - s.kx[i] * q * s.wy[i].re -
+                      rate * lam[i].im;
+        }
+    }
+    solver_free(&s);
+    return 0;
 }
 
 /* Drift RHS of one state; F may be NULL. Returns -1 on bad n. */

@@ -46,6 +46,13 @@ class Oracle:
         L.orc_active_mask.argtypes = [C.c_long, _u8p]
         L.orc_rhs.restype = C.c_int
         L.orc_rhs.argtypes = [C.c_long, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_ssprk3_steps.restype = C.c_int
+        L.orc_ssprk3_steps.argtypes = [C.c_long, C.c_double, C.c_double, C.c_double, C.c_long,
+                                       _dp, _dp, C.c_long]
+        L.orc_linearized_drift.restype = C.c_int
+        L.orc_linearized_drift.argtypes = [C.c_long, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_adjoint_drift.restype = C.c_int
+        L.orc_adjoint_drift.argtypes = [C.c_long, C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_rk4_steps.restype = C.c_int
         L.orc_rk4_steps.argtypes = [C.c_long, C.c_double, C.c_double, C.c_double, C.c_long, _dp,
                                     _dp, C.c_long]
@@ -118,6 +125,32 @@ class Oracle:
         rc = self.lib.orc_rhs(n, nu, alpha, _ptr(w), _ptr(F) if F is not None else None,
                               _ptr(out))
         if rc:
+            raise ValueError("bad grid")
+        return out
+
+    def ssprk3(self, state, nu, alpha, dt, steps, forcing=None):
+        """`steps` SSPRK(3,3) steps with one held forcing field (src/solver.cpp:235-246)."""
+        w = _c(state).copy()
+        n = w.shape[0]
+        F = _c(forcing) if forcing is not None else None
+        if self.lib.orc_ssprk3_steps(n, nu, alpha, dt, steps, _ptr(w),
+                                     _ptr(F) if F is not None else None, 0):
+            raise ValueError("bad grid")
+        return w
+
+    def linearized_drift(self, state, direction, nu, alpha):
+        """Solver::linearized_drift (src/solver.cpp:264-290)."""
+        w, d = _c(state), _c(direction)
+        out = np.zeros_like(w)
+        if self.lib.orc_linearized_drift(w.shape[0], nu, alpha, _ptr(w), _ptr(d), _ptr(out)):
+            raise ValueError("bad grid")
+        return out
+
+    def adjoint_drift(self, state, cotangent, nu, alpha):
+        """Solver::adjoint_drift (src/solver.cpp:291-323)."""
+        w, c = _c(state), _c(cotangent)
+        out = np.zeros_like(w)
+        if self.lib.orc_adjoint_drift(w.shape[0], nu, alpha, _ptr(w), _ptr(c), _ptr(out)):
             raise ValueError("bad grid")
         return out
 
@@ -345,6 +378,39 @@ class Reference:
     def initialize(self, n, name):
         out = np.zeros((n, n), dtype=np.complex128)
         self._check(self.lib.ref_initialize(n, name.encode(), _ptr(out)))
+        return out
+
+    def run_ssprk3(self, initial, nu, alpha, dt, steps, forcing=None, kmin=1, kmax=4):
+        """Reference Solver::step(dt, TimeIntegrator::ssprk3) with a held forcing."""
+        L = self.lib
+        L.ref_run_ssprk3.restype = C.c_int
+        L.ref_run_ssprk3.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t,
+                                     C.c_int, C.c_int, _dp, _dp, _dp]
+        w = _c(initial)
+        out = np.zeros_like(w)
+        F = _c(forcing) if forcing is not None else None
+        self._check(L.ref_run_ssprk3(w.shape[0], nu, alpha, dt, steps, kmin, kmax, _ptr(w),
+                                     _ptr(F) if F is not None else None, _ptr(out)))
+        return out
+
+    def linearized_drift(self, state, direction, nu, alpha):
+        """Reference Solver::linearized_drift."""
+        L = self.lib
+        L.ref_linearized_drift.restype = C.c_int
+        L.ref_linearized_drift.argtypes = [C.c_size_t, C.c_double, C.c_double, _dp, _dp, _dp]
+        w, d = _c(state), _c(direction)
+        out = np.zeros_like(w)
+        self._check(L.ref_linearized_drift(w.shape[0], nu, alpha, _ptr(w), _ptr(d), _ptr(out)))
+        return out
+
+    def adjoint_drift(self, state, cotangent, nu, alpha):
+        """Reference Solver::adjoint_drift."""
+        L = self.lib
+        L.ref_adjoint_drift.restype = C.c_int
+        L.ref_adjoint_drift.argtypes = [C.c_size_t, C.c_double, C.c_double, _dp, _dp, _dp]
+        w, c = _c(state), _c(cotangent)
+        out = np.zeros_like(w)
+        self._check(L.ref_adjoint_drift(w.shape[0], nu, alpha, _ptr(w), _ptr(c), _ptr(out)))
         return out
 
     def initialize_physical(self, physical):

@@ -115,6 +115,38 @@ int ref_run(std::size_t n, double nu, double alpha, double dt, std::size_t steps
     });
 }
 
+// `steps` reference SSPRK(3,3) steps (Solver::step(dt, TimeIntegrator::ssprk3))
+// from a spectral state with one forcing field held on the shell pairs.
+int ref_run_ssprk3(std::size_t n, double nu, double alpha, double dt, std::size_t steps,
+                   int kmin, int kmax, const double *initial, const double *forcing,
+                   double *out) {
+    return guarded([&] {
+        hzl::Solver s(n, frozen(nu, alpha, kmin, kmax), 0);
+        s.update_latent_state(to_field(initial, n * n),
+                              forcing ? to_field(forcing, n * n) : hzl::Field(n * n));
+        for (std::size_t k = 0; k < steps; ++k)
+            s.step(dt, hzl::TimeIntegrator::ssprk3);
+        from_field(s.coefficients(), out);
+    });
+}
+
+// Solver::linearized_drift and Solver::adjoint_drift of a spectral state.
+int ref_linearized_drift(std::size_t n, double nu, double alpha, const double *state,
+                         const double *direction, double *out) {
+    return guarded([&] {
+        hzl::Solver s(n, frozen(nu, alpha, 1, 1), 0);
+        from_field(s.linearized_drift(to_field(state, n * n), to_field(direction, n * n)), out);
+    });
+}
+
+int ref_adjoint_drift(std::size_t n, double nu, double alpha, const double *state,
+                      const double *cotangent, double *out) {
+    return guarded([&] {
+        hzl::Solver s(n, frozen(nu, alpha, 1, 1), 0);
+        from_field(s.adjoint_drift(to_field(state, n * n), to_field(cotangent, n * n)), out);
+    });
+}
+
 // Physical-space initial condition through Solver::initialize_physical (one
 // forward FFT and projection, src/solver.cpp:162-173); returns the spectrum.
 int ref_initialize_physical(std::size_t n, const double *physical_real, double *out) {

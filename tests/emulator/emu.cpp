@@ -148,6 +148,38 @@ struct Sim {
     }
 
     void col(int mode, double dt, const cx<R> *e, long long es) {
+        StageArgs<R> a = args(dt, e, es);
+        col_any(mode, a);
+    }
+
+    void col_any(int mode, const StageArgs<R> &a) {
+        switch (mode) {
+        case 0:
+            return col_mode<0>(a);
+        case 1:
+            return col_mode<1>(a);
+        case 2:
+            return col_mode<2>(a);
+        case 3:
+            return col_mode<3>(a);
+        case 4:
+            return col_mode<4>(a);
+        case 5:
+            return col_mode<5>(a);
+        case 6:
+            return col_mode<6>(a);
+        case 7:
+            return col_mode<7>(a);
+        case 8:
+            return col_mode<8>(a);
+        case 9:
+            return col_mode<9>(a);
+        default:
+            return col_mode<10>(a);
+        }
+    }
+
+    StageArgs<R> args(double dt, const cx<R> *e, long long es) {
         StageArgs<R> a;
         a.B = B;
         a.w0 = w0.data();
@@ -164,17 +196,77 @@ struct Sim {
         a.h_half = (R)(dt / 2);
         a.h_full = (R)dt;
         a.h_sixth = (R)(dt / 6);
-        switch (mode) {
-        case 0:
-            return col_mode<0>(a);
-        case 1:
-            return col_mode<1>(a);
-        case 2:
-            return col_mode<2>(a);
-        case 3:
-            return col_mode<3>(a);
-        default:
-            return col_mode<4>(a);
+        return a;
+    }
+
+    // Row pass over an arbitrary mixed buffer into an arbitrary nonlinear field.
+    void row_on(cx<R> *mx, cx<R> *out) {
+        RowArgs<R> a{B, mx, out, tw_row.data(), nullptr};
+        const long blocks = ((long)B * (N / 2) + GR::ROW_PPW - 1) / GR::ROW_PPW;
+        launch(blocks, GR::ROW_THREADS, sizeof(cx<R>) * RowShape<GR>::template lds_elems<R>(),
+               [&](const CpuCtx &ctx, long b, unsigned char *lds) {
+                   row_body<N, R, false, GR>(ctx, a, reinterpret_cast<cx<R> *>(lds), b);
+               });
+    }
+
+    // J(x, y) into nl: two-state column pass (mode 8) and a row pass.
+    void jacobian(cx<R> *x, cx<R> *y, std::vector<cx<R>> &mx, std::vector<cx<R>> &out) {
+        StageArgs<R> a = args(1.0, nullptr, 0);
+        a.w0 = x;
+        a.w1 = y;
+        a.mix = mx.data();
+        col_any(8, a);
+        row_on(mx.data(), out.data());
+    }
+
+    // kind 0: drift of w0; 1: tangent along aux; 2: adjoint applied to aux.
+    void calibration(int kind, std::vector<cx<R>> &aux, std::vector<cx<R>> &out) {
+        std::vector<cx<R>> mA((size_t)B * 4 * G::MIXF), mB(mA.size()), nA((size_t)B * G::MIXF),
+            nB(nA.size()), nC(nA.size());
+        StageArgs<R> a = args(1.0, nullptr, 0);
+        a.out = out.data();
+        if (kind == 0) {
+            jacobian(w0.data(), w0.data(), mA, nA);
+            a.nl = nA.data();
+            a.w1 = w0.data();
+            col_any(9, a);
+        } else if (kind == 1) {
+            jacobian(aux.data(), w0.data(), mA, nA);
+            jacobian(w0.data(), aux.data(), mB, nB);
+            a.nl = nA.data();
+            a.nl2 = nB.data();
+            a.w1 = aux.data();
+            col_any(9, a);
+        } else {
+            jacobian(w0.data(), aux.data(), mA, nA);
+            StageArgs<R> m0 = args(1.0, nullptr, 0);
+            m0.mix = mB.data();
+            col_any(0, m0);
+            ProductArgs<R> pa{B, mA.data(), mB.data(), nB.data(), nC.data(), tw_row.data()};
+            const long blocks = ((long)B * (N / 2) + GR::ROW_PPW - 1) / GR::ROW_PPW;
+            launch(blocks, GR::ROW_THREADS,
+                   sizeof(cx<R>) * GR::ROW_PPW * GroupFFT<N, GR::E, R>::LDS_ELEMS,
+                   [&](const CpuCtx &ctx, long b, unsigned char *lds) {
+                       row_body_products<N, R, GR>(ctx, pa, reinterpret_cast<cx<R> *>(lds), b);
+                   });
+            a.nl = nA.data();
+            a.nl2 = nB.data();
+            a.nl3 = nC.data();
+            a.w1 = aux.data();
+            col_any(10, a);
+        }
+    }
+
+    // SSPRK(3,3) steps with device-style noise (modes 5, 6, 7).
+    void ssp_steps(double dt, int count) {
+        for (int s = 0; s < count; ++s) {
+            col(0, dt, nullptr, 0);
+            make_noise(dt);
+            for (int stage = 1; stage <= 3; ++stage) {
+                row(stage == 1);
+                col(4 + stage, dt, nullptr, 0);
+            }
+            ++step;
         }
     }
 
@@ -412,6 +504,32 @@ int proj_head_any(long M, int C, int H, const float *X, const float *Q1, const f
     return 0;
 }
 
+// Calibration drifts (kind 0 drift, 1 tangent along aux, 2 adjoint of aux) or
+// (kind 3) `steps` SSPRK(3,3) steps, of B members from reference Fields,
+// in the default launch shape.
+template <int N, typename R>
+int calib(int kind, int B, double nu, double alpha, double sigma, double amp, int kf, int kmin,
+          int kmax, uint64_t seed, double dt, int steps, const double *state, const double *aux,
+          double *out) {
+    Sim<N, R> s(B, nu, alpha, sigma, amp, kf, kmin, kmax, seed);
+    const size_t nn = (size_t)N * N;
+    std::vector<cx<R>> x((size_t)B * Geometry<N>::SPEC), o(x.size());
+    for (int m = 0; m < B; ++m) {
+        full_to_half<R>(N, state + 2 * nn * m, s.w0.data() + (size_t)m * Geometry<N>::SPEC);
+        if (aux)
+            full_to_half<R>(N, aux + 2 * nn * m, x.data() + (size_t)m * Geometry<N>::SPEC);
+    }
+    if (kind == 3) {
+        s.ssp_steps(dt, steps);
+        o = s.w0;
+    } else {
+        s.calibration(kind, x, o);
+    }
+    for (int m = 0; m < B; ++m)
+        half_to_full<R>(N, o.data() + (size_t)m * Geometry<N>::SPEC, out + 2 * nn * m);
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -467,6 +585,24 @@ int emu_run_variant(int n, int cv, int rv, int B, double nu, double alpha, doubl
         HZL_ROW_VARIANTS(HZL_X)
 #undef HZL_X
     }
+    return -1;
+}
+
+int emu_calibration(int n, int precision, int kind, int B, double nu, double alpha,
+                    double sigma, double amp, int kf, int kmin, int kmax, uint64_t seed,
+                    double dt, int steps, const double *state, const double *aux, double *out) {
+#define HZL_EMU_CAL(NN)                                                                        \
+    case NN:                                                                                   \
+        return precision == 0 ? calib<NN, float>(kind, B, nu, alpha, sigma, amp, kf, kmin,     \
+                                                 kmax, seed, dt, steps, state, aux, out)       \
+                              : calib<NN, double>(kind, B, nu, alpha, sigma, amp, kf, kmin,    \
+                                                  kmax, seed, dt, steps, state, aux, out);
+    switch (n) {
+        HZL_EMU_CAL(16)
+        HZL_EMU_CAL(32)
+        HZL_EMU_CAL(64)
+    }
+#undef HZL_EMU_CAL
     return -1;
 }
 

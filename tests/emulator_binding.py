@@ -33,6 +33,10 @@ class Emulator:
                                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_int, C.c_uint64, C.c_double, C.c_int, C.c_int, _dp,
                                       _dp, _dp]
+        L.emu_calibration.restype = C.c_int
+        L.emu_calibration.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_int, C.c_uint64, C.c_double, C.c_int, _dp, _dp, _dp]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
 
@@ -104,3 +108,20 @@ class Emulator:
                                       out.ctypes.data_as(_dp), diag.ctypes.data_as(_dp))
         assert rc == 0, "no such variant"
         return out, diag
+
+    def calibration(self, kind, state, aux=None, nu=0.01, alpha=0.1, precision="fp64", dt=0.0,
+                    steps=0, sigma=0.0, amp=0.0, kf=4, kmin=1, kmax=4, seed=0):
+        """kind "drift", "tangent" (along aux), "adjoint" (of aux) or "ssprk3"
+        (`steps` steps of dt) for members state[m] (reference Fields)."""
+        code = {"drift": 0, "tangent": 1, "adjoint": 2, "ssprk3": 3}[kind]
+        st = np.ascontiguousarray(state, dtype=np.complex128)
+        members, n = st.shape[0], st.shape[-1]
+        ax = None if aux is None else np.ascontiguousarray(aux, dtype=np.complex128)
+        out = np.zeros_like(st)
+        rc = self.lib.emu_calibration(n, 0 if precision == "fp32" else 1, code, members, nu,
+                                      alpha, sigma, amp, kf, kmin, kmax, seed, dt, steps,
+                                      st.ctypes.data_as(_dp),
+                                      None if ax is None else ax.ctypes.data_as(_dp),
+                                      out.ctypes.data_as(_dp))
+        assert rc == 0
+        return out

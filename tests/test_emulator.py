@@ -59,3 +59,20 @@ def test_launch_shape_variants_match_oracle(emu, orc, n, cv, rv):
     got, _ = emu.run_variant(n, 1, cv, rv, 1e-3, 2, seed=SEED)
     want = orc.trajectories(n, 1e-3, 0.0, 1e-3, 2, SEED, 0, 1, kmax=4)
     assert rel(got[0], want[0]) < 2e-7
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-14), ("fp32", 2e-6)])
+def test_calibration_drifts_and_ssprk3_match_oracle(emu, orc, precision, tol):
+    """Drift, its tangent and adjoint (two-state column pass, row pass,
+    adjoint products) and SSPRK(3,3) stages, from the kernel bodies."""
+    n, nu, alpha = 32, 0.01, 0.1
+    S = np.stack([orc.grf(n, 60 + i) for i in range(2)])
+    D = np.stack([orc.grf(n, 70 + i) for i in range(2)])
+    got = {k: emu.calibration(k, S, D if k in ("tangent", "adjoint") else None, nu=nu,
+                              alpha=alpha, precision=precision, dt=2e-3, steps=3)
+           for k in ("drift", "tangent", "adjoint", "ssprk3")}
+    for m in range(2):
+        assert rel(got["drift"][m], orc.rhs(S[m], nu, alpha)) < tol
+        assert rel(got["tangent"][m], orc.linearized_drift(S[m], D[m], nu, alpha)) < tol
+        assert rel(got["adjoint"][m], orc.adjoint_drift(S[m], D[m], nu, alpha)) < tol
+        assert rel(got["ssprk3"][m], orc.ssprk3(S[m], nu, alpha, 2e-3, 3)) < tol

@@ -339,3 +339,62 @@ def test_enkf_scalar_update_dense_closed_form(orc):
     same = np.repeat(X[:1], M, axis=0)
     np.testing.assert_array_equal(orc.enkf_scalar_update(same, np.full(M, h[0]), y, sd, eps),
                                   same)
+
+
+# ---- SSPRK(3,3), tangent and adjoint drifts (src/solver.cpp:235-246, 264-323)
+
+def test_ssprk3_single_shell_closed_form(orc):
+    """On a single-shell field the Jacobian vanishes, so each mode follows
+    w' = -lam w + F: any three-stage third-order RK maps w to a w + b F with
+    a = 1 - z + z^2/2 - z^3/6 (its stability polynomial, z = lam dt) and
+    b = (1 - a) / lam (the fixed point F / lam is preserved)."""
+    n, nu, alpha, dt, k0 = 32, 0.02, 0.1, 0.01, 5
+    w = np.zeros((n, n), dtype=np.complex128)  # index [ky][kx]
+    F = np.zeros_like(w)
+    w[4, 3], w[n - 4, n - 3] = 0.1 + 0.4j, 0.1 - 0.4j  # (kx, ky) = (3, 4), |k| = 5
+    F[3, 4], F[n - 3, n - 4] = 0.05j, -0.05j          # (kx, ky) = (4, 3), |k| = 5
+    lam = nu * k0 * k0 + alpha
+    z = lam * dt
+    a = 1 - z + z * z / 2 - z ** 3 / 6
+    b = (1 - a) / lam
+    want = w.copy()
+    for _ in range(7):
+        want = a * want + b * F
+    got = orc.ssprk3(w, nu, alpha, dt, 7, F)
+    assert rel(got, want) < 1e-14
+
+
+def test_ssprk3_matches_reference(orc, ref):
+    n = 32
+    w = orc.grf(n, 21)
+    F = np.zeros((n, n), dtype=np.complex128)
+    F[0, 3], F[0, n - 3] = 0.2 - 0.1j, 0.2 + 0.1j
+    got = orc.ssprk3(w, 0.01, 0.1, 2e-3, 6, F)
+    want = ref.run_ssprk3(w, 0.01, 0.1, 2e-3, 6, F, kmin=3, kmax=3)
+    assert rel(got, want) < 1e-14
+
+
+def test_linearized_drift_is_the_drift_derivative(orc):
+    """The drift is quadratic in the state, so the central difference
+    (f(s + d) - f(s - d)) / 2 equals f'(s) d exactly (up to rounding)."""
+    n, nu, alpha = 32, 0.01, 0.1
+    s, d = orc.grf(n, 31), orc.grf(n, 32)
+    fd = (orc.rhs(s + d, nu, alpha) - orc.rhs(s - d, nu, alpha)) / 2
+    assert rel(orc.linearized_drift(s, d, nu, alpha), fd) < 1e-13
+
+
+def test_adjoint_drift_satisfies_the_adjoint_identity(orc):
+    """Re <lam, L d> = Re <L* lam, d> for the tangent L at s (sum over all
+    coefficients); a sign, index or operand slip breaks it."""
+    n, nu, alpha = 32, 0.01, 0.1
+    s, d, lam = orc.grf(n, 41), orc.grf(n, 42), orc.grf(n, 43)
+    lhs = np.vdot(lam, orc.linearized_drift(s, d, nu, alpha)).real
+    rhs = np.vdot(orc.adjoint_drift(s, lam, nu, alpha), d).real
+    assert abs(lhs - rhs) < 1e-13 * abs(lhs)
+
+
+def test_tangent_and_adjoint_match_reference(orc, ref):
+    n, nu, alpha = 32, 0.01, 0.1
+    s, d, lam = orc.grf(n, 51), orc.grf(n, 52), orc.grf(n, 53)
+    assert rel(orc.linearized_drift(s, d, nu, alpha), ref.linearized_drift(s, d, nu, alpha)) < 1e-14
+    assert rel(orc.adjoint_drift(s, lam, nu, alpha), ref.adjoint_drift(s, lam, nu, alpha)) < 1e-14
