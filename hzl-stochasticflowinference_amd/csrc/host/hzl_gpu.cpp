@@ -265,6 +265,59 @@ void Ensemble::step(double dt, std::size_t steps) {
     check(hzl_ensemble_step(handle_, dt, steps));
 }
 
+void Ensemble::step(double dt, std::size_t steps, TimeIntegrator integrator) {
+    check(hzl_ensemble_step_integrator(handle_, dt, steps,
+                                       integrator == TimeIntegrator::rk4 ? HZL_RK4 : HZL_SSPRK3));
+}
+
+namespace {
+std::vector<double> pack_fields(const std::vector<Field> &fields, std::size_t count,
+                                std::size_t n) {
+    if (fields.size() != count)
+        throw std::invalid_argument("one field per member expected");
+    std::vector<double> flat;
+    flat.reserve(2 * n * n * count);
+    for (const auto &f : fields) {
+        if (f.size() != n * n)
+            throw std::invalid_argument("field size mismatch");
+        for (const auto &c : f) {
+            flat.push_back(c.real());
+            flat.push_back(c.imag());
+        }
+    }
+    return flat;
+}
+
+std::vector<Field> unpack_fields(const std::vector<double> &flat, std::size_t count,
+                                 std::size_t n) {
+    std::vector<Field> out(count, Field(n * n));
+    for (std::size_t m = 0; m < count; ++m)
+        for (std::size_t i = 0; i < n * n; ++i)
+            out[m][i] = Complex(flat[2 * (m * n * n + i)], flat[2 * (m * n * n + i) + 1]);
+    return out;
+}
+} // namespace
+
+std::vector<Field> Ensemble::drift() {
+    std::vector<double> out(2 * n_ * n_ * members_);
+    check(hzl_ensemble_drift(handle_, out.data()));
+    return unpack_fields(out, members_, n_);
+}
+
+std::vector<Field> Ensemble::linearized_drift(const std::vector<Field> &directions) {
+    const auto in = pack_fields(directions, members_, n_);
+    std::vector<double> out(in.size());
+    check(hzl_ensemble_linearized_drift(handle_, in.data(), out.data()));
+    return unpack_fields(out, members_, n_);
+}
+
+std::vector<Field> Ensemble::adjoint_drift(const std::vector<Field> &cotangents) {
+    const auto in = pack_fields(cotangents, members_, n_);
+    std::vector<double> out(in.size());
+    check(hzl_ensemble_adjoint_drift(handle_, in.data(), out.data()));
+    return unpack_fields(out, members_, n_);
+}
+
 void Ensemble::step(double dt, const Field &shared_midpoint_forcing) {
     if (shared_midpoint_forcing.size() != n_ * n_)
         throw std::invalid_argument("GPU forcing size mismatch");

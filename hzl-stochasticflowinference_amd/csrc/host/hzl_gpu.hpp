@@ -58,6 +58,8 @@ class Fourier2D {
 };
 [[nodiscard]] int wave_number(std::size_t index, std::size_t n);
 
+enum class TimeIntegrator { rk4, ssprk3 };
+
 struct Parameters {
     double viscosity = 0.01;
     double friction = 0.1;
@@ -111,7 +113,14 @@ class Ensemble {
     void set_all(const Field &coefficients); // every member
     [[nodiscard]] Field state(std::size_t member) const;
     void step(double dt, std::size_t steps = 1);
+    // Solver::step(dt, integrator) (src/solver.cpp:217-254) for every member
+    void step(double dt, std::size_t steps, TimeIntegrator integrator);
     void step(double dt, const Field &shared_midpoint_forcing);
+    // Solver::drift / linearized_drift / adjoint_drift (src/solver.cpp:255-323)
+    // of every member's state; one Field per member in and out
+    [[nodiscard]] std::vector<Field> drift();
+    [[nodiscard]] std::vector<Field> linearized_drift(const std::vector<Field> &directions);
+    [[nodiscard]] std::vector<Field> adjoint_drift(const std::vector<Field> &cotangents);
     void synchronize();
     [[nodiscard]] std::vector<Diagnostics> diagnostics();
     [[nodiscard]] std::size_t size() const noexcept {

@@ -9,6 +9,10 @@
 //   pf <csv> <out>                  bootstrap_filter(data, p, 8, 555)
 //   enkf <csv> <out>                ensemble_kalman_filter(data, p, 8, 555)
 //   errors                          invalid arguments raise std::invalid_argument
+//   calibration <n> <out>           Ensemble (FP64, 2 members from multimode and
+//                                   taylor-green): drift, linearized_drift along
+//                                   each other's state, adjoint_drift, then
+//                                   5 SSPRK3 steps of 1e-3
 //   checkpoint <in> <out>           Checkpoint::load then Checkpoint::save
 //   resume <in> <steps> <out>       CudaSolver from a checkpoint's vorticity, `steps`
 //                                   steps of 1e-3 holding its forcing
@@ -140,6 +144,28 @@ int main(int argc, char **argv) {
             append(out, c.forcing);
             append(out, s.coefficients());
             write(argv[4], out);
+        } else if (mode == "calibration" && argc == 4) {
+            const std::size_t n = std::stoul(argv[2]);
+            Parameters p;
+            p.viscosity = 1e-2;
+            p.friction = 0.1;
+            p.forcing_sigma = 0;
+            Ensemble e(n, 2, p, Precision::fp64, 7);
+            const Field a = initial_condition(n, "multimode"), b = initial_condition(n, "taylor-green");
+            e.set_state(0, a);
+            e.set_state(1, b);
+            std::vector<double> out;
+            for (const auto &f : e.drift())
+                append(out, f);
+            for (const auto &f : e.linearized_drift({b, a}))
+                append(out, f);
+            for (const auto &f : e.adjoint_drift({b, a}))
+                append(out, f);
+            e.step(1e-3, 5, TimeIntegrator::ssprk3);
+            e.synchronize();
+            append(out, e.state(0));
+            append(out, e.state(1));
+            write(argv[3], out);
         } else if (mode == "errors") {
             Parameters p;
             const bool ok =

@@ -209,3 +209,24 @@ def test_cudasolver_resumes_from_checkpoint(tmp_path, orc):
     w0, F, got = (field(raw[k * 2 * n * n:(k + 1) * 2 * n * n], n) for k in range(3))
     want = orc.rk4(w0, 0.02, 0.1, 1e-3, 50, forcing=F)
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+
+
+@pytest.mark.gpu
+def test_reference_signature_calibration_matches_oracle(tmp_path, orc):
+    """hzl::Ensemble::drift / linearized_drift / adjoint_drift /
+    step(dt, steps, TimeIntegrator::ssprk3) through the C++ drop-in layer."""
+    n = 32
+    run("calibration", n, tmp_path / "c.bin")
+    raw = np.fromfile(tmp_path / "c.bin")
+    f = [field(raw[k * 2 * n * n:(k + 1) * 2 * n * n], n) for k in range(8)]
+    run("initial", n, "multimode", tmp_path / "a.bin")
+    run("initial", n, "taylor-green", tmp_path / "b.bin")
+    a, b = field(np.fromfile(tmp_path / "a.bin"), n), field(np.fromfile(tmp_path / "b.bin"), n)
+    nu, al = 1e-2, 0.1
+    want = [orc.rhs(a, nu, al), orc.rhs(b, nu, al),
+            orc.linearized_drift(a, b, nu, al), orc.linearized_drift(b, a, nu, al),
+            orc.adjoint_drift(a, b, nu, al), orc.adjoint_drift(b, a, nu, al),
+            orc.ssprk3(a, nu, al, 1e-3, 5), orc.ssprk3(b, nu, al, 1e-3, 5)]
+    for g, w in zip(f, want):
+        scale = max(np.linalg.norm(w), 1e-300)
+        assert np.linalg.norm(g - w) / scale < 1e-12
