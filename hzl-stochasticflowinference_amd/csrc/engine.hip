@@ -115,6 +115,95 @@ __global__ void __launch_bounds__(256) k_pf_commit(int B, long spec, const cx<R>
         dst[i] = src[i];
 }
 
+// ----------------------------------------------- adjoint likelihood ----
+// y = a x + b z over `count` complex entries (z may be null; y may alias x or z).
+template <typename R>
+__global__ void __launch_bounds__(256) k_axpby(long count, cx<R> *y, R a, const cx<R> *x, R b,
+                                               const cx<R> *z) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count)
+        return;
+    const cx<R> xv = x[i];
+    cx<R> r = {a * xv.x, a * xv.y};
+    if (z) {
+        const cx<R> zv = z[i];
+        r = {r.x + b * zv.x, r.y + b * zv.y};
+    }
+    y[i] = r;
+}
+
+// pred[m * nobs + o]: member m's physical vorticity at observation o, from
+// the mixed w~ field (one thread per member and observation).
+template <typename R> __global__ void __launch_bounds__(256) k_point_values(PfArgs<R> a) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)a.B * a.nobs)
+        return;
+    const long m = i / a.nobs;
+    const int o = (int)(i % a.nobs);
+    const cx<R> *w = a.mix + m * 4 * (long)(a.K + 1) * a.N + 2 * (long)(a.K + 1) * a.N;
+    a.pred[i] = point_value(a, w, a.ox[o], a.oy[o]);
+}
+
+// cot[m] += sum_o weight[m][o] exp(-2 pi i (kx x_o + ky y_o) / n) on the
+// retained half plane: the forward transform of point residuals injected at
+// the observation sites (src/adjoint.cpp:96-114).
+template <typename R>
+__global__ void __launch_bounds__(256) k_inject(int B, int n, int K, cx<R> *cot, int nobs,
+                                                const int *ox, const int *oy, const double *weight,
+                                                const double *cs, const double *sn) {
+    const int J = 2 * K + 1;
+    const long spec = (long)(K + 1) * J;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)B * spec)
+        return;
+    const long m = i / spec;
+    const int c = (int)((i % spec) / J), ky = (int)(i % J) - K;
+    if (c == 0 && ky == 0)
+        return;
+    double re = 0, im = 0;
+    for (int o = 0; o < nobs; ++o) {
+        const long ph = (((long)c * ox[o] + (long)ky * oy[o]) % n + n) % n;
+        const double wv = weight[m * nobs + o];
+        re += wv * cs[ph];
+        im -= wv * sn[ph];
+    }
+    cot[i] = {cot[i].x + (R)re, cot[i].y + (R)im};
+}
+
+// Parameter terms of the adjoint gradient (src/adjoint.cpp:115-123): with
+// inner = Re(conj(W) S) summed over the full grid (kx > 0 entries twice),
+// grad[2m] -= nu_m sum |k|^2 inner, grad[2m+1] -= alpha_m sum inner.
+template <typename R>
+__global__ void __launch_bounds__(256) k_param_terms(int K, const cx<R> *S, const cx<R> *W,
+                                                     const double *nu, const double *alpha,
+                                                     double *grad) {
+    __shared__ double red[2 * 256];
+    const int J = 2 * K + 1;
+    const long spec = (long)(K + 1) * J, m = blockIdx.x;
+    double a = 0, b = 0;
+    for (long e = threadIdx.x; e < spec; e += 256) {
+        const int c = (int)(e / J), ky = (int)(e % J) - K;
+        const cx<R> s = S[m * spec + e], w = W[m * spec + e];
+        const double inner = (c > 0 ? 2.0 : 1.0) * ((double)w.x * s.x + (double)w.y * s.y);
+        a += (double)(c * c + ky * ky) * inner;
+        b += inner;
+    }
+    red[threadIdx.x] = a;
+    red[256 + threadIdx.x] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h /= 2) {
+        if ((int)threadIdx.x < h) {
+            red[threadIdx.x] += red[threadIdx.x + h];
+            red[256 + threadIdx.x] += red[256 + threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        grad[2 * m] -= nu[m] * red[0];
+        grad[2 * m + 1] -= alpha[m] * red[256];
+    }
+}
+
 // Device counters read by replayed graphs: [0] step index, [1] filter group,
 // [2] last-group flag.
 __global__ void k_set_counters(uint64_t *c, uint64_t step, uint64_t group, uint64_t is_last) {
@@ -571,12 +660,7 @@ template <typename R> class Engine final : public EngineBase {
     void drift(double *out) override {
         flush_parameters();
         calib_buffers();
-        jacobian(w0_.p, w0_.p, calA_.p, nlA_.p); // J(w, w)
-        StageArgs<R> a = stage_args(1.0, nullptr, 0);
-        a.nl = nlA_.p;
-        a.w1 = w0_.p;
-        a.out = cout_.p;
-        run_col(9, a);
+        drift_into(w0_.p, cout_.p);
         download_fields(cout_.p, out);
     }
 
@@ -606,20 +690,219 @@ template <typename R> class Engine final : public EngineBase {
         flush_parameters();
         calib_buffers();
         upload_fields(cotangents, caux_.p);
-        jacobian(w0_.p, caux_.p, calA_.p, nlA_.p); // J(w, lam); calA holds lam~
-        StageArgs<R> m0 = stage_args(1.0, nullptr, 0);
-        m0.mix = calB_.p; // mixed fields of the state: w~, (ky w)~
-        run_col(0, m0);
-        ProductArgs<R> pa{(int)B_, calA_.p, calB_.p, nlB_.p, nlC_.p, tw_.p};
-        dispatch_products<R>(n_, pa, stream_);
-        StageArgs<R> a = stage_args(1.0, nullptr, 0);
-        a.nl = nlA_.p;
-        a.nl2 = nlB_.p;
-        a.nl3 = nlC_.p;
-        a.w1 = caux_.p;
-        a.out = cout_.p;
-        run_col(10, a);
+        adjoint_into(w0_.p, caux_.p, cout_.p);
         download_fields(cout_.p, out);
+    }
+
+    // likelihood_adjoint (src/adjoint.cpp:43-161) with member m's parameters
+    // (log nu, log alpha) = log_params[2m, 2m+1]: forward RK4 from `initial`
+    // without forcing, checkpoints every `stride` steps, Gaussian residuals at
+    // the observations; then per checkpoint block (last first) a replay of
+    // its states and the reverse sweep of each step's RK4 stages through the
+    // batched adjoint drift.  counts (optional): checkpoint fields, peak
+    // block fields, replayed steps (the reference's AdjointEvaluation).
+    void likelihood_adjoint(const double *initial, double dt, size_t nobs, const size_t *ostep,
+                            const int *ox, const int *oy, const double *oval, const double *osd,
+                            const double *log_params, size_t stride, double *objective,
+                            double *gradient, size_t *counts) override {
+        if (nobs == 0 || stride == 0 || !std::isfinite(dt) || dt <= 0)
+            throw std::invalid_argument("adjoint requires observations, dt > 0 and a positive "
+                                        "checkpoint stride");
+        for (size_t o = 0; o < nobs; ++o)
+            if ((o > 0 && ostep[o] < ostep[o - 1]) || ox[o] < 0 || oy[o] < 0 || ox[o] >= n_ ||
+                oy[o] >= n_ || !std::isfinite(oval[o]) || !std::isfinite(osd[o]) || osd[o] <= 0)
+                throw std::invalid_argument("observations must be time ordered, finite, in "
+                                            "bounds, and have positive uncertainty");
+        // members outside the reference's domain get +inf and harmless parameters
+        std::vector<char> valid(B_, 1);
+        const double largest = (double)((n_ - 1) / 3);
+        for (size_t m = 0; m < B_; ++m) {
+            const double q0 = log_params[2 * m], q1 = log_params[2 * m + 1];
+            valid[m] = std::isfinite(q0) && std::isfinite(q1) && q0 >= -12 && q0 <= 0 &&
+                       q1 >= -12 && q1 <= 0;
+            if (valid[m] &&
+                dt * (2 * std::exp(q0) * largest * largest + std::exp(q1)) > 2.5)
+                valid[m] = 0;
+            nu_[m] = valid[m] ? std::exp(q0) : 0.0;
+            alpha_[m] = valid[m] ? std::exp(q1) : 0.0;
+            sigma_[m] = 0;
+            amp_[m] = 0;
+        }
+        push_parameters();
+        fill_state(initial);
+        calib_buffers();
+        if (pf_cs_.count != (size_t)n_) {
+            check(hipStreamSynchronize(stream_), "table growth");
+            clear_graphs();
+            std::vector<double> cs(n_), sn(n_);
+            for (int j = 0; j < n_; ++j) {
+                cs[j] = std::cos(2 * PI * j / n_);
+                sn[j] = std::sin(2 * PI * j / n_);
+            }
+            pf_cs_ = DeviceBuffer<double>(n_);
+            pf_sn_ = DeviceBuffer<double>(n_);
+            upload(pf_cs_.p, cs.data(), n_);
+            upload(pf_sn_.p, sn.data(), n_);
+        }
+        const size_t last = ostep[nobs - 1];
+        std::vector<size_t> cstep{0};
+        for (size_t step = 1; step <= last; ++step)
+            if (step % stride == 0 || step == last)
+                cstep.push_back(step);
+        const size_t nck = cstep.size(), SB = B_ * spec_;
+        DeviceBuffer<cx<R>> ck(nck * SB);
+        auto copy = [&](cx<R> *dst, const cx<R> *src) {
+            check(hipMemcpyAsync(dst, src, SB * sizeof(cx<R>), hipMemcpyDeviceToDevice, stream_),
+                  "state copy");
+        };
+        auto axpby = [&](cx<R> *y, double a, const cx<R> *x, double b, const cx<R> *z) {
+            hipLaunchKernelGGL(k_axpby<R>, dim3((unsigned)((SB + 255) / 256)), dim3(256), 0,
+                               stream_, (long)SB, y, (R)a, x, (R)b, z);
+        };
+        // observation groups: [first, end) rows sharing a step
+        std::vector<size_t> gfirst;
+        for (size_t o = 0; o < nobs; ++o)
+            if (o == 0 || ostep[o] != ostep[o - 1])
+                gfirst.push_back(o);
+        gfirst.push_back(nobs);
+        DeviceBuffer<int> d_ox(nobs), d_oy(nobs);
+        upload(d_ox.p, ox, nobs);
+        upload(d_oy.p, oy, nobs);
+        std::vector<double> weight(B_ * nobs, 0.0), value(B_, 0.0);
+        DeviceBuffer<double> d_pred(B_ * nobs), d_weight(B_ * nobs);
+        // -- forward
+        copy(ck.p, w0_.p);
+        size_t g = 0, ci = 1;
+        for (size_t step = 0; step <= last; ++step) {
+            if (step > 0) {
+                advance(dt, nullptr, 0);
+                if (ci < nck && cstep[ci] == step)
+                    copy(ck.p + (ci++) * SB, w0_.p);
+            }
+            if (g + 1 < gfirst.size() && ostep[gfirst[g]] == step) {
+                const size_t o0 = gfirst[g], cnt = gfirst[g + 1] - o0;
+                ensure_mixed(dt);
+                PfArgs<R> pa{(int)B_, n_, K_, mix_.p, (int)cnt, d_ox.p + o0, d_oy.p + o0,
+                             nullptr, nullptr, pf_cs_.p, pf_sn_.p, nullptr, nullptr, nullptr,
+                             d_pred.p};
+                const long total = (long)B_ * (long)cnt;
+                hipLaunchKernelGGL(k_point_values<R>, dim3((unsigned)((total + 255) / 256)),
+                                   dim3(256), 0, stream_, pa);
+                std::vector<double> pred((size_t)total);
+                check(hipMemcpyAsync(pred.data(), d_pred.p, pred.size() * sizeof(double),
+                                     hipMemcpyDeviceToHost, stream_),
+                      "predictions");
+                check(hipStreamSynchronize(stream_), "predictions");
+                for (size_t m = 0; m < B_; ++m)
+                    for (size_t k = 0; k < cnt; ++k) {
+                        const size_t o = o0 + k;
+                        const double r = (pred[m * cnt + k] - oval[o]) / osd[o];
+                        value[m] += 0.5 * r * r;
+                        weight[m * nobs + o] = r / osd[o];
+                    }
+                ++g;
+            }
+        }
+        upload(d_weight.p, weight.data(), weight.size());
+        // -- reverse sweep
+        std::vector<DeviceBuffer<cx<R>>> wk;
+        for (int k = 0; k < 11; ++k)
+            wk.emplace_back(SB);
+        cx<R> *S2 = wk[0].p, *S3 = wk[1].p, *S4 = wk[2].p, *kb = wk[3].p, *cot = wk[4].p,
+              *prev = wk[5].p, *K1 = wk[6].p, *K2 = wk[7].p, *K3 = wk[8].p, *K4 = wk[9].p,
+              *bb = wk[10].p;
+        axpby(cot, 0.0, w0_.p, 0.0, nullptr); // zero
+        DeviceBuffer<double> d_grad(2 * B_);
+        {
+            std::vector<double> z(2 * B_, 0.0);
+            upload(d_grad.p, z.data(), z.size());
+        }
+        size_t peak = 0, replayed = 0, obs_left = nobs;
+        auto inject = [&](size_t step) {
+            size_t hi = obs_left, lo = obs_left;
+            while (lo > 0 && ostep[lo - 1] == step)
+                --lo;
+            if (lo == hi)
+                return;
+            // weights of rows [lo, hi): pass the member-major weights of those rows
+            std::vector<double> w(B_ * (hi - lo));
+            for (size_t m = 0; m < B_; ++m)
+                for (size_t o = lo; o < hi; ++o)
+                    w[m * (hi - lo) + (o - lo)] = weight[m * nobs + o];
+            upload(d_pred.p, w.data(), w.size());
+            hipLaunchKernelGGL(k_inject<R>, dim3((unsigned)((SB + 255) / 256)), dim3(256), 0,
+                               stream_, (int)B_, n_, K_, cot, (int)(hi - lo), d_ox.p + lo,
+                               d_oy.p + lo, (const double *)d_pred.p, (const double *)pf_cs_.p,
+                               (const double *)pf_sn_.p);
+            obs_left = lo;
+        };
+        auto terms = [&](const cx<R> *S, const cx<R> *W) {
+            hipLaunchKernelGGL(k_param_terms<R>, dim3((unsigned)B_), dim3(256), 0, stream_, K_, S,
+                               W, (const double *)d_nu_.p, (const double *)d_alpha_.p, d_grad.p);
+        };
+        for (size_t b = nck - 1; b >= 1; --b) {
+            const size_t begin = cstep[b - 1], end = cstep[b], len = end - begin;
+            DeviceBuffer<cx<R>> states((len + 1) * SB);
+            copy(states.p, ck.p + (b - 1) * SB);
+            for (size_t k = 0; k < len; ++k) {
+                copy(w0_.p, states.p + k * SB);
+                mix_valid_ = false;
+                advance(dt, nullptr, 0);
+                copy(states.p + (k + 1) * SB, w0_.p);
+                ++replayed;
+            }
+            peak = std::max(peak, len + 1);
+            for (size_t step = end; step > begin; --step) {
+                inject(step);
+                const cx<R> *S1 = states.p + (step - begin - 1) * SB;
+                drift_into(S1, kb);
+                axpby(S2, 1.0, S1, dt / 2, kb);
+                drift_into(S2, kb);
+                axpby(S3, 1.0, S1, dt / 2, kb);
+                drift_into(S3, kb);
+                axpby(S4, 1.0, S1, dt, kb);
+                ++replayed;
+                copy(prev, cot);
+                axpby(K1, dt / 6, cot, 0.0, nullptr);
+                axpby(K2, dt / 3, cot, 0.0, nullptr);
+                axpby(K3, dt / 3, cot, 0.0, nullptr);
+                axpby(K4, dt / 6, cot, 0.0, nullptr);
+                terms(S4, K4);
+                adjoint_into(S4, K4, bb);
+                axpby(prev, 1.0, prev, 1.0, bb);
+                axpby(K3, 1.0, K3, dt, bb);
+                terms(S3, K3);
+                adjoint_into(S3, K3, bb);
+                axpby(prev, 1.0, prev, 1.0, bb);
+                axpby(K2, 1.0, K2, dt / 2, bb);
+                terms(S2, K2);
+                adjoint_into(S2, K2, bb);
+                axpby(prev, 1.0, prev, 1.0, bb);
+                axpby(K1, 1.0, K1, dt / 2, bb);
+                terms(S1, K1);
+                adjoint_into(S1, K1, bb);
+                axpby(prev, 1.0, prev, 1.0, bb);
+                std::swap(cot, prev);
+            }
+            check(hipStreamSynchronize(stream_), "adjoint block");
+        }
+        check(hipGetLastError(), "adjoint kernels");
+        std::vector<double> grad(2 * B_);
+        check(hipMemcpy(grad.data(), d_grad.p, grad.size() * sizeof(double),
+                        hipMemcpyDeviceToHost),
+              "gradient download");
+        for (size_t m = 0; m < B_; ++m) {
+            objective[m] = valid[m] ? value[m] : std::numeric_limits<double>::infinity();
+            gradient[2 * m] = valid[m] ? grad[2 * m] : 0.0;
+            gradient[2 * m + 1] = valid[m] ? grad[2 * m + 1] : 0.0;
+        }
+        if (counts) {
+            counts[0] = nck;
+            counts[1] = peak;
+            counts[2] = replayed;
+        }
+        mix_valid_ = false;
+        params_dirty_ = false;
     }
 
     // ------------------------------------------------ launch shapes ----
@@ -1175,6 +1458,34 @@ template <typename R> class Engine final : public EngineBase {
         nlC_ = DeviceBuffer<cx<R>>(B_ * mixf_);
     }
 
+    // Drift of the states S into out (retained half planes, B members).
+    void drift_into(const cx<R> *S, cx<R> *out) {
+        jacobian(S, S, calA_.p, nlA_.p); // J(S, S)
+        StageArgs<R> a = stage_args(1.0, nullptr, 0);
+        a.nl = nlA_.p;
+        a.w1 = const_cast<cx<R> *>(S);
+        a.out = out;
+        run_col(9, a);
+    }
+
+    // Adjoint drift at states S applied to cotangents W, into out.
+    void adjoint_into(const cx<R> *S, const cx<R> *W, cx<R> *out) {
+        jacobian(S, W, calA_.p, nlA_.p); // J(S, W); calA holds W~
+        StageArgs<R> m0 = stage_args(1.0, nullptr, 0);
+        m0.w0 = const_cast<cx<R> *>(S);
+        m0.mix = calB_.p; // mixed fields of the state: w~, (ky w)~
+        run_col(0, m0);
+        ProductArgs<R> pa{(int)B_, calA_.p, calB_.p, nlB_.p, nlC_.p, tw_.p};
+        dispatch_products<R>(n_, pa, stream_);
+        StageArgs<R> a = stage_args(1.0, nullptr, 0);
+        a.nl = nlA_.p;
+        a.nl2 = nlB_.p;
+        a.nl3 = nlC_.p;
+        a.w1 = const_cast<cx<R> *>(W);
+        a.out = out;
+        run_col(10, a);
+    }
+
     // J(a, b) = u(a) . grad(b) of every member into the nonlinear mixed field
     // nl: two-state column pass (psi of a, w of b) into mix, then a row pass.
     void jacobian(const cx<R> *a_state, const cx<R> *b_state, cx<R> *mix, cx<R> *nl) {
@@ -1580,6 +1891,19 @@ int hzl_ensemble_adjoint_drift(hzl_ensemble_t h, const double *cotangents, doubl
         if (!cotangents || !out)
             throw std::invalid_argument("null argument");
         engine_of(h).adjoint_drift(cotangents, out);
+    });
+}
+
+int hzl_likelihood_adjoint(hzl_ensemble_t h, const double *initial_field, double dt,
+                           size_t nobs, const size_t *steps, const int *x, const int *y,
+                           const double *value, const double *sd, const double *log_params,
+                           size_t stride, double *objective, double *gradient, size_t *counts) {
+    return guarded([&] {
+        if (!initial_field || !steps || !x || !y || !value || !sd || !log_params || !objective ||
+            !gradient)
+            throw std::invalid_argument("null argument");
+        engine_of(h).likelihood_adjoint(initial_field, dt, nobs, steps, x, y, value, sd,
+                                        log_params, stride, objective, gradient, counts);
     });
 }
 

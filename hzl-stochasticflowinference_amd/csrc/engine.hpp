@@ -79,6 +79,11 @@ class EngineBase {
     virtual void drift(double *out) = 0;
     virtual void linearized_drift(const double *directions, double *out) = 0;
     virtual void adjoint_drift(const double *cotangents, double *out) = 0;
+    virtual void likelihood_adjoint(const double *initial, double dt, size_t nobs,
+                                    const size_t *ostep, const int *ox, const int *oy,
+                                    const double *oval, const double *osd,
+                                    const double *log_params, size_t stride, double *objective,
+                                    double *gradient, size_t *counts) = 0;
     // launch-shape variants of the RK4 kernels (variant 0: the default)
     virtual int col_variants() const = 0;
     virtual int row_variants() const = 0;

@@ -27,7 +27,7 @@ EXPORTS = (
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
     "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
     "hzl_ensemble_step_integrator", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
-    "hzl_ensemble_adjoint_drift",
+    "hzl_ensemble_adjoint_drift", "hzl_likelihood_adjoint",
     "hzl_ensemble_variant_count", "hzl_ensemble_set_tuning", "hzl_ensemble_get_tuning",
     "hzl_ensemble_autotune",
     "hzl_ensemble_step_forced",
@@ -97,12 +97,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_get_state.argtypes = [vp, C.c_size_t, _dp]
     L.hzl_ensemble_init_grf.argtypes = [vp]
     L.hzl_ensemble_step.argtypes = [vp, C.c_double, C.c_size_t]
+    ipp = C.POINTER(C.c_int)
     L.hzl_ensemble_set_graph.argtypes = [vp, C.c_int]
     L.hzl_ensemble_step_integrator.argtypes = [vp, C.c_double, C.c_size_t, C.c_int]
     L.hzl_ensemble_drift.argtypes = [vp, _dp]
     L.hzl_ensemble_linearized_drift.argtypes = [vp, _dp, _dp]
     L.hzl_ensemble_adjoint_drift.argtypes = [vp, _dp, _dp]
-    ipp = C.POINTER(C.c_int)
+    szp = C.POINTER(C.c_size_t)
+    L.hzl_likelihood_adjoint.argtypes = [vp, _dp, C.c_double, C.c_size_t, szp, ipp, ipp, _dp, _dp,
+                                         _dp, C.c_size_t, _dp, _dp, szp]
     L.hzl_ensemble_variant_count.argtypes = [vp, ipp, ipp]
     L.hzl_ensemble_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.hzl_ensemble_get_tuning.argtypes = [vp, ipp, ipp]
@@ -563,6 +566,37 @@ def save_checkpoint(path: str, c: dict) -> None:
     out += [f"{g(a.real)} {g(a.imag)} {g(b.real)} {g(b.imag)}" for a, b in zip(w, F)]
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
+
+
+def likelihood_adjoint(ensemble: Ensemble, initial, observations, dt: float, log_params,
+                       stride: int = 20):
+    """Batched likelihood_adjoint (src/adjoint.cpp:43-161): member m evaluates
+    the deterministic negative log-likelihood of `observations` (rows of
+    step, x, y, value, sd) and its gradient at log_params[m] = (log nu,
+    log alpha), from `initial` (one spectral field).  Returns (values
+    [members], gradients [members, 2], counts (checkpoint fields, peak block
+    fields, replayed steps))."""
+    obs = np.asarray(observations, dtype=np.float64)
+    if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
+        raise ValueError("observations must be rows of (step, x, y, value, sd)")
+    q = np.ascontiguousarray(log_params, dtype=np.float64).reshape(ensemble.members, 2)
+    init = _fields(initial, 1, ensemble.n)
+    steps = np.ascontiguousarray(obs[:, 0], dtype=np.uint64).astype(np.uintp)
+    x = np.ascontiguousarray(obs[:, 1], dtype=np.int32)
+    y = np.ascontiguousarray(obs[:, 2], dtype=np.int32)
+    v = np.ascontiguousarray(obs[:, 3])
+    sd = np.ascontiguousarray(obs[:, 4])
+    values = np.zeros(ensemble.members)
+    grads = np.zeros((ensemble.members, 2))
+    counts = np.zeros(3, dtype=np.uintp)
+    ip, szp = C.POINTER(C.c_int), C.POINTER(C.c_size_t)
+    _check(_lib.hzl_likelihood_adjoint(ensemble._h, init.ctypes.data_as(_dp), dt, len(obs),
+                                       steps.ctypes.data_as(szp), x.ctypes.data_as(ip),
+                                       y.ctypes.data_as(ip), v.ctypes.data_as(_dp),
+                                       sd.ctypes.data_as(_dp), q.ctypes.data_as(_dp), stride,
+                                       values.ctypes.data_as(_dp), grads.ctypes.data_as(_dp),
+                                       counts.ctypes.data_as(szp)))
+    return values, grads, tuple(int(c) for c in counts)
 
 
 def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,

@@ -133,6 +133,25 @@ int hzl_ensemble_drift(hzl_ensemble_t h, double *out);
 int hzl_ensemble_linearized_drift(hzl_ensemble_t h, const double *directions, double *out);
 int hzl_ensemble_adjoint_drift(hzl_ensemble_t h, const double *cotangents, double *out);
 
+/* Deterministic negative log-likelihood and its gradient with respect to
+ * (log viscosity, log friction) by the discrete adjoint of RK4 with state
+ * checkpoints every `stride` steps, for every member's own parameter vector
+ * at once: member m runs nu = exp(log_params[2m]), alpha =
+ * exp(log_params[2m + 1]) from initial_field (one reference Field) with no
+ * forcing; observations are nobs rows (steps ascending, grid x, y, values,
+ * standard deviations), as Dataset::observations.  objective[m] receives
+ * sum 0.5 ((w(x, y) - value) / sd)^2, gradient[2m..2m+1] its gradient;
+ * objective[m] = +inf and gradient 0 where the reference returns +inf
+ * (parameters outside [-12, 0], or dt beyond the explicit diffusion limit).
+ * counts (optional, 3): checkpoint fields, peak block fields, replayed steps.
+ * The members' states, parameters and step counter are overwritten.
+ * Replaces: likelihood_adjoint (src/adjoint.cpp:43-161), batched over
+ * parameter vectors. */
+int hzl_likelihood_adjoint(hzl_ensemble_t h, const double *initial_field, double dt,
+                           size_t nobs, const size_t *steps, const int *x, const int *y,
+                           const double *value, const double *sd, const double *log_params,
+                           size_t stride, double *objective, double *gradient, size_t *counts);
+
 /* Replay launch sequences from captured HIP graphs (default on).  A
  * hzl_ensemble_step of 2..200 steps (longer calls in chunks of 200), and a
  * whole particle-filter cycle of hzl_pf_assimilate with 1..200 steps

@@ -474,6 +474,18 @@ class Reference:
         self._check(self.lib.ref_negative_log_likelihood(h, log_nu, log_alpha, _ptr(out)))
         return float(out[0])
 
+    def likelihood_adjoint(self, h, log_nu, log_alpha, stride):
+        """hzl::likelihood_adjoint: dict(value, gradient, checkpoint_fields,
+        peak_block_fields, replayed_steps)."""
+        L = self.lib
+        L.ref_likelihood_adjoint.restype = C.c_int
+        L.ref_likelihood_adjoint.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_size_t, _dp]
+        out = np.zeros(6)
+        self._check(L.ref_likelihood_adjoint(h, log_nu, log_alpha, stride, _ptr(out)))
+        return dict(value=float(out[0]), gradient=out[1:3].copy(),
+                    checkpoint_fields=int(out[3]), peak_block_fields=int(out[4]),
+                    replayed_steps=int(out[5]))
+
     def dataset_save(self, h, path):
         self._check(self.lib.ref_dataset_save(h, str(path).encode()))
 

@@ -354,6 +354,22 @@ int ref_negative_log_likelihood(void *d, double log_nu, double log_alpha, double
     });
 }
 
+// likelihood_adjoint (src/adjoint.cpp:43-161): out = value, gradient[0..1],
+// checkpoint_fields, peak_block_fields, replayed_steps.
+int ref_likelihood_adjoint(void *d, double log_nu, double log_alpha, std::size_t stride,
+                           double *out) {
+    return guarded([&] {
+        const auto r = hzl::likelihood_adjoint(static_cast<RefDataset *>(d)->data,
+                                               {log_nu, log_alpha}, stride);
+        out[0] = r.objective.value;
+        out[1] = r.objective.gradient[0];
+        out[2] = r.objective.gradient[1];
+        out[3] = (double)r.checkpoint_fields;
+        out[4] = (double)r.peak_block_fields;
+        out[5] = (double)r.replayed_steps;
+    });
+}
+
 // ensemble_kalman_filter (src/filter.cpp:204-356): per record (step,
 // forecast_sensor_rmse, posterior_spread) and the mean physical field.
 int ref_enkf(void *d, double nu, double alpha, double sigma, double tau, std::size_t members,

@@ -24,7 +24,10 @@ Fixtures
                    reference's Solver::load fields of it and the JSON sidecar its
                    diagnostics.
 
-Usage: python tests/golden/make_golden.py [config0 drift32 pf16 checkpoint16]
+  adjoint_16.json  the reference's likelihood_adjoint (value, gradient) on the
+                   pf_deterministic_16 dataset at four parameter vectors, strides 7, 4.
+
+Usage: python tests/golden/make_golden.py [config0 drift32 pf16 checkpoint16 adjoint16]
 """
 from __future__ import annotations
 
@@ -128,9 +131,34 @@ def checkpoint16(ref: Reference) -> None:
          vorticity=w, forcing=F)
 
 
+def adjoint16(ref: Reference) -> None:
+    """The reference's likelihood_adjoint on the deterministic 16x16 dataset
+    of pf_deterministic_16 at a few (log nu, log alpha), two strides."""
+    h, rows = ref.synthetic(16, 0.01, 30, 10, 0.03, 0.1, 0.0, 0.5, 2, 3, 0.2, 987)
+    qs = [(float(np.log(0.03)), float(np.log(0.1))), (float(np.log(0.02)), float(np.log(0.15))),
+          (float(np.log(0.05)), float(np.log(0.05))), (-13.0, -1.0)]
+    out = []
+    for stride in (7, 4):
+        for q in qs:
+            r = ref.likelihood_adjoint(h, q[0], q[1], stride)
+            out.append(dict(q=list(q), stride=stride, value=r["value"],
+                            gradient=[float(x) for x in r["gradient"]],
+                            checkpoint_fields=r["checkpoint_fields"],
+                            replayed_steps=r["replayed_steps"]))
+    ref.dataset_free(h)
+    meta = dict(recipe="likelihood_adjoint(synthetic_observations(16, 0.01, 30, 10, p, 0.2, "
+                       "987), q, stride) with nu 0.03, alpha 0.1, sigma 0; initial state "
+                       "initialize('multimode'); observations = pf_deterministic_16 rows",
+                n=16, dt=0.01, evaluations=out)
+    with open(os.path.join(HERE, "adjoint_16.json"), "w") as f:
+        json.dump(meta, f, indent=2, sort_keys=True)
+        f.write("\n")
+    print("wrote adjoint_16.json")
+
+
 def main() -> None:
     orc, ref = Oracle(), Reference()
-    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16"]
+    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16", "adjoint16"]
     if "config0" in which:
         config0(orc, ref)
     if "drift32" in which:
@@ -139,6 +167,8 @@ def main() -> None:
         pf16(ref)
     if "checkpoint16" in which:
         checkpoint16(ref)
+    if "adjoint16" in which:
+        adjoint16(ref)
 
 
 if __name__ == "__main__":

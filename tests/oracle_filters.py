@@ -254,3 +254,97 @@ def oracle_joint_filter(orc, initial, obs, dt, seed, nu, alpha=0.0, amp=0.1, kf=
                             resampled=resampled))
         current = int(target)
     return dict(log_likelihood=total, resampling_events=events, history=history, states=states)
+
+
+def oracle_likelihood_adjoint(orc, initial, obs, dt, q, stride):
+    """likelihood_adjoint (src/adjoint.cpp:43-161) from the oracle's pinned
+    primitives: forward RK4 with checkpoints every `stride` steps, Gaussian
+    residuals at the observations, then per block a replay and the reverse
+    sweep of the RK4 stages through adjoint_drift, with the parameter terms
+    -nu k^2 <K, S> and -alpha <K, S>.  Returns (value, gradient[2])."""
+    n = initial.shape[0]
+    q0, q1 = float(q[0]), float(q[1])
+    if not all(np.isfinite([q0, q1])) or not (-12 <= q0 <= 0 and -12 <= q1 <= 0):
+        return np.inf, np.zeros(2)
+    nu, alpha = np.exp(q0), np.exp(q1)
+    K = (n - 1) // 3
+    if dt * (2 * nu * K * K + alpha) > 2.5:
+        return np.inf, np.zeros(2)
+    k = np.fft.fftfreq(n, 1.0 / n)
+    kx, ky = np.meshgrid(k, k)
+    ksq = kx * kx + ky * ky
+    mask = orc.active_mask(n).astype(bool)
+    steps = obs[:, 0].astype(np.int64)
+    last = int(steps[-1])
+    w = initial.copy()
+    checkpoints, cstep = [w.copy()], [0]
+    residual = np.zeros(len(obs))
+    value, o = 0.0, 0
+    for step in range(last + 1):
+        if step > 0:
+            w = orc.rk4(w, nu, alpha, dt, 1)
+            if step % stride == 0 or step == last:
+                checkpoints.append(w.copy())
+                cstep.append(step)
+        if o < len(obs) and steps[o] == step:
+            field = physical(orc, w)
+            while o < len(obs) and steps[o] == step:
+                _, x, y, v, sd = obs[o]
+                r = (field[int(y), int(x)] - v) / sd
+                value += 0.5 * r * r
+                residual[o] = r / sd
+                o += 1
+    drift = lambda s: orc.rhs(s, nu, alpha)
+    grad = np.zeros(2)
+    cot = np.zeros((n, n), dtype=np.complex128)
+    o = len(obs)
+
+    def inject(step):
+        nonlocal o, cot
+        if o == 0 or steps[o - 1] != step:
+            return
+        phys = np.zeros((n, n), dtype=np.complex128)
+        while o > 0 and steps[o - 1] == step:
+            o -= 1
+            phys[int(obs[o, 2]), int(obs[o, 1])] += residual[o] * n * n
+        cot = cot + np.where(mask, orc.fft2d(phys), 0)
+
+    def contribution(state, weight):
+        inner = np.real(np.conj(weight) * state)
+        grad[0] -= nu * np.sum(ksq * inner)
+        grad[1] -= alpha * np.sum(inner)
+
+    for b in range(len(checkpoints) - 1, 0, -1):
+        begin, end = cstep[b - 1], cstep[b]
+        states = [checkpoints[b - 1]]
+        for _ in range(begin, end):
+            s = states[-1]
+            k1 = drift(s)
+            k2 = drift(s + dt / 2 * k1)
+            k3 = drift(s + dt / 2 * k2)
+            k4 = drift(s + dt * k3)
+            states.append(s + dt / 6 * (k1 + 2.0 * k2 + 2.0 * k3 + k4))
+        for step in range(end, begin, -1):
+            inject(step)
+            s1 = states[step - begin - 1]
+            s2 = s1 + dt / 2 * drift(s1)
+            s3 = s1 + dt / 2 * drift(s2)
+            s4 = s1 + dt * drift(s3)
+            prev = cot.copy()
+            K1, K2, K3, K4 = cot * (dt / 6), cot * (dt / 3), cot * (dt / 3), cot * (dt / 6)
+            contribution(s4, K4)
+            b4 = orc.adjoint_drift(s4, K4, nu, alpha)
+            prev += b4
+            K3 = K3 + dt * b4
+            contribution(s3, K3)
+            b3 = orc.adjoint_drift(s3, K3, nu, alpha)
+            prev += b3
+            K2 = K2 + dt / 2 * b3
+            contribution(s2, K2)
+            b2 = orc.adjoint_drift(s2, K2, nu, alpha)
+            prev += b2
+            K1 = K1 + dt / 2 * b2
+            contribution(s1, K1)
+            prev += orc.adjoint_drift(s1, K1, nu, alpha)
+            cot = prev
+    return value, grad

@@ -41,3 +41,26 @@ def test_localized_update_without_taper_is_the_plain_update(orc):
     got = localized_scalar_update(orc, fields, h, 0.3, 0.2, eps, 3, 4, 1e9)
     want = orc.enkf_scalar_update(fields.reshape(P, -1), h, 0.3, 0.2, eps).reshape(P, n, n)
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+
+
+def test_likelihood_adjoint_matches_reference_and_differences(orc, ref):
+    """The restated discrete adjoint equals the reference's likelihood_adjoint
+    (value and gradient) and the central differences of its own value."""
+    from oracle_filters import oracle_likelihood_adjoint
+    h, rows = ref.synthetic(16, 0.01, 30, 10, 0.03, 0.1, 0.0, 0.5, 2, 3, 0.2, 987)
+    init = ref.initialize(16, "multimode")
+    for q in ((np.log(0.03), np.log(0.1)), (np.log(0.02), np.log(0.15)), (-13.0, -1.0)):
+        want = ref.likelihood_adjoint(h, q[0], q[1], 7)
+        v, g = oracle_likelihood_adjoint(orc, init, rows, 0.01, q, 7)
+        if not np.isfinite(want["value"]):
+            assert not np.isfinite(v)
+            continue
+        assert abs(v - want["value"]) < 1e-12 * abs(want["value"])
+        np.testing.assert_allclose(g, want["gradient"], rtol=1e-10, atol=0)
+        e = 1e-5
+        fd = [(oracle_likelihood_adjoint(orc, init, rows, 0.01, (q[0] + e, q[1]), 7)[0] -
+               oracle_likelihood_adjoint(orc, init, rows, 0.01, (q[0] - e, q[1]), 7)[0]) / (2 * e),
+              (oracle_likelihood_adjoint(orc, init, rows, 0.01, (q[0], q[1] + e), 7)[0] -
+               oracle_likelihood_adjoint(orc, init, rows, 0.01, (q[0], q[1] - e), 7)[0]) / (2 * e)]
+        np.testing.assert_allclose(g, fd, rtol=1e-6, atol=0)
+    ref.dataset_free(h)
