@@ -132,6 +132,13 @@ __global__ void __launch_bounds__(256) k_axpby(long count, cx<R> *y, R a, const 
     y[i] = r;
 }
 
+// y = 0 over `count` complex entries (a fill, so stale NaNs are cleared too).
+template <typename R> __global__ void __launch_bounds__(256) k_zero(long count, cx<R> *y) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count)
+        y[i] = {R(0), R(0)};
+}
+
 // pred[m * nobs + o]: member m's physical vorticity at observation o, from
 // the mixed w~ field (one thread per member and observation).
 template <typename R> __global__ void __launch_bounds__(256) k_point_values(PfArgs<R> a) {
@@ -202,6 +209,25 @@ __global__ void __launch_bounds__(256) k_param_terms(int K, const cx<R> *S, cons
         grad[2 * m] -= nu[m] * red[0];
         grad[2 * m + 1] -= alpha[m] * red[256];
     }
+}
+
+// out -= c X on the retained half plane with c = nu_m |k|^2 (j = 0) or
+// alpha_m (j = 1): the explicit parameter derivatives of the drift with
+// respect to log nu and log alpha (src/solver.cpp:333-340).
+template <typename R>
+__global__ void __launch_bounds__(256) k_param_source(int B, int K, cx<R> *out, const cx<R> *X,
+                                                      const double *nu, const double *alpha,
+                                                      int j) {
+    const int J = 2 * K + 1;
+    const long spec = (long)(K + 1) * J;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)B * spec)
+        return;
+    const long m = i / spec;
+    const int c = (int)((i % spec) / J), ky = (int)(i % J) - K;
+    const R f = j == 0 ? (R)(nu[m] * (double)(c * c + ky * ky)) : (R)alpha[m];
+    const cx<R> x = X[i], o = out[i];
+    out[i] = {o.x - f * x.x, o.y - f * x.y};
 }
 
 // Device counters read by replayed graphs: [0] step index, [1] filter group,
@@ -811,7 +837,8 @@ template <typename R> class Engine final : public EngineBase {
         cx<R> *S2 = wk[0].p, *S3 = wk[1].p, *S4 = wk[2].p, *kb = wk[3].p, *cot = wk[4].p,
               *prev = wk[5].p, *K1 = wk[6].p, *K2 = wk[7].p, *K3 = wk[8].p, *K4 = wk[9].p,
               *bb = wk[10].p;
-        axpby(cot, 0.0, w0_.p, 0.0, nullptr); // zero
+        hipLaunchKernelGGL(k_zero<R>, dim3((unsigned)((SB + 255) / 256)), dim3(256), 0, stream_,
+                           (long)SB, cot);
         DeviceBuffer<double> d_grad(2 * B_);
         {
             std::vector<double> z(2 * B_, 0.0);
@@ -903,6 +930,131 @@ template <typename R> class Engine final : public EngineBase {
         }
         mix_valid_ = false;
         params_dirty_ = false;
+    }
+
+    // Solver::step_tangent (src/solver.cpp:324-380), batched: `steps` RK4 steps
+    // of every member together with its two sensitivities (with respect to
+    // log nu and log alpha).  tangents: [member][2][n*n] reference Fields,
+    // read (tangents_in) and written (tangents_out, may alias).
+    void step_tangent(double dt, size_t steps, const double *tangents_in,
+                      double *tangents_out) override {
+        for (size_t m = 0; m < B_; ++m)
+            if (sigma_[m] != 0 || amp_[m] != 0)
+                throw std::invalid_argument(
+                    "tangent implementation currently requires zero forcing");
+        validate_dt(dt);
+        flush_parameters();
+        tangent_buffers();
+        const size_t nn = (size_t)n_ * n_;
+        std::vector<double> f(2 * nn * B_);
+        for (int j = 0; j < 2; ++j) {
+            for (size_t m = 0; m < B_; ++m)
+                std::memcpy(f.data() + 2 * nn * m, tangents_in + 2 * nn * (2 * m + j),
+                            2 * nn * sizeof(double));
+            upload_fields(f.data(), tg_[j].p);
+        }
+        for (size_t s = 0; s < steps; ++s)
+            tangent_step(dt);
+        for (int j = 0; j < 2; ++j) {
+            download_fields(tg_[j].p, f.data());
+            for (size_t m = 0; m < B_; ++m)
+                std::memcpy(tangents_out + 2 * nn * (2 * m + j), f.data() + 2 * nn * m,
+                            2 * nn * sizeof(double));
+        }
+    }
+
+    // likelihood_gradient (src/inference.cpp:340-382), batched over members'
+    // (log nu, log alpha): the deterministic negative log-likelihood and its
+    // gradient by forward sensitivities (step_tangent from zero tangents).
+    void likelihood_gradient(const double *initial, double dt, size_t nobs, const size_t *ostep,
+                             const int *ox, const int *oy, const double *oval,
+                             const double *osd, const double *log_params, double *objective,
+                             double *gradient) override {
+        if (nobs == 0 || !std::isfinite(dt) || dt <= 0)
+            throw std::invalid_argument("likelihood gradient requires observations and dt > 0");
+        for (size_t o = 0; o < nobs; ++o)
+            if ((o > 0 && ostep[o] < ostep[o - 1]) || ox[o] < 0 || oy[o] < 0 || ox[o] >= n_ ||
+                oy[o] >= n_ || !std::isfinite(oval[o]) || !std::isfinite(osd[o]) || osd[o] <= 0)
+                throw std::invalid_argument("observations must be time ordered, finite, in "
+                                            "bounds, and have positive uncertainty");
+        std::vector<char> valid(B_, 1);
+        const double largest = (double)((n_ - 1) / 3);
+        for (size_t m = 0; m < B_; ++m) {
+            const double q0 = log_params[2 * m], q1 = log_params[2 * m + 1];
+            valid[m] = std::isfinite(q0) && std::isfinite(q1) && q0 >= -12 && q0 <= 0 &&
+                       q1 >= -12 && q1 <= 0;
+            if (valid[m] &&
+                dt * (2 * std::exp(q0) * largest * largest + std::exp(q1)) > 2.5)
+                valid[m] = 0;
+            nu_[m] = valid[m] ? std::exp(q0) : 0.0;
+            alpha_[m] = valid[m] ? std::exp(q1) : 0.0;
+            sigma_[m] = 0;
+            amp_[m] = 0;
+        }
+        push_parameters();
+        fill_state(initial);
+        calib_buffers();
+        tangent_buffers();
+        point_tables();
+        for (int j = 0; j < 2; ++j)
+            hipLaunchKernelGGL(k_zero<R>, dim3((unsigned)((B_ * spec_ + 255) / 256)), dim3(256),
+                               0, stream_, (long)(B_ * spec_), tg_[j].p);
+        DeviceBuffer<int> d_ox(nobs), d_oy(nobs);
+        upload(d_ox.p, ox, nobs);
+        upload(d_oy.p, oy, nobs);
+        DeviceBuffer<double> d_pred(3 * B_ * nobs);
+        std::vector<double> value(B_, 0.0), g0(B_, 0.0), g1(B_, 0.0);
+        size_t step = 0, o = 0;
+        while (o < nobs) {
+            while (step < ostep[o]) {
+                tangent_step(dt);
+                ++step;
+            }
+            size_t hi = o;
+            while (hi < nobs && ostep[hi] == step)
+                ++hi;
+            const size_t cnt = hi - o;
+            // point values of the state (mixed fields of w0) and of both tangents
+            const cx<R> *mixes[3];
+            ensure_mixed(dt);
+            mixes[0] = mix_.p;
+            for (int j = 0; j < 2; ++j) {
+                StageArgs<R> m0 = stage_args(1.0, nullptr, 0);
+                m0.w0 = tg_[j].p;
+                m0.mix = j == 0 ? calA_.p : calB_.p;
+                run_col(0, m0);
+                mixes[1 + j] = m0.mix;
+            }
+            for (int f = 0; f < 3; ++f) {
+                PfArgs<R> pa{(int)B_, n_, K_, mixes[f], (int)cnt, d_ox.p + o, d_oy.p + o,
+                             nullptr, nullptr, pf_cs_.p, pf_sn_.p, nullptr, nullptr, nullptr,
+                             d_pred.p + (size_t)f * B_ * cnt};
+                const long total = (long)(B_ * cnt);
+                hipLaunchKernelGGL(k_point_values<R>, dim3((unsigned)((total + 255) / 256)),
+                                   dim3(256), 0, stream_, pa);
+            }
+            std::vector<double> pred(3 * B_ * cnt);
+            check(hipMemcpyAsync(pred.data(), d_pred.p, pred.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, stream_),
+                  "predictions");
+            check(hipStreamSynchronize(stream_), "predictions");
+            for (size_t m = 0; m < B_; ++m)
+                for (size_t k = 0; k < cnt; ++k) {
+                    const size_t r = o + k;
+                    const double res = (pred[m * cnt + k] - oval[r]) / osd[r];
+                    value[m] += 0.5 * res * res;
+                    g0[m] += res * pred[(B_ + m) * cnt + k] / osd[r];
+                    g1[m] += res * pred[(2 * B_ + m) * cnt + k] / osd[r];
+                }
+            o = hi;
+        }
+        check(hipGetLastError(), "tangent kernels");
+        for (size_t m = 0; m < B_; ++m) {
+            objective[m] = valid[m] ? value[m] : std::numeric_limits<double>::infinity();
+            gradient[2 * m] = valid[m] ? g0[m] : 0.0;
+            gradient[2 * m + 1] = valid[m] ? g1[m] : 0.0;
+        }
+        mix_valid_ = false;
     }
 
     // ------------------------------------------------ launch shapes ----
@@ -1458,6 +1610,107 @@ template <typename R> class Engine final : public EngineBase {
         nlC_ = DeviceBuffer<cx<R>>(B_ * mixf_);
     }
 
+    // Tangent state: two sensitivity fields and the step's work fields, plus
+    // the calibration buffers the tangent drift runs through.
+    void tangent_buffers() {
+        calib_buffers();
+        if (tg_[0].count)
+            return;
+        for (auto &b : tg_)
+            b = DeviceBuffer<cx<R>>(B_ * spec_);
+        for (auto &b : tw_fields_)
+            b = DeviceBuffer<cx<R>>(B_ * spec_);
+        slope1_ = DeviceBuffer<cx<R>>(B_ * spec_);
+    }
+
+    // cos / sin (2 pi j / n) for point evaluation.
+    void point_tables() {
+        if (pf_cs_.count == (size_t)n_)
+            return;
+        check(hipStreamSynchronize(stream_), "table growth");
+        clear_graphs();
+        std::vector<double> cs(n_), sn(n_);
+        for (int j = 0; j < n_; ++j) {
+            cs[j] = std::cos(2 * PI * j / n_);
+            sn[j] = std::sin(2 * PI * j / n_);
+        }
+        pf_cs_ = DeviceBuffer<double>(n_);
+        pf_sn_ = DeviceBuffer<double>(n_);
+        upload(pf_cs_.p, cs.data(), n_);
+        upload(pf_sn_.p, sn.data(), n_);
+    }
+
+    // Tangent of the drift at X along D into out.
+    void linearized_into(const cx<R> *X, const cx<R> *D, cx<R> *out) {
+        if (!calB_.p || !nlB_.p || !out)
+            throw std::logic_error("tangent pass on unallocated buffers");
+        jacobian(D, X, calA_.p, nlA_.p); // J(D, X)
+        jacobian(X, D, calB_.p, nlB_.p); // J(X, D)
+        StageArgs<R> a = stage_args(1.0, nullptr, 0);
+        a.nl = nlA_.p;
+        a.nl2 = nlB_.p;
+        a.w1 = const_cast<cx<R> *>(D);
+        a.out = out;
+        run_col(9, a);
+    }
+
+    // One step of Solver::step_tangent (src/solver.cpp:324-380) for all
+    // members (no forcing): the state w0_ by RK4 through the drift, and the
+    // sensitivities tg_[0..1] by RK4 of t' = L_X t - (nu k^2 | alpha) X at the
+    // same stages, accumulated in the reference's order.
+    void tangent_step(double dt) {
+        const size_t SB = B_ * spec_;
+        const dim3 grid((unsigned)((SB + 255) / 256));
+        auto axpby = [&](cx<R> *y, double a, const cx<R> *x, double b, const cx<R> *z) {
+            hipLaunchKernelGGL(k_axpby<R>, grid, dim3(256), 0, stream_, (long)SB, y, (R)a, x,
+                               (R)b, z);
+        };
+        cx<R> *X = tw_fields_[0].p, *kb = tw_fields_[1].p, *kacc = tw_fields_[2].p;
+        cx<R> *U[2] = {tw_fields_[3].p, tw_fields_[4].p};
+        cx<R> *acc[2] = {tw_fields_[5].p, tw_fields_[6].p};
+        cx<R> *slope[2] = {tw_fields_[7].p, slope1_.p}; // this (then the last) stage's slopes
+        const double stage_c[4] = {0.0, dt / 2, dt / 2, dt}, weight[4] = {1, 2, 2, 1};
+        for (int i = 0; i < 4; ++i) {
+            // stage state X = w + c k_{i-1} and tangents U_j = T_j + c t_{i-1,j}
+            const cx<R> *Xs = w0_.p;
+            const cx<R> *Us[2] = {tg_[0].p, tg_[1].p};
+            if (i > 0) {
+                axpby(X, 1.0, w0_.p, stage_c[i], kb);
+                Xs = X;
+                for (int j = 0; j < 2; ++j) {
+                    axpby(U[j], 1.0, tg_[j].p, stage_c[i], slope[j]);
+                    Us[j] = U[j];
+                }
+            }
+            // drift of the stage into kb
+            jacobian(Xs, Xs, calA_.p, nlA_.p);
+            StageArgs<R> c = stage_args(dt, nullptr, 0);
+            c.nl = nlA_.p;
+            c.w1 = const_cast<cx<R> *>(Xs);
+            c.out = kb;
+            run_col(9, c);
+            if (i == 0)
+                axpby(kacc, 1.0, kb, 0.0, nullptr);
+            else
+                axpby(kacc, 1.0, kacc, weight[i], kb);
+            for (int j = 0; j < 2; ++j) {
+                linearized_into(Xs, Us[j], slope[j]);
+                hipLaunchKernelGGL(k_param_source<R>, grid, dim3(256), 0, stream_, (int)B_, K_,
+                                   slope[j], Xs, (const double *)d_nu_.p,
+                                   (const double *)d_alpha_.p, j);
+                if (i == 0)
+                    axpby(acc[j], 1.0, slope[j], 0.0, nullptr);
+                else
+                    axpby(acc[j], 1.0, acc[j], weight[i], slope[j]);
+            }
+        }
+        axpby(w0_.p, 1.0, w0_.p, dt / 6, kacc);
+        for (int j = 0; j < 2; ++j)
+            axpby(tg_[j].p, 1.0, tg_[j].p, dt / 6, acc[j]);
+        mix_valid_ = false;
+        ++step_;
+    }
+
     // Drift of the states S into out (retained half planes, B members).
     void drift_into(const cx<R> *S, cx<R> *out) {
         jacobian(S, S, calA_.p, nlA_.p); // J(S, S)
@@ -1470,6 +1723,8 @@ template <typename R> class Engine final : public EngineBase {
 
     // Adjoint drift at states S applied to cotangents W, into out.
     void adjoint_into(const cx<R> *S, const cx<R> *W, cx<R> *out) {
+        if (!calB_.p || !nlB_.p || !nlC_.p || !out)
+            throw std::logic_error("adjoint pass on unallocated buffers");
         jacobian(S, W, calA_.p, nlA_.p); // J(S, W); calA holds W~
         StageArgs<R> m0 = stage_args(1.0, nullptr, 0);
         m0.w0 = const_cast<cx<R> *>(S);
@@ -1489,6 +1744,8 @@ template <typename R> class Engine final : public EngineBase {
     // J(a, b) = u(a) . grad(b) of every member into the nonlinear mixed field
     // nl: two-state column pass (psi of a, w of b) into mix, then a row pass.
     void jacobian(const cx<R> *a_state, const cx<R> *b_state, cx<R> *mix, cx<R> *nl) {
+        if (!a_state || !b_state || !mix || !nl)
+            throw std::logic_error("Jacobian pass on unallocated buffers");
         StageArgs<R> c = stage_args(1.0, nullptr, 0);
         c.w0 = const_cast<cx<R> *>(a_state);
         c.w1 = const_cast<cx<R> *>(b_state);
@@ -1715,6 +1972,8 @@ template <typename R> class Engine final : public EngineBase {
     bool params_dirty_ = false, mix_valid_ = false;
     uint64_t step_ = 0;
     double last_dt_ = 0, max_dt_ = 0, last_cfl_rate_ = 0;
+    // tangent state: sensitivities, stage work fields, the second slope field
+    DeviceBuffer<cx<R>> tg_[2], tw_fields_[8], slope1_;
     // calibration drifts: input fields, result, mixed and nonlinear fields
     DeviceBuffer<cx<R>> caux_, cout_, calA_, calB_, nlA_, nlB_, nlC_;
     // launch shapes (variants.hpp) and their twiddle tables
@@ -1891,6 +2150,28 @@ int hzl_ensemble_adjoint_drift(hzl_ensemble_t h, const double *cotangents, doubl
         if (!cotangents || !out)
             throw std::invalid_argument("null argument");
         engine_of(h).adjoint_drift(cotangents, out);
+    });
+}
+
+int hzl_ensemble_step_tangent(hzl_ensemble_t h, double dt, size_t steps,
+                              const double *tangents_in, double *tangents_out) {
+    return guarded([&] {
+        if (!tangents_in || !tangents_out)
+            throw std::invalid_argument("null argument");
+        engine_of(h).step_tangent(dt, steps, tangents_in, tangents_out);
+    });
+}
+
+int hzl_likelihood_gradient(hzl_ensemble_t h, const double *initial_field, double dt,
+                            size_t nobs, const size_t *steps, const int *x, const int *y,
+                            const double *value, const double *sd, const double *log_params,
+                            double *objective, double *gradient) {
+    return guarded([&] {
+        if (!initial_field || !steps || !x || !y || !value || !sd || !log_params || !objective ||
+            !gradient)
+            throw std::invalid_argument("null argument");
+        engine_of(h).likelihood_gradient(initial_field, dt, nobs, steps, x, y, value, sd,
+                                         log_params, objective, gradient);
     });
 }
 

@@ -79,6 +79,13 @@ class EngineBase {
     virtual void drift(double *out) = 0;
     virtual void linearized_drift(const double *directions, double *out) = 0;
     virtual void adjoint_drift(const double *cotangents, double *out) = 0;
+    virtual void step_tangent(double dt, size_t steps, const double *tangents_in,
+                              double *tangents_out) = 0;
+    virtual void likelihood_gradient(const double *initial, double dt, size_t nobs,
+                                     const size_t *ostep, const int *ox, const int *oy,
+                                     const double *oval, const double *osd,
+                                     const double *log_params, double *objective,
+                                     double *gradient) = 0;
     virtual void likelihood_adjoint(const double *initial, double dt, size_t nobs,
                                     const size_t *ostep, const int *ox, const int *oy,
                                     const double *oval, const double *osd,

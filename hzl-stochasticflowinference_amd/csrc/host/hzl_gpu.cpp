@@ -338,6 +338,63 @@ std::vector<Diagnostics> Ensemble::diagnostics() {
     return out;
 }
 
+// -------------------------------------------------- likelihood_adjoint --
+std::vector<AdjointEvaluation> likelihood_adjoint(const Dataset &data,
+                                                  const std::vector<Vector> &log_parameters,
+                                                  std::size_t checkpoint_stride) {
+    data.validate();
+    if (data.stochastic || checkpoint_stride == 0 || log_parameters.empty())
+        throw std::invalid_argument(
+            "adjoint requires deterministic data, two parameters, and positive checkpoint stride");
+    for (const auto &q : log_parameters)
+        if (q.size() != 2)
+            throw std::invalid_argument(
+                "adjoint requires deterministic data, two parameters, and positive checkpoint "
+                "stride");
+    const std::size_t B = log_parameters.size(), nobs = data.observations.size();
+    Parameters p;
+    p.forcing_sigma = 0;
+    p.forcing_min = data.forcing_min;
+    p.forcing_max = data.forcing_max;
+    Ensemble e(data.n, B, p, Precision::fp64, 0);
+    std::vector<std::size_t> steps(nobs);
+    std::vector<int> x(nobs), y(nobs);
+    std::vector<double> value(nobs), sd(nobs), q(2 * B), obj(B), grad(2 * B);
+    for (std::size_t o = 0; o < nobs; ++o) {
+        const auto &ob = data.observations[o];
+        steps[o] = ob.step;
+        x[o] = (int)ob.x;
+        y[o] = (int)ob.y;
+        value[o] = ob.value;
+        sd[o] = ob.standard_deviation;
+    }
+    for (std::size_t m = 0; m < B; ++m) {
+        q[2 * m] = log_parameters[m][0];
+        q[2 * m + 1] = log_parameters[m][1];
+    }
+    const Field init = initial_condition(data.n, "multimode");
+    std::size_t counts[3] = {0, 0, 0};
+    check(hzl_likelihood_adjoint(e.handle(), raw(init), data.dt, nobs, steps.data(), x.data(),
+                                 y.data(), value.data(), sd.data(), q.data(), checkpoint_stride,
+                                 obj.data(), grad.data(), counts));
+    std::vector<AdjointEvaluation> out(B);
+    for (std::size_t m = 0; m < B; ++m) {
+        out[m].objective.value = obj[m];
+        out[m].objective.gradient = {grad[2 * m], grad[2 * m + 1]};
+        if (std::isfinite(obj[m])) {
+            out[m].checkpoint_fields = counts[0];
+            out[m].peak_block_fields = counts[1];
+            out[m].replayed_steps = counts[2];
+        }
+    }
+    return out;
+}
+
+AdjointEvaluation likelihood_adjoint(const Dataset &data, const Vector &log_parameters,
+                                     std::size_t checkpoint_stride) {
+    return likelihood_adjoint(data, std::vector<Vector>{log_parameters}, checkpoint_stride)[0];
+}
+
 // -------------------------------------------------------------- Dataset --
 void Dataset::validate() const {
     Parameters p;

@@ -218,6 +218,25 @@ std::vector<EnsembleFilterRecord> ensemble_kalman_filter(Ensemble &members, cons
                                                          EnKFOptions options = {},
                                                          bool batch = false);
 
+using Vector = std::vector<double>;
+struct ValueGradient {
+    double value{};
+    Vector gradient;
+};
+struct AdjointEvaluation {
+    ValueGradient objective;
+    std::size_t checkpoint_fields = 0, peak_block_fields = 0, replayed_steps = 0;
+};
+// Deterministic negative log-likelihood and its gradient with respect to
+// (log viscosity, log friction) by the discrete adjoint of RK4
+// (src/adjoint.cpp:43-161), from initialize("multimode"), on the GPU in FP64.
+AdjointEvaluation likelihood_adjoint(const Dataset &data, const Vector &log_parameters,
+                                     std::size_t checkpoint_stride = 20);
+// The same for many parameter vectors at once (one GPU member each).
+std::vector<AdjointEvaluation> likelihood_adjoint(const Dataset &data,
+                                                  const std::vector<Vector> &log_parameters,
+                                                  std::size_t checkpoint_stride = 20);
+
 // Independent members from the zero state, final diagnostics in member order
 // (src/inference.cpp:555-590); `workers` is accepted and unused.
 std::vector<Diagnostics> ensemble(std::size_t n, Parameters p, double dt, std::size_t steps,

@@ -27,7 +27,8 @@ EXPORTS = (
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
     "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
     "hzl_ensemble_step_integrator", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
-    "hzl_ensemble_adjoint_drift", "hzl_likelihood_adjoint",
+    "hzl_ensemble_adjoint_drift", "hzl_likelihood_adjoint", "hzl_ensemble_step_tangent",
+    "hzl_likelihood_gradient",
     "hzl_ensemble_variant_count", "hzl_ensemble_set_tuning", "hzl_ensemble_get_tuning",
     "hzl_ensemble_autotune",
     "hzl_ensemble_step_forced",
@@ -104,6 +105,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_linearized_drift.argtypes = [vp, _dp, _dp]
     L.hzl_ensemble_adjoint_drift.argtypes = [vp, _dp, _dp]
     szp = C.POINTER(C.c_size_t)
+    L.hzl_ensemble_step_tangent.argtypes = [vp, C.c_double, C.c_size_t, _dp, _dp]
+    L.hzl_likelihood_gradient.argtypes = [vp, _dp, C.c_double, C.c_size_t, szp, ipp, ipp, _dp,
+                                          _dp, _dp, _dp, _dp]
     L.hzl_likelihood_adjoint.argtypes = [vp, _dp, C.c_double, C.c_size_t, szp, ipp, ipp, _dp, _dp,
                                          _dp, C.c_size_t, _dp, _dp, szp]
     L.hzl_ensemble_variant_count.argtypes = [vp, ipp, ipp]
@@ -245,6 +249,15 @@ class Ensemble:
             if code is None:
                 raise HzlInvalidArgument("integrator must be rk4 or ssprk3")
             _check(_lib.hzl_ensemble_step_integrator(self._h, dt, steps, code))
+
+    def step_tangent(self, dt: float, steps: int, tangents) -> np.ndarray:
+        """Solver::step_tangent for every member: tangents (members, 2, n, n)
+        in, advanced sensitivities out (the states advance too)."""
+        t = np.ascontiguousarray(tangents, dtype=np.complex128).reshape(
+            self.members, 2, self.n, self.n).copy()
+        _check(_lib.hzl_ensemble_step_tangent(self._h, dt, steps, t.ctypes.data_as(_dp),
+                                              t.ctypes.data_as(_dp)))
+        return t
 
     def drift(self) -> np.ndarray:
         """Solver::drift of every member's state: (members, n, n) spectral."""
@@ -566,6 +579,34 @@ def save_checkpoint(path: str, c: dict) -> None:
     out += [f"{g(a.real)} {g(a.imag)} {g(b.real)} {g(b.imag)}" for a, b in zip(w, F)]
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
+
+
+def _observation_arrays(observations):
+    obs = np.asarray(observations, dtype=np.float64)
+    if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
+        raise ValueError("observations must be rows of (step, x, y, value, sd)")
+    return (np.ascontiguousarray(obs[:, 0], dtype=np.uint64).astype(np.uintp),
+            np.ascontiguousarray(obs[:, 1], dtype=np.int32),
+            np.ascontiguousarray(obs[:, 2], dtype=np.int32),
+            np.ascontiguousarray(obs[:, 3]), np.ascontiguousarray(obs[:, 4]))
+
+
+def likelihood_gradient(ensemble: Ensemble, initial, observations, dt: float, log_params):
+    """Batched likelihood_gradient (src/inference.cpp:340-382): the objective
+    of likelihood_adjoint with its gradient by forward sensitivities.
+    Returns (values [members], gradients [members, 2])."""
+    steps, x, y, v, sd = _observation_arrays(observations)
+    q = np.ascontiguousarray(log_params, dtype=np.float64).reshape(ensemble.members, 2)
+    init = _fields(initial, 1, ensemble.n)
+    values = np.zeros(ensemble.members)
+    grads = np.zeros((ensemble.members, 2))
+    ip, szp = C.POINTER(C.c_int), C.POINTER(C.c_size_t)
+    _check(_lib.hzl_likelihood_gradient(ensemble._h, init.ctypes.data_as(_dp), dt, len(x),
+                                        steps.ctypes.data_as(szp), x.ctypes.data_as(ip),
+                                        y.ctypes.data_as(ip), v.ctypes.data_as(_dp),
+                                        sd.ctypes.data_as(_dp), q.ctypes.data_as(_dp),
+                                        values.ctypes.data_as(_dp), grads.ctypes.data_as(_dp)))
+    return values, grads
 
 
 def likelihood_adjoint(ensemble: Ensemble, initial, observations, dt: float, log_params,

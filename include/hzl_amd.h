@@ -133,6 +133,23 @@ int hzl_ensemble_drift(hzl_ensemble_t h, double *out);
 int hzl_ensemble_linearized_drift(hzl_ensemble_t h, const double *directions, double *out);
 int hzl_ensemble_adjoint_drift(hzl_ensemble_t h, const double *cotangents, double *out);
 
+/* `steps` RK4 steps of every member together with its sensitivities to
+ * (log viscosity, log friction): tangents_in / tangents_out hold per member
+ * two reference Fields [member][2][n*n] (may alias).  Members must have no
+ * forcing (sigma = 0, Kolmogorov amplitude 0), as the reference requires.
+ * Replaces: Solver::step_tangent (src/solver.cpp:324-380), batched. */
+int hzl_ensemble_step_tangent(hzl_ensemble_t h, double dt, size_t steps,
+                              const double *tangents_in, double *tangents_out);
+
+/* The same objective as hzl_likelihood_adjoint below, with its gradient by
+ * forward sensitivities (step_tangent from zero tangents) instead of the
+ * adjoint; member m at log_params[2m..2m+1].
+ * Replaces: likelihood_gradient (src/inference.cpp:340-382), batched. */
+int hzl_likelihood_gradient(hzl_ensemble_t h, const double *initial_field, double dt,
+                            size_t nobs, const size_t *steps, const int *x, const int *y,
+                            const double *value, const double *sd, const double *log_params,
+                            double *objective, double *gradient);
+
 /* Deterministic negative log-likelihood and its gradient with respect to
  * (log viscosity, log friction) by the discrete adjoint of RK4 with state
  * checkpoints every `stride` steps, for every member's own parameter vector

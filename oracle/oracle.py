@@ -474,6 +474,28 @@ class Reference:
         self._check(self.lib.ref_negative_log_likelihood(h, log_nu, log_alpha, _ptr(out)))
         return float(out[0])
 
+    def likelihood_gradient(self, h, log_nu, log_alpha):
+        """hzl::likelihood_gradient: (value, gradient[2])."""
+        L = self.lib
+        L.ref_likelihood_gradient.restype = C.c_int
+        L.ref_likelihood_gradient.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+        out = np.zeros(3)
+        self._check(L.ref_likelihood_gradient(h, log_nu, log_alpha, _ptr(out)))
+        return float(out[0]), out[1:3].copy()
+
+    def step_tangent(self, state, tangents, nu, alpha, dt, steps):
+        """`steps` reference Solver::step_tangent steps: (state, tangents[2])."""
+        L = self.lib
+        L.ref_step_tangent.restype = C.c_int
+        L.ref_step_tangent.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                       C.c_size_t, _dp, _dp, _dp, _dp]
+        w = _c(state)
+        t = np.ascontiguousarray(tangents, dtype=np.complex128)
+        wo, to = np.zeros_like(w), np.zeros_like(t)
+        self._check(L.ref_step_tangent(w.shape[0], nu, alpha, dt, steps, _ptr(w), _ptr(t),
+                                       _ptr(wo), _ptr(to)))
+        return wo, to
+
     def likelihood_adjoint(self, h, log_nu, log_alpha, stride):
         """hzl::likelihood_adjoint: dict(value, gradient, checkpoint_fields,
         peak_block_fields, replayed_steps)."""

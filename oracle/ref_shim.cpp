@@ -15,6 +15,7 @@
 #include "hzl/filter.hpp"
 #include "hzl/inference.hpp"
 #include "hzl/solver.hpp"
+#include <array>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -351,6 +352,34 @@ int ref_negative_log_likelihood(void *d, double log_nu, double log_alpha, double
     return guarded([&] {
         *out = hzl::negative_log_likelihood(static_cast<RefDataset *>(d)->data,
                                             {log_nu, log_alpha});
+    });
+}
+
+// likelihood_gradient (src/inference.cpp:340-382): out = value, gradient[0..1].
+int ref_likelihood_gradient(void *d, double log_nu, double log_alpha, double *out) {
+    return guarded([&] {
+        const auto r = hzl::likelihood_gradient(static_cast<RefDataset *>(d)->data,
+                                                {log_nu, log_alpha});
+        out[0] = r.value;
+        out[1] = r.gradient[0];
+        out[2] = r.gradient[1];
+    });
+}
+
+// `steps` Solver::step_tangent steps from a spectral state and two
+// sensitivity fields (no forcing); writes the state and the sensitivities.
+int ref_step_tangent(std::size_t n, double nu, double alpha, double dt, std::size_t steps,
+                     const double *state, const double *tangents, double *state_out,
+                     double *tangents_out) {
+    return guarded([&] {
+        hzl::Solver s(n, frozen(nu, alpha, 1, 1), 0);
+        s.update_latent_state(to_field(state, n * n), hzl::Field(n * n));
+        std::array<hzl::Field, 2> t{to_field(tangents, n * n), to_field(tangents + 2 * n * n, n * n)};
+        for (std::size_t k = 0; k < steps; ++k)
+            s.step_tangent(dt, t);
+        from_field(s.coefficients(), state_out);
+        from_field(t[0], tangents_out);
+        from_field(t[1], tangents_out + 2 * n * n);
     });
 }
 

@@ -9,6 +9,9 @@
 //   pf <csv> <out>                  bootstrap_filter(data, p, 8, 555)
 //   enkf <csv> <out>                ensemble_kalman_filter(data, p, 8, 555)
 //   errors                          invalid arguments raise std::invalid_argument
+//   adjoint <csv> <stride> <out>    likelihood_adjoint(data, q, stride) at three q:
+//                                   value, gradient, checkpoint fields,
+//                                   peak block fields, replayed steps each
 //   calibration <n> <out>           Ensemble (FP64, 2 members from multimode and
 //                                   taylor-green): drift, linearized_drift along
 //                                   each other's state, adjoint_drift, then
@@ -166,6 +169,20 @@ int main(int argc, char **argv) {
             append(out, e.state(0));
             append(out, e.state(1));
             write(argv[3], out);
+        } else if (mode == "adjoint" && argc == 5) {
+            const Dataset data = Dataset::load(argv[2]);
+            const std::size_t stride = std::stoul(argv[3]);
+            const std::vector<Vector> qs{{std::log(0.03), std::log(0.1)},
+                                         {std::log(0.02), std::log(0.15)},
+                                         {-13.0, -1.0}};
+            std::vector<double> out;
+            for (const auto &q : qs) {
+                const AdjointEvaluation r = likelihood_adjoint(data, q, stride);
+                out.insert(out.end(), {r.objective.value, r.objective.gradient[0],
+                                       r.objective.gradient[1], (double)r.checkpoint_fields,
+                                       (double)r.peak_block_fields, (double)r.replayed_steps});
+            }
+            write(argv[4], out);
         } else if (mode == "errors") {
             Parameters p;
             const bool ok =

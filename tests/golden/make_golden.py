@@ -25,7 +25,8 @@ Fixtures
                    diagnostics.
 
   adjoint_16.json  the reference's likelihood_adjoint (value, gradient) on the
-                   pf_deterministic_16 dataset at four parameter vectors, strides 7, 4.
+                   pf_deterministic_16 dataset at four parameter vectors, strides 7, 4,
+                   and its likelihood_gradient (forward sensitivities) at the same vectors.
 
 Usage: python tests/golden/make_golden.py [config0 drift32 pf16 checkpoint16 adjoint16]
 """
@@ -145,11 +146,15 @@ def adjoint16(ref: Reference) -> None:
                             gradient=[float(x) for x in r["gradient"]],
                             checkpoint_fields=r["checkpoint_fields"],
                             replayed_steps=r["replayed_steps"]))
+    tangent = []
+    for q in qs:
+        v, g = ref.likelihood_gradient(h, q[0], q[1])
+        tangent.append(dict(q=list(q), value=v, gradient=[float(x) for x in g]))
     ref.dataset_free(h)
     meta = dict(recipe="likelihood_adjoint(synthetic_observations(16, 0.01, 30, 10, p, 0.2, "
                        "987), q, stride) with nu 0.03, alpha 0.1, sigma 0; initial state "
                        "initialize('multimode'); observations = pf_deterministic_16 rows",
-                n=16, dt=0.01, evaluations=out)
+                n=16, dt=0.01, evaluations=out, likelihood_gradient=tangent)
     with open(os.path.join(HERE, "adjoint_16.json"), "w") as f:
         json.dump(meta, f, indent=2, sort_keys=True)
         f.write("\n")

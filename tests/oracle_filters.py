@@ -348,3 +348,63 @@ def oracle_likelihood_adjoint(orc, initial, obs, dt, q, stride):
             prev += orc.adjoint_drift(s1, K1, nu, alpha)
             cot = prev
     return value, grad
+
+
+def oracle_step_tangent(orc, w, tangents, nu, alpha, dt):
+    """One Solver::step_tangent step (src/solver.cpp:324-380) from the pinned
+    drift and tangent drift: RK4 of the state and of t' = L_X t - c_j X with
+    c_0 = nu |k|^2, c_1 = alpha.  Returns (state, [t0, t1])."""
+    n = w.shape[0]
+    k = np.fft.fftfreq(n, 1.0 / n)
+    kx, ky = np.meshgrid(k, k)
+    cj = (nu * (kx * kx + ky * ky), alpha)
+    drift = lambda s: orc.rhs(s, nu, alpha)
+
+    def slope(X, T):
+        return [orc.linearized_drift(X, T[j], nu, alpha) - cj[j] * X for j in range(2)]
+
+    T = [np.asarray(t, dtype=np.complex128) for t in tangents]
+    k1 = drift(w)
+    t1 = slope(w, T)
+    X = w + dt / 2 * k1
+    k2 = drift(X)
+    t2 = slope(X, [T[j] + dt / 2 * t1[j] for j in range(2)])
+    X = w + dt / 2 * k2
+    k3 = drift(X)
+    t3 = slope(X, [T[j] + dt / 2 * t2[j] for j in range(2)])
+    X = w + dt * k3
+    k4 = drift(X)
+    t4 = slope(X, [T[j] + dt * t3[j] for j in range(2)])
+    w_new = w + dt / 6 * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+    T_new = [T[j] + dt / 6 * (t1[j] + 2.0 * t2[j] + 2.0 * t3[j] + t4[j]) for j in range(2)]
+    return w_new, T_new
+
+
+def oracle_likelihood_gradient(orc, initial, obs, dt, q):
+    """likelihood_gradient (src/inference.cpp:340-382): the objective of
+    likelihood_adjoint with its gradient by forward sensitivities."""
+    n = initial.shape[0]
+    q0, q1 = float(q[0]), float(q[1])
+    if not all(np.isfinite([q0, q1])) or not (-12 <= q0 <= 0 and -12 <= q1 <= 0):
+        return np.inf, np.zeros(2)
+    nu, alpha = np.exp(q0), np.exp(q1)
+    K = (n - 1) // 3
+    if dt * (2 * nu * K * K + alpha) > 2.5:
+        return np.inf, np.zeros(2)
+    w = initial.copy()
+    T = [np.zeros_like(w), np.zeros_like(w)]
+    step, cached = 0, -1
+    value, grad = 0.0, np.zeros(2)
+    for s_, x, y, v, sd in obs:
+        while step < int(s_):
+            w, T = oracle_step_tangent(orc, w, T, nu, alpha, dt)
+            step += 1
+        if cached != step:
+            field = physical(orc, w)
+            pt = [physical(orc, t) for t in T]
+            cached = step
+        r = (field[int(y), int(x)] - v) / sd
+        value += 0.5 * r * r
+        for j in range(2):
+            grad[j] += r * pt[j][int(y), int(x)] / sd
+    return value, grad

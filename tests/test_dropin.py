@@ -230,3 +230,22 @@ def test_reference_signature_calibration_matches_oracle(tmp_path, orc):
     for g, w in zip(f, want):
         scale = max(np.linalg.norm(w), 1e-300)
         assert np.linalg.norm(g - w) / scale < 1e-12
+
+
+@pytest.mark.gpu
+def test_reference_signature_likelihood_adjoint_matches_golden(tmp_path):
+    """hzl::likelihood_adjoint(Dataset, q, stride) through the C++ drop-in on
+    the reference-written observation file: value, gradient and the
+    reference's bookkeeping counts."""
+    meta = json.load(open(os.path.join(GOLDEN, "adjoint_16.json")))
+    for stride in (7, 4):
+        run("adjoint", CSV, stride, tmp_path / "a.bin")
+        raw = np.fromfile(tmp_path / "a.bin").reshape(3, 6)
+        want = [e for e in meta["evaluations"] if e["stride"] == stride]
+        for got, w in zip(raw, [want[0], want[1], want[3]]):
+            if not np.isfinite(w["value"]):
+                assert not np.isfinite(got[0]) and got[1] == got[2] == 0
+                continue
+            assert abs(got[0] - w["value"]) < 1e-11 * abs(w["value"])
+            np.testing.assert_allclose(got[1:3], w["gradient"], rtol=1e-9, atol=0)
+            assert got[3] == w["checkpoint_fields"] and got[5] == w["replayed_steps"]

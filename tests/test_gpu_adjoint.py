@@ -70,3 +70,44 @@ def test_matches_oracle_at_32(pkg, orc):
         wv, wg = oracle_likelihood_adjoint(orc, init, rows, dt, q[k], stride)
         assert abs(v[k] - wv) < 1e-11 * abs(wv)
         np.testing.assert_allclose(grad[k], wg, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("precision,vtol,gtol", [("fp64", 1e-11, 1e-9), ("fp32", 1e-5, 2e-3)])
+def test_likelihood_gradient_matches_reference_golden(pkg, precision, vtol, gtol):
+    """Forward-sensitivity gradient (batched step_tangent) against the
+    reference's likelihood_gradient values."""
+    meta = json.load(open(os.path.join(GOLDEN, "adjoint_16.json")))
+    g = np.load(os.path.join(GOLDEN, "pf_deterministic_16.npz"))
+    init = full_from_half(g["initial"], 16)
+    evals = meta["likelihood_gradient"]
+    q = np.array([e["q"] for e in evals])
+    with pkg.Ensemble(16, len(evals), precision, seed=1) as ens:
+        v, grad = pkg.likelihood_gradient(ens, init, g["rows"], meta["dt"], q)
+    for k, e in enumerate(evals):
+        if not np.isfinite(e["value"]):
+            assert not np.isfinite(v[k]) and np.all(grad[k] == 0)
+            continue
+        assert abs(v[k] - e["value"]) < vtol * abs(e["value"])
+        np.testing.assert_allclose(grad[k], e["gradient"], rtol=gtol, atol=0)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_step_tangent_matches_oracle(pkg, orc, precision, tol):
+    from oracle_filters import oracle_step_tangent
+    n, dt, steps = 32, 5e-3, 3
+    nus, alphas = [0.01, 0.004], [0.1, 0.02]
+    S = np.stack([orc.grf(n, 500 + m) for m in range(2)])
+    T = np.stack([[orc.grf(n, 600 + 2 * m), orc.grf(n, 601 + 2 * m)] for m in range(2)])
+    with pkg.Ensemble(n, 2, precision, sigma=0.0, amp=0.0, seed=1) as ens:
+        for m in range(2):
+            ens.set_member_parameters(m, nus[m], alphas[m], 0.0, 0.0)
+        ens.set_state(S)
+        got_t = ens.step_tangent(dt, steps, T)
+        got_w = ens.get_state()
+    for m in range(2):
+        w, t = S[m], [T[m, 0], T[m, 1]]
+        for _ in range(steps):
+            w, t = oracle_step_tangent(orc, w, t, nus[m], alphas[m], dt)
+        assert np.linalg.norm(got_w[m] - w) / np.linalg.norm(w) < tol
+        for j in range(2):
+            assert np.linalg.norm(got_t[m, j] - t[j]) / np.linalg.norm(t[j]) < tol

@@ -64,3 +64,31 @@ def test_likelihood_adjoint_matches_reference_and_differences(orc, ref):
                oracle_likelihood_adjoint(orc, init, rows, 0.01, (q[0], q[1] - e), 7)[0]) / (2 * e)]
         np.testing.assert_allclose(g, fd, rtol=1e-6, atol=0)
     ref.dataset_free(h)
+
+
+def test_step_tangent_and_likelihood_gradient_match_reference(orc, ref):
+    """The restated step_tangent equals the reference's; the forward-
+    sensitivity gradient equals the reference's likelihood_gradient and the
+    adjoint gradient of the same discrete objective."""
+    from oracle_filters import (oracle_likelihood_adjoint, oracle_likelihood_gradient,
+                                oracle_step_tangent)
+    n, nu, alpha, dt = 32, 0.01, 0.1, 5e-3
+    w = orc.grf(n, 5)
+    T = [orc.grf(n, 6), orc.grf(n, 7)]
+    ws, Ts = w, T
+    for _ in range(3):
+        ws, Ts = oracle_step_tangent(orc, ws, Ts, nu, alpha, dt)
+    rw, rT = ref.step_tangent(w, np.stack(T), nu, alpha, dt, 3)
+    assert np.linalg.norm(ws - rw) < 1e-14 * np.linalg.norm(rw)
+    for j in range(2):
+        assert np.linalg.norm(Ts[j] - rT[j]) < 1e-13 * np.linalg.norm(rT[j])
+    h, rows = ref.synthetic(16, 0.01, 30, 10, 0.03, 0.1, 0.0, 0.5, 2, 3, 0.2, 987)
+    init = ref.initialize(16, "multimode")
+    for q in ((np.log(0.03), np.log(0.1)), (np.log(0.02), np.log(0.15))):
+        rv, rg = ref.likelihood_gradient(h, *q)
+        v, g = oracle_likelihood_gradient(orc, init, rows, 0.01, q)
+        assert abs(v - rv) < 1e-12 * abs(rv)
+        np.testing.assert_allclose(g, rg, rtol=1e-10, atol=0)
+        _, ga = oracle_likelihood_adjoint(orc, init, rows, 0.01, q, 7)
+        np.testing.assert_allclose(g, ga, rtol=1e-9, atol=0)
+    ref.dataset_free(h)
