@@ -390,6 +390,36 @@ std::vector<AdjointEvaluation> likelihood_adjoint(const Dataset &data,
     return out;
 }
 
+ValueGradient likelihood_gradient(const Dataset &data, const Vector &log_parameters) {
+    data.validate();
+    if (data.stochastic || log_parameters.size() != 2)
+        throw std::invalid_argument(
+            "tangent likelihood requires deterministic data and two parameters");
+    const std::size_t nobs = data.observations.size();
+    Parameters p;
+    p.forcing_sigma = 0;
+    p.forcing_min = data.forcing_min;
+    p.forcing_max = data.forcing_max;
+    Ensemble e(data.n, 1, p, Precision::fp64, 0);
+    std::vector<std::size_t> steps(nobs);
+    std::vector<int> x(nobs), y(nobs);
+    std::vector<double> value(nobs), sd(nobs);
+    for (std::size_t o = 0; o < nobs; ++o) {
+        const auto &ob = data.observations[o];
+        steps[o] = ob.step;
+        x[o] = (int)ob.x;
+        y[o] = (int)ob.y;
+        value[o] = ob.value;
+        sd[o] = ob.standard_deviation;
+    }
+    const Field init = initial_condition(data.n, "multimode");
+    double obj = 0, grad[2] = {0, 0};
+    check(hzl_likelihood_gradient(e.handle(), raw(init), data.dt, nobs, steps.data(), x.data(),
+                                  y.data(), value.data(), sd.data(), log_parameters.data(), &obj,
+                                  grad));
+    return ValueGradient{obj, {grad[0], grad[1]}};
+}
+
 AdjointEvaluation likelihood_adjoint(const Dataset &data, const Vector &log_parameters,
                                      std::size_t checkpoint_stride) {
     return likelihood_adjoint(data, std::vector<Vector>{log_parameters}, checkpoint_stride)[0];

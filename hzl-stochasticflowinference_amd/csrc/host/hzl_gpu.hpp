@@ -236,6 +236,9 @@ AdjointEvaluation likelihood_adjoint(const Dataset &data, const Vector &log_para
 std::vector<AdjointEvaluation> likelihood_adjoint(const Dataset &data,
                                                   const std::vector<Vector> &log_parameters,
                                                   std::size_t checkpoint_stride = 20);
+// The same objective with its gradient by forward sensitivities
+// (Solver::step_tangent; src/inference.cpp:340-382), on the GPU in FP64.
+ValueGradient likelihood_gradient(const Dataset &data, const Vector &log_parameters);
 
 // Independent members from the zero state, final diagnostics in member order
 // (src/inference.cpp:555-590); `workers` is accepted and unused.
