@@ -98,10 +98,26 @@ def build_emulator() -> str:
 
 
 def build_all() -> None:
-    build_engine()
+    # the CPU lane emulator (test infrastructure) compiles alongside the engine
+    import threading
+    errors = []
+
+    def emulator():
+        try:
+            build_emulator()
+        except Exception as e:  # reported after the engine build
+            errors.append(e)
+
+    t = threading.Thread(target=emulator)
+    t.start()
+    try:
+        build_engine()
+    finally:
+        t.join()
+    if errors:
+        raise errors[0]
     build_dropin()
     build_oracle()
-    build_emulator()
 
 
 if __name__ == "__main__":
