@@ -129,6 +129,61 @@ def cpu_baseline(n: int, nu: float, dt: float) -> dict:
             "kind": kind, "sample": what, "seconds": round(sec, 3)}
 
 
+def pmcmc_replay_64(pkg, chains: int = 4, particles: int = 8, samples: int = 6) -> dict:
+    """The reduced 64^2 FP64 replay of the particle-marginal Metropolis run:
+    the GPU chains (FP64) against the FP64 CPU replay of the reference's
+    particle_metropolis (tests/oracle_filters.py) on the same seeded window;
+    posterior mean / std of (nu, amplitude) over all chain states from both.
+    CPU-baseline leg (uses the oracle)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as orc_mod  # CPU baseline leg only
+    from oracle_filters import oracle_pmcmc
+    n, dt, window, every, nobs, sd, scale = 64, 1e-3, 60, 20, 32, 0.05, 0.2
+    rng = np.random.default_rng(SEED + 21)
+    truth = pkg.Ensemble(n, 1, "fp64", nu=1e-3, sigma=0.05, amp=0.1, seed=SEED + 22)
+    truth.init_grf()
+    init = truth.get_state()[0]
+    rows = []
+    for t in range(every, window + 1, every):
+        truth.step(dt, every)
+        ph = (np.fft.ifft2(truth.get_state()[0]) * n * n).real
+        xs, ys = rng.integers(0, n, nobs), rng.integers(0, n, nobs)
+        rows += [(t, x, y, ph[y, x] + sd * rng.standard_normal(), sd) for x, y in zip(xs, ys)]
+    truth.close()
+    rows = np.array(rows)
+    lower, upper = (np.log(2e-4), np.log(0.02)), (np.log(5e-3), np.log(0.5))
+    q0 = np.stack([np.log(rng.uniform(5e-4, 2e-3, chains)),
+                   np.log(rng.uniform(0.05, 0.2, chains))], 1)
+    with pkg.Ensemble(n, chains * particles, "fp64", sigma=0.05, seed=SEED + 23) as ens:
+        t0 = time.perf_counter()
+        got = pkg.particle_metropolis(ens, rows, dt, init, q0, samples, scale, SEED,
+                                      forcing_sigma=0.05)
+        gpu_s = time.perf_counter() - t0
+    orc = orc_mod.Oracle()
+    t0 = time.perf_counter()
+    want = oracle_pmcmc(orc, init, rows, dt, q0, samples, scale, SEED, lower, upper, particles,
+                        sigma=0.05)
+    cpu_s = time.perf_counter() - t0
+    cq = np.array([[r[0] for r in chain] for chain in want])  # [chain][sample][2]
+    cll = np.array([[r[1] for r in chain] for chain in want])
+    cacc = np.array([[r[3] for r in chain] for chain in want])
+
+    def summary(q):
+        v = np.exp(q.reshape(-1, 2))
+        return {"nu_mean": float(v[:, 0].mean()), "nu_std": float(v[:, 0].std()),
+                "amp_mean": float(v[:, 1].mean()), "amp_std": float(v[:, 1].std())}
+
+    return {"config": f"{chains} chains x {particles} particles at {n}x{n} FP64, {window}-step "
+                      f"window, {nobs} obs every {every} steps sd={sd}, {samples} Metropolis "
+                      "steps; CPU = FP64 oracle replay of particle_metropolis, one thread",
+            "gpu": summary(got["q"]), "cpu_reference": summary(cq),
+            "identical_decisions": bool(np.array_equal(got["accepted"], cacc)),
+            "max_rel_loglik_diff": float(np.max(np.abs(got["log_likelihood"] - cll) /
+                                                np.abs(cll))),
+            "gpu_seconds": round(gpu_s, 3), "cpu_seconds": round(cpu_s, 3)}
+
+
 def rms_error(mean_field, truth) -> float:
     """Grid RMS of (mean - truth) by Parseval on full spectra whose physical
     field is sum_k c_k e^{ik.x}."""
@@ -530,6 +585,7 @@ def main() -> int:
         line["uneven_pf"] = uneven_pf(pkg)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, nu, dt)
+        line["cpu_baseline"]["pmcmc_replay_64"] = pmcmc_replay_64(pkg)
     print(json.dumps(line), flush=True)
     return 0
 
