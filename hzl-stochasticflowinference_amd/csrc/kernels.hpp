@@ -54,11 +54,13 @@ template <> struct Shape<512> {
 };
 
 // Launch shape of the column and row kernels at grid N: EV values per lane
-// (an N = T x EV two-pass transform), CT threads per column workgroup and PPW
-// row pairs per row workgroup (0: the default).  Shape<N> gives the default;
-// other shapes are variants the autotuner times.
-template <int N, int EV = Shape<N>::E, int CT = 64, int PPW = 0> struct Geometry {
+// (an N = T x EV two-pass transform), CT threads per column workgroup, PPW
+// row pairs per row workgroup (0: the default), and NT: stream the mixed and
+// nonlinear fields with non-temporal loads and stores.  Shape<N> gives the
+// default; other shapes are variants the autotuner times.
+template <int N, int EV = Shape<N>::E, int CT = 64, int PPW = 0, int NT = 0> struct Geometry {
     static constexpr int NN = N;
+    static constexpr bool STREAM = NT != 0;
     static constexpr int E = EV;
     static constexpr int T = N / E;
     static constexpr int K = (N - 1) / 3;
@@ -99,6 +101,32 @@ template <class FFT, typename R> HZL_HD TwArg<FFT, R> lane_twiddles(const cx<R> 
         return FFT::load_twiddles(tw, t);
     else
         return tw;
+}
+
+// Loads and stores of fields that are written once and read once per stage
+// (mixed and nonlinear fields): with STREAM, non-temporal hints so they do not
+// displace cached data (the column pass's mixed-field stores, whole 128-byte
+// lines per group, and the loads of both passes).
+template <bool STREAM, typename R> HZL_HD void stream_store(cx<R> *p, cx<R> v) {
+#if __HIP_DEVICE_COMPILE__
+    if constexpr (STREAM) {
+        typedef R v2 __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(v2{v.x, v.y}, reinterpret_cast<v2 *>(p));
+        return;
+    }
+#endif
+    *p = v;
+}
+
+template <bool STREAM, typename R> HZL_HD cx<R> stream_load(const cx<R> *p) {
+#if __HIP_DEVICE_COMPILE__
+    if constexpr (STREAM) {
+        typedef R v2 __attribute__((ext_vector_type(2)));
+        const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+        return {x.x, x.y};
+    }
+#endif
+    return *p;
 }
 
 HZL_HD long tiled(int KP, int c, int y) {
@@ -319,7 +347,7 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
         const cx<R> *nl = a.nl + (long)m * G::MIXF;
 #pragma unroll
         for (int j = 0; j < E; ++j)
-            v[j] = live ? nl[tiled(KP, c, t + T * j)] : cx<R>{0, 0};
+            v[j] = live ? stream_load<G::STREAM>(nl + tiled(KP, c, t + T * j)) : cx<R>{0, 0};
         // All state loads of the column are issued together (one memory round
         // trip for the column, not one per slot).  Stage 2 rebuilds
         // w2 = w_n + dt/2 k1 from acc = k1 exactly as stage 1 formed it, so
@@ -444,7 +472,7 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
         if (live) {
 #pragma unroll
             for (int n2 = 0; n2 < E; ++n2)
-                mix[f * G::MIXF + tiled(KP, c, t + T * n2)] = u[n2];
+                stream_store<G::STREAM>(mix + f * G::MIXF + tiled(KP, c, t + T * n2), u[n2]);
         }
     }
 }
@@ -498,8 +526,9 @@ template <class GEO, typename R> struct RowTile {
             if (e < ELEMS && grow < total_rows) {
                 const long mm = grow / N;
                 const int yy = (int)(grow % N);
-                v[it] = *reinterpret_cast<const pair2 *>(a.mix + (mm * 4 + f) * G::MIXF +
-                                                         (long)(yy >> 4) * PER_TILE + rem);
+                const cx<R> *src = a.mix + (mm * 4 + f) * G::MIXF + (long)(yy >> 4) * PER_TILE + rem;
+                v[it].u = stream_load<G::STREAM>(src);
+                v[it].v = stream_load<G::STREAM>(src + 1);
             }
         }
     }
@@ -627,6 +656,9 @@ HZL_HD void row_finish(const Ctx &ctx, const RowArgs<R> &a, cx<R> *L, int t, con
             const cx<R> p0 = {R(0.5) * (qa.x + qb.x), R(0.5) * (qa.y - qb.y)};
             const cx<R> p1 = {R(0.5) * (qa.y + qb.y), R(0.5) * (qb.x - qa.x)};
             const long o = tiled(KP, k, y0);
+            // ordinary stores: a lane writes 16 bytes of a 128-byte line whose other
+            // parts come from the other row pairs, which L2 merges (non-temporal
+            // stores measured 1.05x the bytes here)
             nl[o] = p0;
             nl[o + 1] = p1;
         }

@@ -84,8 +84,8 @@ int default_elems(int n) {
 } // namespace
 
 uint64_t variant_table_id() {
-#define HZL_X(NN, IDX, E, A, W) #NN "," #IDX "," #E "," #A "," #W ";"
-#define HZL_Y(NN, IDX, E, A, W, D) #NN "," #IDX "," #E "," #A "," #W "," #D ";"
+#define HZL_X(NN, IDX, E, A, W, S) #NN "," #IDX "," #E "," #A "," #W "," #S ";"
+#define HZL_Y(NN, IDX, E, A, W, D, S) #NN "," #IDX "," #E "," #A "," #W "," #D "," #S ";"
     static const char table[] = "col:" HZL_COL_VARIANTS(HZL_X) "row:" HZL_ROW_VARIANTS(HZL_Y);
 #undef HZL_X
 #undef HZL_Y
@@ -97,7 +97,7 @@ uint64_t variant_table_id() {
 
 int col_variant_count(int n) {
     int c = 1;
-#define HZL_X(NN, IDX, E, CT, W) c += n == NN;
+#define HZL_X(NN, IDX, E, CT, W, S) c += n == NN;
     HZL_COL_VARIANTS(HZL_X)
 #undef HZL_X
     return c;
@@ -105,7 +105,7 @@ int col_variant_count(int n) {
 
 int row_variant_count(int n) {
     int c = 1;
-#define HZL_X(NN, IDX, E, PPW, W, D) c += n == NN;
+#define HZL_X(NN, IDX, E, PPW, W, D, S) c += n == NN;
     HZL_ROW_VARIANTS(HZL_X)
 #undef HZL_X
     return c;
@@ -114,7 +114,7 @@ int row_variant_count(int n) {
 int col_variant_elems(int n, int v) {
     if (v == 0)
         return default_elems(n);
-#define HZL_X(NN, IDX, E, CT, W)                                                               \
+#define HZL_X(NN, IDX, E, CT, W, S)                                                            \
     if (n == NN && v == IDX)                                                                   \
         return E;
     HZL_COL_VARIANTS(HZL_X)
@@ -125,7 +125,7 @@ int col_variant_elems(int n, int v) {
 int row_variant_elems(int n, int v) {
     if (v == 0)
         return default_elems(n);
-#define HZL_X(NN, IDX, E, PPW, W, D)                                                           \
+#define HZL_X(NN, IDX, E, PPW, W, D, S)                                                        \
     if (n == NN && v == IDX)                                                                   \
         return E;
     HZL_ROW_VARIANTS(HZL_X)
@@ -134,9 +134,9 @@ int row_variant_elems(int n, int v) {
 }
 
 bool launch_col_variant(int n, int v, int mode, const StageArgs<float> &a, hipStream_t s) {
-#define HZL_X(NN, IDX, E, CT, W)                                                               \
+#define HZL_X(NN, IDX, E, CT, W, S)                                                            \
     if (n == NN && v == IDX) {                                                                 \
-        col_v<NN, Geometry<NN, E, CT>, W>(mode, a, s);                                         \
+        col_v<NN, Geometry<NN, E, CT, 0, S>, W>(mode, a, s);                                   \
         return true;                                                                           \
     }
     HZL_COL_VARIANTS(HZL_X)
@@ -145,9 +145,9 @@ bool launch_col_variant(int n, int v, int mode, const StageArgs<float> &a, hipSt
 }
 
 bool launch_row_variant(int n, int v, const RowArgs<float> &a, hipStream_t s) {
-#define HZL_X(NN, IDX, E, PPW, W, D)                                                           \
+#define HZL_X(NN, IDX, E, PPW, W, D, S)                                                        \
     if (n == NN && v == IDX) {                                                                 \
-        row_v<NN, Geometry<NN, E, 64, PPW>, W, D>(a, s);                                       \
+        row_v<NN, Geometry<NN, E, 64, PPW, S>, W, D>(a, s);                                    \
         return true;                                                                           \
     }
     HZL_ROW_VARIANTS(HZL_X)

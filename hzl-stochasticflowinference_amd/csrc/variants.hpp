@@ -3,9 +3,9 @@
 // register budgets of engine.hip).
 //
 // A column variant is (values per lane E, i.e. the T x E split of the y
-// transform; lanes per workgroup; waves-per-EU hint).  A row variant is (E of
-// the x transforms; row pairs per workgroup; waves-per-EU hint; staged or
-// direct spectra).  Every
+// transform; lanes per workgroup; waves-per-EU hint; non-temporal streams).
+// A row variant is (E of the x transforms; row pairs per workgroup;
+// waves-per-EU hint; staged or direct spectra; non-temporal streams).  Every
 // variant computes the same stage; only the FFT factorisation (and so the
 // FP32 rounding) and the scheduling differ.
 #pragma once
@@ -13,33 +13,40 @@
 #include <cstdint>
 #include <hip/hip_runtime.h>
 
-// X(N, index, E, lanes per workgroup, waves per EU)
+// X(N, index, E, lanes per workgroup, waves per EU, non-temporal streams)
 #define HZL_COL_VARIANTS(X)                                                                    \
-    X(128, 1, 16, 128, 1)                                                                       \
-    X(128, 2, 32, 64, 1)                                                                        \
-    X(128, 3, 16, 64, 2)                                                                        \
-    X(256, 1, 16, 128, 1)                                                                       \
-    X(256, 2, 32, 64, 1)                                                                        \
-    X(256, 3, 16, 64, 2)                                                                        \
-    X(512, 1, 32, 128, 1)                                                                       \
-    X(512, 2, 32, 64, 2)
+    X(128, 1, 16, 128, 1, 0)                                                                    \
+    X(128, 2, 32, 64, 1, 0)                                                                     \
+    X(128, 3, 16, 64, 2, 0)                                                                     \
+    X(128, 4, 16, 64, 1, 1)                                                                     \
+    X(256, 1, 16, 128, 1, 0)                                                                    \
+    X(256, 2, 32, 64, 1, 0)                                                                     \
+    X(256, 3, 16, 64, 2, 0)                                                                     \
+    X(256, 4, 16, 64, 1, 1)                                                                     \
+    X(256, 5, 16, 128, 1, 1)                                                                    \
+    X(512, 1, 32, 128, 1, 0)                                                                    \
+    X(512, 2, 32, 64, 2, 0)                                                                     \
+    X(512, 3, 32, 64, 1, 1)
 
-// X(N, index, E, row pairs per workgroup, waves per EU, direct): direct = 1
-// builds the row spectra straight from HBM (row_body_direct) instead of
-// staging the tiles through LDS.
+// X(N, index, E, row pairs per workgroup, waves per EU, direct, non-temporal
+// streams): direct = 1 builds the row spectra straight from HBM
+// (row_body_direct) instead of staging the tiles through LDS.
 #define HZL_ROW_VARIANTS(X)                                                                    \
-    X(128, 1, 16, 16, 2, 0)                                                                     \
-    X(128, 2, 32, 16, 2, 0)                                                                     \
-    X(128, 3, 16, 8, 3, 0)                                                                      \
-    X(128, 4, 16, 8, 1, 1)                                                                      \
-    X(256, 1, 16, 8, 3, 0)                                                                      \
-    X(256, 2, 16, 8, 4, 0)                                                                      \
-    X(256, 3, 16, 16, 2, 0)                                                                     \
-    X(256, 4, 32, 8, 2, 0)                                                                      \
-    X(256, 5, 16, 8, 1, 1)                                                                      \
-    X(256, 6, 16, 8, 2, 1)                                                                      \
-    X(512, 1, 32, 8, 3, 0)                                                                      \
-    X(512, 2, 32, 8, 1, 1)
+    X(128, 1, 16, 16, 2, 0, 0)                                                                  \
+    X(128, 2, 32, 16, 2, 0, 0)                                                                  \
+    X(128, 3, 16, 8, 3, 0, 0)                                                                   \
+    X(128, 4, 16, 8, 1, 1, 0)                                                                   \
+    X(128, 5, 16, 8, 2, 0, 1)                                                                   \
+    X(256, 1, 16, 8, 3, 0, 0)                                                                   \
+    X(256, 2, 16, 8, 4, 0, 0)                                                                   \
+    X(256, 3, 16, 16, 2, 0, 0)                                                                  \
+    X(256, 4, 32, 8, 2, 0, 0)                                                                   \
+    X(256, 5, 16, 8, 1, 1, 0)                                                                   \
+    X(256, 6, 16, 8, 2, 1, 0)                                                                   \
+    X(256, 7, 16, 8, 2, 0, 1)                                                                   \
+    X(512, 1, 32, 8, 3, 0, 0)                                                                   \
+    X(512, 2, 32, 8, 1, 1, 0)                                                                   \
+    X(512, 3, 32, 8, 2, 0, 1)
 
 namespace hzl_amd {
 

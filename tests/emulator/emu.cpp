@@ -568,18 +568,18 @@ int emu_run_variant(int n, int cv, int rv, int B, double nu, double alpha, doubl
                     double amp, int kf, int kmin, int kmax, uint64_t seed, double dt, int steps,
                     int grf, const double *init, double *out, double *diag) {
     if (rv == 0) {
-#define HZL_X(NN, IDX, E, CT, W)                                                               \
+#define HZL_X(NN, IDX, E, CT, W, S)                                                            \
     if (n == NN && cv == IDX)                                                                  \
-        return run<NN, float, Geometry<NN, E, CT>, Geometry<NN>>(                              \
+        return run<NN, float, Geometry<NN, E, CT, 0, S>, Geometry<NN>>(                        \
             B, nu, alpha, sigma, amp, kf, kmin, kmax, seed, dt, steps, grf, init, nullptr, 1, out, \
             diag);
         HZL_COL_VARIANTS(HZL_X)
 #undef HZL_X
     }
     if (cv == 0) {
-#define HZL_X(NN, IDX, E, PPW, W, D)                                                           \
+#define HZL_X(NN, IDX, E, PPW, W, D, S)                                                        \
     if (n == NN && rv == IDX)                                                                  \
-        return run<NN, float, Geometry<NN>, Geometry<NN, E, 64, PPW>, (D != 0)>(               \
+        return run<NN, float, Geometry<NN>, Geometry<NN, E, 64, PPW, S>, (D != 0)>(            \
             B, nu, alpha, sigma, amp, kf, kmin, kmax, seed, dt, steps, grf, init, nullptr, 1, out, \
             diag);
         HZL_ROW_VARIANTS(HZL_X)
