@@ -57,7 +57,9 @@ template <> struct Shape<512> {
 // (an N = T x EV two-pass transform), CT threads per column workgroup, PPW
 // row pairs per row workgroup (0: the default), and NT: stream the mixed and
 // nonlinear fields with non-temporal loads and stores.  Shape<N> gives the
-// default; other shapes are variants the autotuner times.
+// default; other shapes are variants the autotuner times.  (Streaming the RK4
+// state, stage and accumulator arrays as well was measured 30% slower: later
+// stages re-read them from cache.)
 template <int N, int EV = Shape<N>::E, int CT = 64, int PPW = 0, int NT = 0> struct Geometry {
     static constexpr int NN = N;
     static constexpr bool STREAM = NT != 0;

@@ -24,6 +24,7 @@
     X(256, 3, 16, 64, 2, 0)                                                                     \
     X(256, 4, 16, 64, 1, 1)                                                                     \
     X(256, 5, 16, 128, 1, 1)                                                                    \
+    X(256, 6, 16, 256, 1, 1)                                                                    \
     X(512, 1, 32, 128, 1, 0)                                                                    \
     X(512, 2, 32, 64, 2, 0)                                                                     \
     X(512, 3, 32, 64, 1, 1)
@@ -44,6 +45,7 @@
     X(256, 5, 16, 8, 1, 1, 0)                                                                   \
     X(256, 6, 16, 8, 2, 1, 0)                                                                   \
     X(256, 7, 16, 8, 2, 0, 1)                                                                   \
+    X(256, 8, 16, 16, 2, 0, 1)                                                                  \
     X(512, 1, 32, 8, 3, 0, 0)                                                                   \
     X(512, 2, 32, 8, 1, 1, 0)                                                                   \
     X(512, 3, 32, 8, 2, 0, 1)
