@@ -55,14 +55,18 @@ template <> struct Shape<512> {
 
 // Launch shape of the column and row kernels at grid N: EV values per lane
 // (an N = T x EV two-pass transform), CT threads per column workgroup, PPW
-// row pairs per row workgroup (0: the default), and NT: stream the mixed and
-// nonlinear fields with non-temporal loads and stores.  Shape<N> gives the
+// row pairs per row workgroup (0: the default), and NT: bit 0 streams the
+// mixed and nonlinear fields with non-temporal loads and stores; bit 1 keeps
+// twiddles in the table (L1) rather than in registers where that brings a
+// kernel under the 168-register budget of three waves per SIMD: the column
+// stages that hold three state arrays (RK4 stages 3 and 4) and the row pass.  Shape<N> gives the
 // default; other shapes are variants the autotuner times.  (Streaming the RK4
 // state, stage and accumulator arrays as well was measured 30% slower: later
 // stages re-read them from cache.)
 template <int N, int EV = Shape<N>::E, int CT = 64, int PPW = 0, int NT = 0> struct Geometry {
     static constexpr int NN = N;
-    static constexpr bool STREAM = NT != 0;
+    static constexpr bool STREAM = (NT & 1) != 0;
+    static constexpr bool TABLE_TW = (NT & 2) != 0;
     static constexpr int E = EV;
     static constexpr int T = N / E;
     static constexpr int K = (N - 1) / 3;
@@ -93,13 +97,14 @@ HZL_HD float recip(float x) {
 }
 
 // Twiddle argument of a group's transforms: the lane's twiddles in registers
-// when the split is square (E == T: one set serves forward and inverse), else
-// the table itself.
-template <class FFT, typename R>
-using TwArg = std::conditional_t<FFT::Q == 1, typename FFT::Twiddles, const cx<R> *>;
+// when the split is square (E == T: one set serves forward and inverse) and
+// REG allows it, else the table itself.
+template <class FFT, typename R, bool REG = true>
+using TwArg = std::conditional_t<FFT::Q == 1 && REG, typename FFT::Twiddles, const cx<R> *>;
 
-template <class FFT, typename R> HZL_HD TwArg<FFT, R> lane_twiddles(const cx<R> *tw, int t) {
-    if constexpr (FFT::Q == 1)
+template <class FFT, typename R, bool REG = true>
+HZL_HD TwArg<FFT, R, REG> lane_twiddles(const cx<R> *tw, int t) {
+    if constexpr (FFT::Q == 1 && REG)
         return FFT::load_twiddles(tw, t);
     else
         return tw;
@@ -282,7 +287,8 @@ HZL_HD void col_body(const Ctx &ctx, const StageArgs<R> &a, cx<R> *lds, long blo
     const int m = live ? (int)(gc / KP) : 0, c = live ? (int)(gc % KP) : 0;
     cx<R> *L = lds + g * FFT::LDS_ELEMS;
     const long soff = (long)m * G::SPEC + (long)c * J;
-    const TwArg<FFT, R> twa = lane_twiddles<FFT>(a.tw, t);
+    constexpr bool REG_TW = !(G::TABLE_TW && (MODE == 3 || MODE == 4));
+    const TwArg<FFT, R, REG_TW> twa = lane_twiddles<FFT, R, REG_TW>(a.tw, t);
     cx<R> v[E];
     cx<R> vb[MODE == 8 ? E : 1]; // the w-source state of mode 8
 
@@ -699,7 +705,7 @@ HZL_HD void row_body(const Ctx &ctx, const RowArgs<R> &a, cx<R> *lds, long block
     const int ly = 2 * g; // local row of y0
     cx<R> *S0 = lds, *S1 = lds + RS::CHUNK;
     cx<R> *L = lds + g * FFT::LDS_ELEMS;
-    const TwArg<FFT, R> twa = lane_twiddles<FFT>(a.tw, t);
+    const TwArg<FFT, R, !G::TABLE_TW> twa = lane_twiddles<FFT, R, !G::TABLE_TW>(a.tw, t);
 
     R pa[E], pb[E], ua[CFL ? E : 1], ub[CFL ? E : 1];
     R vmax = R(0);

@@ -13,7 +13,8 @@
 #include <cstdint>
 #include <hip/hip_runtime.h>
 
-// X(N, index, E, lanes per workgroup, waves per EU, non-temporal streams)
+// X(N, index, E, lanes per workgroup, waves per EU, NT flags of Geometry: 1 streams the
+// mixed and nonlinear fields, 2 reads the twiddles of RK4 stages 3-4 from the table)
 #define HZL_COL_VARIANTS(X)                                                                    \
     X(128, 1, 16, 128, 1, 0)                                                                    \
     X(128, 2, 32, 64, 1, 0)                                                                     \
@@ -25,13 +26,16 @@
     X(256, 4, 16, 64, 1, 1)                                                                     \
     X(256, 5, 16, 128, 1, 1)                                                                    \
     X(256, 6, 16, 256, 1, 1)                                                                    \
+    X(256, 7, 16, 256, 1, 3)                                                                    \
+    X(256, 8, 16, 128, 1, 3)                                                                    \
     X(512, 1, 32, 128, 1, 0)                                                                    \
     X(512, 2, 32, 64, 2, 0)                                                                     \
     X(512, 3, 32, 64, 1, 1)
 
-// X(N, index, E, row pairs per workgroup, waves per EU, direct, non-temporal
-// streams): direct = 1 builds the row spectra straight from HBM
-// (row_body_direct) instead of staging the tiles through LDS.
+// X(N, index, E, row pairs per workgroup, waves per EU, direct, NT flags of Geometry (1:
+// non-temporal loads of the mixed fields, 2: twiddles from the table)): direct = 1 builds
+// the row spectra straight from HBM (row_body_direct) instead of staging the tiles
+// through LDS.
 #define HZL_ROW_VARIANTS(X)                                                                    \
     X(128, 1, 16, 16, 2, 0, 0)                                                                  \
     X(128, 2, 32, 16, 2, 0, 0)                                                                  \
@@ -46,6 +50,8 @@
     X(256, 6, 16, 8, 2, 1, 0)                                                                   \
     X(256, 7, 16, 8, 2, 0, 1)                                                                   \
     X(256, 8, 16, 16, 2, 0, 1)                                                                  \
+    X(256, 9, 16, 8, 3, 0, 2)                                                                   \
+    X(256, 10, 16, 8, 3, 0, 3)                                                                  \
     X(512, 1, 32, 8, 3, 0, 0)                                                                   \
     X(512, 2, 32, 8, 1, 1, 0)                                                                   \
     X(512, 3, 32, 8, 2, 0, 1)

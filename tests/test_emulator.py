@@ -52,7 +52,8 @@ def test_forced_pipeline_matches_oracle(emu, orc):
 
 @pytest.mark.parametrize("n,cv,rv", [(128, 1, 0), (128, 2, 0), (128, 3, 0), (128, 0, 1),
                                      (128, 0, 2), (128, 0, 3), (256, 2, 0), (256, 0, 3),
-                                     (256, 0, 4), (128, 0, 4), (256, 0, 5)])
+                                     (256, 0, 4), (128, 0, 4), (256, 0, 5),
+                                     (256, 7, 0), (256, 0, 9), (256, 0, 10)])
 def test_launch_shape_variants_match_oracle(emu, orc, n, cv, rv):
     """Autotuner variants (other T x E splits, workgroup sizes, row tiles)
     compute the same RK4 step as the FP64 oracle."""
