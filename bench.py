@@ -60,6 +60,37 @@ def measured_traffic(kernel: str, n: int, members: int):
     return None
 
 
+def stream_ceiling() -> int:
+    """Sustained HBM bandwidth of plain streaming kernels on this device (torch's
+    copy, fill and sum over 2 GiB buffers, 10 timed launches each with HIP
+    events): the practical ceiling the engine's kernels compare against."""
+    import torch
+    n = 1 << 29  # float32 elements = 2 GiB
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    out = {}
+
+    def timed(fn, nbytes):
+        for _ in range(2):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(10):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        sec = a.elapsed_time(b) / 1e3 / 10
+        return round(nbytes / sec / 1e9, 1)
+
+    out["copy_GBps"] = timed(lambda: y.copy_(x), 2 * 4 * n)
+    out["fill_GBps"] = timed(lambda: y.fill_(2.0), 4 * n)
+    out["sum_GBps"] = timed(lambda: x.sum(), 4 * n)
+    out["peak_GBps"] = HBM_PEAK / 1e9
+    print(json.dumps({"stream_ceiling": out}))
+    return 0
+
+
 def calibrate_copy() -> int:
     """A plain device copy of known size (1 GiB, 3 dispatches) for scaling the
     FETCH_SIZE / WRITE_SIZE counters."""
@@ -468,11 +499,15 @@ def main() -> int:
                     help="default launch shapes only (no autotuning; for profiling)")
     ap.add_argument("--tuning", default=None,
                     help="force launch-shape variants 'col,row' (no timing; for profiling)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="only time plain copy / fill / sum kernels (practical HBM ceiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.stream_ceiling:
+        return stream_ceiling()
     if args.no_tune:
         os.environ["HZL_AUTOTUNE"] = "0"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

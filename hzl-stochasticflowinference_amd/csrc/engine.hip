@@ -50,6 +50,13 @@ template <typename R> __global__ void __launch_bounds__(256) k_noise(NoiseArgs<R
     noise_body<R>(a, (long)blockIdx.x * 256 + threadIdx.x);
 }
 
+// One 64-lane workgroup per member (grid = members).
+template <typename R> __global__ void __launch_bounds__(OU_LANES) k_ou(OuArgs<R> a) {
+    __shared__ uint64_t s[MT_N];
+    __shared__ int pos;
+    ou_body<R>(DevCtx{}, a, s, &pos, (int)blockIdx.x);
+}
+
 template <int N, typename R, bool CFL>
 __global__ void __launch_bounds__(Geometry<N>::ROW_THREADS)
     __attribute__((amdgpu_waves_per_eu(HZL_ROW_WPE))) k_row(RowArgs<R> a) {
@@ -664,16 +671,22 @@ template <typename R> class Engine final : public EngineBase {
         flush_parameters();
         for (size_t s = 0; s < steps; ++s) {
             ensure_mixed(dt);
-            NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_, noise_.p};
-            const long count = (long)B_ * npairs_;
-            hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
-                               stream_, na);
+            if (ou_) {
+                launch_ou(1, dt / 2, noise_.p);
+            } else {
+                NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_, noise_.p};
+                const long count = (long)B_ * npairs_;
+                hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256),
+                                   0, stream_, na);
+            }
             RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_row_, nullptr};
             for (int stage = 1; stage <= 3; ++stage) {
                 r.cfl = stage == 1 ? cfl_.p : nullptr;
                 run_row(r);
                 run_col(4 + stage, stage_args(dt, nullptr, 0));
             }
+            if (ou_)
+                launch_ou(1, dt / 2, nullptr);
             check(hipGetLastError(), "SSPRK3 kernels");
             ++step_;
             last_dt_ = dt;
@@ -1233,9 +1246,12 @@ template <typename R> class Engine final : public EngineBase {
         RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_row_, cfl_diag_.p};
         run_row(r);
         DeviceBuffer<double> d(B_ * 6);
-        // the noise buffer still holds the last step's shell noise
-        DiagArgs<R> a{K_, w0_.p, d_nu_.p, forcing_model(), step_ > 0 ? 1 : 0, cfl_diag_.p,
-                      n_ / 2, n_, d.p};
+        // white forcing: the noise buffer still holds the last step's shell
+        // noise; OU: the current coefficients (Solver::forcing(), src/solver.cpp:389)
+        if (ou_)
+            launch_ou(3, 0.0, noise_.p);
+        DiagArgs<R> a{K_, w0_.p, d_nu_.p, forcing_model(), (ou_ || step_ > 0) ? 1 : 0,
+                      cfl_diag_.p, n_ / 2, n_, d.p};
         hipLaunchKernelGGL(k_diag<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
         check(hipGetLastError(), "diagnostics kernel");
         check(hipMemcpyAsync(out, d.p, B_ * 6 * sizeof(double), hipMemcpyDeviceToHost, stream_),
@@ -1306,6 +1322,13 @@ template <typename R> class Engine final : public EngineBase {
                 for (size_t i = 0; i < P; ++i)
                     keys_[f * P + i] = stream_seed(seeds[f], i);
             upload(d_keys_.p, keys_.data(), B_);
+            if (ou_) {
+                // new particles: Solver(n, p, stream_seed(seed_f, i)) draws its
+                // stationary OU start (src/solver.cpp:120-139)
+                flush_parameters();
+                launch_ou(0, 0.0, nullptr);
+                check(hipGetLastError(), "OU start");
+            }
         }
         for (int j = 0; j < n_; ++j) {
             cs[j] = std::cos(2 * PI * j / n_);
@@ -1802,11 +1825,15 @@ template <typename R> class Engine final : public EngineBase {
     void launch_step(double dt, const cx<R> *ext, long long ext_stride,
                      const uint64_t *step_base, uint64_t step_off, hipEvent_t *events) {
         if (!ext) {
-            NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_off, noise_.p,
-                            step_base};
-            const long count = (long)B_ * npairs_;
-            hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
-                               stream_, na);
+            if (ou_) {
+                launch_ou(1, dt / 2, noise_.p);
+            } else {
+                NoiseArgs<R> na{(int)B_, npairs_, d_sigma_.p, d_keys_.p, dt, step_off, noise_.p,
+                                step_base};
+                const long count = (long)B_ * npairs_;
+                hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((count + 255) / 256)), dim3(256),
+                                   0, stream_, na);
+            }
         }
         RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_row_, nullptr};
         if (events)
@@ -1820,6 +1847,8 @@ template <typename R> class Engine final : public EngineBase {
             if (events)
                 check(hipEventRecord(events[2 * stage], stream_), "event");
         }
+        if (!ext && ou_)
+            launch_ou(1, dt / 2, nullptr);
         check(hipGetLastError(), "RK4 kernels");
     }
 
@@ -1865,6 +1894,126 @@ template <typename R> class Engine final : public EngineBase {
                            (const double *)pf_stats_.p, P);
         hipLaunchKernelGGL(k_pf_commit<R>, grid, dim3(256), 0, stream_, (int)B_, (long)spec_,
                            (const cx<R> *)w1_.p, w0_.p, (const double *)pf_stats_.p, P);
+        if (ou_) {
+            // children keep their ancestor's OU coefficients and restart their
+            // stream from the key k_pf_normalize gave them (Solver copy +
+            // reseed_future, src/filter.cpp:112-115)
+            const long fc = (long)B_ * npairs_;
+            const dim3 fgrid((unsigned)((fc + 255) / 256));
+            hipLaunchKernelGGL(k_pf_gather<double>, fgrid, dim3(256), 0, stream_, (int)B_,
+                               (long)npairs_, (const cx<double> *)ou_F_.p, ou_tmp_.p,
+                               (const int *)pf_anc_.p, (const double *)pf_stats_.p, P);
+            hipLaunchKernelGGL(k_pf_commit<double>, fgrid, dim3(256), 0, stream_, (int)B_,
+                               (long)npairs_, (const cx<double> *)ou_tmp_.p, ou_F_.p,
+                               (const double *)pf_stats_.p, P);
+            launch_ou(2, 0.0, nullptr, pf_stats_.p, P);
+        }
+    }
+
+    // One call of the OU kernel over every member (modes of OuArgs).
+    void launch_ou(int mode, double h, cx<R> *noise, const double *stats = nullptr, int P = 1) {
+        OuArgs<R> a{(int)B_,   npairs_,     ou_mt_.p, ou_F_.p, ou_map_.p, d_sigma_.p,
+                    d_keys_.p, p_.forcing_tau, h,     mode,    noise,     stats,
+                    P};
+        hipLaunchKernelGGL(k_ou<R>, dim3((unsigned)B_), dim3(OU_LANES), 0, stream_, a);
+    }
+
+    void set_forcing_process(int process) override {
+        if (process != HZL_FORCING_WHITE && process != HZL_FORCING_OU)
+            throw std::invalid_argument("forcing process must be HZL_FORCING_WHITE or HZL_FORCING_OU");
+        check(hipStreamSynchronize(stream_), "forcing process switch");
+        clear_graphs(); // captured steps hold the previous process's kernels
+        ou_ = process == HZL_FORCING_OU;
+        if (!ou_)
+            return;
+        if (!(p_.forcing_tau > 0) || !std::isfinite(p_.forcing_tau))
+            throw std::invalid_argument("OU forcing needs a finite forcing_tau > 0");
+        if (!ou_mt_.count) {
+            // forcing_pairs (src/solver.cpp:67-83): first mode i = y n + x of
+            // each conjugate pair, i < partner, inside the mask and the shell
+            const int n = n_, w = 2 * p_.forcing_max + 1;
+            const auto table = pair_table(p_.forcing_min, p_.forcing_max);
+            std::vector<int> map;
+            ou_first_.clear();
+            for (int y = 0; y < n; ++y)
+                for (int x = 0; x < n; ++x) {
+                    const int a = x < n / 2 ? x : x - n, b = y < n / 2 ? y : y - n;
+                    const long i = (long)y * n + x,
+                               partner = (long)((n - y) % n) * n + (n - x) % n;
+                    const int k2 = a * a + b * b;
+                    if (3 * std::abs(a) < n && 3 * std::abs(b) < n && i < partner &&
+                        k2 >= p_.forcing_min * p_.forcing_min &&
+                        k2 <= p_.forcing_max * p_.forcing_max) {
+                        const bool flip = a < 0;
+                        const int c = flip ? -a : a, kb = flip ? -b : b;
+                        const int p = table[(size_t)(c * w + kb + p_.forcing_max)];
+                        if (p < 0)
+                            throw std::logic_error("OU pair outside the engine's shell table");
+                        map.push_back(flip ? ~p : p);
+                        ou_first_.push_back(i);
+                    }
+                }
+            if ((int)map.size() != npairs_)
+                throw std::logic_error("OU pair count differs from the engine's shell");
+            ou_map_ = DeviceBuffer<int>(map.size());
+            upload(ou_map_.p, map.data(), map.size());
+            ou_mt_ = DeviceBuffer<uint64_t>((size_t)(MT_N + 1) * B_);
+            ou_F_ = DeviceBuffer<cx<double>>(B_ * (size_t)npairs_);
+            ou_tmp_ = DeviceBuffer<cx<double>>(B_ * (size_t)npairs_);
+        }
+        flush_parameters();
+        launch_ou(0, 0.0, nullptr);
+        check(hipGetLastError(), "OU start");
+    }
+
+    void reseed_future(size_t m, uint64_t seed) override {
+        member_check(m);
+        keys_[m] = seed;
+        upload(d_keys_.p + m, &keys_[m], 1);
+        if (!ou_)
+            return;
+        // mt19937_64 seeding ([rand.eng.mers]), position at the end of the
+        // block so the first draw regenerates it; words strided by members
+        std::vector<uint64_t> s(MT_N + 1);
+        s[0] = seed;
+        for (int i = 1; i < MT_N; ++i)
+            s[i] = 6364136223846793005ULL * (s[i - 1] ^ (s[i - 1] >> 62)) + (uint64_t)i;
+        s[MT_N] = MT_N;
+        check(hipMemcpy2DAsync(ou_mt_.p + m, B_ * sizeof(uint64_t), s.data(), sizeof(uint64_t),
+                               sizeof(uint64_t), MT_N + 1, hipMemcpyHostToDevice, stream_),
+              "stream reseed");
+        check(hipStreamSynchronize(stream_), "stream reseed");
+    }
+
+    int forcing_process() const override {
+        return ou_ ? HZL_FORCING_OU : HZL_FORCING_WHITE;
+    }
+
+    void get_forcing(size_t m, double *field) override {
+        if (!ou_)
+            throw std::invalid_argument("the OU forcing process is not selected");
+        const size_t first = m == (size_t)-1 ? 0 : m, count = m == (size_t)-1 ? B_ : 1;
+        if (m != (size_t)-1)
+            member_check(m);
+        std::vector<cx<double>> F((size_t)npairs_ * count);
+        check(hipMemcpyAsync(F.data(), ou_F_.p + first * npairs_, F.size() * sizeof(cx<double>),
+                             hipMemcpyDeviceToHost, stream_),
+              "forcing download");
+        check(hipStreamSynchronize(stream_), "forcing download");
+        const size_t nn = (size_t)n_ * n_;
+        for (size_t k = 0; k < count; ++k) {
+            double *f = field + 2 * nn * k;
+            std::fill(f, f + 2 * nn, 0.0);
+            for (int j = 0; j < npairs_; ++j) {
+                const long i = ou_first_[j], y = i / n_, x = i % n_;
+                const long partner = ((n_ - y) % n_) * n_ + (n_ - x) % n_;
+                const cx<double> v = F[k * npairs_ + j];
+                f[2 * i] = v.x;
+                f[2 * i + 1] = v.y;
+                f[2 * partner] = v.x;
+                f[2 * partner + 1] = -v.y;
+            }
+        }
     }
 
     // ------------------------------------------------ HIP graphs ----
@@ -1976,6 +2125,15 @@ template <typename R> class Engine final : public EngineBase {
     DeviceBuffer<uint64_t> d_keys_;
     std::vector<double> nu_, alpha_, sigma_, amp_;
     std::vector<uint64_t> keys_;
+    // OU forcing process (set_forcing_process): stream words [MT_N + 1][B],
+    // coefficients [B][P] in the reference's pair order, their resampling
+    // copy, the engine pair index of each (~index when conjugated) and the
+    // grid index of each pair's first mode.
+    bool ou_ = false;
+    DeviceBuffer<uint64_t> ou_mt_;
+    DeviceBuffer<cx<double>> ou_F_, ou_tmp_;
+    DeviceBuffer<int> ou_map_;
+    std::vector<long> ou_first_;
     bool params_dirty_ = false, mix_valid_ = false;
     uint64_t step_ = 0;
     double last_dt_ = 0, max_dt_ = 0, last_cfl_rate_ = 0;
@@ -2134,6 +2292,22 @@ int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps) {
 
 int hzl_ensemble_step_integrator(hzl_ensemble_t h, double dt, size_t steps, int integrator) {
     return guarded([&] { engine_of(h).step_integrator(dt, steps, integrator); });
+}
+
+int hzl_ensemble_set_forcing_process(hzl_ensemble_t h, int process) {
+    return guarded([&] { engine_of(h).set_forcing_process(process); });
+}
+
+int hzl_ensemble_reseed_future(hzl_ensemble_t h, size_t m, uint64_t seed) {
+    return guarded([&] { engine_of(h).reseed_future(m, seed); });
+}
+
+int hzl_ensemble_get_forcing(hzl_ensemble_t h, size_t m, double *field) {
+    return guarded([&] {
+        if (!field)
+            throw std::invalid_argument("null field");
+        engine_of(h).get_forcing(m, field);
+    });
 }
 
 int hzl_ensemble_drift(hzl_ensemble_t h, double *out) {

@@ -76,6 +76,13 @@ class EngineBase {
     virtual void set_graph(bool on) = 0;
     // SSPRK3 stepping and the calibration drifts (batched over members)
     virtual void step_integrator(double dt, size_t steps, int integrator) = 0;
+    // Forcing process of the shell noise: HZL_FORCING_WHITE (the engine's
+    // white-in-time noise) or HZL_FORCING_OU (the reference's OU process,
+    // mt19937_64 streams per member); get_forcing: current OU coefficients.
+    virtual void set_forcing_process(int process) = 0;
+    virtual int forcing_process() const = 0;
+    virtual void get_forcing(size_t m, double *field) = 0;
+    virtual void reseed_future(size_t m, uint64_t seed) = 0;
     virtual void drift(double *out) = 0;
     virtual void linearized_drift(const double *directions, double *out) = 0;
     virtual void adjoint_drift(const double *cotangents, double *out) = 0;

@@ -717,6 +717,11 @@ int hzl_enkf_begin(hzl_ensemble_t h, uint64_t seed, int mode) {
             throw std::invalid_argument("EnKF mode must be HZL_ENKF_SERIAL or HZL_ENKF_BATCH");
         if (mode == HZL_ENKF_BATCH && v.precision != HZL_FP32)
             throw std::invalid_argument("batch EnKF analysis needs an FP32 ensemble");
+        // the reference's analysis also updates the OU forcing as a latent
+        // state (src/filter.cpp:237-253,311-321); this analysis updates the
+        // vorticity only, so it runs with the white-in-time forcing
+        if (e.forcing_process() == HZL_FORCING_OU)
+            throw std::invalid_argument("EnKF runs with the white forcing process, not OU");
         e.enkf = std::make_unique<hzl_amd::EnkfState>(
             mode, hzl_amd::stream_seed(seed, (uint64_t)v.B));
     });

@@ -6,6 +6,8 @@
 #include <fstream>
 #include <iomanip>
 #include <limits>
+#include <numbers>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 
@@ -259,6 +261,21 @@ Field Ensemble::state(std::size_t member) const {
     Field out(n_ * n_);
     check(hzl_ensemble_get_state(handle_, member, raw(out)));
     return out;
+}
+
+void Ensemble::set_forcing_process(ForcingProcess process) {
+    check(hzl_ensemble_set_forcing_process(
+        handle_, process == ForcingProcess::white ? HZL_FORCING_WHITE : HZL_FORCING_OU));
+}
+
+Field Ensemble::forcing(std::size_t member) const {
+    Field out(n_ * n_);
+    check(hzl_ensemble_get_forcing(handle_, member, raw(out)));
+    return out;
+}
+
+void Ensemble::reseed_future(std::size_t member, std::uint64_t seed) {
+    check(hzl_ensemble_reseed_future(handle_, member, seed));
 }
 
 void Ensemble::step(double dt, std::size_t steps) {
@@ -644,12 +661,144 @@ FilterResult bootstrap_filter(Ensemble &particles, const Dataset &data, std::uin
 FilterResult bootstrap_filter(const Dataset &data, Parameters p, std::size_t particles,
                               std::uint64_t seed, bool retain_states) {
     data.validate();
-    if (particles == 0)
-        throw std::invalid_argument("particle filter requires at least one particle");
     p = with_data_shell(p, data);
+    p.validate(data.n);
+    if (particles < 2)
+        throw std::invalid_argument("particle filter needs at least two particles");
+    // particle i is Solver(n, p, stream_seed(seed, i)) from multimode with its
+    // own OU forcing (src/filter.cpp:46-51)
     Ensemble e(data.n, particles, p, Precision::fp64, seed);
     e.set_all(initial_condition(data.n, "multimode"));
+    e.set_forcing_process(ForcingProcess::ornstein_uhlenbeck);
     return bootstrap_filter(e, data, seed, retain_states);
+}
+
+namespace {
+
+constexpr double negative_infinity = -std::numeric_limits<double>::infinity();
+
+// NormalStream (src/solver.cpp:41-54): mt19937_64, open-interval uniforms,
+// Box-Muller returning r cos and caching r sin.
+class NormalStream {
+  public:
+    explicit NormalStream(std::uint64_t seed) : engine_(seed) {}
+    double operator()() {
+        if (has_spare_) {
+            has_spare_ = false;
+            return spare_;
+        }
+        const auto open = [this] { return ((double)(engine_() >> 11) + 0.5) * 0x1.0p-53; };
+        const double r = std::sqrt(-2 * std::log(open())), angle = 2 * std::numbers::pi * open();
+        spare_ = r * std::sin(angle);
+        has_spare_ = true;
+        return r * std::cos(angle);
+    }
+
+  private:
+    std::mt19937_64 engine_;
+    bool has_spare_ = false;
+    double spare_ = 0;
+};
+
+// decode (src/filter.cpp:15-24)
+Parameters decode(const Dataset &data, const Vector &q) {
+    Parameters p;
+    p.viscosity = std::exp(q[0]);
+    p.friction = std::exp(q[1]);
+    p.forcing_sigma = std::exp(q[2]);
+    p.forcing_tau = std::exp(q[3]);
+    p.forcing_min = data.forcing_min;
+    p.forcing_max = data.forcing_max;
+    return p;
+}
+
+} // namespace
+
+double stochastic_log_prior(const Vector &q) {
+    if (q.size() != 4)
+        throw std::invalid_argument(
+            "stochastic inference expects log(nu), log(alpha), log(sigma), log(tau)");
+    const double centers[] = {std::log(0.03), std::log(0.1), std::log(0.2), std::log(0.5)};
+    double result = 0;
+    for (std::size_t i = 0; i < q.size(); ++i) {
+        if (!std::isfinite(q[i]) || q[i] < -10 || q[i] > (i < 2 ? 0 : 2))
+            return negative_infinity;
+        const double z = (q[i] - centers[i]) / 1.5;
+        result -= 0.5 * z * z;
+    }
+    return result;
+}
+
+double particle_log_likelihood(const Dataset &data, const Vector &q, std::size_t particles,
+                               std::uint64_t seed) {
+    if (!std::isfinite(stochastic_log_prior(q)))
+        return negative_infinity;
+    const Parameters p = decode(data, q);
+    const double largest = (double)((data.n - 1) / 3);
+    if (data.dt * (2 * p.viscosity * largest * largest + p.friction) > 2.5)
+        return negative_infinity;
+    return bootstrap_filter(data, p, particles, seed, false).log_likelihood;
+}
+
+double averaged_particle_log_likelihood(const Dataset &data, const Vector &q,
+                                        std::size_t particles, std::size_t replicates,
+                                        std::uint64_t seed) {
+    if (replicates == 0)
+        throw std::invalid_argument("averaged particle likelihood needs at least one replicate");
+    Vector values(replicates);
+    for (std::size_t i = 0; i < replicates; ++i)
+        values[i] = particle_log_likelihood(data, q, particles, stream_seed(seed, i));
+    const double maximum = *std::max_element(values.begin(), values.end());
+    if (!std::isfinite(maximum))
+        return negative_infinity;
+    double total = 0;
+    for (double v : values)
+        total += std::exp(v - maximum);
+    return maximum + std::log(total / (double)replicates);
+}
+
+// Pseudo-marginal random-walk Metropolis (src/filter.cpp:166-203): proposals
+// from NormalStream(seed), likelihood seeds and acceptance uniforms from
+// mt19937_64(stream_seed(seed, 1)); a rejected proposal keeps the current
+// likelihood estimate and its seed.  Each likelihood is one GPU filter.
+std::vector<ParticleSample> particle_metropolis(const Dataset &data, Vector current,
+                                                double scale, std::size_t count,
+                                                std::size_t particles, std::uint64_t seed) {
+    if (!data.stochastic)
+        throw std::invalid_argument("particle sampler expects a stochastic observation dataset");
+    if (count == 0 || !std::isfinite(scale) || scale <= 0)
+        throw std::invalid_argument("invalid particle sampler configuration");
+    NormalStream normal(seed);
+    std::mt19937_64 random(stream_seed(seed, 1));
+    const auto uniform = [&random] { return (double)(random() >> 11) * 0x1.0p-53; };
+    auto likelihood_seed = random();
+    double likelihood = particle_log_likelihood(data, current, particles, likelihood_seed);
+    double density = likelihood + stochastic_log_prior(current);
+    if (!std::isfinite(density))
+        throw std::invalid_argument("initial particle posterior estimate is not finite");
+    std::vector<ParticleSample> chain;
+    chain.reserve(count);
+    for (std::size_t i = 0; i < count; ++i) {
+        auto proposal = current;
+        for (auto &q : proposal)
+            q += scale * normal();
+        const auto proposed_seed = random();
+        const double proposed_likelihood =
+            particle_log_likelihood(data, proposal, particles, proposed_seed);
+        const double proposed_density = proposed_likelihood + stochastic_log_prior(proposal);
+        const double draw = uniform();
+        const bool accepted = std::isfinite(proposed_density) &&
+                              std::log(std::max(draw, std::numeric_limits<double>::min())) <
+                                  std::min(0.0, proposed_density - density);
+        if (accepted) {
+            current = std::move(proposal);
+            likelihood = proposed_likelihood;
+            density = proposed_density;
+            likelihood_seed = proposed_seed;
+        }
+        chain.push_back({current, likelihood, density, likelihood_seed, accepted});
+    }
+    return chain;
 }
 
 std::vector<EnsembleFilterRecord> ensemble_kalman_filter(Ensemble &members, const Dataset &data,
@@ -707,7 +856,10 @@ std::vector<Diagnostics> ensemble(std::size_t n, Parameters p, double dt, std::s
                                   std::size_t count, std::size_t workers, std::uint64_t seed) {
     if (count == 0 || workers == 0)
         throw std::invalid_argument("ensemble count and workers must be positive");
+    // member i is Solver(n, p, stream_seed(seed, i)) from zero vorticity with
+    // its own OU forcing (src/inference.cpp:563-575)
     Ensemble e(n, count, p, Precision::fp64, seed);
+    e.set_forcing_process(ForcingProcess::ornstein_uhlenbeck);
     e.step(dt, steps);
     e.synchronize();
     auto d = e.diagnostics();

@@ -9,16 +9,21 @@
 //   hzl::Fourier2D, wave_number (include/hzl/fft.hpp:10-25)
 //   hzl::Observation, Dataset   (include/hzl/inference.hpp:33-49)
 //   hzl::ensemble               (include/hzl/inference.hpp:84-85)
-//   hzl::bootstrap_filter, ensemble_kalman_filter and their records
-//                               (include/hzl/filter.hpp:5-54)
+//   hzl::bootstrap_filter, ensemble_kalman_filter, particle_log_likelihood,
+//   averaged_particle_log_likelihood, particle_metropolis, stochastic_log_prior
+//   and their records           (include/hzl/filter.hpp:5-54)
 // compiles and links against libhzl_gpu.so instead of the reference library.
-// Differences of behaviour, all on the stochastic-forcing side:
-//   * the engine's forcing is white in time (variance sigma^2 dt / 2 per real
-//     component on the shell) instead of the reference's Ornstein-Uhlenbeck
-//     process; with forcing_sigma = 0 every function here reproduces the
-//     reference's deterministic results (filters, EnKF, ensemble diagnostics);
-//   * the EnKF analysis updates the vorticity only (white forcing carries no
-//     latent state);
+// The reference-signature solver functions (bootstrap_filter, ensemble and the
+// particle-likelihood family) run the reference's own forcing process: the
+// Ornstein-Uhlenbeck shell forcing with one mt19937_64 NormalStream per member
+// seeded as the reference seeds its Solvers.  Differences of behaviour:
+//   * ensemble_kalman_filter runs with the engine's white-in-time forcing and
+//     updates the vorticity only (the reference also updates the OU forcing
+//     as a latent state); with forcing_sigma = 0 it reproduces the
+//     reference's results;
+//   * hzl::Ensemble defaults to the white-in-time forcing (variance
+//     sigma^2 / (2 dt) per component held through the step); select the OU
+//     process with set_forcing_process;
 //   * CudaSolver's CFL check is read when the step's results are read
 //     (synchronize / coefficients) instead of after every step.
 // hzl::Ensemble (GPU batches of members) has no reference counterpart: it
@@ -96,6 +101,7 @@ class CudaSolver {
 };
 
 enum class Precision { fp32, fp64 };
+enum class ForcingProcess { white, ornstein_uhlenbeck };
 
 // A batch of members on one grid in one engine handle.  Members start at
 // zero; member m's noise stream is keyed by stream_seed(seed, m).
@@ -123,6 +129,14 @@ class Ensemble {
     [[nodiscard]] std::vector<Field> adjoint_drift(const std::vector<Field> &cotangents);
     void synchronize();
     [[nodiscard]] std::vector<Diagnostics> diagnostics();
+    // Shell forcing process: white in time (default) or the reference's OU
+    // process (ShellForcing, src/solver.cpp:84-119); selecting OU draws every
+    // member's stationary start from its stream.
+    void set_forcing_process(ForcingProcess process);
+    // Solver::forcing() of one member (OU process only)
+    [[nodiscard]] Field forcing(std::size_t member) const;
+    // Solver::reseed_future (include/hzl/solver.hpp:73-75)
+    void reseed_future(std::size_t member, std::uint64_t seed);
     [[nodiscard]] std::size_t size() const noexcept {
         return n_;
     }
@@ -219,6 +233,28 @@ std::vector<EnsembleFilterRecord> ensemble_kalman_filter(Ensemble &members, cons
                                                          bool batch = false);
 
 using Vector = std::vector<double>;
+
+// Particle-marginal inference over q = (log nu, log alpha, log sigma, log tau)
+// (src/filter.cpp:127-203): Gaussian log prior with bounds, the bootstrap
+// filter's log-likelihood (OU forcing, multimode particles, FP64 on the GPU),
+// its log-mean-exp over replicates, and the pseudo-marginal random-walk
+// Metropolis chain with the reference's streams.
+struct ParticleSample {
+    Vector parameters;
+    double log_likelihood{}, log_posterior{};
+    std::uint64_t likelihood_seed{};
+    bool accepted{};
+};
+double stochastic_log_prior(const Vector &log_parameters);
+double particle_log_likelihood(const Dataset &data, const Vector &log_parameters,
+                               std::size_t particles, std::uint64_t seed);
+double averaged_particle_log_likelihood(const Dataset &data, const Vector &log_parameters,
+                                        std::size_t particles, std::size_t replicates,
+                                        std::uint64_t seed);
+std::vector<ParticleSample> particle_metropolis(const Dataset &data, Vector initial,
+                                                double proposal_scale, std::size_t count,
+                                                std::size_t particles, std::uint64_t seed);
+
 struct ValueGradient {
     double value{};
     Vector gradient;

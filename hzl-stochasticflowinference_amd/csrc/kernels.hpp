@@ -214,6 +214,145 @@ template <typename R> HZL_HD void noise_body(const NoiseArgs<R> &a, long i) {
     a.noise[i] = v;
 }
 
+// ------------------------------------------------ OU shell forcing ----
+// The reference's forcing process (src/solver.cpp:41-54,84-105,174-176,
+// 217-254): per member an mt19937_64 stream read through NormalStream
+// (Box-Muller, first draw r cos, second r sin) and one FP64 OU coefficient
+// per shell pair, in the reference's pair order (forcing_pairs,
+// src/solver.cpp:67-83: grid index y*n + x of the pair's first mode, rising).
+// Every use draws two normals per pair, so a stream never holds a cached
+// second normal between calls and its position stays even: pair j of a call
+// takes mt outputs pos + 2j and pos + 2j + 1, and the lanes of the member's
+// workgroup draw in parallel.  The 312 state words are staged in LDS; the
+// regeneration (twist) runs in two parallel phases, words 0..155 from the old
+// words and 156..311 from the new words 0..155.
+constexpr int OU_LANES = 64;
+constexpr int MT_N = 312;
+
+template <typename R> struct OuArgs {
+    int B, npairs;
+    uint64_t *mt;          // [MT_N + 1][B]: state words, then the position
+    cx<double> *F;         // [B][npairs] coefficients, reference pair order
+    const int *map;        // [npairs] engine pair index, ~index when conjugated
+    const double *sigma;   // per member
+    const uint64_t *keys;  // per member seeds (modes 0 and 2)
+    double tau, h;         // OU time constant, transition interval (mode 1)
+    int mode;              // 0: seed + stationary start, 1: transition, 2: reseed,
+                           // 3: publish the current coefficients to noise (no draws)
+    cx<R> *noise;          // modes 1, 3: forcing [B][npairs] (engine order) or null
+    const double *stats;   // mode 2: only members of filters that resampled
+    int P;                 // particles per filter (mode 2)
+};
+
+HZL_HD uint64_t mt_temper(uint64_t y) {
+    y ^= (y >> 29) & 0x5555555555555555ULL;
+    y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+    y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+    return y ^ (y >> 43);
+}
+
+HZL_HD uint64_t mt_mix(uint64_t a, uint64_t b) {
+    const uint64_t x = (a & 0xFFFFFFFF80000000ULL) | (b & 0x7FFFFFFFULL);
+    return (x >> 1) ^ ((x & 1) ? 0xB5026F5AA96619E9ULL : 0ULL);
+}
+
+// NormalStream's open-interval uniform ((x >> 11) + 0.5) 2^-53.
+HZL_HD double mt_open_uniform(uint64_t x) {
+    return ((double)(x >> 11) + 0.5) * 0x1.0p-53;
+}
+
+template <typename R, class Ctx>
+HZL_HD void ou_body(const Ctx &ctx, const OuArgs<R> &a, uint64_t *s, int *shared_pos, int m) {
+    const int lane = ctx.tid();
+    const long B = a.B;
+    if (a.stats && a.stats[8 * (m / a.P) + 3] == 0)
+        return; // filter did not resample (uniform across the workgroup)
+    if (a.mode == 3) {
+        for (int j = lane; j < a.npairs; j += OU_LANES) {
+            const cx<double> v = a.F[(long)m * a.npairs + j];
+            const int p = a.map[j] >= 0 ? a.map[j] : ~a.map[j];
+            a.noise[(long)m * a.npairs + p] =
+                a.map[j] >= 0 ? cx<R>{(R)v.x, (R)v.y} : cx<R>{(R)v.x, (R)(-v.y)};
+        }
+        return;
+    }
+    if (a.mode == 0 || a.mode == 2) {
+        if (lane == 0) {
+            s[0] = a.keys[m];
+            for (int i = 1; i < MT_N; ++i)
+                s[i] = 6364136223846793005ULL * (s[i - 1] ^ (s[i - 1] >> 62)) + (uint64_t)i;
+            *shared_pos = MT_N;
+        }
+    } else {
+        for (int i = lane; i < MT_N; i += OU_LANES)
+            s[i] = a.mt[(long)i * B + m];
+        if (lane == 0)
+            *shared_pos = (int)a.mt[(long)MT_N * B + m];
+    }
+    ctx.sync();
+    int pos = *shared_pos;
+    if (a.mode != 2) {
+        const double sg = a.sigma[m];
+        double decay = 0, scale;
+        if (a.mode == 0) {
+            scale = sg * sqrt(a.tau / 4);
+        } else {
+            decay = exp(-a.h / a.tau);
+            scale = sg * sqrt(-a.tau * expm1(-2 * a.h / a.tau) / 4);
+        }
+        int done = 0;
+        while (done < a.npairs) {
+            if (pos >= MT_N) {
+                uint64_t nv[(MT_N / 2 + OU_LANES - 1) / OU_LANES];
+                int c = 0;
+                for (int i = lane; i < MT_N / 2; i += OU_LANES)
+                    nv[c++] = s[i + MT_N / 2] ^ mt_mix(s[i], s[i + 1]);
+                ctx.sync();
+                c = 0;
+                for (int i = lane; i < MT_N / 2; i += OU_LANES)
+                    s[i] = nv[c++];
+                ctx.sync();
+                c = 0;
+                for (int i = MT_N / 2 + lane; i < MT_N; i += OU_LANES)
+                    nv[c++] = s[i - MT_N / 2] ^ mt_mix(s[i], s[(i + 1) % MT_N]);
+                ctx.sync();
+                c = 0;
+                for (int i = MT_N / 2 + lane; i < MT_N; i += OU_LANES)
+                    s[i] = nv[c++];
+                ctx.sync();
+                pos = 0;
+            }
+            const int count = (a.npairs - done) < (MT_N - pos) / 2 ? (a.npairs - done)
+                                                                      : (MT_N - pos) / 2;
+            for (int l = lane; l < count; l += OU_LANES) {
+                const int j = done + l;
+                const double u1 = mt_open_uniform(mt_temper(s[pos + 2 * l]));
+                const double u2 = mt_open_uniform(mt_temper(s[pos + 2 * l + 1]));
+                const double r = sqrt(-2 * log(u1)), angle = 2 * 3.14159265358979323846 * u2;
+                const double re = r * cos(angle), im = r * sin(angle);
+                cx<double> &f = a.F[(long)m * a.npairs + j];
+                cx<double> v;
+                if (a.mode == 0)
+                    v = {scale * re, scale * im};
+                else
+                    v = {decay * f.x + scale * re, decay * f.y + scale * im};
+                f = v;
+                if (a.noise) {
+                    const int p = a.map[j] >= 0 ? a.map[j] : ~a.map[j];
+                    a.noise[(long)m * a.npairs + p] =
+                        a.map[j] >= 0 ? cx<R>{(R)v.x, (R)v.y} : cx<R>{(R)v.x, (R)(-v.y)};
+                }
+            }
+            pos += 2 * count;
+            done += count;
+        }
+    }
+    for (int i = lane; i < MT_N; i += OU_LANES)
+        a.mt[(long)i * B + m] = s[i];
+    if (lane == 0)
+        a.mt[(long)MT_N * B + m] = (uint64_t)pos;
+}
+
 template <typename R> struct ForcingModel {
     const cx<R> *noise;  // [B][npairs] shell noise of the current step, or null
     int npairs;

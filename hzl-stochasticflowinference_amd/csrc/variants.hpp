@@ -52,6 +52,7 @@
     X(256, 8, 16, 16, 2, 0, 1)                                                                  \
     X(256, 9, 16, 8, 3, 0, 2)                                                                   \
     X(256, 10, 16, 8, 3, 0, 3)                                                                  \
+    X(256, 11, 16, 16, 3, 0, 3)                                                                 \
     X(512, 1, 32, 8, 3, 0, 0)                                                                   \
     X(512, 2, 32, 8, 1, 1, 0)                                                                   \
     X(512, 3, 32, 8, 2, 0, 1)

@@ -26,7 +26,8 @@ EXPORTS = (
     "hzl_last_error", "hzl_ensemble_create", "hzl_ensemble_destroy",
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
     "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
-    "hzl_ensemble_step_integrator", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
+    "hzl_ensemble_step_integrator", "hzl_ensemble_set_forcing_process",
+    "hzl_ensemble_get_forcing", "hzl_ensemble_reseed_future", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
     "hzl_ensemble_adjoint_drift", "hzl_likelihood_adjoint", "hzl_ensemble_step_tangent",
     "hzl_likelihood_gradient",
     "hzl_ensemble_variant_count", "hzl_ensemble_set_tuning", "hzl_ensemble_get_tuning",
@@ -101,6 +102,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     ipp = C.POINTER(C.c_int)
     L.hzl_ensemble_set_graph.argtypes = [vp, C.c_int]
     L.hzl_ensemble_step_integrator.argtypes = [vp, C.c_double, C.c_size_t, C.c_int]
+    L.hzl_ensemble_set_forcing_process.argtypes = [vp, C.c_int]
+    L.hzl_ensemble_get_forcing.argtypes = [vp, C.c_size_t, _dp]
+    L.hzl_ensemble_reseed_future.argtypes = [vp, C.c_size_t, C.c_uint64]
     L.hzl_ensemble_drift.argtypes = [vp, _dp]
     L.hzl_ensemble_linearized_drift.argtypes = [vp, _dp, _dp]
     L.hzl_ensemble_adjoint_drift.argtypes = [vp, _dp, _dp]
@@ -249,6 +253,39 @@ class Ensemble:
             if code is None:
                 raise HzlInvalidArgument("integrator must be rk4 or ssprk3")
             _check(_lib.hzl_ensemble_step_integrator(self._h, dt, steps, code))
+
+    def set_forcing_process(self, process: str):
+        """Shell forcing process: "white" (default, white in time) or "ou", the
+        reference's Ornstein-Uhlenbeck process with an mt19937_64 NormalStream
+        per member (ShellForcing, src/solver.cpp:84-119); selecting "ou" draws
+        every member's stationary start from its stream seed."""
+        code = {"white": 0, "ou": 1}.get(process)
+        if code is None:
+            raise HzlInvalidArgument("forcing process must be white or ou")
+        _check(_lib.hzl_ensemble_set_forcing_process(self._h, code))
+
+    def reseed_future(self, member: int, seed: int):
+        """Solver::reseed_future: member's future innovations from `seed`."""
+        _check(_lib.hzl_ensemble_reseed_future(self._h, member, C.c_uint64(seed)))
+
+    def set_member_seeds(self, seeds):
+        """Stream seed of every member (as Solver(n, p, seeds[m]))."""
+        seeds = list(seeds)
+        if len(seeds) != self.members:
+            raise HzlInvalidArgument("one seed per member")
+        for m, s in enumerate(seeds):
+            self.reseed_future(m, int(s))
+
+    def get_forcing(self, member=None) -> np.ndarray:
+        """Current OU forcing coefficients (Solver::forcing()): (n, n) spectral
+        for one member, (members, n, n) when member is None."""
+        if member is None:
+            out = np.zeros((self.members, self.n, self.n), dtype=np.complex128)
+            member = ALL
+        else:
+            out = np.zeros((self.n, self.n), dtype=np.complex128)
+        _check(_lib.hzl_ensemble_get_forcing(self._h, member, out.ctypes.data_as(_dp)))
+        return out
 
     def step_tangent(self, dt: float, steps: int, tangents) -> np.ndarray:
         """Solver::step_tangent for every member: tangents (members, 2, n, n)

@@ -44,6 +44,9 @@ extern "C" {
 /* time integrators (reference hzl::TimeIntegrator, include/hzl/solver.hpp:12) */
 #define HZL_RK4 0
 #define HZL_SSPRK3 1
+/* Forcing processes of the shell noise (hzl_ensemble_set_forcing_process). */
+#define HZL_FORCING_WHITE 0
+#define HZL_FORCING_OU 1
 
 typedef struct hzl_ensemble *hzl_ensemble_t;
 typedef struct hzl_surrogate *hzl_surrogate_t;
@@ -115,6 +118,35 @@ int hzl_ensemble_step(hzl_ensemble_t h, double dt, size_t steps);
  * forcing held through its three stages.  Replaces: Solver::step(dt,
  * TimeIntegrator) (src/solver.cpp:217-254), batched; same time-step checks. */
 int hzl_ensemble_step_integrator(hzl_ensemble_t h, double dt, size_t steps, int integrator);
+
+/* Forcing process of the shell noise.  HZL_FORCING_WHITE (the default): white
+ * in time, variance sigma^2 / (2 dt) per component held through the step.
+ * HZL_FORCING_OU: the reference's Ornstein-Uhlenbeck shell forcing -- per member
+ * an mt19937_64 NormalStream seeded with the member's stream seed
+ * (stream_seed(seed, m) at creation; stream_seed(seed_f, i) after
+ * hzl_pf_begin_filters), a stationary start with variance sigma^2 tau / 4 per
+ * component drawn when the process is selected (and again when
+ * hzl_pf_begin_filters resets the seeds), an exact half-step transition before
+ * and after every step with the midpoint held through the stages, and
+ * particle-filter children that keep their ancestor's coefficients and restart
+ * their stream from stream_seed(seed, (g+1) P + i).  tau is the creation
+ * parameters' forcing_tau; sigma is per member.  Replaces: ShellForcing /
+ * initialize_ou / transition_ou / Solver::reseed_future (src/solver.cpp:84-119,
+ * 174-176,217-254; include/hzl/solver.hpp:73-75). */
+int hzl_ensemble_set_forcing_process(hzl_ensemble_t h, int process);
+
+/* Current OU forcing coefficients of member m (or of every member when
+ * m == (size_t)-1) as reference Fields (n*n interleaved doubles, zero off the
+ * shell pairs).  Replaces: Solver::forcing() (include/hzl/solver.hpp:83-85).
+ * HZL_EINVAL unless the OU process is selected. */
+int hzl_ensemble_get_forcing(hzl_ensemble_t h, size_t m, double *field);
+
+/* Future innovations of member m come from stream seed `seed`: the member's
+ * Philox key (white forcing) or a fresh mt19937_64(seed) NormalStream with the
+ * OU coefficients kept (OU forcing).  Replaces: Solver::reseed_future
+ * (include/hzl/solver.hpp:73-75); with it a member reproduces a reference
+ * Solver(n, p, seed) whose forcing process is selected afterwards. */
+int hzl_ensemble_reseed_future(hzl_ensemble_t h, size_t m, uint64_t seed);
 
 /* Calibration drifts of every member's current state, batched: out,
  * directions and cotangents hold one reference Field per member (members

@@ -20,6 +20,9 @@
  *   - serial perturbed-observation EnKF scalar update             (src/filter.cpp:265-321)
  *   - SSPRK(3,3) step, tangent and adjoint drifts                  (src/solver.cpp:235-246,
  *                                                                  264-323)
+ *   - OU shell forcing: pair order, stationary start, exact
+ *     transition, forcing advanced by dt/2 around each step       (src/solver.cpp:67-119,
+ *                                                                  120-139,174-176,217-254)
  * plus the engine's own input definitions that have no reference counterpart
  * (Philox4x32-10 counter RNG, shell white noise, Kolmogorov forcing, Gaussian
  * random-field initial condition), written from their published definitions.
@@ -43,6 +46,9 @@
  *                 reference build
  *   tangent    -- central difference of the (quadratic) drift, reference build
  *   adjoint    -- adjoint identity <lambda, L d> = <L* lambda, d>, reference build
+ *   OU forcing -- reference build (Solver with sigma > 0: trajectory and forcing),
+ *                 stationary variance sigma^2 tau / 4, one-step autocorrelation
+ *                 e^{-dt/tau}, pair set = forcing_pairs
  */
 #include <math.h>
 #include <stdint.h>
@@ -590,6 +596,95 @@ double orc_normal(orc_mt *s) {
     s->spare = r * sin(angle);
     s->has_spare = 1;
     return r * cos(angle);
+}
+
+/* ------------------------------------------------ OU shell forcing ---- */
+/* forcing_pairs (src/solver.cpp:67-83) in the reference's order: grid index
+ * i = y n + x of each conjugate pair's first mode (i < partner) inside the
+ * mask and kmin^2 <= k^2 <= kmax^2.  Returns the count; first may be NULL. */
+int orc_ou_pairs(long n, int kmin, int kmax, long *first) {
+    int count = 0;
+    for (long y = 0; y < n; ++y)
+        for (long x = 0; x < n; ++x) {
+            const int a = orc_wave_number(x, n), b = orc_wave_number(y, n);
+            const long i = y * n + x, partner = ((n - y) % n) * n + (n - x) % n;
+            const int k2 = a * a + b * b;
+            if (3 * abs(a) < n && 3 * abs(b) < n && i < partner && k2 >= kmin * kmin &&
+                k2 <= kmax * kmax) {
+                if (first)
+                    first[count] = i;
+                ++count;
+            }
+        }
+    return count;
+}
+
+static long ou_partner(long n, long i) {
+    const long y = i / n, x = i % n;
+    return ((n - y) % n) * n + (n - x) % n;
+}
+
+/* initialize_ou (src/solver.cpp:84-92): F = sigma sqrt(tau / 4) (xi1 + i xi2)
+ * per pair, conjugate on the partner; two normals per pair, in pair order. */
+void orc_ou_initialize(orc_mt *s, long n, int npairs, const long *first, double sigma,
+                       double tau, double *Fd) {
+    cpx *F = (cpx *)Fd;
+    const double scale = sigma * sqrt(tau / 4);
+    for (int j = 0; j < npairs; ++j) {
+        const double re = orc_normal(s), im = orc_normal(s);
+        F[first[j]].re = scale * re;
+        F[first[j]].im = scale * im;
+        F[ou_partner(n, first[j])].re = scale * re;
+        F[ou_partner(n, first[j])].im = -(scale * im);
+    }
+}
+
+/* transition_ou (src/solver.cpp:93-105): exact OU step over dt,
+ * F <- e^{-dt/tau} F + sigma sqrt(-tau expm1(-2 dt / tau) / 4) (xi1 + i xi2). */
+void orc_ou_transition(orc_mt *s, long n, int npairs, const long *first, double sigma,
+                       double tau, double dt, double *Fd) {
+    cpx *F = (cpx *)Fd;
+    const double decay = exp(-dt / tau), scale = sigma * sqrt(-tau * expm1(-2 * dt / tau) / 4);
+    for (int j = 0; j < npairs; ++j) {
+        const double re = orc_normal(s), im = orc_normal(s);
+        cpx *f = &F[first[j]];
+        f->re = decay * f->re + scale * re;
+        f->im = decay * f->im + scale * im;
+        F[ou_partner(n, first[j])].re = f->re;
+        F[ou_partner(n, first[j])].im = -f->im;
+    }
+}
+
+/* Solver(n, p, seed) + `steps` Solver::step(dt) from the spectral state w (in
+ * place): the stream mt19937_64(seed) draws the stationary OU start, then each
+ * step advances the forcing by dt/2, takes one RK4 step with that midpoint
+ * forcing held, and advances it by dt/2 (src/solver.cpp:120-139,217-254).
+ * F_out receives the final forcing (n*n complex). Returns -1 on bad input. */
+int orc_run_ou(long n, double nu, double alpha, double sigma, double tau, int kmin, int kmax,
+               uint64_t seed, double dt, long steps, double *w, double *F_out) {
+    const int np = orc_ou_pairs(n, kmin, kmax, NULL);
+    if (np <= 0 || !(tau > 0))
+        return -1;
+    long *first = (long *)malloc(sizeof(long) * (size_t)np);
+    orc_ou_pairs(n, kmin, kmax, first);
+    orc_mt st;
+    orc_mt_seed(&st, seed);
+    memset(F_out, 0, sizeof(double) * 2 * (size_t)(n * n));
+    orc_ou_initialize(&st, n, np, first, sigma, tau, F_out);
+    orc_solver s;
+    if (solver_make(&s, n)) {
+        solver_free(&s);
+        free(first);
+        return -1;
+    }
+    for (long k = 0; k < steps; ++k) {
+        orc_ou_transition(&st, n, np, first, sigma, tau, dt / 2, F_out);
+        solver_step(&s, nu, alpha, dt, (cpx *)w, (const cpx *)F_out);
+        orc_ou_transition(&st, n, np, first, sigma, tau, dt / 2, F_out);
+    }
+    solver_free(&s);
+    free(first);
+    return 0;
 }
 
 /* Half-open uniform used by the filter's resampler (src/filter.cpp:11-13). */

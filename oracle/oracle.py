@@ -77,6 +77,15 @@ class Oracle:
         L.orc_step_forcing.restype = C.c_int
         L.orc_step_forcing.argtypes = [C.c_long, C.c_double, C.c_int, C.c_double, C.c_int,
                                        C.c_int, C.c_double, C.c_uint64, C.c_uint64, _dp]
+        L.orc_ou_pairs.restype = C.c_int
+        L.orc_ou_pairs.argtypes = [C.c_long, C.c_int, C.c_int, C.POINTER(C.c_long)]
+        L.orc_ou_initialize.argtypes = [C.c_void_p, C.c_long, C.c_int, C.POINTER(C.c_long),
+                                        C.c_double, C.c_double, _dp]
+        L.orc_ou_transition.argtypes = [C.c_void_p, C.c_long, C.c_int, C.POINTER(C.c_long),
+                                        C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_run_ou.restype = C.c_int
+        L.orc_run_ou.argtypes = [C.c_long, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_long, _dp, _dp]
         L.orc_grf.restype = C.c_int
         L.orc_grf.argtypes = [C.c_long, C.c_uint64, _dp]
         L.orc_member_trajectory.restype = C.c_int
@@ -166,6 +175,51 @@ class Oracle:
         if rc:
             raise ValueError("bad grid")
         return w
+
+    # -- OU shell forcing (src/solver.cpp:67-119) -----------------------------
+    def ou_pairs(self, n, kmin, kmax):
+        """Grid indices y*n + x of the first mode of every forcing pair, in the
+        reference's order."""
+        count = self.lib.orc_ou_pairs(n, kmin, kmax, None)
+        first = (C.c_long * count)()
+        self.lib.orc_ou_pairs(n, kmin, kmax, first)
+        return np.array(first[:], dtype=np.int64)
+
+    class OuProcess:
+        """One member's OU forcing: mt19937_64(seed) NormalStream and the
+        n x n forcing field; the constructor draws the stationary start."""
+
+        def __init__(self, orc, n, sigma, tau, kmin, kmax, seed):
+            self.orc, self.n, self.sigma, self.tau = orc, n, sigma, tau
+            first = orc.ou_pairs(n, kmin, kmax)
+            self.np = len(first)
+            self._first = (C.c_long * self.np)(*first.tolist())
+            self.state = orc.mt(seed)
+            self.F = np.zeros((n, n), dtype=np.complex128)
+            orc.lib.orc_ou_initialize(self.state, n, self.np, self._first, sigma, tau,
+                                      _ptr(self.F))
+
+        def transition(self, dt):
+            self.orc.lib.orc_ou_transition(self.state, self.n, self.np, self._first, self.sigma,
+                                           self.tau, dt, _ptr(self.F))
+
+        def reseed(self, seed):
+            """Solver::reseed_future: a new stream, the coefficients kept."""
+            self.state = self.orc.mt(seed)
+
+    def ou_process(self, n, sigma, tau, kmin, kmax, seed):
+        return Oracle.OuProcess(self, n, sigma, tau, kmin, kmax, seed)
+
+    def run_ou(self, state, nu, alpha, sigma, tau, kmin, kmax, seed, dt, steps):
+        """Solver(n, p, seed) with OU forcing from `state`: `steps` RK4 steps;
+        returns (state, forcing)."""
+        w = _c(state).copy()
+        n = w.shape[0]
+        F = np.zeros_like(w)
+        if self.lib.orc_run_ou(n, nu, alpha, sigma, tau, kmin, kmax, seed, dt, steps, _ptr(w),
+                               _ptr(F)):
+            raise ValueError("bad grid or tau")
+        return w, F
 
     def diagnostics(self, state, nu, forcing=None):
         w = _c(state)
@@ -314,6 +368,10 @@ class Reference:
                               C.c_int, C.c_int, _dp, _dp, C.c_size_t, _dp]
         L.ref_initialize_physical.restype = C.c_int
         L.ref_initialize_physical.argtypes = [C.c_size_t, _dp, _dp]
+        L.ref_run_ou.restype = C.c_int
+        L.ref_run_ou.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_size_t, _dp, _dp,
+                                 _dp]
         L.ref_initialize.restype = C.c_int
         L.ref_initialize.argtypes = [C.c_size_t, C.c_char_p, _dp]
         L.ref_drift.restype = C.c_int
@@ -364,6 +422,16 @@ class Reference:
         out = np.zeros(count)
         self.lib.ref_normals(seed, count, _ptr(out))
         return out
+
+    def run_ou(self, initial, nu, alpha, sigma, tau, kmin, kmax, seed, dt, steps):
+        """hzl::Solver(n, p, seed) with OU forcing (sigma > 0) from `initial`
+        (its constructor's forcing kept): `steps` steps; (state, forcing)."""
+        w = _c(initial)
+        n = w.shape[0]
+        out_w, out_F = np.zeros_like(w), np.zeros_like(w)
+        self._check(self.lib.ref_run_ou(n, nu, alpha, sigma, tau, kmin, kmax, seed, dt, steps,
+                                        _ptr(w), _ptr(out_w), _ptr(out_F)))
+        return out_w, out_F
 
     def run(self, initial, nu, alpha, dt, steps, forcing=None, per_step=False, kmin=1, kmax=4):
         w = _c(initial)
@@ -468,6 +536,22 @@ class Reference:
         k = int(summary[2])
         return dict(log_likelihood=summary[0], resampling_events=int(summary[1]),
                     records=records[:k], means=mean[:k] if means else None)
+
+    def particle_metropolis(self, h, q0, scale, count, particles, seed):
+        """hzl::particle_metropolis: rows (q[4], log_likelihood, log_posterior,
+        likelihood_seed, accepted)."""
+        L = self.lib
+        L.ref_particle_metropolis.restype = C.c_int
+        L.ref_particle_metropolis.argtypes = [C.c_void_p, _dp, C.c_double, C.c_size_t,
+                                              C.c_size_t, C.c_uint64, _dp]
+        q = np.ascontiguousarray(q0, dtype=np.float64)
+        out = np.zeros((count, 8))
+        self._check(L.ref_particle_metropolis(h, _ptr(q), scale, count, particles, seed,
+                                              _ptr(out)))
+        seeds = out[:, 6].copy().view(np.uint64)
+        return [dict(q=out[i, :4].copy(), log_likelihood=out[i, 4], log_posterior=out[i, 5],
+                     likelihood_seed=int(seeds[i]), accepted=bool(out[i, 7]))
+                for i in range(count)]
 
     def negative_log_likelihood(self, h, log_nu, log_alpha):
         out = np.zeros(1)

@@ -116,6 +116,30 @@ int ref_run(std::size_t n, double nu, double alpha, double dt, std::size_t steps
     });
 }
 
+// Solver(n, p, seed) with OU shell forcing (sigma, tau on kmin..kmax): the
+// given spectral state (the constructor's forcing kept), `steps` RK4 steps;
+// final state and forcing out.
+int ref_run_ou(std::size_t n, double nu, double alpha, double sigma, double tau, int kmin,
+               int kmax, std::uint64_t seed, double dt, std::size_t steps, const double *initial,
+               double *out_w, double *out_F) {
+    return guarded([&] {
+        hzl::Parameters p;
+        p.viscosity = nu;
+        p.friction = alpha;
+        p.forcing_sigma = sigma;
+        p.forcing_tau = tau;
+        p.forcing_min = kmin;
+        p.forcing_max = kmax;
+        hzl::Solver s(n, p, seed);
+        const hzl::Field start_forcing = s.forcing(); // copy: the call rewrites forcing_
+        s.update_latent_state(to_field(initial, n * n), start_forcing);
+        for (std::size_t k = 0; k < steps; ++k)
+            s.step(dt);
+        from_field(s.coefficients(), out_w);
+        from_field(s.forcing(), out_F);
+    });
+}
+
 // `steps` reference SSPRK(3,3) steps (Solver::step(dt, TimeIntegrator::ssprk3))
 // from a spectral state with one forcing field held on the shell pairs.
 int ref_run_ssprk3(std::size_t n, double nu, double alpha, double dt, std::size_t steps,
@@ -338,6 +362,26 @@ int ref_bootstrap_filter(void *d, double nu, double alpha, double sigma, double 
                 for (std::size_t j = 0; j < f.size(); ++j)
                     means[i * f.size() + j] = f[j].real();
             }
+        }
+    });
+}
+
+// particle_metropolis (src/filter.cpp:166-203) from q0 (4 log parameters):
+// per sample the 4 parameters, log-likelihood, log-posterior, likelihood seed
+// (as a double bit pattern) and the accepted flag, 8 doubles per sample.
+int ref_particle_metropolis(void *d, const double *q0, double scale, std::size_t count,
+                            std::size_t particles, std::uint64_t seed, double *out) {
+    return guarded([&] {
+        const auto &data = static_cast<RefDataset *>(d)->data;
+        const auto chain = hzl::particle_metropolis(data, {q0[0], q0[1], q0[2], q0[3]}, scale,
+                                                    count, particles, seed);
+        for (std::size_t i = 0; i < chain.size(); ++i) {
+            for (int j = 0; j < 4; ++j)
+                out[8 * i + j] = chain[i].parameters[j];
+            out[8 * i + 4] = chain[i].log_likelihood;
+            out[8 * i + 5] = chain[i].log_posterior;
+            std::memcpy(&out[8 * i + 6], &chain[i].likelihood_seed, sizeof(double));
+            out[8 * i + 7] = chain[i].accepted ? 1.0 : 0.0;
         }
     });
 }

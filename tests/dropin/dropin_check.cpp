@@ -8,6 +8,13 @@
 //   forced <n> <steps> <out>        CudaSolver::step(dt, F) with a fixed forcing
 //   pf <csv> <out>                  bootstrap_filter(data, p, 8, 555)
 //   enkf <csv> <out>                ensemble_kalman_filter(data, p, 8, 555)
+//   pf_ou <csv> <out>               bootstrap_filter(data, p, 8, 555) with sigma 0.2,
+//                                   tau 0.5 (the OU forcing process)
+//   pmcmc <csv> <count> <out>       particle_metropolis(data, log(0.03, 0.1, 0.2, 0.5),
+//                                   0.1, count, 8, 123): per sample q[4], log-likelihood,
+//                                   log-posterior, likelihood seed (bits), accepted
+//   ensemble_ou <out>               ensemble(16, p, 0.01, 20, 4, 2, 42) with sigma 0.2,
+//                                   tau 0.5 on 2..3: energy, enstrophy, injection each
 //   errors                          invalid arguments raise std::invalid_argument
 //   adjoint <csv> <stride> <out>    likelihood_adjoint(data, q, stride) at three q:
 //                                   value, gradient, checkpoint fields,
@@ -119,6 +126,44 @@ int main(int argc, char **argv) {
                 append(out, h.mean_vorticity);
             }
             write(argv[3], out);
+        } else if (mode == "pf_ou" && argc == 4) {
+            const Dataset data = Dataset::load(argv[2]);
+            Parameters p;
+            p.viscosity = 0.03;
+            p.friction = 0.1;
+            p.forcing_sigma = 0.2;
+            p.forcing_tau = 0.5;
+            const FilterResult r = bootstrap_filter(data, p, 8, 555);
+            std::vector<double> out{r.log_likelihood, (double)r.resampling_events};
+            for (const auto &h : r.history) {
+                out.insert(out.end(), {(double)h.step, h.log_likelihood_increment,
+                                       h.effective_particles, h.maximum_weight});
+                append(out, h.mean_vorticity);
+            }
+            write(argv[3], out);
+        } else if (mode == "pmcmc" && argc == 5) {
+            const Dataset data = Dataset::load(argv[2]);
+            const Vector q0{std::log(0.03), std::log(0.1), std::log(0.2), std::log(0.5)};
+            const auto chain = particle_metropolis(data, q0, 0.1, std::stoul(argv[3]), 8, 123);
+            std::vector<double> out;
+            for (const auto &c : chain) {
+                out.insert(out.end(), c.parameters.begin(), c.parameters.end());
+                double bits;
+                std::memcpy(&bits, &c.likelihood_seed, sizeof bits);
+                out.insert(out.end(), {c.log_likelihood, c.log_posterior, bits,
+                                       c.accepted ? 1.0 : 0.0});
+            }
+            write(argv[4], out);
+        } else if (mode == "ensemble_ou" && argc == 3) {
+            Parameters p;
+            p.viscosity = 0.03;
+            p.friction = 0.1;
+            p.forcing_sigma = 0.2;
+            p.forcing_tau = 0.5;
+            std::vector<double> out;
+            for (const auto &d : ensemble(16, p, 0.01, 20, 4, 2, 42))
+                out.insert(out.end(), {d.energy, d.enstrophy, d.injection});
+            write(argv[2], out);
         } else if (mode == "enkf" && argc == 4) {
             const Dataset data = Dataset::load(argv[2]);
             Parameters p;

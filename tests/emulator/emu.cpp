@@ -534,6 +534,32 @@ int calib(int kind, int B, double nu, double alpha, double sigma, double amp, in
 
 extern "C" {
 
+// OU forcing kernel body (kernels.hpp ou_body) over B members for `calls`
+// calls: modes[c] (0 start, 1 transition over hs[c], 2 reseed from
+// keys + c * B); after each call the coefficients [B][npairs] and, for
+// transitions, the midpoint forcing in engine order go to F_hist / noise_hist.
+int emu_ou(int B, int npairs, const int *map, const uint64_t *keys, const double *sigma,
+           double tau, int calls, const int *modes, const double *hs, double *F_hist,
+           double *noise_hist) {
+    std::vector<uint64_t> mt((size_t)(MT_N + 1) * B);
+    std::vector<cx<double>> F((size_t)B * npairs), noise((size_t)B * npairs);
+    for (int c = 0; c < calls; ++c) {
+        OuArgs<double> a{B,        npairs,   mt.data(), F.data(), map,         sigma,
+                         keys + (long)c * B, tau, hs[c], modes[c], modes[c] == 1 ? noise.data() : nullptr,
+                         nullptr,  1};
+        launch(B, OU_LANES, sizeof(uint64_t) * MT_N + 64,
+               [&](const CpuCtx &ctx, long b, unsigned char *lds) {
+                   auto *words = reinterpret_cast<uint64_t *>(lds);
+                   auto *pos = reinterpret_cast<int *>(lds + sizeof(uint64_t) * MT_N);
+                   ou_body<double>(ctx, a, words, pos, (int)b);
+               });
+        std::memcpy(F_hist + (size_t)c * 2 * B * npairs, F.data(), sizeof(double) * 2 * B * npairs);
+        std::memcpy(noise_hist + (size_t)c * 2 * B * npairs, noise.data(),
+                    sizeof(double) * 2 * B * npairs);
+    }
+    return 0;
+}
+
 int emu_proj_head(int bf16, long M, int C, int H, const float *X, const float *Q1,
                   const float *q1, const float *Q2, const float *q2, float *out) {
     if (bf16)

@@ -37,6 +37,9 @@ class Emulator:
         L.emu_calibration.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_int, C.c_uint64, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.emu_ou.restype = C.c_int
+        L.emu_ou.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint64), _dp,
+                             C.c_double, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
 
@@ -74,6 +77,26 @@ class Emulator:
                                     out.ctypes.data_as(fp))
         assert rc == 0
         return out
+
+    def ou(self, npairs, pair_map, keys, sigma, tau, modes, hs):
+        """ou_body over members for a sequence of calls; keys is (calls, B)
+        (mode 0 / 2 seeds per call).  Returns (F, noise), each
+        (calls, B, npairs) complex."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        calls, B = keys.shape
+        F = np.zeros((calls, B, npairs), dtype=np.complex128)
+        noise = np.zeros_like(F)
+        m = np.ascontiguousarray(pair_map, dtype=np.int32)
+        mo = np.ascontiguousarray(modes, dtype=np.int32)
+        h = np.ascontiguousarray(hs, dtype=np.float64)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        rc = self.lib.emu_ou(B, npairs, m.ctypes.data_as(C.POINTER(C.c_int)),
+                             keys.ctypes.data_as(C.POINTER(C.c_uint64)), sg.ctypes.data_as(_dp),
+                             tau, calls, mo.ctypes.data_as(C.POINTER(C.c_int)),
+                             h.ctypes.data_as(_dp), F.ctypes.data_as(_dp),
+                             noise.ctypes.data_as(_dp))
+        assert rc == 0
+        return F, noise
 
     def fft_lines(self, x, inverse=False, precision="fp32"):
         d = np.ascontiguousarray(x, dtype=np.complex128).copy()

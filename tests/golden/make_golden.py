@@ -112,6 +112,46 @@ def pf16(ref: Reference) -> None:
          enkf_records=enkf["records"], enkf_means=enkf["means"])
 
 
+def ou16(ref: Reference) -> None:
+    """The reference's OU shell forcing: a Solver trajectory with sigma > 0 and
+    a bootstrap filter over a stochastic synthetic dataset."""
+    n, nu, alpha, sigma, tau, dt = 16, 0.03, 0.1, 0.2, 0.5, 0.01
+    init = ref.initialize(n, "multimode")
+    runs = {}
+    for seed, kmin, kmax, steps in ((77, 2, 3, 25), (123456789, 1, 4, 40)):
+        w, F = ref.run_ou(init, nu, alpha, sigma, tau, kmin, kmax, seed, dt, steps)
+        runs[f"run_{seed}"] = (w, F, dict(seed=seed, kmin=kmin, kmax=kmax, steps=steps))
+    h, rows = ref.synthetic(n, dt, 60, 10, nu, alpha, sigma, tau, 2, 3, 0.05, 987)
+    pf = ref.bootstrap_filter(h, nu, alpha, sigma, tau, 8, 555, means=True, n=n)
+    q0 = np.log([nu, alpha, sigma, tau])
+    chain = ref.particle_metropolis(h, q0, 0.1, 6, 8, 123)
+    # the same stochastic dataset as written by the reference's Dataset::save
+    ref.dataset_save(h, os.path.join(HERE, "ou_observations_16.csv"))
+    ref.dataset_free(h)
+    energies = ref.ensemble(n, nu, alpha, sigma, tau, 2, 3, dt, 20, 4, 2, 42)
+    arrays = dict(initial=half(init), rows=rows, records=pf["records"], means=pf["means"],
+                  chain_q=np.array([c["q"] for c in chain]),
+                  chain_values=np.array([[c["log_likelihood"], c["log_posterior"]]
+                                         for c in chain]),
+                  chain_seeds=np.array([c["likelihood_seed"] for c in chain], dtype=np.uint64),
+                  chain_accepted=np.array([c["accepted"] for c in chain]),
+                  ensemble_energy=energies)
+    for k, (w, F, _) in runs.items():
+        arrays[k + "_state"] = half(w)
+        arrays[k + "_forcing"] = half(F)
+    save("ou_16",
+         dict(recipe="Solver(16, p, seed) with OU forcing from initialize('multimode'), "
+                     "`steps` x Solver::step(dt) (ref_run_ou); synthetic_observations(16, "
+                     "0.01, 60, 10, p, 0.05, 987) with sigma 0.2, tau 0.5 and "
+                     "bootstrap_filter(data, p, 8, 555); particle_metropolis(data, "
+                     "log(nu, alpha, sigma, tau), 0.1, 6, 8, 123); ensemble(16, p, 0.01, 20, "
+                     "4, 2, 42) energies",
+              n=n, nu=nu, alpha=alpha, sigma=sigma, tau=tau, dt=dt, particles=8, seed=555,
+              runs=[v[2] for v in runs.values()], log_likelihood=pf["log_likelihood"],
+              resampling_events=pf["resampling_events"]),
+         **arrays)
+
+
 def checkpoint16(ref: Reference) -> None:
     """A reference Solver::save file (OU forcing active, random stream
     advanced) and the reference's own Solver::load diagnostics of it."""
@@ -163,7 +203,7 @@ def adjoint16(ref: Reference) -> None:
 
 def main() -> None:
     orc, ref = Oracle(), Reference()
-    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16", "adjoint16"]
+    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16", "adjoint16", "ou16"]
     if "config0" in which:
         config0(orc, ref)
     if "drift32" in which:
@@ -174,6 +214,8 @@ def main() -> None:
         checkpoint16(ref)
     if "adjoint16" in which:
         adjoint16(ref)
+    if "ou16" in which:
+        ou16(ref)
 
 
 if __name__ == "__main__":

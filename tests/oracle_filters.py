@@ -2,10 +2,13 @@
 primitives (oracle/hzl_oracle.c).  TEST ONLY.
 
 oracle_bootstrap_filter restates reference bootstrap_filter
-(src/filter.cpp:38-126) with the engine's forcing model (white shell noise keyed
-per particle by Philox, Kolmogorov forcing) so that the GPU filter can be
-compared group by group: same particle keys, same resampling uniforms
-(mt19937_64(stream_seed(seed, UINT64_MAX))), same reseeding of children.
+(src/filter.cpp:38-126) with either of the engine's forcing processes -- white
+shell noise keyed per particle by Philox, or the reference's own OU process
+(mt19937_64 NormalStream per particle, children keep the ancestor's forcing and
+restart their stream, src/filter.cpp:112-115) -- plus Kolmogorov forcing, so
+that the GPU filter can be compared group by group: same particle keys, same
+resampling uniforms (mt19937_64(stream_seed(seed, UINT64_MAX))), same
+reseeding of children.
 """
 from __future__ import annotations
 
@@ -19,11 +22,14 @@ def physical(orc, field):
 
 
 def oracle_bootstrap_filter(orc, initial, obs, dt, seed, nu, alpha=0.0, amp=0.1, kf=4,
-                            sigma=0.05, kmin=1, kmax=4):
-    """initial: (P, n, n) spectral particle states.  obs rows (step, x, y, value, sd)."""
+                            sigma=0.05, kmin=1, kmax=4, forcing="white", tau=0.5):
+    """initial: (P, n, n) spectral particle states.  obs rows (step, x, y, value, sd).
+    forcing: "white" or "ou" (OU processes started from the particle keys)."""
     states = np.array(initial, dtype=np.complex128)
     P, n = states.shape[0], states.shape[1]
     keys = [orc.stream_seed(seed, i) for i in range(P)]
+    ou = [orc.ou_process(n, sigma, tau, kmin, kmax, k) for k in keys] if forcing == "ou" else None
+    kolmogorov = orc.step_forcing(n, amp, kf, 0.0, kmin, kmax, dt, 0, 0)
     weights = np.full(P, 1.0 / P)
     rng = orc.mt(orc.stream_seed(seed, UINT64_MAX))
     steps = obs[:, 0].astype(np.int64)
@@ -32,9 +38,15 @@ def oracle_bootstrap_filter(orc, initial, obs, dt, seed, nu, alpha=0.0, amp=0.1,
     for g, target in enumerate(groups):
         sel = obs[steps == target]
         for i in range(P):
-            states[i] = orc.trajectory(n, nu, alpha, dt, int(target - current), keys[i],
-                                       amp=amp, kf=kf, sigma=sigma, kmin=kmin, kmax=kmax,
-                                       step0=current, initial=states[i])
+            if ou is None:
+                states[i] = orc.trajectory(n, nu, alpha, dt, int(target - current), keys[i],
+                                           amp=amp, kf=kf, sigma=sigma, kmin=kmin, kmax=kmax,
+                                           step0=current, initial=states[i])
+                continue
+            for _ in range(int(target - current)):
+                ou[i].transition(dt / 2)
+                states[i] = orc.rk4(states[i], nu, alpha, dt, 1, ou[i].F + kolmogorov)
+                ou[i].transition(dt / 2)
         logs = np.zeros(P)
         for i in range(P):
             ph = physical(orc, states[i])
@@ -49,6 +61,12 @@ def oracle_bootstrap_filter(orc, initial, obs, dt, seed, nu, alpha=0.0, amp=0.1,
             anc = orc.systematic_resample(w, u)
             states = states[anc].copy()
             keys = [orc.stream_seed(seed, (g + 1) * P + i) for i in range(P)]
+            if ou is not None:
+                kept = [ou[a].F.copy() for a in anc]
+                ou = [orc.ou_process(n, sigma, tau, kmin, kmax, k) for k in keys]
+                for proc, F, k in zip(ou, kept, keys):
+                    proc.reseed(k)  # reseed_future: the stream restarts, no draws
+                    proc.F = F      # the ancestor's forcing
             w = np.full(P, 1.0 / P)
             events += 1
             resampled = True

@@ -139,6 +139,60 @@ def test_reference_signature_filters_match_golden(tmp_path):
         np.testing.assert_allclose(field(r[3:], n).real, mean, rtol=0, atol=1e-10)
 
 
+OU_CSV = os.path.join(GOLDEN, "ou_observations_16.csv")
+
+
+def ou_golden():
+    return (np.load(os.path.join(GOLDEN, "ou_16.npz")),
+            json.load(open(os.path.join(GOLDEN, "ou_16.json"))))
+
+
+@pytest.mark.gpu
+def test_reference_signature_filter_with_ou_forcing_matches_golden(tmp_path):
+    """bootstrap_filter(data, p, 8, 555) with sigma > 0 runs the reference's OU
+    forcing: the reference's own result over its stochastic dataset (golden
+    ou_16), resampling events and per-group records included."""
+    g, meta = ou_golden()
+    n = meta["n"]
+    run("pf_ou", OU_CSV, tmp_path / "pf.bin")
+    raw = np.fromfile(tmp_path / "pf.bin")
+    assert raw[1] == meta["resampling_events"] >= 1
+    assert abs(raw[0] - meta["log_likelihood"]) < 1e-9 * abs(meta["log_likelihood"])
+    rec = 4 + 2 * n * n
+    for k, (want, mean) in enumerate(zip(g["records"], g["means"])):
+        r = raw[2 + k * rec: 2 + (k + 1) * rec]
+        assert abs(r[1] - want[0]) < 1e-9 and abs(r[2] - want[1]) < 1e-8
+        assert abs(r[3] - want[2]) < 1e-10
+        np.testing.assert_allclose(field(r[4:], n).real, mean, rtol=0, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_reference_signature_particle_metropolis_matches_golden(tmp_path):
+    """particle_metropolis with the reference's streams and prior: the same
+    proposals, likelihood seeds and accept/reject decisions as the reference's
+    chain, likelihoods to 1e-9 (each one a GPU filter with OU forcing)."""
+    g, _ = ou_golden()
+    count = len(g["chain_accepted"])
+    run("pmcmc", OU_CSV, count, tmp_path / "chain.bin", timeout=300)
+    raw = np.fromfile(tmp_path / "chain.bin").reshape(count, 8)
+    seeds = raw[:, 6].copy().view(np.uint64)
+    assert np.array_equal(seeds, g["chain_seeds"])
+    assert np.array_equal(raw[:, 7].astype(bool), g["chain_accepted"])
+    np.testing.assert_allclose(raw[:, :4], g["chain_q"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(raw[:, 4:6], g["chain_values"], rtol=1e-9, atol=0)
+
+
+@pytest.mark.gpu
+def test_reference_signature_ensemble_with_ou_forcing_matches_golden(tmp_path):
+    """ensemble(n, p, dt, steps, count, workers, seed) from zero vorticity with
+    OU forcing: every member's energy equals the reference's."""
+    g, _ = ou_golden()
+    run("ensemble_ou", tmp_path / "e.bin")
+    raw = np.fromfile(tmp_path / "e.bin").reshape(-1, 3)
+    np.testing.assert_allclose(raw[:, 0], g["ensemble_energy"], rtol=1e-10, atol=0)
+    assert np.all(raw[:, 0] > 0)
+
+
 @pytest.mark.gpu
 def test_reference_error_contract():
     assert run("errors").strip() == "ok"

@@ -53,7 +53,8 @@ def test_forced_pipeline_matches_oracle(emu, orc):
 @pytest.mark.parametrize("n,cv,rv", [(128, 1, 0), (128, 2, 0), (128, 3, 0), (128, 0, 1),
                                      (128, 0, 2), (128, 0, 3), (256, 2, 0), (256, 0, 3),
                                      (256, 0, 4), (128, 0, 4), (256, 0, 5),
-                                     (256, 7, 0), (256, 0, 9), (256, 0, 10)])
+                                     (256, 7, 0), (256, 0, 9), (256, 0, 10),
+                                     (256, 0, 11)])
 def test_launch_shape_variants_match_oracle(emu, orc, n, cv, rv):
     """Autotuner variants (other T x E splits, workgroup sizes, row tiles)
     compute the same RK4 step as the FP64 oracle."""
@@ -77,3 +78,41 @@ def test_calibration_drifts_and_ssprk3_match_oracle(emu, orc, precision, tol):
         assert rel(got["tangent"][m], orc.linearized_drift(S[m], D[m], nu, alpha)) < tol
         assert rel(got["adjoint"][m], orc.adjoint_drift(S[m], D[m], nu, alpha)) < tol
         assert rel(got["ssprk3"][m], orc.ssprk3(S[m], nu, alpha, 2e-3, 3)) < tol
+
+
+@pytest.mark.parametrize("n,kmin,kmax,members,calls", [(32, 1, 4, 3, 20), (64, 1, 8, 2, 5)])
+def test_ou_body_matches_oracle(emu, orc, n, kmin, kmax, members, calls):
+    """The device OU kernel body (parallel draws, two-phase regeneration of the
+    mt19937_64 block, resampling reseed) on the CPU lanes equals the oracle's
+    sequential NormalStream process; 98 pairs exercise more pairs than lanes and
+    a regeneration inside one call.  The midpoint forcing lands at the engine
+    pair index, conjugated where the canonical mode is the partner."""
+    first = orc.ou_pairs(n, kmin, kmax)
+    P = len(first)
+    rng = np.random.default_rng(1)
+    pmap = np.arange(P)
+    rng.shuffle(pmap)
+    conj = rng.random(P) < 0.4
+    emap = np.where(conj, ~pmap, pmap)
+    keys0 = np.array([orc.stream_seed(99, m) for m in range(members)], dtype=np.uint64)
+    modes = [0] + [1] * (calls - 1)
+    modes[calls // 2] = 2
+    keys = np.tile(keys0, (calls, 1))
+    keys[calls // 2] = [orc.stream_seed(7, m) for m in range(members)]
+    hs = [0.0] + [0.004] * (calls - 1)
+    sigma = np.array([0.05 * (m + 1) for m in range(members)])
+    tau = 0.37
+    F, noise = emu.ou(P, emap, keys, sigma, tau, modes, hs)
+    for m in range(members):
+        proc = orc.ou_process(n, sigma[m], tau, kmin, kmax, int(keys0[m]))
+        for c in range(calls):
+            if c > 0:
+                if modes[c] == 2:
+                    proc.reseed(int(keys[c][m]))
+                else:
+                    proc.transition(hs[c])
+            want = proc.F.ravel()[first]
+            assert np.abs(F[c, m] - want).max() <= 1e-15 * np.abs(want).max()
+            if modes[c] == 1:
+                got = np.where(conj, np.conj(noise[c, m][pmap]), noise[c, m][pmap])
+                assert np.abs(got - want).max() <= 1e-15 * np.abs(want).max()

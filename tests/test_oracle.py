@@ -398,3 +398,99 @@ def test_tangent_and_adjoint_match_reference(orc, ref):
     s, d, lam = orc.grf(n, 51), orc.grf(n, 52), orc.grf(n, 53)
     assert rel(orc.linearized_drift(s, d, nu, alpha), ref.linearized_drift(s, d, nu, alpha)) < 1e-14
     assert rel(orc.adjoint_drift(s, lam, nu, alpha), ref.adjoint_drift(s, lam, nu, alpha)) < 1e-14
+
+
+# -- OU shell forcing (src/solver.cpp:67-119,217-254) ---------------------------
+def test_ou_pairs_are_the_forcing_shell(orc):
+    """The reference-order pair list holds every shell mode of the strict 2/3
+    mask exactly once up to conjugation, with the first mode before its
+    partner, and has the size of the engine's canonical shell."""
+    for n, kmin, kmax in ((16, 2, 3), (32, 1, 4), (64, 1, 8)):
+        first = orc.ou_pairs(n, kmin, kmax)
+        assert np.all(np.diff(first) > 0)
+        seen = set()
+        for i in first:
+            y, x = divmod(int(i), n)
+            a, b = (x if x < n // 2 else x - n), (y if y < n // 2 else y - n)
+            assert kmin * kmin <= a * a + b * b <= kmax * kmax
+            assert 3 * abs(a) < n and 3 * abs(b) < n
+            partner = ((n - y) % n) * n + (n - x) % n
+            assert i < partner
+            seen.add((a, b))
+            seen.add((-a, -b))
+        brute = {(a, b) for a in range(-kmax, kmax + 1) for b in range(-kmax, kmax + 1)
+                 if kmin * kmin <= a * a + b * b <= kmax * kmax and 3 * abs(a) < n
+                 and 3 * abs(b) < n}
+        assert seen == brute
+        assert len(first) == orc.lib.orc_shell_modes(n, kmin, kmax, None, None)
+
+
+def test_ou_process_statistics(orc):
+    """Stationary start: each component has variance sigma^2 tau / 4; after an
+    interval h the coefficient keeps e^{-h/tau} of itself and the variance is
+    unchanged (exact OU transition).  Sampled over 2000 independent streams."""
+    n, sigma, tau, h = 16, 0.3, 0.5, 0.2
+    first = orc.ou_pairs(n, 1, 2)
+    start, after = [], []
+    for s in range(2000):
+        p = orc.ou_process(n, sigma, tau, 1, 2, orc.stream_seed(17, s))
+        start.append(p.F.ravel()[first].copy())
+        p.transition(h)
+        after.append(p.F.ravel()[first].copy())
+    start, after = np.array(start), np.array(after)
+    var = sigma * sigma * tau / 4
+    for part in (np.real, np.imag):
+        assert abs(part(start).var() / var - 1) < 0.05
+        assert abs(part(after).var() / var - 1) < 0.05
+        corr = np.mean(part(start) * part(after)) / var
+        assert abs(corr - np.exp(-h / tau)) < 0.03
+
+
+def test_ou_trajectory_matches_reference_golden(orc):
+    """orc_run_ou against the reference's own Solver(16, p, seed) with OU
+    forcing (golden ou_16, written by oracle/_ref)."""
+    import json
+    g = np.load(os.path.join(GOLDEN, "ou_16.npz"))
+    meta = json.load(open(os.path.join(GOLDEN, "ou_16.json")))
+    n = meta["n"]
+    w0 = full_from_half(g["initial"], n)
+    for run in meta["runs"]:
+        w, F = orc.run_ou(w0, meta["nu"], meta["alpha"], meta["sigma"], meta["tau"],
+                          run["kmin"], run["kmax"], run["seed"], meta["dt"], run["steps"])
+        key = f"run_{run['seed']}"
+        assert rel(w, full_from_half(g[key + "_state"], n)) < 1e-14
+        assert rel(F, full_from_half(g[key + "_forcing"], n)) < 1e-14
+
+
+def test_ou_filter_matches_reference_golden(orc):
+    """The oracle bootstrap filter with OU forcing reproduces the reference's
+    bootstrap_filter over a stochastic dataset (golden ou_16): likelihood,
+    resampling events, per-group records and weighted means."""
+    import json
+    from oracle_filters import oracle_bootstrap_filter
+    g = np.load(os.path.join(GOLDEN, "ou_16.npz"))
+    meta = json.load(open(os.path.join(GOLDEN, "ou_16.json")))
+    n, P = meta["n"], meta["particles"]
+    w0 = full_from_half(g["initial"], n)
+    got = oracle_bootstrap_filter(orc, np.stack([w0] * P), g["rows"], meta["dt"], meta["seed"],
+                                  meta["nu"], alpha=meta["alpha"], amp=0.0,
+                                  sigma=meta["sigma"], kmin=2, kmax=3, forcing="ou",
+                                  tau=meta["tau"])
+    assert meta["resampling_events"] >= 1
+    assert got["resampling_events"] == meta["resampling_events"]
+    assert abs(got["log_likelihood"] - meta["log_likelihood"]) < 1e-12
+    for h, rec, mean in zip(got["history"], g["records"], g["means"]):
+        np.testing.assert_allclose([h["increment"], h["ess"], h["max_weight"]], rec, rtol=0,
+                                   atol=1e-12)
+        np.testing.assert_allclose(orc.fft2d(h["mean_field"], True).real, mean, rtol=0,
+                                   atol=1e-13)
+
+
+def test_ou_matches_reference_live(orc, ref):
+    n = 16
+    w0 = orc.grf(n, 5)
+    for sigma, tau, kmin, kmax, seed, dt, steps in ((0.2, 0.5, 2, 3, 77, 0.01, 7),
+                                                     (0.05, 0.3, 1, 4, 123456789, 0.005, 40)):
+        a, Fa = orc.run_ou(w0, 0.02, 0.1, sigma, tau, kmin, kmax, seed, dt, steps)
+        b, Fb = ref.run_ou(w0, 0.02, 0.1, sigma, tau, kmin, kmax, seed, dt, steps)
+        assert rel(a, b) < 1e-15 and rel(Fa, Fb) < 1e-15
