@@ -4,6 +4,9 @@
 
 Products (all in-tree, git-ignored, shipped to the GPU host with the tree):
     hzl-stochasticflowinference_amd/libhzl_amd.so   C-ABI engine, HIP for gfx950
+    hzl-stochasticflowinference_amd/libhzl_gpu.so   C++ drop-in (reference signatures)
+    hzl-stochasticflowinference_amd/hzl_amd_cli     the reference CLI's ensemble / filter /
+                                                    particle-sample commands on the GPU
 Test infrastructure (never loaded by the product path):
     oracle/liboracle.so          FP64 C oracle
     oracle/_ref/libhzlref.so     unmodified reference CPU library (only where
@@ -64,6 +67,8 @@ DROPIN_SRC = os.path.join(CSRC, "host", "hzl_gpu.cpp")
 DROPIN_LIB = os.path.join(PKG, "libhzl_gpu.so")
 DROPIN_CHECK_SRC = os.path.join(ROOT, "tests", "dropin", "dropin_check.cpp")
 DROPIN_CHECK = os.path.join(ROOT, "tests", "dropin", "dropin_check")
+CLI_SRC = os.path.join(CSRC, "host", "hzl_cli.cpp")
+CLI = os.path.join(PKG, "hzl_amd_cli")
 
 
 def build_dropin() -> str:
@@ -79,6 +84,10 @@ def build_dropin() -> str:
           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../hzl-stochasticflowinference_amd",
           "-o", tmp])
     os.replace(tmp, DROPIN_CHECK)
+    tmp = CLI + ".tmp"
+    _run(["g++", *flags, CLI_SRC, "-L", PKG, "-lhzl_gpu", "-lhzl_amd", "-Wl,-rpath,$ORIGIN",
+          "-o", tmp])
+    os.replace(tmp, CLI)
     return DROPIN_LIB
 
 

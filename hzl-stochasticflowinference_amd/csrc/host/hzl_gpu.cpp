@@ -863,8 +863,11 @@ std::vector<Diagnostics> ensemble(std::size_t n, Parameters p, double dt, std::s
     e.step(dt, steps);
     e.synchronize();
     auto d = e.diagnostics();
+    double time = 0; // Solver::time(): t += dt per step (src/solver.cpp:253)
+    for (std::size_t s = 0; s < steps; ++s)
+        time += dt;
     for (auto &x : d)
-        x.time = dt * (double)steps;
+        x.time = time;
     return d;
 }
 

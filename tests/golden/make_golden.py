@@ -152,6 +152,49 @@ def ou16(ref: Reference) -> None:
          **arrays)
 
 
+# The reference CLI's flows (src/main.cpp:384-526), run by the reference's own
+# executable (oracle/_ref/hzl_ref_cli, make -C oracle ref-cli) in a scratch
+# directory; every file it writes is kept under cli/<case>/.
+CLI_CASES = {
+    "filter_particle": ["filter", "--data", "{golden}/ou_observations_16.csv", "--particles", "8",
+                        "--seed", "555", "--nu", "0.03", "--alpha", "0.1", "--sigma", "0.2",
+                        "--tau", "0.5"],
+    "filter_enkf": ["filter", "--method", "enkf", "--data", "{golden}/observations_v2_16.csv",
+                    "--particles", "8", "--seed", "555", "--nu", "0.03", "--alpha", "0.1",
+                    "--sigma", "0", "--inflation", "1.02", "--forecast-steps", "5",
+                    "--forecast-every", "2"],
+    "particle_sample": ["particle-sample", "--data", "{golden}/ou_observations_16.csv",
+                        "--samples", "4", "--particles", "8", "--proposal", "0.1", "--seed",
+                        "123"],
+    "ensemble": ["ensemble", "--n", "16", "--steps", "20", "--count", "4", "--workers", "2",
+                 "--dt", "0.01", "--seed", "42", "--nu", "0.03", "--alpha", "0.1", "--sigma",
+                 "0.2", "--tau", "0.5"],
+}
+
+
+def cli16() -> None:
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "hzl_ref_cli")
+    if not os.path.exists(exe):
+        raise SystemExit("build the reference CLI first: make -C oracle ref-cli")
+    for case, argv in CLI_CASES.items():
+        args = [a.replace("{golden}", HERE) for a in argv]
+        with tempfile.TemporaryDirectory() as tmp:
+            p = subprocess.run([exe, *args, "--output", "out"], cwd=tmp, capture_output=True,
+                               text=True, check=True)
+            dst = os.path.join(HERE, "cli", case)
+            shutil.rmtree(dst, ignore_errors=True)
+            shutil.copytree(os.path.join(tmp, "out"), dst)
+            with open(os.path.join(dst, "stdout.txt"), "w") as f:
+                f.write(p.stdout)
+        print("wrote", dst)
+    with open(os.path.join(HERE, "cli", "cases.json"), "w") as f:
+        json.dump(CLI_CASES, f, indent=2, sort_keys=True)
+        f.write("\n")
+
+
 def checkpoint16(ref: Reference) -> None:
     """A reference Solver::save file (OU forcing active, random stream
     advanced) and the reference's own Solver::load diagnostics of it."""
@@ -203,7 +246,8 @@ def adjoint16(ref: Reference) -> None:
 
 def main() -> None:
     orc, ref = Oracle(), Reference()
-    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16", "adjoint16", "ou16"]
+    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16", "adjoint16", "ou16",
+                             "cli16"]
     if "config0" in which:
         config0(orc, ref)
     if "drift32" in which:
@@ -216,6 +260,8 @@ def main() -> None:
         adjoint16(ref)
     if "ou16" in which:
         ou16(ref)
+    if "cli16" in which:
+        cli16()
 
 
 if __name__ == "__main__":
