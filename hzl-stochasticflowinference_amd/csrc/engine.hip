@@ -1500,10 +1500,16 @@ template <typename R> class Engine final : public EngineBase {
 
     EngineViews views() override {
         flush_parameters();
-        return EngineViews{n_,        K_,        (int)B_,        sizeof(R) == 4 ? 0 : 1,
-                           w0_.p,     d_nu_.p,   d_alpha_.p,     d_sigma_.p,
-                           d_amp_.p,  d_keys_.p, pairs_.p,       p_.forcing_min,
-                           p_.forcing_max, p_.kolmogorov_wavenumber, stream_};
+        EngineViews v{n_,        K_,        (int)B_,        sizeof(R) == 4 ? 0 : 1,
+                      w0_.p,     d_nu_.p,   d_alpha_.p,     d_sigma_.p,
+                      d_amp_.p,  d_keys_.p, pairs_.p,       p_.forcing_min,
+                      p_.forcing_max, p_.kolmogorov_wavenumber, stream_};
+        if (ou_) {
+            v.ou_F = ou_F_.p;
+            v.npairs = npairs_;
+            v.ou_off = ou_off_.p;
+        }
+        return v;
     }
 
     void invalidate_mixed() override {
@@ -1933,7 +1939,7 @@ template <typename R> class Engine final : public EngineBase {
             // each conjugate pair, i < partner, inside the mask and the shell
             const int n = n_, w = 2 * p_.forcing_max + 1;
             const auto table = pair_table(p_.forcing_min, p_.forcing_max);
-            std::vector<int> map;
+            std::vector<int> map, off;
             ou_first_.clear();
             for (int y = 0; y < n; ++y)
                 for (int x = 0; x < n; ++x) {
@@ -1950,6 +1956,8 @@ template <typename R> class Engine final : public EngineBase {
                         if (p < 0)
                             throw std::logic_error("OU pair outside the engine's shell table");
                         map.push_back(flip ? ~p : p);
+                        const int o = c * J_ + kb + K_;
+                        off.push_back(flip ? ~o : o);
                         ou_first_.push_back(i);
                     }
                 }
@@ -1957,6 +1965,8 @@ template <typename R> class Engine final : public EngineBase {
                 throw std::logic_error("OU pair count differs from the engine's shell");
             ou_map_ = DeviceBuffer<int>(map.size());
             upload(ou_map_.p, map.data(), map.size());
+            ou_off_ = DeviceBuffer<int>(off.size());
+            upload(ou_off_.p, off.data(), off.size());
             ou_mt_ = DeviceBuffer<uint64_t>((size_t)(MT_N + 1) * B_);
             ou_F_ = DeviceBuffer<cx<double>>(B_ * (size_t)npairs_);
             ou_tmp_ = DeviceBuffer<cx<double>>(B_ * (size_t)npairs_);
@@ -2132,7 +2142,7 @@ template <typename R> class Engine final : public EngineBase {
     bool ou_ = false;
     DeviceBuffer<uint64_t> ou_mt_;
     DeviceBuffer<cx<double>> ou_F_, ou_tmp_;
-    DeviceBuffer<int> ou_map_;
+    DeviceBuffer<int> ou_map_, ou_off_;
     std::vector<long> ou_first_;
     bool params_dirty_ = false, mix_valid_ = false;
     uint64_t step_ = 0;

@@ -56,6 +56,13 @@ struct EngineViews {
     const int *pair_of;
     int kmin, kmax, kf;
     hipStream_t stream;
+    // OU forcing process (null / 0 with the white process): coefficients
+    // [B][npairs] in the reference's pair order and, per pair, the half-plane
+    // offset c (2K+1) + ky + K of its canonical mode (~offset when the pair's
+    // first mode is the conjugate of the stored one)
+    cx<double> *ou_F = nullptr;
+    int npairs = 0;
+    const int *ou_off = nullptr;
 };
 
 // State that another module keeps per ensemble handle (owned by the handle).

@@ -187,6 +187,90 @@ __global__ void k_enkf_mean_partial(const cx<R> *w, int B, long spec, int chunks
     partial[(long)c * spec + e] = {re, im};
 }
 
+// ---- OU forcing state (the reference's latent forcing, src/filter.cpp:237-319):
+// F[B][P] coefficients per shell pair, one thread per pair.
+// Inflation F = mean + lambda (F - mean), mean = sum over members of F / B.
+__global__ void __launch_bounds__(256) k_ou_inflate(cx<double> *F, int B, int P, double lambda) {
+    const int j = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (j >= P)
+        return;
+    cx<double> mu = {0, 0};
+    for (int i = 0; i < B; ++i) {
+        mu.x += F[(long)i * P + j].x / B;
+        mu.y += F[(long)i * P + j].y / B;
+    }
+    for (int i = 0; i < B; ++i) {
+        cx<double> &f = F[(long)i * P + j];
+        f = {mu.x + lambda * (f.x - mu.x), mu.y + lambda * (f.y - mu.y)};
+    }
+}
+
+// fcov[j] = sum over members of (F_ij - mean_j) delta_i (delta prescaled by 1/(M-1)).
+__global__ void __launch_bounds__(256) k_ou_cov(const cx<double> *F, int B, int P,
+                                                const double *delta, cx<double> *fcov) {
+    const int j = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (j >= P)
+        return;
+    cx<double> mu = {0, 0}, c = {0, 0};
+    for (int i = 0; i < B; ++i) {
+        mu.x += F[(long)i * P + j].x / B;
+        mu.y += F[(long)i * P + j].y / B;
+    }
+    for (int i = 0; i < B; ++i) {
+        c.x += (F[(long)i * P + j].x - mu.x) * delta[i];
+        c.y += (F[(long)i * P + j].y - mu.y) * delta[i];
+    }
+    fcov[j] = c;
+}
+
+// y = 0 over n complex entries (a fill kernel, safe under graph capture).
+__global__ void __launch_bounds__(256) k_zero_cx(cx<double> *y, long n) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e < n)
+        y[e] = {0, 0};
+}
+
+// Pair covariance into a half-plane field (zero elsewhere), for the taper.
+__global__ void __launch_bounds__(256) k_ou_scatter(const cx<double> *fcov, int P,
+                                                    const int *off, int K, cx<double> *field) {
+    const int j = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (j >= P)
+        return;
+    const int J = 2 * K + 1, o = off[j] >= 0 ? off[j] : ~off[j];
+    const cx<double> v = off[j] >= 0 ? fcov[j] : cx<double>{fcov[j].x, -fcov[j].y};
+    field[o] = v;
+    if (o / J == 0) // kx = 0: the conjugate mode is stored too
+        field[(o / J) * J + (2 * K - o % J)] = {v.x, -v.y};
+}
+
+// Tapered pair covariance back from the half-plane field, projected as
+// update_latent_state projects the forcing (src/solver.cpp:412-416): a pair
+// is the mean of its mode and its partner's conjugate.
+__global__ void __launch_bounds__(256) k_ou_gather(const cx<double> *field, int P,
+                                                   const int *off, int K, cx<double> *fcov) {
+    const int j = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (j >= P)
+        return;
+    const int J = 2 * K + 1, o = off[j] >= 0 ? off[j] : ~off[j];
+    cx<double> v = field[o];
+    if (o / J == 0) {
+        const cx<double> m = field[2 * K - o % J];
+        v = {0.5 * (v.x + m.x), 0.5 * (v.y - m.y)};
+    }
+    fcov[j] = off[j] >= 0 ? v : cx<double>{v.x, -v.y};
+}
+
+// F_ij += fcov_j g_i (the member's scaled innovation).
+__global__ void __launch_bounds__(256) k_ou_update(cx<double> *F, int B, int P,
+                                                   const cx<double> *fcov, const double *gain) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)B * P)
+        return;
+    const int i = (int)(e / P), j = (int)(e % P);
+    const double g = gain[i];
+    F[e] = {F[e].x + fcov[j].x * g, F[e].y + fcov[j].y * g};
+}
+
 // Multiplicative anomaly inflation w = mean + lambda (w - mean).
 template <typename R>
 __global__ void k_enkf_inflate(cx<R> *w, int B, long spec, const cx<double> *mean,
@@ -381,7 +465,7 @@ struct EnkfState : ModuleState {
     NormalStream normals;
     Buf<cx<double>> ex, ey, partial, cov, mean;
     Buf<double> pred, pred2, delta, gain, eps, spread, value, sd, spart, cs, sn, phys;
-    Buf<cx<double>> mixed;
+    Buf<cx<double>> mixed, fcov, ffield;
     // batch mode (FP32)
     Buf<float> H, HX, HA, P, S, Sinv, Dm, Z, HXpart;
     EnkfState(int mode_, uint64_t seed) : mode(mode_), normals(seed) {}
@@ -507,6 +591,9 @@ template <typename R> struct Enkf {
             mean();
             hipLaunchKernelGGL(k_enkf_inflate<R>, grid1d((long)B * spec), dim3(256), 0,
                                ev.stream, w, B, spec, st.mean.p, inflation);
+            if (ev.ou_F)
+                hipLaunchKernelGGL(k_ou_inflate, grid1d(ev.npairs), dim3(256), 0, ev.stream,
+                                   ev.ou_F, B, ev.npairs, inflation);
         }
         // forecast sensor RMSE of the ensemble mean, before any update
         // (src/filter.cpp:254-264)
@@ -541,6 +628,11 @@ template <typename R> struct Enkf {
     }
 
     void localize(int xo, int yo, double radius) {
+        localize(st.cov.p, xo, yo, radius);
+    }
+
+    // taper the half-plane FP64 field cov in place
+    void localize(cx<double> *cov, int xo, int yo, double radius) {
         const int n = ev.n;
         if (!st.cs.p || st.cs.n < (size_t)n) {
             std::vector<double> c(n), sv(n);
@@ -554,14 +646,14 @@ template <typename R> struct Enkf {
         const long mix = (long)(K + 1) * n;
         st.mixed.alloc((size_t)mix);
         st.phys.alloc((size_t)n * n);
-        hipLaunchKernelGGL(k_loc_inv_y, grid1d(mix), dim3(256), 0, ev.stream, st.cov.p, K, n,
+        hipLaunchKernelGGL(k_loc_inv_y, grid1d(mix), dim3(256), 0, ev.stream, cov, K, n,
                            st.cs.p, st.sn.p, st.mixed.p);
         hipLaunchKernelGGL(k_loc_inv_x_taper, grid1d((long)n * n), dim3(256), 0, ev.stream,
                            st.mixed.p, K, n, st.cs.p, st.sn.p, xo, yo, radius, st.phys.p);
         hipLaunchKernelGGL(k_loc_fwd_x, grid1d(mix), dim3(256), 0, ev.stream, st.phys.p, K, n,
                            st.cs.p, st.sn.p, st.mixed.p);
         hipLaunchKernelGGL(k_loc_fwd_y, grid1d(spec), dim3(256), 0, ev.stream, st.mixed.p, K, n,
-                           st.cs.p, st.sn.p, st.cov.p);
+                           st.cs.p, st.sn.p, cov);
     }
 
     void serial(size_t nobs, const int *x, const int *y, const double *value, const double *sd,
@@ -586,6 +678,26 @@ template <typename R> struct Enkf {
                                st.partial.p, chunks, spec, st.cov.p);
             if (radius > 0)
                 localize(x[o], y[o], radius);
+            if (ev.ou_F) {
+                // the forcing state: its covariance with the prediction (all
+                // members before any update), tapered like the state's
+                const int P = ev.npairs;
+                st.fcov.alloc((size_t)P);
+                hipLaunchKernelGGL(k_ou_cov, grid1d(P), dim3(256), 0, ev.stream, ev.ou_F, B, P,
+                                   st.delta.p, st.fcov.p);
+                if (radius > 0) {
+                    st.ffield.alloc((size_t)spec);
+                    hipLaunchKernelGGL(k_zero_cx, grid1d(spec), dim3(256), 0, ev.stream,
+                                       st.ffield.p, spec);
+                    hipLaunchKernelGGL(k_ou_scatter, grid1d(P), dim3(256), 0, ev.stream,
+                                       st.fcov.p, P, ev.ou_off, K, st.ffield.p);
+                    localize(st.ffield.p, x[o], y[o], radius);
+                    hipLaunchKernelGGL(k_ou_gather, grid1d(P), dim3(256), 0, ev.stream,
+                                       st.ffield.p, P, ev.ou_off, K, st.fcov.p);
+                }
+                hipLaunchKernelGGL(k_ou_update, grid1d((long)B * P), dim3(256), 0, ev.stream,
+                                   ev.ou_F, B, P, st.fcov.p, st.gain.p);
+            }
             const bool more = o + 1 < nobs;
             hipLaunchKernelGGL(k_enkf_update<R>, dim3((unsigned)B), dim3(256), 0, ev.stream, w,
                                spec, K, st.cov.p, st.gain.p,
@@ -717,11 +829,11 @@ int hzl_enkf_begin(hzl_ensemble_t h, uint64_t seed, int mode) {
             throw std::invalid_argument("EnKF mode must be HZL_ENKF_SERIAL or HZL_ENKF_BATCH");
         if (mode == HZL_ENKF_BATCH && v.precision != HZL_FP32)
             throw std::invalid_argument("batch EnKF analysis needs an FP32 ensemble");
-        // the reference's analysis also updates the OU forcing as a latent
-        // state (src/filter.cpp:237-253,311-321); this analysis updates the
-        // vorticity only, so it runs with the white-in-time forcing
-        if (e.forcing_process() == HZL_FORCING_OU)
-            throw std::invalid_argument("EnKF runs with the white forcing process, not OU");
+        // the OU forcing is a latent state of the serial analysis, as in the
+        // reference (src/filter.cpp:237-253,273-321); the batch analysis
+        // updates the vorticity only
+        if (mode == HZL_ENKF_BATCH && e.forcing_process() == HZL_FORCING_OU)
+            throw std::invalid_argument("batch EnKF analysis runs with the white forcing process");
         e.enkf = std::make_unique<hzl_amd::EnkfState>(
             mode, hzl_amd::stream_seed(seed, (uint64_t)v.B));
     });

@@ -847,8 +847,12 @@ std::vector<EnsembleFilterRecord> ensemble_kalman_filter(const Dataset &data, Pa
     if (count < 2)
         throw std::invalid_argument("EnKF requires at least two ensemble members");
     p = with_data_shell(p, data);
+    p.validate(data.n);
+    // member i is Solver(n, p, stream_seed(seed, i)) from multimode; its OU
+    // forcing is part of the analysed state (src/filter.cpp:217-223,237-321)
     Ensemble e(data.n, count, p, Precision::fp64, seed);
     e.set_all(initial_condition(data.n, "multimode"));
+    e.set_forcing_process(ForcingProcess::ornstein_uhlenbeck);
     return ensemble_kalman_filter(e, data, seed, options, false);
 }
 

@@ -13,14 +13,13 @@
 //   averaged_particle_log_likelihood, particle_metropolis, stochastic_log_prior
 //   and their records           (include/hzl/filter.hpp:5-54)
 // compiles and links against libhzl_gpu.so instead of the reference library.
-// The reference-signature solver functions (bootstrap_filter, ensemble and the
-// particle-likelihood family) run the reference's own forcing process: the
-// Ornstein-Uhlenbeck shell forcing with one mt19937_64 NormalStream per member
-// seeded as the reference seeds its Solvers.  Differences of behaviour:
-//   * ensemble_kalman_filter runs with the engine's white-in-time forcing and
-//     updates the vorticity only (the reference also updates the OU forcing
-//     as a latent state); with forcing_sigma = 0 it reproduces the
-//     reference's results;
+// The reference-signature solver functions (bootstrap_filter,
+// ensemble_kalman_filter, ensemble and the particle-likelihood family) run the
+// reference's own forcing process: the Ornstein-Uhlenbeck shell forcing with
+// one mt19937_64 NormalStream per member seeded as the reference seeds its
+// Solvers (the EnKF analyses it as a latent state).  Differences of behaviour:
+//   * the batch (MFMA) EnKF analysis of an hzl::Ensemble updates the vorticity
+//     only and needs the white-in-time forcing;
 //   * hzl::Ensemble defaults to the white-in-time forcing (variance
 //     sigma^2 / (2 dt) per component held through the step); select the OU
 //     process with set_forcing_process;
@@ -221,7 +220,7 @@ struct EnKFOptions {
     std::size_t forecast_every = 1;
 };
 // Serial perturbed-observation EnKF; members start from initialize("multimode")
-// (src/filter.cpp:204-356).
+// with OU forcing, analysed together with the vorticity (src/filter.cpp:204-356).
 std::vector<EnsembleFilterRecord> ensemble_kalman_filter(const Dataset &data, Parameters p,
                                                          std::size_t members, std::uint64_t seed,
                                                          EnKFOptions options = {});

@@ -123,6 +123,8 @@ def ou16(ref: Reference) -> None:
         runs[f"run_{seed}"] = (w, F, dict(seed=seed, kmin=kmin, kmax=kmax, steps=steps))
     h, rows = ref.synthetic(n, dt, 60, 10, nu, alpha, sigma, tau, 2, 3, 0.05, 987)
     pf = ref.bootstrap_filter(h, nu, alpha, sigma, tau, 8, 555, means=True, n=n)
+    enkf = ref.enkf(h, nu, alpha, sigma, tau, 8, 555, n=n)
+    enkf_loc = ref.enkf(h, nu, alpha, sigma, tau, 8, 555, inflation=1.05, radius=3.0, n=n)
     q0 = np.log([nu, alpha, sigma, tau])
     chain = ref.particle_metropolis(h, q0, 0.1, 6, 8, 123)
     # the same stochastic dataset as written by the reference's Dataset::save
@@ -135,7 +137,9 @@ def ou16(ref: Reference) -> None:
                                          for c in chain]),
                   chain_seeds=np.array([c["likelihood_seed"] for c in chain], dtype=np.uint64),
                   chain_accepted=np.array([c["accepted"] for c in chain]),
-                  ensemble_energy=energies)
+                  ensemble_energy=energies, enkf_records=enkf["records"],
+                  enkf_means=enkf["means"], enkf_loc_records=enkf_loc["records"],
+                  enkf_loc_means=enkf_loc["means"])
     for k, (w, F, _) in runs.items():
         arrays[k + "_state"] = half(w)
         arrays[k + "_forcing"] = half(F)
@@ -143,7 +147,9 @@ def ou16(ref: Reference) -> None:
          dict(recipe="Solver(16, p, seed) with OU forcing from initialize('multimode'), "
                      "`steps` x Solver::step(dt) (ref_run_ou); synthetic_observations(16, "
                      "0.01, 60, 10, p, 0.05, 987) with sigma 0.2, tau 0.5 and "
-                     "bootstrap_filter(data, p, 8, 555); particle_metropolis(data, "
+                     "bootstrap_filter(data, p, 8, 555); ensemble_kalman_filter(data, p, "
+                     "8, 555) plain and with inflation 1.05, localisation radius 3; "
+                     "particle_metropolis(data, "
                      "log(nu, alpha, sigma, tau), 0.1, 6, 8, 123); ensemble(16, p, 0.01, 20, "
                      "4, 2, 42) energies",
               n=n, nu=nu, alpha=alpha, sigma=sigma, tau=tau, dt=dt, particles=8, seed=555,
@@ -159,6 +165,10 @@ CLI_CASES = {
     "filter_particle": ["filter", "--data", "{golden}/ou_observations_16.csv", "--particles", "8",
                         "--seed", "555", "--nu", "0.03", "--alpha", "0.1", "--sigma", "0.2",
                         "--tau", "0.5"],
+    "filter_enkf_ou": ["filter", "--method", "enkf", "--data", "{golden}/ou_observations_16.csv",
+                       "--particles", "8", "--seed", "555", "--nu", "0.03", "--alpha", "0.1",
+                       "--sigma", "0.2", "--tau", "0.5", "--inflation", "1.05",
+                       "--localization-radius", "3", "--forecast-steps", "3"],
     "filter_enkf": ["filter", "--method", "enkf", "--data", "{golden}/observations_v2_16.csv",
                     "--particles", "8", "--seed", "555", "--nu", "0.03", "--alpha", "0.1",
                     "--sigma", "0", "--inflation", "1.02", "--forecast-steps", "5",

@@ -120,27 +120,60 @@ def localized_scalar_update(orc, states, h, y, sd, eps, xo, yo, radius):
     return (X + g[:, None] * cov.ravel()[None, :]).reshape(P, n, n)
 
 
+def project_forcing(orc, F, kmin, kmax):
+    """update_latent_state's forcing part (src/solver.cpp:412-416): keep the
+    shell pairs, each the mean of itself and its partner's conjugate."""
+    n = F.shape[0]
+    out = np.zeros_like(F)
+    flat, res = F.ravel(), out.reshape(-1)
+    for i in orc.ou_pairs(n, kmin, kmax):
+        y, x = divmod(int(i), n)
+        j = ((n - y) % n) * n + (n - x) % n
+        res[i] = 0.5 * (flat[i] + np.conj(flat[j]))
+        res[j] = np.conj(res[i])
+    return out
+
+
 def oracle_enkf(orc, initial, obs, dt, ens_seed, filter_seed, nu, alpha=0.0, amp=0.1, kf=4,
-                sigma=0.05, kmin=1, kmax=4, inflation=1.0, mode="serial", radius=0.0):
+                sigma=0.05, kmin=1, kmax=4, inflation=1.0, mode="serial", radius=0.0,
+                forcing="white", tau=0.5):
     """Perturbed-observation EnKF in FP64 on full spectra, restating reference
-    ensemble_kalman_filter (src/filter.cpp:204-356) with the engine's forcing
-    model; mode "batch" applies the dense all-observation analysis instead of
-    the scalar sequence, with the same perturbations."""
+    ensemble_kalman_filter (src/filter.cpp:204-356) with either forcing process:
+    "white" (the engine's, no forcing state) or "ou" (the reference's: each
+    member's OU forcing is inflated and updated with its own covariance,
+    tapered like the state's, then projected on the shell pairs); mode "batch"
+    applies the dense all-observation analysis instead of the scalar sequence,
+    with the same perturbations (white forcing only)."""
     states = np.array(initial, dtype=np.complex128)
     P, n = states.shape[0], states.shape[1]
     keys = [orc.stream_seed(ens_seed, i) for i in range(P)]
+    ou = [orc.ou_process(n, sigma, tau, kmin, kmax, k) for k in keys] if forcing == "ou" else None
+    if ou is not None and mode != "serial":
+        raise ValueError("the OU forcing state is updated by the serial analysis only")
+    kolmogorov = orc.step_forcing(n, amp, kf, 0.0, kmin, kmax, dt, 0, 0)
     normals = orc.mt(orc.stream_seed(filter_seed, P))
     steps = obs[:, 0].astype(np.int64)
     current, history = 0, []
     for target in np.unique(steps):
         sel = obs[steps == target]
         for i in range(P):
-            states[i] = orc.trajectory(n, nu, alpha, dt, int(target - current), keys[i],
-                                       amp=amp, kf=kf, sigma=sigma, kmin=kmin, kmax=kmax,
-                                       step0=current, initial=states[i])
+            if ou is None:
+                states[i] = orc.trajectory(n, nu, alpha, dt, int(target - current), keys[i],
+                                           amp=amp, kf=kf, sigma=sigma, kmin=kmin, kmax=kmax,
+                                           step0=current, initial=states[i])
+                continue
+            for _ in range(int(target - current)):
+                ou[i].transition(dt / 2)
+                states[i] = orc.rk4(states[i], nu, alpha, dt, 1, ou[i].F + kolmogorov)
+                ou[i].transition(dt / 2)
         if inflation > 1:
             mu = states.mean(axis=0)
             states = mu + inflation * (states - mu)
+            if ou is not None:
+                Fs = np.stack([p.F for p in ou])
+                muF = Fs.mean(axis=0)
+                for p, F in zip(ou, muF + inflation * (Fs - muF)):
+                    p.F = project_forcing(orc, F, kmin, kmax)
         fc = physical(orc, states.mean(axis=0))
         xs, ys = sel[:, 1].astype(int), sel[:, 2].astype(int)
         rmse = float(np.sqrt(np.mean((fc[ys, xs] - sel[:, 3]) ** 2)))
@@ -148,6 +181,18 @@ def oracle_enkf(orc, initial, obs, dt, ens_seed, filter_seed, nu, alpha=0.0, amp
         if mode == "serial":
             for o, row in enumerate(sel):
                 h = np.array([physical(orc, s)[int(row[2]), int(row[1])] for s in states])
+                if ou is not None:
+                    # the forcing state's update (src/filter.cpp:273-319)
+                    Fs = np.stack([p.F for p in ou])
+                    d = h - h.mean()
+                    var = np.sum(d * d) / (P - 1)
+                    fcov = np.sum((Fs - Fs.mean(0)) * (d / (P - 1))[:, None, None], axis=0)
+                    if radius > 0:
+                        tap = taper_field(n, int(row[1]), int(row[2]), radius)
+                        fcov = orc.fft2d(orc.fft2d(fcov, inverse=True) * tap)
+                    g = (row[3] + row[4] * eps[o] - h) / (var + row[4] * row[4])
+                    for p, gi in zip(ou, g):
+                        p.F = project_forcing(orc, p.F + fcov * gi, kmin, kmax)
                 if radius > 0:
                     upd = localized_scalar_update(orc, states, h, row[3], row[4], eps[o],
                                                   int(row[1]), int(row[2]), radius)

@@ -139,6 +139,17 @@ def test_filter_enkf_matches_reference_cli(tmp_path):
 
 
 @pytest.mark.gpu
+def test_filter_enkf_with_ou_forcing_matches_reference_cli(tmp_path):
+    """filter --method enkf with sigma > 0: the OU forcing analysed as a latent
+    state, with inflation, localisation and free forecasts."""
+    out, want = run_case("filter_enkf_ou", tmp_path)
+    compare_csv(out / "filter.csv", os.path.join(want, "filter.csv"), {"step", "phase"})
+    compare_csv(out / "reconstruction.csv", os.path.join(want, "reconstruction.csv"),
+                {"step", "phase", "ix", "iy"}, rtol=1e-8)
+    compare_json(out / "summary.json", os.path.join(want, "summary.json"))
+
+
+@pytest.mark.gpu
 def test_particle_sample_matches_reference_cli(tmp_path):
     out, want = run_case("particle_sample", tmp_path)
     compare_csv(out / "chain.csv", os.path.join(want, "chain.csv"),

@@ -9,7 +9,11 @@ Checks:
   * the FP64 oracle filter with OU forcing (tests/oracle_filters.py) on a
     32^2 case that resamples, with Kolmogorov forcing on top;
   * graph replay of OU steps and filter cycles bit-identical to eager launches;
-  * errors: forcing of a white-noise ensemble, EnKF with OU forcing, tau <= 0.
+  * the reference's own ensemble_kalman_filter with sigma > 0 (golden ou_16):
+    the OU forcing analysed as a latent state, plain and with inflation and
+    localisation;
+  * errors: forcing of a white-noise ensemble, batch EnKF with OU forcing,
+    tau <= 0.
 """
 from __future__ import annotations
 
@@ -95,6 +99,29 @@ def test_ou_filter_matches_reference_golden(pkg):
         np.testing.assert_allclose(phys, mean, rtol=0, atol=1e-10)
 
 
+@pytest.mark.parametrize("key,inflation,radius", [("enkf", 1.0, 0.0),
+                                                  ("enkf_loc", 1.05, 3.0)])
+def test_ou_enkf_matches_reference_golden(pkg, key, inflation, radius):
+    """Serial EnKF with the OU forcing as a latent state against the reference's
+    ensemble_kalman_filter with sigma 0.2 (golden ou_16)."""
+    g, meta = golden()
+    n, P = meta["n"], meta["particles"]
+    w0 = full_from_half(g["initial"], n)
+    with pkg.Ensemble(n, P, "fp64", nu=meta["nu"], alpha=meta["alpha"], sigma=meta["sigma"],
+                      tau=meta["tau"], kmin=2, kmax=3, amp=0.0, seed=meta["seed"]) as ens:
+        ens.set_state(np.stack([w0] * P))
+        ens.set_forcing_process("ou")
+        hist = pkg.ensemble_kalman_filter(ens, g["rows"], meta["dt"], meta["seed"],
+                                          inflation=inflation, localization_radius=radius)
+    assert len(hist) == len(g[key + "_records"])
+    for h, rec, mean in zip(hist, g[key + "_records"], g[key + "_means"]):
+        assert h["step"] == rec[0]
+        assert abs(h["forecast_sensor_rmse"] - rec[1]) < 1e-10
+        assert abs(h["posterior_spread"] - rec[2]) < 1e-10
+        phys = (np.fft.ifft2(h["mean_field"]) * n * n).real
+        np.testing.assert_allclose(phys, mean, rtol=0, atol=1e-10)
+
+
 def synthetic(orc, n, nu, dt, groups, every, nobs, truth_key, seed, sd):
     rng = np.random.default_rng(seed)
     w = orc.grf(n, truth_key)
@@ -160,8 +187,9 @@ def test_ou_errors(pkg):
         with pytest.raises(pkg.HzlInvalidArgument):
             ens.set_forcing_process("pink")
         ens.set_forcing_process("ou")
-        with pytest.raises(pkg.HzlInvalidArgument):
-            pkg.ensemble_kalman_filter(ens, np.array([[2, 1, 1, 0.0, 0.1]]), 0.01, 3)
+        with pytest.raises(pkg.HzlInvalidArgument):  # batch needs white forcing (and FP32)
+            pkg.ensemble_kalman_filter(ens, np.array([[2, 1, 1, 0.0, 0.1]]), 0.01, 3,
+                                       mode="batch")
         ens.set_forcing_process("white")
         with pytest.raises(pkg.HzlInvalidArgument):
             ens.get_forcing(0)

@@ -494,3 +494,23 @@ def test_ou_matches_reference_live(orc, ref):
         a, Fa = orc.run_ou(w0, 0.02, 0.1, sigma, tau, kmin, kmax, seed, dt, steps)
         b, Fb = ref.run_ou(w0, 0.02, 0.1, sigma, tau, kmin, kmax, seed, dt, steps)
         assert rel(a, b) < 1e-15 and rel(Fa, Fb) < 1e-15
+
+
+def test_ou_enkf_matches_reference_golden(orc):
+    """The oracle EnKF with the OU forcing as a latent state reproduces the
+    reference's ensemble_kalman_filter over a stochastic dataset (golden ou_16),
+    plain and with inflation 1.05 and localisation radius 3."""
+    import json
+    from oracle_filters import oracle_enkf
+    g = np.load(os.path.join(GOLDEN, "ou_16.npz"))
+    meta = json.load(open(os.path.join(GOLDEN, "ou_16.json")))
+    n, P = meta["n"], meta["particles"]
+    w0 = np.stack([full_from_half(g["initial"], n)] * P)
+    for infl, rad, key in ((1.0, 0.0, "enkf"), (1.05, 3.0, "enkf_loc")):
+        got = oracle_enkf(orc, w0, g["rows"], meta["dt"], meta["seed"], meta["seed"], meta["nu"],
+                          alpha=meta["alpha"], amp=0.0, sigma=meta["sigma"], kmin=2, kmax=3,
+                          inflation=infl, radius=rad, forcing="ou", tau=meta["tau"])
+        rec = np.array([[h["step"], h["forecast_sensor_rmse"], h["posterior_spread"]] for h in got])
+        np.testing.assert_allclose(rec, g[key + "_records"], rtol=0, atol=1e-13)
+        means = np.array([orc.fft2d(h["mean_field"], True).real for h in got])
+        np.testing.assert_allclose(means, g[key + "_means"], rtol=0, atol=1e-13)
