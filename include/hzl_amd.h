@@ -328,7 +328,10 @@ int hzl_ensemble_set_step_index(hzl_ensemble_t h, uint64_t step);
 #define HZL_ENKF_BATCH 1  /* all observations of a group at once (MFMA, FP32) */
 
 /* Start a filter: observation perturbations from the reference NormalStream
- * seeded stream_seed(seed, members) (src/filter.cpp:224). */
+ * seeded stream_seed(seed, members) (src/filter.cpp:224).  With the OU forcing
+ * process the serial analysis carries each member's forcing as a latent state
+ * (inflated and updated with its own covariance, src/filter.cpp:237-321);
+ * HZL_ENKF_BATCH needs the white forcing process (HZL_EINVAL otherwise). */
 int hzl_enkf_begin(hzl_ensemble_t h, uint64_t seed, int mode);
 
 /* Advance every member `steps` RK4 steps (engine forcing), apply the
