@@ -574,6 +574,17 @@ def main() -> int:
     label = f"k_{dominant}<{n},float>" + (f" variant {variant}" if variant else "")
     traffic = measured_traffic(label, n, B)
     diag = ens.diagnostics()
+    # the same steps with the reference's OU shell forcing (per-member
+    # mt19937_64 streams, two k_ou launches per step)
+    ens.set_forcing_process("ou")
+    ens.step(dt, 2)
+    ens.synchronize()
+    e2, e3 = timer.event(), timer.event()
+    timer.record(e2, stream)
+    ens.step(dt, args.steps)
+    timer.record(e3, stream)
+    ou_ms = timer.elapsed_ms(e2, e3) / args.steps
+    ens.synchronize()
     ens.close()
 
     line = {
@@ -607,6 +618,9 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": int(k_bytes),
                      "avg_launch_ms": round(k_ms, 4),
                      "row_ms": round(row_ms, 4), "col_ms": round(col_ms, 4)},
+        "ou_forcing": {"member_steps_per_s": round(B / (ou_ms / 1e3), 1),
+                       "ms_per_step": round(ou_ms, 4),
+                       "overhead_vs_white": round(ou_ms / ms_per_step - 1, 4)},
         "sanity": {"mean_enstrophy": float(diag[:, 1].mean()),
                    "max_cfl_rate": float(diag[:, 4].max())},
     }

@@ -1995,6 +1995,34 @@ template <typename R> class Engine final : public EngineBase {
         check(hipStreamSynchronize(stream_), "stream reseed");
     }
 
+    void set_forcing_state(size_t m, const uint64_t *engine, const double *field) override {
+        if (!ou_)
+            throw std::invalid_argument("the OU forcing process is not selected");
+        member_check(m);
+        if (engine[MT_N] > (uint64_t)MT_N || engine[MT_N] % 2 != 0)
+            throw std::invalid_argument("stream position must be even and at most 312");
+        std::vector<cx<double>> F((size_t)npairs_);
+        for (int j = 0; j < npairs_; ++j) {
+            const long i = ou_first_[j];
+            F[j] = {field[2 * i], field[2 * i + 1]};
+            if (!std::isfinite(F[j].x) || !std::isfinite(F[j].y))
+                throw std::invalid_argument("nonfinite forcing");
+        }
+        upload(ou_F_.p + m * npairs_, F.data(), F.size());
+        check(hipMemcpy2DAsync(ou_mt_.p + m, B_ * sizeof(uint64_t), engine, sizeof(uint64_t),
+                               sizeof(uint64_t), MT_N + 1, hipMemcpyHostToDevice, stream_),
+              "stream upload");
+        check(hipStreamSynchronize(stream_), "stream upload");
+    }
+
+    void get_forcing_state(size_t m, uint64_t *engine, double *field) override {
+        get_forcing(m, field); // checks the process and the member
+        check(hipMemcpy2DAsync(engine, sizeof(uint64_t), ou_mt_.p + m, B_ * sizeof(uint64_t),
+                               sizeof(uint64_t), MT_N + 1, hipMemcpyDeviceToHost, stream_),
+              "stream download");
+        check(hipStreamSynchronize(stream_), "stream download");
+    }
+
     int forcing_process() const override {
         return ou_ ? HZL_FORCING_OU : HZL_FORCING_WHITE;
     }
@@ -2306,6 +2334,24 @@ int hzl_ensemble_step_integrator(hzl_ensemble_t h, double dt, size_t steps, int 
 
 int hzl_ensemble_set_forcing_process(hzl_ensemble_t h, int process) {
     return guarded([&] { engine_of(h).set_forcing_process(process); });
+}
+
+int hzl_ensemble_set_forcing_state(hzl_ensemble_t h, size_t m, const uint64_t *engine,
+                                   const double *field) {
+    return guarded([&] {
+        if (!engine || !field)
+            throw std::invalid_argument("null stream or field");
+        engine_of(h).set_forcing_state(m, engine, field);
+    });
+}
+
+int hzl_ensemble_get_forcing_state(hzl_ensemble_t h, size_t m, uint64_t *engine,
+                                   double *field) {
+    return guarded([&] {
+        if (!engine || !field)
+            throw std::invalid_argument("null stream or field");
+        engine_of(h).get_forcing_state(m, engine, field);
+    });
 }
 
 int hzl_ensemble_reseed_future(hzl_ensemble_t h, size_t m, uint64_t seed) {

@@ -90,6 +90,8 @@ class EngineBase {
     virtual int forcing_process() const = 0;
     virtual void get_forcing(size_t m, double *field) = 0;
     virtual void reseed_future(size_t m, uint64_t seed) = 0;
+    virtual void set_forcing_state(size_t m, const uint64_t *engine, const double *field) = 0;
+    virtual void get_forcing_state(size_t m, uint64_t *engine, double *field) = 0;
     virtual void drift(double *out) = 0;
     virtual void linearized_drift(const double *directions, double *out) = 0;
     virtual void adjoint_drift(const double *cotangents, double *out) = 0;

@@ -278,6 +278,20 @@ void Ensemble::reseed_future(std::size_t member, std::uint64_t seed) {
     check(hzl_ensemble_reseed_future(handle_, member, seed));
 }
 
+void Ensemble::set_forcing_state(std::size_t member, const std::vector<std::uint64_t> &engine,
+                                 const Field &forcing) {
+    if (engine.size() != 313 || forcing.size() != n_ * n_)
+        throw std::invalid_argument("forcing state size mismatch");
+    check(hzl_ensemble_set_forcing_state(handle_, member, engine.data(), raw(forcing)));
+}
+
+std::pair<std::vector<std::uint64_t>, Field> Ensemble::forcing_state(std::size_t member) const {
+    std::vector<std::uint64_t> engine(313);
+    Field forcing(n_ * n_);
+    check(hzl_ensemble_get_forcing_state(handle_, member, engine.data(), raw(forcing)));
+    return {std::move(engine), std::move(forcing)};
+}
+
 void Ensemble::step(double dt, std::size_t steps) {
     check(hzl_ensemble_step(handle_, dt, steps));
 }

@@ -36,6 +36,7 @@
 #include <filesystem>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace hzl {
@@ -136,6 +137,13 @@ class Ensemble {
     [[nodiscard]] Field forcing(std::size_t member) const;
     // Solver::reseed_future (include/hzl/solver.hpp:73-75)
     void reseed_future(std::size_t member, std::uint64_t seed);
+    // A member's OU stream (312 words and the position, Checkpoint::random_engine)
+    // and forcing (OU process only): with a Checkpoint's, the member resumes
+    // the checkpointed Solver's stochastic run draw for draw.
+    void set_forcing_state(std::size_t member, const std::vector<std::uint64_t> &engine,
+                           const Field &forcing);
+    [[nodiscard]] std::pair<std::vector<std::uint64_t>, Field>
+    forcing_state(std::size_t member) const;
     [[nodiscard]] std::size_t size() const noexcept {
         return n_;
     }

@@ -27,7 +27,8 @@ EXPORTS = (
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
     "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
     "hzl_ensemble_step_integrator", "hzl_ensemble_set_forcing_process",
-    "hzl_ensemble_get_forcing", "hzl_ensemble_reseed_future", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
+    "hzl_ensemble_get_forcing", "hzl_ensemble_reseed_future", "hzl_ensemble_set_forcing_state",
+    "hzl_ensemble_get_forcing_state", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
     "hzl_ensemble_adjoint_drift", "hzl_likelihood_adjoint", "hzl_ensemble_step_tangent",
     "hzl_likelihood_gradient",
     "hzl_ensemble_variant_count", "hzl_ensemble_set_tuning", "hzl_ensemble_get_tuning",
@@ -105,6 +106,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_set_forcing_process.argtypes = [vp, C.c_int]
     L.hzl_ensemble_get_forcing.argtypes = [vp, C.c_size_t, _dp]
     L.hzl_ensemble_reseed_future.argtypes = [vp, C.c_size_t, C.c_uint64]
+    u64p = C.POINTER(C.c_uint64)
+    L.hzl_ensemble_set_forcing_state.argtypes = [vp, C.c_size_t, u64p, _dp]
+    L.hzl_ensemble_get_forcing_state.argtypes = [vp, C.c_size_t, u64p, _dp]
     L.hzl_ensemble_drift.argtypes = [vp, _dp]
     L.hzl_ensemble_linearized_drift.argtypes = [vp, _dp, _dp]
     L.hzl_ensemble_adjoint_drift.argtypes = [vp, _dp, _dp]
@@ -267,6 +271,25 @@ class Ensemble:
     def reseed_future(self, member: int, seed: int):
         """Solver::reseed_future: member's future innovations from `seed`."""
         _check(_lib.hzl_ensemble_reseed_future(self._h, member, C.c_uint64(seed)))
+
+    def set_forcing_state(self, member: int, engine, forcing):
+        """Member's OU stream (313 words: state, position) and forcing field."""
+        words = np.ascontiguousarray(engine, dtype=np.uint64)
+        if words.size != 313:
+            raise HzlInvalidArgument("the stream is 312 words and the position")
+        F = np.ascontiguousarray(forcing, dtype=np.complex128).reshape(self.n, self.n)
+        _check(_lib.hzl_ensemble_set_forcing_state(
+            self._h, member, words.ctypes.data_as(C.POINTER(C.c_uint64)),
+            F.ctypes.data_as(_dp)))
+
+    def get_forcing_state(self, member: int):
+        """(stream words[313], forcing (n, n)) of a member (OU process)."""
+        words = np.zeros(313, dtype=np.uint64)
+        F = np.zeros((self.n, self.n), dtype=np.complex128)
+        _check(_lib.hzl_ensemble_get_forcing_state(
+            self._h, member, words.ctypes.data_as(C.POINTER(C.c_uint64)),
+            F.ctypes.data_as(_dp)))
+        return words, F
 
     def set_member_seeds(self, seeds):
         """Stream seed of every member (as Solver(n, p, seeds[m]))."""

@@ -148,6 +148,20 @@ int hzl_ensemble_get_forcing(hzl_ensemble_t h, size_t m, double *field);
  * Solver(n, p, seed) whose forcing process is selected afterwards. */
 int hzl_ensemble_reseed_future(hzl_ensemble_t h, size_t m, uint64_t seed);
 
+/* Member m's OU forcing state: its mt19937_64 stream as 313 words (the 312
+ * state words, then the position, the text order of std::mt19937_64 and of
+ * HZL_CHECKPOINT_V1 files) and its forcing as a reference Field (n*n
+ * interleaved doubles; the shell pairs are read, each pair's first mode).  A
+ * stream taken from a checkpoint must not hold a cached second normal (the
+ * reference's OU draws never leave one).  With set, a member resumes a
+ * reference Solver's stochastic run draw for draw.  Replaces: the random stream
+ * and forcing of Solver::load / Solver::save (src/solver.cpp:434-491).
+ * HZL_EINVAL unless the OU process is selected. */
+int hzl_ensemble_set_forcing_state(hzl_ensemble_t h, size_t m, const uint64_t *engine,
+                                   const double *field);
+int hzl_ensemble_get_forcing_state(hzl_ensemble_t h, size_t m, uint64_t *engine,
+                                   double *field);
+
 /* Calibration drifts of every member's current state, batched: out,
  * directions and cotangents hold one reference Field per member (members
  * n*n interleaved doubles; input modes outside the dealiasing mask are

@@ -553,6 +553,19 @@ class Reference:
                      likelihood_seed=int(seeds[i]), accepted=bool(out[i, 7]))
                 for i in range(count)]
 
+    def checkpoint_ou(self, n, nu, alpha, sigma, tau, kmin, kmax, seed, dt, before, after,
+                      path_before, path_after):
+        """Reference Solver with OU forcing from multimode: checkpoints after
+        `before` and after `before + after` steps."""
+        L = self.lib
+        L.ref_checkpoint_ou.restype = C.c_int
+        L.ref_checkpoint_ou.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_double,
+                                        C.c_size_t, C.c_size_t, C.c_char_p, C.c_char_p]
+        self._check(L.ref_checkpoint_ou(n, nu, alpha, sigma, tau, kmin, kmax, seed, dt, before,
+                                        after, str(path_before).encode(),
+                                        str(path_after).encode()))
+
     def negative_log_likelihood(self, h, log_nu, log_alpha):
         out = np.zeros(1)
         self._check(self.lib.ref_negative_log_likelihood(h, log_nu, log_alpha, _ptr(out)))

@@ -140,6 +140,30 @@ int ref_run_ou(std::size_t n, double nu, double alpha, double sigma, double tau,
     });
 }
 
+// Solver(n, p, seed) with OU forcing from multimode: `before` steps, save a
+// checkpoint, `after` more steps, save another (Solver::save, src/solver.cpp:434-454).
+int ref_checkpoint_ou(std::size_t n, double nu, double alpha, double sigma, double tau,
+                      int kmin, int kmax, std::uint64_t seed, double dt, std::size_t before,
+                      std::size_t after, const char *path_before, const char *path_after) {
+    return guarded([&] {
+        hzl::Parameters p;
+        p.viscosity = nu;
+        p.friction = alpha;
+        p.forcing_sigma = sigma;
+        p.forcing_tau = tau;
+        p.forcing_min = kmin;
+        p.forcing_max = kmax;
+        hzl::Solver s(n, p, seed);
+        s.initialize("multimode");
+        for (std::size_t k = 0; k < before; ++k)
+            s.step(dt);
+        s.save(path_before);
+        for (std::size_t k = 0; k < after; ++k)
+            s.step(dt);
+        s.save(path_after);
+    });
+}
+
 // `steps` reference SSPRK(3,3) steps (Solver::step(dt, TimeIntegrator::ssprk3))
 // from a spectral state with one forcing field held on the shell pairs.
 int ref_run_ssprk3(std::size_t n, double nu, double alpha, double dt, std::size_t steps,

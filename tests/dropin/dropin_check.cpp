@@ -19,6 +19,9 @@
 //   adjoint <csv> <stride> <out>    likelihood_adjoint(data, q, stride) at three q:
 //                                   value, gradient, checkpoint fields,
 //                                   peak block fields, replayed steps each
+//   resume_ou <in> <steps> <out>    Ensemble member from a checkpoint with OU forcing:
+//                                   state, stream and forcing loaded, `steps` steps of
+//                                   0.01, saved as a checkpoint
 //   calibration <n> <out>           Ensemble (FP64, 2 members from multimode and
 //                                   taylor-green): drift, linearized_drift along
 //                                   each other's state, adjoint_drift, then
@@ -192,6 +195,25 @@ int main(int argc, char **argv) {
             append(out, c.forcing);
             append(out, s.coefficients());
             write(argv[4], out);
+        } else if (mode == "resume_ou" && argc == 5) {
+            Checkpoint c = Checkpoint::load(argv[2]);
+            if (c.random_has_spare)
+                throw std::runtime_error("stream holds a cached normal");
+            const std::size_t steps = std::stoul(argv[3]);
+            const double dt = 0.01;
+            Ensemble e(c.n, 1, c.parameters, Precision::fp64, 0);
+            e.set_state(0, c.vorticity);
+            e.set_forcing_process(ForcingProcess::ornstein_uhlenbeck);
+            e.set_forcing_state(0, c.random_engine, c.forcing);
+            e.step(dt, steps);
+            e.synchronize();
+            c.vorticity = e.state(0);
+            auto [engine, forcing] = e.forcing_state(0);
+            c.random_engine = engine;
+            c.forcing = forcing;
+            for (std::size_t s = 0; s < steps; ++s)
+                c.time += dt;
+            c.save(argv[4]);
         } else if (mode == "calibration" && argc == 4) {
             const std::size_t n = std::stoul(argv[2]);
             Parameters p;

@@ -143,6 +143,10 @@ def ou16(ref: Reference) -> None:
     for k, (w, F, _) in runs.items():
         arrays[k + "_state"] = half(w)
         arrays[k + "_forcing"] = half(F)
+    # a stochastic run checkpointed mid-way (Solver::save), and 15 steps later
+    ref.checkpoint_ou(n, nu, alpha, sigma, tau, 2, 3, 31337, dt, 10, 15,
+                      os.path.join(HERE, "checkpoint_ou_before_16.txt"),
+                      os.path.join(HERE, "checkpoint_ou_after_16.txt"))
     save("ou_16",
          dict(recipe="Solver(16, p, seed) with OU forcing from initialize('multimode'), "
                      "`steps` x Solver::step(dt) (ref_run_ou); synthetic_observations(16, "
@@ -151,7 +155,8 @@ def ou16(ref: Reference) -> None:
                      "8, 555) plain and with inflation 1.05, localisation radius 3; "
                      "particle_metropolis(data, "
                      "log(nu, alpha, sigma, tau), 0.1, 6, 8, 123); ensemble(16, p, 0.01, 20, "
-                     "4, 2, 42) energies",
+                     "4, 2, 42) energies; Solver(16, p, 31337) from multimode checkpointed "
+                     "after 10 steps of 0.01 and again after 15 more (checkpoint_ou_*_16.txt)",
               n=n, nu=nu, alpha=alpha, sigma=sigma, tau=tau, dt=dt, particles=8, seed=555,
               runs=[v[2] for v in runs.values()], log_likelihood=pf["log_likelihood"],
               resampling_events=pf["resampling_events"]),

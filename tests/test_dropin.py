@@ -303,3 +303,35 @@ def test_reference_signature_likelihood_adjoint_matches_golden(tmp_path):
             assert abs(got[0] - w["value"]) < 1e-11 * abs(w["value"])
             np.testing.assert_allclose(got[1:3], w["gradient"], rtol=1e-9, atol=0)
             assert got[3] == w["checkpoint_fields"] and got[5] == w["replayed_steps"]
+
+
+@pytest.mark.gpu
+def test_stochastic_resume_from_reference_checkpoint(tmp_path, pkg):
+    """A GPU member loads a reference checkpoint of a stochastic (OU) run --
+    vorticity, mt19937_64 stream and forcing -- and continues it: after 15
+    steps its state, forcing and stream equal the reference Solver's own
+    checkpoint 15 steps later (C++ drop-in and Python binding)."""
+    before = os.path.join(GOLDEN, "checkpoint_ou_before_16.txt")
+    after = pkg.load_checkpoint(os.path.join(GOLDEN, "checkpoint_ou_after_16.txt"))
+    run("resume_ou", before, 15, tmp_path / "resumed.txt")
+    got = pkg.load_checkpoint(str(tmp_path / "resumed.txt"))
+    assert got["random_engine"] == after["random_engine"]
+    assert got["time"] == after["time"]
+    for key, tol in (("vorticity", 1e-12), ("forcing", 1e-13)):
+        err = np.linalg.norm(got[key] - after[key]) / np.linalg.norm(after[key])
+        assert err < tol, (key, err)
+    # the same through the Python binding
+    c = pkg.load_checkpoint(before)
+    p = c["parameters"]
+    with pkg.Ensemble(16, 1, "fp64", nu=p["viscosity"], alpha=p["friction"],
+                      sigma=p["forcing_sigma"], tau=p["forcing_tau"], kmin=p["forcing_min"],
+                      kmax=p["forcing_max"], amp=0.0) as ens:
+        ens.set_state(c["vorticity"][None])
+        ens.set_forcing_process("ou")
+        ens.set_forcing_state(0, c["random_engine"], c["forcing"])
+        ens.step(0.01, 15)
+        words, F = ens.get_forcing_state(0)
+        w = ens.get_state(0)
+    assert [int(x) for x in words] == after["random_engine"]
+    assert np.linalg.norm(w - after["vorticity"]) < 1e-12 * np.linalg.norm(after["vorticity"])
+    assert np.linalg.norm(F - after["forcing"]) < 1e-13 * np.linalg.norm(after["forcing"])
