@@ -129,6 +129,11 @@ class MultiFilter {
 
     // Particle i of the union starts on noise key stream_seed(seed, i).
     void begin(uint64_t seed, size_t max_groups) {
+        // children copy states across grids with their own gather; the OU
+        // forcing state (coefficients, streams) does not follow them
+        for (EngineBase *e : engs_)
+            if (e->forcing_process() == HZL_FORCING_OU)
+                throw std::invalid_argument("a joint filter runs with the white forcing process");
         seed_ = seed;
         group_ = 0;
         const size_t P = total_;

@@ -1353,10 +1353,20 @@ int hzl_surrogate_spread(hzl_surrogate_t h, hzl_ensemble_t ensemble, double *rms
     });
 }
 
+namespace {
+// The learned forecast adds the white shell noise's variance; an OU ensemble's
+// forcing state would not advance through it.
+void require_white(hzl_amd::EngineBase &e) {
+    if (e.forcing_process() == HZL_FORCING_OU)
+        throw std::invalid_argument("the surrogate forecast runs with the white forcing process");
+}
+} // namespace
+
 int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt,
                                     size_t steps, int precision, double threshold,
                                     int *used_solver, double *spread) {
     return guarded([&] {
+        require_white(hzl_amd::engine_of(ensemble));
         const int r = net_of(h).forecast_or_solve(hzl_amd::engine_of(ensemble), dt, steps,
                                                   precision, threshold, spread);
         if (used_solver)
@@ -1371,6 +1381,7 @@ int hzl_surrogate_set_graph(hzl_surrogate_t h, int enabled) {
 int hzl_surrogate_forecast(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt, size_t steps,
                            int precision) {
     return guarded([&] {
+        require_white(hzl_amd::engine_of(ensemble));
         net_of(h).forecast(hzl_amd::engine_of(ensemble), dt, steps, precision);
     });
 }

@@ -318,7 +318,8 @@ int hzl_pf_begin_filters(hzl_ensemble_t h, const uint64_t *seeds, size_t filters
  * indices refer to the coarsest grid.  Particle i of the union starts on
  * noise key stream_seed(seed, i); everything else as hzl_pf_begin /
  * hzl_pf_assimilate (record[4]) / hzl_pf_result.  The ensembles must not be
- * used for other work while the filter runs. */
+ * used for other work while the filter runs, and run the white forcing
+ * process (HZL_EINVAL at begin for an OU ensemble). */
 int hzl_pf_joint_create(hzl_pf_joint_t *out, const hzl_ensemble_t *ensembles, size_t count);
 int hzl_pf_joint_destroy(hzl_pf_joint_t h);
 int hzl_pf_joint_begin(hzl_pf_joint_t h, uint64_t seed, size_t max_groups);
@@ -495,7 +496,9 @@ int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, 
 
 /* Advance every member of an FP32 ensemble by steps * dt with the network
  * (steps and dt must equal the trained horizon).  The ensemble's step counter
- * advances by `steps`; filter calls may follow with steps = 0. */
+ * advances by `steps`; filter calls may follow with steps = 0.  The ensemble
+ * runs the white forcing process (HZL_EINVAL for OU; also for
+ * hzl_surrogate_forecast_or_solve). */
 int hzl_surrogate_forecast(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt, size_t steps,
                            int precision);
 

@@ -193,5 +193,9 @@ def test_ou_errors(pkg):
         ens.set_forcing_process("white")
         with pytest.raises(pkg.HzlInvalidArgument):
             ens.get_forcing(0)
+    with pkg.Ensemble(16, 4, "fp64", sigma=0.1) as a, pkg.Ensemble(32, 2, "fp64", sigma=0.1) as b:
+        a.set_forcing_process("ou")
+        with pytest.raises(pkg.HzlInvalidArgument):  # joint filters run white forcing
+            pkg.joint_bootstrap_filter([a, b], np.array([[2, 1, 1, 0.0, 0.1]]), 0.01, 3)
     with pytest.raises(pkg.HzlInvalidArgument):  # tau must be positive at creation
         pkg.Ensemble(16, 2, "fp64", sigma=0.1, tau=0.0)
