@@ -1927,13 +1927,13 @@ template <typename R> class Engine final : public EngineBase {
     void set_forcing_process(int process) override {
         if (process != HZL_FORCING_WHITE && process != HZL_FORCING_OU)
             throw std::invalid_argument("forcing process must be HZL_FORCING_WHITE or HZL_FORCING_OU");
+        if (process == HZL_FORCING_OU && (!(p_.forcing_tau > 0) || !std::isfinite(p_.forcing_tau)))
+            throw std::invalid_argument("OU forcing needs a finite forcing_tau > 0");
         check(hipStreamSynchronize(stream_), "forcing process switch");
         clear_graphs(); // captured steps hold the previous process's kernels
         ou_ = process == HZL_FORCING_OU;
         if (!ou_)
             return;
-        if (!(p_.forcing_tau > 0) || !std::isfinite(p_.forcing_tau))
-            throw std::invalid_argument("OU forcing needs a finite forcing_tau > 0");
         if (!ou_mt_.count) {
             // forcing_pairs (src/solver.cpp:67-83): first mode i = y n + x of
             // each conjugate pair, i < partner, inside the mask and the shell

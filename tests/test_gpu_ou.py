@@ -23,7 +23,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle_filters import oracle_bootstrap_filter
+from oracle_filters import oracle_bootstrap_filter, oracle_enkf
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -120,6 +120,27 @@ def test_ou_enkf_matches_reference_golden(pkg, key, inflation, radius):
         assert abs(h["posterior_spread"] - rec[2]) < 1e-10
         phys = (np.fft.ifft2(h["mean_field"]) * n * n).real
         np.testing.assert_allclose(phys, mean, rtol=0, atol=1e-10)
+
+
+def test_ou_enkf_fp32_matches_oracle(pkg, orc):
+    """FP32 states with the FP64 OU forcing state: serial EnKF with inflation and
+    localisation against the FP64 oracle (OU latent forcing)."""
+    n, P, nu, dt, seed, tau = 32, 16, 2e-3, 2e-3, 21, 0.3
+    rows = synthetic(orc, n, nu, dt, groups=3, every=10, nobs=12, truth_key=41, seed=5, sd=0.3)
+    base = orc.grf(n, 41)
+    init = np.stack([base + 0.05 * orc.grf(n, orc.stream_seed(6, i)) for i in range(P)])
+    want = oracle_enkf(orc, init, rows, dt, seed, seed, nu, sigma=1.0, forcing="ou", tau=tau,
+                       inflation=1.05, radius=4.0)
+    with pkg.Ensemble(n, P, "fp32", nu=nu, sigma=1.0, tau=tau, amp=0.1, kf=4, seed=seed) as ens:
+        ens.set_state(init)
+        ens.set_forcing_process("ou")
+        got = pkg.ensemble_kalman_filter(ens, rows, dt, seed, inflation=1.05,
+                                         localization_radius=4.0)
+    for a, b in zip(got, want):
+        assert abs(a["forecast_sensor_rmse"] - b["forecast_sensor_rmse"]) < 1e-5 * max(
+            1.0, b["forecast_sensor_rmse"])
+        assert abs(a["posterior_spread"] - b["posterior_spread"]) < 1e-5 * b["posterior_spread"]
+        assert rel(a["mean_field"], b["mean_field"]) < 1e-5
 
 
 def synthetic(orc, n, nu, dt, groups, every, nobs, truth_key, seed, sd):
