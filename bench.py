@@ -481,7 +481,7 @@ def uneven_pf(pkg, cycles: int = 5) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--members", type=int, default=1024)
     ap.add_argument("--n", type=int, default=256)
