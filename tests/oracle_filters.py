@@ -220,7 +220,7 @@ def oracle_enkf(orc, initial, obs, dt, ens_seed, filter_seed, nu, alpha=0.0, amp
 
 
 def oracle_pmcmc(orc, initial, obs, dt, q0, samples, scale, seed, lower, upper, particles,
-                 alpha=0.0, sigma=0.05, kf=4, kmin=1, kmax=4):
+                 alpha=0.0, sigma=0.05, kf=4, kmin=1, kmax=4, forcing="white", tau=0.5):
     """Reference particle_metropolis (src/filter.cpp:166-203) per chain with the
     engine's forcing model and a log-uniform prior on (nu, amplitude): chain c
     seeds NormalStream(stream_seed(seed, c)) for proposals and
@@ -239,7 +239,7 @@ def oracle_pmcmc(orc, initial, obs, dt, q0, samples, scale, seed, lower, upper, 
             return -np.inf
         return oracle_bootstrap_filter(orc, init, obs, dt, lseed, np.exp(q[0]), alpha=alpha,
                                        amp=np.exp(q[1]), kf=kf, sigma=sigma, kmin=kmin,
-                                       kmax=kmax)["log_likelihood"]
+                                       kmax=kmax, forcing=forcing, tau=tau)["log_likelihood"]
 
     out = []
     for c, qc in enumerate(np.asarray(q0, dtype=np.float64).reshape(-1, 2)):

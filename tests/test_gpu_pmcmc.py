@@ -39,6 +39,31 @@ def test_batched_pmcmc_matches_oracle_chains(pkg, orc):
     assert 0 < accepted < C * S
 
 
+def test_batched_pmcmc_with_ou_forcing_matches_oracle_chains(pkg, orc):
+    """The same chains with the reference's OU forcing: every likelihood
+    evaluation starts new particles (seeds stream_seed(lseed, i), stationary OU
+    start), as the reference's particle_log_likelihood builds new Solvers."""
+    n, P, C, S, dt, seed = 32, 8, 2, 3, 2e-3, 77
+    rows, _ = synthetic(orc, n, 1e-3, dt, groups=3, every=10, nobs=16, truth_key=556, seed=5,
+                        sd=0.5)
+    init = orc.grf(n, 556)
+    q0 = np.array([[np.log(1e-3), np.log(0.1)], [np.log(2e-3), np.log(0.05)]])
+    lower, upper = (np.log(2e-4), np.log(0.02)), (np.log(5e-3), np.log(0.5))
+    want = oracle_pmcmc(orc, init, rows, dt, q0, S, 0.4, seed, lower, upper, P, sigma=0.5,
+                        forcing="ou", tau=0.3)
+    with pkg.Ensemble(n, P * C, "fp64", nu=1e-3, sigma=0.5, tau=0.3, amp=0.1, kf=4,
+                      seed=1) as ens:
+        ens.set_forcing_process("ou")
+        got = pkg.particle_metropolis(ens, rows, dt, init, q0, S, 0.4, seed, friction=0.0,
+                                      forcing_sigma=0.5)
+    for c in range(C):
+        for s in range(S):
+            q, ll, dens, acc = want[c][s]
+            assert got["accepted"][c, s] == acc, (c, s)
+            np.testing.assert_array_equal(got["q"][c, s], q)
+            assert abs(got["log_likelihood"][c, s] - ll) <= 1e-9 * abs(ll)
+
+
 def test_pmcmc_rejects_bad_configuration(pkg, orc):
     n = 32
     init = orc.grf(n, 1)
