@@ -11,6 +11,8 @@ Test infrastructure (never loaded by the product path):
     oracle/liboracle.so          FP64 C oracle
     oracle/_ref/libhzlref.so     unmodified reference CPU library (only where
                                  the reference sources are present)
+    oracle/_ref/hzl_ref_cli      the reference's own CLI (same condition; writes the
+                                 golden CLI outputs of tests/golden/cli)
     tests/emulator/libhzl_emu.so CPU lane emulator of the kernel bodies
 """
 from __future__ import annotations
@@ -94,7 +96,7 @@ def build_dropin() -> str:
 def build_oracle() -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
     if os.path.isdir(os.path.join(REFERENCE, "src")):
-        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref",
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "ref-cli",
               f"REFERENCE={REFERENCE}"])
 
 
