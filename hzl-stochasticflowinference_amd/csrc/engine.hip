@@ -1983,16 +1983,13 @@ template <typename R> class Engine final : public EngineBase {
         if (!ou_)
             return;
         // mt19937_64 seeding ([rand.eng.mers]), position at the end of the
-        // block so the first draw regenerates it; words strided by members
+        // block so the first draw regenerates it
         std::vector<uint64_t> s(MT_N + 1);
         s[0] = seed;
         for (int i = 1; i < MT_N; ++i)
             s[i] = 6364136223846793005ULL * (s[i - 1] ^ (s[i - 1] >> 62)) + (uint64_t)i;
         s[MT_N] = MT_N;
-        check(hipMemcpy2DAsync(ou_mt_.p + m, B_ * sizeof(uint64_t), s.data(), sizeof(uint64_t),
-                               sizeof(uint64_t), MT_N + 1, hipMemcpyHostToDevice, stream_),
-              "stream reseed");
-        check(hipStreamSynchronize(stream_), "stream reseed");
+        upload(ou_mt_.p + m * (MT_N + 1), s.data(), s.size());
     }
 
     void set_forcing_state(size_t m, const uint64_t *engine, const double *field) override {
@@ -2009,16 +2006,13 @@ template <typename R> class Engine final : public EngineBase {
                 throw std::invalid_argument("nonfinite forcing");
         }
         upload(ou_F_.p + m * npairs_, F.data(), F.size());
-        check(hipMemcpy2DAsync(ou_mt_.p + m, B_ * sizeof(uint64_t), engine, sizeof(uint64_t),
-                               sizeof(uint64_t), MT_N + 1, hipMemcpyHostToDevice, stream_),
-              "stream upload");
-        check(hipStreamSynchronize(stream_), "stream upload");
+        upload(ou_mt_.p + m * (MT_N + 1), engine, MT_N + 1);
     }
 
     void get_forcing_state(size_t m, uint64_t *engine, double *field) override {
         get_forcing(m, field); // checks the process and the member
-        check(hipMemcpy2DAsync(engine, sizeof(uint64_t), ou_mt_.p + m, B_ * sizeof(uint64_t),
-                               sizeof(uint64_t), MT_N + 1, hipMemcpyDeviceToHost, stream_),
+        check(hipMemcpyAsync(engine, ou_mt_.p + m * (MT_N + 1), (MT_N + 1) * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, stream_),
               "stream download");
         check(hipStreamSynchronize(stream_), "stream download");
     }
@@ -2163,7 +2157,7 @@ template <typename R> class Engine final : public EngineBase {
     DeviceBuffer<uint64_t> d_keys_;
     std::vector<double> nu_, alpha_, sigma_, amp_;
     std::vector<uint64_t> keys_;
-    // OU forcing process (set_forcing_process): stream words [MT_N + 1][B],
+    // OU forcing process (set_forcing_process): stream words [B][MT_N + 1],
     // coefficients [B][P] in the reference's pair order, their resampling
     // copy, the engine pair index of each (~index when conjugated) and the
     // grid index of each pair's first mode.

@@ -223,15 +223,17 @@ template <typename R> HZL_HD void noise_body(const NoiseArgs<R> &a, long i) {
 // Every use draws two normals per pair, so a stream never holds a cached
 // second normal between calls and its position stays even: pair j of a call
 // takes mt outputs pos + 2j and pos + 2j + 1, and the lanes of the member's
-// workgroup draw in parallel.  The 312 state words are staged in LDS; the
-// regeneration (twist) runs in two parallel phases, words 0..155 from the old
-// words and 156..311 from the new words 0..155.
+// workgroup draw in parallel.  Tempering only reads the state words, so a call
+// that stays inside the current block reads the 2 P words it draws and writes
+// back the position alone.  A call that crosses the block end stages the 312
+// words in LDS and regenerates them (twist) in two parallel phases, words
+// 0..155 from the old words and 156..311 from the new words 0..155.
 constexpr int OU_LANES = 64;
 constexpr int MT_N = 312;
 
 template <typename R> struct OuArgs {
     int B, npairs;
-    uint64_t *mt;          // [MT_N + 1][B]: state words, then the position
+    uint64_t *mt;          // [B][MT_N + 1]: state words, then the position
     cx<double> *F;         // [B][npairs] coefficients, reference pair order
     const int *map;        // [npairs] engine pair index, ~index when conjugated
     const double *sigma;   // per member
@@ -264,7 +266,7 @@ HZL_HD double mt_open_uniform(uint64_t x) {
 template <typename R, class Ctx>
 HZL_HD void ou_body(const Ctx &ctx, const OuArgs<R> &a, uint64_t *s, int *shared_pos, int m) {
     const int lane = ctx.tid();
-    const long B = a.B;
+    uint64_t *words = a.mt + (long)m * (MT_N + 1);
     if (a.stats && a.stats[8 * (m / a.P) + 3] == 0)
         return; // filter did not resample (uniform across the workgroup)
     if (a.mode == 3) {
@@ -284,10 +286,34 @@ HZL_HD void ou_body(const Ctx &ctx, const OuArgs<R> &a, uint64_t *s, int *shared
             *shared_pos = MT_N;
         }
     } else {
+        const int p0 = (int)words[MT_N];
+        if (p0 + 2 * a.npairs <= MT_N) {
+            // inside the current block: draw straight from the state words
+            const double sg = a.sigma[m], decay = exp(-a.h / a.tau),
+                         scale = sg * sqrt(-a.tau * expm1(-2 * a.h / a.tau) / 4);
+            for (int j = lane; j < a.npairs; j += OU_LANES) {
+                const double u1 = mt_open_uniform(mt_temper(words[p0 + 2 * j]));
+                const double u2 = mt_open_uniform(mt_temper(words[p0 + 2 * j + 1]));
+                const double r = sqrt(-2 * log(u1)), angle = 2 * 3.14159265358979323846 * u2;
+                cx<double> &f = a.F[(long)m * a.npairs + j];
+                const cx<double> v = {decay * f.x + scale * (r * cos(angle)),
+                                      decay * f.y + scale * (r * sin(angle))};
+                f = v;
+                if (a.noise) {
+                    const int p = a.map[j] >= 0 ? a.map[j] : ~a.map[j];
+                    a.noise[(long)m * a.npairs + p] =
+                        a.map[j] >= 0 ? cx<R>{(R)v.x, (R)v.y} : cx<R>{(R)v.x, (R)(-v.y)};
+                }
+            }
+            ctx.sync(); // every lane has read the position before it moves
+            if (lane == 0)
+                words[MT_N] = (uint64_t)(p0 + 2 * a.npairs);
+            return;
+        }
         for (int i = lane; i < MT_N; i += OU_LANES)
-            s[i] = a.mt[(long)i * B + m];
+            s[i] = words[i];
         if (lane == 0)
-            *shared_pos = (int)a.mt[(long)MT_N * B + m];
+            *shared_pos = p0;
     }
     ctx.sync();
     int pos = *shared_pos;
@@ -348,9 +374,9 @@ HZL_HD void ou_body(const Ctx &ctx, const OuArgs<R> &a, uint64_t *s, int *shared
         }
     }
     for (int i = lane; i < MT_N; i += OU_LANES)
-        a.mt[(long)i * B + m] = s[i];
+        words[i] = s[i];
     if (lane == 0)
-        a.mt[(long)MT_N * B + m] = (uint64_t)pos;
+        words[MT_N] = (uint64_t)pos;
 }
 
 template <typename R> struct ForcingModel {
