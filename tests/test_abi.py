@@ -49,3 +49,18 @@ def _fresh_load(pkg, path):
         return pkg.engine.load_library(path)
     finally:
         pkg.engine._lib = saved
+
+
+def test_header_compiles_as_c99_and_cxx():
+    """The C-ABI header is plain C (cgo / ctypes / JNI callers include it from C)."""
+    import shutil
+    import subprocess
+    header = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "include", "hzl_amd.h")
+    for cmd in (["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                 "-x", "c", header],
+                ["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++", header]):
+        if shutil.which(cmd[0]) is None:
+            pytest.skip(f"{cmd[0]} not installed")
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        assert p.returncode == 0, p.stderr
