@@ -51,6 +51,14 @@ template <typename R> __global__ void __launch_bounds__(256) k_noise(NoiseArgs<R
 }
 
 // One 64-lane workgroup per member (grid = members).
+// One workgroup per member: its n spectrum bins (spectrum_body).
+template <typename R>
+__global__ void __launch_bounds__(256) k_spectrum(int K, int n, long spec, const cx<R> *w,
+                                                  double *out) {
+    spectrum_body<R>(DevCtx{}, K, n, w + (long)blockIdx.x * spec, out + (long)blockIdx.x * n,
+                     256);
+}
+
 template <typename R> __global__ void __launch_bounds__(OU_LANES) k_ou(OuArgs<R> a) {
     __shared__ uint64_t s[MT_N];
     __shared__ int pos;
@@ -510,6 +518,7 @@ template <typename R> class Engine final : public EngineBase {
         nu_.assign(B_, p.viscosity);
         alpha_.assign(B_, p.friction);
         sigma_.assign(B_, p.forcing_sigma);
+        tau_.assign(B_, p.forcing_tau);
         amp_.assign(B_, p.kolmogorov_amplitude);
         keys_.resize(B_);
         for (size_t m = 0; m < B_; ++m)
@@ -517,6 +526,7 @@ template <typename R> class Engine final : public EngineBase {
         d_nu_ = DeviceBuffer<double>(B_);
         d_alpha_ = DeviceBuffer<double>(B_);
         d_sigma_ = DeviceBuffer<double>(B_);
+        d_tau_ = DeviceBuffer<double>(B_);
         d_amp_ = DeviceBuffer<double>(B_);
         d_keys_ = DeviceBuffer<uint64_t>(B_);
         upload(d_keys_.p, keys_.data(), B_);
@@ -583,6 +593,14 @@ template <typename R> class Engine final : public EngineBase {
         alpha_[m] = alpha;
         sigma_[m] = sigma;
         amp_[m] = amp;
+        params_dirty_ = true;
+    }
+
+    void set_member_tau(size_t m, double tau) override {
+        member_check(m);
+        if (!std::isfinite(tau) || tau <= 0)
+            throw std::invalid_argument("tau must be finite and positive");
+        tau_[m] = tau;
         params_dirty_ = true;
     }
 
@@ -1259,6 +1277,17 @@ template <typename R> class Engine final : public EngineBase {
         check(hipStreamSynchronize(stream_), "diagnostics");
     }
 
+    void energy_spectrum(double *out) override {
+        DeviceBuffer<double> bins(B_ * (size_t)n_);
+        hipLaunchKernelGGL(k_spectrum<R>, dim3((unsigned)B_), dim3(256), 0, stream_, K_, n_,
+                           (long)spec_, (const cx<R> *)w0_.p, bins.p);
+        check(hipGetLastError(), "spectrum kernel");
+        check(hipMemcpyAsync(out, bins.p, bins.count * sizeof(double), hipMemcpyDeviceToHost,
+                             stream_),
+              "spectrum download");
+        check(hipStreamSynchronize(stream_), "spectrum");
+    }
+
     void profile_step(double dt, float *ms) override {
         validate_dt(dt);
         hipEvent_t ev[9];
@@ -1355,6 +1384,9 @@ template <typename R> class Engine final : public EngineBase {
                 !std::isfinite(sd[o]) || sd[o] <= 0)
                 throw std::invalid_argument("observations must be finite, in bounds, and have "
                                             "positive uncertainty");
+        if (!is_last && pf_group_ >= (uint64_t)pf_max_uniforms_)
+            throw std::invalid_argument(
+                "more observation groups than the filter was started for (max_groups)");
         if (steps)
             validate_dt(dt);
         stage_observations(nobs, x, y, value, sd);
@@ -1542,6 +1574,7 @@ template <typename R> class Engine final : public EngineBase {
         upload(d_nu_.p, nu_.data(), B_);
         upload(d_alpha_.p, alpha_.data(), B_);
         upload(d_sigma_.p, sigma_.data(), B_);
+        upload(d_tau_.p, tau_.data(), B_);
         upload(d_amp_.p, amp_.data(), B_);
         params_dirty_ = false;
     }
@@ -1918,8 +1951,8 @@ template <typename R> class Engine final : public EngineBase {
 
     // One call of the OU kernel over every member (modes of OuArgs).
     void launch_ou(int mode, double h, cx<R> *noise, const double *stats = nullptr, int P = 1) {
-        OuArgs<R> a{(int)B_,   npairs_,     ou_mt_.p, ou_F_.p, ou_map_.p, d_sigma_.p,
-                    d_keys_.p, p_.forcing_tau, h,     mode,    noise,     stats,
+        OuArgs<R> a{(int)B_,   npairs_,  ou_mt_.p, ou_F_.p, ou_map_.p, d_sigma_.p,
+                    d_keys_.p, d_tau_.p, h,        mode,    noise,     stats,
                     P};
         hipLaunchKernelGGL(k_ou<R>, dim3((unsigned)B_), dim3(OU_LANES), 0, stream_, a);
     }
@@ -2153,9 +2186,9 @@ template <typename R> class Engine final : public EngineBase {
     int npairs_ = 0;
     DeviceBuffer<double> cfl_, cfl_diag_;
     DeviceBuffer<int> pairs_;
-    DeviceBuffer<double> d_nu_, d_alpha_, d_sigma_, d_amp_;
+    DeviceBuffer<double> d_nu_, d_alpha_, d_sigma_, d_tau_, d_amp_;
     DeviceBuffer<uint64_t> d_keys_;
-    std::vector<double> nu_, alpha_, sigma_, amp_;
+    std::vector<double> nu_, alpha_, sigma_, tau_, amp_;
     std::vector<uint64_t> keys_;
     // OU forcing process (set_forcing_process): stream words [B][MT_N + 1],
     // coefficients [B][P] in the reference's pair order, their resampling
@@ -2186,21 +2219,36 @@ template <typename R> class Engine final : public EngineBase {
     uint64_t graph_clock_ = 0;
 };
 
-// Parameters::validate (src/solver.cpp:31-40) plus the engine's own limits.
+// Model-parameter rules of Parameters::validate (src/solver.cpp:31-40): each
+// row is a condition that must hold and the reference's message when it does
+// not, checked in order.
+void validate_model(size_t n, const hzl_parameters &p) {
+    const auto nonnegative = [](double v) { return std::isfinite(v) && v >= 0; };
+    const long long shell_top = 3LL * p.forcing_max;
+    const struct {
+        bool holds;
+        const char *message;
+    } rules[] = {
+        {nonnegative(p.viscosity) && nonnegative(p.friction) && nonnegative(p.forcing_sigma) &&
+             std::isfinite(p.forcing_tau) && p.forcing_tau > 0,
+         "nu, alpha, sigma must be finite and nonnegative; tau must be positive"},
+        {p.forcing_min >= 1 && p.forcing_max >= p.forcing_min && shell_top < (long long)n,
+         "forcing shell must lie strictly inside the dealiased grid"},
+    };
+    for (const auto &r : rules)
+        if (!r.holds)
+            throw std::invalid_argument(r.message);
+}
+
+// Everything an ensemble needs: a supported grid and precision, at least one
+// member, the model rules, and a Kolmogorov term inside the dealiased grid.
 void validate(size_t n, size_t members, int precision, const hzl_parameters &p) {
     (void)elems_per_thread((int)n);
     if (members == 0)
         throw std::invalid_argument("ensemble needs at least one member");
     if (precision != HZL_FP32 && precision != HZL_FP64)
         throw std::invalid_argument("precision must be HZL_FP32 or HZL_FP64");
-    if (!std::isfinite(p.viscosity) || p.viscosity < 0 || !std::isfinite(p.friction) ||
-        p.friction < 0 || !std::isfinite(p.forcing_sigma) || p.forcing_sigma < 0 ||
-        !std::isfinite(p.forcing_tau) || p.forcing_tau <= 0)
-        throw std::invalid_argument(
-            "nu, alpha, sigma must be finite and nonnegative; tau must be positive");
-    if (p.forcing_min < 1 || p.forcing_max < p.forcing_min ||
-        3 * static_cast<long long>(p.forcing_max) >= static_cast<long long>(n))
-        throw std::invalid_argument("forcing shell must lie strictly inside the dealiased grid");
+    validate_model(n, p);
     if (!std::isfinite(p.kolmogorov_amplitude) || p.kolmogorov_wavenumber < 0 ||
         3LL * p.kolmogorov_wavenumber >= (long long)n)
         throw std::invalid_argument("Kolmogorov forcing must lie inside the dealiased grid");
@@ -2287,6 +2335,22 @@ int hzl_ensemble_autotune(hzl_ensemble_t h, int use_cache, double *col_ms, doubl
 
 int hzl_ensemble_destroy(hzl_ensemble_t h) {
     return guarded([&] { delete h; });
+}
+
+int hzl_parameters_validate(size_t n, const hzl_parameters *params) {
+    return guarded([&] {
+        if (!params)
+            throw std::invalid_argument("null argument");
+        hzl_amd::validate_model(n, *params);
+    });
+}
+
+uint64_t hzl_stream_seed(uint64_t base, uint64_t index) {
+    return hzl_amd::stream_seed(base, index);
+}
+
+int hzl_ensemble_set_member_tau(hzl_ensemble_t h, size_t member, double tau) {
+    return guarded([&] { engine_of(h).set_member_tau(member, tau); });
 }
 
 int hzl_ensemble_set_member_parameters(hzl_ensemble_t h, size_t member, double viscosity,
@@ -2440,6 +2504,14 @@ int hzl_ensemble_diagnostics(hzl_ensemble_t h, double *out) {
         if (!out)
             throw std::invalid_argument("null output");
         engine_of(h).diagnostics(out);
+    });
+}
+
+int hzl_ensemble_energy_spectrum(hzl_ensemble_t h, double *out) {
+    return guarded([&] {
+        if (!out)
+            throw std::invalid_argument("null argument");
+        engine_of(h).energy_spectrum(out);
     });
 }
 

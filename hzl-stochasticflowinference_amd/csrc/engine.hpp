@@ -76,6 +76,8 @@ class EngineBase {
     std::unique_ptr<ModuleState> enkf; // ensemble Kalman filter state, if started
     virtual void set_member_parameters(size_t m, double nu, double alpha, double sigma,
                                        double amp) = 0;
+    // Per-member OU time constant (the creation parameters' forcing_tau until set).
+    virtual void set_member_tau(size_t m, double tau) = 0;
     virtual void set_state(size_t m, const double *field) = 0;
     virtual void get_state(size_t m, double *field) = 0;
     virtual void init_grf() = 0;
@@ -116,6 +118,7 @@ class EngineBase {
     virtual void step_forced(double dt, const double *forcing, bool shared) = 0;
     virtual void synchronize() = 0;
     virtual void diagnostics(double *out) = 0;
+    virtual void energy_spectrum(double *out) = 0; // [B][n] shell bins
     virtual void profile_step(double dt, float *ms) = 0;
     virtual void pf_begin(uint64_t seed, size_t max_groups) = 0;
     virtual void pf_assimilate(double dt, size_t steps, size_t nobs, const int *x, const int *y,

@@ -1,349 +1,481 @@
-// hzl_cli.cpp -- the reference CLI's `ensemble`, `filter` and `particle-sample`
-// commands (src/main.cpp:384-526) over the GPU drop-in layer (libhzl_gpu.so).
+// hzl_cli.cpp -- command-line front end of the GPU drop-in (libhzl_gpu.so) for
+// three flows of the reference executable: `ensemble`, `filter` and
+// `particle-sample` (reference src/main.cpp:384-526 define their contract).
 //
-// Same command names, options, defaults, output directory rules, files and
-// their columns as the reference's flows, so scripts that drive those commands
-// switch to the GPU engine by changing the executable's name.  Options are
-// unique `--name value` pairs; an option the command does not read is an
-// error; the output directory must be new or empty and receives
-// invocation.json with every resolved option.  Errors print `error: <what>` on
-// stderr and exit 1.  The other reference commands (calibration, ROM, Monte
-// Carlo experiments) are outside this engine and are rejected as unknown.
+// Every command is a row of the table in commands(): its options (name, type,
+// default) in the order their values are checked, an optional check that runs
+// before anything is written, the default output directory, and the action.
+// The contract kept from the reference, so scripts can switch executables:
+//   * arguments after the command are unique `--name value` pairs;
+//   * an option the command does not declare is an error, reported after the
+//     values of the declared ones have been checked;
+//   * the output directory must be new or empty; it receives invocation.json
+//     ({"schema":1,"command":..,"options":{name: value in force}}, values as
+//     given or as the default is spelled by std::to_string);
+//   * the same output files, columns, summary keys and stdout lines;
+//   * failures print `error: <message>` on stderr and exit with status 1.
+// The reference's other commands (experiments, ROM, calibration) are not part
+// of this engine and are reported as unknown commands.
 #include "hzl_gpu.hpp"
+#include "text_io.hpp"
 
-#include <charconv>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <filesystem>
-#include <fstream>
-#include <iomanip>
+#include <functional>
 #include <iostream>
 #include <limits>
 #include <map>
-#include <set>
 #include <stdexcept>
 #include <string>
 #include <string_view>
-#include <type_traits>
+#include <variant>
+#include <vector>
 
 namespace {
 
-// --name value pairs; get() records the value in force (given or default),
-// finish() rejects options no one read (src/main.cpp:23-95).
-class Options {
-  public:
-    Options(int argc, char **argv) : command_(argv[1]) {
-        for (int i = 2; i < argc; i += 2) {
-            const std::string key = argv[i];
-            if (key.rfind("--", 0) != 0 || i + 1 >= argc || given_.count(key))
-                throw std::invalid_argument("options require unique --name value pairs");
-            given_[key] = argv[i + 1];
-        }
-    }
-    std::string get(const std::string &key, const std::string &fallback) {
-        read_.insert(key);
-        const auto it = given_.find(key);
-        const std::string value = it == given_.end() ? fallback : it->second;
-        resolved_[key.substr(2)] = value;
-        return value;
-    }
-    template <class T> T number(const std::string &key, T fallback) {
-        const std::string text = get(key, std::to_string(fallback));
-        T value{};
-        const auto [end, error] = std::from_chars(text.data(), text.data() + text.size(), value);
-        if (error != std::errc{} || end != text.data() + text.size())
-            throw std::invalid_argument("invalid value for " + key);
+namespace fs = std::filesystem;
+using hzl::text::fixed6;
+using hzl::text::JsonObject;
+using hzl::text::LineWriter;
+
+enum class Kind { text, count, seed, integer, real };
+// seeds are unsigned long long, counts std::size_t (distinct types on LP64)
+using Value = std::variant<std::string, std::size_t, unsigned long long, int, double>;
+
+struct OptionSpec {
+    std::string name; // without the leading "--"
+    Kind kind;
+    Value fallback;
+};
+
+// The spelling of a default in invocation.json (std::to_string's).
+std::string spell(const Value &v) {
+    return std::visit(
+        [](const auto &x) -> std::string {
+            using T = std::decay_t<decltype(x)>;
+            if constexpr (std::is_same_v<T, std::string>)
+                return x;
+            else if constexpr (std::is_same_v<T, double>)
+                return fixed6(x);
+            else
+                return hzl::text::number(x);
+        },
+        v);
+}
+
+Value parse(const OptionSpec &o, const std::string &text) {
+    const std::string flag = "--" + o.name;
+    const auto typed = [&](auto zero) -> Value {
+        using T = decltype(zero);
+        T v{};
+        // from_chars over the whole token, as the reference parses option values
+        const auto r = std::from_chars(text.data(), text.data() + text.size(), v);
+        if (r.ec != std::errc{} || r.ptr != text.data() + text.size())
+            throw std::invalid_argument("invalid value for " + flag);
         if constexpr (std::is_floating_point_v<T>)
-            if (!std::isfinite(value))
-                throw std::invalid_argument("nonfinite value for " + key);
-        return value;
+            if (!std::isfinite(v))
+                throw std::invalid_argument("nonfinite value for " + flag);
+        return v;
+    };
+    switch (o.kind) {
+    case Kind::text:
+        return text;
+    case Kind::count:
+        return typed(std::size_t{});
+    case Kind::seed:
+        return typed((unsigned long long)0);
+    case Kind::integer:
+        return typed(int{});
+    case Kind::real:
+        return typed(double{});
     }
-    void finish() const {
-        for (const auto &kv : given_)
-            if (!read_.count(kv.first))
-                throw std::invalid_argument("unknown option " + kv.first);
+    throw std::logic_error("option kind");
+}
+
+// Resolved values of one invocation, by option name.
+class Values {
+  public:
+    template <class T> const T &get(const std::string &name) const {
+        return std::get<T>(values_.at(name));
     }
-    // invocation.json: {"schema":1,"command":...,"options":{resolved, by name}}
-    void record(const std::filesystem::path &path) const {
-        std::ofstream out(path);
-        const auto quoted = [&out](const std::string &v) {
-            out << '"';
-            for (unsigned char c : v) {
-                if (c == '"' || c == '\\')
-                    out << '\\' << (char)c;
-                else if (c < 32)
-                    out << "\\u" << std::hex << std::setw(4) << std::setfill('0') << (unsigned)c
-                        << std::dec;
-                else
-                    out << (char)c;
-            }
-            out << '"';
-        };
-        out << "{\"schema\":1,\"command\":";
-        quoted(command_);
-        out << ",\"options\":{";
-        bool first = true;
-        for (const auto &kv : resolved_) {
-            if (!first)
-                out << ',';
-            first = false;
-            quoted(kv.first);
-            out << ':';
-            quoted(kv.second);
-        }
-        out << "}}\n";
-        if (!out)
-            throw std::runtime_error("cannot record invocation");
+    double real(const std::string &name) const { return get<double>(name); }
+    std::size_t count(const std::string &name) const { return get<std::size_t>(name); }
+    std::uint64_t seed(const std::string &name) const { return get<unsigned long long>(name); }
+    const std::string &text(const std::string &name) const { return get<std::string>(name); }
+    void set(const std::string &name, Value v, std::string spelled) {
+        values_[name] = std::move(v);
+        shown_[name] = std::move(spelled);
+    }
+    const std::map<std::string, std::string> &shown() const { return shown_; }
+
+    hzl::Parameters physics() const {
+        hzl::Parameters p;
+        p.viscosity = real("nu");
+        p.friction = real("alpha");
+        p.forcing_sigma = real("sigma");
+        p.forcing_tau = real("tau");
+        p.forcing_min = get<int>("kmin");
+        p.forcing_max = get<int>("kmax");
+        return p;
     }
 
   private:
-    std::string command_;
-    std::map<std::string, std::string> given_, resolved_;
-    std::set<std::string> read_;
+    std::map<std::string, Value> values_;
+    std::map<std::string, std::string> shown_;
 };
 
-std::ofstream output(const std::filesystem::path &path) {
-    std::ofstream out(path);
-    out.exceptions(std::ios::badbit | std::ios::failbit);
-    out << std::setprecision(17);
-    return out;
+struct Command {
+    std::string name;
+    std::string default_output;
+    std::vector<OptionSpec> options;
+    std::function<void(const Values &)> check; // before anything is written
+    std::function<void(const Values &, const fs::path &)> run;
+};
+
+// The six model parameters every command that builds solvers accepts.
+std::vector<OptionSpec> physics_options() {
+    const hzl::Parameters d;
+    return {{"nu", Kind::real, d.viscosity},        {"alpha", Kind::real, d.friction},
+            {"sigma", Kind::real, d.forcing_sigma}, {"tau", Kind::real, d.forcing_tau},
+            {"kmin", Kind::integer, d.forcing_min}, {"kmax", Kind::integer, d.forcing_max}};
 }
 
-// --output (default results/<name>) must be new or empty; every option has been
-// read by now, so unknown ones fail before anything is written.
-std::filesystem::path directory(Options &args, const std::string &name) {
-    const std::filesystem::path path = args.get("--output", "results/" + name);
-    if (std::filesystem::exists(path) && !std::filesystem::is_empty(path))
-        throw std::invalid_argument(
-            "output directory is not empty; choose a new --output directory");
-    args.finish();
-    std::filesystem::create_directories(path);
-    args.record(path / "invocation.json");
-    return path;
+std::vector<OptionSpec> concat(std::vector<OptionSpec> a, const std::vector<OptionSpec> &b) {
+    a.insert(a.end(), b.begin(), b.end());
+    return a;
 }
 
-hzl::Parameters parameters(Options &args) {
-    hzl::Parameters p;
-    p.viscosity = args.number("--nu", p.viscosity);
-    p.friction = args.number("--alpha", p.friction);
-    p.forcing_sigma = args.number("--sigma", p.forcing_sigma);
-    p.forcing_tau = args.number("--tau", p.forcing_tau);
-    p.forcing_min = args.number("--kmin", p.forcing_min);
-    p.forcing_max = args.number("--kmax", p.forcing_max);
-    return p;
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-double seconds_since(std::chrono::steady_clock::time_point start) {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+LineWriter csv(const fs::path &path) {
+    return LineWriter(path, ',', "cannot write " + path.string());
 }
 
-// Welford running mean / variance (src/inference.cpp:540-554).
-struct Moments {
-    std::size_t count = 0;
-    double mean = 0, m2 = 0;
-    void add(double x) {
+void emit(const fs::path &path, const std::string &text) {
+    hzl::text::write_file(path, text, "cannot write " + path.string());
+}
+
+// Physical field of a record: rows of (ix, iy, value) in y-major order.
+template <class Prefix>
+void field_rows(LineWriter &out, std::size_t n, const hzl::Field &f, Prefix prefix) {
+    for (std::size_t y = 0; y < n; ++y)
+        for (std::size_t x = 0; x < n; ++x) {
+            prefix(out);
+            out.line(x, y, f[y * n + x].real());
+        }
+}
+
+// Sample mean and the standard error of the mean (sample variance / count).
+struct MeanError {
+    double mean = 0, standard_error = 0;
+};
+MeanError mean_error(const std::vector<double> &v) {
+    for (double x : v)
         if (!std::isfinite(x))
             throw std::invalid_argument("nonfinite statistical sample");
-        ++count;
-        const double delta = x - mean;
-        mean += delta / (double)count;
-        m2 += delta * (x - mean);
-    }
-    double standard_error() const {
-        const double var = count > 1 ? m2 / (double)(count - 1)
-                                     : std::numeric_limits<double>::quiet_NaN();
-        return std::sqrt(var / (double)count);
-    }
-};
-
-// `ensemble` (src/main.cpp:384-416): independent members from zero vorticity.
-void ensemble_command(Options &args) {
-    const auto n = args.number<std::size_t>("--n", 32), steps = args.number<std::size_t>("--steps", 200),
-               count = args.number<std::size_t>("--count", 16),
-               workers = args.number<std::size_t>("--workers", 4);
-    const double dt = args.number("--dt", 0.005);
-    const auto seed = args.number<std::uint64_t>("--seed", 42);
-    const hzl::Parameters p = parameters(args);
-    const auto path = directory(args, "ensemble");
-    args.finish();
-    const auto start = std::chrono::steady_clock::now();
-    const auto results = hzl::ensemble(n, p, dt, steps, count, workers, seed);
-    const double seconds = seconds_since(start);
-    auto members = output(path / "members.csv");
-    members << "member,seed,time,energy,enstrophy,dissipation\n";
-    Moments energy;
-    for (std::size_t i = 0; i < results.size(); ++i) {
-        const auto &d = results[i];
-        energy.add(d.energy);
-        members << i << ',' << hzl::stream_seed(seed, i) << ',' << d.time << ',' << d.energy
-                << ',' << d.enstrophy << ',' << d.dissipation << '\n';
-    }
-    p.validate(n);
-    auto meta = output(path / "metadata.json");
-    meta << "{\n  \"schema\": 1,\n  \"version\": \"0.1.0\",\n  \"backend\": "
-            "\"mi355x-gfx950-hip-fp64\",\n  \"compiler\": \""
-         << __VERSION__ << "\",\n"
-         << "  \"n\": " << n << ",\n  \"dt\": " << dt << ",\n  \"integrator\": \"rk4\""
-         << ",\n  \"steps_this_run\": " << steps << ",\n  \"seed\": " << seed << ",\n"
-         << "  \"initial\": \"zero\",\n  \"nu\": " << p.viscosity << ",\n  \"alpha\": "
-         << p.friction << ",\n"
-         << "  \"sigma\": " << p.forcing_sigma << ",\n  \"tau\": " << p.forcing_tau
-         << ",\n  \"kmin\": " << p.forcing_min << ",\n  \"kmax\": " << p.forcing_max << ",\n"
-         << "  \"final_time\": " << dt * (double)steps
-         << ",\n  \"wall_seconds_including_diagnostics\": " << seconds << "\n}\n";
-    auto summary = output(path / "summary.json");
-    summary << "{\"count\":" << count << ",\"workers\":" << workers
-            << ",\"mean_energy\":" << energy.mean << ",\"energy_standard_error\":"
-            << (count > 1 ? std::to_string(energy.standard_error()) : "null")
-            << ",\"wall_seconds\":" << seconds << "}\n";
-    std::cout << "ensemble complete: " << path << "; energy mean=" << energy.mean
-              << ", SE=" << energy.standard_error() << ", seconds=" << seconds << '\n';
+    MeanError r;
+    double sum = 0;
+    for (double x : v)
+        sum += x;
+    r.mean = v.empty() ? 0 : sum / (double)v.size();
+    double ss = 0;
+    for (double x : v)
+        ss += (x - r.mean) * (x - r.mean);
+    const double variance =
+        v.size() > 1 ? ss / (double)(v.size() - 1) : std::numeric_limits<double>::quiet_NaN();
+    r.standard_error = std::sqrt(variance / (double)v.size());
+    return r;
 }
 
-// `filter` (src/main.cpp:417-485): bootstrap particle filter or serial EnKF.
-void filter_command(Options &args) {
-    const auto input = args.get("--data", "results/observations/observations.csv");
-    const auto count = args.number<std::size_t>("--particles", 64);
-    const auto method = args.get("--method", "particle");
-    const auto seed = args.number<std::uint64_t>("--seed", 2026);
-    hzl::EnKFOptions options;
-    options.inflation = args.number("--inflation", 1.0);
-    options.localization_radius = args.number("--localization-radius", 0.0);
-    options.forecast_steps = args.number<std::size_t>("--forecast-steps", 0);
-    options.forecast_every = args.number<std::size_t>("--forecast-every", 1);
-    const hzl::Parameters p = parameters(args);
-    const auto path = directory(args, "filter");
-    args.finish();
-    const auto data = hzl::Dataset::load(input);
-    data.save(path / "observations.csv");
-    const auto start = std::chrono::steady_clock::now();
+// ------------------------------------------------------------- ensemble --
+void run_ensemble(const Values &a, const fs::path &dir) {
+    const std::size_t n = a.count("n"), steps = a.count("steps"), count = a.count("count"),
+                      workers = a.count("workers");
+    const double dt = a.real("dt");
+    const std::uint64_t seed = a.seed("seed");
+    const hzl::Parameters p = a.physics();
+    const auto t0 = std::chrono::steady_clock::now();
+    const auto members = hzl::ensemble(n, p, dt, steps, count, workers, seed);
+    const double seconds = seconds_since(t0);
+
+    auto table = csv(dir / "members.csv");
+    table.line("member", "seed", "time", "energy", "enstrophy", "dissipation");
+    std::vector<double> energies;
+    for (std::size_t i = 0; i < members.size(); ++i) {
+        const hzl::Diagnostics &d = members[i];
+        table.line(i, hzl::stream_seed(seed, i), d.time, d.energy, d.enstrophy, d.dissipation);
+        energies.push_back(d.energy);
+    }
+    table.close();
+    const MeanError e = mean_error(energies);
+    p.validate(n);
+
+    JsonObject meta;
+    meta.value("schema", 1)
+        .value("version", "0.1.0")
+        .value("backend", "mi355x-gfx950-hip-fp64")
+        .value("compiler", __VERSION__)
+        .value("n", n)
+        .value("dt", dt)
+        .value("integrator", "rk4")
+        .value("steps_this_run", steps)
+        .value("seed", seed)
+        .value("initial", "zero")
+        .value("nu", p.viscosity)
+        .value("alpha", p.friction)
+        .value("sigma", p.forcing_sigma)
+        .value("tau", p.forcing_tau)
+        .value("kmin", p.forcing_min)
+        .value("kmax", p.forcing_max)
+        .value("final_time", dt * (double)steps)
+        .value("wall_seconds_including_diagnostics", seconds);
+    emit(dir / "metadata.json", meta.pretty());
+
+    JsonObject summary;
+    summary.value("count", count)
+        .value("workers", workers)
+        .value("mean_energy", e.mean)
+        .raw("energy_standard_error", count > 1 ? fixed6(e.standard_error) : "null")
+        .value("wall_seconds", seconds);
+    emit(dir / "summary.json", summary.compact());
+    std::cout << "ensemble complete: " << dir << "; energy mean=" << e.mean
+              << ", SE=" << e.standard_error << ", seconds=" << seconds << '\n';
+}
+
+// --------------------------------------------------------------- filter --
+void run_filter(const Values &a, const fs::path &dir) {
+    const std::size_t count = a.count("particles");
+    const std::uint64_t seed = a.seed("seed");
+    const std::string &method = a.text("method");
+    const hzl::Parameters p = a.physics();
+    const hzl::Dataset data = hzl::Dataset::load(a.text("data"));
+    data.save(dir / "observations.csv");
+    const std::size_t n = data.n;
+    const auto t0 = std::chrono::steady_clock::now();
+
     if (method == "enkf") {
-        const auto history = hzl::ensemble_kalman_filter(data, p, count, seed, options);
-        auto diagnostics = output(path / "filter.csv"),
-             fields = output(path / "reconstruction.csv");
-        diagnostics << "step,phase,forecast_sensor_rmse,posterior_spread\n";
-        fields << "step,phase,ix,iy,vorticity\n";
+        hzl::EnKFOptions o;
+        o.inflation = a.real("inflation");
+        o.localization_radius = a.real("localization-radius");
+        o.forecast_steps = a.count("forecast-steps");
+        o.forecast_every = a.count("forecast-every");
+        const auto history = hzl::ensemble_kalman_filter(data, p, count, seed, o);
+        auto records = csv(dir / "filter.csv");
+        auto fields = csv(dir / "reconstruction.csv");
+        records.line("step", "phase", "forecast_sensor_rmse", "posterior_spread");
+        fields.line("step", "phase", "ix", "iy", "vorticity");
         for (const auto &r : history) {
             const char *phase = r.is_forecast ? "forecast" : "analysis";
-            diagnostics << r.step << ',' << phase << ',';
-            if (!r.is_forecast)
-                diagnostics << r.forecast_sensor_rmse;
-            diagnostics << ',' << r.posterior_spread << '\n';
-            for (std::size_t y = 0; y < data.n; ++y)
-                for (std::size_t x = 0; x < data.n; ++x)
-                    fields << r.step << ',' << phase << ',' << x << ',' << y << ','
-                           << r.mean_vorticity[y * data.n + x].real() << '\n';
+            records.add(r.step).add(phase);
+            if (r.is_forecast)
+                records.skip(); // free forecasts have no sensor RMSE
+            else
+                records.add(r.forecast_sensor_rmse);
+            records.add(r.posterior_spread).end();
+            field_rows(fields, n, r.mean_vorticity,
+                       [&](LineWriter &w) { w.add(r.step).add(phase); });
         }
-        auto summary = output(path / "summary.json");
-        summary << "{\"method\":\"serial perturbed-observation EnKF\",\"members\":" << count
-                << ",\"seed\":" << seed << ",\"inflation\":" << options.inflation
-                << ",\"localization_radius_grid_cells\":" << options.localization_radius
-                << ",\"forecast_steps\":" << options.forecast_steps
-                << ",\"forecast_every\":" << options.forecast_every
-                << ",\"wall_seconds\":" << seconds_since(start) << "}\n";
-        std::cout << "ensemble Kalman reconstruction complete: " << path << '\n';
+        records.close();
+        fields.close();
+        JsonObject summary;
+        summary.value("method", "serial perturbed-observation EnKF")
+            .value("members", count)
+            .value("seed", seed)
+            .value("inflation", o.inflation)
+            .value("localization_radius_grid_cells", o.localization_radius)
+            .value("forecast_steps", o.forecast_steps)
+            .value("forecast_every", o.forecast_every)
+            .value("wall_seconds", seconds_since(t0));
+        emit(dir / "summary.json", summary.compact());
+        std::cout << "ensemble Kalman reconstruction complete: " << dir << '\n';
         return;
     }
     if (method != "particle")
         throw std::invalid_argument("filter method must be particle or enkf");
-    const auto result = hzl::bootstrap_filter(data, p, count, seed);
+
+    const hzl::FilterResult result = hzl::bootstrap_filter(data, p, count, seed);
     if (!std::isfinite(result.log_likelihood))
         throw std::runtime_error("particle likelihood collapsed numerically");
-    auto diagnostics = output(path / "filter.csv"), fields = output(path / "reconstruction.csv");
-    diagnostics << "step,log_likelihood_increment,effective_particles,maximum_weight\n";
-    fields << "step,ix,iy,vorticity\n";
+    auto records = csv(dir / "filter.csv");
+    auto fields = csv(dir / "reconstruction.csv");
+    records.line("step", "log_likelihood_increment", "effective_particles", "maximum_weight");
+    fields.line("step", "ix", "iy", "vorticity");
     for (const auto &r : result.history) {
-        diagnostics << r.step << ',' << r.log_likelihood_increment << ','
-                    << r.effective_particles << ',' << r.maximum_weight << '\n';
-        for (std::size_t y = 0; y < data.n; ++y)
-            for (std::size_t x = 0; x < data.n; ++x)
-                fields << r.step << ',' << x << ',' << y << ','
-                       << r.mean_vorticity[y * data.n + x].real() << '\n';
+        records.line(r.step, r.log_likelihood_increment, r.effective_particles, r.maximum_weight);
+        field_rows(fields, n, r.mean_vorticity, [&](LineWriter &w) { w.add(r.step); });
     }
-    auto summary = output(path / "summary.json");
-    summary << "{\"particles\":" << count << ",\"seed\":" << seed
-            << ",\"log_likelihood_estimate\":" << result.log_likelihood
-            << ",\"resampling_events\":" << result.resampling_events
-            << ",\"wall_seconds\":" << seconds_since(start) << "}\n";
-    std::cout << "particle filter complete: " << path
+    records.close();
+    fields.close();
+    JsonObject summary;
+    summary.value("particles", count)
+        .value("seed", seed)
+        .value("log_likelihood_estimate", result.log_likelihood)
+        .value("resampling_events", result.resampling_events)
+        .value("wall_seconds", seconds_since(t0));
+    emit(dir / "summary.json", summary.compact());
+    std::cout << "particle filter complete: " << dir
               << "; log likelihood estimate=" << result.log_likelihood
               << ", resampling events=" << result.resampling_events << '\n';
 }
 
-// `particle-sample` (src/main.cpp:486-526): pseudo-marginal Metropolis chain.
-void particle_sample_command(Options &args) {
-    const auto input = args.get("--data", "results/observations/training.csv");
-    const auto count = args.number<std::size_t>("--samples", 500),
-               particles = args.number<std::size_t>("--particles", 32);
-    const auto seed = args.number<std::uint64_t>("--seed", 123);
-    const double scale = args.number("--proposal", 0.05);
-    const double nu = args.number("--initial-nu", 0.03),
-                 alpha = args.number("--initial-alpha", 0.1),
-                 sigma = args.number("--initial-sigma", 0.2),
-                 tau = args.number("--initial-tau", 0.5);
-    if (nu <= 0 || alpha <= 0 || sigma <= 0 || tau <= 0)
-        throw std::invalid_argument("initial stochastic parameters must be positive");
-    const hzl::Vector initial{std::log(nu), std::log(alpha), std::log(sigma), std::log(tau)};
-    const auto path = directory(args, "particle-posterior");
-    args.finish();
-    const auto data = hzl::Dataset::load(input);
-    data.save(path / "training.csv");
-    const auto chain = hzl::particle_metropolis(data, initial, scale, count, particles, seed);
-    auto out = output(path / "chain.csv");
-    out << "iteration,nu,alpha,sigma,tau,log_likelihood,log_posterior,likelihood_seed,accepted\n";
+// ------------------------------------------------------ particle-sample --
+const char *const kInitial[4] = {"initial-nu", "initial-alpha", "initial-sigma", "initial-tau"};
+
+void check_particle_sample(const Values &a) {
+    for (const char *name : kInitial)
+        if (!(a.real(name) > 0))
+            throw std::invalid_argument("initial stochastic parameters must be positive");
+}
+
+void run_particle_sample(const Values &a, const fs::path &dir) {
+    const std::size_t samples = a.count("samples"), particles = a.count("particles");
+    hzl::Vector q0;
+    for (const char *name : kInitial)
+        q0.push_back(std::log(a.real(name)));
+    const hzl::Dataset data = hzl::Dataset::load(a.text("data"));
+    data.save(dir / "training.csv");
+    const auto chain = hzl::particle_metropolis(data, q0, a.real("proposal"), samples, particles,
+                                                a.seed("seed"));
+    auto out = csv(dir / "chain.csv");
+    out.line("iteration", "nu", "alpha", "sigma", "tau", "log_likelihood", "log_posterior",
+             "likelihood_seed", "accepted");
     std::size_t accepted = 0;
     for (std::size_t i = 0; i < chain.size(); ++i) {
-        const auto &s = chain[i];
-        accepted += s.accepted;
-        out << i;
+        const hzl::ParticleSample &s = chain[i];
+        out.add(i);
         for (double q : s.parameters)
-            out << ',' << std::exp(q);
-        out << ',' << s.log_likelihood << ',' << s.log_posterior << ',' << s.likelihood_seed
-            << ',' << s.accepted << '\n';
+            out.add(std::exp(q));
+        out.line(s.log_likelihood, s.log_posterior, s.likelihood_seed, s.accepted);
+        accepted += s.accepted ? 1 : 0;
     }
-    const double rate = (double)accepted / (double)count;
-    auto summary = output(path / "summary.json");
-    summary << "{\"samples\":" << count << ",\"particles\":" << particles
-            << ",\"acceptance_rate\":" << rate
-            << ",\"method\":\"pseudo-marginal random-walk Metropolis\"}\n";
-    std::cout << "particle posterior chain written: " << path << "; acceptance=" << rate
-              << '\n';
+    out.close();
+    const double rate = (double)accepted / (double)samples;
+    JsonObject summary;
+    summary.value("samples", samples)
+        .value("particles", particles)
+        .value("acceptance_rate", rate)
+        .value("method", "pseudo-marginal random-walk Metropolis");
+    emit(dir / "summary.json", summary.compact());
+    std::cout << "particle posterior chain written: " << dir << "; acceptance=" << rate << '\n';
+}
+
+const std::vector<Command> &commands() {
+    static const std::vector<Command> table = {
+        {"ensemble",
+         "results/ensemble",
+         concat({{"n", Kind::count, std::size_t{32}},
+                 {"steps", Kind::count, std::size_t{200}},
+                 {"count", Kind::count, std::size_t{16}},
+                 {"workers", Kind::count, std::size_t{4}},
+                 {"dt", Kind::real, 0.005},
+                 {"seed", Kind::seed, 42ULL}},
+                physics_options()),
+         nullptr,
+         run_ensemble},
+        {"filter",
+         "results/filter",
+         concat({{"data", Kind::text, std::string("results/observations/observations.csv")},
+                 {"particles", Kind::count, std::size_t{64}},
+                 {"method", Kind::text, std::string("particle")},
+                 {"seed", Kind::seed, 2026ULL},
+                 {"inflation", Kind::real, 1.0},
+                 {"localization-radius", Kind::real, 0.0},
+                 {"forecast-steps", Kind::count, std::size_t{0}},
+                 {"forecast-every", Kind::count, std::size_t{1}}},
+                physics_options()),
+         nullptr,
+         run_filter},
+        {"particle-sample",
+         "results/particle-posterior",
+         {{"data", Kind::text, std::string("results/observations/training.csv")},
+          {"samples", Kind::count, std::size_t{500}},
+          {"particles", Kind::count, std::size_t{32}},
+          {"seed", Kind::seed, 123ULL},
+          {"proposal", Kind::real, 0.05},
+          {"initial-nu", Kind::real, 0.03},
+          {"initial-alpha", Kind::real, 0.1},
+          {"initial-sigma", Kind::real, 0.2},
+          {"initial-tau", Kind::real, 0.5}},
+         check_particle_sample,
+         run_particle_sample},
+    };
+    return table;
 }
 
 void usage() {
-    std::cout << "HZL stochastic turbulence engine (MI355X) 0.1.0\n"
-              << "Commands: ensemble, filter, particle-sample\n"
-              << "Parameters: --nu 0.01 --alpha 0.1 --sigma 0.2 --tau 0.5 --kmin 2 --kmax 3\n"
-              << "ensemble: --n 32 --steps 200 --dt 0.005 --count 16 --workers 4 --seed 42 "
-                 "(zero initial vorticity)\n"
-              << "filter: --data observations.csv --particles 64 --method particle|enkf "
-                 "--seed 2026 --inflation 1 --localization-radius 0 --forecast-steps 0 "
-                 "--forecast-every 1 and physical parameters\n"
-              << "particle-sample: --data training.csv --particles 32 --samples 500 "
-                 "--proposal 0.05 --seed 123 --initial-nu 0.03 --initial-alpha 0.1 "
-                 "--initial-sigma 0.2 --initial-tau 0.5\n"
-              << "All commands: --output NEW_DIRECTORY\n";
+    std::cout << "HZL stochastic turbulence engine (MI355X) 0.1.0\nCommands:\n";
+    for (const Command &c : commands()) {
+        std::cout << "  " << c.name;
+        for (const OptionSpec &o : c.options)
+            std::cout << " --" << o.name << ' ' << spell(o.fallback);
+        std::cout << '\n';
+    }
+    std::cout << "All commands: --output NEW_DIRECTORY\n";
+}
+
+int dispatch(int argc, char **argv) {
+    if (argc < 2 || std::string_view(argv[1]) == "--help") {
+        usage();
+        return 0;
+    }
+    // unique --name value pairs
+    std::map<std::string, std::string> given;
+    for (int i = 2; i < argc; i += 2) {
+        const std::string key = argv[i];
+        if (key.size() < 2 || key.compare(0, 2, "--") != 0 || i + 1 >= argc ||
+            !given.emplace(key, argv[i + 1]).second)
+            throw std::invalid_argument("options require unique --name value pairs");
+    }
+    const Command *command = nullptr;
+    for (const Command &c : commands())
+        if (c.name == argv[1])
+            command = &c;
+    if (!command)
+        throw std::invalid_argument("unknown command: " + std::string(argv[1]));
+
+    Values values;
+    std::map<std::string, bool> declared;
+    for (const OptionSpec &o : command->options) {
+        const auto it = given.find("--" + o.name);
+        const std::string shown = it == given.end() ? spell(o.fallback) : it->second;
+        values.set(o.name, parse(o, shown), shown);
+        declared["--" + o.name] = true;
+    }
+    if (command->check)
+        command->check(values);
+    const auto out = given.find("--output");
+    const std::string dir = out == given.end() ? command->default_output : out->second;
+    values.set("output", dir, dir);
+    declared["--output"] = true;
+    if (fs::exists(dir) && !fs::is_empty(dir))
+        throw std::invalid_argument(
+            "output directory is not empty; choose a new --output directory");
+    for (const auto &kv : given)
+        if (!declared.count(kv.first))
+            throw std::invalid_argument("unknown option " + kv.first);
+
+    fs::create_directories(dir);
+    JsonObject options;
+    for (const auto &kv : values.shown())
+        options.value(kv.first, kv.second);
+    std::string invocation = options.compact();
+    invocation.pop_back(); // the object is nested: drop its newline
+    emit(fs::path(dir) / "invocation.json",
+         "{\"schema\":1,\"command\":" + hzl::text::json_string(command->name) +
+             ",\"options\":" + invocation + "}\n");
+    command->run(values, dir);
+    return 0;
 }
 
 } // namespace
 
 int main(int argc, char **argv) {
     try {
-        if (argc < 2 || std::string_view(argv[1]) == "--help") {
-            usage();
-            return 0;
-        }
-        Options args(argc, argv);
-        const std::string command = argv[1];
-        if (command == "ensemble")
-            ensemble_command(args);
-        else if (command == "filter")
-            filter_command(args);
-        else if (command == "particle-sample")
-            particle_sample_command(args);
-        else
-            throw std::invalid_argument("unknown command: " + command);
-        return 0;
+        return dispatch(argc, argv);
     } catch (const std::exception &error) {
         std::cerr << "error: " << error.what() << '\n';
         return 1;

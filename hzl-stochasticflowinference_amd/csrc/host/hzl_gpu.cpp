@@ -1,14 +1,17 @@
 // hzl_gpu.cpp -- C++ drop-in layer over the C-ABI (see hzl_gpu.hpp).
+//
+// Everything that computes runs in libhzl_amd.so: the solver, filters and
+// samplers on the GPU, and the reference's parameter rules, stream seeding and
+// prior behind the C-ABI.  This file converts between the reference's types
+// (Field, Parameters, Dataset, records) and the ABI's flat arrays, and holds
+// the two text formats (HZL_OBSERVATIONS_V2, HZL_CHECKPOINT_V1) through the
+// shared encodings of text_io.hpp.
 #include "hzl_gpu.hpp"
+#include "text_io.hpp"
 
 #include <algorithm>
 #include <cmath>
-#include <fstream>
-#include <iomanip>
 #include <limits>
-#include <numbers>
-#include <random>
-#include <sstream>
 #include <stdexcept>
 
 namespace hzl {
@@ -16,6 +19,7 @@ namespace hzl {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr double kNegativeInfinity = -std::numeric_limits<double>::infinity();
 
 // C-ABI status -> the reference's exception types (src/cuda_solver.cu:11-19).
 void check(int rc) {
@@ -43,6 +47,32 @@ Field physical(std::size_t n, Field spectral) {
     Fourier2D(n).inverse(spectral);
     return spectral;
 }
+
+// Index geometry of an n x n spectral field in the reference layout y n + x:
+// signed wave numbers, the conjugate partner, the 2/3-retained set (without
+// the mean) and the forcing shell of a parameter set.
+struct Grid {
+    std::size_t n;
+    int kx(std::size_t i) const { return wave_number(i % n, n); }
+    int ky(std::size_t i) const { return wave_number(i / n, n); }
+    std::size_t partner(std::size_t i) const {
+        return ((n - i / n) % n) * n + (n - i % n) % n;
+    }
+    bool retained(std::size_t i) const {
+        const int third = (int)n;
+        return i != 0 && 3 * std::abs(kx(i)) < third && 3 * std::abs(ky(i)) < third;
+    }
+    bool in_shell(std::size_t i, const Parameters &p) const {
+        const int k2 = kx(i) * kx(i) + ky(i) * ky(i);
+        return retained(i) && k2 >= p.forcing_min * p.forcing_min &&
+               k2 <= p.forcing_max * p.forcing_max;
+    }
+    void mask(Field &f) const {
+        for (std::size_t i = 0; i < f.size(); ++i)
+            if (!retained(i))
+                f[i] = Complex{};
+    }
+};
 
 } // namespace
 
@@ -108,64 +138,46 @@ int wave_number(std::size_t index, std::size_t n) {
 
 // ----------------------------------------------------------- parameters --
 void Parameters::validate(std::size_t n) const {
-    if (!std::isfinite(viscosity) || viscosity < 0 || !std::isfinite(friction) || friction < 0 ||
-        !std::isfinite(forcing_sigma) || forcing_sigma < 0 || !std::isfinite(forcing_tau) ||
-        forcing_tau <= 0)
-        throw std::invalid_argument(
-            "nu, alpha, sigma must be finite and nonnegative; tau must be positive");
-    if (forcing_min < 1 || forcing_max < forcing_min ||
-        3 * static_cast<long long>(forcing_max) >= static_cast<long long>(n))
-        throw std::invalid_argument("forcing shell must lie strictly inside the dealiased grid");
+    const hzl_parameters p = abi_parameters(*this, 0.0, 0);
+    check(hzl_parameters_validate(n, &p));
 }
 
 std::uint64_t stream_seed(std::uint64_t base, std::uint64_t index) {
-    // SplitMix64 finaliser of base + golden-ratio increments (src/solver.cpp:61-66)
-    std::uint64_t x = base + 0x9e3779b97f4a7c15ULL * (index + 1);
-    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
-    return x ^ (x >> 31);
+    return hzl_stream_seed(base, index);
 }
-
-namespace {
-void mask(std::size_t n, Field &f) {
-    for (std::size_t y = 0; y < n; ++y)
-        for (std::size_t x = 0; x < n; ++x) {
-            const int a = wave_number(x, n), b = wave_number(y, n);
-            const bool active = 3 * std::abs(a) < (int)n && 3 * std::abs(b) < (int)n &&
-                                (x != 0 || y != 0);
-            if (!active)
-                f[y * n + x] = 0;
-        }
-}
-} // namespace
 
 Field initial_condition(std::size_t n, const std::string &name) {
-    if (name != "taylor-green" && name != "multimode" && name != "zero")
+    // Solver::initialize's named fields (src/solver.cpp:140-161): Taylor-Green
+    // 2 cos x cos y, plus three modes for multimode, or zero
+    const bool zero = name == "zero", multimode = name == "multimode";
+    if (!zero && !multimode && name != "taylor-green")
         throw std::invalid_argument("unknown initial condition");
     Field w(n * n);
-    for (std::size_t y = 0; y < n; ++y)
-        for (std::size_t x = 0; x < n; ++x) {
-            const double a = 2 * kPi * (double)x / (double)n, b = 2 * kPi * (double)y / (double)n;
-            double v = name == "zero" ? 0.0 : 2 * std::cos(a) * std::cos(b);
-            if (name == "multimode")
-                v += 0.6 * std::sin(2 * a + b) + 0.4 * std::cos(a - 3 * b) +
-                     0.3 * std::sin(3 * a + 2 * b);
-            w[y * n + x] = v;
-        }
+    const double h = 2 * kPi / (double)n;
+    for (std::size_t i = 0; i < n * n; ++i) {
+        const double x = h * (double)(i % n), y = h * (double)(i / n);
+        double v = zero ? 0.0 : 2 * std::cos(x) * std::cos(y);
+        if (multimode)
+            v += 0.6 * std::sin(2 * x + y) + 0.4 * std::cos(x - 3 * y) +
+                 0.3 * std::sin(3 * x + 2 * y);
+        w[i] = v;
+    }
     Fourier2D(n).forward(w);
-    mask(n, w);
+    Grid{n}.mask(w);
     return w;
 }
 
 Field physical_to_spectral(std::size_t n, const Field &vorticity) {
     if (vorticity.size() != n * n)
         throw std::invalid_argument("physical initial field size mismatch");
-    for (const auto &v : vorticity)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || v.imag() != 0)
-            throw std::invalid_argument("physical initial vorticity must be finite and real");
+    const bool real_and_finite = std::all_of(vorticity.begin(), vorticity.end(), [](Complex v) {
+        return std::isfinite(v.real()) && std::isfinite(v.imag()) && v.imag() == 0;
+    });
+    if (!real_and_finite)
+        throw std::invalid_argument("physical initial vorticity must be finite and real");
     Field w = vorticity;
     Fourier2D(n).forward(w);
-    mask(n, w);
+    Grid{n}.mask(w);
     return w;
 }
 
@@ -359,6 +371,15 @@ void Ensemble::synchronize() {
     check(hzl_ensemble_synchronize(handle_));
 }
 
+std::vector<std::vector<double>> Ensemble::energy_spectra() {
+    std::vector<double> bins(members_ * n_);
+    check(hzl_ensemble_energy_spectrum(handle_, bins.data()));
+    std::vector<std::vector<double>> out(members_);
+    for (std::size_t m = 0; m < members_; ++m)
+        out[m].assign(bins.begin() + (long)(m * n_), bins.begin() + (long)((m + 1) * n_));
+    return out;
+}
+
 std::vector<Diagnostics> Ensemble::diagnostics() {
     std::vector<double> d(6 * members_);
     check(hzl_ensemble_diagnostics(handle_, d.data()));
@@ -457,60 +478,134 @@ AdjointEvaluation likelihood_adjoint(const Dataset &data, const Vector &log_para
 }
 
 // -------------------------------------------------------------- Dataset --
+// Dataset::validate (src/inference.cpp:197-213): the grid and the forcing
+// shell, then the dataset's own rules.
 void Dataset::validate() const {
-    Parameters p;
-    p.forcing_min = forcing_min;
-    p.forcing_max = forcing_max;
     (void)Fourier2D(n);
-    p.validate(n);
-    if (n < 16 || !std::isfinite(dt) || dt <= 0 || observations.empty())
+    Parameters shell;
+    shell.forcing_min = forcing_min;
+    shell.forcing_max = forcing_max;
+    shell.validate(n);
+    if (n < 16 || !(std::isfinite(dt) && dt > 0) || observations.empty())
         throw std::invalid_argument("invalid observation dataset");
-    std::size_t previous = 0;
-    for (const auto &o : observations) {
-        if (o.step < previous || o.x >= n || o.y >= n || !std::isfinite(o.value) ||
-            !std::isfinite(o.standard_deviation) || o.standard_deviation <= 0)
-            throw std::invalid_argument("observations must be time ordered, finite, in bounds, and "
-                                        "have positive uncertainty");
-        previous = o.step;
-    }
-}
-
-void Dataset::save(const std::filesystem::path &path) const {
-    validate();
-    std::ofstream out(path);
-    out << std::setprecision(17);
-    out << "HZL_OBSERVATIONS_V2," << n << ',' << dt << ',' << stochastic << ',' << forcing_min
-        << ',' << forcing_max << "\nstep,x,y,vorticity,standard_deviation\n";
-    for (const auto &o : observations)
-        out << o.step << ',' << o.x << ',' << o.y << ',' << o.value << ',' << o.standard_deviation
-            << '\n';
-    if (!out)
-        throw std::runtime_error("observation write failed");
+    std::size_t last = 0;
+    const auto admissible = [&](const Observation &o) {
+        const bool ok = o.step >= last && o.x < n && o.y < n && std::isfinite(o.value) &&
+                        std::isfinite(o.standard_deviation) && o.standard_deviation > 0;
+        last = o.step;
+        return ok;
+    };
+    if (!std::all_of(observations.begin(), observations.end(), admissible))
+        throw std::invalid_argument("observations must be time ordered, finite, in bounds, and "
+                                    "have positive uncertainty");
 }
 
 namespace {
-// Retained strictly-inside-2/3 mask without the mean (src/solver.cpp:126-135).
-bool retained(std::size_t n, std::size_t i) {
-    const int a = wave_number(i % n, n), b = wave_number(i / n, n);
-    return 3 * std::abs(a) < (int)n && 3 * std::abs(b) < (int)n && i != 0;
+// HZL_OBSERVATIONS_V2 (src/inference.cpp:214-255): a header line
+// "magic,n,dt,stochastic,forcing_min,forcing_max" (V1: "magic,n,dt"), the
+// column line, then one observation per line; numbers with 17 digits.
+constexpr const char *kObservationsV1 = "HZL_OBSERVATIONS_V1";
+constexpr const char *kObservationsV2 = "HZL_OBSERVATIONS_V2";
+constexpr const char *kObservationColumns = "step,x,y,vorticity,standard_deviation";
+} // namespace
+
+void Dataset::save(const std::filesystem::path &path) const {
+    validate();
+    text::LineWriter out(path, ',', "observation write failed");
+    out.line(kObservationsV2, n, dt, stochastic, forcing_min, forcing_max);
+    out.line(kObservationColumns);
+    for (const Observation &o : observations)
+        out.line(o.step, o.x, o.y, o.value, o.standard_deviation);
+    out.close();
 }
-// Indices of the forcing shell pairs, both members (src/solver.cpp:67-83).
-std::vector<bool> shell_indices(std::size_t n, const Parameters &p) {
-    std::vector<bool> forced(n * n, false);
-    bool any = false;
-    for (std::size_t i = 0; i < n * n; ++i) {
-        const int a = wave_number(i % n, n), b = wave_number(i / n, n);
-        const int k2 = a * a + b * b;
-        if (retained(n, i) && k2 >= p.forcing_min * p.forcing_min &&
-            k2 <= p.forcing_max * p.forcing_max) {
-            forced[i] = true;
-            any = true;
-        }
+
+Dataset Dataset::load(const std::filesystem::path &path) {
+    const std::string text = text::Scanner::slurp(path);
+    const auto lines = text::lines_of(text);
+    if (lines.empty())
+        throw std::runtime_error("cannot read observation dataset");
+    Dataset data;
+    text::Scanner header(std::string(lines[0]), true);
+    std::string magic;
+    if (!header.read(magic) || !header.read(data.n) || !header.read(data.dt) ||
+        (magic != kObservationsV1 && magic != kObservationsV2))
+        throw std::runtime_error("invalid dataset header");
+    if (magic == kObservationsV2 && !(header.read(data.stochastic) &&
+                                      header.read(data.forcing_min) &&
+                                      header.read(data.forcing_max)))
+        throw std::runtime_error("invalid stochastic dataset metadata");
+    if (lines.size() < 2 || lines[1] != kObservationColumns)
+        throw std::runtime_error("invalid dataset columns");
+    for (std::size_t l = 2; l < lines.size(); ++l) {
+        if (lines[l].empty())
+            continue;
+        text::Scanner row(std::string(lines[l]), true);
+        Observation o;
+        if (!(row.read(o.step) && row.read(o.x) && row.read(o.y) && row.read(o.value) &&
+              row.read(o.standard_deviation)) ||
+            !row.at_end())
+            throw std::runtime_error("invalid observation row");
+        data.observations.push_back(o);
     }
-    if (!any)
-        throw std::invalid_argument("empty forcing shell");
-    return forced;
+    data.validate();
+    return data;
 }
+
+// Observations of a truth run (synthetic_observations, src/inference.cpp:256-288):
+// one GPU member started from "multimode" with the reference's OU forcing on
+// stream stream_seed(seed, 0); every `every` steps its physical field at the
+// eight sensors x_j = (j n / 8 + 1) mod n, y_j = (j^2 + 3 j + 2) mod n, plus
+// noise from NormalStream(stream_seed(seed, 1)) drawn in sensor order.
+Dataset synthetic_observations(std::size_t n, double dt, std::size_t steps, std::size_t every,
+                               Parameters truth, double noise, std::uint64_t seed,
+                               std::vector<Field> *truth_fields) {
+    if (every == 0 || steps < every || !std::isfinite(noise) || noise <= 0)
+        throw std::invalid_argument(
+            "observation experiment requires steps >= every and positive noise");
+    (void)Fourier2D(n);
+    truth.validate(n);
+    constexpr std::size_t kSensors = 8;
+    const std::size_t times = steps / every;
+    std::vector<double> draws(kSensors * times);
+    check(hzl_normal_draws(stream_seed(seed, 1), draws.size(), draws.data()));
+
+    // member 0 of an ensemble created with `seed` runs the OU forcing of
+    // Solver(n, truth, stream_seed(seed, 0))
+    Ensemble run(n, 1, truth, Precision::fp64, seed);
+    run.set_all(initial_condition(n, "multimode"));
+    run.set_forcing_process(ForcingProcess::ornstein_uhlenbeck);
+
+    Dataset data;
+    data.n = n;
+    data.dt = dt;
+    data.stochastic = truth.forcing_sigma > 0;
+    data.forcing_min = truth.forcing_min;
+    data.forcing_max = truth.forcing_max;
+    if (truth_fields)
+        truth_fields->clear();
+    for (std::size_t t = 0; t < times; ++t) {
+        run.step(dt, every);
+        run.synchronize();
+        const Field field = physical(n, run.state(0));
+        for (std::size_t j = 0; j < kSensors; ++j) {
+            const std::size_t x = (j * n / kSensors + 1) % n, y = (j * j + 3 * j + 2) % n;
+            data.observations.push_back({(t + 1) * every, x, y,
+                                         field[y * n + x].real() + noise * draws[kSensors * t + j],
+                                         noise});
+        }
+        if (truth_fields)
+            truth_fields->push_back(field);
+    }
+    data.validate();
+    return data;
+}
+
+// ----------------------------------------------------------- Checkpoint --
+// HZL_CHECKPOINT_V1 (src/solver.cpp:434-491), whitespace separated:
+//   magic / n time / nu alpha sigma tau kmin kmax / 312 engine words and the
+//   position / has_spare spare / per index y n + x: Re w, Im w, Re F, Im F.
+namespace {
+constexpr const char *kCheckpointMagic = "HZL_CHECKPOINT_V1";
 constexpr std::size_t kEngineWords = 312;
 } // namespace
 
@@ -518,99 +613,76 @@ void Checkpoint::save(const std::filesystem::path &path) const {
     if (vorticity.size() != n * n || forcing.size() != n * n ||
         random_engine.size() != kEngineWords + 1)
         throw std::invalid_argument("checkpoint fields do not match the grid");
-    std::ofstream out(path);
-    if (!out)
-        throw std::runtime_error("cannot open checkpoint for writing");
-    out << std::setprecision(17);
-    const auto &p = parameters;
-    out << "HZL_CHECKPOINT_V1\n"
-        << n << ' ' << time << '\n'
-        << p.viscosity << ' ' << p.friction << ' ' << p.forcing_sigma << ' ' << p.forcing_tau
-        << ' ' << p.forcing_min << ' ' << p.forcing_max << '\n';
-    for (std::size_t k = 0; k < kEngineWords; ++k)
-        out << random_engine[k] << ' ';
-    out << random_engine[kEngineWords] << '\n' << (random_has_spare ? 1 : 0) << ' ' << random_spare
-        << '\n';
+    text::LineWriter out(path, ' ', "checkpoint write failed", "cannot open checkpoint for writing");
+    const Parameters &p = parameters;
+    out.line(kCheckpointMagic);
+    out.line(n, time);
+    out.line(p.viscosity, p.friction, p.forcing_sigma, p.forcing_tau, p.forcing_min,
+             p.forcing_max);
+    for (std::uint64_t word : random_engine)
+        out.add(word);
+    out.end();
+    out.line(random_has_spare, random_spare);
     for (std::size_t i = 0; i < n * n; ++i)
-        out << vorticity[i].real() << ' ' << vorticity[i].imag() << ' ' << forcing[i].real() << ' '
-            << forcing[i].imag() << '\n';
-    if (!out)
-        throw std::runtime_error("checkpoint write failed");
+        out.line(vorticity[i].real(), vorticity[i].imag(), forcing[i].real(), forcing[i].imag());
+    out.close();
 }
 
 Checkpoint Checkpoint::load(const std::filesystem::path &path) {
-    std::ifstream in(path);
-    std::string magic;
+    text::Scanner in(text::Scanner::slurp(path), false);
     Checkpoint c;
     Parameters &p = c.parameters;
-    if (!(in >> magic >> c.n >> c.time >> p.viscosity >> p.friction >> p.forcing_sigma >>
-          p.forcing_tau >> p.forcing_min >> p.forcing_max) ||
-        magic != "HZL_CHECKPOINT_V1" || !std::isfinite(c.time) || c.time < 0)
+    std::string magic;
+    const bool header = in.read(magic) && in.read(c.n) && in.read(c.time) &&
+                        in.read(p.viscosity) && in.read(p.friction) &&
+                        in.read(p.forcing_sigma) && in.read(p.forcing_tau) &&
+                        in.read(p.forcing_min) && in.read(p.forcing_max);
+    if (!header || magic != kCheckpointMagic || !std::isfinite(c.time) || c.time < 0)
         throw std::runtime_error("invalid checkpoint header");
-    Fourier2D grid_check(c.n); // power-of-two grid, as the reference's Solver(n, p, 0)
+    // the reference rebuilds Solver(n, p, 0): grid, parameters, non-empty shell
+    (void)Fourier2D(c.n);
     p.validate(c.n);
-    const auto forced = shell_indices(c.n, p);
+    const Grid grid{c.n};
+    const std::size_t nn = c.n * c.n;
+    std::vector<bool> shell(nn);
+    for (std::size_t i = 0; i < nn; ++i)
+        shell[i] = grid.in_shell(i, p);
+    if (std::none_of(shell.begin(), shell.end(), [](bool b) { return b; }))
+        throw std::invalid_argument("empty forcing shell");
+
     c.random_engine.resize(kEngineWords + 1);
-    for (auto &w : c.random_engine)
-        if (!(in >> w))
-            throw std::runtime_error("invalid checkpoint random stream");
-    if (c.random_engine[kEngineWords] > kEngineWords ||
-        !(in >> c.random_has_spare >> c.random_spare) || !std::isfinite(c.random_spare))
+    bool stream = std::all_of(c.random_engine.begin(), c.random_engine.end(),
+                              [&](std::uint64_t &w) { return in.read(w); });
+    stream = stream && c.random_engine[kEngineWords] <= kEngineWords &&
+             in.read(c.random_has_spare) && in.read(c.random_spare) &&
+             std::isfinite(c.random_spare);
+    if (!stream)
         throw std::runtime_error("invalid checkpoint random stream");
-    const std::size_t n = c.n;
-    c.vorticity.assign(n * n, Complex{});
-    c.forcing.assign(n * n, Complex{});
-    for (std::size_t i = 0; i < n * n; ++i) {
-        double a = 0, b = 0, f = 0, g = 0;
-        if (!(in >> a >> b >> f >> g) || !std::isfinite(a) || !std::isfinite(b) ||
-            !std::isfinite(f) || !std::isfinite(g))
-            throw std::runtime_error("invalid checkpoint field");
-        c.vorticity[i] = {a, b};
-        c.forcing[i] = {f, g};
+
+    c.vorticity.assign(nn, Complex{});
+    c.forcing.assign(nn, Complex{});
+    for (std::size_t i = 0; i < nn; ++i) {
+        double v[4];
+        for (double &x : v)
+            if (!in.read(x) || !std::isfinite(x))
+                throw std::runtime_error("invalid checkpoint field");
+        c.vorticity[i] = {v[0], v[1]};
+        c.forcing[i] = {v[2], v[3]};
     }
-    for (std::size_t i = 0; i < n * n; ++i) {
-        const std::size_t y = i / n, x = i % n, partner = ((n - y) % n) * n + (n - x) % n;
+    // masked vorticity, forcing only on the shell, conjugate symmetry
+    // (vorticity to 1e-10 relative, forcing exactly)
+    for (std::size_t i = 0; i < nn; ++i) {
+        const std::size_t j = grid.partner(i);
         const Complex w = c.vorticity[i], F = c.forcing[i];
-        if ((!retained(n, i) && w != Complex{}) || (!forced[i] && F != Complex{}) ||
-            std::abs(w - std::conj(c.vorticity[partner])) > 1e-10 * (1 + std::abs(w)) ||
-            F != std::conj(c.forcing[partner]))
+        const bool fits = (grid.retained(i) || w == Complex{}) && (shell[i] || F == Complex{}) &&
+                          std::abs(w - std::conj(c.vorticity[j])) <= 1e-10 * (1 + std::abs(w)) &&
+                          F == std::conj(c.forcing[j]);
+        if (!fits)
             throw std::runtime_error("checkpoint violates spectral constraints");
     }
-    std::string trailing;
-    if (in >> trailing)
+    if (!in.at_end())
         throw std::runtime_error("unexpected checkpoint data");
     return c;
-}
-
-Dataset Dataset::load(const std::filesystem::path &path) {
-    std::ifstream in(path);
-    std::string line, magic;
-    Dataset data;
-    if (!std::getline(in, line))
-        throw std::runtime_error("cannot read observation dataset");
-    std::replace(line.begin(), line.end(), ',', ' ');
-    std::istringstream header(line);
-    if (!(header >> magic >> data.n >> data.dt) ||
-        (magic != "HZL_OBSERVATIONS_V1" && magic != "HZL_OBSERVATIONS_V2"))
-        throw std::runtime_error("invalid dataset header");
-    if (magic == "HZL_OBSERVATIONS_V2" &&
-        !(header >> data.stochastic >> data.forcing_min >> data.forcing_max))
-        throw std::runtime_error("invalid stochastic dataset metadata");
-    if (!std::getline(in, line) || line != "step,x,y,vorticity,standard_deviation")
-        throw std::runtime_error("invalid dataset columns");
-    while (std::getline(in, line)) {
-        if (line.empty())
-            continue;
-        std::replace(line.begin(), line.end(), ',', ' ');
-        std::istringstream row(line);
-        Observation o;
-        std::string extra;
-        if (!(row >> o.step >> o.x >> o.y >> o.value >> o.standard_deviation) || (row >> extra))
-            throw std::runtime_error("invalid observation row");
-        data.observations.push_back(o);
-    }
-    data.validate();
-    return data;
 }
 
 // -------------------------------------------------------------- filters --
@@ -660,13 +732,15 @@ FilterResult bootstrap_filter(Ensemble &particles, const Dataset &data, std::uin
         check(hzl_pf_assimilate(particles.handle(), data.dt, gr.step - current, gr.x.size(),
                                 gr.x.data(), gr.y.data(), gr.value.data(), gr.sd.data(),
                                 g + 1 == groups.size(), rec, retain_states ? raw(mean) : nullptr));
+        // every weight -inf: the reference returns before recording the group
+        // (src/filter.cpp:76-80)
+        if (!std::isfinite(rec[0]))
+            break;
         FilterRecord r{gr.step, rec[0], rec[1], rec[2], {}};
         if (retain_states)
             r.mean_vorticity = physical(n, std::move(mean));
         result.history.push_back(std::move(r));
         current = gr.step;
-        if (!std::isfinite(rec[0]))
-            break;
     }
     check(hzl_pf_result(particles.handle(), &result.log_likelihood, &result.resampling_events));
     return result;
@@ -687,131 +761,136 @@ FilterResult bootstrap_filter(const Dataset &data, Parameters p, std::size_t par
     return bootstrap_filter(e, data, seed, retain_states);
 }
 
+// ------------------------------------------------- particle likelihoods --
+// The particle-marginal family over q = (log nu, log alpha, log sigma,
+// log tau) (src/filter.cpp:127-203) runs on the engine's batched drivers: the
+// prior and the sampler's streams live behind the C-ABI, every likelihood is a
+// GPU bootstrap filter with the OU forcing (FP64), and replicates or chain
+// proposals are segments of one ensemble.
 namespace {
 
-constexpr double negative_infinity = -std::numeric_limits<double>::infinity();
-
-// NormalStream (src/solver.cpp:41-54): mt19937_64, open-interval uniforms,
-// Box-Muller returning r cos and caching r sin.
-class NormalStream {
-  public:
-    explicit NormalStream(std::uint64_t seed) : engine_(seed) {}
-    double operator()() {
-        if (has_spare_) {
-            has_spare_ = false;
-            return spare_;
+// Observation rows as the batched drivers take them.
+struct Rows {
+    std::vector<int> step, x, y;
+    std::vector<double> value, sd;
+    explicit Rows(const Dataset &data) {
+        for (const Observation &o : data.observations) {
+            step.push_back((int)o.step);
+            x.push_back((int)o.x);
+            y.push_back((int)o.y);
+            value.push_back(o.value);
+            sd.push_back(o.standard_deviation);
         }
-        const auto open = [this] { return ((double)(engine_() >> 11) + 0.5) * 0x1.0p-53; };
-        const double r = std::sqrt(-2 * std::log(open())), angle = 2 * std::numbers::pi * open();
-        spare_ = r * std::sin(angle);
-        has_spare_ = true;
-        return r * std::cos(angle);
     }
-
-  private:
-    std::mt19937_64 engine_;
-    bool has_spare_ = false;
-    double spare_ = 0;
 };
 
-// decode (src/filter.cpp:15-24)
-Parameters decode(const Dataset &data, const Vector &q) {
+// The data's grid shell on default parameters: the batched drivers set every
+// member's nu, alpha, sigma and tau from its filter's q.
+Parameters shell_of(const Dataset &data) {
     Parameters p;
-    p.viscosity = std::exp(q[0]);
-    p.friction = std::exp(q[1]);
-    p.forcing_sigma = std::exp(q[2]);
-    p.forcing_tau = std::exp(q[3]);
     p.forcing_min = data.forcing_min;
     p.forcing_max = data.forcing_max;
     return p;
 }
 
+hzl_pmcmc_options stochastic_options() {
+    hzl_pmcmc_options o{};
+    o.prior = HZL_PRIOR_STOCHASTIC;
+    return o;
+}
+
+// particle_log_likelihood at one q for every seed (one filter per seed).
+Vector likelihoods(const Dataset &data, const Vector &q, std::size_t particles,
+                   const std::vector<std::uint64_t> &seeds) {
+    double prior = 0;
+    check(hzl_stochastic_log_prior(q.data(), q.size(), &prior));
+    if (!std::isfinite(prior))
+        return Vector(seeds.size(), kNegativeInfinity);
+    data.validate();
+    if (particles < 2)
+        throw std::invalid_argument("particle filter needs at least two particles");
+    Ensemble e(data.n, seeds.size() * particles, shell_of(data), Precision::fp64, 0);
+    e.set_forcing_process(ForcingProcess::ornstein_uhlenbeck);
+    const Rows rows(data);
+    const Field start = initial_condition(data.n, "multimode");
+    const hzl_pmcmc_options o = stochastic_options();
+    Vector points, ll(seeds.size());
+    for (std::size_t f = 0; f < seeds.size(); ++f)
+        points.insert(points.end(), q.begin(), q.end());
+    check(hzl_pf_log_likelihoods(e.handle(), &o, data.dt, rows.step.size(), rows.step.data(),
+                                 rows.x.data(), rows.y.data(), rows.value.data(), rows.sd.data(),
+                                 raw(start), seeds.size(), points.data(), seeds.data(),
+                                 ll.data()));
+    return ll;
+}
+
 } // namespace
 
 double stochastic_log_prior(const Vector &q) {
-    if (q.size() != 4)
-        throw std::invalid_argument(
-            "stochastic inference expects log(nu), log(alpha), log(sigma), log(tau)");
-    const double centers[] = {std::log(0.03), std::log(0.1), std::log(0.2), std::log(0.5)};
-    double result = 0;
-    for (std::size_t i = 0; i < q.size(); ++i) {
-        if (!std::isfinite(q[i]) || q[i] < -10 || q[i] > (i < 2 ? 0 : 2))
-            return negative_infinity;
-        const double z = (q[i] - centers[i]) / 1.5;
-        result -= 0.5 * z * z;
-    }
-    return result;
+    double out = 0;
+    check(hzl_stochastic_log_prior(q.data(), q.size(), &out));
+    return out;
 }
 
 double particle_log_likelihood(const Dataset &data, const Vector &q, std::size_t particles,
                                std::uint64_t seed) {
-    if (!std::isfinite(stochastic_log_prior(q)))
-        return negative_infinity;
-    const Parameters p = decode(data, q);
-    const double largest = (double)((data.n - 1) / 3);
-    if (data.dt * (2 * p.viscosity * largest * largest + p.friction) > 2.5)
-        return negative_infinity;
-    return bootstrap_filter(data, p, particles, seed, false).log_likelihood;
+    return likelihoods(data, q, particles, {seed})[0];
 }
 
+// The replicates run as one batch of filters; their log-mean-exp is formed
+// around the largest estimate.
 double averaged_particle_log_likelihood(const Dataset &data, const Vector &q,
                                         std::size_t particles, std::size_t replicates,
                                         std::uint64_t seed) {
     if (replicates == 0)
         throw std::invalid_argument("averaged particle likelihood needs at least one replicate");
-    Vector values(replicates);
+    std::vector<std::uint64_t> seeds(replicates);
     for (std::size_t i = 0; i < replicates; ++i)
-        values[i] = particle_log_likelihood(data, q, particles, stream_seed(seed, i));
-    const double maximum = *std::max_element(values.begin(), values.end());
-    if (!std::isfinite(maximum))
-        return negative_infinity;
-    double total = 0;
-    for (double v : values)
-        total += std::exp(v - maximum);
-    return maximum + std::log(total / (double)replicates);
+        seeds[i] = stream_seed(seed, i);
+    const Vector ll = likelihoods(data, q, particles, seeds);
+    const double top = *std::max_element(ll.begin(), ll.end());
+    if (!std::isfinite(top))
+        return kNegativeInfinity;
+    double mean = 0;
+    for (double v : ll)
+        mean += std::exp(v - top);
+    return top + std::log(mean / (double)replicates);
 }
 
-// Pseudo-marginal random-walk Metropolis (src/filter.cpp:166-203): proposals
-// from NormalStream(seed), likelihood seeds and acceptance uniforms from
-// mt19937_64(stream_seed(seed, 1)); a rejected proposal keeps the current
-// likelihood estimate and its seed.  Each likelihood is one GPU filter.
-std::vector<ParticleSample> particle_metropolis(const Dataset &data, Vector current,
+// One chain of the batched sampler with the reference's own streams
+// (chain seed = seed) and prior.
+std::vector<ParticleSample> particle_metropolis(const Dataset &data, Vector initial,
                                                 double scale, std::size_t count,
                                                 std::size_t particles, std::uint64_t seed) {
     if (!data.stochastic)
         throw std::invalid_argument("particle sampler expects a stochastic observation dataset");
     if (count == 0 || !std::isfinite(scale) || scale <= 0)
         throw std::invalid_argument("invalid particle sampler configuration");
-    NormalStream normal(seed);
-    std::mt19937_64 random(stream_seed(seed, 1));
-    const auto uniform = [&random] { return (double)(random() >> 11) * 0x1.0p-53; };
-    auto likelihood_seed = random();
-    double likelihood = particle_log_likelihood(data, current, particles, likelihood_seed);
-    double density = likelihood + stochastic_log_prior(current);
-    if (!std::isfinite(density))
-        throw std::invalid_argument("initial particle posterior estimate is not finite");
-    std::vector<ParticleSample> chain;
-    chain.reserve(count);
-    for (std::size_t i = 0; i < count; ++i) {
-        auto proposal = current;
-        for (auto &q : proposal)
-            q += scale * normal();
-        const auto proposed_seed = random();
-        const double proposed_likelihood =
-            particle_log_likelihood(data, proposal, particles, proposed_seed);
-        const double proposed_density = proposed_likelihood + stochastic_log_prior(proposal);
-        const double draw = uniform();
-        const bool accepted = std::isfinite(proposed_density) &&
-                              std::log(std::max(draw, std::numeric_limits<double>::min())) <
-                                  std::min(0.0, proposed_density - density);
-        if (accepted) {
-            current = std::move(proposal);
-            likelihood = proposed_likelihood;
-            density = proposed_density;
-            likelihood_seed = proposed_seed;
-        }
-        chain.push_back({current, likelihood, density, likelihood_seed, accepted});
-    }
+    (void)stochastic_log_prior(initial); // four components
+    data.validate();
+    if (particles < 2)
+        throw std::invalid_argument("particle filter needs at least two particles");
+    Ensemble e(data.n, particles, shell_of(data), Precision::fp64, 0);
+    e.set_forcing_process(ForcingProcess::ornstein_uhlenbeck);
+    const Rows rows(data);
+    const Field start = initial_condition(data.n, "multimode");
+    hzl_pmcmc_options o = stochastic_options();
+    o.chains = 1;
+    o.samples = count;
+    o.proposal_scale = scale;
+    o.chain_seeds = &seed;
+    constexpr std::size_t D = 4;
+    Vector q(D * count), ll(count), density(count);
+    std::vector<int> accepted(count);
+    std::vector<std::uint64_t> seeds(count);
+    check(hzl_pmcmc_run(e.handle(), &o, data.dt, rows.step.size(), rows.step.data(),
+                        rows.x.data(), rows.y.data(), rows.value.data(), rows.sd.data(),
+                        raw(start), initial.data(), q.data(), ll.data(), density.data(),
+                        accepted.data(), seeds.data()));
+    std::vector<ParticleSample> chain(count);
+    for (std::size_t s = 0; s < count; ++s)
+        chain[s] = ParticleSample{Vector(q.begin() + (long)(D * s), q.begin() + (long)(D * s + D)),
+                                  ll[s], density[s], seeds[s], accepted[s] != 0};
     return chain;
 }
 

@@ -129,6 +129,8 @@ class Ensemble {
     [[nodiscard]] std::vector<Field> adjoint_drift(const std::vector<Field> &cotangents);
     void synchronize();
     [[nodiscard]] std::vector<Diagnostics> diagnostics();
+    // Solver::energy_spectrum (src/solver.cpp:426-433) of every member: n bins each
+    [[nodiscard]] std::vector<std::vector<double>> energy_spectra();
     // Shell forcing process: white in time (default) or the reference's OU
     // process (ShellForcing, src/solver.cpp:84-119); selecting OU draws every
     // member's stationary start from its stream.
@@ -196,6 +198,15 @@ struct Dataset {
     void save(const std::filesystem::path &path) const;
     static Dataset load(const std::filesystem::path &path);
 };
+
+// Observations of a truth run with the reference's OU forcing on the GPU
+// (synthetic_observations, src/inference.cpp:256-288): Solver(n, truth,
+// stream_seed(seed, 0)) from "multimode", eight fixed sensors every `every`
+// steps with noise from NormalStream(stream_seed(seed, 1)); truth_fields
+// (optional) receives the physical fields at those times.
+Dataset synthetic_observations(std::size_t n, double dt, std::size_t steps, std::size_t every,
+                               Parameters truth, double noise, std::uint64_t seed,
+                               std::vector<Field> *truth_fields = nullptr);
 
 struct FilterRecord {
     std::size_t step{};

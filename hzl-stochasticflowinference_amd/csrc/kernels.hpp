@@ -238,7 +238,8 @@ template <typename R> struct OuArgs {
     const int *map;        // [npairs] engine pair index, ~index when conjugated
     const double *sigma;   // per member
     const uint64_t *keys;  // per member seeds (modes 0 and 2)
-    double tau, h;         // OU time constant, transition interval (mode 1)
+    const double *tau;     // per member OU time constant
+    double h;              // transition interval (mode 1)
     int mode;              // 0: seed + stationary start, 1: transition, 2: reseed,
                            // 3: publish the current coefficients to noise (no draws)
     cx<R> *noise;          // modes 1, 3: forcing [B][npairs] (engine order) or null
@@ -289,8 +290,8 @@ HZL_HD void ou_body(const Ctx &ctx, const OuArgs<R> &a, uint64_t *s, int *shared
         const int p0 = (int)words[MT_N];
         if (p0 + 2 * a.npairs <= MT_N) {
             // inside the current block: draw straight from the state words
-            const double sg = a.sigma[m], decay = exp(-a.h / a.tau),
-                         scale = sg * sqrt(-a.tau * expm1(-2 * a.h / a.tau) / 4);
+            const double sg = a.sigma[m], tau = a.tau[m], decay = exp(-a.h / tau),
+                         scale = sg * sqrt(-tau * expm1(-2 * a.h / tau) / 4);
             for (int j = lane; j < a.npairs; j += OU_LANES) {
                 const double u1 = mt_open_uniform(mt_temper(words[p0 + 2 * j]));
                 const double u2 = mt_open_uniform(mt_temper(words[p0 + 2 * j + 1]));
@@ -318,13 +319,13 @@ HZL_HD void ou_body(const Ctx &ctx, const OuArgs<R> &a, uint64_t *s, int *shared
     ctx.sync();
     int pos = *shared_pos;
     if (a.mode != 2) {
-        const double sg = a.sigma[m];
+        const double sg = a.sigma[m], tau = a.tau[m];
         double decay = 0, scale;
         if (a.mode == 0) {
-            scale = sg * sqrt(a.tau / 4);
+            scale = sg * sqrt(tau / 4);
         } else {
-            decay = exp(-a.h / a.tau);
-            scale = sg * sqrt(-a.tau * expm1(-2 * a.h / a.tau) / 4);
+            decay = exp(-a.h / tau);
+            scale = sg * sqrt(-tau * expm1(-2 * a.h / tau) / 4);
         }
         int done = 0;
         while (done < a.npairs) {
@@ -1152,6 +1153,51 @@ template <typename R> struct DiagArgs {
     int n;
     double *out; // [B][6]: energy, enstrophy, dissipation, injection, cfl_rate, unused
 };
+
+// floor(sqrt(x)) and ceil(sqrt(x)) of a nonnegative integer, exact.
+HZL_HD long isqrt_floor(long x) {
+    long r = (long)sqrt((double)x);
+    while (r * r > x)
+        --r;
+    while ((r + 1) * (r + 1) <= x)
+        ++r;
+    return r;
+}
+HZL_HD long isqrt_ceil(long x) {
+    const long r = isqrt_floor(x);
+    return r * r == x ? r : r + 1;
+}
+
+// Shell energy spectrum of one member (Solver::energy_spectrum,
+// src/solver.cpp:426-433): bin s = floor(|k| + 1/2) of n bins collects
+// 2 pi^2 |w_k|^2 / k^2 over the full plane, i.e. over the retained half plane
+// with kx > 0 counted twice.  For integer |k|^2 the bin test is exact in
+// integers: k lies in bin s >= 1 iff s^2 - s + 1 <= |k|^2 <= s^2 + s.  Lane t
+// owns bins t, t + nt, ... and walks each annulus column by column in a fixed
+// order, so the sums are the same on every run.
+template <typename R, class Ctx>
+HZL_HD void spectrum_body(const Ctx &ctx, int K, int n, const cx<R> *w, double *out, int nt) {
+    const int J = 2 * K + 1;
+    const double two_pi2 = 2 * 3.14159265358979323846 * 3.14159265358979323846;
+    for (int s = ctx.tid(); s < n; s += nt) {
+        double acc = 0;
+        const long lo = (long)s * s - s + 1, hi = (long)s * s + s;
+        for (int c = 0; s > 0 && c <= K && (long)c * c <= hi; ++c) {
+            const long c2 = (long)c * c;
+            const long a = lo - c2 <= 0 ? 0 : isqrt_ceil(lo - c2);
+            const long b = isqrt_floor(hi - c2) < K ? isqrt_floor(hi - c2) : K;
+            const double mult = c > 0 ? 2.0 : 1.0;
+            for (long m = a; m <= b; ++m)
+                for (int sign = 1; sign >= (m == 0 ? 1 : -1); sign -= 2) {
+                    const long ky = sign * m, k2 = c2 + ky * ky;
+                    const cx<R> v = w[(long)c * J + ky + K];
+                    const double re = (double)v.x, im = (double)v.y;
+                    acc += mult * two_pi2 * (re * re + im * im) / (double)k2;
+                }
+        }
+        out[s] = acc;
+    }
+}
 
 // Energy, enstrophy, dissipation and injection in FP64 over the half plane
 // (kx > 0 entries count twice, the stored kx = 0 column once), reference

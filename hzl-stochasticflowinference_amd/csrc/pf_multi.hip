@@ -169,6 +169,9 @@ class MultiFilter {
         for (size_t o = 0; o < nobs; ++o)
             if (x[o] < 0 || y[o] < 0 || x[o] >= coarse_ || y[o] >= coarse_)
                 throw std::invalid_argument("observations must lie on the coarsest grid");
+        if (!is_last && group_ >= (uint64_t)max_uniforms_)
+            throw std::invalid_argument(
+                "more observation groups than the filter was started for (max_groups)");
         // forecasts: launched on every ensemble's stream before any waits
         if (steps)
             for (EngineBase *e : engs_)

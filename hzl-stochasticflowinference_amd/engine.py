@@ -40,6 +40,9 @@ EXPORTS = (
     "hzl_pf_begin", "hzl_pf_assimilate", "hzl_pf_result",
     "hzl_pf_begin_filters", "hzl_ensemble_fill_state", "hzl_ensemble_set_step_index",
     "hzl_enkf_begin", "hzl_enkf_assimilate", "hzl_ensemble_mean_spread", "hzl_pmcmc_run",
+    "hzl_parameters_validate", "hzl_stream_seed", "hzl_ensemble_set_member_tau",
+    "hzl_ensemble_energy_spectrum", "hzl_normal_draws", "hzl_stochastic_log_prior",
+    "hzl_pf_log_likelihoods",
     "hzl_pf_joint_create", "hzl_pf_joint_destroy", "hzl_pf_joint_begin",
     "hzl_pf_joint_assimilate", "hzl_pf_joint_result",
     "hzl_surrogate_create", "hzl_surrogate_destroy", "hzl_surrogate_param_count",
@@ -60,12 +63,16 @@ class Parameters(C.Structure):
                 ("kolmogorov_amplitude", C.c_double), ("kolmogorov_wavenumber", C.c_int)]
 
 
+PRIOR_LOG_UNIFORM, PRIOR_STOCHASTIC = 0, 1
+
+
 class PmcmcOptions(C.Structure):
     """hzl_pmcmc_options (include/hzl_amd.h)."""
     _fields_ = [("chains", C.c_size_t), ("samples", C.c_size_t),
                 ("proposal_scale", C.c_double), ("log_lower", C.c_double * 2),
                 ("log_upper", C.c_double * 2), ("friction", C.c_double),
-                ("forcing_sigma", C.c_double), ("seed", C.c_uint64)]
+                ("forcing_sigma", C.c_double), ("seed", C.c_uint64), ("prior", C.c_int),
+                ("chain_seeds", C.POINTER(C.c_uint64))]
 
 
 class HzlError(RuntimeError):
@@ -148,7 +155,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                           _dp, C.c_int, _dp]
     L.hzl_pf_joint_result.argtypes = [vp, _dp, C.POINTER(C.c_size_t)]
     L.hzl_pmcmc_run.argtypes = [vp, C.POINTER(PmcmcOptions), C.c_double, C.c_size_t, ip, ip, ip,
-                                _dp, _dp, _dp, _dp, _dp, _dp, _dp, ip]
+                                _dp, _dp, _dp, _dp, _dp, _dp, _dp, ip, u64p]
+    L.hzl_pf_log_likelihoods.argtypes = [vp, C.POINTER(PmcmcOptions), C.c_double, C.c_size_t,
+                                         ip, ip, ip, _dp, _dp, _dp, C.c_size_t, _dp, u64p, _dp]
+    L.hzl_parameters_validate.argtypes = [C.c_size_t, C.POINTER(Parameters)]
+    L.hzl_stream_seed.restype = C.c_uint64
+    L.hzl_stream_seed.argtypes = [C.c_uint64, C.c_uint64]
+    L.hzl_ensemble_set_member_tau.argtypes = [vp, C.c_size_t, C.c_double]
+    L.hzl_ensemble_energy_spectrum.argtypes = [vp, _dp]
+    L.hzl_normal_draws.argtypes = [C.c_uint64, C.c_size_t, _dp]
+    L.hzl_stochastic_log_prior.argtypes = [_dp, C.c_size_t, _dp]
     fp, sz = C.POINTER(C.c_float), C.c_size_t
     L.hzl_surrogate_create.argtypes = [C.POINTER(vp), sz, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_uint64]
@@ -382,6 +398,17 @@ class Ensemble:
         out = np.zeros((self.members, 6))
         _check(_lib.hzl_ensemble_diagnostics(self._h, out.ctypes.data_as(_dp)))
         return out
+
+    def energy_spectrum(self) -> np.ndarray:
+        """Shell energy spectrum of every member, (members, n) FP64
+        (Solver::energy_spectrum, src/solver.cpp:426-433)."""
+        out = np.zeros((self.members, self.n))
+        _check(_lib.hzl_ensemble_energy_spectrum(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def set_member_tau(self, member: int, tau: float) -> None:
+        """Per-member OU time constant."""
+        _check(_lib.hzl_ensemble_set_member_tau(self._h, member, tau))
 
     def profile_step(self, dt: float) -> np.ndarray:
         """Durations (ms) of the 8 launches of one step: row1, col1, ..., row4, col4."""
@@ -723,11 +750,11 @@ def bootstrap_filter(particles: Ensemble, observations, dt: float, seed: int,
         rec, mean = particles.pf_assimilate(dt, int(target - current), sel[:, 1], sel[:, 2],
                                             sel[:, 3], sel[:, 4], gi == len(groups) - 1,
                                             mean=retain_states)
+        if not np.isfinite(rec[0]):
+            break  # every weight -inf: no record for this group (src/filter.cpp:76-80)
         history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
                             max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean))
         current = int(target)
-        if not np.isfinite(rec[0]):
-            break
     ll, events = particles.pf_result()
     return dict(log_likelihood=ll, resampling_events=events, history=history)
 
@@ -761,6 +788,8 @@ def joint_bootstrap_filter(ensembles, observations, dt: float, seed: int):
                                              v.ctypes.data_as(_dp), sd.ctypes.data_as(_dp),
                                              int(gi == len(groups) - 1),
                                              rec.ctypes.data_as(_dp)))
+            if not np.isfinite(rec[0]):
+                break  # every weight -inf (src/filter.cpp:76-80)
             history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
                                 max_weight=rec[2], resampled=bool(rec[3])))
             current = int(target)
@@ -827,7 +856,7 @@ def particle_metropolis(particles: Ensemble, observations, dt: float, initial_fi
     o = PmcmcOptions(chains, samples, proposal_scale,
                      (C.c_double * 2)(np.log(nu_bounds[0]), np.log(amp_bounds[0])),
                      (C.c_double * 2)(np.log(nu_bounds[1]), np.log(amp_bounds[1])),
-                     friction, forcing_sigma, seed)
+                     friction, forcing_sigma, seed, PRIOR_LOG_UNIFORM, None)
     step = np.ascontiguousarray(obs[:, 0], dtype=np.int32)
     x = np.ascontiguousarray(obs[:, 1], dtype=np.int32)
     y = np.ascontiguousarray(obs[:, 2], dtype=np.int32)
@@ -844,7 +873,7 @@ def particle_metropolis(particles: Ensemble, observations, dt: float, initial_fi
                               val.ctypes.data_as(_dp), sd.ctypes.data_as(_dp),
                               f.ctypes.data_as(_dp), q0.ctypes.data_as(_dp),
                               q.ctypes.data_as(_dp), ll.ctypes.data_as(_dp),
-                              dens.ctypes.data_as(_dp), acc.ctypes.data_as(ip)))
+                              dens.ctypes.data_as(_dp), acc.ctypes.data_as(ip), None))
     particles._filters = chains
     return dict(q=q, log_likelihood=ll, log_density=dens, accepted=acc.astype(bool))
 
@@ -1021,11 +1050,11 @@ def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt
                                                          fallback_threshold, precision)
         rec, mean = particles.pf_assimilate(dt, 0, sel[:, 1], sel[:, 2], sel[:, 3], sel[:, 4],
                                             gi == len(groups) - 1, mean=retain_states)
+        if not np.isfinite(rec[0]):
+            break  # every weight -inf (src/filter.cpp:76-80)
         history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
                             max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean,
                             solver=solver, spread=spread))
         current = int(target)
-        if not np.isfinite(rec[0]):
-            break
     ll, events = particles.pf_result()
     return dict(log_likelihood=ll, resampling_events=events, history=history)

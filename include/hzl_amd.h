@@ -8,7 +8,7 @@
  *   hzl::ensemble    (include/hzl/inference.hpp:84-85, src/inference.cpp:555-590)
  * Each entry point below names the reference member it replaces.  A C++ layer
  * that restores those exact class signatures on top of this ABI lives in
- * hzl-stochasticflowinference_amd/csrc/host/cuda_solver_dropin.cpp.
+ * hzl-stochasticflowinference_amd/csrc/host/hzl_gpu.{hpp,cpp} (libhzl_gpu.so).
  *
  * Conventions
  *   - every function returns HZL_OK (0) or an error code; the message of the
@@ -53,11 +53,13 @@ typedef struct hzl_surrogate *hzl_surrogate_t;
 typedef struct hzl_pf_joint *hzl_pf_joint_t;
 
 /* Model parameters.  The first six fields mirror hzl::Parameters
- * (include/hzl/solver.hpp:15-23).  The engine's stochastic forcing is white in
- * time on the shell forcing_min <= |k| <= forcing_max with per-component
- * increment variance forcing_sigma^2 dt / 2 (forcing_tau is accepted and
- * ignored), plus a deterministic Kolmogorov term
- * kolmogorov_amplitude * sin(kolmogorov_wavenumber * y). */
+ * (include/hzl/solver.hpp:15-23).  The shell forcing acts on
+ * forcing_min <= |k| <= forcing_max.  Under the default white-in-time process
+ * its per-component increment variance is forcing_sigma^2 dt / 2 and
+ * forcing_tau is not used; under the OU process (hzl_ensemble_set_forcing_process)
+ * forcing_tau is every member's time constant until hzl_ensemble_set_member_tau
+ * changes it.  A deterministic Kolmogorov term
+ * kolmogorov_amplitude * sin(kolmogorov_wavenumber * y) is added. */
 typedef struct hzl_parameters {
     double viscosity;
     double friction;
@@ -71,6 +73,17 @@ typedef struct hzl_parameters {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *hzl_last_error(void);
+
+/* Check the model fields of params (the first six) for an n x n grid, with
+ * the reference's rules and messages (HZL_EINVAL): nu, alpha, sigma finite and
+ * nonnegative, tau finite and positive, 1 <= forcing_min <= forcing_max and
+ * 3 forcing_max < n.  The grid itself is not checked.
+ * Replaces: Parameters::validate (src/solver.cpp:31-40). */
+int hzl_parameters_validate(size_t n, const hzl_parameters *params);
+
+/* SplitMix64 mixing of (base, index + 1): the seed of stream `index` derived
+ * from `base`.  Replaces: stream_seed (src/solver.cpp:61-66). */
+uint64_t hzl_stream_seed(uint64_t base, uint64_t index);
 
 /* Ensemble of `members` independent members on an n x n grid, n a power of
  * two in [16, 512].  Member m's random streams are keyed by
@@ -88,6 +101,13 @@ int hzl_ensemble_destroy(hzl_ensemble_t h);
 int hzl_ensemble_set_member_parameters(hzl_ensemble_t h, size_t member, double viscosity,
                                        double friction, double forcing_sigma,
                                        double kolmogorov_amplitude);
+
+/* Per-member OU time constant (> 0, finite); used by the OU forcing process
+ * from the next transition on (and by the stationary start that
+ * hzl_pf_begin_filters draws).  Lets every member carry its own
+ * hzl::Parameters, as the reference's per-proposal Solvers do
+ * (src/filter.cpp:15-24,141-149). */
+int hzl_ensemble_set_member_tau(hzl_ensemble_t h, size_t member, double tau);
 
 /* Upload one member's spectral state (reference Field layout); non-retained
  * modes are dropped as the CudaSolver constructor masks its upload
@@ -129,8 +149,9 @@ int hzl_ensemble_step_integrator(hzl_ensemble_t h, double dt, size_t steps, int 
  * hzl_pf_begin_filters resets the seeds), an exact half-step transition before
  * and after every step with the midpoint held through the stages, and
  * particle-filter children that keep their ancestor's coefficients and restart
- * their stream from stream_seed(seed, (g+1) P + i).  tau is the creation
- * parameters' forcing_tau; sigma is per member.  Replaces: ShellForcing /
+ * their stream from stream_seed(seed, (g+1) P + i).  sigma and tau are per
+ * member (hzl_ensemble_set_member_parameters, hzl_ensemble_set_member_tau;
+ * the creation parameters' values until set).  Replaces: ShellForcing /
  * initialize_ou / transition_ou / Solver::reseed_future (src/solver.cpp:84-119,
  * 174-176,217-254; include/hzl/solver.hpp:73-75). */
 int hzl_ensemble_set_forcing_process(hzl_ensemble_t h, int process);
@@ -266,6 +287,12 @@ int hzl_ensemble_synchronize(hzl_ensemble_t h);
  * Replaces: Solver::diagnostics (src/solver.cpp:381-397). */
 int hzl_ensemble_diagnostics(hzl_ensemble_t h, double *out);
 
+/* Shell energy spectrum of every member's current state: out[m * n + s]
+ * receives bin s of member m, the sum of 2 pi^2 |w_k|^2 / |k|^2 over the modes
+ * with floor(|k| + 1/2) = s (FP64 on the device, fixed summation order).
+ * Replaces: Solver::energy_spectrum (src/solver.cpp:426-433), batched. */
+int hzl_ensemble_energy_spectrum(hzl_ensemble_t h, double *out);
+
 /* One engine-forcing RK4 step (as hzl_ensemble_step with steps = 1) with HIP
  * events around each launch; kernel_ms[8] receives the durations of
  * row(stage 1), col(stage 1), ..., row(stage 4), col(stage 4) in milliseconds.
@@ -281,8 +308,9 @@ int hzl_ensemble_profile_step(hzl_ensemble_t h, double dt, float *kernel_ms);
  * Systematic resampling uses one uniform per event from
  * mt19937_64(stream_seed(seed, UINT64_MAX)) (src/filter.cpp:54,102-110). */
 
-/* Start a filter: weights 1/N, log-likelihood 0; at most max_groups
- * resampling events are supported. */
+/* Start a filter: weights 1/N, log-likelihood 0, for at most max_groups
+ * observation groups that may resample (every group but the last):
+ * hzl_pf_assimilate of a further non-final group fails with HZL_EINVAL. */
 int hzl_pf_begin(hzl_ensemble_t h, uint64_t seed, size_t max_groups);
 
 /* Advance every particle `steps` RK4 steps (engine forcing), then assimilate
@@ -368,31 +396,75 @@ int hzl_ensemble_mean_spread(hzl_ensemble_t h, double *spread, double *mean_fiel
 
 /* ------------------------------------------------------------------------
  * Batched particle-marginal Metropolis-Hastings, reference
- * particle_metropolis (src/filter.cpp:166-203), one chain per filter segment.
+ * particle_metropolis (src/filter.cpp:166-203), one chain per filter segment,
+ * and batched particle likelihoods (particle_log_likelihood,
+ * src/filter.cpp:141-149).
  * ------------------------------------------------------------------------ */
+/* Parameter vectors and their priors. */
+#define HZL_PRIOR_LOG_UNIFORM 0 /* q = (log nu, log Kolmogorov amplitude), uniform in
+                                   [log_lower, log_upper]; alpha and sigma fixed */
+#define HZL_PRIOR_STOCHASTIC 1  /* q = (log nu, log alpha, log sigma, log tau) with the
+                                   reference's stochastic_log_prior
+                                   (src/filter.cpp:127-140); no Kolmogorov term */
+
 typedef struct hzl_pmcmc_options {
     size_t chains;         /* members / chains particles per chain's filter */
     size_t samples;        /* Metropolis steps per chain */
-    double proposal_scale; /* random-walk step on (log nu, log amplitude) */
-    double log_lower[2];   /* log-uniform prior bounds of nu and amplitude */
+    double proposal_scale; /* random-walk step on every component of q */
+    double log_lower[2];   /* HZL_PRIOR_LOG_UNIFORM: bounds of log nu, log amplitude */
     double log_upper[2];
-    double friction;       /* fixed alpha and forcing sigma of every member */
+    double friction;       /* HZL_PRIOR_LOG_UNIFORM: fixed alpha and forcing sigma */
     double forcing_sigma;
-    uint64_t seed;         /* chain c uses stream_seed(seed, c) */
+    uint64_t seed;         /* chain c's stream seed is stream_seed(seed, c) ... */
+    int prior;             /* HZL_PRIOR_* */
+    const uint64_t *chain_seeds; /* ... or chain_seeds[c] when not null */
 } hzl_pmcmc_options;
 
-/* Run every chain `samples` steps over one window of observations (rows with
- * step >= 1, time ordered), every likelihood evaluation restarting all
- * particles from initial_field at step 0.  initial_q[2 x chains] holds each
- * chain's starting (log nu, log amplitude).  Outputs per chain c and sample s
- * at index c * samples + s: samples_q[2 x ...] the chain state after the step,
+/* The first `count` draws of the reference's NormalStream(seed): mt19937_64,
+ * open-interval uniforms ((x >> 11) + 0.5) 2^-53, Box-Muller returning
+ * r cos and then the kept r sin.  Replaces: NormalStream (src/solver.cpp:41-54)
+ * where a caller consumes one stream in order (synthetic observations). */
+int hzl_normal_draws(uint64_t seed, size_t count, double *out);
+
+/* Log prior density of the reference's stochastic parameter vector
+ * (log nu, log alpha, log sigma, log tau): -inf outside its bounds.  HZL_EINVAL
+ * with the reference's message unless dims == 4.
+ * Replaces: stochastic_log_prior (src/filter.cpp:127-140). */
+int hzl_stochastic_log_prior(const double *q, size_t dims, double *out);
+
+/* Log-likelihood estimates of `filters` independent bootstrap filters over
+ * the ensemble's members (members / filters particles each): filter f runs
+ * at parameter vector q[f * dims ..] (dims 2 or 4 by options->prior; the
+ * options' chains, samples, proposal and seed fields are not used) with
+ * filter seed seeds[f], its particles restarting from initial_field at step 0
+ * on noise keys stream_seed(seeds[f], i).  -inf where q lies outside the
+ * prior or beyond the explicit-step limit dt (2 nu K^2 + alpha) <= 2.5.
+ * Observation rows must be time ordered from step 0.  With the OU forcing
+ * process selected this is the reference's particle_log_likelihood for every
+ * filter at once (src/filter.cpp:141-149; bootstrap_filter(.., false)). */
+int hzl_pf_log_likelihoods(hzl_ensemble_t particles, const hzl_pmcmc_options *options,
+                           double dt, size_t nobs, const int *step, const int *x, const int *y,
+                           const double *value, const double *sd, const double *initial_field,
+                           size_t filters, const double *q, const uint64_t *seeds,
+                           double *log_likelihood);
+
+/* Run every chain `samples` steps over one window of observations (rows time
+ * ordered from step 0), every likelihood evaluation restarting all
+ * particles from initial_field at step 0.  initial_q[dims x chains] holds
+ * each chain's starting point.  Outputs per chain c and sample s at index
+ * k = c * samples + s: samples_q[dims x ...] the chain state after the step,
  * samples_ll its retained likelihood estimate, samples_density (optional)
- * log-likelihood + log-prior, samples_accepted (optional) 0/1. */
+ * log-likelihood + log-prior, samples_accepted (optional) 0/1, samples_seed
+ * (optional) the filter seed of the retained estimate.  With one chain,
+ * chain_seeds = {seed}, HZL_PRIOR_STOCHASTIC and the OU forcing process, this
+ * is the reference's particle_metropolis(data, q, scale, samples, particles,
+ * seed) draw for draw.  HZL_EINVAL "initial particle posterior estimate is
+ * not finite" when a chain's starting density is not finite. */
 int hzl_pmcmc_run(hzl_ensemble_t particles, const hzl_pmcmc_options *options, double dt,
                   size_t nobs, const int *step, const int *x, const int *y, const double *value,
                   const double *sd, const double *initial_field, const double *initial_q,
                   double *samples_q, double *samples_ll, double *samples_density,
-                  int *samples_accepted);
+                  int *samples_accepted, uint64_t *samples_seed);
 
 /* Number of steps taken so far (the noise counter of the next step). */
 int hzl_ensemble_step_index(hzl_ensemble_t h, uint64_t *out);

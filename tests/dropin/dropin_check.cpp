@@ -15,6 +15,15 @@
 //                                   log-posterior, likelihood seed (bits), accepted
 //   ensemble_ou <out>               ensemble(16, p, 0.01, 20, 4, 2, 42) with sigma 0.2,
 //                                   tau 0.5 on 2..3: energy, enstrophy, injection each
+//   synthetic <sigma> <every> <steps> <noise> <out>
+//                                   synthetic_observations(16, 0.01, steps, every, p, noise,
+//                                   987) with nu 0.03, alpha 0.1, tau 0.5 on 2..3: rows
+//                                   (step, x, y, value, sd), then the truth fields
+//   spectrum <n> <out>              Ensemble (FP64) from multimode and taylor-green:
+//                                   energy_spectra of both (n bins each)
+//   likelihoods <csv> <out>         particle_log_likelihood at three seeds,
+//                                   averaged_particle_log_likelihood over them (three
+//                                   replicates) and over one, stochastic_log_prior
 //   errors                          invalid arguments raise std::invalid_argument
 //   adjoint <csv> <stride> <out>    likelihood_adjoint(data, q, stride) at three q:
 //                                   value, gradient, checkpoint fields,
@@ -250,6 +259,45 @@ int main(int argc, char **argv) {
                                        (double)r.peak_block_fields, (double)r.replayed_steps});
             }
             write(argv[4], out);
+        } else if (mode == "synthetic" && argc == 7) {
+            Parameters p;
+            p.viscosity = 0.03;
+            p.friction = 0.1;
+            p.forcing_sigma = std::stod(argv[2]);
+            p.forcing_tau = 0.5;
+            std::vector<Field> truth;
+            const Dataset d = synthetic_observations(16, 0.01, std::stoul(argv[4]),
+                                                     std::stoul(argv[3]), p, std::stod(argv[5]),
+                                                     987, &truth);
+            std::vector<double> out;
+            for (const auto &o : d.observations)
+                out.insert(out.end(), {(double)o.step, (double)o.x, (double)o.y, o.value,
+                                       o.standard_deviation});
+            for (const auto &f : truth)
+                append(out, f);
+            write(argv[6], out);
+        } else if (mode == "spectrum" && argc == 4) {
+            const std::size_t n = std::stoul(argv[2]);
+            Parameters p;
+            p.forcing_sigma = 0;
+            Ensemble e(n, 2, p, Precision::fp64, 7);
+            e.set_state(0, initial_condition(n, "multimode"));
+            e.set_state(1, initial_condition(n, "taylor-green"));
+            std::vector<double> out;
+            for (const auto &bins : e.energy_spectra())
+                out.insert(out.end(), bins.begin(), bins.end());
+            write(argv[3], out);
+        } else if (mode == "likelihoods" && argc == 4) {
+            const Dataset data = Dataset::load(argv[2]);
+            const Vector q{std::log(0.03), std::log(0.1), std::log(0.2), std::log(0.5)};
+            std::vector<double> out;
+            for (std::uint64_t i = 0; i < 3; ++i)
+                out.push_back(particle_log_likelihood(data, q, 8, stream_seed(555, i)));
+            out.push_back(averaged_particle_log_likelihood(data, q, 8, 3, 555));
+            out.push_back(averaged_particle_log_likelihood(data, q, 8, 1, 555));
+            out.push_back(stochastic_log_prior(q));
+            out.push_back(particle_log_likelihood(data, {0.5, -1, -1, -1}, 8, 1)); // outside
+            write(argv[3], out);
         } else if (mode == "errors") {
             Parameters p;
             const bool ok =

@@ -534,12 +534,13 @@ int calib(int kind, int B, double nu, double alpha, double sigma, double amp, in
 
 extern "C" {
 
-// OU forcing kernel body (kernels.hpp ou_body) over B members for `calls`
+// OU forcing kernel body (kernels.hpp ou_body) over B members (per-member sigma
+// and tau) for `calls`
 // calls: modes[c] (0 start, 1 transition over hs[c], 2 reseed from
 // keys + c * B); after each call the coefficients [B][npairs] and, for
 // transitions, the midpoint forcing in engine order go to F_hist / noise_hist.
 int emu_ou(int B, int npairs, const int *map, const uint64_t *keys, const double *sigma,
-           double tau, int calls, const int *modes, const double *hs, double *F_hist,
+           const double *tau, int calls, const int *modes, const double *hs, double *F_hist,
            double *noise_hist) {
     std::vector<uint64_t> mt((size_t)(MT_N + 1) * B);
     std::vector<cx<double>> F((size_t)B * npairs), noise((size_t)B * npairs);

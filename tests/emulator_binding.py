@@ -39,7 +39,7 @@ class Emulator:
                                       C.c_int, C.c_uint64, C.c_double, C.c_int, _dp, _dp, _dp]
         L.emu_ou.restype = C.c_int
         L.emu_ou.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint64), _dp,
-                             C.c_double, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp]
+                             _dp, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
 
@@ -79,7 +79,8 @@ class Emulator:
         return out
 
     def ou(self, npairs, pair_map, keys, sigma, tau, modes, hs):
-        """ou_body over members for a sequence of calls; keys is (calls, B)
+        """ou_body over members for a sequence of calls (sigma and tau: one
+        value per member, or one tau for all); keys is (calls, B)
         (mode 0 / 2 seeds per call).  Returns (F, noise), each
         (calls, B, npairs) complex."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
@@ -90,9 +91,10 @@ class Emulator:
         mo = np.ascontiguousarray(modes, dtype=np.int32)
         h = np.ascontiguousarray(hs, dtype=np.float64)
         sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        tu = np.ascontiguousarray(np.broadcast_to(np.asarray(tau, dtype=np.float64), (B,)))
         rc = self.lib.emu_ou(B, npairs, m.ctypes.data_as(C.POINTER(C.c_int)),
                              keys.ctypes.data_as(C.POINTER(C.c_uint64)), sg.ctypes.data_as(_dp),
-                             tau, calls, mo.ctypes.data_as(C.POINTER(C.c_int)),
+                             tu.ctypes.data_as(_dp), calls, mo.ctypes.data_as(C.POINTER(C.c_int)),
                              h.ctypes.data_as(_dp), F.ctypes.data_as(_dp),
                              noise.ctypes.data_as(_dp))
         assert rc == 0

@@ -17,6 +17,18 @@ import numpy as np
 UINT64_MAX = (1 << 64) - 1
 
 
+def full_from_half(h, n):
+    """Reference Field (n x n) from a retained half plane [kx][ky + K]."""
+    K = (n - 1) // 3
+    out = np.zeros((n, n), dtype=np.complex128)
+    for c in range(K + 1):
+        for j, ky in enumerate(range(-K, K + 1)):
+            out[ky % n, c] = h[c, j]
+            if c > 0:
+                out[(-ky) % n, n - c] = np.conj(h[c, j])
+    return out
+
+
 def physical(orc, field):
     return orc.fft2d(field, inverse=True).real
 
@@ -219,39 +231,70 @@ def oracle_enkf(orc, initial, obs, dt, ens_seed, filter_seed, nu, alpha=0.0, amp
     return history
 
 
+STOCHASTIC_PRIOR = ((np.log(0.03), -10.0, 0.0), (np.log(0.1), -10.0, 0.0),
+                    (np.log(0.2), -10.0, 2.0), (np.log(0.5), -10.0, 2.0))
+
+
+def stochastic_log_prior(q):
+    """Reference stochastic_log_prior (src/filter.cpp:127-140): Gaussian of
+    width 1.5 around (log 0.03, log 0.1, log 0.2, log 0.5), -inf outside
+    [-10, 0] (first two) and [-10, 2] (last two)."""
+    total = 0.0
+    for qi, (c, lo, hi) in zip(q, STOCHASTIC_PRIOR):
+        if not (np.isfinite(qi) and lo <= qi <= hi):
+            return -np.inf
+        total -= 0.5 * ((qi - c) / 1.5) ** 2
+    return total
+
+
 def oracle_pmcmc(orc, initial, obs, dt, q0, samples, scale, seed, lower, upper, particles,
-                 alpha=0.0, sigma=0.05, kf=4, kmin=1, kmax=4, forcing="white", tau=0.5):
-    """Reference particle_metropolis (src/filter.cpp:166-203) per chain with the
-    engine's forcing model and a log-uniform prior on (nu, amplitude): chain c
-    seeds NormalStream(stream_seed(seed, c)) for proposals and
-    mt19937_64(stream_seed(stream_seed(seed, c), 1)) for likelihood seeds and
-    acceptance uniforms."""
+                 alpha=0.0, sigma=0.05, kf=4, kmin=1, kmax=4, forcing="white", tau=0.5,
+                 prior="log_uniform", chain_seeds=None):
+    """Reference particle_metropolis (src/filter.cpp:166-203) per chain.
+
+    prior "log_uniform": q = (log nu, log amplitude), uniform in [lower, upper],
+    alpha / sigma / tau fixed, Kolmogorov amplitude exp(q1) (BASELINE's chains);
+    prior "stochastic": q = (log nu, log alpha, log sigma, log tau) with the
+    reference's stochastic_log_prior and no Kolmogorov term (the reference's own
+    sampler; use forcing="ou").  Chain c's stream seed s is chain_seeds[c] when
+    given, else stream_seed(seed, c): NormalStream(s) for proposals,
+    mt19937_64(stream_seed(s, 1)) for likelihood seeds and acceptance uniforms.
+    Rows per sample: (q, log-likelihood, log-density, accepted, likelihood seed)."""
     n = initial.shape[0]
     K = (n - 1) // 3
     init = np.stack([initial] * particles)
+    D = 4 if prior == "stochastic" else 2
 
     def log_prior(q):
+        if prior == "stochastic":
+            return stochastic_log_prior(q)
         ok = all(np.isfinite(q[i]) and lower[i] <= q[i] <= upper[i] for i in range(2))
         return 0.0 if ok else -np.inf
 
     def like(q, lseed):
-        if log_prior(q) == -np.inf or dt * (2 * np.exp(q[0]) * K * K + alpha) > 2.5:
+        a = np.exp(q[1]) if prior == "stochastic" else alpha
+        if log_prior(q) == -np.inf or dt * (2 * np.exp(q[0]) * K * K + a) > 2.5:
             return -np.inf
-        return oracle_bootstrap_filter(orc, init, obs, dt, lseed, np.exp(q[0]), alpha=alpha,
-                                       amp=np.exp(q[1]), kf=kf, sigma=sigma, kmin=kmin,
-                                       kmax=kmax, forcing=forcing, tau=tau)["log_likelihood"]
+        if prior == "stochastic":
+            kw = dict(alpha=a, amp=0.0, sigma=np.exp(q[2]), tau=np.exp(q[3]))
+        else:
+            kw = dict(alpha=alpha, amp=np.exp(q[1]), sigma=sigma, tau=tau)
+        return oracle_bootstrap_filter(orc, init, obs, dt, lseed, np.exp(q[0]), kf=kf, kmin=kmin,
+                                       kmax=kmax, forcing=forcing, **kw)["log_likelihood"]
 
     out = []
-    for c, qc in enumerate(np.asarray(q0, dtype=np.float64).reshape(-1, 2)):
-        s = orc.stream_seed(seed, c)
+    for c, qc in enumerate(np.asarray(q0, dtype=np.float64).reshape(-1, D)):
+        s = int(chain_seeds[c]) if chain_seeds is not None else orc.stream_seed(seed, c)
         normal, random = orc.mt(s), orc.mt(orc.stream_seed(s, 1))
         cur = qc.copy()
         lseed = orc.mt_next(random)
         ll = like(cur, lseed)
         dens = ll + log_prior(cur)
+        if not np.isfinite(dens):
+            raise ValueError("initial particle posterior estimate is not finite")
         rows = []
         for _ in range(samples):
-            prop = cur + scale * np.array([orc.normal(normal), orc.normal(normal)])
+            prop = cur + scale * np.array([orc.normal(normal) for _ in range(D)])
             pseed = orc.mt_next(random)
             pll = like(prop, pseed)
             pd = pll + log_prior(prop)
@@ -259,7 +302,7 @@ def oracle_pmcmc(orc, initial, obs, dt, q0, samples, scale, seed, lower, upper, 
             acc = bool(np.isfinite(pd) and np.log(max(u, np.finfo(float).tiny)) < min(0.0, pd - dens))
             if acc:
                 cur, ll, dens, lseed = prop, pll, pd, pseed
-            rows.append((cur.copy(), ll, dens, acc))
+            rows.append((cur.copy(), ll, dens, acc, lseed))
         out.append(rows)
     return out
 

@@ -335,3 +335,54 @@ def test_stochastic_resume_from_reference_checkpoint(tmp_path, pkg):
     assert [int(x) for x in words] == after["random_engine"]
     assert np.linalg.norm(w - after["vorticity"]) < 1e-12 * np.linalg.norm(after["vorticity"])
     assert np.linalg.norm(F - after["forcing"]) < 1e-13 * np.linalg.norm(after["forcing"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["stochastic", "deterministic"])
+def test_synthetic_observations_match_reference(tmp_path, case):
+    """synthetic_observations on the GPU (truth member with the reference's OU
+    forcing, noise from NormalStream(stream_seed(seed, 1))) against the rows the
+    reference's own synthetic_observations wrote: sensor positions and steps
+    exact, values to 1e-10."""
+    if case == "stochastic":  # golden ou_16: (16, 0.01, 60, 10, sigma 0.2, noise 0.05, 987)
+        g, _ = ou_golden()
+        args, rows = (0.2, 10, 60, 0.05), g["rows"]
+    else:  # golden pf_deterministic_16: (16, 0.01, 30, 10, sigma 0, noise 0.2, 987)
+        args = (0.0, 10, 30, 0.2)
+        rows = np.load(os.path.join(GOLDEN, "pf_deterministic_16.npz"))["rows"]
+    run("synthetic", *args, tmp_path / "s.bin")
+    raw = np.fromfile(tmp_path / "s.bin")
+    got = raw[:rows.size].reshape(rows.shape)
+    np.testing.assert_array_equal(got[:, [0, 1, 2, 4]], rows[:, [0, 1, 2, 4]])
+    np.testing.assert_allclose(got[:, 3], rows[:, 3], rtol=0, atol=1e-10)
+    truth = field(raw[rows.size:], 16, count=len(rows) // 8)
+    assert np.abs(truth.imag).max() < 1e-12  # physical fields of the truth run
+
+
+@pytest.mark.gpu
+def test_energy_spectra_match_oracle(tmp_path, orc):
+    n = 32
+    run("spectrum", n, tmp_path / "b.bin")
+    bins = np.fromfile(tmp_path / "b.bin").reshape(2, n)
+    for m, name in enumerate(("multimode", "taylor-green")):
+        run("initial", n, name, tmp_path / "w.bin")
+        want = orc.energy_spectrum(field(np.fromfile(tmp_path / "w.bin"), n))
+        np.testing.assert_allclose(bins[m], want, rtol=1e-13, atol=1e-17)
+    assert abs(bins[1][1] - np.pi ** 2) < 1e-12  # Taylor-Green: pi^2 in shell 1
+
+
+@pytest.mark.gpu
+def test_particle_likelihood_family_batches(tmp_path):
+    """particle_log_likelihood runs one GPU filter; the averaged likelihood runs
+    its replicates as one batch of filters and equals the log-mean-exp of the
+    single evaluations with the replicate seeds stream_seed(seed, i); one
+    replicate is the single evaluation itself (reference
+    tests/verification.cpp:198-202); a q outside the prior is -inf."""
+    run("likelihoods", OU_CSV, tmp_path / "l.bin")
+    v = np.fromfile(tmp_path / "l.bin")
+    single, avg3, avg1, prior, outside = v[:3], v[3], v[4], v[5], v[6]
+    assert np.all(np.isfinite(single))
+    top = single.max()
+    assert abs(avg3 - (top + np.log(np.mean(np.exp(single - top))))) < 1e-12 * abs(avg3)
+    assert avg1 == single[0]
+    assert prior == 0.0 and outside == -np.inf

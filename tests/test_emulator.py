@@ -101,10 +101,10 @@ def test_ou_body_matches_oracle(emu, orc, n, kmin, kmax, members, calls):
     keys[calls // 2] = [orc.stream_seed(7, m) for m in range(members)]
     hs = [0.0] + [0.004] * (calls - 1)
     sigma = np.array([0.05 * (m + 1) for m in range(members)])
-    tau = 0.37
+    tau = np.array([0.37 * (1 + 0.5 * m) for m in range(members)])  # per-member time constants
     F, noise = emu.ou(P, emap, keys, sigma, tau, modes, hs)
     for m in range(members):
-        proc = orc.ou_process(n, sigma[m], tau, kmin, kmax, int(keys0[m]))
+        proc = orc.ou_process(n, sigma[m], tau[m], kmin, kmax, int(keys0[m]))
         for c in range(calls):
             if c > 0:
                 if modes[c] == 2:

@@ -193,3 +193,22 @@ def test_fp32_accuracy_256_after_100_steps(pkg, orc):
     errs = [rel(got[m], want[m]) for m in range(members)]
     print("256^2 FP32 100-step relative errors:", errs)
     assert max(errs) <= 1e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_energy_spectrum_matches_oracle(pkg, orc, precision, tol):
+    """Device shell spectrum (Solver::energy_spectrum) of every member after a
+    few steps against the FP64 oracle's spectrum of the same downloaded state;
+    the bins sum to the member's energy."""
+    n, members = 64, 3
+    with pkg.Ensemble(n, members, precision, nu=1e-3, sigma=0.05, seed=5) as ens:
+        ens.init_grf()
+        ens.step(1e-3, 3)
+        ens.synchronize()
+        bins = ens.energy_spectrum()
+        states = ens.get_state()
+        diag = ens.diagnostics()
+    for m in range(members):
+        want = orc.energy_spectrum(states[m])
+        assert np.abs(bins[m] - want).max() <= tol * want.max()
+        assert abs(bins[m].sum() - diag[m, 0]) <= tol * diag[m, 0]

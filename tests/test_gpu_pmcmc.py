@@ -29,7 +29,7 @@ def test_batched_pmcmc_matches_oracle_chains(pkg, orc):
     accepted = 0
     for c in range(C):
         for s in range(S):
-            q, ll, dens, acc = want[c][s]
+            q, ll, dens, acc = want[c][s][:4]
             assert got["accepted"][c, s] == acc, (c, s)
             np.testing.assert_array_equal(got["q"][c, s], q)
             assert abs(got["log_likelihood"][c, s] - ll) <= 1e-9 * abs(ll)
@@ -58,7 +58,7 @@ def test_batched_pmcmc_with_ou_forcing_matches_oracle_chains(pkg, orc):
                                       forcing_sigma=0.5)
     for c in range(C):
         for s in range(S):
-            q, ll, dens, acc = want[c][s]
+            q, ll, dens, acc = want[c][s][:4]
             assert got["accepted"][c, s] == acc, (c, s)
             np.testing.assert_array_equal(got["q"][c, s], q)
             assert abs(got["log_likelihood"][c, s] - ll) <= 1e-9 * abs(ll)

@@ -514,3 +514,35 @@ def test_ou_enkf_matches_reference_golden(orc):
         np.testing.assert_allclose(rec, g[key + "_records"], rtol=0, atol=1e-13)
         means = np.array([orc.fft2d(h["mean_field"], True).real for h in got])
         np.testing.assert_allclose(means, g[key + "_means"], rtol=0, atol=1e-13)
+
+
+def test_energy_spectrum_closed_forms(orc):
+    """Solver::energy_spectrum (src/solver.cpp:426-433): Taylor-Green
+    2 cos x cos y puts its energy pi^2 (tests/verification.cpp:60-61) in shell
+    1; one conjugate pair at (3, 4) puts 2 pi^2 |c|^2 / 25 twice into shell 5;
+    a mode with |k| = 1.5 + eps rounds up (bin of (1, 1) is 1, of (2, 1) is 2);
+    the bins always sum to the energy."""
+    n = 16
+    tg = np.zeros((n, n), dtype=np.complex128)
+    for ky in (1, -1):
+        for kx in (1, -1):
+            tg[ky % n, kx % n] = 0.5
+    b = orc.energy_spectrum(tg)
+    assert abs(b[1] - np.pi ** 2) < 1e-13 and np.count_nonzero(b) == 1
+    one = np.zeros((n, n), dtype=np.complex128)
+    one[4, 3], one[-4 % n, -3 % n] = 0.3 + 0.4j, 0.3 - 0.4j
+    b = orc.energy_spectrum(one)
+    assert abs(b[5] - 2 * 2 * np.pi ** 2 * 0.25 / 25) < 1e-15 and np.count_nonzero(b) == 1
+    two = np.zeros((n, n), dtype=np.complex128)
+    two[1, 2], two[-1 % n, -2 % n] = 1.0, 1.0   # |k| = sqrt(5) = 2.236 -> bin 2
+    b = orc.energy_spectrum(two)
+    assert b[2] > 0 and np.count_nonzero(b) == 1
+    w = orc.grf(32, 3)
+    assert abs(orc.energy_spectrum(w).sum() - orc.diagnostics(w, 0.0)["energy"]) < 1e-13
+
+
+def test_energy_spectrum_matches_reference(orc, ref):
+    for n, name in ((16, "multimode"), (32, "taylor-green")):
+        w = ref.initialize(n, name)
+        np.testing.assert_allclose(orc.energy_spectrum(w), ref.energy_spectrum(w), rtol=1e-13,
+                                   atol=1e-16)
