@@ -84,6 +84,12 @@ template <typename R> __global__ void __launch_bounds__(256) k_grf(GrfArgs<R> a)
     grf_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
 }
 
+__global__ void __launch_bounds__(256) k_fill_f64(long count, double value, double *y) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count)
+        y[i] = value;
+}
+
 template <typename R> __global__ void __launch_bounds__(256) k_diag(DiagArgs<R> a) {
     __shared__ double red[3 * 256];
     diag_body<R>(DevCtx{}, a, red, 256, (long)blockIdx.x);
@@ -497,6 +503,7 @@ template <typename R> class Engine final : public EngineBase {
         : n_(n), B_(members), K_((n - 1) / 3), KP_(K_ + 1), J_(2 * K_ + 1), p_(p) {
         spec_ = (size_t)KP_ * J_;
         mixf_ = (size_t)KP_ * n;
+        check(hipGetDevice(&device_), "hipGetDevice");
         check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
         w0_ = DeviceBuffer<cx<R>>(B_ * spec_);
         w1_ = DeviceBuffer<cx<R>>(B_ * spec_);
@@ -1252,6 +1259,14 @@ template <typename R> class Engine final : public EngineBase {
     }
 
     void diagnostics(double *out) override {
+        DeviceBuffer<double> staging(B_ * 6);
+        diagnostics_with(out, staging.p);
+    }
+
+    // Diagnostics through `staging` (6 doubles per member, device memory the
+    // caller owns); allocates nothing.  Every entry of the B x 6 block is
+    // written before it is read.
+    void diagnostics_with(double *out, double *staging) override {
         flush_parameters();
         // CFL rate of the current state (src/solver.cpp:392-396): one row pass
         // over the mixed fields of the current state into a private buffer.
@@ -1259,23 +1274,27 @@ template <typename R> class Engine final : public EngineBase {
             run_col(0, stage_args(1.0, nullptr, 0));
             mix_valid_ = true;
         }
-        check(hipMemsetAsync(cfl_diag_.p, 0, cfl_diag_.count * sizeof(double), stream_),
-              "clear CFL");
+        const long cfl_count = (long)cfl_diag_.count;
+        hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((cfl_count + 255) / 256)), dim3(256), 0,
+                           stream_, cfl_count, 0.0, cfl_diag_.p);
         RowArgs<R> r{(int)B_, mix_.p, nl_.p, tw_row_, cfl_diag_.p};
         run_row(r);
-        DeviceBuffer<double> d(B_ * 6);
         // white forcing: the noise buffer still holds the last step's shell
         // noise; OU: the current coefficients (Solver::forcing(), src/solver.cpp:389)
         if (ou_)
             launch_ou(3, 0.0, noise_.p);
         DiagArgs<R> a{K_, w0_.p, d_nu_.p, forcing_model(), (ou_ || step_ > 0) ? 1 : 0,
-                      cfl_diag_.p, n_ / 2, n_, d.p};
+                      cfl_diag_.p, n_ / 2, n_, staging};
         hipLaunchKernelGGL(k_diag<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
         check(hipGetLastError(), "diagnostics kernel");
-        check(hipMemcpyAsync(out, d.p, B_ * 6 * sizeof(double), hipMemcpyDeviceToHost, stream_),
+        check(hipMemcpyAsync(out, staging, B_ * 6 * sizeof(double), hipMemcpyDeviceToHost,
+                             stream_),
               "diagnostics download");
         check(hipStreamSynchronize(stream_), "diagnostics");
     }
+
+    size_t members() const override { return B_; }
+    int device() const override { return device_; }
 
     void energy_spectrum(double *out) override {
         DeviceBuffer<double> bins(B_ * (size_t)n_);
@@ -2216,6 +2235,7 @@ template <typename R> class Engine final : public EngineBase {
     std::vector<CachedGraph> graphs_;
     GraphKey warm_key_{};
     bool warm_ = false, use_graph_ = true;
+    int device_ = 0; // the device current at creation (owns every buffer)
     uint64_t graph_clock_ = 0;
 };
 
@@ -2504,6 +2524,57 @@ int hzl_ensemble_diagnostics(hzl_ensemble_t h, double *out) {
         if (!out)
             throw std::invalid_argument("null output");
         engine_of(h).diagnostics(out);
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------- allocation-free diagnostics --
+// Staging of hzl_ensemble_diagnostics_noalloc: 6 doubles per member for at
+// most kNoAllocMembers members, one copy of the array on every GPU the library
+// runs on, held until the process ends.  Host code never names the array in a
+// launch or copy: each device's own address comes from hipGetSymbolAddress.
+constexpr size_t kNoAllocMembers = 65536;
+static __device__ __align__(256) double scratch[6 * kNoAllocMembers];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double *scratch_address[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller's current device is restored).
+double *scratch_on(int device) {
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (device < 0 || device >= kMaxDevices)
+        throw std::runtime_error("device index out of range for the diagnostics scratch");
+    if (!scratch_address[device]) {
+        int current = 0;
+        hzl_amd::check(hipGetDevice(&current), "hipGetDevice");
+        if (current != device)
+            hzl_amd::check(hipSetDevice(device), "hipSetDevice");
+        void *ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (current != device)
+            hzl_amd::check(hipSetDevice(current), "hipSetDevice");
+        hzl_amd::check(rc, "hipGetSymbolAddress(scratch)");
+        scratch_address[device] = static_cast<double *>(ptr);
+    }
+    return scratch_address[device];
+}
+} // namespace
+
+extern "C" {
+
+int hzl_ensemble_diagnostics_noalloc(hzl_ensemble_t h, double *out) {
+    return guarded([&] {
+        if (!out)
+            throw std::invalid_argument("null argument");
+        hzl_amd::EngineBase &e = engine_of(h);
+        if (e.members() > kNoAllocMembers)
+            throw std::invalid_argument(
+                "diagnostics without allocation take at most 65536 members");
+        e.diagnostics_with(out, scratch_on(e.device()));
     });
 }
 

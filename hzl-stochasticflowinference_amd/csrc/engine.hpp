@@ -118,6 +118,10 @@ class EngineBase {
     virtual void step_forced(double dt, const double *forcing, bool shared) = 0;
     virtual void synchronize() = 0;
     virtual void diagnostics(double *out) = 0;
+    // diagnostics through caller-owned device staging (B x 6 doubles)
+    virtual void diagnostics_with(double *out, double *staging) = 0;
+    virtual size_t members() const = 0;
+    virtual int device() const = 0; // the device the handle's buffers live on
     virtual void energy_spectrum(double *out) = 0; // [B][n] shell bins
     virtual void profile_step(double dt, float *ms) = 0;
     virtual void pf_begin(uint64_t seed, size_t max_groups) = 0;

@@ -43,6 +43,7 @@ EXPORTS = (
     "hzl_parameters_validate", "hzl_stream_seed", "hzl_ensemble_set_member_tau",
     "hzl_ensemble_energy_spectrum", "hzl_normal_draws", "hzl_stochastic_log_prior",
     "hzl_pf_log_likelihoods",
+    "hzl_ensemble_diagnostics_noalloc",
     "hzl_pf_joint_create", "hzl_pf_joint_destroy", "hzl_pf_joint_begin",
     "hzl_pf_joint_assimilate", "hzl_pf_joint_result",
     "hzl_surrogate_create", "hzl_surrogate_destroy", "hzl_surrogate_param_count",
@@ -163,6 +164,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_stream_seed.argtypes = [C.c_uint64, C.c_uint64]
     L.hzl_ensemble_set_member_tau.argtypes = [vp, C.c_size_t, C.c_double]
     L.hzl_ensemble_energy_spectrum.argtypes = [vp, _dp]
+    L.hzl_ensemble_diagnostics_noalloc.argtypes = [vp, _dp]
     L.hzl_normal_draws.argtypes = [C.c_uint64, C.c_size_t, _dp]
     L.hzl_stochastic_log_prior.argtypes = [_dp, C.c_size_t, _dp]
     fp, sz = C.POINTER(C.c_float), C.c_size_t

@@ -287,6 +287,16 @@ int hzl_ensemble_synchronize(hzl_ensemble_t h);
  * Replaces: Solver::diagnostics (src/solver.cpp:381-397). */
 int hzl_ensemble_diagnostics(hzl_ensemble_t h, double *out);
 
+/* hzl_ensemble_diagnostics without any device allocation: the same kernels
+ * and the same out[6 x members], staged through a fixed device array the
+ * library holds on every GPU it runs on (3 MiB, for the life of the process)
+ * instead of a buffer allocated per call.  At most 65536 members (HZL_EINVAL
+ * beyond).  Calls on one GPU share that array: calls that overlap on the same
+ * GPU (from several threads or handles) race and must be serialised by the
+ * caller.  No counterpart in the reference; for callers that cannot allocate
+ * device memory (tight serving or sampling loops, allocation-free harnesses). */
+int hzl_ensemble_diagnostics_noalloc(hzl_ensemble_t h, double *out);
+
 /* Shell energy spectrum of every member's current state: out[m * n + s]
  * receives bin s of member m, the sum of 2 pi^2 |w_k|^2 / |k|^2 over the modes
  * with floor(|k| + 1/2) = s (FP64 on the device, fixed summation order).
