@@ -12,7 +12,18 @@ reseeding of children.
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
+
+# Particles advance independently; the oracle's C calls release the GIL, so a
+# thread pool runs them in parallel (at most 16 workers).
+_POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+
+
+def _each(fn, count):
+    return list(_POOL.map(fn, range(count)))
 
 UINT64_MAX = (1 << 64) - 1
 
@@ -49,16 +60,20 @@ def oracle_bootstrap_filter(orc, initial, obs, dt, seed, nu, alpha=0.0, amp=0.1,
     current, total, events, history = 0, 0.0, 0, []
     for g, target in enumerate(groups):
         sel = obs[steps == target]
-        for i in range(P):
+
+        def advance(i):
             if ou is None:
-                states[i] = orc.trajectory(n, nu, alpha, dt, int(target - current), keys[i],
-                                           amp=amp, kf=kf, sigma=sigma, kmin=kmin, kmax=kmax,
-                                           step0=current, initial=states[i])
-                continue
+                return orc.trajectory(n, nu, alpha, dt, int(target - current), keys[i], amp=amp,
+                                      kf=kf, sigma=sigma, kmin=kmin, kmax=kmax, step0=current,
+                                      initial=states[i])
+            w = states[i]
             for _ in range(int(target - current)):
                 ou[i].transition(dt / 2)
-                states[i] = orc.rk4(states[i], nu, alpha, dt, 1, ou[i].F + kolmogorov)
+                w = orc.rk4(w, nu, alpha, dt, 1, ou[i].F + kolmogorov)
                 ou[i].transition(dt / 2)
+            return w
+
+        states = np.stack(_each(advance, P))
         logs = np.zeros(P)
         for i in range(P):
             ph = physical(orc, states[i])
@@ -337,10 +352,10 @@ def oracle_joint_filter(orc, initial, obs, dt, seed, nu, alpha=0.0, amp=0.1, kf=
     for g, target in enumerate(groups):
         sel = obs[steps == target]
         logs = np.zeros(N)
+        states = _each(lambda i: orc.trajectory(grid[i], nu, alpha, dt, int(target - current),
+                                                keys[i], amp=amp, kf=kf, sigma=sigma, kmin=kmin,
+                                                kmax=kmax, step0=current, initial=states[i]), N)
         for i in range(N):
-            states[i] = orc.trajectory(grid[i], nu, alpha, dt, int(target - current), keys[i],
-                                       amp=amp, kf=kf, sigma=sigma, kmin=kmin, kmax=kmax,
-                                       step0=current, initial=states[i])
             r = grid[i] // coarse
             ph = physical(orc, states[i])
             pred = ph[(sel[:, 2] * r).astype(int), (sel[:, 1] * r).astype(int)]
