@@ -544,6 +544,7 @@ template <typename R> class Engine final : public EngineBase {
         check(hipStreamSynchronize(stream_), "engine setup");
     }
     ~Engine() override {
+        enkf.reset(); // module state may wait on stream_
         if (stream_) {
             (void)hipStreamSynchronize(stream_);
             clear_graphs();
@@ -1563,6 +1564,28 @@ template <typename R> class Engine final : public EngineBase {
         return v;
     }
 
+    void prepare_steps(double dt) override {
+        validate_dt(dt);
+        flush_parameters();
+        set_counters(0, 0);
+    }
+
+    void enqueue_steps(double dt, size_t steps) override {
+        run_col(0, stage_args(dt, nullptr, 0)); // the current state's mixed fields
+        for (size_t s = 0; s < steps; ++s)
+            launch_step(dt, nullptr, 0, ctr_.p, s, nullptr);
+    }
+
+    void commit_steps(double dt, size_t steps) override {
+        step_ += steps;
+        last_dt_ = dt;
+        max_dt_ = std::max(max_dt_, dt);
+        mix_valid_ = true; // enqueue_steps left the mixed fields of the new state
+    }
+
+    uint64_t launch_epoch() const override { return launch_epoch_; }
+    bool graphs_enabled() const override { return use_graph_; }
+
     void invalidate_mixed() override {
         mix_valid_ = false;
     }
@@ -2178,6 +2201,7 @@ template <typename R> class Engine final : public EngineBase {
             (void)hipGraphExecDestroy(g.exec);
         graphs_.clear();
         warm_ = false;
+        ++launch_epoch_; // sequences other modules captured are stale too
     }
 
     int n_;
@@ -2234,8 +2258,9 @@ template <typename R> class Engine final : public EngineBase {
     DeviceBuffer<uint64_t> ctr_;
     std::vector<CachedGraph> graphs_;
     GraphKey warm_key_{};
-    bool warm_ = false, use_graph_ = true;
+    bool warm_ = false, use_graph_ = graphs_by_default();
     int device_ = 0; // the device current at creation (owns every buffer)
+    uint64_t launch_epoch_ = 0;
     uint64_t graph_clock_ = 0;
 };
 

@@ -142,6 +142,21 @@ class EngineBase {
     // device views for other engine modules (surrogate forecast, EnKF)
     virtual EngineViews views() = 0;
     virtual void invalidate_mixed() = 0;
+    // RK4 steps inside another module's launch sequence (which that module may
+    // capture as a HIP graph): prepare_steps(dt) runs eagerly (time-step
+    // checks, parameters, and the device step counter set to the current step
+    // index); enqueue_steps(dt, steps) launches the mixed fields of the current
+    // state and `steps` RK4 steps whose noise reads the step index from the
+    // device counter, so a replay continues the stream; commit_steps(dt, steps)
+    // then advances the host's step index.
+    virtual void prepare_steps(double dt) = 0;
+    virtual void enqueue_steps(double dt, size_t steps) = 0;
+    virtual void commit_steps(double dt, size_t steps) = 0;
+    // Captured launch sequences stay valid while this value is unchanged (it
+    // moves whenever kernels, launch shapes, buffers or the forcing process
+    // change), and only while graph replay is enabled.
+    virtual uint64_t launch_epoch() const = 0;
+    virtual bool graphs_enabled() const = 0;
     virtual void advance_counter(uint64_t steps) = 0;
     virtual void rewind_counter(uint64_t steps) = 0;
 };
@@ -151,7 +166,16 @@ class EngineBase {
 #include "hzl_amd.h"
 #include <stdexcept>
 #include <string>
+#include <cstdlib>
+#include <cstring>
 namespace hzl_amd {
+// Default of graph replay for new handles: on, unless the environment sets
+// HZL_GRAPHS=0 (launch-by-launch, e.g. under a kernel tracer).
+inline bool graphs_by_default() {
+    const char *v = std::getenv("HZL_GRAPHS");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
 // Engine behind a C-ABI handle (throws std::invalid_argument on null).
 EngineBase &engine_of(hzl_ensemble_t h);
 

@@ -13,20 +13,28 @@
 //     X_a = X + P S^-1 (D - H X),  P = X'^T (HX)' / (M - 1),
 //     S = (HX)'^T (HX)' / (M - 1) + diag(sd^2),
 //     with the four dense products on MFMA (FP32 operands, FP32 accumulation)
-//     and the m x m Cholesky factorisation and inverse in FP64 on the host.
+//     and S = L L^T factorised and solved in FP64 by one workgroup.
 // Serial mode optionally tapers each covariance with the Gaspari-Cohn
 // function of the periodic grid distance to the observation, in physical
 // space (src/filter.cpp:290-308).
 // Observation perturbations eps come from the reference NormalStream
 // (mt19937_64 seeded stream_seed(seed, M), Box-Muller with a cached second
 // draw; src/solver.cpp:41-54), consumed observation-major, member-minor as in
-// src/filter.cpp:311-313.  Point predictions are direct evaluations of the
-// retained Fourier sum at the grid point (exactly the physical field there).
+// src/filter.cpp:311-313, drawn on the device (normal_stream_body).  Point
+// predictions are direct evaluations of the retained Fourier sum at the grid
+// point (exactly the physical field there).
+//
+// A cycle (forecast steps, inflation, forecast sensor RMSE, perturbations,
+// analysis, posterior mean and spread) is one launch sequence on the engine's
+// stream reading the observation group from device memory: after one pinned
+// upload of the group it runs, or replays as a captured HIP graph, with one
+// host synchronisation at its end for the two record values.
 #include "engine.hpp"
 #include "gemm_launch.hpp"
 #include "tables.hpp"
 
 #include <cmath>
+#include <cstring>
 #include <random>
 #include <stdexcept>
 #include <vector>
@@ -73,9 +81,10 @@ __global__ void __launch_bounds__(256)
 // One block: mean and variance (1/(M-1)) of the predictions, prescaled
 // anomalies delta_i = (h_i - hbar)/(M-1) and gains g_i = (y + sd eps_i - h_i)/(var + sd^2).
 __global__ void __launch_bounds__(256)
-    k_enkf_innov(const double *pred, int B, double value, double sd, const double *eps,
-                 double *delta, double *gain) {
+    k_enkf_innov(const double *pred, int B, const double *value_o, const double *sd_o,
+                 const double *eps, double *delta, double *gain) {
     __shared__ double red[256];
+    const double value = *value_o, sd = *sd_o;
     double s = 0;
     for (int i = threadIdx.x; i < B; i += 256)
         s += pred[i];
@@ -352,11 +361,12 @@ __global__ void k_loc_inv_y(const cx<double> *cov, int K, int n, const double *c
 }
 
 __global__ void k_loc_inv_x_taper(const cx<double> *mixed, int K, int n, const double *cs,
-                                  const double *sn, int xo, int yo, double radius,
-                                  double *phys) {
+                                  const double *sn, const int *xo_p, const int *yo_p,
+                                  double radius, double *phys) {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long)n * n)
         return;
+    const int xo = *xo_p, yo = *yo_p;
     const int y = (int)(i / n), x = (int)(i % n);
     double v = 0;
     for (int c = 0; c <= K; ++c) {
@@ -439,83 +449,205 @@ template <typename R> __global__ void k_enkf_symmetrise(cx<R> *w, int B, long sp
     col[K - ky] = {sr, (R)-si};
 }
 
-// The reference NormalStream (src/solver.cpp:41-54).
-struct NormalStream {
-    std::mt19937_64 engine;
-    bool has_spare = false;
-    double spare = 0;
-    explicit NormalStream(uint64_t seed) : engine(seed) {}
-    double uniform() {
-        return ((double)(engine() >> 11) + 0.5) * 0x1.0p-53;
+// Point-evaluation tables of the staged observations: ex[o][c] =
+// e^{2 pi i (c x_o mod n) / n}, ey[o][j] = e^{2 pi i (ky y_o mod n) / n},
+// ky = j - K, read from cos / sin tables of 2 pi t / n.
+__global__ void k_enkf_phase(const int *ox, const int *oy, int nobs, int K, int n,
+                             const double *cs, const double *sn, cx<double> *ex,
+                             cx<double> *ey) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const int J = 2 * K + 1;
+    if (i >= (long)nobs * (K + 1 + J))
+        return;
+    const int o = (int)(i / (K + 1 + J)), r = (int)(i % (K + 1 + J));
+    if (r <= K) {
+        const int t = (int)(((long)r * ox[o]) % n);
+        ex[(long)o * (K + 1) + r] = {cs[t], sn[t]};
+    } else {
+        const long ky = r - (K + 1) - K;
+        const int t = (int)(((ky * oy[o]) % n + n) % n);
+        ey[(long)o * J + (r - K - 1)] = {cs[t], sn[t]};
     }
-    double operator()() {
-        if (has_spare) {
-            has_spare = false;
-            return spare;
-        }
-        const double r = std::sqrt(-2 * std::log(uniform())), angle = 2 * PI * uniform();
-        spare = r * std::sin(angle);
-        has_spare = true;
-        return r * std::cos(angle);
-    }
-};
+}
 
+// rec[0] = sqrt(mean over o of (pred[o] - value[o])^2): the ensemble mean's
+// forecast sensor RMSE (src/filter.cpp:254-264); clears the cycle's flags.
+__global__ void __launch_bounds__(256)
+    k_enkf_rmse(const double *pred, const double *value, int nobs, double *rec) {
+    __shared__ double red[256];
+    double s = 0;
+    for (int o = threadIdx.x; o < nobs; o += 256) {
+        const double d = pred[o] - value[o];
+        s += d * d;
+    }
+    const double t = block_sum(s, red);
+    if (threadIdx.x == 0) {
+        rec[0] = sqrt(t / nobs);
+        rec[2] = 0; // the factorisation's failure flag of this cycle
+        rec[3] = 0;
+    }
+}
+
+// *out = sqrt(sum of part[0..count) / (B - 1)): the posterior spread from the
+// per-chunk, per-mode squared anomalies (src/filter.cpp:343-347).
+__global__ void __launch_bounds__(256)
+    k_enkf_spread_final(const double *part, long count, int B, double *out) {
+    __shared__ double red[256];
+    double s = 0;
+    for (long e = threadIdx.x; e < count; e += 256)
+        s += part[e];
+    const double t = block_sum(s, red);
+    if (threadIdx.x == 0)
+        *out = sqrt(t / (B - 1));
+}
+
+// One workgroup continues the filter's NormalStream (normal_stream_body).
+__global__ void __launch_bounds__(256)
+    k_normal_stream(uint64_t *mt, double *spare, long count, double *out) {
+    __shared__ uint64_t s[MT_N];
+    normal_stream_body(DevCtx{}, mt, spare, count, out, s, 256);
+}
+
+// Batch mode: observation operator rows H[o][2e], H[o][2e + 1] with
+// Re(w e^{i phi}) = w.x cos - w.y sin, weighted 2 for kx > 0 modes.
+__global__ void k_enkf_build_H(const cx<double> *ex, const cx<double> *ey, int m, int K,
+                               float *H) {
+    const int J = 2 * K + 1;
+    const long spec = (long)(K + 1) * J, i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)m * spec)
+        return;
+    const int o = (int)(i / spec);
+    const long e = i % spec;
+    const int c = (int)(e / J), j = (int)(e % J);
+    const cx<double> a = ex[(long)o * (K + 1) + c], b = ey[(long)o * J + j];
+    const double wgt = c ? 2.0 : 1.0, pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+    H[(long)o * 2 * spec + 2 * e] = (float)(wgt * pr);
+    H[(long)o * 2 * spec + 2 * e + 1] = (float)(-wgt * pi);
+}
+
+// Batch mode, one workgroup: L L^T = (S + S^T) / 2 + diag(sd^2) in FP64
+// (lower triangle of L, row-major m x m; right-looking, one column per step).
+// flag[0] is set when a pivot is not positive.
+__global__ void __launch_bounds__(256)
+    k_enkf_cholesky(const float *S, const double *sd, int m, double *L, double *flag) {
+    __shared__ double pivot;
+    const int t = threadIdx.x;
+    for (long idx = t; idx < (long)m * m; idx += 256) {
+        const int i = (int)(idx / m), j = (int)(idx % m);
+        L[idx] = j <= i ? 0.5 * ((double)S[idx] + (double)S[(long)j * m + i]) +
+                              (i == j ? sd[i] * sd[i] : 0.0)
+                        : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        if (t == 0) {
+            const double d = L[(long)k * m + k];
+            if (!(d > 0))
+                flag[0] = 1;
+            pivot = d > 0 ? sqrt(d) : 1.0;
+            L[(long)k * m + k] = pivot;
+        }
+        __syncthreads();
+        const double p = pivot;
+        for (int i = k + 1 + t; i < m; i += 256)
+            L[(long)i * m + k] /= p;
+        __syncthreads();
+        for (int i = k + 1 + t; i < m; i += 256) {
+            const double lik = L[(long)i * m + k];
+            for (int j = k + 1; j <= i; ++j)
+                L[(long)i * m + j] -= lik * L[(long)j * m + k];
+        }
+        __syncthreads();
+    }
+}
+
+// Batch mode, one thread per member column: Z[:, i] = S^-1 Dm[:, i] by
+// forward and back substitution with L (FP64, work in zw[m][B]).
+__global__ void __launch_bounds__(256)
+    k_enkf_chol_solve(const double *L, int m, const float *Dm, int B, double *zw, float *Z) {
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= B)
+        return;
+    for (int r = 0; r < m; ++r) {
+        double s = (double)Dm[(long)r * B + i];
+        for (int k = 0; k < r; ++k)
+            s -= L[(long)r * m + k] * zw[(long)k * B + i];
+        zw[(long)r * B + i] = s / L[(long)r * m + r];
+    }
+    for (int r = m - 1; r >= 0; --r) {
+        double s = zw[(long)r * B + i];
+        for (int k = r + 1; k < m; ++k)
+            s -= L[(long)k * m + r] * zw[(long)k * B + i];
+        zw[(long)r * B + i] = s / L[(long)r * m + r];
+        Z[(long)r * B + i] = (float)zw[(long)r * B + i];
+    }
+}
+
+// Filter state kept between observation groups: the perturbation stream
+// (device mt19937_64 words, position, kept second normal), the device buffers
+// of every cycle, and the captured cycle graph.
 struct EnkfState : ModuleState {
     int mode;
-    NormalStream normals;
+    Buf<uint64_t> mt;   // 312 words + position
+    Buf<double> spare;  // has_spare, spare
     Buf<cx<double>> ex, ey, partial, cov, mean;
-    Buf<double> pred, pred2, delta, gain, eps, spread, value, sd, spart, cs, sn, phys;
+    Buf<double> pred, pred2, delta, gain, eps, spart, cs, sn, phys, obs, rec;
     Buf<cx<double>> mixed, fcov, ffield;
-    // batch mode (FP32)
-    Buf<float> H, HX, HA, P, S, Sinv, Dm, Z, HXpart;
-    EnkfState(int mode_, uint64_t seed) : mode(mode_), normals(seed) {}
+    // batch mode (FP32 products, FP64 factorisation)
+    Buf<float> H, HX, HA, P, S, Dm, Z, HXpart;
+    Buf<double> L, zw;
+    // pinned staging of one observation group and of the record values
+    double *host_obs = nullptr, *host_rec = nullptr;
+    size_t host_obs_cap = 0;
+    hipEvent_t uploaded = nullptr; // the last group's upload out of host_obs
+    // cycle graph: captured on the second consecutive call with the same key
+    struct Key {
+        uint64_t epoch, generation;
+        double dt, inflation, radius;
+        size_t steps, nobs;
+        bool operator==(const Key &o) const {
+            return epoch == o.epoch && generation == o.generation && dt == o.dt &&
+                   inflation == o.inflation && radius == o.radius && steps == o.steps &&
+                   nobs == o.nobs;
+        }
+    };
+    hipGraphExec_t exec = nullptr;
+    Key exec_key{}, warm_key{};
+    bool warm = false;
+    hipStream_t stream = nullptr;
+
+    EnkfState(int mode_, uint64_t seed, hipStream_t s) : mode(mode_), stream(s) {
+        // std::mt19937_64(seed): the standard seeding recurrence, position 312
+        std::vector<uint64_t> w(MT_N + 1);
+        w[0] = seed;
+        for (int i = 1; i < MT_N; ++i)
+            w[i] = 6364136223846793005ULL * (w[i - 1] ^ (w[i - 1] >> 62)) + (uint64_t)i;
+        w[MT_N] = MT_N;
+        const double sp[2] = {0, 0};
+        mt.alloc(w.size());
+        spare.alloc(2);
+        hcheck(hipMemcpyAsync(mt.p, w.data(), w.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              s),
+               "EnKF stream");
+        hcheck(hipMemcpyAsync(spare.p, sp, sizeof sp, hipMemcpyHostToDevice, s), "EnKF stream");
+        hcheck(hipStreamSynchronize(s), "EnKF stream");
+        hcheck(hipHostMalloc(reinterpret_cast<void **>(&host_rec), 4 * sizeof(double)),
+               "hipHostMalloc");
+    }
+    EnkfState(int mode_, hipStream_t s) : mode(mode_), stream(s) {} // mean / spread only
+    ~EnkfState() override {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (uploaded)
+            (void)hipEventDestroy(uploaded);
+        if (host_obs)
+            (void)hipHostFree(host_obs);
+        if (host_rec)
+            (void)hipHostFree(host_rec);
+    }
 };
-
-namespace {
-template <typename T> void upload(Buf<T> &b, const T *h, size_t n, hipStream_t s) {
-    b.alloc(n);
-    hcheck(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, s), "EnKF upload");
-}
-
-} // namespace
-
-// Cholesky factorisation and inverse of an SPD matrix (FP64, in place: a
-// becomes the inverse).
-inline void spd_inverse(std::vector<double> &a, int m) {
-    std::vector<double> L((size_t)m * m, 0.0);
-    for (int j = 0; j < m; ++j) {
-        double d = a[(size_t)j * m + j];
-        for (int k = 0; k < j; ++k)
-            d -= L[(size_t)j * m + k] * L[(size_t)j * m + k];
-        if (!(d > 0))
-            throw std::runtime_error("EnKF innovation covariance is not positive definite");
-        L[(size_t)j * m + j] = std::sqrt(d);
-        for (int i = j + 1; i < m; ++i) {
-            double s = a[(size_t)i * m + j];
-            for (int k = 0; k < j; ++k)
-                s -= L[(size_t)i * m + k] * L[(size_t)j * m + k];
-            L[(size_t)i * m + j] = s / L[(size_t)j * m + j];
-        }
-    }
-    // inverse of L (lower), then A^-1 = L^-T L^-1
-    std::vector<double> Li((size_t)m * m, 0.0);
-    for (int j = 0; j < m; ++j) {
-        Li[(size_t)j * m + j] = 1.0 / L[(size_t)j * m + j];
-        for (int i = j + 1; i < m; ++i) {
-            double s = 0;
-            for (int k = j; k < i; ++k)
-                s -= L[(size_t)i * m + k] * Li[(size_t)k * m + j];
-            Li[(size_t)i * m + j] = s / L[(size_t)i * m + i];
-        }
-    }
-    for (int i = 0; i < m; ++i)
-        for (int j = 0; j <= i; ++j) {
-            double s = 0;
-            for (int k = i; k < m; ++k)
-                s += Li[(size_t)k * m + i] * Li[(size_t)k * m + j];
-            a[(size_t)i * m + j] = a[(size_t)j * m + i] = s;
-        }
-}
 
 template <typename R> struct Enkf {
     EngineBase &eng;
@@ -533,6 +665,12 @@ template <typename R> struct Enkf {
 
     static constexpr int CHUNKS = 64; // member chunks of the two-pass reductions
 
+    // the staged group: values, sds, then x and y as ints
+    const double *value_d() const { return st.obs.p; }
+    const double *sd_d(size_t nobs) const { return st.obs.p + nobs; }
+    const int *x_d(size_t nobs) const { return reinterpret_cast<const int *>(st.obs.p + 2 * nobs); }
+    const int *y_d(size_t nobs) const { return x_d(nobs) + nobs; }
+
     void mean() {
         st.mean.alloc((size_t)spec);
         st.partial.alloc((size_t)CHUNKS * spec);
@@ -542,21 +680,25 @@ template <typename R> struct Enkf {
                            CHUNKS, spec, st.mean.p);
     }
 
-    // posterior spread (after mean()), src/filter.cpp:343-347
-    double spread() {
+    // posterior spread into *out (device), after mean() (src/filter.cpp:343-347)
+    void spread_into(double *out) {
         st.spart.alloc((size_t)CHUNKS * spec);
         hipLaunchKernelGGL(k_enkf_spread_partial<R>, dim3((unsigned)((spec + 255) / 256), CHUNKS),
                            dim3(256), 0, ev.stream, w, ev.B, spec, K, st.mean.p, CHUNKS,
                            st.spart.p);
-        std::vector<double> h((size_t)CHUNKS * spec);
-        hcheck(hipMemcpyAsync(h.data(), st.spart.p, h.size() * sizeof(double),
-                              hipMemcpyDeviceToHost, ev.stream),
+        hipLaunchKernelGGL(k_enkf_spread_final, dim3(1), dim3(256), 0, ev.stream, st.spart.p,
+                           (long)CHUNKS * spec, ev.B, out);
+    }
+
+    // posterior spread to the host (after mean())
+    double spread() {
+        st.rec.alloc(4);
+        spread_into(st.rec.p + 1);
+        double v = 0;
+        hcheck(hipMemcpyAsync(&v, st.rec.p + 1, sizeof v, hipMemcpyDeviceToHost, ev.stream),
                "spread download");
         hcheck(hipStreamSynchronize(ev.stream), "spread");
-        double s = 0;
-        for (double v : h)
-            s += v;
-        return std::sqrt(s / (ev.B - 1));
+        return v;
     }
 
     void mean_field(double *field) {
@@ -568,25 +710,97 @@ template <typename R> struct Enkf {
         half_to_full<double>(ev.n, h.data(), field);
     }
 
-    void assimilate(size_t nobs, const int *x, const int *y, const double *value,
-                    const double *sd, double inflation, double radius, double *record,
-                    double *mean_out) {
-        const int B = ev.B, n = ev.n;
-        // point-evaluation tables of every observation
-        std::vector<cx<double>> ex(nobs * (K + 1)), ey(nobs * J);
-        for (size_t o = 0; o < nobs; ++o) {
-            for (int c = 0; c <= K; ++c) {
-                const double a = 2 * PI * (double)((long)c * x[o] % n) / n;
-                ex[o * (K + 1) + c] = {std::cos(a), std::sin(a)};
+    // Every device buffer a cycle of `nobs` observations touches (allocations
+    // happen here, never inside a launch sequence).
+    void reserve(size_t nobs) {
+        const size_t B = (size_t)ev.B, n = (size_t)ev.n;
+        st.obs.alloc(3 * nobs + 1);
+        st.ex.alloc(nobs * (K + 1));
+        st.ey.alloc(nobs * J);
+        st.eps.alloc(nobs * B);
+        st.pred.alloc(std::max(nobs, B));
+        st.rec.alloc(4);
+        st.mean.alloc((size_t)spec);
+        st.partial.alloc((size_t)CHUNKS * spec);
+        st.spart.alloc((size_t)CHUNKS * spec);
+        if (!st.cs.p) {
+            std::vector<double> c(n), sv(n);
+            for (size_t j = 0; j < n; ++j) {
+                c[j] = std::cos(2 * PI * (double)j / (double)n);
+                sv[j] = std::sin(2 * PI * (double)j / (double)n);
             }
-            for (int j = 0; j < J; ++j) {
-                const long ky = j - K;
-                const double a = 2 * PI * (double)((((ky * y[o]) % n) + n) % n) / n;
-                ey[o * J + j] = {std::cos(a), std::sin(a)};
-            }
+            st.cs.alloc(n);
+            st.sn.alloc(n);
+            hcheck(hipMemcpyAsync(st.cs.p, c.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                  ev.stream),
+                   "EnKF tables");
+            hcheck(hipMemcpyAsync(st.sn.p, sv.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                  ev.stream),
+                   "EnKF tables");
+            hcheck(hipStreamSynchronize(ev.stream), "EnKF tables");
         }
-        upload(st.ex, ex.data(), ex.size(), ev.stream);
-        upload(st.ey, ey.data(), ey.size(), ev.stream);
+        if (st.mode == HZL_ENKF_SERIAL) {
+            st.pred2.alloc(B);
+            st.delta.alloc(B);
+            st.gain.alloc(B);
+            st.cov.alloc((size_t)spec);
+            st.mixed.alloc((size_t)(K + 1) * n);
+            st.phys.alloc(n * n);
+            if (ev.ou_F) {
+                st.fcov.alloc((size_t)ev.npairs);
+                st.ffield.alloc((size_t)spec);
+            }
+        } else {
+            const size_t m = nobs, D = 2 * (size_t)spec, KC = 256;
+            const size_t slices = D / KC + (D % KC ? 1 : 0);
+            st.H.alloc(m * D);
+            st.HX.alloc(B * m);
+            st.HA.alloc(B * m);
+            st.Dm.alloc(m * B);
+            st.P.alloc(D * m);
+            st.S.alloc(m * m);
+            st.Z.alloc(m * B);
+            st.HXpart.alloc(slices * B * m);
+            st.L.alloc(m * m);
+            st.zw.alloc(m * B);
+        }
+        if (st.host_obs_cap < 3 * nobs + 1) {
+            hcheck(hipStreamSynchronize(ev.stream), "staging growth");
+            if (st.host_obs)
+                (void)hipHostFree(st.host_obs);
+            st.host_obs = nullptr;
+            hcheck(hipHostMalloc(reinterpret_cast<void **>(&st.host_obs),
+                                 (3 * nobs + 1) * sizeof(double)),
+                   "hipHostMalloc");
+            st.host_obs_cap = 3 * nobs + 1;
+        }
+    }
+
+    // One pinned upload of the group: [values | sds | x, y as ints].
+    void stage(size_t nobs, const int *x, const int *y, const double *value, const double *sd) {
+        if (!st.uploaded)
+            hcheck(hipEventCreateWithFlags(&st.uploaded, hipEventDisableTiming), "event");
+        else
+            hcheck(hipEventSynchronize(st.uploaded), "staging reuse"); // done long ago
+        std::memcpy(st.host_obs, value, nobs * sizeof(double));
+        std::memcpy(st.host_obs + nobs, sd, nobs * sizeof(double));
+        int *xy = reinterpret_cast<int *>(st.host_obs + 2 * nobs);
+        std::memcpy(xy, x, nobs * sizeof(int));
+        std::memcpy(xy + nobs, y, nobs * sizeof(int));
+        hcheck(hipMemcpyAsync(st.obs.p, st.host_obs, 3 * nobs * sizeof(double),
+                              hipMemcpyHostToDevice, ev.stream),
+               "observation upload");
+        hcheck(hipEventRecord(st.uploaded, ev.stream), "event");
+    }
+
+    // The launches of one cycle (capturable: device buffers only).
+    void launches(size_t nobs, double inflation, double radius, double dt, size_t steps) {
+        const int B = ev.B;
+        if (steps)
+            eng.enqueue_steps(dt, steps);
+        const long tab = (long)nobs * (K + 1 + J);
+        hipLaunchKernelGGL(k_enkf_phase, grid1d(tab), dim3(256), 0, ev.stream, x_d(nobs),
+                           y_d(nobs), (int)nobs, K, ev.n, st.cs.p, st.sn.p, st.ex.p, st.ey.p);
         if (inflation > 1) {
             mean();
             hipLaunchKernelGGL(k_enkf_inflate<R>, grid1d((long)B * spec), dim3(256), 0,
@@ -598,100 +812,128 @@ template <typename R> struct Enkf {
         // forecast sensor RMSE of the ensemble mean, before any update
         // (src/filter.cpp:254-264)
         mean();
-        st.pred.alloc(std::max<size_t>(nobs, (size_t)B));
         hipLaunchKernelGGL(k_enkf_predict<double>, dim3(1, (unsigned)nobs), dim3(256), 0,
                            ev.stream, st.mean.p, spec, K, st.ex.p, st.ey.p, st.pred.p, 1L);
-        std::vector<double> fc(nobs);
-        hcheck(hipMemcpyAsync(fc.data(), st.pred.p, nobs * sizeof(double), hipMemcpyDeviceToHost,
-                              ev.stream),
-               "forecast download");
-        hcheck(hipStreamSynchronize(ev.stream), "forecast");
-        double err = 0;
-        for (size_t o = 0; o < nobs; ++o)
-            err += (fc[o] - value[o]) * (fc[o] - value[o]);
-        record[0] = std::sqrt(err / (double)nobs);
+        hipLaunchKernelGGL(k_enkf_rmse, dim3(1), dim3(256), 0, ev.stream, st.pred.p, value_d(),
+                           (int)nobs, st.rec.p);
         // perturbations, observation-major then member
-        std::vector<double> eps(nobs * (size_t)B);
-        for (auto &e : eps)
-            e = st.normals();
-        upload(st.eps, eps.data(), eps.size(), ev.stream);
+        hipLaunchKernelGGL(k_normal_stream, dim3(1), dim3(256), 0, ev.stream, st.mt.p,
+                           st.spare.p, (long)(nobs * (size_t)B), st.eps.p);
         if (st.mode == HZL_ENKF_SERIAL)
-            serial(nobs, x, y, value, sd, radius);
+            serial(nobs, radius);
         else
-            batch(nobs, value, sd);
-        hcheck(hipGetLastError(), "EnKF kernels");
-        eng.invalidate_mixed();
+            batch(nobs);
         mean();
-        record[1] = spread();
-        if (mean_out)
-            mean_field(mean_out);
+        spread_into(st.rec.p + 1);
     }
 
-    void localize(int xo, int yo, double radius) {
-        localize(st.cov.p, xo, yo, radius);
-    }
-
-    // taper the half-plane FP64 field cov in place
-    void localize(cx<double> *cov, int xo, int yo, double radius) {
-        const int n = ev.n;
-        if (!st.cs.p || st.cs.n < (size_t)n) {
-            std::vector<double> c(n), sv(n);
-            for (int j = 0; j < n; ++j) {
-                c[j] = std::cos(2 * PI * j / n);
-                sv[j] = std::sin(2 * PI * j / n);
+    void assimilate(size_t nobs, const int *x, const int *y, const double *value,
+                    const double *sd, double inflation, double radius, double dt, size_t steps,
+                    double *record, double *mean_out) {
+        reserve(nobs);
+        stage(nobs, x, y, value, sd);
+        if (steps)
+            eng.prepare_steps(dt);
+        const EnkfState::Key key{eng.launch_epoch(), alloc_generation(), dt,   inflation,
+                                 radius,             steps,              nobs};
+        const bool graphs = eng.graphs_enabled();
+        if (graphs && st.exec && st.exec_key == key) {
+            hcheck(hipGraphLaunch(st.exec, ev.stream), "EnKF graph launch");
+        } else if (graphs && st.warm && st.warm_key == key) {
+            if (st.exec) {
+                (void)hipGraphExecDestroy(st.exec);
+                st.exec = nullptr;
             }
-            upload(st.cs, c.data(), n, ev.stream);
-            upload(st.sn, sv.data(), n, ev.stream);
+            hipGraph_t g = nullptr;
+            hcheck(hipStreamBeginCapture(ev.stream, hipStreamCaptureModeThreadLocal), "capture");
+            try {
+                launches(nobs, inflation, radius, dt, steps);
+            } catch (...) {
+                (void)hipStreamEndCapture(ev.stream, &g);
+                if (g)
+                    (void)hipGraphDestroy(g);
+                throw;
+            }
+            hcheck(hipStreamEndCapture(ev.stream, &g), "capture end");
+            const hipError_t e = hipGraphInstantiate(&st.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hcheck(e, "EnKF graph instantiate");
+            st.exec_key = key;
+            st.warm = false;
+            hcheck(hipGraphLaunch(st.exec, ev.stream), "EnKF graph launch");
+        } else {
+            launches(nobs, inflation, radius, dt, steps);
+            st.warm = graphs;
+            // the key after this run's allocations (none happen in a replay)
+            st.warm_key = EnkfState::Key{eng.launch_epoch(), alloc_generation(), dt, inflation,
+                                         radius, steps, nobs};
         }
+        hcheck(hipGetLastError(), "EnKF kernels");
+        if (steps)
+            eng.commit_steps(dt, steps);
+        eng.invalidate_mixed();
+        hcheck(hipMemcpyAsync(st.host_rec, st.rec.p, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                              ev.stream),
+               "record download");
+        std::vector<cx<double>> h;
+        if (mean_out) {
+            h.resize((size_t)spec);
+            hcheck(hipMemcpyAsync(h.data(), st.mean.p, spec * sizeof(cx<double>),
+                                  hipMemcpyDeviceToHost, ev.stream),
+                   "mean download");
+        }
+        hcheck(hipStreamSynchronize(ev.stream), "EnKF cycle"); // the cycle's one wait
+        if (st.host_rec[2] != 0)
+            throw std::runtime_error("EnKF innovation covariance is not positive definite");
+        record[0] = st.host_rec[0];
+        record[1] = st.host_rec[1];
+        if (mean_out)
+            half_to_full<double>(ev.n, h.data(), mean_out);
+    }
+
+    // taper the half-plane FP64 field cov in place around observation o
+    void localize(cx<double> *cov, size_t nobs, size_t o, double radius) {
+        const int n = ev.n;
         const long mix = (long)(K + 1) * n;
-        st.mixed.alloc((size_t)mix);
-        st.phys.alloc((size_t)n * n);
         hipLaunchKernelGGL(k_loc_inv_y, grid1d(mix), dim3(256), 0, ev.stream, cov, K, n,
                            st.cs.p, st.sn.p, st.mixed.p);
         hipLaunchKernelGGL(k_loc_inv_x_taper, grid1d((long)n * n), dim3(256), 0, ev.stream,
-                           st.mixed.p, K, n, st.cs.p, st.sn.p, xo, yo, radius, st.phys.p);
+                           st.mixed.p, K, n, st.cs.p, st.sn.p, x_d(nobs) + o, y_d(nobs) + o,
+                           radius, st.phys.p);
         hipLaunchKernelGGL(k_loc_fwd_x, grid1d(mix), dim3(256), 0, ev.stream, st.phys.p, K, n,
                            st.cs.p, st.sn.p, st.mixed.p);
         hipLaunchKernelGGL(k_loc_fwd_y, grid1d(spec), dim3(256), 0, ev.stream, st.mixed.p, K, n,
                            st.cs.p, st.sn.p, cov);
     }
 
-    void serial(size_t nobs, const int *x, const int *y, const double *value, const double *sd,
-                double radius) {
+    void serial(size_t nobs, double radius) {
         const int B = ev.B, chunks = CHUNKS;
-        st.pred.alloc((size_t)B);
-        st.pred2.alloc((size_t)B);
-        st.delta.alloc((size_t)B);
-        st.gain.alloc((size_t)B);
-        st.partial.alloc((size_t)chunks * spec);
-        st.cov.alloc((size_t)spec);
         double *cur = st.pred.p, *next = st.pred2.p;
         hipLaunchKernelGGL(k_enkf_predict<R>, dim3((unsigned)B, 1), dim3(256), 0, ev.stream, w,
                            spec, K, st.ex.p, st.ey.p, cur, 0L);
         for (size_t o = 0; o < nobs; ++o) {
-            hipLaunchKernelGGL(k_enkf_innov, dim3(1), dim3(256), 0, ev.stream, cur, B, value[o],
-                               sd[o], st.eps.p + o * (size_t)B, st.delta.p, st.gain.p);
+            hipLaunchKernelGGL(k_enkf_innov, dim3(1), dim3(256), 0, ev.stream, cur, B,
+                               value_d() + o, sd_d(nobs) + o, st.eps.p + o * (size_t)B,
+                               st.delta.p, st.gain.p);
             hipLaunchKernelGGL(k_enkf_cov_partial<R>, dim3((unsigned)((spec + 255) / 256), chunks),
                                dim3(256), 0, ev.stream, w, B, spec, st.delta.p, chunks,
                                st.partial.p);
             hipLaunchKernelGGL(k_enkf_cov_final, grid1d(spec), dim3(256), 0, ev.stream,
                                st.partial.p, chunks, spec, st.cov.p);
             if (radius > 0)
-                localize(x[o], y[o], radius);
+                localize(st.cov.p, nobs, o, radius);
             if (ev.ou_F) {
                 // the forcing state: its covariance with the prediction (all
                 // members before any update), tapered like the state's
                 const int P = ev.npairs;
-                st.fcov.alloc((size_t)P);
                 hipLaunchKernelGGL(k_ou_cov, grid1d(P), dim3(256), 0, ev.stream, ev.ou_F, B, P,
                                    st.delta.p, st.fcov.p);
                 if (radius > 0) {
-                    st.ffield.alloc((size_t)spec);
                     hipLaunchKernelGGL(k_zero_cx, grid1d(spec), dim3(256), 0, ev.stream,
                                        st.ffield.p, spec);
                     hipLaunchKernelGGL(k_ou_scatter, grid1d(P), dim3(256), 0, ev.stream,
                                        st.fcov.p, P, ev.ou_off, K, st.ffield.p);
-                    localize(st.ffield.p, x[o], y[o], radius);
+                    localize(st.ffield.p, nobs, o, radius);
                     hipLaunchKernelGGL(k_ou_gather, grid1d(P), dim3(256), 0, ev.stream,
                                        st.ffield.p, P, ev.ou_off, K, st.fcov.p);
                 }
@@ -707,44 +949,18 @@ template <typename R> struct Enkf {
         }
     }
 
-    void batch(size_t nobs, const double *value, const double *sd);
+    void batch(size_t nobs);
 };
 
-template <> void Enkf<double>::batch(size_t, const double *, const double *) {
+template <> void Enkf<double>::batch(size_t) {
     throw std::invalid_argument("batch EnKF analysis needs an FP32 ensemble");
 }
 
-template <> void Enkf<float>::batch(size_t nobs, const double *value, const double *sd) {
+template <> void Enkf<float>::batch(size_t nobs) {
     const int B = ev.B, m = (int)nobs;
     const long D = 2 * spec; // real components per member
-    // observation operator rows: Re(w ph) = w.x ph.x - w.y ph.y, weighted
-    std::vector<float> H((size_t)m * D);
-    std::vector<cx<double>> ex((size_t)m * (K + 1)), ey((size_t)m * J);
-    hcheck(hipMemcpyAsync(ex.data(), st.ex.p, ex.size() * sizeof(cx<double>),
-                          hipMemcpyDeviceToHost, ev.stream),
-           "tables");
-    hcheck(hipMemcpyAsync(ey.data(), st.ey.p, ey.size() * sizeof(cx<double>),
-                          hipMemcpyDeviceToHost, ev.stream),
-           "tables");
-    hcheck(hipStreamSynchronize(ev.stream), "tables");
-    for (int o = 0; o < m; ++o)
-        for (long e = 0; e < spec; ++e) {
-            const int c = (int)(e / J), j = (int)(e % J);
-            const cx<double> a = ex[(size_t)o * (K + 1) + c], b = ey[(size_t)o * J + j];
-            const double wgt = c ? 2.0 : 1.0, pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
-            H[(size_t)o * D + 2 * e] = (float)(wgt * pr);
-            H[(size_t)o * D + 2 * e + 1] = (float)(-wgt * pi);
-        }
-    upload(st.H, H.data(), H.size(), ev.stream);
-    upload(st.value, value, nobs, ev.stream);
-    upload(st.sd, sd, nobs, ev.stream);
-    st.HX.alloc((size_t)B * m);
-    st.HA.alloc((size_t)B * m);
-    st.Dm.alloc((size_t)m * B);
-    st.P.alloc((size_t)D * m);
-    st.S.alloc((size_t)m * m);
-    st.Sinv.alloc((size_t)m * m);
-    st.Z.alloc((size_t)m * B);
+    hipLaunchKernelGGL(k_enkf_build_H, grid1d((long)m * spec), dim3(256), 0, ev.stream, st.ex.p,
+                       st.ey.p, m, K, st.H.p);
     const Span<const float> X{reinterpret_cast<const float *>(w), (size_t)B * D};
     const Span<void> Xw{reinterpret_cast<float *>(w), (size_t)B * D};
     // HX = X H^T, split along the long K = 2 spec axis: full slices of KC, a
@@ -752,7 +968,6 @@ template <> void Enkf<float>::batch(size_t nobs, const double *value, const doub
     {
         const long KC = 256, full = D / KC, rest = D - full * KC, slices = full + (rest ? 1 : 0);
         const long count = (long)B * m;
-        st.HXpart.alloc((size_t)(slices * count));
         if (full) {
             Gemm<float> g1(B, m, (int)KC, X, D, 1, cs(st.H), D, 1, vs(st.HXpart), m, 1, OUT_F32);
             g1.batch((int)full, KC, KC, count);
@@ -768,7 +983,7 @@ template <> void Enkf<float>::batch(size_t nobs, const double *value, const doub
                            (int)slices, count, st.HX.p);
     }
     hipLaunchKernelGGL(k_enkf_batch_anomalies, dim3((unsigned)m), dim3(256), 0, ev.stream,
-                       st.HX.p, B, m, st.value.p, st.sd.p, st.eps.p, st.HA.p, st.Dm.p);
+                       st.HX.p, B, m, value_d(), sd_d(nobs), st.eps.p, st.HA.p, st.Dm.p);
     // P = X^T HA / (M - 1)
     Gemm<float> g2((int)D, m, B, X, 1, D, cs(st.HA), 1, m, vs(st.P), m, 1, OUT_F32);
     g2.a.alpha = 1.0f / (float)(B - 1);
@@ -777,23 +992,11 @@ template <> void Enkf<float>::batch(size_t nobs, const double *value, const doub
     Gemm<float> g3(m, m, B, cs(st.HA), 1, m, cs(st.HA), 1, m, vs(st.S), m, 1, OUT_F32);
     g3.a.alpha = 1.0f / (float)(B - 1);
     g3.run(ev.stream);
-    std::vector<float> Sf((size_t)m * m);
-    hcheck(hipMemcpyAsync(Sf.data(), st.S.p, Sf.size() * sizeof(float), hipMemcpyDeviceToHost,
-                          ev.stream),
-           "S download");
-    hcheck(hipStreamSynchronize(ev.stream), "S");
-    std::vector<double> S((size_t)m * m);
-    for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j)
-            S[(size_t)i * m + j] = 0.5 * ((double)Sf[(size_t)i * m + j] + Sf[(size_t)j * m + i]) +
-                                   (i == j ? sd[i] * sd[i] : 0.0);
-    spd_inverse(S, m);
-    for (size_t i = 0; i < S.size(); ++i)
-        Sf[i] = (float)S[i];
-    upload(st.Sinv, Sf.data(), Sf.size(), ev.stream);
-    // Z = S^-1 Dm
-    Gemm<float> g4(m, B, m, cs(st.Sinv), m, 1, cs(st.Dm), 1, B, vs(st.Z), B, 1, OUT_F32);
-    g4.run(ev.stream);
+    // Z = S^-1 Dm with S + diag(sd^2) = L L^T in FP64
+    hipLaunchKernelGGL(k_enkf_cholesky, dim3(1), dim3(256), 0, ev.stream, st.S.p, sd_d(nobs), m,
+                       st.L.p, st.rec.p + 2);
+    hipLaunchKernelGGL(k_enkf_chol_solve, grid1d(B), dim3(256), 0, ev.stream, st.L.p, m,
+                       st.Dm.p, B, st.zw.p, st.Z.p);
     // X += Z^T P^T
     Gemm<float> g5(B, (int)D, m, cs(st.Z), 1, B, cs(st.P), m, 1, Xw, D, 1, OUT_F32);
     g5.a.beta = 1.f;
@@ -834,8 +1037,9 @@ int hzl_enkf_begin(hzl_ensemble_t h, uint64_t seed, int mode) {
         // updates the vorticity only
         if (mode == HZL_ENKF_BATCH && e.forcing_process() == HZL_FORCING_OU)
             throw std::invalid_argument("batch EnKF analysis runs with the white forcing process");
+        e.enkf.reset(); // a running filter's graph and stream state go first
         e.enkf = std::make_unique<hzl_amd::EnkfState>(
-            mode, hzl_amd::stream_seed(seed, (uint64_t)v.B));
+            mode, hzl_amd::stream_seed(seed, (uint64_t)v.B), v.stream);
     });
 }
 
@@ -863,14 +1067,14 @@ int hzl_enkf_assimilate(hzl_ensemble_t h, double dt, size_t steps, size_t nobs, 
                 !std::isfinite(sd[o]) || sd[o] <= 0)
                 throw std::invalid_argument("observations must be finite, in bounds, and have "
                                             "positive uncertainty");
-        if (steps)
-            e.step(dt, steps);
         if (v.precision == HZL_FP32)
             hzl_amd::Enkf<float>(e, st).assimilate(nobs, x, y, value, sd, inflation,
-                                                   localization_radius, record, mean_field);
+                                                   localization_radius, dt, steps, record,
+                                                   mean_field);
         else
             hzl_amd::Enkf<double>(e, st).assimilate(nobs, x, y, value, sd, inflation,
-                                                    localization_radius, record, mean_field);
+                                                    localization_radius, dt, steps, record,
+                                                    mean_field);
     });
 }
 
@@ -880,7 +1084,7 @@ int hzl_ensemble_mean_spread(hzl_ensemble_t h, double *spread, double *mean_fiel
         const hzl_amd::EngineViews v = e.views();
         if (v.B < 2)
             throw std::invalid_argument("spread needs at least two members");
-        hzl_amd::EnkfState scratch(HZL_ENKF_SERIAL, 0);
+        hzl_amd::EnkfState scratch(HZL_ENKF_SERIAL, v.stream);
         double s = 0;
         if (v.precision == HZL_FP32) {
             hzl_amd::Enkf<float> k(e, scratch);

@@ -1154,6 +1154,78 @@ template <typename R> struct DiagArgs {
     double *out; // [B][6]: energy, enstrophy, dissipation, injection, cfl_rate, unused
 };
 
+// The reference NormalStream (src/solver.cpp:41-54) continued on the device
+// by one workgroup of nt lanes: `count` normals of the stream whose state is
+// mt[0..311] (mt19937_64 words), mt[312] (position) and sp = {has_spare,
+// spare}, written to out[0..count) in stream order, the state advanced.
+// Normals come in Box-Muller pairs (r cos, r sin) from two consecutive outputs;
+// a stream that holds a kept r sin returns it first, and an odd remainder
+// keeps the last pair's r sin.  Every pair takes two outputs, so the position
+// stays even and a pair never straddles a block of 312; the block's pairs are
+// drawn in parallel (tempering only reads the words) and the block is
+// regenerated in two parallel phases, words 0..155 from old words, then
+// 156..311 from the new words 0..155, the sequential recurrence's order.
+// s: 312 words of workgroup scratch.
+template <class Ctx>
+HZL_HD void normal_stream_body(const Ctx &ctx, uint64_t *mt, double *sp, long count, double *out,
+                               uint64_t *s, int nt) {
+    const int lane = ctx.tid();
+    if (count <= 0)
+        return;
+    for (int i = lane; i < MT_N; i += nt)
+        s[i] = mt[i];
+    int pos = (int)mt[MT_N];
+    const long first = sp[0] != 0 ? 1 : 0; // the kept r sin goes first
+    if (first && lane == 0)
+        out[0] = sp[1];
+    const long rest = count - first, pairs = (rest + 1) / 2;
+    ctx.sync();
+    long done = 0;
+    while (done < pairs) {
+        if (pos >= MT_N) {
+            uint64_t nv[(MT_N / 2 + 31) / 32];
+            int c = 0;
+            for (int i = lane; i < MT_N / 2; i += nt)
+                nv[c++] = s[i + MT_N / 2] ^ mt_mix(s[i], s[i + 1]);
+            ctx.sync();
+            c = 0;
+            for (int i = lane; i < MT_N / 2; i += nt)
+                s[i] = nv[c++];
+            ctx.sync();
+            c = 0;
+            for (int i = MT_N / 2 + lane; i < MT_N; i += nt)
+                nv[c++] = s[i - MT_N / 2] ^ mt_mix(s[i], s[(i + 1) % MT_N]);
+            ctx.sync();
+            c = 0;
+            for (int i = MT_N / 2 + lane; i < MT_N; i += nt)
+                s[i] = nv[c++];
+            ctx.sync();
+            pos = 0;
+        }
+        const long avail = (MT_N - pos) / 2, take = pairs - done < avail ? pairs - done : avail;
+        for (long p = lane; p < take; p += nt) {
+            const double u1 = mt_open_uniform(mt_temper(s[pos + 2 * p]));
+            const double u2 = mt_open_uniform(mt_temper(s[pos + 2 * p + 1]));
+            const double r = sqrt(-2 * log(u1)), angle = 2 * 3.14159265358979323846 * u2;
+            const long k = first + 2 * (done + p);
+            out[k] = r * cos(angle);
+            if (k + 1 < count)
+                out[k + 1] = r * sin(angle);
+            else
+                sp[1] = r * sin(angle); // odd remainder: kept for the next call
+        }
+        pos += 2 * (int)take;
+        done += take;
+    }
+    ctx.sync();
+    for (int i = lane; i < MT_N; i += nt)
+        mt[i] = s[i];
+    if (lane == 0) {
+        mt[MT_N] = (uint64_t)pos;
+        sp[0] = (rest % 2 == 1) ? 1.0 : 0.0;
+    }
+}
+
 // floor(sqrt(x)) and ceil(sqrt(x)) of a nonnegative integer, exact.
 HZL_HD long isqrt_floor(long x) {
     long r = (long)sqrt((double)x);

@@ -1195,7 +1195,7 @@ class Surrogate {
     Buf<double> hnu_, ds_nu_, bnu_, lsum_, spread_;
     double error_rms_ = -1;
     Buf<uint64_t> counter_dev_;
-    bool use_graph_ = true, warm_ = false;
+    bool use_graph_ = graphs_by_default(), warm_ = false;
     GraphKey graph_key_{}, warm_key_{};
     hipGraphExec_t graph_ = nullptr;
     Buf<int> bidx_;

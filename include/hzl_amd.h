@@ -236,7 +236,9 @@ int hzl_likelihood_adjoint(hzl_ensemble_t h, const double *initial_field, double
                            const double *value, const double *sd, const double *log_params,
                            size_t stride, double *objective, double *gradient, size_t *counts);
 
-/* Replay launch sequences from captured HIP graphs (default on).  A
+/* Replay launch sequences from captured HIP graphs (default on; a handle
+ * created while the environment holds HZL_GRAPHS=0 starts with replay off,
+ * as does a surrogate's forecast graph).  A
  * hzl_ensemble_step of 2..200 steps (longer calls in chunks of 200), and a
  * whole particle-filter cycle of hzl_pf_assimilate with 1..200 steps
  * (forecast, log-weights, normalisation, resampling copies, which are

@@ -561,6 +561,27 @@ int emu_ou(int B, int npairs, const int *map, const uint64_t *keys, const double
     return 0;
 }
 
+// The device NormalStream (kernels.hpp normal_stream_body) on `lanes` CPU
+// lanes: a stream seeded like std::mt19937_64(seed), then `calls` calls of
+// counts[c] normals each, written one after another to out.
+int emu_normal_stream(uint64_t seed, int lanes, int calls, const long *counts, double *out) {
+    std::vector<uint64_t> mt(MT_N + 1);
+    mt[0] = seed;
+    for (int i = 1; i < MT_N; ++i)
+        mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+    mt[MT_N] = MT_N;
+    double sp[2] = {0, 0};
+    long offset = 0;
+    for (int c = 0; c < calls; ++c) {
+        launch(1, lanes, sizeof(uint64_t) * MT_N, [&](const CpuCtx &ctx, long, unsigned char *lds) {
+            normal_stream_body(ctx, mt.data(), sp, counts[c], out + offset,
+                               reinterpret_cast<uint64_t *>(lds), lanes);
+        });
+        offset += counts[c];
+    }
+    return 0;
+}
+
 int emu_proj_head(int bf16, long M, int C, int H, const float *X, const float *Q1,
                   const float *q1, const float *Q2, const float *q2, float *out) {
     if (bf16)

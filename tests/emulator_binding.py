@@ -40,6 +40,8 @@ class Emulator:
         L.emu_ou.restype = C.c_int
         L.emu_ou.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint64), _dp,
                              _dp, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp]
+        L.emu_normal_stream.restype = C.c_int
+        L.emu_normal_stream.argtypes = [C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_long), _dp]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
 
@@ -75,6 +77,15 @@ class Emulator:
                                     Q1.ctypes.data_as(fp), q1.ctypes.data_as(fp),
                                     Q2.ctypes.data_as(fp), q2.ctypes.data_as(fp),
                                     out.ctypes.data_as(fp))
+        assert rc == 0
+        return out
+
+    def normal_stream(self, seed, counts, lanes=64):
+        """normal_stream_body: the normals of successive calls of counts[c] each."""
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        out = np.zeros(int(cnt.sum()))
+        rc = self.lib.emu_normal_stream(seed, lanes, len(cnt), cnt.ctypes.data_as(C.POINTER(C.c_long)),
+                                        out.ctypes.data_as(_dp))
         assert rc == 0
         return out
 

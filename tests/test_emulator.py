@@ -116,3 +116,18 @@ def test_ou_body_matches_oracle(emu, orc, n, kmin, kmax, members, calls):
             if modes[c] == 1:
                 got = np.where(conj, np.conj(noise[c, m][pmap]), noise[c, m][pmap])
                 assert np.abs(got - want).max() <= 1e-15 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("lanes", [64, 256])
+def test_device_normal_stream_is_the_reference_stream(emu, orc, lanes):
+    """normal_stream_body (the EnKF's perturbations on the device) continues
+    one NormalStream across calls of any size: a kept r sin first, odd
+    remainders, several 312-word blocks in one call, empty calls; every draw
+    equals the oracle's NormalStream (itself the reference's, src/solver.cpp:41-54)."""
+    seed = orc.stream_seed(77, 1024)
+    counts = [1, 7, 0, 311, 312, 313, 2, 1000, 5, 156, 157]
+    got = emu.normal_stream(seed, counts, lanes)
+    st = orc.mt(seed)
+    want = np.array([orc.normal(st) for _ in range(sum(counts))])
+    # same draws; libm sin / cos may differ by an ulp (a misordered draw is O(1))
+    np.testing.assert_allclose(got, want, rtol=1e-15, atol=1e-15)
