@@ -1486,6 +1486,16 @@ template <typename R> class Engine final : public EngineBase {
     // filter spanning several ensembles), on this ensemble's stream.
     void pf_logweights(size_t nobs, const int *x, const int *y, const double *value,
                        const double *sd, const double *wprev, double *logw) override {
+        pf_stage_logweights(nobs, x, y, value, sd);
+        if (!mix_valid_) {
+            run_col(0, stage_args(1.0, nullptr, 0));
+            mix_valid_ = true;
+        }
+        pf_enqueue_logweights(nobs, wprev, logw);
+    }
+
+    void pf_stage_logweights(size_t nobs, const int *x, const int *y, const double *value,
+                             const double *sd) override {
         if (nobs == 0)
             throw std::invalid_argument("observation group is empty");
         for (size_t o = 0; o < nobs; ++o)
@@ -1494,10 +1504,6 @@ template <typename R> class Engine final : public EngineBase {
                 throw std::invalid_argument("observations must be finite, in bounds, and have "
                                             "positive uncertainty");
         flush_parameters();
-        if (!mix_valid_) {
-            run_col(0, stage_args(1.0, nullptr, 0));
-            mix_valid_ = true;
-        }
         if (pf_cs_.count != (size_t)n_) {
             check(hipStreamSynchronize(stream_), "table growth");
             clear_graphs();
@@ -1512,11 +1518,13 @@ template <typename R> class Engine final : public EngineBase {
             upload(pf_sn_.p, sn.data(), n_);
         }
         stage_observations(nobs, x, y, value, sd);
+    }
+
+    void pf_enqueue_logweights(size_t nobs, const double *wprev, double *logw) override {
         PfArgs<R> a{(int)B_,  n_,       K_,       mix_.p,    (int)nobs,
                     obs_x_,   obs_y_,   obs_v_,   obs_sd_,   pf_cs_.p,
                     pf_sn_.p, const_cast<double *>(wprev), nullptr, logw, nullptr};
         hipLaunchKernelGGL(k_pf_loglik<R>, dim3((unsigned)B_), dim3(256), 0, stream_, a);
-        check(hipGetLastError(), "particle log-weights");
     }
 
     size_t pf_filters() const override {

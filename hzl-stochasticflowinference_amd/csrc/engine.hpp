@@ -134,6 +134,12 @@ class EngineBase {
     virtual size_t pf_filters() const = 0;
     virtual void pf_logweights(size_t nobs, const int *x, const int *y, const double *value,
                                const double *sd, const double *wprev, double *logw) = 0;
+    // The same in two parts for a joint filter's captured cycle: check and
+    // stage one observation group (eager), then launch the log-weight kernel
+    // on the current mixed fields (capturable).
+    virtual void pf_stage_logweights(size_t nobs, const int *x, const int *y,
+                                     const double *value, const double *sd) = 0;
+    virtual void pf_enqueue_logweights(size_t nobs, const double *wprev, double *logw) = 0;
     virtual void fill_state(const double *field) = 0;
     virtual void set_step_index(uint64_t step) = 0;
     virtual uint64_t step_index() const = 0;

@@ -14,6 +14,14 @@
 // of the forward-normalised coefficients, which keeps the physical field's
 // resolved part).  Observation grid indices are given on the coarsest grid;
 // an ensemble r times finer reads the point (r x, r y).
+//
+// A cycle is one launch sequence over all the streams: from the filter's own
+// stream it forks to every ensemble's stream (mixed fields, RK4 steps,
+// log-weights) and joins back for the normalisation, resampling gather and key
+// scatter.  With graph replay on it is captured as one HIP graph (fork / join
+// through events) on its second identical call and replayed afterwards; the
+// group index and last-group flag are read from device memory, and the host
+// waits once per cycle, for the four record values.
 #include "engine.hpp"
 #include "gemm_launch.hpp"
 
@@ -73,6 +81,14 @@ __global__ void k_pf_multi_copy(const double *stats, const cx<R> *src, cx<R> *ds
         dst[i] = src[i];
 }
 
+// The group index and last-group flag of the coming normalisation.
+__global__ void k_set_cycle(uint64_t *cycle, uint64_t group, uint64_t is_last) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cycle[0] = group;
+        cycle[1] = is_last;
+    }
+}
+
 __global__ void k_copy_keys(const uint64_t *src, uint64_t *dst, long n) {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i < n)
@@ -106,21 +122,38 @@ class MultiFilter {
         }
         if (total_ < 2)
             throw std::invalid_argument("particle filter needs at least two particles");
+        // device addresses, sizes and streams are fixed for an ensemble's
+        // lifetime: read once, so launch sequences touch no engine state
+        for (EngineBase *e : engs_)
+            views_.push_back(e->views());
         hcheck(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
         hcheck(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
+        hcheck(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
         for (size_t e = 0; e < engs_.size(); ++e) {
-            hipEvent_t ev;
+            hipEvent_t ev, ej;
             hcheck(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+            hcheck(hipEventCreateWithFlags(&ej, hipEventDisableTiming), "hipEventCreate");
             ready_.push_back(ev);
+            join_.push_back(ej);
         }
+        hcheck(hipHostMalloc(reinterpret_cast<void **>(&host_stats_), 8 * sizeof(double)),
+               "hipHostMalloc");
     }
     ~MultiFilter() {
         if (st_)
             (void)hipStreamSynchronize(st_);
+        if (exec_)
+            (void)hipGraphExecDestroy(exec_);
         for (hipEvent_t e : ready_)
+            (void)hipEventDestroy(e);
+        for (hipEvent_t e : join_)
             (void)hipEventDestroy(e);
         if (done_)
             (void)hipEventDestroy(done_);
+        if (fork_)
+            (void)hipEventDestroy(fork_);
+        if (host_stats_)
+            (void)hipHostFree(host_stats_);
         if (st_)
             (void)hipStreamDestroy(st_);
     }
@@ -155,6 +188,12 @@ class MultiFilter {
         wnorm_.alloc(P);
         logw_.alloc(P);
         anc_.alloc(P);
+        cycle_.alloc(2);
+        size_t need = 0; // resampling scratch: every child's retained modes
+        for (const EngineViews &v : views_)
+            need += (size_t)v.B * (v.K + 1) * (2 * v.K + 1);
+        scratch_.alloc(need * (views_[0].precision == 0 ? sizeof(cx<float>) : sizeof(cx<double>)) /
+                           sizeof(cx<double>) + 1);
         scatter_keys();
         hcheck(hipStreamSynchronize(st_), "joint filter start");
         on_ = true;
@@ -172,45 +211,77 @@ class MultiFilter {
         if (!is_last && group_ >= (uint64_t)max_uniforms_)
             throw std::invalid_argument(
                 "more observation groups than the filter was started for (max_groups)");
-        // forecasts: launched on every ensemble's stream before any waits
-        if (steps)
-            for (EngineBase *e : engs_)
-                e->step(dt, steps);
+        // eager preparation on every ensemble's stream: observations staged at
+        // its resolution, time-step checks and the device step counter
         std::vector<int> xs(nobs), ys(nobs);
         for (size_t k = 0; k < engs_.size(); ++k) {
-            const EngineViews v = engs_[k]->views();
-            const int r = v.n / coarse_;
+            const int r = views_[k].n / coarse_;
             for (size_t o = 0; o < nobs; ++o) {
                 xs[o] = x[o] * r;
                 ys[o] = y[o] * r;
             }
-            engs_[k]->pf_logweights(nobs, xs.data(), ys.data(), value, sd, wstate_.p + first_[k],
-                                    logw_.p + first_[k]);
-            hcheck(hipEventRecord(ready_[k], v.stream), "event");
+            engs_[k]->pf_stage_logweights(nobs, xs.data(), ys.data(), value, sd);
+            if (steps)
+                engs_[k]->prepare_steps(dt);
+        }
+        hipLaunchKernelGGL(k_set_cycle, dim3(1), dim3(64), 0, st_, cycle_.p, group_,
+                           (uint64_t)(is_last ? 1 : 0));
+        for (size_t k = 0; k < engs_.size(); ++k) {
+            hcheck(hipEventRecord(ready_[k], views_[k].stream), "event");
             hcheck(hipStreamWaitEvent(st_, ready_[k], 0), "event wait");
         }
-        PfControl c{(int)total_, logw_.p, wstate_.p,      wnorm_.p,       stats_.p,
-                    anc_.p,      uniforms_.p, max_uniforms_, keys_.p,       seeds_.p,
-                    group_,      is_last ? 1 : 0};
-        hipLaunchKernelGGL(k_pf_multi_normalize, dim3(1), dim3(256), 0, st_, c);
-        const EngineViews v0 = engs_[0]->views();
-        if (v0.precision == 0)
-            gather<float>();
-        else
-            gather<double>();
-        scatter_keys();
-        hcheck(hipGetLastError(), "joint filter kernels");
-        double st[8];
-        hcheck(hipMemcpyAsync(st, stats_.p, sizeof(st), hipMemcpyDeviceToHost, st_), "stats");
-        hcheck(hipEventRecord(done_, st_), "event");
-        for (EngineBase *e : engs_) {
-            hcheck(hipStreamWaitEvent(e->views().stream, done_, 0), "event wait");
-            e->invalidate_mixed();
+        Key key{dt, steps, nobs, alloc_generation(), {}};
+        for (size_t k = 0; k < engs_.size(); ++k)
+            key.epochs[k] = engs_[k]->launch_epoch();
+        bool graphs = true;
+        for (EngineBase *e : engs_)
+            graphs = graphs && e->graphs_enabled();
+        if (graphs && exec_ && exec_key_ == key) {
+            hcheck(hipGraphLaunch(exec_, st_), "joint graph launch");
+        } else if (graphs && warm_ && warm_key_ == key) {
+            if (exec_) {
+                (void)hipGraphExecDestroy(exec_);
+                exec_ = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            hcheck(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
+            try {
+                launches(dt, steps, nobs);
+            } catch (...) {
+                (void)hipStreamEndCapture(st_, &g);
+                if (g)
+                    (void)hipGraphDestroy(g);
+                throw;
+            }
+            hcheck(hipStreamEndCapture(st_, &g), "capture end");
+            const hipError_t e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hcheck(e, "joint graph instantiate");
+            exec_key_ = key;
+            warm_ = false;
+            hcheck(hipGraphLaunch(exec_, st_), "joint graph launch");
+        } else {
+            launches(dt, steps, nobs);
+            warm_ = graphs;
+            warm_key_ = key;
+            warm_key_.generation = alloc_generation(); // after this run's allocations
         }
-        hcheck(hipStreamSynchronize(st_), "joint filter");
+        hcheck(hipGetLastError(), "joint filter kernels");
+        // every ensemble's next work waits for the resampling copies
+        hcheck(hipEventRecord(done_, st_), "event");
+        for (size_t k = 0; k < engs_.size(); ++k) {
+            hcheck(hipStreamWaitEvent(views_[k].stream, done_, 0), "event wait");
+            if (steps)
+                engs_[k]->commit_steps(dt, steps);
+            engs_[k]->invalidate_mixed();
+        }
+        hcheck(hipMemcpyAsync(host_stats_, stats_.p, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                              st_),
+               "stats");
+        hcheck(hipStreamSynchronize(st_), "joint filter"); // the cycle's one wait
         if (record)
             for (int k = 0; k < 4; ++k)
-                record[k] = st[k];
+                record[k] = host_stats_[k];
         ++group_;
     }
 
@@ -235,9 +306,34 @@ class MultiFilter {
         hcheck(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, st_), "upload");
     }
 
+    // One cycle's launches (capturable): fork to the ensembles' streams for
+    // their mixed fields, steps and log-weights, join back for the
+    // normalisation, the resampling gather and the key scatter.
+    void launches(double dt, size_t steps, size_t nobs) {
+        hcheck(hipEventRecord(fork_, st_), "event");
+        for (size_t k = 0; k < engs_.size(); ++k) {
+            EngineBase *e = engs_[k];
+            const hipStream_t s = views_[k].stream;
+            hcheck(hipStreamWaitEvent(s, fork_, 0), "event wait");
+            e->enqueue_steps(dt, steps);
+            e->pf_enqueue_logweights(nobs, wstate_.p + first_[k], logw_.p + first_[k]);
+            hcheck(hipEventRecord(join_[k], s), "event");
+            hcheck(hipStreamWaitEvent(st_, join_[k], 0), "event wait");
+        }
+        PfControl c{(int)total_, logw_.p, wstate_.p,      wnorm_.p,       stats_.p,
+                    anc_.p,      uniforms_.p, max_uniforms_, keys_.p,       seeds_.p,
+                    0,           0,           cycle_.p};
+        hipLaunchKernelGGL(k_pf_multi_normalize, dim3(1), dim3(256), 0, st_, c);
+        if (views_[0].precision == 0)
+            gather<float>();
+        else
+            gather<double>();
+        scatter_keys();
+    }
+
     void scatter_keys() {
         for (size_t k = 0; k < engs_.size(); ++k) {
-            const EngineViews v = engs_[k]->views();
+            const EngineViews &v = views_[k];
             hipLaunchKernelGGL(k_copy_keys, grid1d(v.B), dim3(256), 0, st_, keys_.p + first_[k],
                                v.keys, (long)v.B);
         }
@@ -247,7 +343,7 @@ class MultiFilter {
         MultiSources<R> src{};
         src.count = (int)engs_.size();
         for (size_t k = 0; k < engs_.size(); ++k) {
-            const EngineViews v = engs_[k]->views();
+            const EngineViews &v = views_[k];
             src.w[k] = reinterpret_cast<const cx<R> *>(v.w0);
             src.K[k] = v.K;
             src.first[k] = (long)first_[k];
@@ -257,7 +353,7 @@ class MultiFilter {
         std::vector<long> counts(engs_.size());
         size_t need = 0;
         for (size_t k = 0; k < engs_.size(); ++k) {
-            const EngineViews v = engs_[k]->views();
+            const EngineViews &v = views_[k];
             counts[k] = (long)v.B * (v.K + 1) * (2 * v.K + 1);
             need += (size_t)counts[k];
         }
@@ -265,14 +361,14 @@ class MultiFilter {
         cx<R> *base = reinterpret_cast<cx<R> *>(scratch_.p);
         long off = 0;
         for (size_t k = 0; k < engs_.size(); ++k) {
-            const EngineViews v = engs_[k]->views();
+            const EngineViews &v = views_[k];
             hipLaunchKernelGGL(k_pf_multi_gather<R>, grid1d(counts[k]), dim3(256), 0, st_, src,
                                (int)k, (long)v.B, v.K, anc_.p, stats_.p, base + off);
             off += counts[k];
         }
         off = 0;
         for (size_t k = 0; k < engs_.size(); ++k) {
-            const EngineViews v = engs_[k]->views();
+            const EngineViews &v = views_[k];
             hipLaunchKernelGGL(k_pf_multi_copy<R>, grid1d(counts[k]), dim3(256), 0, st_,
                                stats_.p, base + off, reinterpret_cast<cx<R> *>(v.w0), counts[k]);
             off += counts[k];
@@ -280,12 +376,32 @@ class MultiFilter {
     }
 
     std::vector<EngineBase *> engs_;
+    std::vector<EngineViews> views_;
     std::vector<size_t> first_;
     size_t total_ = 0;
     int coarse_ = 0;
     hipStream_t st_ = nullptr;
-    hipEvent_t done_ = nullptr;
-    std::vector<hipEvent_t> ready_;
+    hipEvent_t done_ = nullptr, fork_ = nullptr;
+    std::vector<hipEvent_t> ready_, join_; // before the cycle (eager), inside it (captured)
+    double *host_stats_ = nullptr;
+    Buf<uint64_t> cycle_; // group index, last-group flag
+    // the captured cycle: key of its launch sequence
+    struct Key {
+        double dt;
+        size_t steps, nobs;
+        uint64_t generation;
+        uint64_t epochs[MULTI_MAX];
+        bool operator==(const Key &o) const {
+            for (int k = 0; k < MULTI_MAX; ++k)
+                if (epochs[k] != o.epochs[k])
+                    return false;
+            return dt == o.dt && steps == o.steps && nobs == o.nobs &&
+                   generation == o.generation;
+        }
+    };
+    hipGraphExec_t exec_ = nullptr;
+    Key exec_key_{}, warm_key_{};
+    bool warm_ = false;
     Buf<double> wstate_, wnorm_, logw_, stats_, uniforms_;
     Buf<int> anc_;
     Buf<uint64_t> keys_, seeds_;

@@ -114,3 +114,36 @@ def test_enkf_cycle_replays_match_eager(pkg, case):
         a.step(dt, 3)
         b.step(dt, 3)
         assert np.array_equal(a.get_state(), b.get_state())
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_joint_filter_cycle_replays_match_eager(pkg, precision):
+    """The mixed-grid filter's cycle (fork to both grids' streams for steps and
+    log-weights, join for normalisation and the cross-grid gather) replays one
+    captured graph; every group's record and the final states equal the run
+    with graph replay off, bit for bit."""
+    rng = np.random.default_rng(7)
+    rows = []
+    for g in range(6):
+        xs, ys = rng.integers(0, 32, 12), rng.integers(0, 32, 12)
+        sd = 0.05 if g % 2 == 0 else 5.0
+        rows += [((g + 1) * 4, x, y, v, sd) for x, y, v in zip(xs, ys, rng.normal(0, 0.3, 12))]
+    rows = np.array(rows, dtype=np.float64)
+    runs = []
+    for graphs in (True, False):
+        with pkg.Ensemble(32, 10, precision, nu=2e-3, sigma=0.2, amp=0.1, seed=1) as a, \
+                pkg.Ensemble(64, 6, precision, nu=2e-3, sigma=0.2, amp=0.1, seed=2) as b:
+            a.init_grf()
+            b.init_grf()
+            a.set_graph(graphs)
+            b.set_graph(graphs)
+            res = pkg.joint_bootstrap_filter([a, b], rows, 2e-3, 13)
+            runs.append((res, a.get_state(), b.get_state(), a.step_index, b.step_index))
+    (ra, a1, b1, ia, ib), (rb, a2, b2, ja, jb) = runs
+    flags = [h["resampled"] for h in rb["history"]]
+    assert any(flags) and (ia, ib) == (ja, jb) == (24, 24)
+    assert ra["log_likelihood"] == rb["log_likelihood"]
+    for ha, hb in zip(ra["history"], rb["history"]):
+        for k in ("increment", "ess", "max_weight", "resampled"):
+            assert ha[k] == hb[k], k
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
