@@ -188,6 +188,11 @@ EngineBase &engine_of(hzl_ensemble_t h);
 // Message of the last failed C-ABI call on the calling thread.
 std::string &error_slot();
 
+// Advance every member of `eng` over steps * dt with a trained surrogate
+// network (csrc/surrogate.hip; as hzl_surrogate_forecast).
+void surrogate_forecast(hzl_surrogate_t s, EngineBase &eng, double dt, size_t steps,
+                        int precision);
+
 // Run f, mapping exceptions to C-ABI codes: std::invalid_argument ->
 // HZL_EINVAL, anything else -> HZL_ERUNTIME (message kept in error_slot()).
 template <class F> int guarded(F &&f) {

@@ -90,11 +90,14 @@ struct Model {
     double friction, sigma;    // fixed under the log-uniform prior
     double dt;
     int K;
+    hzl_surrogate_t surrogate; // forecast model of every interval (null: the solver)
+    int surrogate_precision;
 
     explicit Model(const hzl_pmcmc_options &o, double step, int k)
         : prior(o.prior), dims(o.prior == HZL_PRIOR_STOCHASTIC ? 4 : 2),
           lower{o.log_lower[0], o.log_lower[1]}, upper{o.log_upper[0], o.log_upper[1]},
-          friction(o.friction), sigma(o.forcing_sigma), dt(step), K(k) {
+          friction(o.friction), sigma(o.forcing_sigma), dt(step), K(k), surrogate(o.surrogate),
+          surrogate_precision(o.surrogate_precision) {
         if (prior != HZL_PRIOR_LOG_UNIFORM && prior != HZL_PRIOR_STOCHASTIC)
             throw std::invalid_argument("prior must be HZL_PRIOR_LOG_UNIFORM or HZL_PRIOR_STOCHASTIC");
         if (prior == HZL_PRIOR_LOG_UNIFORM) {
@@ -174,8 +177,13 @@ void evaluate(EngineBase &eng, const Model &model, const Window &w, const double
     size_t now = 0;
     for (size_t g = 0; g < w.step.size(); ++g) {
         const size_t b = w.begin[g], e = w.begin[g + 1];
-        eng.pf_assimilate(model.dt, w.step[g] - now, e - b, w.x + b, w.y + b, w.value + b,
-                          w.sd + b, g + 1 == w.step.size(), nullptr, nullptr);
+        size_t steps = w.step[g] - now;
+        if (model.surrogate && steps) {
+            surrogate_forecast(model.surrogate, eng, model.dt, steps, model.surrogate_precision);
+            steps = 0; // the network advanced the particles
+        }
+        eng.pf_assimilate(model.dt, steps, e - b, w.x + b, w.y + b, w.value + b, w.sd + b,
+                          g + 1 == w.step.size(), nullptr, nullptr);
         now = w.step[g];
     }
     ll.assign(F, NEG_INF);

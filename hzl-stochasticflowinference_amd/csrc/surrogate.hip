@@ -6,7 +6,7 @@
 // forecast of the filter loops (src/filter.cpp:46-48, src/inference.cpp:555-590).
 //
 // Network on an internal G x G grid (G = 64 for a 128^2 state):
-//   features [s][y][x][3] = (w * in_scale, log(nu / 1e-3), sin(kf y))
+//   features [s][y][x][3] = (w * in_scale, log(nu / 1e-3), (amp / 0.1) sin(kf y))
 //   lift   v0 = feat Lw^T + Lb                                    (3 -> C)
 //   layer  v' = act(IDFT(R . DFT(v)) + v W^T + b)   act = GELU, none on the last
 //          DFT keeps kx in [0, KX), ky in [-KY/2, KY/2); R[mode] is C x C complex
@@ -50,8 +50,9 @@ template <typename T> __global__ void __launch_bounds__(256) k_proj_head(HeadArg
 #define HZL_GID ((long)blockIdx.x * 256 + threadIdx.x)
 
 template <typename T>
-__global__ void k_feature(T *f, const double *nu, int S, int G, int CIN, int kf) {
-    feature_body<T>(f, nu, S, G, CIN, kf, HZL_GID);
+__global__ void k_feature(T *f, const double *nu, const double *amp, int S, int G, int CIN,
+                          int kf) {
+    feature_body<T>(f, nu, amp, S, G, CIN, kf, HZL_GID);
 }
 template <typename T> __global__ void k_gelu_back(const T *go, const T *pre, T *g, long n) {
     gelu_back_body<T>(go, pre, g, n, HZL_GID);
@@ -101,6 +102,23 @@ __global__ void k_gather_states(const cx<float> *src, const int *idx, long spec,
 __global__ void k_diff_states(const cx<float> *a, const cx<float> *b, cx<float> *d, long n) {
     diff_states_body(a, b, d, n, HZL_GID);
 }
+// part[e] = sum over members of w_e |a_m[e] - b_m[e]|^2 (w = 2 for kx > 0):
+// with the sum over e, the members' total squared grid-RMS difference.
+__global__ void k_sqdiff_modes(const cx<float> *a, const cx<float> *b, int B, int K,
+                               double *part) {
+    const int J = 2 * K + 1;
+    const long spec = (long)(K + 1) * J, e = HZL_GID;
+    if (e >= spec)
+        return;
+    double s = 0;
+    for (int m = 0; m < B; ++m) {
+        const cx<float> u = a[(long)m * spec + e], v = b[(long)m * spec + e];
+        const double dr = (double)u.x - (double)v.x, di = (double)u.y - (double)v.y;
+        s += dr * dr + di * di;
+    }
+    part[e] = (e / J > 0 ? 2.0 : 1.0) * s;
+}
+
 __global__ void k_gather_nu(const double *nu, const int *idx, double *out, int S) {
     const long i = HZL_GID;
     if (i < S)
@@ -361,28 +379,28 @@ class Surrogate {
 
     // ---- host-array entry points (reference Field layout) -----------------
     void forward_host(int precision, size_t S, const double *fields, const double *nu,
-                      double *grid) {
+                      const double *amp, double *grid) {
         check_count(S);
         check_precision(precision);
         st_ = own_;
         upload_fields(S, fields, hx_);
-        upload_nu(S, nu);
+        upload_conditions(S, nu, amp);
         if (precision == HZL_BF16) {
-            forward(wb_, hx_.p, hnu_.p, (int)S, false);
+            forward(wb_, hx_.p, hnu_.p, hamp_.p, (int)S, false);
             download_out(wb_, S, grid);
         } else {
-            forward(wf_, hx_.p, hnu_.p, (int)S, false);
+            forward(wf_, hx_.p, hnu_.p, hamp_.p, (int)S, false);
             download_out(wf_, S, grid);
         }
     }
     void loss_grad_host(size_t S, const double *fields, const double *next, const double *nu,
-                        double *loss, float *grad) {
+                        const double *amp, double *loss, float *grad) {
         check_count(S);
         st_ = own_;
         upload_fields(S, fields, hx_);
         upload_fields(S, next, hy_);
-        upload_nu(S, nu);
-        *loss = loss_and_grad(wf_, hx_.p, hy_.p, hnu_.p, (int)S, true);
+        upload_conditions(S, nu, amp);
+        *loss = loss_and_grad(wf_, hx_.p, hy_.p, hnu_.p, hamp_.p, (int)S, true);
         hcheck(hipMemcpy(grad, grad_.p, prm_.total * sizeof(float), hipMemcpyDeviceToHost),
                "gradient download");
     }
@@ -429,6 +447,9 @@ class Surrogate {
                                   v.stream),
                    "dataset copy");
             hcheck(hipMemcpyAsync(ds_nu_.p + pairs_, v.nu, B * sizeof(double),
+                                  hipMemcpyDeviceToDevice, v.stream),
+                   "dataset copy");
+            hcheck(hipMemcpyAsync(ds_amp_.p + pairs_, v.amp, B * sizeof(double),
                                   hipMemcpyDeviceToDevice, v.stream),
                    "dataset copy");
             if (scratch) {
@@ -479,7 +500,7 @@ class Surrogate {
             for (size_t b = 0; b < nb; ++b, ++t) {
                 std::copy(order.begin() + b * batch, order.begin() + (b + 1) * batch, idx.begin());
                 gather_batch(idx);
-                sum += loss_and_grad(wf_, bx_.p, by_.p, bnu_.p, (int)batch, true);
+                sum += loss_and_grad(wf_, bx_.p, by_.p, bnu_.p, bamp_.p, (int)batch, true);
                 const double f = 0.5 * (1 + std::cos(PI * (double)t / (double)total));
                 adam((float)(lr * (0.05 + 0.95 * f)));
             }
@@ -504,9 +525,10 @@ class Surrogate {
             for (size_t i = 0; i < S; ++i)
                 idx[i] = (int)(first + done + i);
             gather_batch(idx);
-            const double l = precision == HZL_BF16
-                                 ? loss_and_grad(wb_, bx_.p, by_.p, bnu_.p, (int)S, false)
-                                 : loss_and_grad(wf_, bx_.p, by_.p, bnu_.p, (int)S, false);
+            const double l =
+                precision == HZL_BF16
+                    ? loss_and_grad(wb_, bx_.p, by_.p, bnu_.p, bamp_.p, (int)S, false)
+                    : loss_and_grad(wf_, bx_.p, by_.p, bnu_.p, bamp_.p, (int)S, false);
             sum += l * (double)S;
             done += S;
         }
@@ -541,6 +563,56 @@ class Surrogate {
         for (double v : h)
             sum += v;
         return std::sqrt(sum);
+    }
+
+    // Live judge of the forecast error: the first J members of `eng` (J =
+    // members of `judge`) are copied into the judge ensemble with their nu,
+    // alpha and amplitude and no shell noise, forecast over the horizon by the
+    // network and, from the same states, by the plain solver; the grid RMS
+    // difference per member becomes the error estimate.  `eng` is unchanged.
+    double judge(EngineBase &eng, EngineBase &jg, double dt, size_t steps, int precision) {
+        const EngineViews ev = check_engine(eng), jv = check_engine(jg);
+        if (jv.B > ev.B)
+            throw std::invalid_argument("the judge ensemble has more members than the ensemble");
+        if (jg.forcing_process() != HZL_FORCING_WHITE)
+            throw std::invalid_argument("the judge ensemble runs the white forcing process");
+        const size_t J = (size_t)jv.B, spec = (size_t)s_.spec();
+        const size_t bytes = J * spec * sizeof(cx<float>);
+        hcheck(hipStreamSynchronize(ev.stream), "judge");
+        std::vector<double> nu(J), alpha(J), amp(J);
+        hcheck(hipMemcpy(nu.data(), ev.nu, J * sizeof(double), hipMemcpyDeviceToHost), "nu");
+        hcheck(hipMemcpy(alpha.data(), ev.alpha, J * sizeof(double), hipMemcpyDeviceToHost),
+               "alpha");
+        hcheck(hipMemcpy(amp.data(), ev.amp, J * sizeof(double), hipMemcpyDeviceToHost), "amp");
+        for (size_t m = 0; m < J; ++m)
+            jg.set_member_parameters(m, nu[m], alpha[m], 0.0, amp[m]);
+        judge_x_.alloc(J * spec);
+        judge_y_.alloc(J * spec);
+        spread_.alloc(spec);
+        hcheck(hipMemcpy(judge_x_.p, ev.w0, bytes, hipMemcpyDeviceToDevice), "judge states");
+        hcheck(hipMemcpy(jv.w0, judge_x_.p, bytes, hipMemcpyDeviceToDevice), "judge states");
+        jg.invalidate_mixed();
+        forecast(jg, dt, steps, precision); // sigma 0: the network's forecast alone
+        hcheck(hipMemcpyAsync(judge_y_.p, jv.w0, bytes, hipMemcpyDeviceToDevice, jv.stream),
+               "judge forecast");
+        hcheck(hipMemcpyAsync(jv.w0, judge_x_.p, bytes, hipMemcpyDeviceToDevice, jv.stream),
+               "judge states");
+        jg.rewind_counter(steps);
+        jg.invalidate_mixed();
+        jg.step(dt, steps);
+        hipLaunchKernelGGL(k_sqdiff_modes, grid1d((long)spec), dim3(256), 0, jv.stream,
+                           reinterpret_cast<const cx<float> *>(jv.w0), judge_y_.p, (int)J, s_.K,
+                           spread_.p);
+        std::vector<double> h(spec);
+        hcheck(hipMemcpyAsync(h.data(), spread_.p, spec * sizeof(double), hipMemcpyDeviceToHost,
+                              jv.stream),
+               "judge download");
+        jg.synchronize();
+        double sum = 0;
+        for (double v : h)
+            sum += v;
+        error_rms_ = std::sqrt(sum / (double)J);
+        return error_rms_;
     }
 
     // Forecast with the network, then compare its error estimate with the
@@ -593,7 +665,7 @@ class Surrogate {
             prepare_weights(wb_);
         else
             prepare_weights(wf_);
-        const GraphKey key{ev.w0,    ev.nu,       ev.alpha,  ev.sigma,   ev.keys,
+        const GraphKey key{ev.w0,    ev.nu,       ev.amp,    ev.alpha,   ev.sigma,  ev.keys,
                            ev.pair_of, ev.stream, ev.B,      ev.kmin,    ev.kmax,
                            precision, in_scale_,  out_scale_, (double)steps * dt,
                            alloc_generation()};
@@ -645,10 +717,10 @@ class Surrogate {
             const int S = std::min(chunk, ev.B - s0);
             cx<float> *w0 = reinterpret_cast<cx<float> *>(ev.w0) + (long)s0 * s_.spec();
             if (precision == HZL_BF16) {
-                forward(wb_, w0, ev.nu + s0, S, false);
+                forward(wb_, w0, ev.nu + s0, ev.amp + s0, S, false);
                 grid_to_increment(wb_.out.p, S);
             } else {
-                forward(wf_, w0, ev.nu + s0, S, false);
+                forward(wf_, w0, ev.nu + s0, ev.amp + s0, S, false);
                 grid_to_increment(wf_.out.p, S);
             }
             const long cnt = (long)S * s_.spec();
@@ -663,13 +735,14 @@ class Surrogate {
     // and stream, its sizes, the scales and horizon, and (through the
     // allocation generation) the surrogate's own buffer addresses.
     struct GraphKey {
-        const void *w0, *nu, *alpha, *sigma, *keys, *pair_of;
+        const void *w0, *nu, *amp, *alpha, *sigma, *keys, *pair_of;
         hipStream_t stream;
         int B, kmin, kmax, precision;
         double in_scale, out_scale, horizon;
         uint64_t generation;
         bool operator==(const GraphKey &o) const {
-            return w0 == o.w0 && nu == o.nu && alpha == o.alpha && sigma == o.sigma &&
+            return w0 == o.w0 && nu == o.nu && amp == o.amp && alpha == o.alpha &&
+                   sigma == o.sigma &&
                    keys == o.keys && pair_of == o.pair_of && stream == o.stream && B == o.B &&
                    kmin == o.kmin && kmax == o.kmax && precision == o.precision &&
                    in_scale == o.in_scale && out_scale == o.out_scale &&
@@ -858,7 +931,8 @@ class Surrogate {
     // Network output for S FP32 half-plane states into w.out (S x G x G).
     // With `train` the pre-activations and hidden layer are kept for backward.
     template <typename T>
-    void forward(FnoWork<T> &w, const cx<float> *states, const double *nu, int S, bool train) {
+    void forward(FnoWork<T> &w, const cx<float> *states, const double *nu, const double *amp,
+                 int S, bool train) {
         w.ensure(s_, S, prm_.total, train);
         prepare_weights(w);
         const FnoShape &s = s_;
@@ -867,8 +941,8 @@ class Surrogate {
         const long rper = 4L * C * C * s.modes();
         state_to_grid(states, S, vs(w.feat), (long)G * s.CIN, s.CIN, P * s.CIN, TY,
                       (float)in_scale_);
-        hipLaunchKernelGGL(k_feature<T>, grid1d(SP), dim3(256), 0, st_, w.feat.p, nu, S, G, s.CIN,
-                           s.kf);
+        hipLaunchKernelGGL(k_feature<T>, grid1d(SP), dim3(256), 0, st_, w.feat.p, nu, amp, S, G,
+                           s.CIN, s.kf);
         {
             Gemm<T> a((int)SP, C, s.CIN, cs(w.feat), s.CIN, 1, cs(w.w_T, prm_.Lw), s.CIN, 1,
                       vs(*w.v[0]), C, 1, TY);
@@ -937,8 +1011,8 @@ class Surrogate {
     // want_grad, every parameter gradient (FP32) into grad_.
     template <typename T>
     double loss_and_grad(FnoWork<T> &w, const cx<float> *x, const cx<float> *y, const double *nu,
-                         int S, bool want_grad) {
-        forward(w, x, nu, S, want_grad);
+                         const double *amp, int S, bool want_grad) {
+        forward(w, x, nu, amp, S, want_grad);
         const long n = (long)S * s_.spec(), SP = (long)S * s_.P();
         diff_.alloc((size_t)n);
         hipLaunchKernelGGL(k_diff_states, grid1d(n), dim3(256), 0, st_, y, x, diff_.p, n);
@@ -1094,12 +1168,21 @@ class Surrogate {
                          hipMemcpyHostToDevice),
                "input upload");
     }
-    void upload_nu(size_t S, const double *nu) {
+    // per-sample viscosity and Kolmogorov amplitude (amp null: the reference
+    // amplitude FNO_AMP_REF for every sample)
+    void upload_conditions(size_t S, const double *nu, const double *amp) {
         for (size_t i = 0; i < S; ++i)
-            if (!(nu[i] > 0) || !std::isfinite(nu[i]))
-                throw std::invalid_argument("viscosities must be positive and finite");
+            if (!(nu[i] > 0) || !std::isfinite(nu[i]) || (amp && !std::isfinite(amp[i])))
+                throw std::invalid_argument(
+                    "viscosities must be positive and amplitudes finite");
         hnu_.alloc(S);
+        hamp_.alloc(S);
+        std::vector<double> a(S, FNO_AMP_REF);
+        if (amp)
+            std::copy(amp, amp + S, a.begin());
         hcheck(hipMemcpy(hnu_.p, nu, S * sizeof(double), hipMemcpyHostToDevice), "nu upload");
+        hcheck(hipMemcpy(hamp_.p, a.data(), S * sizeof(double), hipMemcpyHostToDevice),
+               "amplitude upload");
     }
     template <typename T> void download_out(FnoWork<T> &w, size_t S, double *grid) {
         std::vector<float> h((size_t)S * s_.P());
@@ -1115,10 +1198,11 @@ class Surrogate {
             return;
         const size_t cap = std::max(pairs, 2 * cap_), spec = (size_t)s_.spec();
         Buf<cx<float>> x, y;
-        Buf<double> nu;
+        Buf<double> nu, amp;
         x.alloc(cap * spec);
         y.alloc(cap * spec);
         nu.alloc(cap);
+        amp.alloc(cap);
         if (pairs_) {
             hcheck(hipDeviceSynchronize(), "dataset grow");
             hcheck(hipMemcpy(x.p, ds_x_.p, pairs_ * spec * sizeof(cx<float>),
@@ -1129,6 +1213,8 @@ class Surrogate {
                    "dataset grow");
             hcheck(hipMemcpy(nu.p, ds_nu_.p, pairs_ * sizeof(double), hipMemcpyDeviceToDevice),
                    "dataset grow");
+            hcheck(hipMemcpy(amp.p, ds_amp_.p, pairs_ * sizeof(double), hipMemcpyDeviceToDevice),
+                   "dataset grow");
         }
         std::swap(x.p, ds_x_.p);
         std::swap(x.n, ds_x_.n);
@@ -1136,6 +1222,8 @@ class Surrogate {
         std::swap(y.n, ds_y_.n);
         std::swap(nu.p, ds_nu_.p);
         std::swap(nu.n, ds_nu_.n);
+        std::swap(amp.p, ds_amp_.p);
+        std::swap(amp.n, ds_amp_.n);
         cap_ = cap;
     }
 
@@ -1179,11 +1267,14 @@ class Surrogate {
         bx_.alloc((size_t)S * spec);
         by_.alloc((size_t)S * spec);
         bnu_.alloc(S);
+        bamp_.alloc(S);
         hipLaunchKernelGGL(k_gather_states, grid1d((long)S * spec), dim3(256), 0, st_, ds_x_.p,
                            bidx_.p, spec, bx_.p, S);
         hipLaunchKernelGGL(k_gather_states, grid1d((long)S * spec), dim3(256), 0, st_, ds_y_.p,
                            bidx_.p, spec, by_.p, S);
         hipLaunchKernelGGL(k_gather_nu, grid1d(S), dim3(256), 0, st_, ds_nu_.p, bidx_.p, bnu_.p, S);
+        hipLaunchKernelGGL(k_gather_nu, grid1d(S), dim3(256), 0, st_, ds_amp_.p, bidx_.p, bamp_.p,
+                           S);
     }
 
     FnoShape s_;
@@ -1191,8 +1282,8 @@ class Surrogate {
     hipStream_t own_ = nullptr, st_ = nullptr;
     Buf<float> master_, grad_, m_, v_;
     Buf<float> g1_, g2_, f1_, f2_, bm_, dgrid_, incr_, target_, part_, rgrad_;
-    Buf<cx<float>> diff_, hx_, hy_, ds_x_, ds_y_, bx_, by_, backup_;
-    Buf<double> hnu_, ds_nu_, bnu_, lsum_, spread_;
+    Buf<cx<float>> diff_, hx_, hy_, ds_x_, ds_y_, bx_, by_, backup_, judge_x_, judge_y_;
+    Buf<double> hnu_, ds_nu_, bnu_, hamp_, ds_amp_, bamp_, lsum_, spread_;
     double error_rms_ = -1;
     Buf<uint64_t> counter_dev_;
     bool use_graph_ = graphs_by_default(), warm_ = false;
@@ -1217,6 +1308,19 @@ struct hzl_surrogate {
 };
 
 using hzl_amd::guarded;
+
+namespace {
+hzl_amd::Surrogate &net_of(hzl_surrogate_t h);
+} // namespace
+
+namespace hzl_amd {
+void surrogate_forecast(hzl_surrogate_t s, EngineBase &eng, double dt, size_t steps,
+                        int precision) {
+    if (eng.forcing_process() == HZL_FORCING_OU)
+        throw std::invalid_argument("surrogate forecasts run with the white forcing process");
+    net_of(s).forecast(eng, dt, steps, precision);
+}
+} // namespace hzl_amd
 
 namespace {
 hzl_amd::Surrogate &net_of(hzl_surrogate_t h) {
@@ -1283,25 +1387,25 @@ int hzl_surrogate_get_scales(hzl_surrogate_t h, double *scales) {
 }
 
 int hzl_surrogate_forward(hzl_surrogate_t h, int precision, size_t count, const double *fields,
-                          const double *viscosity, double *grid_out) {
+                          const double *viscosity, const double *amplitude, double *grid_out) {
     return guarded([&] {
         need(fields);
         need(viscosity);
         need(grid_out);
-        net_of(h).forward_host(precision, count, fields, viscosity, grid_out);
+        net_of(h).forward_host(precision, count, fields, viscosity, amplitude, grid_out);
     });
 }
 
 int hzl_surrogate_loss_grad(hzl_surrogate_t h, size_t count, const double *fields,
-                            const double *next_fields, const double *viscosity, double *loss,
-                            float *grad) {
+                            const double *next_fields, const double *viscosity,
+                            const double *amplitude, double *loss, float *grad) {
     return guarded([&] {
         need(fields);
         need(next_fields);
         need(viscosity);
         need(loss);
         need(grad);
-        net_of(h).loss_grad_host(count, fields, next_fields, viscosity, loss, grad);
+        net_of(h).loss_grad_host(count, fields, next_fields, viscosity, amplitude, loss, grad);
     });
 }
 
@@ -1376,6 +1480,18 @@ int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, 
 
 int hzl_surrogate_set_graph(hzl_surrogate_t h, int enabled) {
     return guarded([&] { net_of(h).set_graph(enabled != 0); });
+}
+
+int hzl_surrogate_judge(hzl_surrogate_t h, hzl_ensemble_t ensemble, hzl_ensemble_t judge,
+                        double dt, size_t steps, int precision, double *rms) {
+    return guarded([&] {
+        hzl_amd::EngineBase &e = hzl_amd::engine_of(ensemble);
+        hzl_amd::EngineBase &j = hzl_amd::engine_of(judge);
+        require_white(e);
+        const double v = net_of(h).judge(e, j, dt, steps, precision);
+        if (rms)
+            *rms = v;
+    });
 }
 
 int hzl_surrogate_forecast(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt, size_t steps,

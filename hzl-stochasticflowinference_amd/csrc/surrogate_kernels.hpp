@@ -10,16 +10,20 @@ namespace hzl_amd {
 // Input features of pixel p of sample s (layout [s][p][CIN]):
 //   0: w(x, y) * in_scale (already written by the state->grid GEMM)
 //   1: (log nu_s - log 1e-3)
-//   2: sin(kf y) (the Kolmogorov forcing pattern)
+//   2: (amp_s / FNO_AMP_REF) sin(kf y), the sample's Kolmogorov forcing
+//      (amp null: FNO_AMP_REF for every sample)
+constexpr double FNO_AMP_REF = 0.1;
 template <typename T>
-HZL_HD void feature_body(T *feat, const double *nu, int S, int G, int CIN, int kf, long i) {
+HZL_HD void feature_body(T *feat, const double *nu, const double *amp, int S, int G, int CIN,
+                         int kf, long i) {
     const long P = (long)G * G;
     if (i >= S * P)
         return;
     const long s = i / P, p = i % P;
     const int y = (int)(p / G);
+    const double a = amp ? amp[s] / FNO_AMP_REF : 1.0;
     feat[i * CIN + 1] = (T)(float)(log(nu[s]) - log(1e-3));
-    feat[i * CIN + 2] = (T)(float)sin(2.0 * PI * kf * y / G);
+    feat[i * CIN + 2] = (T)(float)(a * sin(2.0 * PI * kf * y / G));
 }
 
 // g = g_out * gelu'(pre) elementwise

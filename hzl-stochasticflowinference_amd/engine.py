@@ -53,6 +53,7 @@ EXPORTS = (
     "hzl_surrogate_evaluate", "hzl_surrogate_error_estimate",
     "hzl_surrogate_set_error_estimate", "hzl_surrogate_spread",
     "hzl_surrogate_forecast_or_solve", "hzl_surrogate_set_graph", "hzl_surrogate_forecast",
+    "hzl_surrogate_judge",
 )
 
 
@@ -73,7 +74,8 @@ class PmcmcOptions(C.Structure):
                 ("proposal_scale", C.c_double), ("log_lower", C.c_double * 2),
                 ("log_upper", C.c_double * 2), ("friction", C.c_double),
                 ("forcing_sigma", C.c_double), ("seed", C.c_uint64), ("prior", C.c_int),
-                ("chain_seeds", C.POINTER(C.c_uint64))]
+                ("chain_seeds", C.POINTER(C.c_uint64)), ("surrogate", C.c_void_p),
+                ("surrogate_precision", C.c_int)]
 
 
 class HzlError(RuntimeError):
@@ -176,8 +178,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_surrogate_set_params.argtypes = [vp, fp]
     L.hzl_surrogate_set_scales.argtypes = [vp, C.c_double, C.c_double]
     L.hzl_surrogate_get_scales.argtypes = [vp, _dp]
-    L.hzl_surrogate_forward.argtypes = [vp, C.c_int, sz, _dp, _dp, _dp]
-    L.hzl_surrogate_loss_grad.argtypes = [vp, sz, _dp, _dp, _dp, _dp, fp]
+    L.hzl_surrogate_forward.argtypes = [vp, C.c_int, sz, _dp, _dp, _dp, _dp]
+    L.hzl_surrogate_loss_grad.argtypes = [vp, sz, _dp, _dp, _dp, _dp, _dp, fp]
+    L.hzl_surrogate_judge.argtypes = [vp, vp, vp, C.c_double, sz, C.c_int, _dp]
     L.hzl_surrogate_collect.argtypes = [vp, vp, C.c_double, sz, sz, C.c_int]
     L.hzl_surrogate_dataset_size.argtypes = [vp, C.POINTER(sz)]
     L.hzl_surrogate_fit.argtypes = [vp, sz, sz, sz, sz, C.c_double, C.c_uint64, _dp]
@@ -846,7 +849,8 @@ def ensemble_kalman_filter(members: Ensemble, observations, dt: float, seed: int
 def particle_metropolis(particles: Ensemble, observations, dt: float, initial_field,
                         initial_q, samples: int, proposal_scale: float, seed: int,
                         nu_bounds=(2e-4, 5e-3), amp_bounds=(0.02, 0.5), friction: float = 0.0,
-                        forcing_sigma: float = 0.05):
+                        forcing_sigma: float = 0.05, surrogate: "Surrogate | None" = None,
+                        surrogate_precision: str = "bf16"):
     """Batched pseudo-marginal random-walk Metropolis over q = (log nu,
     log amplitude), one chain per filter of members / chains particles
     (reference particle_metropolis, src/filter.cpp:166-203, with a
@@ -858,7 +862,9 @@ def particle_metropolis(particles: Ensemble, observations, dt: float, initial_fi
     o = PmcmcOptions(chains, samples, proposal_scale,
                      (C.c_double * 2)(np.log(nu_bounds[0]), np.log(amp_bounds[0])),
                      (C.c_double * 2)(np.log(nu_bounds[1]), np.log(amp_bounds[1])),
-                     friction, forcing_sigma, seed, PRIOR_LOG_UNIFORM, None)
+                     friction, forcing_sigma, seed, PRIOR_LOG_UNIFORM, None,
+                     surrogate._h.value if surrogate is not None else None,
+                     _SUR_PRECISION[surrogate_precision])
     step = np.ascontiguousarray(obs[:, 0], dtype=np.int32)
     x = np.ascontiguousarray(obs[:, 1], dtype=np.int32)
     y = np.ascontiguousarray(obs[:, 2], dtype=np.int32)
@@ -942,28 +948,44 @@ class Surrogate:
     def set_scales(self, input_scale: float, output_scale: float):
         _check(_lib.hzl_surrogate_set_scales(self._h, input_scale, output_scale))
 
-    def forward(self, fields, nu, precision: str = "fp32") -> np.ndarray:
+    def forward(self, fields, nu, precision: str = "fp32", amp=None) -> np.ndarray:
+        """Network output per sample; amp: Kolmogorov amplitudes (default 0.1)."""
         f = np.ascontiguousarray(fields, dtype=np.complex128)
         S = f.shape[0]
         nu = np.ascontiguousarray(np.broadcast_to(nu, (S,)), dtype=np.float64)
+        a = None if amp is None else np.ascontiguousarray(np.broadcast_to(amp, (S,)),
+                                                          dtype=np.float64)
         _fields(f, S, self.state_n)
         out = np.zeros((S, self.grid, self.grid))
         _check(_lib.hzl_surrogate_forward(self._h, _SUR_PRECISION[precision], S,
                                           f.ctypes.data_as(_dp), nu.ctypes.data_as(_dp),
+                                          None if a is None else a.ctypes.data_as(_dp),
                                           out.ctypes.data_as(_dp)))
         return out
 
-    def loss_grad(self, fields, next_fields, nu):
+    def judge(self, ensemble: Ensemble, judge: Ensemble, dt: float, steps: int,
+              precision: str = "bf16") -> float:
+        """Live error estimate: the first judge.members members forecast by the
+        network and by the solver (noise-free); sets and returns the RMS."""
+        v = C.c_double()
+        _check(_lib.hzl_surrogate_judge(self._h, ensemble._h, judge._h, dt, steps,
+                                        _SUR_PRECISION[precision], C.byref(v)))
+        return v.value
+
+    def loss_grad(self, fields, next_fields, nu, amp=None):
         f = np.ascontiguousarray(fields, dtype=np.complex128)
         g = np.ascontiguousarray(next_fields, dtype=np.complex128)
         S = f.shape[0]
         _fields(f, S, self.state_n)
         _fields(g, S, self.state_n)
         nu = np.ascontiguousarray(np.broadcast_to(nu, (S,)), dtype=np.float64)
+        a = None if amp is None else np.ascontiguousarray(np.broadcast_to(amp, (S,)),
+                                                          dtype=np.float64)
         loss = C.c_double()
         grad = np.zeros(self.param_count, dtype=np.float32)
         _check(_lib.hzl_surrogate_loss_grad(self._h, S, f.ctypes.data_as(_dp),
                                             g.ctypes.data_as(_dp), nu.ctypes.data_as(_dp),
+                                            None if a is None else a.ctypes.data_as(_dp),
                                             C.byref(loss), grad.ctypes.data_as(C.POINTER(C.c_float))))
         return loss.value, grad
 
@@ -1026,13 +1048,17 @@ class Surrogate:
 
 def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt: float,
                      seed: int, precision: str = "bf16", retain_states: bool = True,
-                     fallback_threshold: float | None = None):
+                     fallback_threshold: float | None = None, judge: Ensemble | None = None,
+                     judge_every: int = 5):
     """Bootstrap particle filter whose forecast between observation groups is
     one surrogate evaluation (every group interval must equal the surrogate's
     horizon); weighting and resampling as in bootstrap_filter.  With
     fallback_threshold, a group whose surrogate error estimate exceeds
     threshold x the ensemble spread is forecast by the plain solver instead
-    (history records `solver` and `spread`)."""
+    (history records `solver`, `spread` and `error_estimate`).  With a judge
+    ensemble, every judge_every-th group first re-forecasts judge.members
+    particles with the solver and the network (Surrogate.judge) and uses their
+    measured RMS difference as the error estimate."""
     obs = np.asarray(observations, dtype=np.float64)
     if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
         raise ValueError("observations must be rows of (step, x, y, value, sd)")
@@ -1045,6 +1071,8 @@ def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt
     for gi, target in enumerate(groups):
         sel = obs[steps == target]
         solver, spread = False, None
+        if judge is not None and gi % judge_every == 0:
+            surrogate.judge(particles, judge, dt, int(target - current), precision)
         if fallback_threshold is None:
             surrogate.forecast(particles, dt, int(target - current), precision)
         else:
@@ -1056,7 +1084,8 @@ def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt
             break  # every weight -inf (src/filter.cpp:76-80)
         history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
                             max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean,
-                            solver=solver, spread=spread))
+                            solver=solver, spread=spread,
+                            error_estimate=surrogate.error_estimate))
         current = int(target)
     ll, events = particles.pf_result()
     return dict(log_likelihood=ll, resampling_events=events, history=history)

@@ -430,6 +430,10 @@ typedef struct hzl_pmcmc_options {
     uint64_t seed;         /* chain c's stream seed is stream_seed(seed, c) ... */
     int prior;             /* HZL_PRIOR_* */
     const uint64_t *chain_seeds; /* ... or chain_seeds[c] when not null */
+    hzl_surrogate_t surrogate;   /* optional: forecast every observation interval with
+                                    this trained network instead of the solver (white
+                                    forcing; every interval must equal its horizon) */
+    int surrogate_precision;     /* HZL_BF16 or HZL_FP32 operands of that forecast */
 } hzl_pmcmc_options;
 
 /* The first `count` draws of the reference's NormalStream(seed): mt19937_64,
@@ -443,6 +447,11 @@ int hzl_normal_draws(uint64_t seed, size_t count, double *out);
  * with the reference's message unless dims == 4.
  * Replaces: stochastic_log_prior (src/filter.cpp:127-140). */
 int hzl_stochastic_log_prior(const double *q, size_t dims, double *out);
+
+/* Likelihood evaluations with options->surrogate set forecast each interval
+ * with the network for the whole run: a pseudo-marginal chain must target one
+ * likelihood, so the forecast model is chosen per run (judge it first with
+ * hzl_surrogate_judge), never switched between evaluations. */
 
 /* Log-likelihood estimates of `filters` independent bootstrap filters over
  * the ensemble's members (members / filters particles each): filter f runs
@@ -496,7 +505,8 @@ int hzl_ensemble_stream(hzl_ensemble_t h, void **stream);
  * (src/filter.cpp:46-48 advances every particle with the solver) by one
  * network evaluation per observation interval.  The network sees the state
  * on an internal grid x grid lattice (exact truncated DFT of the band
- * |kx|, |ky| <= grid/2 - 1), log viscosity and the Kolmogorov pattern, and
+ * |kx|, |ky| <= grid/2 - 1), log viscosity and the member's Kolmogorov
+ * forcing (amplitude / 0.1 times its pattern), and
  * predicts the increment over a fixed horizon of `horizon_steps` solver
  * steps; modes outside the band decay linearly, and the shell forcing adds
  * white noise of variance forcing_sigma^2 h / 2 per component.
@@ -522,17 +532,18 @@ int hzl_surrogate_set_scales(hzl_surrogate_t h, double input_scale, double outpu
 int hzl_surrogate_get_scales(hzl_surrogate_t h, double *scales);
 
 /* Network output (count x grid x grid doubles) for `count` states in the
- * reference Field layout with per-sample viscosities; precision HZL_FP32 or
- * HZL_BF16 operands. */
+ * reference Field layout with per-sample viscosities and Kolmogorov
+ * amplitudes (amplitude may be null: 0.1 for every sample); precision
+ * HZL_FP32 or HZL_BF16 operands. */
 int hzl_surrogate_forward(hzl_surrogate_t h, int precision, size_t count, const double *fields,
-                          const double *viscosity, double *grid_out);
+                          const double *viscosity, const double *amplitude, double *grid_out);
 
 /* Mean squared error between the network output and the scaled grid
  * increment next - fields, and its gradient with respect to every parameter
  * (FP32 operands). */
 int hzl_surrogate_loss_grad(hzl_surrogate_t h, size_t count, const double *fields,
-                            const double *next_fields, const double *viscosity, double *loss,
-                            float *grad);
+                            const double *next_fields, const double *viscosity,
+                            const double *amplitude, double *loss, float *grad);
 
 /* Append windows x members training pairs from an FP32 ensemble of the
  * surrogate's state_n: each window copies the states as inputs and advances
@@ -577,6 +588,17 @@ int hzl_surrogate_spread(hzl_surrogate_t h, hzl_ensemble_t ensemble, double *rms
 int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt,
                                     size_t steps, int precision, double threshold,
                                     int *used_solver, double *spread);
+
+/* Live judge of the network's forecast error, with the plain solver as the
+ * judge: the first members(judge) members of `ensemble` are copied (states,
+ * nu, alpha, Kolmogorov amplitude) into `judge`, an FP32 ensemble on the same
+ * grid run without shell noise; there the same states are forecast over
+ * steps * dt by the network and by `steps` plain RK4 steps.  The grid RMS
+ * difference per member (vorticity units) becomes the error estimate that
+ * hzl_surrogate_forecast_or_solve compares with the ensemble spread, and is
+ * returned in *rms.  `ensemble` is not changed. */
+int hzl_surrogate_judge(hzl_surrogate_t h, hzl_ensemble_t ensemble, hzl_ensemble_t judge,
+                        double dt, size_t steps, int precision, double *rms);
 
 /* Advance every member of an FP32 ensemble by steps * dt with the network
  * (steps and dt must equal the trained horizon).  The ensemble's step counter

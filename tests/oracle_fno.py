@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 CIN, HIDDEN, KF = 3, 128, 4
+AMP_REF = 0.1  # the forcing channel is (amplitude / AMP_REF) sin(KF y)
 
 
 class Shape:
@@ -106,16 +107,19 @@ def spectral(shape: Shape, v, Rr, Ri):
     return idft_modes(shape, U)
 
 
-def forward(shape: Shape, params, fields, nu, in_scale):
-    """Network output [s][G][G] (float64 tensor); params: flat float64 tensor."""
+def forward(shape: Shape, params, fields, nu, in_scale, amp=None):
+    """Network output [s][G][G] (float64 tensor); params: flat float64 tensor;
+    amp: per-sample Kolmogorov amplitudes (default AMP_REF)."""
     p = shape.unpack(params)
     G = shape.G
     S = len(nu)
     w = sample_grid(shape, fields) * in_scale
     lognu = torch.as_tensor(np.log(np.asarray(nu)) - math.log(1e-3), dtype=torch.float64)
+    a = torch.ones(S, dtype=torch.float64) if amp is None else \
+        torch.as_tensor(np.asarray(amp, dtype=np.float64) / AMP_REF)
     y = torch.arange(G, dtype=torch.float64)
-    feat = torch.stack([w, lognu[:, None, None].expand(S, G, G),
-                        torch.sin(2 * math.pi * KF * y / G)[None, :, None].expand(S, G, G)], -1)
+    pattern = torch.sin(2 * math.pi * KF * y / G)[None, :, None] * a[:, None, None]
+    feat = torch.stack([w, lognu[:, None, None].expand(S, G, G), pattern.expand(S, G, G)], -1)
     v = feat @ p["Lw"].T + p["Lb"]
     for l in range(shape.L):
         z = spectral(shape, v, p[f"Rr{l}"], p[f"Ri{l}"]) + v @ p[f"W{l}"].T + p[f"b{l}"]
@@ -124,15 +128,16 @@ def forward(shape: Shape, params, fields, nu, in_scale):
     return h @ p["Q2"] + p["q2"][0]
 
 
-def loss(shape: Shape, params, fields, next_fields, nu, in_scale, out_scale):
-    out = forward(shape, params, fields, nu, in_scale)
+def loss(shape: Shape, params, fields, next_fields, nu, in_scale, out_scale, amp=None):
+    out = forward(shape, params, fields, nu, in_scale, amp)
     target = sample_grid(shape, np.asarray(next_fields) - np.asarray(fields)) * out_scale
     return torch.mean((out - target) ** 2)
 
 
-def loss_and_grad(shape: Shape, params_np, fields, next_fields, nu, in_scale, out_scale):
+def loss_and_grad(shape: Shape, params_np, fields, next_fields, nu, in_scale, out_scale,
+                  amp=None):
     p = torch.tensor(np.asarray(params_np, dtype=np.float64), requires_grad=True)
-    l = loss(shape, p, fields, next_fields, nu, in_scale, out_scale)
+    l = loss(shape, p, fields, next_fields, nu, in_scale, out_scale, amp)
     l.backward()
     return float(l.detach()), p.grad.numpy().copy()
 

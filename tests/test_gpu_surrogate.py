@@ -43,12 +43,15 @@ def test_forward_matches_reference(pkg, orc, cfg, precision, tol):
     s = shape_of(cfg)
     f = grf_fields(orc, s.n, 3, 11)
     nu = np.array([5e-4, 1e-3, 2e-3])
+    amp = np.array([0.02, 0.1, 0.5])  # the forcing channel follows each sample's amplitude
     scale = 1.0 / np.sqrt(np.mean(F.sample_grid(s, f).numpy() ** 2))
     with pkg.Surrogate(**cfg, seed=7) as sur:
         sur.set_scales(scale, 1.0)
-        got = sur.forward(f, nu, precision)
+        got = sur.forward(f, nu, precision, amp=amp)
+        default = sur.forward(f, nu, precision)
         p = torch.as_tensor(sur.get_params().astype(np.float64))
-    want = F.forward(s, p, f, nu, scale).numpy()
+    want = F.forward(s, p, f, nu, scale, amp).numpy()
+    assert rel(default, F.forward(s, p, f, nu, scale).numpy()) < tol  # amplitude 0.1
     print(cfg, precision, rel(got, want))
     assert rel(got, want) < tol
 
@@ -59,12 +62,13 @@ def test_gradients_match_autograd(pkg, orc, cfg, S):
     f = grf_fields(orc, s.n, S, 21)
     nxt = f + 0.2 * grf_fields(orc, s.n, S, 22)
     nu = np.linspace(6e-4, 1.8e-3, S)
+    amp = np.linspace(0.05, 0.3, S)
     ins, outs = 2.0, 3.0
     with pkg.Surrogate(**cfg, seed=8) as sur:
         sur.set_scales(ins, outs)
-        loss, grad = sur.loss_grad(f, nxt, nu)
+        loss, grad = sur.loss_grad(f, nxt, nu, amp=amp)
         p = sur.get_params().astype(np.float64)
-    want_loss, want = F.loss_and_grad(s, p, f, nxt, nu, ins, outs)
+    want_loss, want = F.loss_and_grad(s, p, f, nxt, nu, ins, outs, amp)
     print(cfg, loss, want_loss, rel(grad, want))
     assert abs(loss - want_loss) < 1e-5 * want_loss
     assert rel(grad, want) < 1e-4
@@ -243,3 +247,41 @@ def test_graph_replays_match_eager_forecasts(pkg, orc):
             a.synchronize()
             b.synchronize()
             np.testing.assert_array_equal(a.get_state(), b.get_state())
+
+
+def test_judge_measures_the_network_error_against_the_solver(pkg, orc):
+    """hzl_surrogate_judge: the first J particles forecast by the network
+    (no shell noise) and by the plain solver from the same states; the RMS
+    of their difference equals the same comparison made by hand with
+    separate ensembles, becomes the error estimate, and leaves the filtered
+    ensemble untouched (states and step index)."""
+    cfg = SMALL
+    s = shape_of(cfg)
+    B, J, dt, steps = 16, 6, 2e-3, 25
+    f = grf_fields(orc, s.n, B, 71)
+    nus = np.linspace(6e-4, 1.8e-3, B)
+    with pkg.Surrogate(**cfg, seed=9) as sur, \
+            pkg.Ensemble(s.n, B, "fp32", sigma=0.05, amp=0.1, seed=6) as ens, \
+            pkg.Ensemble(s.n, J, "fp32", sigma=0.0, amp=0.1, seed=7) as judge, \
+            pkg.Ensemble(s.n, J, "fp32", sigma=0.0, amp=0.1, seed=8) as net, \
+            pkg.Ensemble(s.n, J, "fp32", sigma=0.0, amp=0.1, seed=8) as sol:
+        sur.set_scales(1.5, 4.0)
+        for m in range(B):
+            ens.set_member_parameters(m, nus[m], 0.0, 0.05, 0.1 + 0.01 * m)
+        for e in (net, sol):
+            for m in range(J):
+                e.set_member_parameters(m, nus[m], 0.0, 0.0, 0.1 + 0.01 * m)
+        ens.set_state(f)
+        ens.step(dt, 3)
+        before = ens.get_state()
+        rms = sur.judge(ens, judge, dt, steps, "fp32")
+        assert np.array_equal(ens.get_state(), before) and ens.step_index == 3
+        assert sur.error_estimate == rms
+        net.set_state(before[:J])
+        sol.set_state(before[:J])
+        sur.forecast(net, dt, steps, "fp32")
+        sol.step(dt, steps)
+        d = net.get_state() - sol.get_state()
+        want = np.sqrt(np.sum(np.abs(d) ** 2) / J)
+    print("judge", rms, want)
+    assert rms > 0 and abs(rms - want) <= 1e-5 * want

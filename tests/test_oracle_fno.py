@@ -80,13 +80,14 @@ def test_autograd_matches_finite_differences():
     fields = np.stack([real_field(rng, s.n, s.KS) for _ in range(2)])
     nxt = fields + 0.1 * np.stack([real_field(rng, s.n, s.KS) for _ in range(2)])
     nu = np.array([1e-3, 2e-3])
+    amp = np.array([0.05, 0.4])
     p = 0.3 * rng.standard_normal(s.count())
-    loss, grad = F.loss_and_grad(s, p, fields, nxt, nu, 2.0, 3.0)
+    loss, grad = F.loss_and_grad(s, p, fields, nxt, nu, 2.0, 3.0, amp)
     for trial in range(3):
         d = rng.standard_normal(s.count())
         h = 1e-5
-        lp = float(F.loss(s, torch.as_tensor(p + h * d), fields, nxt, nu, 2.0, 3.0))
-        lm = float(F.loss(s, torch.as_tensor(p - h * d), fields, nxt, nu, 2.0, 3.0))
+        lp = float(F.loss(s, torch.as_tensor(p + h * d), fields, nxt, nu, 2.0, 3.0, amp))
+        lm = float(F.loss(s, torch.as_tensor(p - h * d), fields, nxt, nu, 2.0, 3.0, amp))
         fd = (lp - lm) / (2 * h)
         assert abs(fd - grad @ d) < 1e-6 * max(1.0, abs(fd)), (trial, fd, grad @ d)
     assert math.isfinite(loss) and loss > 0
@@ -106,3 +107,4 @@ def test_network_is_translation_equivariant_in_x():
     a = F.forward(s, p, f[None], [1e-3], 1.5)[0].numpy()
     b = F.forward(s, p, g[None], [1e-3], 1.5)[0].numpy()
     np.testing.assert_allclose(np.roll(a, shift, axis=1), b, rtol=0, atol=1e-10)
+
