@@ -221,105 +221,202 @@ def rms_error(mean_field, truth) -> float:
     return float(np.sqrt(np.sum(np.abs(mean_field - truth) ** 2)))
 
 
-def surrogate_pf(pkg, cycles: int = 20, particles: int = 256, train_windows: int = 20,
-                 epochs: int = 4) -> dict:
-    """Particle filter at 128^2 with the plain solver vs the trained surrogate
-    as the forecast between observation groups.
-
-    Twin experiment (synthetic, seeded): truth = one FP32 member, nu = 1e-3,
-    dt = 1e-3, 0.1 sin(4y) + white noise 0.05 on 1 <= |k| <= 4; every 50 steps
-    (t + 0.05) 128 point vorticity observations with noise sd 0.02.  Particles
-    start at truth + 0.5 x an independent GRF.  The surrogate (4 layers, width
-    32, 16 modes, 64^2 grid) is first trained on 20 x 1024 solver pairs
-    (log-uniform nu in [5e-4, 2e-3], noise-free targets) and runs in BF16.
-    cycles/s counts the whole filter loop (forecast + weighting + resampling).
-    """
-    n, dt, H, sigma, amp = 128, 1e-3, 50, 0.05, 0.1
-    nobs, sd = 128, 0.02
-    out = {"config": f"{particles} particles at {n}x{n}, {cycles} cycles of {H} steps "
-                     f"(dt={dt}), {nobs} obs/cycle sd={sd}; surrogate 4x32ch, 16 modes, "
-                     f"trained on {train_windows}x1024 pairs for {epochs} epochs"}
-    # -- training data and fit
-    rng = np.random.default_rng(SEED)
+def train_surrogate(pkg, n: int, dt: float, H: int, windows: int, epochs: int, nu_range,
+                    amp_range, sigma: float, seed: int):
+    """A surrogate (4 layers, width 32, 16 modes, 64^2 grid) trained on
+    windows x 1024 solver pairs (w(t), w(t + H dt)) of an FP32 ensemble at
+    n^2 with log-uniform nu and Kolmogorov amplitude per member and noise-free
+    targets; the last window is held out.  Returns (surrogate, summary)."""
+    rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
-    data = pkg.Ensemble(n, 1024, "fp32", nu=1e-3, sigma=sigma, amp=amp, seed=SEED + 1)
+    data = pkg.Ensemble(n, 1024, "fp32", nu=1e-3, sigma=sigma, amp=0.1, seed=seed + 1)
     for m in range(1024):
-        data.set_member_parameters(m, float(np.exp(rng.uniform(np.log(5e-4), np.log(2e-3)))),
-                                   0.0, sigma, amp)
+        data.set_member_parameters(
+            m, float(np.exp(rng.uniform(np.log(nu_range[0]), np.log(nu_range[1])))), 0.0, sigma,
+            float(np.exp(rng.uniform(np.log(amp_range[0]), np.log(amp_range[1])))))
     data.init_grf()
-    data.step(dt, 50)
-    sur = pkg.Surrogate(128, 64, 32, 4, 16, seed=SEED)
-    sur.collect(data, dt, H, train_windows, noise_free_targets=True)
+    data.step(dt, H)
+    sur = pkg.Surrogate(n, 64, 32, 4, 16, seed=seed)
+    sur.collect(data, dt, H, windows, noise_free_targets=True)
     data.close()
     t1 = time.perf_counter()
-    train = (train_windows - 1) * 1024
-    losses = sur.fit(0, train, epochs, 32, 3e-3, seed=SEED)
+    train = (windows - 1) * 1024
+    losses = sur.fit(0, train, epochs, 32, 3e-3, seed=seed)
     t2 = time.perf_counter()
     trained = sur.get_params()
     sur.set_params(np.zeros_like(trained))
     persistence = sur.evaluate(train, 1024, "fp32")
     sur.set_params(trained)
     held32 = sur.evaluate(train, 1024, "fp32")
-    held16 = sur.evaluate(train, 1024, "bf16")  # sets the error estimate used below
-    out["training"] = {"pairs": sur.dataset_size, "collect_s": round(t1 - t0, 2),
-                       "fit_s": round(t2 - t1, 2), "epoch_loss": [round(x, 5) for x in losses],
-                       "held_out_loss_fp32": round(held32, 6),
-                       "held_out_loss_bf16": round(held16, 6),
-                       "persistence_loss": round(persistence, 6),
-                       "explained_increment_variance": round(1 - held16 / persistence, 5),
-                       "surrogate_vs_solver_rms": sur.error_estimate}
-    # -- truth and observations
-    truth_ens = pkg.Ensemble(n, 1, "fp32", nu=1e-3, sigma=sigma, amp=amp, seed=SEED + 2)
-    truth_ens.init_grf()
-    truth0 = truth_ens.get_state()
+    held16 = sur.evaluate(train, 1024, "bf16")  # sets the held-out error estimate
+    return sur, {"pairs": sur.dataset_size, "collect_s": round(t1 - t0, 2),
+                 "fit_s": round(t2 - t1, 2), "epoch_loss": [round(x, 5) for x in losses],
+                 "held_out_loss_fp32": round(held32, 6), "held_out_loss_bf16": round(held16, 6),
+                 "persistence_loss": round(persistence, 6),
+                 "explained_increment_variance": round(1 - held16 / persistence, 5),
+                 "held_out_rms_error": sur.error_estimate,
+                 "nu_range": list(nu_range), "amp_range": list(amp_range)}
+
+
+def twin(pkg, n: int, dt: float, H: int, cycles: int, nobs: int, sd: float, nu: float,
+         amp: float, sigma: float, seed: int):
+    """Hidden truth (one FP32 member from a GRF) observed every H steps at
+    nobs random grid points with noise sd.  Returns (rows, truths, truth0)."""
+    rng = np.random.default_rng(seed)
+    truth = pkg.Ensemble(n, 1, "fp32", nu=nu, sigma=sigma, amp=amp, seed=seed + 2)
+    truth.init_grf()
+    truth0 = truth.get_state()
     truths, rows = [], []
     for c in range(cycles):
-        truth_ens.step(dt, H)
-        w = truth_ens.get_state()[0]
+        truth.step(dt, H)
+        w = truth.get_state()[0]
         truths.append(w)
         ph = (np.fft.ifft2(w) * n * n).real
         xs, ys = rng.integers(0, n, nobs), rng.integers(0, n, nobs)
         vals = ph[ys, xs] + sd * rng.standard_normal(nobs)
         rows += [((c + 1) * H, x, y, v, sd) for x, y, v in zip(xs, ys, vals)]
-    truth_ens.close()
-    rows = np.array(rows, dtype=np.float64)
-    pert = pkg.Ensemble(n, particles, "fp32", seed=SEED + 3)
+    truth.close()
+    return np.array(rows, dtype=np.float64), truths, truth0
+
+
+def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: float, amp: float,
+                sigma: float, kinds, threshold: float = 0.1, judge_members: int = 16,
+                judge_every: int = 10, seeds: int = 3) -> dict:
+    """Plain-solver PF and surrogate PFs from the same particles and twin;
+    each timed over the whole filter loop (warm run first), then re-run with
+    the weighted means kept, for `seeds` filter seeds, for the analysis RMSE
+    against the truth (mean over cycles and seeds)."""
+    n = init.shape[-1]
+    out = {}
+    for kind in kinds:
+        def run(retain, seed=SEED):
+            with pkg.Ensemble(n, particles, "fp32", nu=nu, sigma=sigma, amp=amp,
+                              seed=SEED + 4) as pf:
+                pf.set_state(init)
+                pf.synchronize()
+                judge = None
+                if kind == "surrogate_judged":
+                    judge = pkg.Ensemble(n, judge_members, "fp32", nu=nu, sigma=0.0, amp=amp,
+                                         seed=SEED + 5)
+                t = time.perf_counter()
+                if kind == "free":  # no assimilation: the ensemble mean of the forecasts
+                    steps = np.unique(rows[:, 0].astype(int))
+                    hist, now = [], 0
+                    for target in steps:
+                        pf.step(dt, int(target - now))
+                        now = int(target)
+                        hist.append(dict(mean_field=pf.get_state().mean(0) if retain else None,
+                                         ess=float(particles)))
+                    res = {"history": hist, "resampling_events": 0, "log_likelihood": 0.0}
+                elif kind == "plain":
+                    res = pkg.bootstrap_filter(pf, rows, dt, seed, retain_states=retain)
+                elif kind == "surrogate":
+                    res = pkg.surrogate_filter(pf, sur, rows, dt, seed, "bf16",
+                                               retain_states=retain)
+                else:
+                    res = pkg.surrogate_filter(pf, sur, rows, dt, seed, "bf16",
+                                               retain_states=retain,
+                                               fallback_threshold=threshold, judge=judge,
+                                               judge_every=judge_every)
+                pf.synchronize()
+                sec = time.perf_counter() - t
+                if judge is not None:
+                    judge.close()
+            return res, sec
+
+        run(False)  # warm (captures graphs)
+        _, sec = run(False)
+        err = []
+        for k in range(seeds if kind != "free" else 1):
+            r, _ = run(True, SEED + 100 * k)
+            err += [rms_error(h["mean_field"], tr) for h, tr in zip(r["history"], truths)]
+            if k == 0:
+                res = r
+        cycles = len(res["history"])
+        ess = [h["ess"] for h in res["history"]]
+        o = {"cycles_per_s": round(cycles / sec, 2), "ms_per_cycle": round(sec / cycles * 1e3, 3),
+             "analysis_rmse": round(float(np.mean(err)), 6),
+             "ess_mean": round(float(np.mean(ess)), 2), "ess_min": round(float(np.min(ess)), 2),
+             "ess_per_cycle": [round(x, 1) for x in ess],
+             "resampling_events": res["resampling_events"],
+             "log_likelihood": round(res["log_likelihood"], 3)}
+        if kind == "surrogate_judged":
+            o["solver_cycles"] = int(sum(h["solver"] for h in res["history"]))
+            o["error_estimates"] = sorted({round(h["error_estimate"], 6)
+                                           for h in res["history"]})
+            o["spread_mean"] = round(float(np.mean([h["spread"] for h in res["history"]])), 5)
+            o["threshold"] = threshold
+            o["judge"] = f"{judge_members} particles every {judge_every} cycles"
+        out[kind] = o
+    return out
+
+
+def low_mode_particles(pkg, truth0, particles: int, scale: float = 1.0):
+    """A prior the truth could have been drawn from: particles = background +
+    scale x independent GRFs kept on the forced shell |k| <= 4 (the large
+    scales the shell noise keeps uncertain), background = truth + one more
+    such draw, so the free ensemble mean is off by that draw."""
+    n = truth0.shape[-1]
+    k = np.abs(np.fft.fftfreq(n, 1.0 / n))
+    low = (k[None, :] ** 2 + k[:, None] ** 2) <= 16
+    pert = pkg.Ensemble(n, particles + 1, "fp32", seed=SEED + 3)
     pert.init_grf()
-    init = truth0 + 0.5 * pert.get_state()
+    d = scale * pert.get_state() * low[None]
     pert.close()
+    return truth0 + d[0] + d[1:]
 
-    def run(kind, retain):
-        with pkg.Ensemble(n, particles, "fp32", nu=1e-3, sigma=sigma, amp=amp,
-                          seed=SEED + 4) as pf:
-            pf.set_state(init)
-            pf.synchronize()
-            t = time.perf_counter()
-            if kind in ("plain", "plain_no_graph"):
-                pf.set_graph(kind == "plain")
-                res = pkg.bootstrap_filter(pf, rows, dt, SEED, retain_states=retain)
-            elif kind == "surrogate":
-                res = pkg.surrogate_filter(pf, sur, rows, dt, SEED, "bf16", retain_states=retain)
-            else:
-                res = pkg.surrogate_filter(pf, sur, rows, dt, SEED, "bf16", retain_states=retain,
-                                           fallback_threshold=0.1)
-            pf.synchronize()
-            sec = time.perf_counter() - t
-        return res, sec
 
-    for kind in ("plain", "plain_no_graph", "surrogate", "surrogate_fallback"):
-        run(kind, False)  # warm
-        _, sec = run(kind, False)
-        res, _ = run(kind, True)
-        err = [rms_error(h["mean_field"], tr) for h, tr in zip(res["history"], truths)]
-        out[kind] = {"cycles_per_s": round(cycles / sec, 2), "ms_per_cycle": round(sec / cycles * 1e3, 3),
-                     "analysis_rmse": round(float(np.mean(err)), 6),
-                     "log_likelihood": round(res["log_likelihood"], 3),
-                     "resampling_events": res["resampling_events"]}
-        if kind == "surrogate_fallback":
-            out[kind]["solver_cycles"] = int(sum(h["solver"] for h in res["history"]))
-    out["speedup"] = round(out["surrogate"]["cycles_per_s"] / out["plain"]["cycles_per_s"], 2)
-    out["rmse_ratio"] = round(out["surrogate"]["analysis_rmse"] / out["plain"]["analysis_rmse"], 4)
+def surrogate_pf(pkg, cycles: int = 20, train_windows: int = 20, epochs: int = 4,
+                 sd: float = 0.1, nobs: int = 64) -> dict:
+    """Particle filter at 128^2 with the plain solver vs the trained surrogate
+    as the forecast between observation groups, at 256 and 1024 particles.
+
+    Twin experiment (synthetic, seeded): truth = one FP32 member, nu = 1e-3,
+    dt = 1e-3, 0.1 sin(4y) + white noise 0.05 on 1 <= |k| <= 4; every 50 steps
+    (t + 0.05) nobs point vorticity observations with noise sd.  Particles start
+    at a background (truth + one GRF draw on the forced large scales |k| <= 4)
+    plus independent such draws; "free" is the same ensemble without
+    assimilation (the reference for tracking).  The surrogate trains on 20 x 1024
+    solver pairs (log-uniform nu in [5e-4, 2e-3]).  Variants: plain solver,
+    surrogate, and surrogate with the solver as live judge (16 particles
+    re-forecast every 10 cycles) and fallback when the judged error exceeds 0.1
+    x the ensemble spread.  Analysis RMSEs are means over 3 filter seeds.  A
+    second twin at Kolmogorov amplitude 2.0 (20x the training amplitude) shows
+    the judge detecting the network's error and the fallback forecasting with
+    the solver.  cycles/s counts the whole filter loop."""
+    n, dt, H, sigma, amp, nu = 128, 1e-3, 50, 0.05, 0.1, 1e-3
+    out = {"config": f"{n}x{n}, {cycles} cycles of {H} steps (dt={dt}), {nobs} obs/cycle "
+                     f"sd={sd}; surrogate 4x32ch, 16 modes, 64^2 grid"}
+    sur, out["training"] = train_surrogate(pkg, n, dt, H, train_windows, epochs, (5e-4, 2e-3),
+                                           (0.1, 0.1), sigma, SEED)
+    rows, truths, truth0 = twin(pkg, n, dt, H, cycles, nobs, sd, nu, amp, sigma, SEED + 30)
     out["truth_rms"] = round(float(np.mean([np.sqrt(np.sum(np.abs(t) ** 2)) for t in truths])), 5)
+    # modes of the 128^2 state outside the network's band |k| <= 31
+    k = np.abs(np.fft.fftfreq(n, 1.0 / n))
+    outside = (k[None, :] > 31) | (k[:, None] > 31)
+    out["energy_outside_band"] = round(float(np.mean(
+        [np.sum(np.abs(t[outside]) ** 2) / np.sum(np.abs(t) ** 2) for t in truths])), 8)
+    for particles in (256, 1024):
+        init = low_mode_particles(pkg, truth0, particles)
+        r = pf_variants(pkg, sur, rows, truths, init, particles, dt, nu, amp, sigma,
+                        ("free", "plain", "surrogate", "surrogate_judged"))
+        r["speedup"] = round(r["surrogate"]["cycles_per_s"] / r["plain"]["cycles_per_s"], 2)
+        r["speedup_judged"] = round(r["surrogate_judged"]["cycles_per_s"] /
+                                    r["plain"]["cycles_per_s"], 2)
+        r["rmse_ratio"] = round(r["surrogate"]["analysis_rmse"] / r["plain"]["analysis_rmse"], 4)
+        out[f"p{particles}"] = r
+    # outside the training data: the judge sees the network's error and the
+    # fallback forecasts with the solver
+    amp_ood = 2.0
+    rows2, truths2, truth02 = twin(pkg, n, dt, H, cycles, nobs, sd, nu, amp_ood, sigma,
+                                   SEED + 31)
+    init2 = low_mode_particles(pkg, truth02, 256)
+    r = pf_variants(pkg, sur, rows2, truths2, init2, 256, dt, nu, amp_ood, sigma,
+                    ("plain", "surrogate", "surrogate_judged"), seeds=1)
+    r["rmse_ratio"] = round(r["surrogate"]["analysis_rmse"] / r["plain"]["analysis_rmse"], 4)
+    r["rmse_ratio_judged"] = round(r["surrogate_judged"]["analysis_rmse"] /
+                                   r["plain"]["analysis_rmse"], 4)
+    out["amp_2.0_outside_training"] = r
     sur.close()
     return out
 
@@ -381,13 +478,11 @@ def enkf_cycle(pkg, cycles: int = 5, members: int = 1024) -> dict:
     return out
 
 
-def pmcmc_window(pkg, chains: int = 16, particles: int = 128, samples: int = 3) -> dict:
-    """Batched particle-marginal Metropolis at 256^2: 16 chains x 128 particles
-    (2048 FP32 members, one filter per chain), a 500-step window (dt = 1e-3)
-    with 128 point observations every 100 steps (sd 0.02) from a hidden truth
-    (nu = 1e-3, amp = 0.1), log-uniform priors nu in [2e-4, 5e-3], amp in
-    [0.02, 0.5].  Times the initial evaluation plus `samples` Metropolis steps
-    (each one batched filter pass over all chains)."""
+def pmcmc_problem(pkg, chains: int = 16):
+    """The pMCMC window at 256^2: a hidden truth (nu = 1e-3, amp = 0.1) run
+    500 steps of dt = 1e-3 from a GRF, 128 point observations (sd 0.02) every
+    100 steps, and the chains' starting (log nu, log amp), uniform in
+    [5e-4, 2e-3] x [0.05, 0.2].  Returns (rows, initial field, q0)."""
     n, dt, window, every, nobs, sd = 256, 1e-3, 500, 100, 128, 0.02
     rng = np.random.default_rng(SEED + 11)
     truth = pkg.Ensemble(n, 1, "fp32", nu=1e-3, sigma=0.05, amp=0.1, seed=SEED + 12)
@@ -400,23 +495,72 @@ def pmcmc_window(pkg, chains: int = 16, particles: int = 128, samples: int = 3) 
         xs, ys = rng.integers(0, n, nobs), rng.integers(0, n, nobs)
         rows += [(t, x, y, ph[y, x] + sd * rng.standard_normal(), sd) for x, y in zip(xs, ys)]
     truth.close()
-    q0 = np.stack([np.log(rng.uniform(5e-4, 2e-3, chains)), np.log(rng.uniform(0.05, 0.2, chains))], 1)
+    q0 = np.stack([np.log(rng.uniform(5e-4, 2e-3, chains)),
+                   np.log(rng.uniform(0.05, 0.2, chains))], 1)
+    return np.array(rows), init, q0
+
+
+def pmcmc_window(pkg, chains: int = 16, particles: int = 128, samples: int = 3,
+                 surrogate=None) -> dict:
+    """Batched particle-marginal Metropolis at 256^2: 16 chains x 128 particles
+    (2048 FP32 members, one filter per chain) over the window of
+    pmcmc_problem, log-uniform priors nu in [2e-4, 5e-3], amp in [0.02, 0.5].
+    Times the initial evaluation plus `samples` Metropolis steps (each one
+    batched filter pass over all chains); with a surrogate, every interval's
+    forecast is the network's.  Posterior statistics over all chain states."""
+    n, dt, window = 256, 1e-3, 500
+    rows, init, q0 = pmcmc_problem(pkg, chains)
     with pkg.Ensemble(n, chains * particles, "fp32", sigma=0.05, seed=SEED + 13) as ens:
         t0 = time.perf_counter()
-        res = pkg.particle_metropolis(ens, np.array(rows), dt, init, q0, samples, 0.1, SEED,
-                                      forcing_sigma=0.05)
+        res = pkg.particle_metropolis(ens, rows, dt, init, q0, samples, 0.1, SEED,
+                                      forcing_sigma=0.05, surrogate=surrogate)
         ens.synchronize()
         sec = time.perf_counter() - t0
     evals = samples + 1
     member_steps = evals * chains * particles * window
+    nu, amp = np.exp(res["q"][:, :, 0]), np.exp(res["q"][:, :, 1])
     return {"config": f"{chains} chains x {particles} particles at {n}x{n}, {window}-step window, "
-                      f"{nobs} obs every {every} steps sd={sd}, {samples} Metropolis steps",
+                      f"128 obs every 100 steps sd=0.02, {samples} Metropolis steps, "
+                      + ("surrogate forecast" if surrogate is not None else "plain solver"),
             "seconds": round(sec, 3), "seconds_per_mh_step": round(sec / evals, 3),
             "member_steps_per_s": round(member_steps / sec, 1),
             "acceptance": float(res["accepted"].mean()),
             "final_log_likelihood_mean": float(np.mean(res["log_likelihood"][:, -1])),
-            "posterior_nu_mean": float(np.exp(res["q"][:, -1, 0]).mean()),
-            "posterior_amp_mean": float(np.exp(res["q"][:, -1, 1]).mean())}
+            "posterior_nu_mean": float(nu.mean()), "posterior_nu_std": float(nu.std()),
+            "posterior_amp_mean": float(amp.mean()), "posterior_amp_std": float(amp.std()),
+            "truth": {"nu": 1e-3, "amp": 0.1}}
+
+
+def pmcmc_surrogate(pkg, samples_plain: int = 3, samples_surrogate: int = 30) -> dict:
+    """The pMCMC window with the likelihood's forecast by a surrogate trained
+    for it: 256^2 state, horizon 100 steps of dt = 1e-3, 20 x 1024 pairs with
+    log-uniform nu in [2e-4, 5e-3] and amplitude in [0.02, 0.5] (the prior
+    box).  Before the run the plain solver judges the network on 64 particles
+    at the chains' starting parameters (hzl_surrogate_judge); a pseudo-marginal
+    chain keeps one forecast model, so the whole run uses the network.  Both
+    variants' times and posterior mean / std of (nu, amplitude)."""
+    n, dt, H = 256, 1e-3, 100
+    sur, info = train_surrogate(pkg, n, dt, H, 20, 4, (2e-4, 5e-3), (0.02, 0.5), 0.05,
+                                SEED + 40)
+    out = {"training": info}
+    _, init, q0 = pmcmc_problem(pkg)
+    with pkg.Ensemble(n, 64, "fp32", sigma=0.05, seed=SEED + 14) as ens, \
+            pkg.Ensemble(n, 64, "fp32", sigma=0.0, seed=SEED + 15) as judge:
+        ens.set_state(np.stack([init] * 64))
+        for m in range(64):
+            q = q0[m % len(q0)]
+            ens.set_member_parameters(m, float(np.exp(q[0])), 0.0, 0.05, float(np.exp(q[1])))
+        t = time.perf_counter()
+        out["judge_rms_at_start"] = sur.judge(ens, judge, dt, H, "bf16")
+        out["judge_seconds"] = round(time.perf_counter() - t, 3)
+        ens.step(dt, H)
+        out["forecast_spread_at_start"] = sur.spread(ens)
+    out["surrogate"] = pmcmc_window(pkg, samples=samples_surrogate, surrogate=sur)
+    out["plain"] = pmcmc_window(pkg, samples=samples_plain)
+    out["speedup_per_mh_step"] = round(out["plain"]["seconds_per_mh_step"] /
+                                       out["surrogate"]["seconds_per_mh_step"], 2)
+    sur.close()
+    return out
 
 
 def uneven_pf(pkg, cycles: int = 5) -> dict:
@@ -494,6 +638,12 @@ def main() -> int:
     ap.add_argument("--no-uneven", action="store_true", help="skip the two-grid filter")
     ap.add_argument("--uneven-only", action="store_true", help="only the two-grid filter")
     ap.add_argument("--pmcmc-only", action="store_true", help="only the pMCMC window")
+    ap.add_argument("--pmcmc-surrogate-only", action="store_true",
+                    help="only the surrogate-assisted pMCMC comparison")
+    ap.add_argument("--no-pmcmc-surrogate", action="store_true",
+                    help="skip the surrogate-assisted pMCMC comparison")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel from the host (HZL_GRAPHS=0; for kernel tracers)")
     ap.add_argument("--enkf-only", action="store_true", help="only the EnKF cycle")
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch shapes only (no autotuning; for profiling)")
@@ -510,6 +660,8 @@ def main() -> int:
         return stream_ceiling()
     if args.no_tune:
         os.environ["HZL_AUTOTUNE"] = "0"
+    if args.no_graphs:
+        os.environ["HZL_GRAPHS"] = "0"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: members shard across GPUs with no "
               "communication during forecasts, so multi-GPU runs are not part of this build",
@@ -529,6 +681,9 @@ def main() -> int:
         return 0
     if args.pmcmc_only:
         print(json.dumps({"pmcmc": pmcmc_window(pkg)}), flush=True)
+        return 0
+    if args.pmcmc_surrogate_only:
+        print(json.dumps({"pmcmc_surrogate": pmcmc_surrogate(pkg)}), flush=True)
         return 0
     n, B = args.n, args.members
     nu = dt = 5e-4
@@ -630,6 +785,8 @@ def main() -> int:
         line["enkf"] = enkf_cycle(pkg)
     if not args.no_pmcmc:
         line["pmcmc"] = pmcmc_window(pkg)
+    if not args.no_pmcmc_surrogate:
+        line["pmcmc_surrogate"] = pmcmc_surrogate(pkg)
     if not args.no_uneven:
         line["uneven_pf"] = uneven_pf(pkg)
     if not args.no_cpu:

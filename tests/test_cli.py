@@ -166,6 +166,12 @@ def test_ensemble_matches_reference_cli(tmp_path):
                 {"member", "seed", "time"}, rtol=1e-10)
     compare_json(out / "summary.json", os.path.join(want, "summary.json"),
                  skip=("wall_seconds", "energy_standard_error"))
+    # both sides print the standard error with 6 decimals (std::to_string):
+    # equal, or one unit of the sixth decimal apart where the FP64 values
+    # straddle a rounding boundary
+    g_se = json.load(open(out / "summary.json"))["energy_standard_error"]
+    w_se = json.load(open(os.path.join(want, "summary.json")))["energy_standard_error"]
+    assert g_se > 0 and abs(g_se - w_se) <= 1e-6 + 1e-12, (g_se, w_se)
     g = json.load(open(out / "metadata.json"))
     w = json.load(open(os.path.join(want, "metadata.json")))
     for k in ("schema", "n", "dt", "integrator", "steps_this_run", "seed", "initial", "nu",
