@@ -478,12 +478,13 @@ def enkf_cycle(pkg, cycles: int = 5, members: int = 1024) -> dict:
     return out
 
 
-def pmcmc_problem(pkg, chains: int = 16):
+def pmcmc_problem(pkg, chains: int = 16, sd: float = 0.02):
     """The pMCMC window at 256^2: a hidden truth (nu = 1e-3, amp = 0.1) run
-    500 steps of dt = 1e-3 from a GRF, 128 point observations (sd 0.02) every
-    100 steps, and the chains' starting (log nu, log amp), uniform in
-    [5e-4, 2e-3] x [0.05, 0.2].  Returns (rows, initial field, q0)."""
-    n, dt, window, every, nobs, sd = 256, 1e-3, 500, 100, 128, 0.02
+    500 steps of dt = 1e-3 from a GRF, 128 point observations (noise sd,
+    0.02 by default) every 100 steps, and the chains' starting (log nu, log
+    amp), uniform in [5e-4, 2e-3] x [0.05, 0.2].  Returns (rows, initial
+    field, q0)."""
+    n, dt, window, every, nobs = 256, 1e-3, 500, 100, 128
     rng = np.random.default_rng(SEED + 11)
     truth = pkg.Ensemble(n, 1, "fp32", nu=1e-3, sigma=0.05, amp=0.1, seed=SEED + 12)
     truth.init_grf()
@@ -531,14 +532,64 @@ def pmcmc_window(pkg, chains: int = 16, particles: int = 128, samples: int = 3,
             "truth": {"nu": 1e-3, "amp": 0.1}}
 
 
-def pmcmc_surrogate(pkg, samples_plain: int = 3, samples_surrogate: int = 30) -> dict:
+def likelihood_surface(pkg, sur, sd: float, seeds_per_point: int = 4,
+                       forcing_sigma: float = 0.05, particles: int = 128) -> dict:
+    """The likelihood both forecast models give on the pMCMC window (obs noise
+    sd): a 4 x 4 grid of (nu, amplitude), seeds_per_point filters of
+    `particles` particles per point with the same filter seeds for both
+    models.  Per point: the seed-averaged likelihood log mean exp(ll) (the
+    reference's averaged_particle_log_likelihood) and the spread of ll over
+    seeds (the estimator's Monte Carlo noise).  With forcing_sigma = 0 every
+    particle is the same deterministic forecast and the likelihood is a
+    noise-free function of (nu, amplitude): the two surfaces then differ only
+    by the network's forecast error."""
+    n, dt = 256, 1e-3
+    rows, init, _ = pmcmc_problem(pkg, sd=sd)
+    nus, amps = (5e-4, 1e-3, 2e-3, 4e-3), (0.05, 0.1, 0.2, 0.4)
+    S = seeds_per_point
+    q = np.log([[a, b] for a in nus for b in amps for _ in range(S)])
+    seeds = [SEED + 50 + i for i in range(len(q))]
+    out = {"points": "nu in (5e-4, 1e-3, 2e-3, 4e-3) x amplitude in (0.05, 0.1, 0.2, 0.4), "
+                     f"{S} filters x {particles} particles per point, the same filter seeds, "
+                     f"shell noise {forcing_sigma}"}
+    avg = {}
+    with pkg.Ensemble(n, len(q) * particles, "fp32", sigma=forcing_sigma,
+                      seed=SEED + 16) as ens:
+        for kind, model in (("plain", None), ("surrogate", sur)):
+            ens.synchronize()
+            t = time.perf_counter()
+            ll = pkg.pf_log_likelihoods(ens, rows, dt, init, q, seeds,
+                                        forcing_sigma=forcing_sigma,
+                                        surrogate=model).reshape(16, S)
+            sec = time.perf_counter() - t
+            top = ll.max(1, keepdims=True)
+            avg[kind] = (top[:, 0] + np.log(np.mean(np.exp(ll - top), 1)))
+            out[kind] = {"seconds": round(sec, 3),
+                         "averaged_log_likelihood": [round(float(v), 2) for v in avg[kind]],
+                         "seed_sd_mean": round(float(ll.std(1, ddof=1).mean()), 2)
+                         if S > 1 else 0.0}
+    a, b = avg["plain"], avg["surrogate"]
+    rank = lambda v: np.argsort(np.argsort(v))
+    for kind, v in avg.items():
+        i = int(np.argmax(v))
+        out[kind]["argmax"] = [nus[i // 4], amps[i % 4]]
+        out[kind]["range"] = round(float(v.max() - v.min()), 2)
+    out["rank_correlation"] = round(float(np.corrcoef(rank(a), rank(b))[0, 1]), 4)
+    out["max_abs_difference"] = round(float(np.max(np.abs(a - b))), 2)
+    return out
+
+
+def pmcmc_surrogate(pkg, samples: int = 10, samples_long: int = 100) -> dict:
     """The pMCMC window with the likelihood's forecast by a surrogate trained
     for it: 256^2 state, horizon 100 steps of dt = 1e-3, 20 x 1024 pairs with
     log-uniform nu in [2e-4, 5e-3] and amplitude in [0.02, 0.5] (the prior
     box).  Before the run the plain solver judges the network on 64 particles
     at the chains' starting parameters (hzl_surrogate_judge); a pseudo-marginal
-    chain keeps one forecast model, so the whole run uses the network.  Both
-    variants' times and posterior mean / std of (nu, amplitude)."""
+    chain keeps one forecast model, so the whole run uses the network.  The
+    likelihood surface of both models on a 4 x 4 parameter grid with the same
+    filter seeds; both variants' times and posterior mean / std of (nu,
+    amplitude) after the same number of Metropolis steps, and the surrogate
+    chains run ten times longer."""
     n, dt, H = 256, 1e-3, 100
     sur, info = train_surrogate(pkg, n, dt, H, 20, 4, (2e-4, 5e-3), (0.02, 0.5), 0.05,
                                 SEED + 40)
@@ -555,8 +606,14 @@ def pmcmc_surrogate(pkg, samples_plain: int = 3, samples_surrogate: int = 30) ->
         out["judge_seconds"] = round(time.perf_counter() - t, 3)
         ens.step(dt, H)
         out["forecast_spread_at_start"] = sur.spread(ens)
-    out["surrogate"] = pmcmc_window(pkg, samples=samples_surrogate, surrogate=sur)
-    out["plain"] = pmcmc_window(pkg, samples=samples_plain)
+    out["likelihood_surface"] = {f"sd_{sd}": likelihood_surface(pkg, sur, sd)
+                                 for sd in (0.02, 0.2)}
+    out["likelihood_surface"].update({
+        f"noise_free_sd_{sd}": likelihood_surface(pkg, sur, sd, 1, 0.0, 2) for sd in (0.02, 0.2)})
+    out["surrogate"] = pmcmc_window(pkg, samples=samples, surrogate=sur)
+    out["plain"] = pmcmc_window(pkg, samples=samples)
+    out[f"surrogate_{samples_long}_steps"] = pmcmc_window(pkg, samples=samples_long,
+                                                          surrogate=sur)
     out["speedup_per_mh_step"] = round(out["plain"]["seconds_per_mh_step"] /
                                        out["surrogate"]["seconds_per_mh_step"], 2)
     sur.close()

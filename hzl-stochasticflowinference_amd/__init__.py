@@ -10,12 +10,13 @@ from .engine import (ALL, BF16, EXPORTS, FP32, FP64, Ensemble, HzlError, HzlInva
                      joint_bootstrap_filter, likelihood_adjoint, likelihood_gradient,
                      load_checkpoint, load_library,
                      load_observations,
-                     particle_metropolis, save_checkpoint, save_observations, surrogate_filter,
-                     validate_observations)
+                     particle_metropolis, pf_log_likelihoods, save_checkpoint,
+                     save_observations, surrogate_filter, validate_observations)
 
 __all__ = ["ALL", "BF16", "EXPORTS", "FP32", "FP64", "Ensemble", "HzlError",
            "HzlInvalidArgument", "Parameters", "Surrogate", "bootstrap_filter",
            "ensemble_kalman_filter", "joint_bootstrap_filter", "likelihood_adjoint", "likelihood_gradient",
            "load_checkpoint",
-           "load_library", "load_observations", "particle_metropolis", "save_checkpoint",
+           "load_library", "load_observations", "particle_metropolis", "pf_log_likelihoods",
+           "save_checkpoint",
            "save_observations", "surrogate_filter", "validate_observations"]

@@ -886,6 +886,48 @@ def particle_metropolis(particles: Ensemble, observations, dt: float, initial_fi
     return dict(q=q, log_likelihood=ll, log_density=dens, accepted=acc.astype(bool))
 
 
+def pf_log_likelihoods(particles: Ensemble, observations, dt: float, initial_field, q, seeds,
+                       nu_bounds=(2e-4, 5e-3), amp_bounds=(0.02, 0.5), friction: float = 0.0,
+                       forcing_sigma: float = 0.05, surrogate: "Surrogate | None" = None,
+                       surrogate_precision: str = "bf16") -> np.ndarray:
+    """Log-likelihood estimates of len(q) independent bootstrap filters over
+    the ensemble's members (members / len(q) particles each): filter f at
+    q[f] = (log nu, log amplitude) with filter seed seeds[f], its particles
+    restarting from initial_field at step 0 (hzl_pf_log_likelihoods; the
+    likelihood the pMCMC chains evaluate, src/filter.cpp:141-149).  -inf
+    outside the log-uniform prior box.  With a surrogate, every interval's
+    forecast is the network's."""
+    obs = np.asarray(observations, dtype=np.float64)
+    qs = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 2)
+    sd_ = np.ascontiguousarray(seeds, dtype=np.uint64)
+    F = qs.shape[0]
+    if sd_.shape != (F,):
+        raise ValueError("one filter seed per parameter point")
+    o = PmcmcOptions(F, 0, 0.0,
+                     (C.c_double * 2)(np.log(nu_bounds[0]), np.log(amp_bounds[0])),
+                     (C.c_double * 2)(np.log(nu_bounds[1]), np.log(amp_bounds[1])),
+                     friction, forcing_sigma, 0, PRIOR_LOG_UNIFORM, None,
+                     surrogate._h.value if surrogate is not None else None,
+                     _SUR_PRECISION[surrogate_precision])
+    ip = C.POINTER(C.c_int)
+    step = np.ascontiguousarray(obs[:, 0], dtype=np.int32)
+    x = np.ascontiguousarray(obs[:, 1], dtype=np.int32)
+    y = np.ascontiguousarray(obs[:, 2], dtype=np.int32)
+    val = np.ascontiguousarray(obs[:, 3])
+    sd = np.ascontiguousarray(obs[:, 4])
+    f = _fields(initial_field, 1, particles.n)
+    ll = np.zeros(F)
+    _check(_lib.hzl_pf_log_likelihoods(particles._h, C.byref(o), dt, len(obs),
+                                       step.ctypes.data_as(ip), x.ctypes.data_as(ip),
+                                       y.ctypes.data_as(ip), val.ctypes.data_as(_dp),
+                                       sd.ctypes.data_as(_dp), f.ctypes.data_as(_dp), F,
+                                       qs.ctypes.data_as(_dp),
+                                       sd_.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                       ll.ctypes.data_as(_dp)))
+    particles._filters = F
+    return ll
+
+
 _SUR_PRECISION = {"fp32": FP32, "bf16": BF16}
 
 

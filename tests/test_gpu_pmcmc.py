@@ -1,14 +1,17 @@
 """GPU batched particle-marginal Metropolis-Hastings (csrc/pmcmc.hip) against
 the FP64 oracle replay of the reference's particle_metropolis
 (tests/oracle_filters.py), chain by chain: identical proposals, acceptance
-decisions and retained states, likelihood estimates to 1e-9 relative.
+decisions and retained states, likelihood estimates to 1e-9 relative; the
+batched likelihood evaluation against the oracle's filters, and with a
+surrogate forecast against the same filter composed by hand from the network
+forecast and the filter steps.
 """
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
-from oracle_filters import oracle_pmcmc
+from oracle_filters import oracle_bootstrap_filter, oracle_pmcmc
 from test_gpu_filter import synthetic
 
 pytestmark = pytest.mark.gpu
@@ -73,3 +76,62 @@ def test_pmcmc_rejects_bad_configuration(pkg, orc):
             pkg.particle_metropolis(ens, rows, 2e-3, init, np.zeros((3, 2)), 2, 0.1, 1)
         with pytest.raises(pkg.HzlInvalidArgument):  # outside the prior
             pkg.particle_metropolis(ens, rows, 2e-3, init, [[0.0, 0.0], [0.0, 0.0]], 2, 0.1, 1)
+
+
+def test_batched_likelihoods_match_oracle_filters(pkg, orc):
+    """hzl_pf_log_likelihoods: four filters of 8 FP64 particles at their own
+    (nu, amplitude) and seeds, each the oracle's bootstrap filter from the
+    same initial field; a point outside the prior box gives -inf."""
+    n, P, dt = 32, 8, 2e-3
+    rows, _ = synthetic(orc, n, 1e-3, dt, groups=3, every=10, nobs=16, truth_key=557, seed=6,
+                        sd=0.5)
+    init = orc.grf(n, 557)
+    q = np.log([[1e-3, 0.1], [2e-3, 0.05], [6e-4, 0.3], [1e-2, 0.1]])  # last: nu outside
+    seeds = [11, 2 ** 63 + 5, 77, 3]
+    with pkg.Ensemble(n, 4 * P, "fp64", nu=1e-3, sigma=0.1, amp=0.1, kf=4, seed=1) as ens:
+        got = pkg.pf_log_likelihoods(ens, rows, dt, init, q, seeds, forcing_sigma=0.1)
+    assert got[3] == -np.inf
+    for f in range(3):
+        want = oracle_bootstrap_filter(orc, np.stack([init] * P), rows, dt, seeds[f],
+                                       np.exp(q[f, 0]), amp=np.exp(q[f, 1]),
+                                       sigma=0.1)["log_likelihood"]
+        print(f, got[f], want)
+        assert abs(got[f] - want) <= 1e-9 * abs(want)
+
+
+def test_surrogate_likelihoods_are_the_network_forecast_filter(pkg, orc):
+    """With a surrogate the batched likelihoods (and so the pMCMC chains)
+    forecast every interval with the network: the same numbers as filters
+    composed by hand from Surrogate.forecast and pf_assimilate with no solver
+    steps, for the same members, parameters and seeds."""
+    n, P, F, dt, H = 64, 16, 3, 2e-3, 25
+    rows, _ = synthetic(orc, n, 1e-3, dt, groups=3, every=H, nobs=16, truth_key=558, seed=7,
+                        sd=0.3)
+    init = orc.grf(n, 558)
+    q = np.log([[1e-3, 0.1], [2e-3, 0.05], [6e-4, 0.3]])
+    seeds = [5, 6, 7]
+    with pkg.Surrogate(n, 32, 16, 2, 4, seed=9) as sur, \
+            pkg.Ensemble(n, F * P, "fp32", nu=1e-3, sigma=0.05, amp=0.1, seed=1) as ens:
+        sur.set_scales(1.5, 4.0)
+        got = pkg.pf_log_likelihoods(ens, rows, dt, init, q, seeds, surrogate=sur,
+                                     surrogate_precision="fp32")
+        ens.fill_state(init)
+        ens.step_index = 0
+        for f in range(F):
+            for i in range(P):
+                ens.set_member_parameters(f * P + i, np.exp(q[f, 0]), 0.0, 0.05,
+                                          np.exp(q[f, 1]))
+        groups = np.unique(rows[:, 0].astype(int))
+        ens.pf_begin_filters(seeds, max_groups=len(groups) + 1)
+        now = 0
+        for g, t in enumerate(groups):
+            sel = rows[rows[:, 0] == t]
+            sur.forecast(ens, dt, int(t - now), "fp32")
+            ens.pf_assimilate(dt, 0, sel[:, 1], sel[:, 2], sel[:, 3], sel[:, 4],
+                              g == len(groups) - 1)
+            now = int(t)
+        want, _ = ens.pf_result()
+        solver = pkg.pf_log_likelihoods(ens, rows, dt, init, q, seeds)
+    print(got, want, solver)
+    np.testing.assert_array_equal(got, want)
+    assert np.all(np.isfinite(got)) and not np.array_equal(got, solver)
