@@ -31,7 +31,7 @@ SEED = 20261203
 
 
 def model_bytes(n: int, itemsize: int = 8) -> dict:
-    """Algorithmic bytes per member (complex FP32 = 8 bytes)."""
+    """Algorithmic bytes per member (complex FP32 = 8 bytes, FP64 = 16)."""
     K = (n - 1) // 3
     S = (K + 1) * (2 * K + 1) * itemsize  # retained half-plane state
     M = (K + 1) * n * itemsize  # one mixed (kx spectral, y physical) field
@@ -45,6 +45,8 @@ def model_bytes(n: int, itemsize: int = 8) -> dict:
         # traffic 2/4/5/4 S_r over stages 1..4 (stage 1 reads w_n and writes
         # acc; stage 2 rebuilds w2 from acc) -> 3.75 S_r
         "col_launch": 5 * M + 3.75 * S,
+        # what the kernels must move per member-step: 4 x (row + col)
+        "kernel_member_step": 4 * (5 * M + 5 * M + 3.75 * S),
     }
 
 
@@ -136,8 +138,10 @@ def cpu_baseline(n: int, nu: float, dt: float) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc_mod  # CPU baseline leg only
 
-    threads = len(os.sched_getaffinity(0))
-    threads = max(1, min(threads, 16))
+    # worker threads: the CPUs this process may run on, at most 16 (one GPU's
+    # share of the host; the rest belong to the node's other GPUs)
+    affinity = len(os.sched_getaffinity(0))
+    threads = max(1, min(affinity, 16))
     members, steps = 2 * threads, 24
     if os.path.exists(orc_mod.REF_SO):
         ref = orc_mod.Reference()
@@ -157,7 +161,39 @@ def cpu_baseline(n: int, nu: float, dt: float) -> dict:
         what = (f"FP64 oracle restatement, {members} members x {steps} steps at {n}^2, "
                 f"{threads} OpenMP threads")
     return {"value": members * steps / sec, "unit": "member-steps/s", "cores": threads,
-            "kind": kind, "sample": what, "seconds": round(sec, 3)}
+            "kind": kind, "sample": what, "seconds": round(sec, 3),
+            "host_nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "threads_rule": "min(affinity, 16): one GPU's share of the host CPUs"}
+
+
+def fp64_steps(pkg, n: int, B: int, steps: int, warmup: int, dt: float, nu: float,
+               tune: bool) -> dict:
+    """The headline workload in FP64 (the reference's precision): same
+    members, grid, forcing and step count, timed with HIP events on the
+    engine's stream."""
+    ens = pkg.Ensemble(n, B, "fp64", nu=nu, alpha=0.0, sigma=0.05, kmin=1, kmax=4, amp=0.1,
+                       kf=4, seed=SEED)
+    if tune:
+        ens.autotune()
+    ens.init_grf()
+    ens.step(dt, warmup)
+    ens.synchronize()
+    timer = HipTimer()
+    e0, e1 = timer.event(), timer.event()
+    timer.record(e0, ens.stream)
+    ens.step(dt, steps)
+    timer.record(e1, ens.stream)
+    ms = timer.elapsed_ms(e0, e1)
+    ens.synchronize()
+    tuning = ens.tuning
+    ens.close()
+    mb = model_bytes(n, 16)
+    value = B * steps / (ms / 1e3)
+    return {"metric": f"member RK4 steps/s @{n}^2 FP64", "value": round(value, 1),
+            "unit": "member-steps/s", "ms_per_step": round(ms / steps, 4), "members": B,
+            "steps": steps, "tuning": list(tuning),
+            "step_model_frac_of_hbm_peak": round(value * mb["member_step"] / HBM_PEAK, 4),
+            "kernel_frac_of_hbm_peak": round(value * mb["kernel_member_step"] / HBM_PEAK, 4)}
 
 
 def pmcmc_replay_64(pkg, chains: int = 4, particles: int = 8, samples: int = 6) -> dict:
@@ -818,6 +854,9 @@ def main() -> int:
         "step_model": {"bytes_per_member_step": mb["member_step"],
                        "achieved_GBps": round(value * mb["member_step"] / 1e9, 1),
                        "frac_of_hbm_peak": round(value * mb["member_step"] / HBM_PEAK, 4),
+                       "kernel_bytes_per_member_step": int(mb["kernel_member_step"]),
+                       "kernel_frac_of_hbm_peak":
+                           round(value * mb["kernel_member_step"] / HBM_PEAK, 4),
                        "wall_seconds_timed": round(wall, 4)},
         "tuning": {"col_variant": cv, "row_variant": rv,
                    "col_ms_per_step": [round(x, 4) for x in tune_col],
@@ -836,6 +875,7 @@ def main() -> int:
         "sanity": {"mean_enstrophy": float(diag[:, 1].mean()),
                    "max_cfl_rate": float(diag[:, 4].max())},
     }
+    line["fp64"] = fp64_steps(pkg, n, B, args.steps, args.warmup, dt, nu, not args.no_tune)
     if not args.no_pf:
         line["surrogate_pf"] = surrogate_pf(pkg)
     if not args.no_enkf:

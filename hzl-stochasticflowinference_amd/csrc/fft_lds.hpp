@@ -1,9 +1,10 @@
 // fft_lds.hpp -- register + LDS FFT building blocks for gfx950 (CDNA4).
 //
 // A length-N transform is owned by a group of T = N/E threads that each hold E
-// complex values in registers.  It runs as two passes (the "four-step"
-// factorisation N = T * E, Van Loan 1992 sec. 3.3; Stockham-style autosort so
-// no bit reversal is ever materialised):
+// complex values in registers.  When T <= E it runs as two passes (the
+// "four-step" factorisation N = T * E, Van Loan 1992 sec. 3.3; Stockham-style
+// autosort so no bit reversal is ever materialised); when T > E (few values
+// per lane, many lanes per line) as three passes, GroupFFT3 below.  Two passes:
 //
 //   forward  in : v[j]          = x[t + T*j]                (j < E)
 //            pass 1: E-point DFT in registers over j, twiddle W_N^{t*k2}
@@ -26,6 +27,7 @@
 // barrier.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #define HZL_HD __host__ __device__ __forceinline__
 
@@ -145,13 +147,13 @@ template <int L, bool INV, typename R> HZL_HD void dft(cx<R> *v) {
     }
 }
 
-// Two-pass transform of length N by a group of T = N/E threads.
+// Two-pass transform of length N by a group of T = N/E threads (T | E).
 // `lds` is the group's private region of at least LDS_ELEMS entries.
 // `tw[t*E + k2]` = exp(-2 pi i t k2 / N) for t < T, k2 < E.
 // ctx.wsync() orders the exchange within the wavefront; every lane of the
 // wavefront must call forward/inverse the same number of times (idle groups
 // included).
-template <int N, int E, typename R> struct GroupFFT {
+template <int N, int E, typename R> struct GroupFFT2 {
     static constexpr int T = N / E;
     static constexpr int Q = E / T;
     static constexpr int LD = T + 1;
@@ -273,5 +275,127 @@ template <int N, int E, typename R> struct GroupFFT {
         ctx.wsync();
     }
 };
+
+// Three-pass transform of length N = E * T by a group of T = E * F lanes
+// (F | E) that each hold E values -- the split for lanes that hold few values
+// (E = 8: N = 512 over 64 lanes, 256 over 32, 128 over 16).  Natural order in
+// and out:
+//
+//   n = t + T j              (lane t, slot j),  t = t1 + E t2, t1 < E, t2 < F
+//   k = k0 + E k1 + T k2     (k0 < E, k1 < F, k2 < E)
+//   pass 1, lane t:          E-point DFT over j -> k0, twiddle W_N^{t k0}
+//   exchange 1:              lane t takes the pairs (t1, k0) = (p % E, p / E),
+//                            p = t + T r for r < E / F, each with its F values
+//                            over t2
+//   pass 2:                  F-point DFTs over t2 -> k1, twiddle W_T^{t1 k1}
+//   exchange 2:              lane L = k0 + E k1 takes its E values over t1
+//   pass 3:                  E-point DFT over t1 -> k2
+//
+// so lane L ends with X[L + T k2] in slot k2.  As input and output orders
+// coincide, the inverse is the same three passes with conjugate twiddles.
+// Twiddle table (host): tw[t E + k0] = W_N^{t k0} (t < T, k0 < E), then
+// tw[N + t1 F + k1] = W_T^{t1 k1} (t1 < E, k1 < F).
+template <int N, int E, typename R> struct GroupFFT3 {
+    static constexpr int T = N / E;
+    static constexpr int F = T / E;
+    static constexpr int RP = E / F; // (t1, k0) pairs per lane in pass 2
+    static constexpr int Q = 1;      // a lane's twiddles fit in registers
+    // exchange pitches: rows of T (pass-1 output, one row per k0) padded so
+    // the pass-2 reads of E consecutive lanes per row spread over the banks;
+    // rows of E ((k0, k1) groups) padded to an odd pitch
+    static constexpr int P1 = T + 8;
+    static constexpr int P2 = E + 1;
+    static constexpr int LDS_ELEMS = E * P1 > T * P2 ? E * P1 : T * P2;
+    static constexpr int TABLE = N + T; // twiddle table entries
+    static_assert(T * E == N && F * E == T && F >= 1 && F <= E && (E % F) == 0,
+                  "need N = E * E * F with F | E");
+    static_assert(LDS_ELEMS >= N, "LDS region must hold one natural-order line");
+    static_assert(64 % T == 0, "a group must not straddle a wavefront");
+
+    HZL_HD static constexpr int index(int t, int idx) {
+        return t + T * idx;
+    }
+    HZL_HD static constexpr bool slot_outside(int idx, int K) {
+        const int lo = T * idx;
+        return lo > K && lo + T - 1 < N - K;
+    }
+
+    struct Twiddles {
+        cx<R> w1[E]; // W_N^{t k0}
+        cx<R> w2[F]; // W_T^{t1 k1}, t1 = t % E
+    };
+    HZL_HD static Twiddles load_twiddles(const cx<R> *tw, int t) {
+        Twiddles r;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            r.w1[k] = tw[t * E + k];
+#pragma unroll
+        for (int k = 0; k < F; ++k)
+            r.w2[k] = tw[N + (t % E) * F + k];
+        return r;
+    }
+
+    template <class Ctx>
+    HZL_HD static void forward(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const Twiddles &tr) {
+        run<false>(ctx, v, lds, t, tr.w1, tr.w2);
+    }
+    template <class Ctx>
+    HZL_HD static void inverse(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const Twiddles &tr) {
+        run<true>(ctx, v, lds, t, tr.w1, tr.w2);
+    }
+    template <class Ctx>
+    HZL_HD static void forward(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const cx<R> *tw) {
+        run<false>(ctx, v, lds, t, tw + t * E, tw + N + (t % E) * F);
+    }
+    template <class Ctx>
+    HZL_HD static void inverse(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t,
+                               const cx<R> *tw) {
+        run<true>(ctx, v, lds, t, tw + t * E, tw + N + (t % E) * F);
+    }
+
+    // w1, w2: the lane's pass-1 and pass-2 twiddles (registers or table)
+    template <bool INV, class Ctx, class W1, class W2>
+    HZL_HD static void run(const Ctx &ctx, cx<R> (&v)[E], cx<R> *lds, int t, const W1 &w1,
+                           const W2 &w2) {
+        dft<E, INV>(v);
+#pragma unroll
+        for (int k0 = 0; k0 < E; ++k0)
+            lds[k0 * P1 + t] = k0 == 0 ? v[k0] : (INV ? cmulc(v[k0], w1[k0]) : cmul(v[k0], w1[k0]));
+        ctx.wsync();
+        const int t1 = t % E, kb = t / E;
+        cx<R> u[RP][F];
+#pragma unroll
+        for (int r = 0; r < RP; ++r)
+#pragma unroll
+            for (int t2 = 0; t2 < F; ++t2)
+                u[r][t2] = lds[(kb + F * r) * P1 + t1 + E * t2];
+        ctx.wsync(); // every read of exchange 1 precedes the writes of exchange 2
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+            dft<F, INV>(u[r]);
+#pragma unroll
+            for (int k1 = 0; k1 < F; ++k1) {
+                const cx<R> x = k1 == 0 ? u[r][k1]
+                                        : (INV ? cmulc(u[r][k1], w2[k1]) : cmul(u[r][k1], w2[k1]));
+                lds[((kb + F * r) * F + k1) * P2 + t1] = x;
+            }
+        }
+        ctx.wsync();
+        const int k0 = t % E, k1 = t / E;
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            v[j] = lds[(k0 * F + k1) * P2 + j];
+        dft<E, INV>(v);
+        ctx.wsync();
+    }
+};
+
+// The transform of a T x E split: two passes when each lane holds at least
+// T values, else three.
+template <int N, int E, typename R>
+using GroupFFT = std::conditional_t<(E >= N / E), GroupFFT2<N, E, R>, GroupFFT3<N, E, R>>;
 
 } // namespace hzl_amd

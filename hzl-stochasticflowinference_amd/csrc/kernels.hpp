@@ -33,24 +33,29 @@
 namespace hzl_amd {
 
 // ------------------------------------------------------------- shapes ----
+// Default split of the grid's transforms: E values per lane (two passes when
+// E >= N / E, three when E < N / E) and CT lanes per column workgroup.
 template <int N> struct Shape;
 template <> struct Shape<16> {
-    static constexpr int E = 4;
+    static constexpr int E = 4, CT = 64;
 };
 template <> struct Shape<32> {
-    static constexpr int E = 8;
+    static constexpr int E = 8, CT = 64;
 };
 template <> struct Shape<64> {
-    static constexpr int E = 16;
+    static constexpr int E = 16, CT = 64;
 };
 template <> struct Shape<128> {
-    static constexpr int E = 16;
+    static constexpr int E = 16, CT = 64;
 };
 template <> struct Shape<256> {
-    static constexpr int E = 16;
+    static constexpr int E = 16, CT = 64;
 };
+// 512: three passes over 64 lanes of 8 values (about 115 VGPRs in FP32)
+// rather than two over 16 lanes of 32 (about 250, one or two waves per SIMD);
+// four columns per workgroup
 template <> struct Shape<512> {
-    static constexpr int E = 32;
+    static constexpr int E = 8, CT = 256;
 };
 
 // Launch shape of the column and row kernels at grid N: EV values per lane
@@ -63,7 +68,8 @@ template <> struct Shape<512> {
 // default; other shapes are variants the autotuner times.  (Streaming the RK4
 // state, stage and accumulator arrays as well was measured 30% slower: later
 // stages re-read them from cache.)
-template <int N, int EV = Shape<N>::E, int CT = 64, int PPW = 0, int NT = 0> struct Geometry {
+template <int N, int EV = Shape<N>::E, int CT = Shape<N>::CT, int PPW = 0, int NT = 0>
+struct Geometry {
     static constexpr int NN = N;
     static constexpr bool STREAM = (NT & 1) != 0;
     static constexpr bool TABLE_TW = (NT & 2) != 0;

@@ -35,17 +35,24 @@ inline int elems_per_thread(int n) {
 
 // tw[t*E + k2] = exp(-2 pi i t k2 / n), evaluated in double, rounded once,
 // for the T x E split with E values per lane (default: the grid's Shape).
+// A three-pass split (T > E, T = E F) appends its pass-2 twiddles
+// tw[n + t1*F + k1] = exp(-2 pi i t1 k1 / T), t1 < E, k1 < F.
 template <typename R> std::vector<cx<R>> twiddles(int n, int E = 0) {
     if (E == 0)
         E = elems_per_thread(n);
     const int T = n / E;
-    std::vector<cx<R>> tw((size_t)n);
+    const int F = T > E ? T / E : 0;
+    std::vector<cx<R>> tw((size_t)n + (size_t)(E * F));
+    auto root = [](long p, long m) -> cx<R> {
+        const double a = -2.0 * PI * (double)(p % m) / (double)m;
+        return {(R)std::cos(a), (R)std::sin(a)};
+    };
     for (int t = 0; t < T; ++t)
-        for (int k2 = 0; k2 < E; ++k2) {
-            const long p = ((long)t * k2) % n;
-            const double a = -2.0 * PI * (double)p / (double)n;
-            tw[(size_t)(t * E + k2)] = {(R)std::cos(a), (R)std::sin(a)};
-        }
+        for (int k2 = 0; k2 < E; ++k2)
+            tw[(size_t)(t * E + k2)] = root((long)t * k2, n);
+    for (int t1 = 0; t1 < E && F; ++t1)
+        for (int k1 = 0; k1 < F; ++k1)
+            tw[(size_t)(n + t1 * F + k1)] = root((long)t1 * k1, T);
     return tw;
 }
 

@@ -3,7 +3,8 @@
 // register budgets of engine.hip).
 //
 // A column variant is (values per lane E, i.e. the T x E split of the y
-// transform; lanes per workgroup; waves-per-EU hint; non-temporal streams).
+// transform -- two passes when E >= T, three when E = 8 < T; lanes per
+// workgroup; waves-per-EU hint; non-temporal streams).
 // A row variant is (E of the x transforms; row pairs per workgroup;
 // waves-per-EU hint; staged or direct spectra; non-temporal streams).  Every
 // variant computes the same stage; only the FFT factorisation (and so the
@@ -30,7 +31,14 @@
     X(256, 8, 16, 128, 1, 3)                                                                    \
     X(512, 1, 32, 128, 1, 0)                                                                    \
     X(512, 2, 32, 64, 2, 0)                                                                     \
-    X(512, 3, 32, 64, 1, 1)
+    X(512, 3, 32, 64, 1, 1)                                                                     \
+    X(512, 4, 8, 64, 2, 0)                                                                      \
+    X(512, 5, 8, 128, 2, 0)                                                                     \
+    X(512, 6, 8, 256, 4, 0)                                                                     \
+    X(512, 7, 8, 256, 4, 1)                                                                     \
+    X(512, 8, 8, 128, 4, 0)                                                                     \
+    X(256, 9, 8, 128, 4, 0)                                                                     \
+    X(256, 10, 8, 256, 2, 1)
 
 // X(N, index, E, row pairs per workgroup, waves per EU, direct, NT flags of Geometry (1:
 // non-temporal loads of the mixed fields, 2: twiddles from the table)): direct = 1 builds
@@ -55,7 +63,14 @@
     X(256, 11, 16, 16, 3, 0, 3)                                                                 \
     X(512, 1, 32, 8, 3, 0, 0)                                                                   \
     X(512, 2, 32, 8, 1, 1, 0)                                                                   \
-    X(512, 3, 32, 8, 2, 0, 1)
+    X(512, 3, 32, 8, 2, 0, 1)                                                                   \
+    X(512, 4, 8, 8, 4, 0, 0)                                                                    \
+    X(512, 5, 8, 8, 2, 1, 0)                                                                    \
+    X(512, 6, 8, 4, 4, 1, 0)                                                                    \
+    X(512, 7, 8, 8, 4, 0, 1)                                                                    \
+    X(256, 12, 8, 8, 4, 0, 0)                                                                   \
+    X(256, 13, 8, 8, 2, 1, 0)                                                                   \
+    X(256, 14, 8, 4, 4, 1, 0)
 
 namespace hzl_amd {
 

@@ -375,11 +375,13 @@ int run_any(int n, int B, double nu, double alpha, double sigma, double amp, int
     return -1;
 }
 
-// Batch of 1-D transforms through GroupFFT: lines[L][N] in natural order.
-template <int N, typename R> int fft_lines(int lines, int inverse, double *data) {
-    using F = GroupFFT<N, Shape<N>::E, R>;
-    constexpr int E = Shape<N>::E, T = N / E;
-    const auto tw = twiddles<R>(N);
+// Batch of 1-D transforms through GroupFFT (E values per lane; two or three
+// passes by the split): lines[L][N] in natural order.
+template <int N, typename R, int E = Shape<N>::E> int fft_lines(int lines, int inverse,
+                                                                 double *data) {
+    using F = GroupFFT<N, E, R>;
+    constexpr int T = N / E;
+    const auto tw = twiddles<R>(N, E);
     std::vector<cx<R>> buf((size_t)lines * N);
     for (size_t i = 0; i < buf.size(); ++i)
         buf[i] = {(R)data[2 * i], (R)data[2 * i + 1]};
@@ -667,6 +669,23 @@ int emu_fft_lines(int n, int precision, int lines, int inverse, double *data) {
         HZL_EMU_FFT(256)
         HZL_EMU_FFT(512)
     }
+#undef HZL_EMU_FFT
+    return -1;
+}
+
+// The same with E values per lane: the splits of the launch-shape variants
+// (three-pass where E < n / E).
+int emu_fft_lines_split(int n, int E, int precision, int lines, int inverse, double *data) {
+#define HZL_EMU_FFT(NN, EE)                                                                    \
+    if (n == NN && E == EE)                                                                    \
+        return precision == 0 ? fft_lines<NN, float, EE>(lines, inverse, data)                 \
+                              : fft_lines<NN, double, EE>(lines, inverse, data);
+    HZL_EMU_FFT(64, 8)
+    HZL_EMU_FFT(128, 8)
+    HZL_EMU_FFT(128, 32)
+    HZL_EMU_FFT(256, 8)
+    HZL_EMU_FFT(256, 32)
+    HZL_EMU_FFT(512, 8)
 #undef HZL_EMU_FFT
     return -1;
 }

@@ -44,6 +44,8 @@ class Emulator:
         L.emu_normal_stream.argtypes = [C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_long), _dp]
         L.emu_fft_lines.restype = C.c_int
         L.emu_fft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.emu_fft_lines_split.restype = C.c_int
+        L.emu_fft_lines_split.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
 
     def gemm(self, bf16, dims, strides, A, B, C0, bias=None, aux=None, want_pre=False,
              alpha=1.0, beta=0.0, act=0, c_bf16=False):
@@ -111,12 +113,18 @@ class Emulator:
         assert rc == 0
         return F, noise
 
-    def fft_lines(self, x, inverse=False, precision="fp32"):
+    def fft_lines(self, x, inverse=False, precision="fp32", elems=None):
+        """GroupFFT over each row of x; elems: values per lane (default: the
+        grid's shape)."""
         d = np.ascontiguousarray(x, dtype=np.complex128).copy()
         lines, n = d.shape
-        rc = self.lib.emu_fft_lines(n, 0 if precision == "fp32" else 1, lines, int(inverse),
-                                    d.ctypes.data_as(_dp))
-        assert rc == 0
+        p = 0 if precision == "fp32" else 1
+        if elems is None:
+            rc = self.lib.emu_fft_lines(n, p, lines, int(inverse), d.ctypes.data_as(_dp))
+        else:
+            rc = self.lib.emu_fft_lines_split(n, elems, p, lines, int(inverse),
+                                              d.ctypes.data_as(_dp))
+        assert rc == 0, "no such split"
         return d
 
     def run(self, n, members, precision, dt, steps, nu=1e-3, alpha=0.0, sigma=0.05, amp=0.1,

@@ -27,6 +27,19 @@ def test_group_fft_lines(emu, n, precision, tol):
     assert rel(emu.fft_lines(x, True, precision), np.fft.ifft(x, axis=1) * n) < tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", 3e-7), ("fp64", 2e-15)])
+@pytest.mark.parametrize("n,elems", [(64, 8), (128, 8), (128, 32), (256, 8), (256, 32),
+                                     (512, 8)])
+def test_group_fft_splits(emu, n, elems, precision, tol):
+    """Other T x E splits of the launch-shape variants: two passes where each
+    lane holds at least T values, three passes (E = 8 over 8 F lanes) where
+    it holds fewer; forward and inverse against numpy."""
+    rng = np.random.default_rng(n + elems)
+    x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    assert rel(emu.fft_lines(x, False, precision, elems), np.fft.fft(x, axis=1)) < tol
+    assert rel(emu.fft_lines(x, True, precision, elems), np.fft.ifft(x, axis=1) * n) < tol
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
 @pytest.mark.parametrize("n", [16, 32, 64])
 def test_rk4_pipeline_matches_oracle(emu, orc, n, precision, tol):
@@ -54,7 +67,9 @@ def test_forced_pipeline_matches_oracle(emu, orc):
                                      (128, 0, 2), (128, 0, 3), (256, 2, 0), (256, 0, 3),
                                      (256, 0, 4), (128, 0, 4), (256, 0, 5),
                                      (256, 7, 0), (256, 0, 9), (256, 0, 10),
-                                     (256, 0, 11)])
+                                     (256, 0, 11), (256, 9, 0), (256, 10, 0), (256, 0, 12),
+                                     (256, 0, 13), (256, 0, 14), (512, 4, 0), (512, 0, 4),
+                                     (512, 0, 5)])
 def test_launch_shape_variants_match_oracle(emu, orc, n, cv, rv):
     """Autotuner variants (other T x E splits, workgroup sizes, row tiles)
     compute the same RK4 step as the FP64 oracle."""

@@ -5,7 +5,7 @@
   the downloaded forecast states and the same perturbations (the split-K
   products over the 7310-long state axis at their real size);
 * config 2: the joint particle filter over 256^2 and 512^2 particles (FP32,
-  the tuned E = 32 transforms at 512^2, the cross-grid gather on resampling)
+  the three-pass transforms at 512^2, the cross-grid gather on resampling)
   against the FP64 oracle's joint filter;
 * config 4: batched pMCMC at 256^2 (FP32) against the oracle's chains.
 """
