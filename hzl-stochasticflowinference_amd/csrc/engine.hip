@@ -39,11 +39,11 @@ namespace hzl_amd {
 #endif
 
 template <int N, int MODE, typename R>
-__global__ void __launch_bounds__(Geometry<N>::COL_THREADS)
+__global__ void __launch_bounds__((DefaultGeometry<N, R>::COL_THREADS))
     __attribute__((amdgpu_waves_per_eu(HZL_COL_WPE))) k_col(StageArgs<R> a) {
-    using G = Geometry<N>;
+    using G = DefaultGeometry<N, R>;
     __shared__ cx<R> lds[G::COL_G * GroupFFT<N, G::E, R>::LDS_ELEMS];
-    col_body<N, MODE, R>(DevCtx{}, a, lds, (long)blockIdx.x);
+    col_body<N, MODE, R, G>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 
 template <typename R> __global__ void __launch_bounds__(256) k_noise(NoiseArgs<R> a) {
@@ -66,15 +66,17 @@ template <typename R> __global__ void __launch_bounds__(OU_LANES) k_ou(OuArgs<R>
 }
 
 template <int N, typename R, bool CFL>
-__global__ void __launch_bounds__(Geometry<N>::ROW_THREADS)
+__global__ void __launch_bounds__((DefaultGeometry<N, R>::ROW_THREADS))
     __attribute__((amdgpu_waves_per_eu(HZL_ROW_WPE))) k_row(RowArgs<R> a) {
-    __shared__ cx<R> lds[RowShape<Geometry<N>>::template lds_elems<R>()];
-    row_body<N, R, CFL>(DevCtx{}, a, lds, (long)blockIdx.x);
+    using G = DefaultGeometry<N, R>;
+    __shared__ cx<R> lds[RowShape<G>::template lds_elems<R>()];
+    row_body<N, R, CFL, G>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 
 template <int N, typename R>
-__global__ void __launch_bounds__(Geometry<N>::ROW_THREADS) k_row_products(ProductArgs<R> a) {
-    using G = Geometry<N>;
+__global__ void __launch_bounds__((DefaultGeometry<N, R>::ROW_THREADS))
+    k_row_products(ProductArgs<R> a) {
+    using G = DefaultGeometry<N, R>;
     __shared__ cx<R> lds[G::ROW_PPW * GroupFFT<N, G::E, R>::LDS_ELEMS];
     row_body_products<N, R, G>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
@@ -297,7 +299,7 @@ template <typename T> struct DeviceBuffer {
 };
 
 template <int N, typename R> void launch_col(int mode, const StageArgs<R> &a, hipStream_t s) {
-    using G = Geometry<N>;
+    using G = DefaultGeometry<N, R>;
     const long cols = (long)a.B * G::KP;
     const dim3 grid((unsigned)((cols + G::COL_G - 1) / G::COL_G)), block(G::COL_THREADS);
     switch (mode) {
@@ -339,7 +341,7 @@ template <int N, typename R> void launch_col(int mode, const StageArgs<R> &a, hi
 }
 
 template <int N, typename R> void launch_products(const ProductArgs<R> &a, hipStream_t s) {
-    using G = Geometry<N>;
+    using G = DefaultGeometry<N, R>;
     const long pairs = (long)a.B * (N / 2);
     hipLaunchKernelGGL((k_row_products<N, R>),
                        dim3((unsigned)((pairs + G::ROW_PPW - 1) / G::ROW_PPW)),
@@ -365,7 +367,7 @@ template <typename R> void dispatch_products(int n, const ProductArgs<R> &a, hip
 }
 
 template <int N, typename R> void launch_row(const RowArgs<R> &a, hipStream_t s) {
-    using G = Geometry<N>;
+    using G = DefaultGeometry<N, R>;
     const long pairs = (long)a.B * (N / 2);
     const long blocks = (pairs + G::ROW_PPW - 1) / G::ROW_PPW;
     if (a.cfl)
@@ -1109,8 +1111,8 @@ template <typename R> class Engine final : public EngineBase {
             throw std::invalid_argument("no such launch-shape variant for this grid and precision");
         check(hipStreamSynchronize(stream_), "tuning switch");
         clear_graphs(); // captured launches name the previous kernels
-        const int ec = cv ? col_variant_elems(n_, cv) : elems_per_thread(n_);
-        const int er = rv ? row_variant_elems(n_, rv) : elems_per_thread(n_);
+        const int ec = cv ? col_variant_elems(n_, cv) : default_split<R>(n_);
+        const int er = rv ? row_variant_elems(n_, rv) : default_split<R>(n_);
         tw_col_ = twiddle_table(ec);
         tw_row_ = twiddle_table(er);
         col_var_ = cv;
@@ -1705,7 +1707,7 @@ template <typename R> class Engine final : public EngineBase {
 
     // Twiddle table of the T x E split (the default one is tw_).
     const cx<R> *twiddle_table(int E) {
-        if (E == elems_per_thread(n_))
+        if (E == default_split<R>(n_))
             return tw_.p;
         for (auto &t : tw_extra_)
             if (t.first == E)

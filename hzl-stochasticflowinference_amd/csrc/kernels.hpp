@@ -89,6 +89,18 @@ struct Geometry {
     static constexpr long MIXF = (long)KP * N; // complex entries per mixed field
 };
 
+// Default launch shape of precision R at grid N.  An FP64 value takes two
+// registers, so from 128^2 up FP64 takes the three-pass split (8 values per
+// lane) where FP32 holds 16 or 32: the two-pass FP64 kernels there needed 256
+// VGPRs (one wave per SIMD) and the row pass spilled to scratch.
+template <int N, typename R> struct DefaultShape {
+    static constexpr bool WIDE = sizeof(R) == 8 && N >= 128;
+    static constexpr int E = WIDE ? 8 : Shape<N>::E;
+    static constexpr int CT = WIDE ? (N == 512 ? 256 : 128) : Shape<N>::CT;
+};
+template <int N, typename R>
+using DefaultGeometry = Geometry<N, DefaultShape<N, R>::E, DefaultShape<N, R>::CT>;
+
 // 1/x: correctly rounded division in FP64; the hardware reciprocal (1 ulp)
 // in FP32, where it feeds psi = w / |k|^2.
 HZL_HD double recip(double x) {

@@ -33,13 +33,33 @@ inline int elems_per_thread(int n) {
     throw std::invalid_argument("grid must be a power of two between 16 and 512");
 }
 
+// Values per lane of the default split of precision R (DefaultShape).
+template <typename R> int default_split(int n) {
+    switch (n) {
+    case 16:
+        return DefaultShape<16, R>::E;
+    case 32:
+        return DefaultShape<32, R>::E;
+    case 64:
+        return DefaultShape<64, R>::E;
+    case 128:
+        return DefaultShape<128, R>::E;
+    case 256:
+        return DefaultShape<256, R>::E;
+    case 512:
+        return DefaultShape<512, R>::E;
+    }
+    throw std::invalid_argument("grid must be a power of two between 16 and 512");
+}
+
 // tw[t*E + k2] = exp(-2 pi i t k2 / n), evaluated in double, rounded once,
-// for the T x E split with E values per lane (default: the grid's Shape).
+// for the T x E split with E values per lane (default: the default split of
+// precision R).
 // A three-pass split (T > E, T = E F) appends its pass-2 twiddles
 // tw[n + t1*F + k1] = exp(-2 pi i t1 k1 / T), t1 < E, k1 < F.
 template <typename R> std::vector<cx<R>> twiddles(int n, int E = 0) {
     if (E == 0)
-        E = elems_per_thread(n);
+        E = default_split<R>(n);
     const int T = n / E;
     const int F = T > E ? T / E : 0;
     std::vector<cx<R>> tw((size_t)n + (size_t)(E * F));

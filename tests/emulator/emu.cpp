@@ -101,7 +101,7 @@ template <class Body> void launch(long blocks, int threads, size_t lds_bytes, Bo
 
 // GC / GR: launch shapes of the column and row kernels (variants.hpp);
 // DIRECT: row spectra straight from the mixed fields (row_body_direct).
-template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>, bool DIRECT = false>
+template <int N, typename R, class GC = DefaultGeometry<N, R>, class GR = DefaultGeometry<N, R>, bool DIRECT = false>
 struct Sim {
     using G = Geometry<N>;
     using FFT = GroupFFT<N, GC::E, R>;
@@ -326,7 +326,7 @@ struct Sim {
     }
 };
 
-template <int N, typename R, class GC = Geometry<N>, class GR = Geometry<N>, bool DIRECT = false>
+template <int N, typename R, class GC = DefaultGeometry<N, R>, class GR = DefaultGeometry<N, R>, bool DIRECT = false>
 int run(int B, double nu, double alpha, double sigma, double amp, int kf, int kmin, int kmax,
         uint64_t seed, double dt, int steps, int grf, const double *init, const double *forcing,
         int forcing_shared, double *out, double *diag) {
