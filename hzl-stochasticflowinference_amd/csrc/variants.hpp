@@ -38,7 +38,10 @@
     X(512, 7, 8, 256, 4, 1)                                                                     \
     X(512, 8, 8, 128, 4, 0)                                                                     \
     X(256, 9, 8, 128, 4, 0)                                                                     \
-    X(256, 10, 8, 256, 2, 1)
+    X(256, 10, 8, 256, 2, 1)                                                                    \
+    X(128, 5, 8, 128, 4, 0)                                                                     \
+    X(128, 6, 8, 256, 2, 1)                                                                     \
+    X(128, 7, 8, 64, 4, 1)
 
 // X(N, index, E, row pairs per workgroup, waves per EU, direct, NT flags of Geometry (1:
 // non-temporal loads of the mixed fields, 2: twiddles from the table)): direct = 1 builds
@@ -70,7 +73,10 @@
     X(512, 7, 8, 8, 4, 0, 1)                                                                    \
     X(256, 12, 8, 8, 4, 0, 0)                                                                   \
     X(256, 13, 8, 8, 2, 1, 0)                                                                   \
-    X(256, 14, 8, 4, 4, 1, 0)
+    X(256, 14, 8, 4, 4, 1, 0)                                                                   \
+    X(128, 6, 8, 8, 4, 0, 0)                                                                    \
+    X(128, 7, 8, 16, 4, 0, 1)                                                                   \
+    X(128, 8, 8, 8, 4, 1, 0)
 
 namespace hzl_amd {
 

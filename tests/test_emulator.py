@@ -69,7 +69,8 @@ def test_forced_pipeline_matches_oracle(emu, orc):
                                      (256, 7, 0), (256, 0, 9), (256, 0, 10),
                                      (256, 0, 11), (256, 9, 0), (256, 10, 0), (256, 0, 12),
                                      (256, 0, 13), (256, 0, 14), (512, 4, 0), (512, 0, 4),
-                                     (512, 0, 5)])
+                                     (512, 0, 5), (128, 5, 0), (128, 6, 0), (128, 7, 0),
+                                     (128, 0, 6), (128, 0, 7), (128, 0, 8)])
 def test_launch_shape_variants_match_oracle(emu, orc, n, cv, rv):
     """Autotuner variants (other T x E splits, workgroup sizes, row tiles)
     compute the same RK4 step as the FP64 oracle."""
