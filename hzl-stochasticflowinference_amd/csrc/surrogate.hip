@@ -81,6 +81,50 @@ __global__ void k_wcolsum(const T *x, const float *w, long M, int N, long ld, in
 __global__ void k_sum_chunks(const float *part, int chunks, long count, float *out, int add) {
     sum_chunks_body(part, chunks, count, out, add, HZL_GID);
 }
+// Column sums of x (M rows, N columns with 256 % N == 0, leading dimension
+// ld), each row weighted by w[m] when W: workgroup b sums rows
+// [b per, (b + 1) per) into part[b][n] with 256 / N row lanes per column,
+// combined in LDS in a fixed order (the result does not depend on timing).
+template <typename T, bool W>
+__global__ void __launch_bounds__(256)
+    k_colsum_blocks(const T *x, const float *w, long M, int N, long ld, long per, float *part) {
+    __shared__ float red[256];
+    const int tid = threadIdx.x, n = tid % N, lane = tid / N, lanes = 256 / N;
+    const long lo = (long)blockIdx.x * per, hi = lo + per < M ? lo + per : M;
+    float s = 0.f;
+    for (long m = lo + lane; m < hi; m += lanes)
+        s += (W ? w[m] : 1.f) * (float)x[m * ld + n];
+    red[tid] = s;
+    __syncthreads();
+    for (int half = lanes / 2; half >= 1; half /= 2) {
+        if (lane < half)
+            red[tid] += red[tid + half * N];
+        __syncthreads();
+    }
+    if (lane == 0)
+        part[(long)blockIdx.x * N + n] = red[tid];
+}
+// out[e] = sum over c < chunks of part[c][e] (+ out[e] when add): 32
+// consecutive elements x 8 chunk groups per workgroup, the groups combined in
+// a fixed order.
+__global__ void __launch_bounds__(256)
+    k_sum_chunks_blocks(const float *part, int chunks, long count, float *out, int add) {
+    __shared__ float red[256];
+    const int tid = threadIdx.x, el = tid % 32, g = tid / 32;
+    const long e = (long)blockIdx.x * 32 + el;
+    float s = 0.f;
+    if (e < count)
+        for (int c = g; c < chunks; c += 8)
+            s += part[(long)c * count + e];
+    red[tid] = s;
+    __syncthreads();
+    if (g == 0 && e < count) {
+        float t = add ? out[e] : 0.f;
+        for (int k = 0; k < 8; ++k)
+            t += red[k * 32 + el];
+        out[e] = t;
+    }
+}
 template <typename T>
 __global__ void k_rcat(const float *Rr, const float *Ri, T *Rcat, int modes, int C) {
     rcat_body<T>(Rr, Ri, Rcat, modes, C, HZL_GID);
@@ -1048,17 +1092,42 @@ class Surrogate {
         Gemm<T> g(M, N, (int)kc, X1, 1, ld1, X2, 1, ld2, vs(part_), N, 1, OUT_F32);
         g.batch(chunks, kc * ld1, kc * ld2, (long)M * N);
         g.run(st_);
-        hipLaunchKernelGGL(k_sum_chunks, grid1d((long)M * N), dim3(256), 0, st_, part_.p, chunks,
-                           (long)M * N, grad_.p + off, 0);
+        sum_chunks(chunks, (long)M * N, grad_.p + off);
+    }
+
+    // out[e] = sum over the chunks of part_[c][e]
+    void sum_chunks(int chunks, long count, float *out) {
+        hipLaunchKernelGGL(k_sum_chunks_blocks, dim3((unsigned)((count + 31) / 32)), dim3(256), 0,
+                           st_, part_.p, chunks, count, out, 0);
+    }
+
+    // out[n] = sum over `rows` rows of (w[m] *) X[m ld + n]: block partial sums
+    // over row ranges, then their sum
+    template <typename T, bool W>
+    void column_sums(const T *X, const float *wgt, long rows, int N, long ld, float *out) {
+        if (256 % N != 0) {
+            const int chunks = 256;
+            part_.alloc((size_t)chunks * N);
+            if constexpr (W)
+                hipLaunchKernelGGL(k_wcolsum<T>, grid1d((long)chunks * N), dim3(256), 0, st_, X,
+                                   wgt, rows, N, ld, chunks, part_.p);
+            else
+                hipLaunchKernelGGL(k_colsum<T>, grid1d((long)chunks * N), dim3(256), 0, st_, X,
+                                   rows, N, ld, chunks, part_.p);
+            hipLaunchKernelGGL(k_sum_chunks, grid1d(N), dim3(256), 0, st_, part_.p, chunks,
+                               (long)N, out, 0);
+            return;
+        }
+        const long per = 512;
+        const int blocks = (int)((rows + per - 1) / per);
+        part_.alloc((size_t)blocks * N);
+        hipLaunchKernelGGL((k_colsum_blocks<T, W>), dim3((unsigned)blocks), dim3(256), 0, st_, X,
+                           wgt, rows, N, ld, per, part_.p);
+        sum_chunks(blocks, (long)N, out);
     }
 
     template <typename T> void bias_grad(const T *X, long rows, int N, long ld, long off) {
-        const int chunks = 256;
-        part_.alloc((size_t)chunks * N);
-        hipLaunchKernelGGL(k_colsum<T>, grid1d((long)chunks * N), dim3(256), 0, st_, X, rows, N,
-                           ld, chunks, part_.p);
-        hipLaunchKernelGGL(k_sum_chunks, grid1d(N), dim3(256), 0, st_, part_.p, chunks, (long)N,
-                           grad_.p + off, 0);
+        column_sums<T, false>(X, nullptr, rows, N, ld, grad_.p + off);
     }
 
     template <typename T> void backward(FnoWork<T> &w, int S) {
@@ -1070,16 +1139,8 @@ class Surrogate {
                            0.f);
         // head: dq2 = sum dout; dQ2[j] = sum dout h[.][j]; dhpre = dout Q2 gelu'(hpre)
         {
-            const int chunks = 256;
-            part_.alloc((size_t)chunks * H);
-            hipLaunchKernelGGL(k_colsum<float>, grid1d(chunks), dim3(256), 0, st_, w.dout.p, SP, 1,
-                               1L, chunks, part_.p);
-            hipLaunchKernelGGL(k_sum_chunks, grid1d(1), dim3(256), 0, st_, part_.p, chunks, 1L,
-                               grad_.p + prm_.q2, 0);
-            hipLaunchKernelGGL(k_wcolsum<T>, grid1d((long)chunks * H), dim3(256), 0, st_, w.h.p,
-                               w.dout.p, SP, H, (long)H, chunks, part_.p);
-            hipLaunchKernelGGL(k_sum_chunks, grid1d(H), dim3(256), 0, st_, part_.p, chunks,
-                               (long)H, grad_.p + prm_.Q2, 0);
+            column_sums<float, false>(w.dout.p, nullptr, SP, 1, 1L, grad_.p + prm_.q2);
+            column_sums<T, true>(w.h.p, w.dout.p, SP, H, (long)H, grad_.p + prm_.Q2);
             hipLaunchKernelGGL(k_head_back<T>, grid1d(SP * H), dim3(256), 0, st_, w.dout.p,
                                master_.p + prm_.Q2, w.hpre.p, w.dh.p, SP, H);
         }
