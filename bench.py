@@ -177,6 +177,8 @@ def fp64_steps(pkg, n: int, B: int, steps: int, warmup: int, dt: float, nu: floa
         ens.autotune()
     ens.init_grf()
     ens.step(dt, warmup)
+    ens.step(dt, steps)  # launch by launch
+    ens.step(dt, steps)  # recorded as a graph, replayed by the timed call
     ens.synchronize()
     timer = HipTimer()
     e0, e1 = timer.event(), timer.event()
@@ -186,12 +188,13 @@ def fp64_steps(pkg, n: int, B: int, steps: int, warmup: int, dt: float, nu: floa
     ms = timer.elapsed_ms(e0, e1)
     ens.synchronize()
     tuning = ens.tuning
+    chains = ens.chains
     ens.close()
     mb = model_bytes(n, 16)
     value = B * steps / (ms / 1e3)
     return {"metric": f"member RK4 steps/s @{n}^2 FP64", "value": round(value, 1),
             "unit": "member-steps/s", "ms_per_step": round(ms / steps, 4), "members": B,
-            "steps": steps, "tuning": list(tuning),
+            "steps": steps, "tuning": list(tuning), "chains": chains,
             "step_model_frac_of_hbm_peak": round(value * mb["member_step"] / HBM_PEAK, 4),
             "kernel_frac_of_hbm_peak": round(value * mb["kernel_member_step"] / HBM_PEAK, 4)}
 
@@ -325,15 +328,19 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
     n = init.shape[-1]
     out = {}
     for kind in kinds:
+        # the judge's scratch ensemble lives as long as the filter setup (like
+        # the surrogate), so its solver steps replay their captured graph after
+        # the warm run
+        judge = None
+        if kind == "surrogate_judged":
+            judge = pkg.Ensemble(n, judge_members, "fp32", nu=nu, sigma=0.0, amp=amp,
+                                 seed=SEED + 5)
+
         def run(retain, seed=SEED):
             with pkg.Ensemble(n, particles, "fp32", nu=nu, sigma=sigma, amp=amp,
                               seed=SEED + 4) as pf:
                 pf.set_state(init)
                 pf.synchronize()
-                judge = None
-                if kind == "surrogate_judged":
-                    judge = pkg.Ensemble(n, judge_members, "fp32", nu=nu, sigma=0.0, amp=amp,
-                                         seed=SEED + 5)
                 t = time.perf_counter()
                 if kind == "free":  # no assimilation: the ensemble mean of the forecasts
                     steps = np.unique(rows[:, 0].astype(int))
@@ -356,8 +363,6 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
                                                judge_every=judge_every)
                 pf.synchronize()
                 sec = time.perf_counter() - t
-                if judge is not None:
-                    judge.close()
             return res, sec
 
         run(False)  # warm (captures graphs)
@@ -368,6 +373,8 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
             err += [rms_error(h["mean_field"], tr) for h, tr in zip(r["history"], truths)]
             if k == 0:
                 res = r
+        if judge is not None:
+            judge.close()
         cycles = len(res["history"])
         ess = [h["ess"] for h in res["history"]]
         o = {"cycles_per_s": round(cycles / sec, 2), "ms_per_cycle": round(sec / cycles * 1e3, 3),
@@ -794,6 +801,10 @@ def main() -> int:
     cv, rv = ens.tuning
     ens.init_grf()
     ens.step(dt, args.warmup)
+    # two untimed calls of the timed size: the first runs launch by launch,
+    # the second records the steps as a HIP graph, which the timed call replays
+    ens.step(dt, args.steps)
+    ens.step(dt, args.steps)
     ens.synchronize()
     timer = HipTimer()
     stream = ens.stream
@@ -807,6 +818,24 @@ def main() -> int:
     ens.synchronize()  # CFL / finiteness checks of every timed step
     ms_per_step = ms / args.steps
     value = B * args.steps / (ms / 1e3)
+    # the same steps with every launch over all members (one chain), for
+    # comparison with the tuned step schedule
+    chains = ens.chains
+    one_chain = None
+    if chains > 1:
+        ens.chains = 1
+        ens.step(dt, args.steps)
+        ens.step(dt, args.steps)
+        ens.synchronize()
+        ea, eb = timer.event(), timer.event()
+        timer.record(ea, stream)
+        ens.step(dt, args.steps)
+        timer.record(eb, stream)
+        one_ms = timer.elapsed_ms(ea, eb)
+        ens.synchronize()
+        ens.chains = chains
+        one_chain = {"member_steps_per_s": round(B * args.steps / (one_ms / 1e3), 1),
+                     "ms_per_step": round(one_ms / args.steps, 4)}
 
     # per-kernel durations (events around each launch, outside the timed region)
     prof = np.stack([ens.profile_step(dt) for _ in range(max(1, args.profile_steps))])
@@ -858,7 +887,8 @@ def main() -> int:
                        "kernel_frac_of_hbm_peak":
                            round(value * mb["kernel_member_step"] / HBM_PEAK, 4),
                        "wall_seconds_timed": round(wall, 4)},
-        "tuning": {"col_variant": cv, "row_variant": rv,
+        "tuning": {"col_variant": cv, "row_variant": rv, "chains": chains,
+                   "one_chain": one_chain,
                    "col_ms_per_step": [round(x, 4) for x in tune_col],
                    "row_ms_per_step": [round(x, 4) for x in tune_row]},
         "roofline": {"bound": "hbm", "kernel": label,

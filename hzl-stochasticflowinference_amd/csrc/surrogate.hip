@@ -378,14 +378,17 @@ class Surrogate {
         conv_build();
     }
     ~Surrogate() {
-        if (graph_) {
+        if (!graphs_.empty()) {
             (void)hipDeviceSynchronize();
-            (void)hipGraphExecDestroy(graph_);
+            for (auto &c : graphs_)
+                (void)hipGraphExecDestroy(c.exec);
         }
         if (own_) {
             (void)hipStreamSynchronize(own_);
             (void)hipStreamDestroy(own_);
         }
+        if (order_ev_)
+            (void)hipEventDestroy(order_ev_);
     }
     Surrogate(const Surrogate &) = delete;
     Surrogate &operator=(const Surrogate &) = delete;
@@ -702,6 +705,13 @@ class Surrogate {
         // counter is the ensemble's step index, so a forecast is a function of
         // the ensemble state alone
         st_ = ev.stream;
+        // the forecast's scratch buffers (activations, counter) are the
+        // surrogate's: a forecast on another ensemble's stream waits on the
+        // device for the previous one (outside any capture)
+        if (!order_ev_)
+            hcheck(hipEventCreateWithFlags(&order_ev_, hipEventDisableTiming), "event");
+        else if (order_stream_ != st_)
+            hcheck(hipStreamWaitEvent(st_, order_ev_, 0), "forecast order");
         counter_dev_.alloc(1);
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st_, counter_dev_.p,
                            eng.step_index());
@@ -715,11 +725,11 @@ class Surrogate {
                            alloc_generation()};
         if (!use_graph_ || ev.B > 1024) {
             launch_forecast(ev, steps, dt, precision);
-        } else if (graph_ && key == graph_key_) {
-            hcheck(hipGraphLaunch(graph_, st_), "forecast graph launch");
-        } else if (warm_ && key == warm_key_) {
+        } else if (CachedForecast *c = cached(key)) {
+            c->used = ++graph_clock_;
+            hcheck(hipGraphLaunch(c->exec, st_), "forecast graph launch");
+        } else if (seen(key)) {
             // second call with this key: every buffer exists, capture the sequence
-            reset_graph();
             hipGraph_t g = nullptr;
             hcheck(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
             try {
@@ -731,17 +741,21 @@ class Surrogate {
                 throw;
             }
             hcheck(hipStreamEndCapture(st_, &g), "capture end");
-            const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             hcheck(e, "forecast graph instantiate");
-            graph_key_ = key;
-            hcheck(hipGraphLaunch(graph_, st_), "forecast graph launch");
+            keep_graph(key, exec);
+            hcheck(hipGraphLaunch(exec, st_), "forecast graph launch");
         } else {
             launch_forecast(ev, steps, dt, precision);
-            warm_ = true;
-            warm_key_ = key;
+            seen_.push_back(key);
+            if (seen_.size() > kSeenKeys)
+                seen_.erase(seen_.begin());
         }
         hcheck(hipGetLastError(), "surrogate forecast");
+        hcheck(hipEventRecord(order_ev_, st_), "forecast order");
+        order_stream_ = st_;
         st_ = own_;
         eng.invalidate_mixed();
         eng.advance_counter(steps);
@@ -794,13 +808,56 @@ class Surrogate {
         }
     };
 
-    void reset_graph() {
-        if (graph_) {
-            (void)hipStreamSynchronize(st_);
-            (void)hipGraphExecDestroy(graph_);
+    struct CachedForecast {
+        GraphKey key;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+
+    // Captured forecasts: up to kForecastGraphs launch sequences (one per
+    // ensemble / horizon / precision key, least recently used evicted), so a
+    // filter that alternates between its particles and a judge ensemble
+    // replays both.  A key is captured on its second call (the first runs
+    // eagerly, so every lazy allocation happens outside capture).
+    CachedForecast *cached(const GraphKey &key) {
+        for (auto &c : graphs_)
+            if (c.key == key)
+                return &c;
+        return nullptr;
+    }
+    bool seen(const GraphKey &key) const {
+        for (const auto &k : seen_)
+            if (k == key)
+                return true;
+        return false;
+    }
+    void keep_graph(const GraphKey &key, hipGraphExec_t exec) {
+        if (graphs_.size() >= kForecastGraphs) {
+            size_t lru = 0;
+            for (size_t i = 1; i < graphs_.size(); ++i)
+                if (graphs_[i].used < graphs_[lru].used)
+                    lru = i;
+            // the evicted sequence may still run on its ensemble's stream
+            (void)hipDeviceSynchronize();
+            (void)hipGraphExecDestroy(graphs_[lru].exec);
+            graphs_.erase(graphs_.begin() + (long)lru);
         }
-        graph_ = nullptr;
-        warm_ = false;
+        graphs_.push_back(CachedForecast{key, exec, ++graph_clock_});
+        for (size_t i = 0; i < seen_.size(); ++i)
+            if (seen_[i] == key) {
+                seen_.erase(seen_.begin() + (long)i);
+                break;
+            }
+    }
+
+    void reset_graph() {
+        if (!graphs_.empty()) {
+            (void)hipDeviceSynchronize();
+            for (auto &c : graphs_)
+                (void)hipGraphExecDestroy(c.exec);
+        }
+        graphs_.clear();
+        seen_.clear();
     }
 
     static FnoShape make_shape(int state_n, int grid, int width, int layers, int modes) {
@@ -1347,9 +1404,13 @@ class Surrogate {
     Buf<double> hnu_, ds_nu_, bnu_, hamp_, ds_amp_, bamp_, lsum_, spread_;
     double error_rms_ = -1;
     Buf<uint64_t> counter_dev_;
-    bool use_graph_ = graphs_by_default(), warm_ = false;
-    GraphKey graph_key_{}, warm_key_{};
-    hipGraphExec_t graph_ = nullptr;
+    bool use_graph_ = graphs_by_default();
+    static constexpr size_t kForecastGraphs = 4, kSeenKeys = 8;
+    std::vector<CachedForecast> graphs_;
+    hipEvent_t order_ev_ = nullptr; // end of the last forecast, on order_stream_
+    hipStream_t order_stream_ = nullptr;
+    std::vector<GraphKey> seen_;
+    uint64_t graph_clock_ = 0;
     Buf<int> bidx_;
     FnoWork<float> wf_;
     FnoWork<__bf16> wb_;

@@ -250,6 +250,21 @@ int hzl_likelihood_adjoint(hzl_ensemble_t h, const double *initial_field, double
  * (src/cuda_solver.cu:172-218). */
 int hzl_ensemble_set_graph(hzl_ensemble_t h, int enabled);
 
+/* Step schedule of hzl_ensemble_step: chains = 1..4 (at most the member
+ * count).  With C chains the members are split into C contiguous ranges,
+ * each stepped as its own chain of (noise, 4 x (row, column)) launches on
+ * its own stream (the handle's stream and C - 1 more that it creates, forked
+ * from and joined to its own, so captured steps stay one graph): while one
+ * range's launch drains its last workgroups, another range's launch fills
+ * the idle compute units.  Members' arithmetic is unchanged, so results are
+ * bit-identical to one chain.  The forecasts of captured particle-filter
+ * and EnKF cycles use the chains too; the joint mixed-grid filter, SSPRK3,
+ * host forcing and the OU forcing process keep one chain.  hzl_ensemble_autotune (and the
+ * automatic tuning at creation) times 1, 2 and 4 chains with the chosen
+ * launch shapes and keeps the fastest.  No counterpart in the reference. */
+int hzl_ensemble_set_chains(hzl_ensemble_t h, int chains);
+int hzl_ensemble_get_chains(hzl_ensemble_t h, int *chains);
+
 /* Launch-shape autotuner.  FP32 ensembles at 128^2, 256^2 and 512^2 can run
  * their column and row kernels in several compiled shapes (values per lane,
  * i.e. the T x E split of each transform; workgroup size; register budget);
@@ -604,7 +619,11 @@ int hzl_surrogate_judge(hzl_surrogate_t h, hzl_ensemble_t ensemble, hzl_ensemble
  * (steps and dt must equal the trained horizon).  The ensemble's step counter
  * advances by `steps`; filter calls may follow with steps = 0.  The ensemble
  * runs the white forcing process (HZL_EINVAL for OU; also for
- * hzl_surrogate_forecast_or_solve). */
+ * hzl_surrogate_forecast_or_solve).  Forecasts of one surrogate share its
+ * scratch buffers: a forecast of another ensemble (another stream) waits on
+ * the device for the previous forecast to finish.  Up to four forecast launch
+ * sequences (per ensemble, horizon and precision) are captured and replayed
+ * when graphs are on. */
 int hzl_surrogate_forecast(hzl_surrogate_t h, hzl_ensemble_t ensemble, double dt, size_t steps,
                            int precision);
 

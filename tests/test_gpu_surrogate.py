@@ -315,6 +315,38 @@ def test_graph_replays_match_eager_forecasts(pkg, orc):
             np.testing.assert_array_equal(a.get_state(), b.get_state())
 
 
+def test_graph_cache_replays_alternating_ensembles(pkg, orc):
+    """One surrogate forecasting two ensembles in turn (a filter's particles
+    and a judge's scratch ensemble) keeps a captured forecast for each; every
+    call equals a graph-free surrogate's on twin ensembles bit for bit."""
+    cfg = SMALL
+    s = shape_of(cfg)
+    dt, steps = 2e-3, 25
+    fa, fc = grf_fields(orc, s.n, 16, 81), grf_fields(orc, s.n, 4, 82)
+    with pkg.Surrogate(**cfg, seed=9) as g, pkg.Surrogate(**cfg, seed=9) as e, \
+            pkg.Ensemble(s.n, 16, "fp32", sigma=0.3, seed=6) as a, \
+            pkg.Ensemble(s.n, 16, "fp32", sigma=0.3, seed=6) as b, \
+            pkg.Ensemble(s.n, 4, "fp32", sigma=0.0, seed=7) as c, \
+            pkg.Ensemble(s.n, 4, "fp32", sigma=0.0, seed=7) as d:
+        for x in (g, e):
+            x.set_scales(1.5, 4.0)
+        e.set_graph(False)
+        a.set_state(fa)
+        b.set_state(fa)
+        c.set_state(fc)
+        d.set_state(fc)
+        for k in range(6):
+            g.forecast(a, dt, steps, "bf16")
+            e.forecast(b, dt, steps, "bf16")
+            if k % 2 == 0:
+                g.forecast(c, dt, steps, "bf16")
+                e.forecast(d, dt, steps, "bf16")
+            for x in (a, b, c, d):
+                x.synchronize()
+            np.testing.assert_array_equal(a.get_state(), b.get_state())
+            np.testing.assert_array_equal(c.get_state(), d.get_state())
+
+
 def test_judge_measures_the_network_error_against_the_solver(pkg, orc):
     """hzl_surrogate_judge: the first J particles forecast by the network
     (no shell noise) and by the plain solver from the same states; the RMS
