@@ -1,0 +1,651 @@
+// fno_fused.hpp -- fused BF16 inference passes of the surrogate's spectral
+// layers (CDNA4 MFMA, one workgroup of 8 wavefronts per sample).
+//
+// A layer of the generic path is five batched products (gemm_mfma.hpp):
+// x-DFT, y-DFT, per-mode channel mixing, inverse y, inverse x fused with the
+// pointwise term.  Each small product round-trips its operands through HBM and
+// stages them tile by tile, so a layer moves ~400 MB at 256 samples and is
+// bound by latency, not bytes.  Here the two transform pairs become one pass
+// each, with every intermediate in LDS:
+//
+//   forward:  V[kx][k'][c] = sum_m Fy2[k'][m] sum_x Fx[2kx+ri][x] v[y][x][c]
+//             (m = ri G + y).  Blocks of YB rows: the rows are staged in LDS
+//             transposed (x fastest, the k order an MFMA operand needs), each
+//             wavefront transforms one row along x (32 x 32 tile, K = G), and
+//             the y transform accumulates over the blocks in registers (four
+//             32 x 32 tiles per wavefront, K = 2 YB per block).
+//   mixing:   the generic batched product over the modes (one launch).
+//   inverse:  out[y][x][c] = act(sum_k Gx[x][k] Bi[y][c][k] + sum_c' W[c][c']
+//             v[y][x][c'] + b[c]),  Bi[y][c][2kx+ro] = sum_k' Gy2[ro G + y][k']
+//             U[kx][k'][c], blocks of YBI rows, Bi in LDS; each wavefront
+//             stages its kx slices of U transposed (k' fastest) in LDS; the
+//             pointwise operand v is read straight from HBM in 16-byte
+//             fragments.
+//   head:     the projection C -> 128 -> 1 with both operands loaded straight
+//             into MFMA fragments.
+//
+// Intermediates are rounded to BF16 where the generic path stores them (the
+// x-transformed rows A, the spectral V and U, the inverse-y rows Bi), so both
+// paths compute the same quantities to FP32 summation order.  Shapes are
+// compile-time: the surrogate's 64^2 grid, 32 channels, 16 x 32 retained modes.
+//
+// MFMA v_mfma_f32_32x32x16_bf16 lane layout (gemm_mfma.hpp): lane l supplies
+// A[l % 32][8 (l / 32) + j] and B[l % 32][8 (l / 32) + j], j < 8; accumulator
+// register i holds D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
+#pragma once
+#include "gemm_mfma.hpp"
+
+#include <cmath>
+
+namespace hzl_amd {
+namespace fno {
+
+constexpr int G = 64, C = 32, KX = 16, KY = 32, CIN = 3;
+// band of the state sampled on the grid: |kx|, |ky| <= KS; kx padded to KXS
+constexpr int KS = G / 2 - 1, JS = 2 * KS + 1, KXS = 32;
+constexpr int THREADS = 512; // 8 wavefronts
+// forward pass: blocks of YB rows
+constexpr int YB = 8, NBLK = G / YB;
+constexpr int VT_LD = G + 8;                  // transposed-row pitch (bf16)
+constexpr int VT_ELEMS = YB * C * VT_LD;      // [y][c][x]
+constexpr int AB_ELEMS = 2 * KX * C * YB;     // [2kx+ri][c][y]
+constexpr int FWD_LDS = VT_ELEMS + AB_ELEMS;  // bf16 elements
+// inverse pass: blocks of YBI rows
+constexpr int YBI = 16, NBLKI = G / YBI;
+constexpr int BI_LD = 2 * KX + 8;             // [y][c][k] pitch (bf16)
+constexpr int BI_ELEMS = YBI * C * BI_LD;
+constexpr int UT_LD = 2 * KY + 8;             // a wavefront's U slice [c][k'] pitch
+constexpr int UT_ELEMS = C * UT_LD;
+constexpr int INV_LDS = BI_ELEMS + (THREADS / 64) * UT_ELEMS;
+constexpr long SAMPLE = (long)G * G * C;      // pixel activations per sample
+constexpr long SPECTRAL = (long)KX * 2 * KY * C;
+
+static_assert(G % YB == 0 && G % YBI == 0, "row blocks must tile the grid");
+static_assert(YB == THREADS / 64, "one row per wavefront in the forward x transform");
+static_assert(2 * KX == 32 && C == 32 && 2 * KY == 64, "tiles are 32 x 32");
+static_assert(2 * KX == 4 * (THREADS / 64), "four y-transform tiles per wavefront");
+static_assert(KX == 2 * (THREADS / 64) && YBI == 2 * (THREADS / 64),
+              "two kx and two rows per wavefront in the inverse pass");
+
+} // namespace fno
+
+struct FnoForwardArgs {
+    const __bf16 *v;   // [S][G][G][C] layer input
+    const __bf16 *Fx;  // [2KX][G]
+    const __bf16 *Fy2; // [2KY][2G]
+    __bf16 *V;         // [S][KX][2KY][C] = [s][kx][kyi][ri][c]
+    long S;
+};
+
+struct FnoInverseArgs {
+    const __bf16 *U;   // [S][KX][2KY][C] = [s][kx][kyi][ro][c] (the mixing product's output)
+    const __bf16 *Gy2; // [2G][2KY]
+    const __bf16 *Gx;  // [G][2KX]
+    const __bf16 *v;   // [S][G][G][C] layer input (pointwise term)
+    const __bf16 *W;   // [C][C], W[c_out][c_in]
+    const float *bias; // [C]
+    __bf16 *out;       // [S][G][G][C]
+    long S;
+    int act;           // ACT_GELU or ACT_NONE
+};
+
+struct FnoLiftArgs {
+    const __bf16 *feat; // [S G G][CIN]
+    const __bf16 *Lw;   // [C][CIN]
+    const float *Lb;    // [C]
+    __bf16 *v;          // [S G G][C]
+    long points;
+};
+
+struct FnoHeadArgs {
+    const __bf16 *X;   // [M][C] last layer's activations
+    const __bf16 *Q1;  // [H][C]
+    const float *q1;   // [H]
+    const float *Q2;   // [H]
+    const float *q2;   // [1]
+    float *out;        // [M]
+    long M;
+};
+constexpr int FNO_H = 128; // hidden channels of the projection
+
+HZL_HD bf16x8_t load8(const __bf16 *p) {
+    return *reinterpret_cast<const bf16x8_t *>(p);
+}
+
+// GELU with erf by Abramowitz & Stegun 7.1.26 (|erf error| <= 1.5e-7, one
+// exponential and one reciprocal): the inference passes' activation, far
+// inside the BF16 rounding of their outputs.  Training keeps gelu() (erff).
+HZL_HD float erf_as(float x) {
+    const float ax = x < 0.f ? -x : x;
+#if __HIP_DEVICE_COMPILE__
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+    const float e = __expf(-ax * ax);
+#else
+    const float t = 1.0f / (1.0f + 0.3275911f * ax);
+    const float e = std::exp(-ax * ax);
+#endif
+    const float poly =
+        t * (0.254829592f +
+             t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float y = 1.0f - poly * e;
+    return x < 0.f ? -y : y;
+}
+HZL_HD float gelu_as(float x) {
+    return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f));
+}
+
+template <class Ctx>
+HZL_HD void fno_mfma(const Ctx &ctx, const bf16x8_t &fa, const bf16x8_t &fb, f32x16_t &acc,
+                     int lane) {
+#if __HIP_DEVICE_COMPILE__
+    (void)ctx;
+    (void)lane;
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc, 0, 0, 0);
+#else
+    ctx.mfma_bf16(fa, fb, acc, lane);
+#endif
+}
+
+// Forward x and y transforms of sample s (workgroup of fno::THREADS lanes,
+// lds: fno::FWD_LDS bf16 elements).
+template <class Ctx>
+HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *lds, long s) {
+    using namespace fno;
+    if (s >= a.S)
+        return; // whole workgroup
+    const int tid = ctx.tid(), wave = tid / 64, lane = tid % 64, r = lane % 32, h = lane / 32;
+    __bf16 *vT = lds, *Ab = lds + VT_ELEMS;
+    const __bf16 *vs = a.v + s * SAMPLE;
+    // x-transform operand A: Fx[r][16 kb + 8 h + j]
+    bf16x8_t fx[G / 16];
+#pragma unroll
+    for (int kb = 0; kb < G / 16; ++kb)
+        fx[kb] = load8(a.Fx + r * G + 16 * kb + 8 * h);
+    // y-transform accumulators: tile q is (kx = 2 wave + q / 2, k' block 32 (q % 2))
+    f32x16_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            acc[q][i] = 0.f;
+    // a block is YB rows of G x C values: CH 16-byte chunks per lane
+    constexpr int PER_ROW = G * C / 8, CH = YB * PER_ROW / THREADS;
+    static_assert(YB * PER_ROW % THREADS == 0, "block chunks must divide over the lanes");
+    bf16x8_t pre[CH];
+    auto load_block = [&](int b) {
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int idx = tid + THREADS * q, y = idx / PER_ROW, rem = idx % PER_ROW;
+            const int x = rem / (C / 8), c0 = (rem % (C / 8)) * 8;
+            pre[q] = load8(vs + ((long)(b * YB + y) * G + x) * C + c0);
+        }
+    };
+    load_block(0);
+    for (int b = 0; b < NBLK; ++b) {
+        // (1) the block's rows, transposed: vT[y][c][x]
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int idx = tid + THREADS * q, y = idx / PER_ROW, rem = idx % PER_ROW;
+            const int x = rem / (C / 8), c0 = (rem % (C / 8)) * 8;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                vT[(y * C + c0 + j) * VT_LD + x] = pre[q][j];
+        }
+        ctx.sync();
+        if (b + 1 < NBLK)
+            load_block(b + 1); // in flight during this block's products
+        // (2) x transform of row y = wave: D[kx'][c] = sum_x Fx[kx'][x] v[y][x][c]
+        {
+            f32x16_t d;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                d[i] = 0.f;
+#pragma unroll
+            for (int kb = 0; kb < G / 16; ++kb) {
+                const bf16x8_t fb = load8(vT + (wave * C + r) * VT_LD + 16 * kb + 8 * h);
+                fno_mfma(ctx, fx[kb], fb, d, lane);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int kxp = 8 * (i / 4) + 4 * h + i % 4;
+                Ab[(kxp * C + r) * YB + wave] = (__bf16)d[i];
+            }
+        }
+        ctx.sync();
+        // (3) y transform, accumulated: k = 8 ri + y over the block (ri = h)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kx = 2 * wave + q / 2, t = q % 2;
+            const bf16x8_t fa = load8(a.Fy2 + (32 * t + r) * (2 * G) + h * G + b * YB);
+            const bf16x8_t fb = load8(Ab + ((2 * kx + h) * C + r) * YB);
+            fno_mfma(ctx, fa, fb, acc[q], lane);
+        }
+    }
+    __bf16 *Vs = a.V + s * SPECTRAL;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int kx = 2 * wave + q / 2, t = q % 2;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int kp = 32 * t + 8 * (i / 4) + 4 * h + i % 4;
+            Vs[((long)kx * 2 * KY + kp) * C + r] = (__bf16)acc[q][i];
+        }
+    }
+}
+
+// Inverse y and x transforms, pointwise term, bias and activation of row
+// block `block % NBLKI` of sample `block / NBLKI` (workgroup of fno::THREADS
+// lanes, lds: fno::INV_LDS bf16 elements).  Row blocks are independent, so
+// a sample spreads over NBLKI workgroups.
+template <class Ctx>
+HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *lds, long block) {
+    using namespace fno;
+    const long s = block / NBLKI;
+    const int b = (int)(block % NBLKI);
+    if (s >= a.S)
+        return; // whole workgroup
+    const int tid = ctx.tid(), wave = tid / 64, lane = tid % 64, r = lane % 32, h = lane / 32;
+    __bf16 *Bi = lds;
+    // inverse-x operand B: Gx[32 xt + r][16 kb + 8 h + j]; pointwise operand A:
+    // W[r][16 kb + 8 h + j]; bias of the accumulator rows c = 8 (i / 4) + 4 h + i % 4
+    bf16x8_t gx[2][2], wf[2];
+#pragma unroll
+    for (int xt = 0; xt < 2; ++xt)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+            gx[xt][kb] = load8(a.Gx + (32 * xt + r) * (2 * KX) + 16 * kb + 8 * h);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+        wf[kb] = load8(a.W + r * C + 16 * kb + 8 * h);
+    float bias[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        bias[i] = a.bias[8 * (i / 4) + 4 * h + i % 4];
+    const __bf16 *Us = a.U + s * SPECTRAL;
+    __bf16 *UT = lds + BI_ELEMS + wave * UT_ELEMS; // this wavefront's U^T slice [c][k']
+    {
+        const int y0 = b * YBI;
+        // (1) inverse y of kx = 2 wave + q: D[m][c], m = 16 ro + yl
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int kx = 2 * wave + q;
+            // U[kx] is [k'][c] (c fastest): the operand needs 8 consecutive k' per
+            // lane, so the 4 KB slice goes through LDS transposed
+            const __bf16 *Uk = Us + (long)kx * 2 * KY * C;
+#pragma unroll
+            for (int j = 0; j < 2 * KY * C / 8 / 64; ++j) {
+                const int e = lane + 64 * j, kp = e / (C / 8), c0 = (e % (C / 8)) * 8;
+                const bf16x8_t u8 = load8(Uk + kp * C + c0);
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj)
+                    UT[(c0 + jj) * UT_LD + kp] = u8[jj];
+            }
+            ctx.wsync();
+            f32x16_t d;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                d[i] = 0.f;
+            const int ro = r / 16, yl = r % 16;
+#pragma unroll
+            for (int kb = 0; kb < (2 * KY) / 16; ++kb) {
+                const bf16x8_t fa = load8(a.Gy2 + (ro * G + y0 + yl) * (2 * KY) + 16 * kb + 8 * h);
+                const bf16x8_t fb = load8(UT + r * UT_LD + 16 * kb + 8 * h);
+                fno_mfma(ctx, fa, fb, d, lane);
+            }
+            ctx.wsync(); // the slice is rewritten for the next kx
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int m = 8 * (i / 4) + 4 * h + i % 4;
+                Bi[((m % 16) * C + r) * BI_LD + 2 * kx + m / 16] = (__bf16)d[i];
+            }
+        }
+        ctx.sync();
+        // (2) rows y0 + 2 wave + p: D[c][x] = sum_k Bi[y][c][k] Gx[x][k] + sum_c' W[c][c'] v[y][x][c']
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int yl = 2 * wave + p;
+            const long row = (s * G + y0 + yl) * (long)G;
+            const __bf16 *vrow = a.v + row * C;
+#pragma unroll
+            for (int xt = 0; xt < 2; ++xt) {
+                f32x16_t d;
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                    d[i] = 0.f;
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb) {
+                    const bf16x8_t fa = load8(Bi + (yl * C + r) * BI_LD + 16 * kb + 8 * h);
+                    fno_mfma(ctx, fa, gx[xt][kb], d, lane);
+                }
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb) {
+                    const bf16x8_t fb = load8(vrow + (32 * xt + r) * C + 16 * kb + 8 * h);
+                    fno_mfma(ctx, wf[kb], fb, d, lane);
+                }
+                __bf16 *o = a.out + (row + 32 * xt + r) * C;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    struct alignas(8) Q {
+                        __bf16 v[4];
+                    } q4;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        float x = d[4 * g + u] + bias[4 * g + u];
+                        if (a.act == ACT_GELU)
+                            x = gelu_as(x);
+                        q4.v[u] = (__bf16)x;
+                    }
+                    *reinterpret_cast<Q *>(o + 8 * g + 4 * h) = q4;
+                }
+            }
+        }
+    }
+}
+
+// Projection and head for the 32 points of wavefront `wave`:
+//   out[m] = q2 + sum_j Q2[j] gelu(q1[j] + sum_c X[m][c] Q1[j][c])
+// Both operands load straight into MFMA fragments (X rows and Q1 rows have C
+// contiguous channels); per-row partial sums are reduced across the 32 lanes
+// of each half-wave.
+template <class Ctx>
+HZL_HD void fno_head_wave(const Ctx &ctx, const FnoHeadArgs &a, long wave, int lane) {
+    using namespace fno;
+    const long m0 = wave * 32;
+    if (m0 >= a.M)
+        return; // whole wavefront
+    const int r = lane % 32, h = lane / 32;
+    bf16x8_t fa[C / 16];
+#pragma unroll
+    for (int kb = 0; kb < C / 16; ++kb) {
+        if (m0 + r < a.M) {
+            fa[kb] = load8(a.X + (m0 + r) * C + 16 * kb + 8 * h);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                fa[kb][j] = (__bf16)0.f;
+        }
+    }
+    float rs[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        rs[i] = 0.f;
+#pragma unroll
+    for (int nt = 0; nt < FNO_H / 32; ++nt) {
+        f32x16_t d;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            d[i] = 0.f;
+#pragma unroll
+        for (int kb = 0; kb < C / 16; ++kb)
+            fno_mfma(ctx, fa[kb], load8(a.Q1 + (32 * nt + r) * C + 16 * kb + 8 * h), d, lane);
+        const float bias = a.q1[32 * nt + r], w = a.Q2[32 * nt + r];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            rs[i] += w * gelu_as(d[i] + bias);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        for (int off = 1; off < 32; off *= 2)
+            rs[i] += ctx.shfl_xor(rs[i], off, lane);
+    if (r == 0) {
+        const float q2 = a.q2[0];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const long m = m0 + 8 * (i / 4) + 4 * h + i % 4;
+            if (m < a.M)
+                a.out[m] = rs[i] + q2;
+        }
+    }
+}
+
+// Band-limited state <-> grid transforms of one sample (FP32 on the matrix
+// cores, v_mfma_f32_32x32x2_f32, one 32 x 32 tile per wavefront; operands
+// from LDS):
+//   to grid:   Bm[2y+ro][kx] = sum_k g1[2y+ro][k] st[kx][k]   (k = 2 jj + ri < 2 JS)
+//              out[y][x] = alpha sum_k Bm[2y + k / KXS][k % KXS] g2[x][k]  (k < 2 KXS)
+//   from grid: dg[2kx+ri][y] = sum_x f1[2kx+ri][x] grid[y][x]
+//              incr[kx][n] = sum_k dg[2kx + k / G][k % G] f2[n][k]   (k < 2 G, n < 2 JS)
+// with the matrices of the surrogate's state <-> grid conversion (g1 [2G][2JS],
+// g2 [G][2KXS], f1 [2KXS][G], f2 [2JS][2G]); st is the sample's retained half
+// plane [kx][ky + K] (K, J = 2K + 1 of the state grid), read for kx <= KS,
+// |ky| <= KS.  MFMA f32 lane layout: lane l supplies A[l % 32][l / 32] and
+// B[l % 32][l / 32]; accumulator i holds D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
+constexpr int SG_THREADS = 256; // 4 wavefronts
+constexpr int SG_LD = 2 * fno::JS + 3;  // 129: staged band rows
+constexpr int BM_LD = fno::KXS + 1;     // 33
+constexpr int GR_LD = fno::G + 1;       // 65
+constexpr int KC = 32;                  // k chunk of a staged constant matrix
+constexpr int KC_LD = KC + 1;
+// staged constants: a KC-column chunk of g1 (2G rows) or f2 (2JS rows), or
+// all of g2 / f1 (pitch 2KXS + 1 = G + 1)
+constexpr int CK_ELEMS = 2 * fno::G * (2 * fno::KXS + 1);
+constexpr int SG_LDS = fno::KXS * SG_LD + 2 * fno::G * BM_LD + CK_ELEMS;  // floats
+constexpr int GS_LDS = fno::G * GR_LD + 2 * fno::KXS * GR_LD + CK_ELEMS;  // floats
+static_assert(2 * fno::JS <= 2 * fno::G && fno::G == 2 * fno::KXS, "chunk area sizes");
+
+struct StateGridArgs {
+    const float *states; // [S][2 spec] retained half planes (re, im)
+    long spec2;          // floats per state
+    int K, J;
+    const float *g1, *g2;
+    void *out;           // out[s cb1 + y cm + x cn], BF16 or FP32
+    int out_bf16;
+    long cm, cn, cb1;
+    float alpha;
+    long S;
+};
+
+struct GridIncrementArgs {
+    const float *grid;   // [S][G][G]
+    const float *f1, *f2;
+    float *incr;         // [S][KS + 1][2 JS]
+    long S;
+};
+
+template <class Ctx>
+HZL_HD void fno_mfma_f32(const Ctx &ctx, float fa, float fb, f32x16_t &acc, int lane) {
+#if __HIP_DEVICE_COMPILE__
+    (void)ctx;
+    (void)lane;
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(fa, fb, acc, 0, 0, 0);
+#else
+    ctx.mfma_f32(fa, fb, acc, lane);
+#endif
+}
+
+// Stage rows [0, rows) x columns [k0, k0 + KC) of a row-major matrix (row
+// length ld, columns beyond `cols` as zero) into dst[row][KC_LD].
+// Staging loops run a fixed, unrolled trip count (MAXE / SG_THREADS), so every
+// lane issues all its global loads before it waits for the first one.
+template <int MAXE, class Ctx, class F>
+HZL_HD void stage_unrolled(const Ctx &ctx, int count, F &&move) {
+    constexpr int TRIPS = (MAXE + SG_THREADS - 1) / SG_THREADS;
+    float v[TRIPS];
+#pragma unroll
+    for (int i = 0; i < TRIPS; ++i) {
+        const int e = ctx.tid() + SG_THREADS * i;
+        v[i] = e < count ? move(e, 0.f, false) : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < TRIPS; ++i) {
+        const int e = ctx.tid() + SG_THREADS * i;
+        if (e < count)
+            move(e, v[i], true);
+    }
+}
+
+template <class Ctx>
+HZL_HD void stage_chunk(const Ctx &ctx, const float *m, int rows, int ld, int cols, int k0,
+                        float *dst) {
+    stage_unrolled<2 * fno::G * KC>(ctx, rows * KC, [&](int e, float v, bool store) {
+        const int r = e / KC, k = k0 + e % KC;
+        if (store) {
+            dst[r * KC_LD + e % KC] = v;
+            return 0.f;
+        }
+        return k < cols ? m[(long)r * ld + k] : 0.f;
+    });
+}
+
+HZL_HD void zero16(f32x16_t &d) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        d[i] = 0.f;
+}
+
+template <class Ctx>
+HZL_HD void fno_state_to_grid_body(const Ctx &ctx, const StateGridArgs &a, float *lds, long s) {
+    using namespace fno;
+    if (s >= a.S)
+        return; // whole workgroup
+    const int t = ctx.tid(), w = t / 64, lane = t % 64, r = lane % 32, h = lane / 32;
+    float *stB = lds, *Bm = lds + KXS * SG_LD, *ck = Bm + 2 * G * BM_LD;
+    const float *st = a.states + s * a.spec2 + 2L * (a.K - KS);
+    // whole rows, pad columns zeroed: the last g1 chunk's zero columns meet them
+    stage_unrolled<KXS * SG_LD>(ctx, KXS * SG_LD, [&](int e, float v, bool store) {
+        const int kx = e / SG_LD, k = e % SG_LD;
+        if (store) {
+            stB[e] = v;
+            return 0.f;
+        }
+        return kx <= KS && k < 2 * JS ? st[2L * kx * a.J + k] : 0.f;
+    });
+    {
+        // Bm rows 32 w .. 32 w + 31 (2y+ro), columns kx; g1 staged KC columns at a time
+        f32x16_t d;
+        zero16(d);
+        for (int kc = 0; kc < 2 * JS; kc += KC) {
+            ctx.sync(); // previous chunk consumed (and, first, the band staged)
+            stage_chunk(ctx, a.g1, 2 * G, 2 * JS, 2 * JS, kc, ck);
+            ctx.sync();
+            // the chunk's zero-filled columns past 2 JS add nothing
+            for (int k = 0; k < KC; k += 2)
+                fno_mfma_f32(ctx, ck[(32 * w + r) * KC_LD + k + h], stB[r * SG_LD + kc + k + h], d,
+                             lane);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            Bm[(32 * w + 8 * (i / 4) + 4 * h + i % 4) * BM_LD + r] = d[i];
+    }
+    ctx.sync();
+    // g2 [G][2KXS] staged whole (pitch 2KXS + 1)
+    stage_unrolled<G * 2 * KXS>(ctx, G * 2 * KXS, [&](int e, float v, bool store) {
+        if (store) {
+            ck[(e / (2 * KXS)) * (2 * KXS + 1) + e % (2 * KXS)] = v;
+            return 0.f;
+        }
+        return a.g2[e];
+    });
+    ctx.sync();
+    {
+        // out tile (rows y: 32 (w / 2) .., columns x: 32 (w % 2) ..)
+        const int yt = w / 2, xt = w % 2;
+        f32x16_t d;
+        zero16(d);
+        for (int k = 0; k < 2 * KXS; k += 2) {
+            const int kk = k + h;
+            fno_mfma_f32(ctx, Bm[(2 * (32 * yt + r) + kk / KXS) * BM_LD + kk % KXS],
+                         ck[(32 * xt + r) * (2 * KXS + 1) + kk], d, lane);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int y = 32 * yt + 8 * (i / 4) + 4 * h + i % 4, x = 32 * xt + r;
+            const long o = s * a.cb1 + (long)y * a.cm + (long)x * a.cn;
+            const float v = a.alpha * d[i];
+            if (a.out_bf16)
+                reinterpret_cast<__bf16 *>(a.out)[o] = (__bf16)v;
+            else
+                reinterpret_cast<float *>(a.out)[o] = v;
+        }
+    }
+}
+
+template <class Ctx>
+HZL_HD void fno_grid_to_increment_body(const Ctx &ctx, const GridIncrementArgs &a, float *lds,
+                                       long s) {
+    using namespace fno;
+    if (s >= a.S)
+        return; // whole workgroup
+    const int t = ctx.tid(), w = t / 64, lane = t % 64, r = lane % 32, h = lane / 32;
+    float *gr = lds, *dg = lds + G * GR_LD, *ck = dg + 2 * KXS * GR_LD;
+    const float *src = a.grid + s * (long)G * G;
+    stage_unrolled<G * G>(ctx, G * G, [&](int e, float v, bool store) {
+        if (store) {
+            gr[(e / G) * GR_LD + e % G] = v;
+            return 0.f;
+        }
+        return src[e];
+    });
+    // f1 [2KXS][G] staged whole (pitch G + 1)
+    stage_unrolled<2 * KXS * G>(ctx, 2 * KXS * G, [&](int e, float v, bool store) {
+        if (store) {
+            ck[(e / G) * (G + 1) + e % G] = v;
+            return 0.f;
+        }
+        return a.f1[e];
+    });
+    ctx.sync();
+    {
+        // dg tile (rows 2kx+ri: 32 (w / 2) .., columns y: 32 (w % 2) ..)
+        const int rt = w / 2, yt = w % 2;
+        f32x16_t d;
+        zero16(d);
+        for (int x = 0; x < G; x += 2)
+            fno_mfma_f32(ctx, ck[(32 * rt + r) * (G + 1) + x + h], gr[(32 * yt + r) * GR_LD + x + h],
+                         d, lane);
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            dg[(32 * rt + 8 * (i / 4) + 4 * h + i % 4) * GR_LD + 32 * yt + r] = d[i];
+    }
+    {
+        // incr tile (rows kx, columns n: 32 w ..); f2 staged KC columns at a time
+        f32x16_t d;
+        zero16(d);
+        const int n = 32 * w + r;
+        for (int kc = 0; kc < 2 * G; kc += KC) {
+            ctx.sync(); // dg written / previous chunk consumed
+            stage_chunk(ctx, a.f2, 2 * JS, 2 * G, 2 * G, kc, ck);
+            ctx.sync();
+            for (int k = 0; k < KC; k += 2) {
+                const int kk = kc + k + h;
+                const float fb = n < 2 * JS ? ck[n * KC_LD + k + h] : 0.f;
+                fno_mfma_f32(ctx, dg[(2 * r + kk / G) * GR_LD + kk % G], fb, d, lane);
+            }
+        }
+        float *o = a.incr + s * (long)(KS + 1) * (2 * JS);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int kx = 8 * (i / 4) + 4 * h + i % 4;
+            if (n < 2 * JS)
+                o[kx * (2 * JS) + n] = d[i];
+        }
+    }
+}
+
+// Lift of the input features to the first layer's channels: thread i writes
+// channels 8 (i % (C / 8)) .. + 8 of point i / (C / 8).
+HZL_HD void fno_lift_body(const FnoLiftArgs &a, long i) {
+    using namespace fno;
+    const long p = i / (C / 8);
+    const int c0 = (int)(i % (C / 8)) * 8;
+    if (p >= a.points)
+        return;
+    float f[CIN];
+#pragma unroll
+    for (int k = 0; k < CIN; ++k)
+        f[k] = (float)a.feat[p * CIN + k];
+    struct alignas(16) O {
+        __bf16 v[8];
+    } o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float x = 0.f;
+#pragma unroll
+        for (int k = 0; k < CIN; ++k)
+            x += f[k] * (float)a.Lw[(c0 + j) * CIN + k];
+        o.v[j] = (__bf16)(x + a.Lb[c0 + j]);
+    }
+    *reinterpret_cast<O *>(a.v + p * C + c0) = o;
+}
+
+} // namespace hzl_amd

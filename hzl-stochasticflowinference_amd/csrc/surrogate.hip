@@ -24,6 +24,7 @@
 #include "engine.hpp"
 #include "gemm_launch.hpp"
 #include "surrogate_kernels.hpp"
+#include "fno_fused.hpp"
 #include "tables.hpp"
 
 #include <algorithm>
@@ -54,6 +55,31 @@ __global__ void k_feature(T *f, const double *nu, const double *amp, int S, int 
                           int kf) {
     feature_body<T>(f, nu, amp, S, G, CIN, kf, HZL_GID);
 }
+// Fused BF16 inference passes of a spectral layer (fno_fused.hpp): one
+// workgroup per sample.
+__global__ void __launch_bounds__(fno::THREADS) k_fno_forward(FnoForwardArgs a) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds[fno::FWD_LDS];
+    fno_forward_body(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+__global__ void __launch_bounds__(fno::THREADS) k_fno_inverse(FnoInverseArgs a) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds[fno::INV_LDS];
+    fno_inverse_body(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+__global__ void k_fno_lift(FnoLiftArgs a) {
+    fno_lift_body(a, HZL_GID);
+}
+__global__ void __launch_bounds__(SG_THREADS) k_state_to_grid(StateGridArgs a) {
+    __shared__ float lds[SG_LDS];
+    fno_state_to_grid_body(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+__global__ void __launch_bounds__(SG_THREADS) k_grid_to_increment(GridIncrementArgs a) {
+    __shared__ float lds[GS_LDS];
+    fno_grid_to_increment_body(DevCtx{}, a, lds, (long)blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_fno_head(FnoHeadArgs a) {
+    fno_head_wave(DevCtx{}, a, (long)blockIdx.x * 4 + threadIdx.x / 64, (int)(threadIdx.x % 64));
+}
+
 template <typename T> __global__ void k_gelu_back(const T *go, const T *pre, T *g, long n) {
     gelu_back_body<T>(go, pre, g, n, HZL_GID);
 }
@@ -766,6 +792,11 @@ class Surrogate {
         reset_graph();
     }
 
+    void set_fused(bool on) {
+        fused_ = on;
+        reset_graph(); // captured forecasts name the previous path's kernels
+    }
+
   private:
     // The launch sequence of one forecast (no allocation or host sync once
     // the buffers exist, so it can be captured).
@@ -776,7 +807,10 @@ class Surrogate {
             cx<float> *w0 = reinterpret_cast<cx<float> *>(ev.w0) + (long)s0 * s_.spec();
             if (precision == HZL_BF16) {
                 forward(wb_, w0, ev.nu + s0, ev.amp + s0, S, false);
-                grid_to_increment(wb_.out.p, S);
+                if (fused_ && fused_shape())
+                    grid_to_increment_fused(wb_.out.p, S);
+                else
+                    grid_to_increment(wb_.out.p, S);
             } else {
                 forward(wf_, w0, ev.nu + s0, ev.amp + s0, S, false);
                 grid_to_increment(wf_.out.p, S);
@@ -996,6 +1030,21 @@ class Surrogate {
         b.run(st_);
     }
 
+    // state_to_grid / grid_to_increment of the fused inference path: one
+    // workgroup per sample (fno_fused.hpp), same matrices and output layouts.
+    void state_to_grid_fused(const cx<float> *states, int S, __bf16 *out, long cm, long cn,
+                             long cb1, float alpha) {
+        StateGridArgs a{reinterpret_cast<const float *>(states), 2 * s_.spec(), s_.K,
+                        2 * s_.K + 1, g1_.p, g2_.p, out, 1, cm, cn, cb1, alpha, (long)S};
+        hipLaunchKernelGGL(k_state_to_grid, dim3((unsigned)S), dim3(SG_THREADS), 0, st_, a);
+    }
+    void grid_to_increment_fused(const float *grid, int S) {
+        const int KS = s_.KS, JS = 2 * KS + 1;
+        incr_.alloc((size_t)S * (KS + 1) * 2 * JS);
+        GridIncrementArgs a{grid, f1_.p, f2_.p, incr_.p, (long)S};
+        hipLaunchKernelGGL(k_grid_to_increment, dim3((unsigned)S), dim3(SG_THREADS), 0, st_, a);
+    }
+
     // Grid increment (S, G, G) -> band modes into incr_ [s][kx][ky+KS][re, im].
     void grid_to_increment(const float *grid, int S) {
         const int G = s_.G, KS = s_.KS, JS = 2 * KS + 1, KXS = FnoShape::KXS;
@@ -1040,6 +1089,18 @@ class Surrogate {
         const int G = s.G, C = s.C, KX = s.KX, KY = s.KY, TY = type_of<T>();
         const long P = s.P(), SP = (long)S * P, SA = s.SA(), SV = s.SV();
         const long rper = 4L * C * C * s.modes();
+        if constexpr (std::is_same_v<T, __bf16>) {
+            if (!train && fused_ && fused_shape()) {
+                if (w.feat.n < (size_t)SP * s.CIN)
+                    throw std::logic_error("fused surrogate features out of bounds");
+                state_to_grid_fused(states, S, w.feat.p, (long)G * s.CIN, s.CIN, P * s.CIN,
+                                    (float)in_scale_);
+                hipLaunchKernelGGL(k_feature<T>, grid1d(SP), dim3(256), 0, st_, w.feat.p, nu, amp,
+                                   S, G, s.CIN, s.kf);
+                layers_fused(w, S);
+                return;
+            }
+        }
         state_to_grid(states, S, vs(w.feat), (long)G * s.CIN, s.CIN, P * s.CIN, TY,
                       (float)in_scale_);
         hipLaunchKernelGGL(k_feature<T>, grid1d(SP), dim3(256), 0, st_, w.feat.p, nu, amp, S, G,
@@ -1096,17 +1157,78 @@ class Surrogate {
             hipLaunchKernelGGL(k_head<T>, grid1d(SP), dim3(256), 0, st_, w.h.p,
                                master_.p + prm_.Q2, master_.p + prm_.q2, w.out.p, SP, s.H);
         } else {
-            if (w.v[s.L]->n < (size_t)SP * C || w.w_T.n < (size_t)(prm_.Q1 + s.H * C) ||
-                w.out.n < (size_t)SP || s.H % 32 != 0 || C > 64)
-                throw std::logic_error("projection operands out of bounds");
-            HeadArgs<T> ha{w.v[s.L]->p,         w.w_T.p + prm_.Q1, master_.p + prm_.q1,
-                           master_.p + prm_.Q2, master_.p + prm_.q2, w.out.p,
-                           SP,                  C,                 s.H};
-            const long waves = (SP + 31) / 32;
-            hipLaunchKernelGGL(k_proj_head<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256),
-                               4 * GemmTile<T>::LDS_PER_WAVE * sizeof(T), st_, ha);
+            head_inference(w, S);
         }
     }
+
+    // Projection and head of an inference pass (k_proj_head) from the last
+    // layer's activations into w.out.
+    template <typename T> void head_inference(FnoWork<T> &w, int S) {
+        const FnoShape &s = s_;
+        const long SP = (long)S * s.P();
+        const int C = s.C;
+        if (w.v[s.L]->n < (size_t)SP * C || w.w_T.n < (size_t)(prm_.Q1 + s.H * C) ||
+            w.out.n < (size_t)SP || s.H % 32 != 0 || C > 64)
+            throw std::logic_error("projection operands out of bounds");
+        HeadArgs<T> ha{w.v[s.L]->p,         w.w_T.p + prm_.Q1, master_.p + prm_.q1,
+                       master_.p + prm_.Q2, master_.p + prm_.q2, w.out.p,
+                       SP,                  C,                 s.H};
+        const long waves = (SP + 31) / 32;
+        hipLaunchKernelGGL(k_proj_head<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256),
+                           4 * GemmTile<T>::LDS_PER_WAVE * sizeof(T), st_, ha);
+    }
+
+    // The network's shape is the one the fused layer passes are compiled for.
+    bool fused_shape() const {
+        return s_.G == fno::G && s_.C == fno::C && s_.KX == fno::KX && s_.KY == fno::KY &&
+               s_.CIN == fno::CIN && s_.H == FNO_H && s_.KS == fno::KS &&
+               FnoShape::KXS == fno::KXS;
+    }
+
+    // Lift, spectral layers and head of a BF16 inference pass through the
+    // fused passes (fno_fused.hpp): per layer one forward-transform pass, the
+    // per-mode mixing product and one inverse-transform pass with the
+    // pointwise term, bias and activation; then the projection and head.
+    // w.feat holds the input features; the output lands in w.out.
+    void layers_fused(FnoWork<__bf16> &w, int S) {
+        using namespace fno;
+        const FnoShape &s = s_;
+        const long SP = (long)S * s.P(), SV = s.SV(), rper = 4L * C * C * s.modes();
+        if (w.feat.n < (size_t)SP * CIN || w.v[0]->n < (size_t)SP * C ||
+            w.w_T.n < (size_t)prm_.total || w.V[0]->n < (size_t)S * SV ||
+            w.U[0]->n < (size_t)S * SV)
+            throw std::logic_error("fused surrogate operands out of bounds");
+        FnoLiftArgs la{w.feat.p, w.w_T.p + prm_.Lw, master_.p + prm_.Lb, w.v[0]->p, SP};
+        hipLaunchKernelGGL(k_fno_lift, grid1d(SP * (C / 8)), dim3(256), 0, st_, la);
+        for (int l = 0; l < s.L; ++l) {
+            FnoForwardArgs fa{w.v[l]->p, w.cst.Fx.p, w.cst.Fy2.p, w.V[l]->p, (long)S};
+            hipLaunchKernelGGL(k_fno_forward, dim3((unsigned)S), dim3(THREADS), 0, st_, fa);
+            {
+                Gemm<__bf16> r(S, 2 * C, 2 * C, cs(*w.V[l]), SV, 1, cs(w.rcat, (size_t)(l * rper)),
+                               2 * C, 1, vs(*w.U[l]), SV, 1, OUT_BF16);
+                r.batch(s.modes(), 2 * C, 4L * C * C, 2 * C);
+                r.run(st_);
+            }
+            FnoInverseArgs ia{w.U[l]->p,
+                              w.cst.Gy2.p,
+                              w.cst.Gx.p,
+                              w.v[l]->p,
+                              w.w_T.p + prm_.W[l],
+                              master_.p + prm_.b[l],
+                              w.v[l + 1]->p,
+                              (long)S,
+                              l + 1 < s.L ? ACT_GELU : ACT_NONE};
+            hipLaunchKernelGGL(k_fno_inverse, dim3((unsigned)(S * NBLKI)), dim3(THREADS), 0, st_,
+                               ia);
+        }
+        if (s.H != FNO_H || w.v[s.L]->n < (size_t)SP * C || w.out.n < (size_t)SP)
+            throw std::logic_error("fused surrogate head operands out of bounds");
+        FnoHeadArgs ha{w.v[s.L]->p,         w.w_T.p + prm_.Q1, master_.p + prm_.q1,
+                       master_.p + prm_.Q2, master_.p + prm_.q2, w.out.p, SP};
+        const long waves = (SP + 31) / 32;
+        hipLaunchKernelGGL(k_fno_head, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st_, ha);
+    }
+
 
     // Mean squared error against the scaled true grid increment; with
     // want_grad, every parameter gradient (FP32) into grad_.
@@ -1405,6 +1527,7 @@ class Surrogate {
     double error_rms_ = -1;
     Buf<uint64_t> counter_dev_;
     bool use_graph_ = graphs_by_default();
+    bool fused_ = true; // BF16 inference through the fused layer passes
     static constexpr size_t kForecastGraphs = 4, kSeenKeys = 8;
     std::vector<CachedForecast> graphs_;
     hipEvent_t order_ev_ = nullptr; // end of the last forecast, on order_stream_
@@ -1598,6 +1721,10 @@ int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, 
         if (used_solver)
             *used_solver = r;
     });
+}
+
+int hzl_surrogate_set_fused(hzl_surrogate_t h, int enabled) {
+    return guarded([&] { net_of(h).set_fused(enabled != 0); });
 }
 
 int hzl_surrogate_set_graph(hzl_surrogate_t h, int enabled) {

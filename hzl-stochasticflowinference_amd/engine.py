@@ -53,7 +53,8 @@ EXPORTS = (
     "hzl_surrogate_collect", "hzl_surrogate_dataset_size", "hzl_surrogate_fit",
     "hzl_surrogate_evaluate", "hzl_surrogate_error_estimate",
     "hzl_surrogate_set_error_estimate", "hzl_surrogate_spread",
-    "hzl_surrogate_forecast_or_solve", "hzl_surrogate_set_graph", "hzl_surrogate_forecast",
+    "hzl_surrogate_forecast_or_solve", "hzl_surrogate_set_graph", "hzl_surrogate_set_fused",
+    "hzl_surrogate_forecast",
     "hzl_surrogate_judge",
 )
 
@@ -191,6 +192,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_surrogate_forecast.argtypes = [vp, vp, C.c_double, sz, C.c_int]
     L.hzl_surrogate_error_estimate.argtypes = [vp, _dp]
     L.hzl_surrogate_set_graph.argtypes = [vp, C.c_int]
+    L.hzl_surrogate_set_fused.argtypes = [vp, C.c_int]
     L.hzl_surrogate_set_error_estimate.argtypes = [vp, C.c_double]
     L.hzl_surrogate_spread.argtypes = [vp, vp, _dp]
     L.hzl_surrogate_forecast_or_solve.argtypes = [vp, vp, C.c_double, sz, C.c_int, C.c_double,
@@ -1076,6 +1078,11 @@ class Surrogate:
     def set_graph(self, enabled: bool):
         """Replay forecasts from a captured HIP graph (default) or launch each kernel."""
         _check(_lib.hzl_surrogate_set_graph(self._h, int(enabled)))
+
+    def set_fused(self, enabled: bool):
+        """BF16 inference through the fused layer passes (default) or the
+        training path's batched products (hzl_surrogate_set_fused)."""
+        _check(_lib.hzl_surrogate_set_fused(self._h, int(enabled)))
 
     @property
     def error_estimate(self) -> float:

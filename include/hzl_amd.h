@@ -588,6 +588,15 @@ int hzl_surrogate_evaluate(hzl_surrogate_t h, int precision, size_t first, size_
  * Off: every forecast launches its kernels one by one. */
 int hzl_surrogate_set_graph(hzl_surrogate_t h, int enabled);
 
+/* BF16 inference path of a network of the default shape (64^2 grid, width
+ * 32, 16 x 32 modes, 3 input features): enabled (the default), each spectral
+ * layer runs as one fused forward-transform pass and one fused inverse-
+ * transform pass (MFMA, one workgroup per sample, intermediates in LDS) around
+ * the per-mode mixing product; disabled, as the five batched products of the
+ * training path.  Both round the same intermediates to BF16.  No counterpart
+ * in the reference. */
+int hzl_surrogate_set_fused(hzl_surrogate_t h, int enabled);
+
 /* The error estimate (negative until evaluated or set). */
 int hzl_surrogate_error_estimate(hzl_surrogate_t h, double *rms);
 int hzl_surrogate_set_error_estimate(hzl_surrogate_t h, double rms);

@@ -6,6 +6,7 @@
 // LDS, and the engine's launch sequence (kernels.hpp header comment) replayed
 // over host arrays.  Used by the CPU test suite to check indexing, layouts and
 // numerics against the oracle without a GPU.  Never linked into the product.
+#include "fno_fused.hpp"
 #include "gemm_mfma.hpp"
 #include "surrogate_kernels.hpp"
 #include "tables.hpp"
@@ -81,15 +82,17 @@ struct CpuCtx {
 };
 
 // Launch `blocks` workgroups of `threads` lanes; body(ctx, block, lds).
+// Each wavefront (64 lanes) exchanges its matrix-instruction operands through
+// its own scratch; every lane of the workgroup reaches each barrier.
 template <class Body> void launch(long blocks, int threads, size_t lds_bytes, Body body) {
     std::barrier<> bar(threads);
     std::vector<unsigned char> lds(lds_bytes + 64);
     std::vector<std::thread> pool;
-    MfmaScratch mx;
+    std::vector<MfmaScratch> mx((size_t)(threads + 63) / 64);
     pool.reserve((size_t)threads);
     for (int lane = 0; lane < threads; ++lane)
         pool.emplace_back([&, lane] {
-            CpuCtx ctx{lane, &bar, &mx};
+            CpuCtx ctx{lane, &bar, &mx[(size_t)lane / 64]};
             for (long b = 0; b < blocks; ++b) {
                 body(ctx, b, lds.data());
                 bar.arrive_and_wait();
@@ -589,6 +592,108 @@ int emu_proj_head(int bf16, long M, int C, int H, const float *X, const float *Q
     if (bf16)
         return proj_head_any<__bf16>(M, C, H, X, Q1, q1, Q2, q2, out);
     return proj_head_any<float>(M, C, H, X, Q1, q1, Q2, q2, out);
+}
+
+// Fused surrogate layer passes (fno_fused.hpp) over S samples; operands
+// arrive as FP32 and are rounded to BF16, outputs return as FP32.
+int emu_fno_forward(long S, const float *v, const float *Fx, const float *Fy2, float *V) {
+    using namespace fno;
+    auto bf = [](const float *x, size_t n) {
+        std::vector<__bf16> o(n);
+        for (size_t i = 0; i < n; ++i)
+            o[i] = (__bf16)x[i];
+        return o;
+    };
+    const auto bv = bf(v, (size_t)S * SAMPLE), bx = bf(Fx, 2 * KX * G), by = bf(Fy2, 2 * KY * 2 * G);
+    std::vector<__bf16> out((size_t)S * SPECTRAL);
+    FnoForwardArgs a{bv.data(), bx.data(), by.data(), out.data(), S};
+    launch(S, THREADS, FWD_LDS * sizeof(__bf16), [&](const CpuCtx &ctx, long b, unsigned char *l) {
+        fno_forward_body(ctx, a, reinterpret_cast<__bf16 *>(l), b);
+    });
+    for (size_t i = 0; i < out.size(); ++i)
+        V[i] = (float)out[i];
+    return 0;
+}
+
+int emu_fno_inverse(long S, const float *U, const float *Gy2, const float *Gx, const float *v,
+                    const float *W, const float *bias, int act, float *out) {
+    using namespace fno;
+    auto bf = [](const float *x, size_t n) {
+        std::vector<__bf16> o(n);
+        for (size_t i = 0; i < n; ++i)
+            o[i] = (__bf16)x[i];
+        return o;
+    };
+    const auto bu = bf(U, (size_t)S * SPECTRAL), by = bf(Gy2, 2 * G * 2 * KY),
+               bx = bf(Gx, G * 2 * KX), bv = bf(v, (size_t)S * SAMPLE), bw = bf(W, C * C);
+    std::vector<__bf16> o((size_t)S * SAMPLE);
+    FnoInverseArgs a{bu.data(), by.data(), bx.data(), bv.data(), bw.data(), bias, o.data(), S,
+                     act};
+    launch(S * NBLKI, THREADS, INV_LDS * sizeof(__bf16),
+           [&](const CpuCtx &ctx, long b, unsigned char *l) {
+               fno_inverse_body(ctx, a, reinterpret_cast<__bf16 *>(l), b);
+           });
+    for (size_t i = 0; i < o.size(); ++i)
+        out[i] = (float)o[i];
+    return 0;
+}
+
+int emu_fno_head(long M, const float *X, const float *Q1, const float *q1, const float *Q2,
+                 const float *q2, float *out) {
+    using namespace fno;
+    std::vector<__bf16> x((size_t)M * C), q((size_t)FNO_H * C);
+    for (size_t i = 0; i < x.size(); ++i)
+        x[i] = (__bf16)X[i];
+    for (size_t i = 0; i < q.size(); ++i)
+        q[i] = (__bf16)Q1[i];
+    FnoHeadArgs a{x.data(), q.data(), q1, Q2, q2, out, M};
+    launch((M + 31) / 32, 64, 0, [&](const CpuCtx &ctx, long wave, unsigned char *) {
+        fno_head_wave(ctx, a, wave, ctx.lane);
+    });
+    return 0;
+}
+
+// Fused state -> grid (FP32 output [S][G][G]) and grid -> increment passes.
+int emu_state_to_grid(long S, int K, const float *states, const float *g1, const float *g2,
+                      float alpha, float *out) {
+    using namespace fno;
+    const long spec2 = 2L * (K + 1) * (2 * K + 1);
+    StateGridArgs a{states, spec2, K, 2 * K + 1, g1, g2, out, 0, G, 1, (long)G * G, alpha, S};
+    launch(S, SG_THREADS, SG_LDS * sizeof(float), [&](const CpuCtx &ctx, long b, unsigned char *l) {
+        fno_state_to_grid_body(ctx, a, reinterpret_cast<float *>(l), b);
+    });
+    return 0;
+}
+
+int emu_grid_to_increment(long S, const float *grid, const float *f1, const float *f2,
+                          float *incr) {
+    using namespace fno;
+    GridIncrementArgs a{grid, f1, f2, incr, S};
+    launch(S, SG_THREADS, GS_LDS * sizeof(float), [&](const CpuCtx &ctx, long b, unsigned char *l) {
+        fno_grid_to_increment_body(ctx, a, reinterpret_cast<float *>(l), b);
+    });
+    return 0;
+}
+
+int emu_gelu_as(long n, const float *x, float *y) {
+    for (long i = 0; i < n; ++i)
+        y[i] = gelu_as(x[i]);
+    return 0;
+}
+
+int emu_fno_lift(long points, const float *feat, const float *Lw, const float *Lb, float *v) {
+    using namespace fno;
+    std::vector<__bf16> f((size_t)points * CIN), w(C * CIN), o((size_t)points * C);
+    for (size_t i = 0; i < f.size(); ++i)
+        f[i] = (__bf16)feat[i];
+    for (size_t i = 0; i < w.size(); ++i)
+        w[i] = (__bf16)Lw[i];
+    FnoLiftArgs a{f.data(), w.data(), Lb, o.data(), points};
+    for (long i = 0; i < points * (C / 8); ++i)
+        fno_lift_body(a, i);
+    for (size_t i = 0; i < o.size(); ++i)
+        v[i] = (float)o[i];
+    return 0;
 }
 
 int emu_gemm(int bf16, const int *dims, const long *strides, const float *A, size_t na,

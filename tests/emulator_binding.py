@@ -25,6 +25,20 @@ class Emulator:
         L.emu_gemm.argtypes = [C.c_int, _ip, _lp, _fp, C.c_size_t, _fp, C.c_size_t, _fp,
                                C.c_size_t, _fp, _fp, C.c_size_t, _fp, C.c_float, C.c_float,
                                C.c_int, C.c_int]
+        L.emu_fno_forward.restype = C.c_int
+        L.emu_fno_forward.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
+        L.emu_fno_inverse.restype = C.c_int
+        L.emu_fno_inverse.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int, _fp]
+        L.emu_fno_head.restype = C.c_int
+        L.emu_fno_head.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp]
+        L.emu_state_to_grid.restype = C.c_int
+        L.emu_state_to_grid.argtypes = [C.c_long, C.c_int, _fp, _fp, _fp, C.c_float, _fp]
+        L.emu_grid_to_increment.restype = C.c_int
+        L.emu_grid_to_increment.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
+        L.emu_gelu_as.restype = C.c_int
+        L.emu_gelu_as.argtypes = [C.c_long, _fp, _fp]
+        L.emu_fno_lift.restype = C.c_int
+        L.emu_fno_lift.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
         L.emu_proj_head.restype = C.c_int
         L.emu_proj_head.argtypes = [C.c_int, C.c_long, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp,
                                     _fp]
@@ -67,6 +81,82 @@ class Emulator:
                                int(c_bf16))
         assert rc == 0
         return out, pre
+
+    @staticmethod
+    def _f32(x):
+        return np.ascontiguousarray(x, dtype=np.float32)
+
+    def fno_forward(self, v, Fx, Fy2):
+        """fno_forward_body over v [S][64][64][32]: V [S][16][64][32]."""
+        v, Fx, Fy2 = self._f32(v), self._f32(Fx), self._f32(Fy2)
+        S = v.shape[0]
+        out = np.zeros((S, 16, 64, 32), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_fno_forward(S, v.ctypes.data_as(fp), Fx.ctypes.data_as(fp),
+                                        Fy2.ctypes.data_as(fp), out.ctypes.data_as(fp)) == 0
+        return out
+
+    def fno_head(self, X, Q1, q1, Q2, q2):
+        """fno_head_wave over X [M][32], Q1 [128][32]: out [M]."""
+        X, Q1 = self._f32(X), self._f32(Q1)
+        q1, Q2, q2 = self._f32(q1), self._f32(Q2), self._f32(np.atleast_1d(q2))
+        out = np.zeros(X.shape[0], dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_fno_head(X.shape[0], X.ctypes.data_as(fp), Q1.ctypes.data_as(fp),
+                                     q1.ctypes.data_as(fp), Q2.ctypes.data_as(fp),
+                                     q2.ctypes.data_as(fp), out.ctypes.data_as(fp)) == 0
+        return out
+
+    def state_to_grid(self, K, states, g1, g2, alpha):
+        """fno_state_to_grid_body: states [S][2 (K+1)(2K+1)] FP32 -> [S][64][64]."""
+        st, g1, g2 = self._f32(states), self._f32(g1), self._f32(g2)
+        S = st.shape[0]
+        out = np.zeros((S, 64, 64), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_state_to_grid(S, K, st.ctypes.data_as(fp), g1.ctypes.data_as(fp),
+                                          g2.ctypes.data_as(fp), alpha,
+                                          out.ctypes.data_as(fp)) == 0
+        return out
+
+    def grid_to_increment(self, grid, f1, f2):
+        """fno_grid_to_increment_body: grid [S][64][64] -> incr [S][32][126]."""
+        grid, f1, f2 = self._f32(grid), self._f32(f1), self._f32(f2)
+        S = grid.shape[0]
+        out = np.zeros((S, 32, 126), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_grid_to_increment(S, grid.ctypes.data_as(fp), f1.ctypes.data_as(fp),
+                                              f2.ctypes.data_as(fp),
+                                              out.ctypes.data_as(fp)) == 0
+        return out
+
+    def gelu_as(self, x):
+        x = self._f32(x)
+        y = np.zeros_like(x)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_gelu_as(x.size, x.ctypes.data_as(fp), y.ctypes.data_as(fp)) == 0
+        return y
+
+    def fno_inverse(self, U, Gy2, Gx, v, W, bias, act):
+        """fno_inverse_body: U [S][16][64][32], v [S][64][64][32] -> [S][64][64][32]."""
+        U, Gy2, Gx, v = self._f32(U), self._f32(Gy2), self._f32(Gx), self._f32(v)
+        W, bias = self._f32(W), self._f32(bias)
+        S = v.shape[0]
+        out = np.zeros_like(v)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_fno_inverse(S, U.ctypes.data_as(fp), Gy2.ctypes.data_as(fp),
+                                        Gx.ctypes.data_as(fp), v.ctypes.data_as(fp),
+                                        W.ctypes.data_as(fp), bias.ctypes.data_as(fp), int(act),
+                                        out.ctypes.data_as(fp)) == 0
+        return out
+
+    def fno_lift(self, feat, Lw, Lb):
+        feat, Lw, Lb = self._f32(feat), self._f32(Lw), self._f32(Lb)
+        points = feat.shape[0]
+        out = np.zeros((points, 32), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_fno_lift(points, feat.ctypes.data_as(fp), Lw.ctypes.data_as(fp),
+                                     Lb.ctypes.data_as(fp), out.ctypes.data_as(fp)) == 0
+        return out
 
     def proj_head(self, bf16, X, Q1, q1, Q2, q2):
         """proj_head_wave on the CPU lanes: X [M][C], Q1 [H][C]; returns out [M]."""

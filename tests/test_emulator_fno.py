@@ -1,0 +1,165 @@
+"""CPU checks of the fused surrogate layer passes (csrc/fno_fused.hpp) through
+the lane emulator, against numpy in FP64 on the same BF16-rounded operands with
+the BF16 roundings the passes make (the x-transformed rows, the spectral
+output, the inverse-y rows, the layer output).
+
+The transform matrices are built here from their definitions (the surrogate
+stores the same ones, csrc/surrogate.hip FnoConstants):
+  Fx  [2kx+ri][x]        (cos, -sin)(2 pi kx x / G)
+  Fy2 [2kyi+ro][ri G+y]  [[c, s], [-s, c]],  theta = 2 pi ky y / G
+  Gy2 [ro G+y][2kyi+ri]  [[c, -s], [s, c]]
+  Gx  [x][2kx+ri]        w / G^2 (cos, -sin),  w = 1 (kx = 0) else 2
+A planted indexing mistake in a pass (a transposed operand, a swapped re/im
+half, a wrong row block) moves whole rows of the output and fails these
+tolerances by orders of magnitude.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from test_emulator_gemm import bf16, gelu
+
+G, C, KX, KY = 64, 32, 16, 32
+
+
+def constants():
+    fx = np.zeros((2 * KX, G))
+    gx = np.zeros((G, 2 * KX))
+    for kx in range(KX):
+        ph = 2 * np.pi * kx * np.arange(G) / G
+        w = (1.0 if kx == 0 else 2.0) / (G * G)
+        fx[2 * kx], fx[2 * kx + 1] = np.cos(ph), -np.sin(ph)
+        gx[:, 2 * kx], gx[:, 2 * kx + 1] = w * np.cos(ph), -w * np.sin(ph)
+    fy = np.zeros((2 * KY, 2 * G))
+    gy = np.zeros((2 * G, 2 * KY))
+    y = np.arange(G)
+    for kyi in range(KY):
+        ky = kyi if kyi < KY // 2 else kyi - KY
+        th = 2 * np.pi * ky * y / G
+        c, s = np.cos(th), np.sin(th)
+        fy[2 * kyi, :G], fy[2 * kyi, G:] = c, s
+        fy[2 * kyi + 1, :G], fy[2 * kyi + 1, G:] = -s, c
+        gy[:G, 2 * kyi], gy[:G, 2 * kyi + 1] = c, -s
+        gy[G:, 2 * kyi], gy[G:, 2 * kyi + 1] = s, c
+    return [bf16(m).astype(np.float64) for m in (fx, fy, gy, gx)]
+
+
+def close(got, want, rel=1.2e-2):
+    """BF16 outputs: elementwise within 2 ulp of BF16 (2^-7) of the larger of
+    the value and 5% of the field's scale (a BF16 rounding flip of one
+    intermediate moves an output by ~2^-8 of that intermediate)."""
+    scale = np.max(np.abs(want))
+    err = np.abs(got - want)
+    bound = rel * np.maximum(np.abs(want), 0.05 * scale)
+    print("max err / scale", float(np.max(err) / scale), "worst ratio",
+          float(np.max(err / bound)))
+    assert np.all(err <= bound)
+    assert np.linalg.norm(got - want) <= 4e-3 * np.linalg.norm(want)
+
+
+def test_forward_pass_matches_numpy(emu):
+    rng = np.random.default_rng(11)
+    S = 2
+    fx, fy, _, _ = constants()
+    v = bf16(rng.standard_normal((S, G, G, C))).astype(np.float64)
+    got = emu.fno_forward(v, fx, fy)
+    # A[s][kx'][y][c] = sum_x Fx[kx'][x] v[s][y][x][c], stored in BF16
+    A = bf16(np.einsum("kx,syxc->skyc", fx, v)).astype(np.float64)
+    A2 = A.reshape(S, KX, 2, G, C).reshape(S, KX, 2 * G, C)  # m = ri G + y
+    want = bf16(np.einsum("pm,skmc->skpc", fy, A2))
+    close(got, want)
+
+
+@pytest.mark.parametrize("act", [0, 1], ids=["linear", "gelu"])
+def test_inverse_pass_matches_numpy(emu, act):
+    rng = np.random.default_rng(12 + act)
+    S = 2
+    _, _, gy, gx = constants()
+    U = bf16(rng.standard_normal((S, KX, 2 * KY, C))).astype(np.float64)  # [s][kx][k'][c]
+    v = bf16(rng.standard_normal((S, G, G, C))).astype(np.float64)
+    W = bf16(rng.standard_normal((C, C)) / math.sqrt(C)).astype(np.float64)
+    bias = rng.standard_normal(C).astype(np.float32)
+    got = emu.fno_inverse(U, gy, gx, v, W, bias, act)
+    # Bi[s][kx'=2kx+ro][y][c] = sum_k' Gy2[ro G + y][k'] U[s][kx][k'][c], in BF16
+    bi = np.einsum("mk,sxkc->sxmc", gy, U)  # [s][kx][m = ro G + y][c]
+    bi = bf16(bi.reshape(S, KX, 2, G, C).reshape(S, 2 * KX, G, C)).astype(np.float64)
+    pre = (np.einsum("xk,skyc->syxc", gx, bi) + np.einsum("syxd,cd->syxc", v, W)
+           + bias.astype(np.float64))
+    want = bf16(gelu(pre) if act else pre)
+    close(got, want)
+
+
+def test_lift_matches_numpy(emu):
+    rng = np.random.default_rng(13)
+    points = 300
+    feat = bf16(rng.standard_normal((points, 3))).astype(np.float64)
+    Lw = bf16(rng.standard_normal((C, 3))).astype(np.float64)
+    Lb = rng.standard_normal(C).astype(np.float32)
+    got = emu.fno_lift(feat, Lw, Lb)
+    want = bf16(feat @ Lw.T + Lb.astype(np.float64))
+    close(got, want)
+
+
+def test_head_matches_numpy(emu):
+    """Projection C -> 128 -> 1 with GELU, 75 points (a ragged last wavefront)."""
+    rng = np.random.default_rng(14)
+    M, H = 75, 128
+    X = bf16(rng.standard_normal((M, C))).astype(np.float64)
+    Q1 = bf16(rng.standard_normal((H, C)) / math.sqrt(C)).astype(np.float64)
+    q1 = rng.standard_normal(H).astype(np.float32)
+    Q2 = (rng.standard_normal(H) / math.sqrt(H)).astype(np.float32)
+    q2 = np.float32(0.3)
+    got = emu.fno_head(X, Q1, q1, Q2, q2)
+    want = gelu(X @ Q1.T + q1.astype(np.float64)) @ Q2.astype(np.float64) + float(q2)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print("head max rel err", err)
+    assert err < 1e-5
+
+
+def test_gelu_as_matches_exact_erf(emu):
+    """Abramowitz & Stegun 7.1.26: |erf error| <= 1.5e-7, so GELU within
+    0.5 |x| 1.5e-7 plus FP32 rounding."""
+    x = np.linspace(-9.0, 9.0, 20001).astype(np.float32)
+    got = emu.gelu_as(x).astype(np.float64)
+    want = gelu(x.astype(np.float64))
+    bound = 0.5 * np.abs(x) * 1.6e-7 + 2e-7 * np.maximum(1.0, np.abs(want))
+    print("max gelu err", float(np.max(np.abs(got - want))))
+    assert np.all(np.abs(got - want) <= bound)
+
+
+@pytest.mark.parametrize("K", [42, 85], ids=["state128", "state256"])
+def test_state_to_grid_matches_numpy(emu, K):
+    """The fused state -> grid pass is the product pair of the generic path:
+    Bm = g1 st_band^T (band kx <= 31, |ky| <= 31 of the retained half plane),
+    out = alpha Bm2 g2^T with Bm2[y][ro 32 + kx] = Bm[2y + ro][kx]."""
+    rng = np.random.default_rng(15 + K)
+    S, J, KS, JS, KXS = 2, 2 * K + 1, 31, 63, 32
+    states = rng.standard_normal((S, K + 1, J, 2)).astype(np.float32)
+    g1 = rng.standard_normal((2 * G, 2 * JS)).astype(np.float32)
+    g2 = rng.standard_normal((G, 2 * KXS)).astype(np.float32)
+    got = emu.state_to_grid(K, states.reshape(S, -1), g1, g2, 0.5)
+    band = states[:, :KS + 1, K - KS:K + KS + 1, :].reshape(S, KS + 1, 2 * JS).astype(np.float64)
+    bm = np.einsum("rk,sxk->srx", g1.astype(np.float64), band)          # [s][2y+ro][kx]
+    bm2 = bm.reshape(S, G, 2 * KXS)                                      # [s][y][ro 32 + kx]
+    want = 0.5 * np.einsum("syk,xk->syx", bm2, g2.astype(np.float64))
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print("state->grid rel err", err)
+    assert err < 2e-5
+
+
+def test_grid_to_increment_matches_numpy(emu):
+    rng = np.random.default_rng(17)
+    S, JS = 2, 63
+    grid = rng.standard_normal((S, G, G)).astype(np.float32)
+    f1 = rng.standard_normal((64, G)).astype(np.float32)
+    f2 = rng.standard_normal((2 * JS, 2 * G)).astype(np.float32)
+    got = emu.grid_to_increment(grid, f1, f2)
+    dg = np.einsum("rx,syx->sry", f1.astype(np.float64), grid.astype(np.float64))  # [s][2kx+ri][y]
+    dg2 = dg.reshape(S, 32, 2 * G)
+    want = np.einsum("skm,nm->skn", dg2, f2.astype(np.float64))
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print("grid->increment rel err", err)
+    assert err < 2e-5

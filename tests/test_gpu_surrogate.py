@@ -59,6 +59,48 @@ def test_forward_matches_reference(pkg, orc, cfg, precision, tol):
     assert rel(got, want) < tol
 
 
+def test_fused_inference_matches_batched_products(pkg, orc):
+    """The fused BF16 layer passes (csrc/fno_fused.hpp: one forward- and one
+    inverse-transform pass per layer) against the training path's batched
+    products on the same parameters and inputs: both round the same
+    intermediates to BF16, so they agree to FP32 summation order and rare
+    BF16 rounding flips; a forecast replayed from its captured graph equals
+    the eager fused forecast bit for bit."""
+    s = shape_of(FULL)
+    S = 8
+    f = grf_fields(orc, s.n, S, 31)
+    nu = np.linspace(5e-4, 2e-3, S)
+    amp = np.linspace(0.05, 0.4, S)
+    scale = 1.0 / np.sqrt(np.mean(F.sample_grid(s, f).numpy() ** 2))
+    with pkg.Surrogate(**FULL, seed=5) as sur:
+        sur.set_scales(scale, 2.0)
+        fused = sur.forward(f, nu, "bf16", amp=amp)
+        sur.set_fused(False)
+        batched = sur.forward(f, nu, "bf16", amp=amp)
+        fp32 = sur.forward(f, nu, "fp32", amp=amp)
+        sur.set_fused(True)
+        p = torch.as_tensor(sur.get_params().astype(np.float64))
+        want = F.forward(s, p, f, nu, scale, amp).numpy()
+        print("fused vs batched", rel(fused, batched), "fused vs fp32", rel(fused, fp32),
+              "fused vs fp64", rel(fused, want), "batched vs fp64", rel(batched, want))
+        assert rel(fused, batched) < 5e-3
+        assert rel(fused, want) < 4e-2
+        # forecasts: captured and replayed (calls 2-4) vs a graph-free twin
+        with pkg.Surrogate(**FULL, seed=5) as eager, \
+                pkg.Ensemble(s.n, S, "fp32", sigma=0.2, seed=3) as a, \
+                pkg.Ensemble(s.n, S, "fp32", sigma=0.2, seed=3) as b:
+            eager.set_scales(scale, 2.0)
+            eager.set_graph(False)
+            a.set_state(f)
+            b.set_state(f)
+            for k in range(4):
+                sur.forecast(a, 1e-3, 50, "bf16")
+                eager.forecast(b, 1e-3, 50, "bf16")
+                a.synchronize()
+                b.synchronize()
+                np.testing.assert_array_equal(a.get_state(), b.get_state())
+
+
 @pytest.mark.parametrize("cfg,S", [(SMALL, 2), (FULL, 1)], ids=["small", "full"])
 def test_gradients_match_autograd(pkg, orc, cfg, S):
     s = shape_of(cfg)
