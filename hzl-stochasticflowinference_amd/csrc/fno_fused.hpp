@@ -265,6 +265,7 @@ HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *ld
     __bf16 *UT = lds + BI_ELEMS + wave * UT_ELEMS; // this wavefront's U^T slice [c][k']
     {
         const int y0 = b * YBI;
+
         // (1) inverse y of kx = 2 wave + q: D[m][c], m = 16 ro + yl
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -394,6 +395,72 @@ HZL_HD void fno_head_wave(const Ctx &ctx, const FnoHeadArgs &a, long wave, int l
             const long m = m0 + 8 * (i / 4) + 4 * h + i % 4;
             if (m < a.M)
                 a.out[m] = rs[i] + q2;
+        }
+    }
+}
+
+// Per-mode channel mixing U[s][mode][n] = sum_k V[s][mode][k] R[mode][n][k]
+// (n = 2 ro + ..., k over the 2C real inputs, the generic product's layouts:
+// V, U [S][modes][2C], R [modes][2C][2C]).  One workgroup per mode keeps the
+// mode's 64 x 64 weights in registers (B operand) and sweeps the samples in
+// 32-row tiles (A operand straight from HBM, 16-byte fragments).
+constexpr int MIX_THREADS = 256;
+struct FnoMixArgs {
+    const __bf16 *V;
+    const __bf16 *R;
+    __bf16 *U;
+    long S;
+    int modes;
+};
+
+template <class Ctx>
+HZL_HD void fno_mix_body(const Ctx &ctx, const FnoMixArgs &a, long mode) {
+    using namespace fno;
+    constexpr int N2 = 2 * C; // 64 outputs and inputs per mode
+    if (mode >= a.modes)
+        return; // whole workgroup
+    const int t = ctx.tid(), w = t / 64, lane = t % 64, r = lane % 32, h = lane / 32;
+    const __bf16 *Rm = a.R + mode * (long)N2 * N2;
+    bf16x8_t rf[2][N2 / 16]; // rf[nt][kb] = R[32 nt + r][16 kb + 8 h ..]
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int kb = 0; kb < N2 / 16; ++kb)
+            rf[nt][kb] = load8(Rm + (32 * nt + r) * N2 + 16 * kb + 8 * h);
+    const long stride = (long)a.modes * N2; // between samples
+    // every wavefront runs the same number of tiles (rows past S are zeros
+    // and are not stored), so matrix instructions stay uniform per workgroup
+    constexpr int WAVES = MIX_THREADS / 64;
+    const long rounds = ((a.S + 31) / 32 + WAVES - 1) / WAVES;
+    for (long it = 0; it < rounds; ++it) {
+        const long tile = it * WAVES + w;
+        const long s = tile * 32 + r;
+        bf16x8_t vf[N2 / 16];
+#pragma unroll
+        for (int kb = 0; kb < N2 / 16; ++kb) {
+            if (s < a.S) {
+                vf[kb] = load8(a.V + s * stride + mode * N2 + 16 * kb + 8 * h);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    vf[kb][j] = (__bf16)0.f;
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            f32x16_t d;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                d[i] = 0.f;
+#pragma unroll
+            for (int kb = 0; kb < N2 / 16; ++kb)
+                fno_mfma(ctx, vf[kb], rf[nt][kb], d, lane);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const long so = tile * 32 + 8 * (i / 4) + 4 * h + i % 4;
+                if (so < a.S)
+                    a.U[so * stride + mode * N2 + 32 * nt + r] = (__bf16)d[i];
+            }
         }
     }
 }

@@ -76,6 +76,9 @@ __global__ void __launch_bounds__(SG_THREADS) k_grid_to_increment(GridIncrementA
     __shared__ float lds[GS_LDS];
     fno_grid_to_increment_body(DevCtx{}, a, lds, (long)blockIdx.x);
 }
+__global__ void __launch_bounds__(MIX_THREADS) k_fno_mix(FnoMixArgs a) {
+    fno_mix_body(DevCtx{}, a, (long)blockIdx.x);
+}
 __global__ void __launch_bounds__(256) k_fno_head(FnoHeadArgs a) {
     fno_head_wave(DevCtx{}, a, (long)blockIdx.x * 4 + threadIdx.x / 64, (int)(threadIdx.x % 64));
 }
@@ -1187,7 +1190,7 @@ class Surrogate {
 
     // Lift, spectral layers and head of a BF16 inference pass through the
     // fused passes (fno_fused.hpp): per layer one forward-transform pass, the
-    // per-mode mixing product and one inverse-transform pass with the
+    // per-mode mixing pass and one inverse-transform pass with the
     // pointwise term, bias and activation; then the projection and head.
     // w.feat holds the input features; the output lands in w.out.
     void layers_fused(FnoWork<__bf16> &w, int S) {
@@ -1203,12 +1206,10 @@ class Surrogate {
         for (int l = 0; l < s.L; ++l) {
             FnoForwardArgs fa{w.v[l]->p, w.cst.Fx.p, w.cst.Fy2.p, w.V[l]->p, (long)S};
             hipLaunchKernelGGL(k_fno_forward, dim3((unsigned)S), dim3(THREADS), 0, st_, fa);
-            {
-                Gemm<__bf16> r(S, 2 * C, 2 * C, cs(*w.V[l]), SV, 1, cs(w.rcat, (size_t)(l * rper)),
-                               2 * C, 1, vs(*w.U[l]), SV, 1, OUT_BF16);
-                r.batch(s.modes(), 2 * C, 4L * C * C, 2 * C);
-                r.run(st_);
-            }
+            if (w.rcat.n < (size_t)((l + 1) * rper))
+                throw std::logic_error("fused surrogate mixing weights out of bounds");
+            FnoMixArgs ma{w.V[l]->p, w.rcat.p + l * rper, w.U[l]->p, (long)S, s.modes()};
+            hipLaunchKernelGGL(k_fno_mix, dim3((unsigned)s.modes()), dim3(MIX_THREADS), 0, st_, ma);
             FnoInverseArgs ia{w.U[l]->p,
                               w.cst.Gy2.p,
                               w.cst.Gx.p,

@@ -675,6 +675,23 @@ int emu_grid_to_increment(long S, const float *grid, const float *f1, const floa
     return 0;
 }
 
+int emu_fno_mix(long S, int modes, const float *V, const float *R, float *U) {
+    using namespace fno;
+    const size_t nv = (size_t)S * modes * 2 * C, nr = (size_t)modes * 4 * C * C;
+    std::vector<__bf16> v(nv), rr(nr), u(nv);
+    for (size_t i = 0; i < nv; ++i)
+        v[i] = (__bf16)V[i];
+    for (size_t i = 0; i < nr; ++i)
+        rr[i] = (__bf16)R[i];
+    FnoMixArgs a{v.data(), rr.data(), u.data(), S, modes};
+    launch(modes, MIX_THREADS, 0, [&](const CpuCtx &ctx, long b, unsigned char *) {
+        fno_mix_body(ctx, a, b);
+    });
+    for (size_t i = 0; i < nv; ++i)
+        U[i] = (float)u[i];
+    return 0;
+}
+
 int emu_gelu_as(long n, const float *x, float *y) {
     for (long i = 0; i < n; ++i)
         y[i] = gelu_as(x[i]);

@@ -35,6 +35,8 @@ class Emulator:
         L.emu_state_to_grid.argtypes = [C.c_long, C.c_int, _fp, _fp, _fp, C.c_float, _fp]
         L.emu_grid_to_increment.restype = C.c_int
         L.emu_grid_to_increment.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
+        L.emu_fno_mix.restype = C.c_int
+        L.emu_fno_mix.argtypes = [C.c_long, C.c_int, _fp, _fp, _fp]
         L.emu_gelu_as.restype = C.c_int
         L.emu_gelu_as.argtypes = [C.c_long, _fp, _fp]
         L.emu_fno_lift.restype = C.c_int
@@ -128,6 +130,16 @@ class Emulator:
                                               f2.ctypes.data_as(fp),
                                               out.ctypes.data_as(fp)) == 0
         return out
+
+    def fno_mix(self, V, R):
+        """fno_mix_body: V [S][modes][64], R [modes][64][64] -> U [S][modes][64]."""
+        V, R = self._f32(V), self._f32(R)
+        S, modes = V.shape[0], V.shape[1]
+        U = np.zeros_like(V)
+        fp = C.POINTER(C.c_float)
+        assert self.lib.emu_fno_mix(S, modes, V.ctypes.data_as(fp), R.ctypes.data_as(fp),
+                                    U.ctypes.data_as(fp)) == 0
+        return U
 
     def gelu_as(self, x):
         x = self._f32(x)

@@ -163,3 +163,14 @@ def test_grid_to_increment_matches_numpy(emu):
     err = np.max(np.abs(got - want)) / np.max(np.abs(want))
     print("grid->increment rel err", err)
     assert err < 2e-5
+
+
+def test_mixing_pass_matches_numpy(emu):
+    """Per-mode mixing: 40 samples (a ragged second sample tile), 3 modes."""
+    rng = np.random.default_rng(19)
+    S, modes = 40, 3
+    V = bf16(rng.standard_normal((S, modes, 2 * C))).astype(np.float64)
+    R = bf16(rng.standard_normal((modes, 2 * C, 2 * C)) / 8).astype(np.float64)
+    got = emu.fno_mix(V, R)
+    want = bf16(np.einsum("smk,mnk->smn", V, R))
+    close(got, want)
