@@ -394,6 +394,36 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
     return out
 
 
+def surrogate_forecast(pkg, reps: int = 20) -> dict:
+    """BF16 surrogate forecast (the bench's network shape: 4 layers, width 32,
+    16 x 32 modes, 64^2 grid; weights as initialised) of 256 and 1024 128^2
+    FP32 particles, graphs on: the fused layer passes against the batched
+    products of the training path, milliseconds per forecast (HIP events on
+    the ensemble's stream, `reps` forecasts after 3 untimed ones)."""
+    timer = HipTimer()
+    out = {"network": "4 layers, width 32, 16 x 32 modes, 64^2 grid, 128^2 state, BF16"}
+    for B in (256, 1024):
+        for fused in (True, False):
+            with pkg.Surrogate(128, 64, 32, 4, 16, seed=SEED) as sur, \
+                    pkg.Ensemble(128, B, "fp32", nu=1e-3, sigma=0.05, amp=0.1, seed=SEED) as e:
+                sur.set_scales(1.5, 4.0)
+                sur.set_fused(fused)
+                e.init_grf()
+                for _ in range(3):
+                    sur.forecast(e, 1e-3, 50, "bf16")
+                e.synchronize()
+                e0, e1 = timer.event(), timer.event()
+                timer.record(e0, e.stream)
+                for _ in range(reps):
+                    sur.forecast(e, 1e-3, 50, "bf16")
+                timer.record(e1, e.stream)
+                ms = timer.elapsed_ms(e0, e1) / reps
+                e.synchronize()
+            out[f"p{B}_{'fused' if fused else 'batched'}_ms"] = round(ms, 4)
+        out[f"p{B}_speedup"] = round(out[f"p{B}_batched_ms"] / out[f"p{B}_fused_ms"], 2)
+    return out
+
+
 def low_mode_particles(pkg, truth0, particles: int, scale: float = 1.0):
     """A prior the truth could have been drawn from: particles = background +
     scale x independent GRFs kept on the forced shell |k| <= 4 (the large
@@ -742,6 +772,8 @@ def main() -> int:
                     help="only the surrogate-assisted pMCMC comparison")
     ap.add_argument("--no-pmcmc-surrogate", action="store_true",
                     help="skip the surrogate-assisted pMCMC comparison")
+    ap.add_argument("--forecast-only", action="store_true",
+                    help="only the BF16 surrogate forecast timing (fused vs batched layers)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel from the host (HZL_GRAPHS=0; for kernel tracers)")
     ap.add_argument("--enkf-only", action="store_true", help="only the EnKF cycle")
@@ -784,6 +816,9 @@ def main() -> int:
         return 0
     if args.pmcmc_surrogate_only:
         print(json.dumps({"pmcmc_surrogate": pmcmc_surrogate(pkg)}), flush=True)
+        return 0
+    if args.forecast_only:
+        print(json.dumps({"surrogate_forecast": surrogate_forecast(pkg)}), flush=True)
         return 0
     n, B = args.n, args.members
     nu = dt = 5e-4
@@ -908,6 +943,7 @@ def main() -> int:
     line["fp64"] = fp64_steps(pkg, n, B, args.steps, args.warmup, dt, nu, not args.no_tune)
     if not args.no_pf:
         line["surrogate_pf"] = surrogate_pf(pkg)
+        line["surrogate_forecast"] = surrogate_forecast(pkg)
     if not args.no_enkf:
         line["enkf"] = enkf_cycle(pkg)
     if not args.no_pmcmc:

@@ -115,23 +115,25 @@ HZL_HD bf16x8_t load8(const __bf16 *p) {
 // GELU with erf by Abramowitz & Stegun 7.1.26 (|erf error| <= 1.5e-7, one
 // exponential and one reciprocal): the inference passes' activation, far
 // inside the BF16 rounding of their outputs.  Training keeps gelu() (erff).
-HZL_HD float erf_as(float x) {
+// Written as gelu(x) = x Phi(x), Phi(x) = 1 - q (x >= 0) or q (x < 0) with
+// q = erfc(|x| / sqrt 2) / 2 = t (a1 + t (a2 + ...)) / 2 e^{-x^2 / 2},
+// t = 1 / (1 + p |x| / sqrt 2).
+HZL_HD float gelu_as(float x) {
+    constexpr float P = 0.3275911f * 0.70710678118654752f;
+    constexpr float A1 = 0.5f * 0.254829592f, A2 = 0.5f * -0.284496736f,
+                    A3 = 0.5f * 1.421413741f, A4 = 0.5f * -1.453152027f,
+                    A5 = 0.5f * 1.061405429f;
+    constexpr float HALF_LOG2E = 0.72134752044448170f; // log2(e) / 2
     const float ax = x < 0.f ? -x : x;
 #if __HIP_DEVICE_COMPILE__
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
-    const float e = __expf(-ax * ax);
+    const float t = __builtin_amdgcn_rcpf(1.0f + P * ax);
+    const float e = __builtin_amdgcn_exp2f(-HALF_LOG2E * (x * x));
 #else
-    const float t = 1.0f / (1.0f + 0.3275911f * ax);
-    const float e = std::exp(-ax * ax);
+    const float t = 1.0f / (1.0f + P * ax);
+    const float e = std::exp2(-HALF_LOG2E * (x * x));
 #endif
-    const float poly =
-        t * (0.254829592f +
-             t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-    const float y = 1.0f - poly * e;
-    return x < 0.f ? -y : y;
-}
-HZL_HD float gelu_as(float x) {
-    return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f));
+    const float q = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * e;
+    return x * (x < 0.f ? q : 1.0f - q);
 }
 
 template <class Ctx>
@@ -345,31 +347,29 @@ HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *ld
 
 // Projection and head for the 32 points of wavefront `wave`:
 //   out[m] = q2 + sum_j Q2[j] gelu(q1[j] + sum_c X[m][c] Q1[j][c])
-// Both operands load straight into MFMA fragments (X rows and Q1 rows have C
-// contiguous channels); per-row partial sums are reduced across the 32 lanes
-// of each half-wave.
+// The hidden activations are computed transposed, D[j][m] (A = Q1 rows, B = X
+// rows, both with C contiguous channels, loaded straight into MFMA fragments),
+// so each lane holds 16 hidden units of one point: the sum over j is a sum in
+// registers plus one exchange between the half-waves.  q1 and Q2 are read
+// through the workgroup's LDS (hq: FNO_H values each).
 template <class Ctx>
-HZL_HD void fno_head_wave(const Ctx &ctx, const FnoHeadArgs &a, long wave, int lane) {
+HZL_HD void fno_head_wave(const Ctx &ctx, const FnoHeadArgs &a, long wave, int lane,
+                          const float *hq) {
     using namespace fno;
     const long m0 = wave * 32;
-    if (m0 >= a.M)
-        return; // whole wavefront
     const int r = lane % 32, h = lane / 32;
-    bf16x8_t fa[C / 16];
+    bf16x8_t xb[C / 16];
 #pragma unroll
     for (int kb = 0; kb < C / 16; ++kb) {
         if (m0 + r < a.M) {
-            fa[kb] = load8(a.X + (m0 + r) * C + 16 * kb + 8 * h);
+            xb[kb] = load8(a.X + (m0 + r) * C + 16 * kb + 8 * h);
         } else {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                fa[kb][j] = (__bf16)0.f;
+                xb[kb][j] = (__bf16)0.f;
         }
     }
-    float rs[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-        rs[i] = 0.f;
+    float rs = 0.f;
 #pragma unroll
     for (int nt = 0; nt < FNO_H / 32; ++nt) {
         f32x16_t d;
@@ -378,25 +378,25 @@ HZL_HD void fno_head_wave(const Ctx &ctx, const FnoHeadArgs &a, long wave, int l
             d[i] = 0.f;
 #pragma unroll
         for (int kb = 0; kb < C / 16; ++kb)
-            fno_mfma(ctx, fa[kb], load8(a.Q1 + (32 * nt + r) * C + 16 * kb + 8 * h), d, lane);
-        const float bias = a.q1[32 * nt + r], w = a.Q2[32 * nt + r];
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-            rs[i] += w * gelu_as(d[i] + bias);
-    }
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-        for (int off = 1; off < 32; off *= 2)
-            rs[i] += ctx.shfl_xor(rs[i], off, lane);
-    if (r == 0) {
-        const float q2 = a.q2[0];
+            fno_mfma(ctx, load8(a.Q1 + (32 * nt + r) * C + 16 * kb + 8 * h), xb[kb], d, lane);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const long m = m0 + 8 * (i / 4) + 4 * h + i % 4;
-            if (m < a.M)
-                a.out[m] = rs[i] + q2;
+            const int j = 32 * nt + 8 * (i / 4) + 4 * h + i % 4;
+            rs += hq[FNO_H + j] * gelu_as(d[i] + hq[j]);
         }
     }
+    rs += ctx.shfl_xor(rs, 32, lane);
+    if (h == 0 && m0 + r < a.M)
+        a.out[m0 + r] = rs + a.q2[0];
+}
+
+// q1 and Q2 into the workgroup's LDS (2 FNO_H floats) for fno_head_wave.
+template <class Ctx> HZL_HD void fno_head_stage(const Ctx &ctx, const FnoHeadArgs &a, float *hq) {
+    for (int e = ctx.tid(); e < FNO_H; e += 256) {
+        hq[e] = a.q1[e];
+        hq[FNO_H + e] = a.Q2[e];
+    }
+    ctx.sync();
 }
 
 // Per-mode channel mixing U[s][mode][n] = sum_k V[s][mode][k] R[mode][n][k]

@@ -80,7 +80,11 @@ __global__ void __launch_bounds__(MIX_THREADS) k_fno_mix(FnoMixArgs a) {
     fno_mix_body(DevCtx{}, a, (long)blockIdx.x);
 }
 __global__ void __launch_bounds__(256) k_fno_head(FnoHeadArgs a) {
-    fno_head_wave(DevCtx{}, a, (long)blockIdx.x * 4 + threadIdx.x / 64, (int)(threadIdx.x % 64));
+    __shared__ float hq[2 * FNO_H];
+    fno_head_stage(DevCtx{}, a, hq);
+    // four wavefronts of 32 points; one past the last point computes zeros, stores nothing
+    fno_head_wave(DevCtx{}, a, (long)blockIdx.x * 4 + threadIdx.x / 64, (int)(threadIdx.x % 64),
+                  hq);
 }
 
 template <typename T> __global__ void k_gelu_back(const T *go, const T *pre, T *g, long n) {

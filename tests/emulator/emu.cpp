@@ -647,9 +647,16 @@ int emu_fno_head(long M, const float *X, const float *Q1, const float *q1, const
     for (size_t i = 0; i < q.size(); ++i)
         q[i] = (__bf16)Q1[i];
     FnoHeadArgs a{x.data(), q.data(), q1, Q2, q2, out, M};
-    launch((M + 31) / 32, 64, 0, [&](const CpuCtx &ctx, long wave, unsigned char *) {
-        fno_head_wave(ctx, a, wave, ctx.lane);
-    });
+    launch((M + 31) / 32, 64, 2 * FNO_H * sizeof(float),
+           [&](const CpuCtx &ctx, long wave, unsigned char *l) {
+               float *hq = reinterpret_cast<float *>(l);
+               for (int e = ctx.lane; e < FNO_H; e += 64) {
+                   hq[e] = a.q1[e];
+                   hq[FNO_H + e] = a.Q2[e];
+               }
+               ctx.sync();
+               fno_head_wave(ctx, a, wave, ctx.lane, hq);
+           });
     return 0;
 }
 
