@@ -188,13 +188,12 @@ def fp64_steps(pkg, n: int, B: int, steps: int, warmup: int, dt: float, nu: floa
     ms = timer.elapsed_ms(e0, e1)
     ens.synchronize()
     tuning = ens.tuning
-    chains = ens.chains
     ens.close()
     mb = model_bytes(n, 16)
     value = B * steps / (ms / 1e3)
     return {"metric": f"member RK4 steps/s @{n}^2 FP64", "value": round(value, 1),
             "unit": "member-steps/s", "ms_per_step": round(ms / steps, 4), "members": B,
-            "steps": steps, "tuning": list(tuning), "chains": chains,
+            "steps": steps, "tuning": list(tuning),
             "step_model_frac_of_hbm_peak": round(value * mb["member_step"] / HBM_PEAK, 4),
             "kernel_frac_of_hbm_peak": round(value * mb["kernel_member_step"] / HBM_PEAK, 4)}
 
@@ -853,25 +852,6 @@ def main() -> int:
     ens.synchronize()  # CFL / finiteness checks of every timed step
     ms_per_step = ms / args.steps
     value = B * args.steps / (ms / 1e3)
-    # the same steps with every launch over all members (one chain), for
-    # comparison with the tuned step schedule
-    chains = ens.chains
-    one_chain = None
-    if chains > 1:
-        ens.chains = 1
-        ens.step(dt, args.steps)
-        ens.step(dt, args.steps)
-        ens.synchronize()
-        ea, eb = timer.event(), timer.event()
-        timer.record(ea, stream)
-        ens.step(dt, args.steps)
-        timer.record(eb, stream)
-        one_ms = timer.elapsed_ms(ea, eb)
-        ens.synchronize()
-        ens.chains = chains
-        one_chain = {"member_steps_per_s": round(B * args.steps / (one_ms / 1e3), 1),
-                     "ms_per_step": round(one_ms / args.steps, 4)}
-
     # per-kernel durations (events around each launch, outside the timed region)
     prof = np.stack([ens.profile_step(dt) for _ in range(max(1, args.profile_steps))])
     ens.synchronize()
@@ -922,8 +902,7 @@ def main() -> int:
                        "kernel_frac_of_hbm_peak":
                            round(value * mb["kernel_member_step"] / HBM_PEAK, 4),
                        "wall_seconds_timed": round(wall, 4)},
-        "tuning": {"col_variant": cv, "row_variant": rv, "chains": chains,
-                   "one_chain": one_chain,
+        "tuning": {"col_variant": cv, "row_variant": rv,
                    "col_ms_per_step": [round(x, 4) for x in tune_col],
                    "row_ms_per_step": [round(x, 4) for x in tune_row]},
         "roofline": {"bound": "hbm", "kernel": label,

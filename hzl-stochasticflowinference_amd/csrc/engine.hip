@@ -424,9 +424,9 @@ std::mutex &tune_mutex() {
     return m;
 }
 
-// A remembered launch choice: column variant, row variant, step chains.
+// A remembered launch choice: column variant, row variant.
 struct TuneChoice {
-    int cv = 0, rv = 0, chains = 1;
+    int cv = 0, rv = 0;
 };
 
 std::map<std::string, TuneChoice> &tune_memo() {
@@ -476,8 +476,6 @@ bool tune_lookup(const std::string &key, TuneChoice *choice) {
             std::string k;
             TuneChoice t;
             if (ls >> k >> t.cv >> t.rv && k == key && t.cv >= 0 && t.rv >= 0) {
-                if (!(ls >> t.chains) || t.chains < 1)
-                    t.chains = 1; // lines written before the schedule was tuned
                 v = t;
                 found = true;
             }
@@ -501,7 +499,7 @@ void tune_store(const std::string &key, const TuneChoice &choice) {
         (void)::mkdir(path.substr(0, i).c_str(), 0755);
     std::ofstream out(path, std::ios::app);
     if (out)
-        out << key << " " << choice.cv << " " << choice.rv << " " << choice.chains << "\n";
+        out << key << " " << choice.cv << " " << choice.rv << "\n";
 }
 
 // ------------------------------------------------------------- engine ----
@@ -558,16 +556,6 @@ template <typename R> class Engine final : public EngineBase {
             clear_graphs();
             (void)hipStreamDestroy(stream_);
         }
-        for (int c = 1; c < kMaxChains; ++c) {
-            if (chain_stream_[c]) {
-                (void)hipStreamSynchronize(chain_stream_[c]);
-                (void)hipStreamDestroy(chain_stream_[c]);
-            }
-            if (join_ev_[c])
-                (void)hipEventDestroy(join_ev_[c]);
-        }
-        if (fork_ev_)
-            (void)hipEventDestroy(fork_ev_);
         if (obs_event_)
             (void)hipEventDestroy(obs_event_);
         if (obs_host_)
@@ -686,10 +674,6 @@ template <typename R> class Engine final : public EngineBase {
             set_counters(0, 0);
             const GraphKey key{1, dt, chunk, 0, 0, nullptr, false};
             run_captured(key, [&]() {
-                if (chained()) {
-                    launch_steps_chained(dt, chunk, ctr_.p, 0);
-                    return;
-                }
                 for (size_t s = 0; s < chunk; ++s)
                     launch_step(dt, nullptr, 0, ctr_.p, s, nullptr);
             });
@@ -707,34 +691,6 @@ template <typename R> class Engine final : public EngineBase {
         clear_graphs();
     }
 
-    // Step schedule: with C chains the members are split into C contiguous
-    // ranges, each a chain of (noise, 4 x (row, column)) launches on its own
-    // stream (the handle's stream and C - 1 more, forked from and joined to
-    // it), so one range's kernels fill the GPU while another's drain: the
-    // tail of every launch, when its last workgroups leave compute units idle,
-    // overlaps the next launch of another chain.  RK4 steps with the white
-    // forcing use the chains (hzl_ensemble_step, the forecasts of captured
-    // particle-filter and EnKF cycles); every member's arithmetic is the same
-    // as in one chain, so results are bit-identical.
-    void set_chains(int chains) override {
-        if (chains < 1 || chains > kMaxChains)
-            throw std::invalid_argument("chains must be between 1 and 4");
-        if ((size_t)chains > B_)
-            throw std::invalid_argument("more chains than members");
-        check(hipStreamSynchronize(stream_), "schedule switch");
-        clear_graphs(); // captured steps name the previous schedule
-        for (int c = 1; c < chains; ++c)
-            if (!chain_stream_[c]) {
-                check(hipStreamCreateWithFlags(&chain_stream_[c], hipStreamNonBlocking),
-                      "hipStreamCreate");
-                check(hipEventCreateWithFlags(&join_ev_[c], hipEventDisableTiming), "event");
-            }
-        if (!fork_ev_)
-            check(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming), "event");
-        chains_ = chains;
-    }
-
-    int get_chains() const override { return chains_; }
 
     // ------------------------------------- SSPRK3 and calibration drifts ----
     // Solver::step(dt, TimeIntegrator::ssprk3) (src/solver.cpp:235-246) for
@@ -1175,50 +1131,6 @@ template <typename R> class Engine final : public EngineBase {
         *rv = row_var_;
     }
 
-    // Whole-step milliseconds of the current launch shapes with 1, 2 and 4
-    // step chains (ms[i], 0 where the chains outnumber the members), measured
-    // as the engine runs them: three calls of 3 steps each, the third timed
-    // (with graphs on it replays the sequence the second captured).  The
-    // caller restores the states; the step counter is rewound here.  Returns
-    // the chain count to use: the fastest when it beats one chain by more
-    // than 2%, the smallest one within 1% of it.
-    int time_chains(double dt, double *ms) {
-        const int counts[3] = {1, 2, 4};
-        const int before = chains_;
-        constexpr size_t kSteps = 3;
-        hipEvent_t e0, e1;
-        check(hipEventCreate(&e0), "hipEventCreate");
-        check(hipEventCreate(&e1), "hipEventCreate");
-        for (int i = 0; i < 3; ++i) {
-            ms[i] = 0;
-            if ((size_t)counts[i] > B_)
-                continue;
-            set_chains(counts[i]);
-            for (int call = 0; call < 3; ++call) {
-                if (call == 2)
-                    check(hipEventRecord(e0, stream_), "event");
-                step(dt, kSteps);
-            }
-            check(hipEventRecord(e1, stream_), "event");
-            check(hipEventSynchronize(e1), "chain timing");
-            float t = 0;
-            check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
-            ms[i] = t / (double)kSteps;
-            rewind_counter(3 * kSteps);
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        set_chains(before);
-        int best = 0;
-        for (int i = 1; i < 3; ++i)
-            if (ms[i] > 0 && ms[i] < 0.98 * ms[0] && ms[i] < ms[best])
-                best = i;
-        for (int i = 1; i < best; ++i)
-            if (ms[i] > 0 && ms[i] < 0.98 * ms[0] && ms[i] <= 1.01 * ms[best])
-                return counts[i];
-        return counts[best];
-    }
-
     // Time every column variant (with the default row shape), keep the
     // fastest, then every row variant with it.  A variant replaces the
     // default only when it is more than 1% faster.  The states are restored
@@ -1227,13 +1139,13 @@ template <typename R> class Engine final : public EngineBase {
     // without timing (the outputs are then left unchanged).
     void tune(bool use_cache, double *col_ms, double *row_ms) override {
         const int nc = col_variants(), nr = row_variants();
+        if (nc == 1 && nr == 1)
+            return;
         const std::string key = tune_key(n_, sizeof(R) == 4 ? HZL_FP32 : HZL_FP64, B_);
         TuneChoice remembered;
         if (use_cache && tune_lookup(key, &remembered) && remembered.cv < nc &&
-            remembered.rv < nr && (size_t)remembered.chains <= B_ &&
-            remembered.chains <= kMaxChains) {
+            remembered.rv < nr) {
             set_tuning(remembered.cv, remembered.rv);
-            set_chains(remembered.chains);
             return;
         }
         flush_parameters();
@@ -1290,9 +1202,6 @@ template <typename R> class Engine final : public EngineBase {
         for (int r = 0; r < nr; ++r)
             rt[r] = measure(bc, r).second;
         const int br = pick(rt);
-        set_tuning(bc, br);
-        double chain_ms[3];
-        const int bch = time_chains(dt, chain_ms);
         check(hipMemcpyAsync(w0_.p, backup.p, B_ * spec_ * sizeof(cx<R>),
                              hipMemcpyDeviceToDevice, stream_),
               "tuning restore");
@@ -1302,8 +1211,7 @@ template <typename R> class Engine final : public EngineBase {
         if (mix_was_valid)
             ensure_mixed(1.0);
         set_tuning(bc, br);
-        set_chains(bch);
-        tune_store(key, TuneChoice{bc, br, bch});
+        tune_store(key, TuneChoice{bc, br});
         if (col_ms)
             std::copy(ct.begin(), ct.end(), col_ms);
         if (row_ms)
@@ -1519,11 +1427,8 @@ template <typename R> class Engine final : public EngineBase {
             set_counters(pf_group_, is_last ? 1 : 0);
             const GraphKey key{2, dt, steps, nobs, pf_filters_, obs_dev_.p, mean_field != nullptr};
             run_captured(key, [&]() {
-                if (chained())
-                    launch_steps_chained(dt, steps, ctr_.p, 0);
-                else
-                    for (size_t s = 0; s < steps; ++s)
-                        launch_step(dt, nullptr, 0, ctr_.p, s, nullptr);
+                for (size_t s = 0; s < steps; ++s)
+                    launch_step(dt, nullptr, 0, ctr_.p, s, nullptr);
                 launch_filter(nobs, true, is_last, mean_field != nullptr);
             });
             step_ += steps;
@@ -1681,12 +1586,8 @@ template <typename R> class Engine final : public EngineBase {
         set_counters(0, 0);
     }
 
-    void enqueue_steps(double dt, size_t steps, bool chains) override {
+    void enqueue_steps(double dt, size_t steps) override {
         run_col(0, stage_args(dt, nullptr, 0)); // the current state's mixed fields
-        if (chains && chained() && steps) {
-            launch_steps_chained(dt, steps, ctr_.p, 0);
-            return;
-        }
         for (size_t s = 0; s < steps; ++s)
             launch_step(dt, nullptr, 0, ctr_.p, s, nullptr);
     }
@@ -1780,8 +1681,8 @@ template <typename R> class Engine final : public EngineBase {
     }
 
     // Column / row launches through the selected launch-shape variant.
-    void run_col(int mode, const StageArgs<R> &a) { run_col(mode, a, stream_); }
-    void run_col(int mode, const StageArgs<R> &a, hipStream_t s) {
+    void run_col(int mode, const StageArgs<R> &a) {
+        const hipStream_t s = stream_;
         if (mode > 4 && col_var_ > 0) {
             // SSPRK3 stages and drift passes run in the default shape (the
             // mixed and nonlinear fields are shape independent)
@@ -1800,8 +1701,8 @@ template <typename R> class Engine final : public EngineBase {
         dispatch_col<R>(n_, mode, a, s);
     }
 
-    void run_row(const RowArgs<R> &a) { run_row(a, stream_); }
-    void run_row(const RowArgs<R> &a, hipStream_t s) {
+    void run_row(const RowArgs<R> &a) {
+        const hipStream_t s = stream_;
         if constexpr (std::is_same_v<R, float>) {
             if (row_var_ > 0) {
                 if (!launch_row_variant(n_, row_var_, a, s))
@@ -2050,70 +1951,6 @@ template <typename R> class Engine final : public EngineBase {
         check(hipGetLastError(), "RK4 kernels");
     }
 
-    bool chained() const { return chains_ > 1 && !ou_; }
-
-    // Launch j of a member range's step chain: 0 draws the step's shell noise,
-    // 2s - 1 is the row pass and 2s the column pass of RK4 stage s (1..4).
-    void launch_piece(size_t first, size_t count, hipStream_t s, double dt, int j,
-                      const uint64_t *step_base, uint64_t step) {
-        if (j == 0) {
-            NoiseArgs<R> na{(int)count,        npairs_, d_sigma_.p + first, d_keys_.p + first, dt,
-                            step,              noise_.p + first * npairs_, step_base};
-            const long n = (long)count * npairs_;
-            hipLaunchKernelGGL(k_noise<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, na);
-            return;
-        }
-        const int stage = (j + 1) / 2;
-        if (j % 2) {
-            RowArgs<R> r{(int)count, mix_.p + first * 4 * mixf_, nl_.p + first * mixf_, tw_row_,
-                         stage == 1 ? cfl_.p + first * (size_t)(n_ / 2) : nullptr};
-            run_row(r, s);
-            return;
-        }
-        StageArgs<R> a = stage_args(dt, nullptr, 0);
-        a.B = (int)count;
-        a.w0 += first * spec_;
-        a.w1 += first * spec_;
-        a.acc += first * spec_;
-        a.nl += first * mixf_;
-        a.mix += first * 4 * mixf_;
-        a.nu += first;
-        a.alpha += first;
-        a.forcing.noise += first * npairs_;
-        a.forcing.amp += first;
-        run_col(stage, a, s);
-    }
-
-    // `count` RK4 steps of the white-forced ensemble in chains_ chains (see
-    // set_chains).  The chain streams fork from and join the handle's stream,
-    // so the sequence is captured as one graph.  Noise step indices are
-    // step_off + s (plus *step_base when given), as launch_step's.  Launches
-    // are issued round-robin over the chains, piece by piece.
-    void launch_steps_chained(double dt, size_t count, const uint64_t *step_base,
-                              uint64_t step_off) {
-        const int C = chains_;
-        hipStream_t st[kMaxChains];
-        size_t first[kMaxChains + 1];
-        for (int c = 0; c <= C; ++c)
-            first[c] = B_ * (size_t)c / (size_t)C;
-        st[0] = stream_;
-        check(hipEventRecord(fork_ev_, stream_), "fork");
-        for (int c = 1; c < C; ++c) {
-            st[c] = chain_stream_[c];
-            check(hipStreamWaitEvent(st[c], fork_ev_, 0), "fork");
-        }
-        constexpr long PER = 9; // launches per step and chain
-        for (long k = 0; k < PER * (long)count; ++k)
-            for (int c = 0; c < C; ++c)
-                launch_piece(first[c], first[c + 1] - first[c], st[c], dt, (int)(k % PER),
-                             step_base, step_off + (uint64_t)(k / PER));
-        for (int c = 1; c < C; ++c) {
-            check(hipEventRecord(join_ev_[c], st[c]), "join");
-            check(hipStreamWaitEvent(stream_, join_ev_[c], 0), "join");
-        }
-        check(hipGetLastError(), "RK4 kernels");
-    }
-
     // One RK4 step launched eagerly.
     void advance(double dt, const cx<R> *ext, long long ext_stride,
                  hipEvent_t *events = nullptr) {
@@ -2122,10 +1959,7 @@ template <typename R> class Engine final : public EngineBase {
             run_col(0, stage_args(dt, ext, ext_stride));
             mix_valid_ = true;
         }
-        if (!ext && !events && chained())
-            launch_steps_chained(dt, 1, nullptr, step_);
-        else
-            launch_step(dt, ext, ext_stride, nullptr, step_, events);
+        launch_step(dt, ext, ext_stride, nullptr, step_, events);
         ++step_;
         last_dt_ = dt;
         max_dt_ = std::max(max_dt_, dt);
@@ -2394,11 +2228,6 @@ template <typename R> class Engine final : public EngineBase {
     hzl_parameters p_;
     size_t spec_ = 0, mixf_ = 0;
     hipStream_t stream_ = nullptr;
-    // step schedule (set_chains): chain c > 0 runs on chain_stream_[c]
-    static constexpr int kMaxChains = 4;
-    int chains_ = 1;
-    hipStream_t chain_stream_[kMaxChains] = {};
-    hipEvent_t fork_ev_ = nullptr, join_ev_[kMaxChains] = {};
     DeviceBuffer<cx<R>> w0_, w1_, acc_, mix_, nl_, ext_, tw_, noise_;
     // particle filter state
     bool pf_on_ = false;
@@ -2719,18 +2548,6 @@ int hzl_likelihood_adjoint(hzl_ensemble_t h, const double *initial_field, double
 
 int hzl_ensemble_set_graph(hzl_ensemble_t h, int enabled) {
     return guarded([&] { engine_of(h).set_graph(enabled != 0); });
-}
-
-int hzl_ensemble_set_chains(hzl_ensemble_t h, int chains) {
-    return guarded([&] { engine_of(h).set_chains(chains); });
-}
-
-int hzl_ensemble_get_chains(hzl_ensemble_t h, int *chains) {
-    return guarded([&] {
-        if (!chains)
-            throw std::invalid_argument("null output");
-        *chains = engine_of(h).get_chains();
-    });
 }
 
 int hzl_ensemble_step_forced(hzl_ensemble_t h, double dt, const double *forcing, int shared) {

@@ -83,9 +83,6 @@ class EngineBase {
     virtual void init_grf() = 0;
     virtual void step(double dt, size_t steps) = 0;
     virtual void set_graph(bool on) = 0;
-    // step schedule: member ranges stepped as chains on their own streams
-    virtual void set_chains(int chains) = 0;
-    virtual int get_chains() const = 0;
     // SSPRK3 stepping and the calibration drifts (batched over members)
     virtual void step_integrator(double dt, size_t steps, int integrator) = 0;
     // Forcing process of the shell noise: HZL_FORCING_WHITE (the engine's
@@ -159,10 +156,7 @@ class EngineBase {
     // device counter, so a replay continues the stream; commit_steps(dt, steps)
     // then advances the host's step index.
     virtual void prepare_steps(double dt) = 0;
-    // chains: run the steps in the handle's step chains (set_chains) when it has
-    // several; the joint mixed-grid filter, whose capture already forks every
-    // ensemble's stream from its own, passes false (one chain per ensemble)
-    virtual void enqueue_steps(double dt, size_t steps, bool chains) = 0;
+    virtual void enqueue_steps(double dt, size_t steps) = 0;
     virtual void commit_steps(double dt, size_t steps) = 0;
     // Captured launch sequences stay valid while this value is unchanged (it
     // moves whenever kernels, launch shapes, buffers or the forcing process

@@ -797,7 +797,7 @@ template <typename R> struct Enkf {
     void launches(size_t nobs, double inflation, double radius, double dt, size_t steps) {
         const int B = ev.B;
         if (steps)
-            eng.enqueue_steps(dt, steps, true);
+            eng.enqueue_steps(dt, steps);
         const long tab = (long)nobs * (K + 1 + J);
         hipLaunchKernelGGL(k_enkf_phase, grid1d(tab), dim3(256), 0, ev.stream, x_d(nobs),
                            y_d(nobs), (int)nobs, K, ev.n, st.cs.p, st.sn.p, st.ex.p, st.ey.p);

@@ -315,7 +315,7 @@ class MultiFilter {
             EngineBase *e = engs_[k];
             const hipStream_t s = views_[k].stream;
             hcheck(hipStreamWaitEvent(s, fork_, 0), "event wait");
-            e->enqueue_steps(dt, steps, false);
+            e->enqueue_steps(dt, steps);
             e->pf_enqueue_logweights(nobs, wstate_.p + first_[k], logw_.p + first_[k]);
             hcheck(hipEventRecord(join_[k], s), "event");
             hcheck(hipStreamWaitEvent(st_, join_[k], 0), "event wait");

@@ -26,7 +26,6 @@ EXPORTS = (
     "hzl_last_error", "hzl_ensemble_create", "hzl_ensemble_destroy",
     "hzl_ensemble_set_member_parameters", "hzl_ensemble_set_state", "hzl_ensemble_get_state",
     "hzl_ensemble_init_grf", "hzl_ensemble_step", "hzl_ensemble_set_graph",
-    "hzl_ensemble_set_chains", "hzl_ensemble_get_chains",
     "hzl_ensemble_step_integrator", "hzl_ensemble_set_forcing_process",
     "hzl_ensemble_get_forcing", "hzl_ensemble_reseed_future", "hzl_ensemble_set_forcing_state",
     "hzl_ensemble_get_forcing_state", "hzl_ensemble_drift", "hzl_ensemble_linearized_drift",
@@ -133,8 +132,6 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_ensemble_variant_count.argtypes = [vp, ipp, ipp]
     L.hzl_ensemble_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.hzl_ensemble_get_tuning.argtypes = [vp, ipp, ipp]
-    L.hzl_ensemble_set_chains.argtypes = [vp, C.c_int]
-    L.hzl_ensemble_get_chains.argtypes = [vp, ipp]
     L.hzl_ensemble_autotune.argtypes = [vp, C.c_int, _dp, _dp]
     L.hzl_ensemble_step_forced.argtypes = [vp, C.c_double, _dp, C.c_int]
     L.hzl_ensemble_synchronize.argtypes = [vp]
@@ -386,18 +383,6 @@ class Ensemble:
     @tuning.setter
     def tuning(self, cr):
         _check(_lib.hzl_ensemble_set_tuning(self._h, int(cr[0]), int(cr[1])))
-
-    @property
-    def chains(self) -> int:
-        """Step schedule: the members are stepped as this many chains of
-        launches on their own streams (hzl_ensemble_set_chains)."""
-        c = C.c_int()
-        _check(_lib.hzl_ensemble_get_chains(self._h, C.byref(c)))
-        return c.value
-
-    @chains.setter
-    def chains(self, c: int):
-        _check(_lib.hzl_ensemble_set_chains(self._h, int(c)))
 
     def autotune(self, use_cache: bool = False):
         """Time every launch-shape variant now and keep the fastest; returns

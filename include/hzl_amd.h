@@ -250,21 +250,6 @@ int hzl_likelihood_adjoint(hzl_ensemble_t h, const double *initial_field, double
  * (src/cuda_solver.cu:172-218). */
 int hzl_ensemble_set_graph(hzl_ensemble_t h, int enabled);
 
-/* Step schedule of hzl_ensemble_step: chains = 1..4 (at most the member
- * count).  With C chains the members are split into C contiguous ranges,
- * each stepped as its own chain of (noise, 4 x (row, column)) launches on
- * its own stream (the handle's stream and C - 1 more that it creates, forked
- * from and joined to its own, so captured steps stay one graph): while one
- * range's launch drains its last workgroups, another range's launch fills
- * the idle compute units.  Members' arithmetic is unchanged, so results are
- * bit-identical to one chain.  The forecasts of captured particle-filter
- * and EnKF cycles use the chains too; the joint mixed-grid filter, SSPRK3,
- * host forcing and the OU forcing process keep one chain.  hzl_ensemble_autotune (and the
- * automatic tuning at creation) times 1, 2 and 4 chains with the chosen
- * launch shapes and keeps the fastest.  No counterpart in the reference. */
-int hzl_ensemble_set_chains(hzl_ensemble_t h, int chains);
-int hzl_ensemble_get_chains(hzl_ensemble_t h, int *chains);
-
 /* Launch-shape autotuner.  FP32 ensembles at 128^2, 256^2 and 512^2 can run
  * their column and row kernels in several compiled shapes (values per lane,
  * i.e. the T x E split of each transform; workgroup size; register budget);

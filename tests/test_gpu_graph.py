@@ -4,9 +4,7 @@ Each ensemble runs twice from the same seed: graphs on (steps and filter
 cycles are recorded on the second call with a key and replayed afterwards,
 reading the step index, filter group and last-group flag from device memory)
 and graphs off.  Every replay is compared bit for bit with the eager run,
-with host work and synchronisations between the replays.  The replaying
-ensemble also runs its steps as several chains (member ranges on their own
-streams, hzl_ensemble_set_chains), which must not change any bit.
+with host work and synchronisations between the replays.
 """
 from __future__ import annotations
 
@@ -16,23 +14,20 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def pair(pkg, n, members, precision, seed=11, chains=1, **kw):
+def pair(pkg, n, members, precision, seed=11, **kw):
     args = dict(nu=1e-3, sigma=0.05, amp=0.1, kf=4, seed=seed)
     args.update(kw)
     a = pkg.Ensemble(n, members, precision, **args)
     b = pkg.Ensemble(n, members, precision, **args)
-    a.chains = chains
-    b.chains = 1
     b.set_graph(False)
     a.init_grf()
     b.init_grf()
     return a, b
 
 
-@pytest.mark.parametrize("chains", [1, 2])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_step_replays_match_eager(pkg, precision, chains):
-    a, b = pair(pkg, 64, 6, precision, chains=chains)
+def test_step_replays_match_eager(pkg, precision):
+    a, b = pair(pkg, 64, 6, precision)
     with a, b:
         # 5-step calls: eager, capture, replays; a second key (7) in between,
         # and 450 steps = two replays of a 200-step graph + 50 eager steps
@@ -48,9 +43,8 @@ def test_step_replays_match_eager(pkg, precision, chains):
         assert np.array_equal(da, db)
 
 
-@pytest.mark.parametrize("chains", [1, 3])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_filter_cycle_replays_match_eager(pkg, precision, chains):
+def test_filter_cycle_replays_match_eager(pkg, precision):
     n, P, dt, every, groups = 32, 32, 2e-3, 6, 9
     rng = np.random.default_rng(5)
     rows = []
@@ -61,7 +55,7 @@ def test_filter_cycle_replays_match_eager(pkg, precision, chains):
         vals = rng.normal(0.0, 0.3, 16)
         rows += [((g + 1) * every, x, y, v, sd) for x, y, v in zip(xs, ys, vals)]
     rows = np.array(rows, dtype=np.float64)
-    a, b = pair(pkg, n, P, precision, seed=21, chains=chains, nu=2e-3, sigma=0.2)
+    a, b = pair(pkg, n, P, precision, seed=21, nu=2e-3, sigma=0.2)
     with a, b:
         ra = pkg.bootstrap_filter(a, rows, dt, seed=4)
         rb = pkg.bootstrap_filter(b, rows, dt, seed=4)
@@ -79,9 +73,8 @@ def test_filter_cycle_replays_match_eager(pkg, precision, chains):
     assert np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("chains", [1, 2])
 @pytest.mark.parametrize("case", ["serial-ou-localized", "serial-white", "batch"])
-def test_enkf_cycle_replays_match_eager(pkg, case, chains):
+def test_enkf_cycle_replays_match_eager(pkg, case):
     """A whole EnKF cycle (forecast steps, inflation, forecast RMSE, device
     perturbations, analysis, mean and spread) replays one captured graph from
     the third group on; records, mean fields and states equal the launch-by-
@@ -95,7 +88,7 @@ def test_enkf_cycle_replays_match_eager(pkg, case, chains):
                  zip(xs, ys, rng.normal(0.0, 0.3, 12))]
     rows = np.array(rows, dtype=np.float64)
     precision = "fp32" if case == "batch" else "fp64"
-    a, b = pair(pkg, n, M, precision, seed=31, chains=chains, nu=2e-3, sigma=0.2, tau=0.4)
+    a, b = pair(pkg, n, M, precision, seed=31, nu=2e-3, sigma=0.2, tau=0.4)
     mode = "batch" if case == "batch" else "serial"
     inflation, radius = (1.05, 4.0) if case == "serial-ou-localized" else (1.0, 0.0)
     with a, b:
@@ -123,15 +116,8 @@ def test_enkf_cycle_replays_match_eager(pkg, case, chains):
         assert np.array_equal(a.get_state(), b.get_state())
 
 
-# The joint filter forecasts every ensemble with one chain (its captured cycle
-# already forks each ensemble's stream); a run that let the ensembles fork
-# their step chains inside that capture ended in a host-side segmentation fault
-# whose cause was not found, so that path is switched off and not run.
-@pytest.mark.parametrize("chains", [
-    1, pytest.param(2, marks=pytest.mark.skip(
-        reason="step chains nested in the joint filter's capture are switched off"))])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_joint_filter_cycle_replays_match_eager(pkg, precision, chains):
+def test_joint_filter_cycle_replays_match_eager(pkg, precision):
     """The mixed-grid filter's cycle (fork to both grids' streams for steps and
     log-weights, join for normalisation and the cross-grid gather) replays one
     captured graph; every group's record and the final states equal the run
@@ -151,7 +137,6 @@ def test_joint_filter_cycle_replays_match_eager(pkg, precision, chains):
             b.init_grf()
             a.set_graph(graphs)
             b.set_graph(graphs)
-            a.chains = b.chains = chains if graphs else 1
             res = pkg.joint_bootstrap_filter([a, b], rows, 2e-3, 13)
             runs.append((res, a.get_state(), b.get_state(), a.step_index, b.step_index))
     (ra, a1, b1, ia, ib), (rb, a2, b2, ja, jb) = runs

@@ -55,14 +55,12 @@ def test_autotune_times_variants_and_remembers(pkg, tmp_path, monkeypatch):
         col, row = ens.autotune()
         after = ens.get_state()
         cv, rv = ens.tuning
-        chains = ens.chains
-        print("col ms", col, "row ms", row, "choice", cv, rv, "chains", chains)
+        print("col ms", col, "row ms", row, "choice", cv, rv)
         assert np.all(col > 0) and np.all(row > 0)
         assert 0 <= cv < len(col) and 0 <= rv < len(row)
         assert np.array_equal(before, after)  # states restored
     lines = cache.read_text().split("\n")
-    assert any(line.endswith(f" {cv} {rv} {chains}") and "/n256/fp32/b64" in line
-               for line in lines)
+    assert any(line.endswith(f" {cv} {rv}") and "/n256/fp32/b64" in line for line in lines)
     with pkg.Ensemble(256, 64, "fp32", seed=3) as again:
         again.autotune(use_cache=True)
-        assert again.tuning == (cv, rv) and again.chains == chains
+        assert again.tuning == (cv, rv)
