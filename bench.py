@@ -331,7 +331,7 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
         # the surrogate), so its solver steps replay their captured graph after
         # the warm run
         judge = None
-        if kind == "surrogate_judged":
+        if kind.startswith("surrogate_judged"):
             judge = pkg.Ensemble(n, judge_members, "fp32", nu=nu, sigma=0.0, amp=amp,
                                  seed=SEED + 5)
 
@@ -359,7 +359,8 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
                     res = pkg.surrogate_filter(pf, sur, rows, dt, seed, "bf16",
                                                retain_states=retain,
                                                fallback_threshold=threshold, judge=judge,
-                                               judge_every=judge_every)
+                                               judge_every=judge_every,
+                                               judge_async=kind == "surrogate_judged_async")
                 pf.synchronize()
                 sec = time.perf_counter() - t
             return res, sec
@@ -382,13 +383,15 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
              "ess_per_cycle": [round(x, 1) for x in ess],
              "resampling_events": res["resampling_events"],
              "log_likelihood": round(res["log_likelihood"], 3)}
-        if kind == "surrogate_judged":
+        if kind.startswith("surrogate_judged"):
             o["solver_cycles"] = int(sum(h["solver"] for h in res["history"]))
             o["error_estimates"] = sorted({round(h["error_estimate"], 6)
                                            for h in res["history"]})
             o["spread_mean"] = round(float(np.mean([h["spread"] for h in res["history"]])), 5)
             o["threshold"] = threshold
             o["judge"] = f"{judge_members} particles every {judge_every} cycles"
+            if kind == "surrogate_judged_async":
+                o["judge"] += " (enqueued after the first, beside the cycles)"
         out[kind] = o
     return out
 
@@ -452,7 +455,8 @@ def surrogate_pf(pkg, cycles: int = 20, train_windows: int = 20, epochs: int = 4
     solver pairs (log-uniform nu in [5e-4, 2e-3]).  Variants: plain solver,
     surrogate, and surrogate with the solver as live judge (16 particles
     re-forecast every 10 cycles) and fallback when the judged error exceeds 0.1
-    x the ensemble spread.  Analysis RMSEs are means over 3 filter seeds.  A
+    x the ensemble spread, with every judge waited for or with the judges after
+    the first enqueued beside the cycles.  Analysis RMSEs are means over 3 filter seeds.  A
     second twin at Kolmogorov amplitude 2.0 (20x the training amplitude) shows
     the judge detecting the network's error and the fallback forecasting with
     the solver.  cycles/s counts the whole filter loop."""
@@ -471,10 +475,13 @@ def surrogate_pf(pkg, cycles: int = 20, train_windows: int = 20, epochs: int = 4
     for particles in (256, 1024):
         init = low_mode_particles(pkg, truth0, particles)
         r = pf_variants(pkg, sur, rows, truths, init, particles, dt, nu, amp, sigma,
-                        ("free", "plain", "surrogate", "surrogate_judged"))
+                        ("free", "plain", "surrogate", "surrogate_judged",
+                         "surrogate_judged_async"))
         r["speedup"] = round(r["surrogate"]["cycles_per_s"] / r["plain"]["cycles_per_s"], 2)
         r["speedup_judged"] = round(r["surrogate_judged"]["cycles_per_s"] /
                                     r["plain"]["cycles_per_s"], 2)
+        r["speedup_judged_async"] = round(r["surrogate_judged_async"]["cycles_per_s"] /
+                                          r["plain"]["cycles_per_s"], 2)
         r["rmse_ratio"] = round(r["surrogate"]["analysis_rmse"] / r["plain"]["analysis_rmse"], 4)
         out[f"p{particles}"] = r
     # outside the training data: the judge sees the network's error and the
@@ -484,10 +491,13 @@ def surrogate_pf(pkg, cycles: int = 20, train_windows: int = 20, epochs: int = 4
                                    SEED + 31)
     init2 = low_mode_particles(pkg, truth02, 256)
     r = pf_variants(pkg, sur, rows2, truths2, init2, 256, dt, nu, amp_ood, sigma,
-                    ("plain", "surrogate", "surrogate_judged"), seeds=1)
+                    ("plain", "surrogate", "surrogate_judged", "surrogate_judged_async"),
+                    seeds=1)
     r["rmse_ratio"] = round(r["surrogate"]["analysis_rmse"] / r["plain"]["analysis_rmse"], 4)
     r["rmse_ratio_judged"] = round(r["surrogate_judged"]["analysis_rmse"] /
                                    r["plain"]["analysis_rmse"], 4)
+    r["rmse_ratio_judged_async"] = round(r["surrogate_judged_async"]["analysis_rmse"] /
+                                         r["plain"]["analysis_rmse"], 4)
     out["amp_2.0_outside_training"] = r
     sur.close()
     return out

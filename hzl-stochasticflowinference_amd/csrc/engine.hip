@@ -610,6 +610,15 @@ template <typename R> class Engine final : public EngineBase {
         params_dirty_ = true;
     }
 
+    void get_member_parameters(size_t m, double *nu, double *alpha, double *sigma,
+                               double *amp) const override {
+        member_check(m);
+        *nu = nu_[m];
+        *alpha = alpha_[m];
+        *sigma = sigma_[m];
+        *amp = amp_[m];
+    }
+
     void set_member_tau(size_t m, double tau) override {
         member_check(m);
         if (!std::isfinite(tau) || tau <= 0)

@@ -76,6 +76,10 @@ class EngineBase {
     std::unique_ptr<ModuleState> enkf; // ensemble Kalman filter state, if started
     virtual void set_member_parameters(size_t m, double nu, double alpha, double sigma,
                                        double amp) = 0;
+    // The host copies of a member's parameters (the values the device holds
+    // once the next launch has uploaded them).
+    virtual void get_member_parameters(size_t m, double *nu, double *alpha, double *sigma,
+                                       double *amp) const = 0;
     // Per-member OU time constant (the creation parameters' forcing_tau until set).
     virtual void set_member_tau(size_t m, double tau) = 0;
     virtual void set_state(size_t m, const double *field) = 0;

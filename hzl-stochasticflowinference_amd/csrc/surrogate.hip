@@ -422,6 +422,13 @@ class Surrogate {
         }
         if (order_ev_)
             (void)hipEventDestroy(order_ev_);
+        if (judge_pending_)
+            (void)hipEventSynchronize(judge_done_ev_);
+        for (hipEvent_t e : {judge_in_ev_, judge_copied_ev_, judge_done_ev_})
+            if (e)
+                (void)hipEventDestroy(e);
+        if (judge_host_)
+            (void)hipHostFree(judge_host_);
     }
     Surrogate(const Surrogate &) = delete;
     Surrogate &operator=(const Surrogate &) = delete;
@@ -651,26 +658,68 @@ class Surrogate {
     // network and, from the same states, by the plain solver; the grid RMS
     // difference per member becomes the error estimate.  `eng` is unchanged.
     double judge(EngineBase &eng, EngineBase &jg, double dt, size_t steps, int precision) {
-        const EngineViews ev = check_engine(eng), jv = check_engine(jg);
-        if (jv.B > ev.B)
+        judge_start(eng, jg, dt, steps, precision);
+        judge_finish(true);
+        return error_rms_;
+    }
+
+    // The same judge, enqueued: the judge ensemble's stream waits for the
+    // ensemble's pending work, copies the judged states, and the ensemble's
+    // stream waits only for that copy; the network forecast of the copies is
+    // ordered with the ensemble's forecasts (shared activations), the solver
+    // run and the comparison overlap whatever the ensemble does next.
+    // judge_finish() folds the result into the error estimate.  One judge is
+    // in flight at a time: a start first finishes the previous one.
+    void judge_start(EngineBase &eng, EngineBase &jg, double dt, size_t steps, int precision) {
+        judge_finish(true);
+        const EngineViews ev = check_engine(eng);
+        const size_t J = jg.members();
+        if (J > (size_t)ev.B)
             throw std::invalid_argument("the judge ensemble has more members than the ensemble");
         if (jg.forcing_process() != HZL_FORCING_WHITE)
             throw std::invalid_argument("the judge ensemble runs the white forcing process");
-        const size_t J = (size_t)jv.B, spec = (size_t)s_.spec();
+        if (&jg == &eng)
+            throw std::invalid_argument("the judge ensemble must differ from the ensemble");
+        check_precision(precision);
+        if (horizon_ && (steps != horizon_ || std::fabs(dt - dt_) > 1e-12 * dt_))
+            throw std::invalid_argument("forecast horizon differs from the trained horizon");
+        // parameters from the ensemble's host copies (what views() uploaded);
+        // unchanged members are not marked for another upload
+        for (size_t m = 0; m < J; ++m) {
+            double nu, alpha, sigma, amp, jn, ja, js, jamp;
+            eng.get_member_parameters(m, &nu, &alpha, &sigma, &amp);
+            jg.get_member_parameters(m, &jn, &ja, &js, &jamp);
+            if (nu != jn || alpha != ja || js != 0.0 || amp != jamp)
+                jg.set_member_parameters(m, nu, alpha, 0.0, amp);
+        }
+        const EngineViews jv = check_engine(jg); // uploads changed parameters
+        const size_t spec = (size_t)s_.spec();
         const size_t bytes = J * spec * sizeof(cx<float>);
-        hcheck(hipStreamSynchronize(ev.stream), "judge");
-        std::vector<double> nu(J), alpha(J), amp(J);
-        hcheck(hipMemcpy(nu.data(), ev.nu, J * sizeof(double), hipMemcpyDeviceToHost), "nu");
-        hcheck(hipMemcpy(alpha.data(), ev.alpha, J * sizeof(double), hipMemcpyDeviceToHost),
-               "alpha");
-        hcheck(hipMemcpy(amp.data(), ev.amp, J * sizeof(double), hipMemcpyDeviceToHost), "amp");
-        for (size_t m = 0; m < J; ++m)
-            jg.set_member_parameters(m, nu[m], alpha[m], 0.0, amp[m]);
         judge_x_.alloc(J * spec);
         judge_y_.alloc(J * spec);
-        spread_.alloc(spec);
-        hcheck(hipMemcpy(judge_x_.p, ev.w0, bytes, hipMemcpyDeviceToDevice), "judge states");
-        hcheck(hipMemcpy(jv.w0, judge_x_.p, bytes, hipMemcpyDeviceToDevice), "judge states");
+        if (judge_host_n_ < spec) {
+            if (judge_host_)
+                hcheck(hipHostFree(judge_host_), "judge sums");
+            judge_host_ = judge_host_dev_ = nullptr;
+            judge_host_n_ = 0;
+            hcheck(hipHostMalloc((void **)&judge_host_, spec * sizeof(double),
+                                 hipHostMallocMapped | hipHostMallocCoherent),
+                   "judge sums");
+            judge_host_n_ = spec;
+            hcheck(hipHostGetDevicePointer((void **)&judge_host_dev_, judge_host_, 0),
+                   "judge sums");
+        }
+        for (hipEvent_t *e : {&judge_in_ev_, &judge_copied_ev_, &judge_done_ev_})
+            if (!*e)
+                hcheck(hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
+        hcheck(hipEventRecord(judge_in_ev_, ev.stream), "judge order");
+        hcheck(hipStreamWaitEvent(jv.stream, judge_in_ev_, 0), "judge order");
+        hcheck(hipMemcpyAsync(judge_x_.p, ev.w0, bytes, hipMemcpyDeviceToDevice, jv.stream),
+               "judge states");
+        hcheck(hipEventRecord(judge_copied_ev_, jv.stream), "judge order");
+        hcheck(hipStreamWaitEvent(ev.stream, judge_copied_ev_, 0), "judge order");
+        hcheck(hipMemcpyAsync(jv.w0, judge_x_.p, bytes, hipMemcpyDeviceToDevice, jv.stream),
+               "judge states");
         jg.invalidate_mixed();
         forecast(jg, dt, steps, precision); // sigma 0: the network's forecast alone
         hcheck(hipMemcpyAsync(judge_y_.p, jv.w0, bytes, hipMemcpyDeviceToDevice, jv.stream),
@@ -680,19 +729,42 @@ class Surrogate {
         jg.rewind_counter(steps);
         jg.invalidate_mixed();
         jg.step(dt, steps);
+        // the per-mode sums go straight to mapped host memory: a download
+        // queued here behind the solver run would hold the device's other
+        // host transfers (the ensemble's spread and statistics) until it ran
         hipLaunchKernelGGL(k_sqdiff_modes, grid1d((long)spec), dim3(256), 0, jv.stream,
                            reinterpret_cast<const cx<float> *>(jv.w0), judge_y_.p, (int)J, s_.K,
-                           spread_.p);
-        std::vector<double> h(spec);
-        hcheck(hipMemcpyAsync(h.data(), spread_.p, spec * sizeof(double), hipMemcpyDeviceToHost,
-                              jv.stream),
-               "judge download");
-        jg.synchronize();
+                           judge_host_dev_);
+        hcheck(hipGetLastError(), "judge difference");
+        hcheck(hipEventRecord(judge_done_ev_, jv.stream), "judge order");
+        judge_members_ = J;
+        judge_modes_ = spec;
+        judge_pending_ = true;
+    }
+
+    // Fold a judge in flight into the error estimate: returns 1 when it did,
+    // 0 when none is in flight or (wait false) it has not finished yet.
+    int judge_finish(bool wait) {
+        if (!judge_pending_)
+            return 0;
+        if (wait) {
+            hcheck(hipEventSynchronize(judge_done_ev_), "judge");
+        } else {
+            const hipError_t e = hipEventQuery(judge_done_ev_);
+            if (e == hipErrorNotReady)
+                return 0;
+            hcheck(e, "judge");
+        }
+        judge_pending_ = false;
         double sum = 0;
-        for (double v : h)
-            sum += v;
-        error_rms_ = std::sqrt(sum / (double)J);
-        return error_rms_;
+        for (size_t i = 0; i < judge_modes_; ++i)
+            sum += judge_host_[i];
+        error_rms_ = std::sqrt(sum / (double)judge_members_);
+        return 1;
+    }
+
+    bool judge_in_flight() const {
+        return judge_pending_;
     }
 
     // Forecast with the network, then compare its error estimate with the
@@ -1537,6 +1609,12 @@ class Surrogate {
     std::vector<CachedForecast> graphs_;
     hipEvent_t order_ev_ = nullptr; // end of the last forecast, on order_stream_
     hipStream_t order_stream_ = nullptr;
+    // judge in flight: its events, per-mode sums in mapped host memory (host
+    // and device addresses), member and mode counts
+    hipEvent_t judge_in_ev_ = nullptr, judge_copied_ev_ = nullptr, judge_done_ev_ = nullptr;
+    double *judge_host_ = nullptr, *judge_host_dev_ = nullptr;
+    size_t judge_host_n_ = 0, judge_members_ = 0, judge_modes_ = 0;
+    bool judge_pending_ = false;
     std::vector<GraphKey> seen_;
     uint64_t graph_clock_ = 0;
     Buf<int> bidx_;
@@ -1745,6 +1823,27 @@ int hzl_surrogate_judge(hzl_surrogate_t h, hzl_ensemble_t ensemble, hzl_ensemble
         const double v = net_of(h).judge(e, j, dt, steps, precision);
         if (rms)
             *rms = v;
+    });
+}
+
+int hzl_surrogate_judge_start(hzl_surrogate_t h, hzl_ensemble_t ensemble, hzl_ensemble_t judge,
+                              double dt, size_t steps, int precision) {
+    return guarded([&] {
+        hzl_amd::EngineBase &e = hzl_amd::engine_of(ensemble);
+        hzl_amd::EngineBase &j = hzl_amd::engine_of(judge);
+        require_white(e);
+        net_of(h).judge_start(e, j, dt, steps, precision);
+    });
+}
+
+int hzl_surrogate_judge_poll(hzl_surrogate_t h, int wait, int *finished, double *rms) {
+    return guarded([&] {
+        hzl_amd::Surrogate &s = net_of(h);
+        const int f = s.judge_finish(wait != 0);
+        if (finished)
+            *finished = f;
+        if (rms)
+            *rms = s.error_estimate();
     });
 }
 

@@ -55,6 +55,8 @@ EXPORTS = (
     "hzl_surrogate_forecast_or_solve", "hzl_surrogate_set_graph", "hzl_surrogate_set_fused",
     "hzl_surrogate_forecast",
     "hzl_surrogate_judge",
+    "hzl_surrogate_judge_start",
+    "hzl_surrogate_judge_poll",
 )
 
 
@@ -182,6 +184,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.hzl_surrogate_forward.argtypes = [vp, C.c_int, sz, _dp, _dp, _dp, _dp]
     L.hzl_surrogate_loss_grad.argtypes = [vp, sz, _dp, _dp, _dp, _dp, _dp, fp]
     L.hzl_surrogate_judge.argtypes = [vp, vp, vp, C.c_double, sz, C.c_int, _dp]
+    L.hzl_surrogate_judge_start.argtypes = [vp, vp, vp, C.c_double, sz, C.c_int]
+    L.hzl_surrogate_judge_poll.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _dp]
     L.hzl_surrogate_collect.argtypes = [vp, vp, C.c_double, sz, sz, C.c_int]
     L.hzl_surrogate_dataset_size.argtypes = [vp, C.POINTER(sz)]
     L.hzl_surrogate_fit.argtypes = [vp, sz, sz, sz, sz, C.c_double, C.c_uint64, _dp]
@@ -1016,6 +1020,21 @@ class Surrogate:
                                         _SUR_PRECISION[precision], C.byref(v)))
         return v.value
 
+    def judge_start(self, ensemble: Ensemble, judge: Ensemble, dt: float, steps: int,
+                    precision: str = "bf16") -> None:
+        """The same judge, enqueued on the judge ensemble's stream without
+        waiting (the ensemble's stream waits only for the copy of the judged
+        states); judge_poll folds its result into the error estimate."""
+        _check(_lib.hzl_surrogate_judge_start(self._h, ensemble._h, judge._h, dt, steps,
+                                              _SUR_PRECISION[precision]))
+
+    def judge_poll(self, wait: bool = False):
+        """(finished, error_estimate): finished is True when this call folded a
+        judge in flight into the estimate (wait=True blocks until it ends)."""
+        f, v = C.c_int(), C.c_double()
+        _check(_lib.hzl_surrogate_judge_poll(self._h, int(wait), C.byref(f), C.byref(v)))
+        return bool(f.value), v.value
+
     def loss_grad(self, fields, next_fields, nu, amp=None):
         f = np.ascontiguousarray(fields, dtype=np.complex128)
         g = np.ascontiguousarray(next_fields, dtype=np.complex128)
@@ -1098,7 +1117,7 @@ class Surrogate:
 def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt: float,
                      seed: int, precision: str = "bf16", retain_states: bool = True,
                      fallback_threshold: float | None = None, judge: Ensemble | None = None,
-                     judge_every: int = 5):
+                     judge_every: int = 5, judge_async: bool = False):
     """Bootstrap particle filter whose forecast between observation groups is
     one surrogate evaluation (every group interval must equal the surrogate's
     horizon); weighting and resampling as in bootstrap_filter.  With
@@ -1107,7 +1126,10 @@ def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt
     (history records `solver`, `spread` and `error_estimate`).  With a judge
     ensemble, every judge_every-th group first re-forecasts judge.members
     particles with the solver and the network (Surrogate.judge) and uses their
-    measured RMS difference as the error estimate."""
+    measured RMS difference as the error estimate.  With judge_async, only the
+    first group's judge is waited for; later ones run on the judge ensemble's
+    stream beside the filter's cycles and each group uses the latest finished
+    judge (history records `judged` on the groups whose estimate is new)."""
     obs = np.asarray(observations, dtype=np.float64)
     if obs.ndim != 2 or obs.shape[1] != 5 or len(obs) == 0:
         raise ValueError("observations must be rows of (step, x, y, value, sd)")
@@ -1119,9 +1141,15 @@ def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt
     history, current = [], 0
     for gi, target in enumerate(groups):
         sel = obs[steps == target]
-        solver, spread = False, None
-        if judge is not None and gi % judge_every == 0:
+        solver, spread, judged = False, None, False
+        if judge is not None and judge_async and gi > 0:
+            # the previous judge has had judge_every groups to finish
+            judged = surrogate.judge_poll(wait=gi % judge_every == 0)[0]
+            if gi % judge_every == 0:
+                surrogate.judge_start(particles, judge, dt, int(target - current), precision)
+        elif judge is not None and gi % judge_every == 0:
             surrogate.judge(particles, judge, dt, int(target - current), precision)
+            judged = True
         if fallback_threshold is None:
             surrogate.forecast(particles, dt, int(target - current), precision)
         else:
@@ -1133,8 +1161,10 @@ def surrogate_filter(particles: Ensemble, surrogate: Surrogate, observations, dt
             break  # every weight -inf (src/filter.cpp:76-80)
         history.append(dict(step=int(target), increment=rec[0], ess=rec[1],
                             max_weight=rec[2], resampled=bool(rec[3]), mean_field=mean,
-                            solver=solver, spread=spread,
+                            solver=solver, spread=spread, judged=judged,
                             error_estimate=surrogate.error_estimate))
         current = int(target)
+    if judge is not None and judge_async:
+        surrogate.judge_poll(wait=True)
     ll, events = particles.pf_result()
     return dict(log_likelihood=ll, resampling_events=events, history=history)

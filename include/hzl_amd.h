@@ -609,6 +609,24 @@ int hzl_surrogate_forecast_or_solve(hzl_surrogate_t h, hzl_ensemble_t ensemble, 
 int hzl_surrogate_judge(hzl_surrogate_t h, hzl_ensemble_t ensemble, hzl_ensemble_t judge,
                         double dt, size_t steps, int precision, double *rms);
 
+/* The same judge, enqueued without waiting: judge's stream waits for the
+ * work already queued on ensemble's stream and copies the judged states;
+ * ensemble's stream waits only for that copy.  The network forecast of the
+ * copies is ordered with the ensemble's forecasts; the solver run and the
+ * comparison overlap the ensemble's later work.  Parameters are taken from
+ * the ensemble's host-side values.  One judge is in flight per surrogate: a
+ * start (or hzl_surrogate_judge) first finishes the previous one.  No
+ * counterpart in the reference: lets a surrogate filter keep judging without
+ * stalling its cycles. */
+int hzl_surrogate_judge_start(hzl_surrogate_t h, hzl_ensemble_t ensemble, hzl_ensemble_t judge,
+                              double dt, size_t steps, int precision);
+
+/* Fold a judge in flight into the error estimate.  wait != 0 blocks until it
+ * has finished; wait == 0 returns at once.  *finished = 1 when a result was
+ * folded in by this call (0: none in flight, or not finished yet); *rms
+ * receives the current error estimate (-1 before any). */
+int hzl_surrogate_judge_poll(hzl_surrogate_t h, int wait, int *finished, double *rms);
+
 /* Advance every member of an FP32 ensemble by steps * dt with the network
  * (steps and dt must equal the trained horizon).  The ensemble's step counter
  * advances by `steps`; filter calls may follow with steps = 0.  The ensemble

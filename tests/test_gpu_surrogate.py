@@ -285,16 +285,30 @@ def test_surrogate_filter_runs_and_tracks(pkg, orc):
         pf.set_state(init)
         fell = pkg.surrogate_filter(pf, sur, rows, dt, 5, precision="bf16",
                                     fallback_threshold=1e-9, judge=judge, judge_every=3)
+    # the same with the judges after the first one running beside the cycles
+    with pkg.Ensemble(n, 256, "fp32", nu=1e-3, sigma=0.02, amp=0.1, seed=12) as pf, \
+            pkg.Ensemble(n, 8, "fp32", nu=1e-3, sigma=0.0, amp=0.1, seed=13) as judge:
+        pf.set_state(init)
+        fell_async = pkg.surrogate_filter(pf, sur, rows, dt, 5, precision="bf16",
+                                          fallback_threshold=1e-9, judge=judge,
+                                          judge_every=3, judge_async=True)
     with pkg.Ensemble(n, 256, "fp32", nu=1e-3, sigma=0.02, amp=0.1, seed=12) as pf:
         pf.set_state(init)
         plain = pkg.bootstrap_filter(pf, rows, dt, 5)
     sur.close()
-    assert all(h["solver"] for h in fell["history"])
-    assert all(h["error_estimate"] > 0 for h in fell["history"])
-    assert fell["log_likelihood"] == plain["log_likelihood"]
-    for a, b in zip(fell["history"], plain["history"]):
-        assert a["increment"] == b["increment"] and a["resampled"] == b["resampled"]
-        np.testing.assert_array_equal(a["mean_field"], b["mean_field"])
+    # synchronous judges at groups 0, 3, 6, 9; enqueued ones started at 3 and 6
+    # are folded in by group 6 and 9 at the latest (the one started at 9 after
+    # the last group)
+    assert [h["judged"] for h in fell["history"]] == [g % 3 == 0 for g in range(10)]
+    assert fell_async["history"][0]["judged"]
+    assert sum(h["judged"] for h in fell_async["history"]) == 3
+    for run in (fell, fell_async):
+        assert all(h["solver"] for h in run["history"])
+        assert all(h["error_estimate"] > 0 for h in run["history"])
+        assert run["log_likelihood"] == plain["log_likelihood"]
+        for a, b in zip(run["history"], plain["history"]):
+            assert a["increment"] == b["increment"] and a["resampled"] == b["resampled"]
+            np.testing.assert_array_equal(a["mean_field"], b["mean_field"])
 
 
 def test_spread_and_solver_fallback(pkg, orc):
@@ -425,3 +439,41 @@ def test_judge_measures_the_network_error_against_the_solver(pkg, orc):
         want = np.sqrt(np.sum(np.abs(d) ** 2) / J)
     print("judge", rms, want)
     assert rms > 0 and abs(rms - want) <= 1e-5 * want
+
+
+def test_async_judge_matches_the_synchronous_judge(pkg, orc):
+    """hzl_surrogate_judge_start / _poll: the enqueued judge folds in the same
+    RMS as hzl_surrogate_judge on the same states, although the ensemble
+    steps on right after the start (its stream waits only for the copy of
+    the judged states), and leaves the ensemble exactly where a twin without
+    any judge ends; a poll with nothing in flight reports nothing new."""
+    cfg = SMALL
+    s = shape_of(cfg)
+    B, J, dt, steps = 16, 6, 2e-3, 25
+    f = grf_fields(orc, s.n, B, 73)
+    nus = np.linspace(6e-4, 1.8e-3, B)
+    with pkg.Surrogate(**cfg, seed=9) as sur, \
+            pkg.Ensemble(s.n, B, "fp32", sigma=0.05, amp=0.1, seed=6) as ens, \
+            pkg.Ensemble(s.n, B, "fp32", sigma=0.05, amp=0.1, seed=6) as twin, \
+            pkg.Ensemble(s.n, J, "fp32", sigma=0.0, amp=0.1, seed=7) as judge:
+        sur.set_scales(1.5, 4.0)
+        for e in (ens, twin):
+            for m in range(B):
+                e.set_member_parameters(m, nus[m], 0.0, 0.05, 0.1 + 0.01 * m)
+            e.set_state(f)
+            e.step(dt, 3)
+        assert sur.judge_poll() == (False, -1.0)
+        for k in range(3):
+            # the synchronous judge of the twin's (identical) states
+            want = sur.judge(twin, judge, dt, steps, "fp32")
+            assert sur.judge_poll(wait=True) == (False, want)
+            sur.error_estimate = 123.0
+            sur.judge_start(ens, judge, dt, steps, "fp32")
+            ens.step(dt, 2)  # queued behind the copy of the judged states
+            finished, v = sur.judge_poll(wait=True)
+            print("async judge", k, v, "sync", want)
+            assert finished and v == want and sur.error_estimate == want
+            assert sur.judge_poll() == (False, want)
+            twin.step(dt, 2)
+            np.testing.assert_array_equal(ens.get_state(), twin.get_state())
+        assert ens.step_index == twin.step_index == 9
