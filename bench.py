@@ -107,6 +107,12 @@ def calibrate_copy() -> int:
     return 0
 
 
+def progress(msg: str) -> None:
+    """One line on stderr per finished part of a long leg (the JSON line stays
+    the only output on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 class HipTimer:
     """HIP events on a given stream (the stream the kernels are launched on)."""
 
@@ -393,6 +399,7 @@ def pf_variants(pkg, sur, rows, truths, init, particles: int, dt: float, nu: flo
             if kind == "surrogate_judged_async":
                 o["judge"] += " (enqueued after the first, beside the cycles)"
         out[kind] = o
+        progress(f"filter {kind}, {particles} particles: {o['cycles_per_s']} cycles/s")
     return out
 
 
@@ -465,6 +472,7 @@ def surrogate_pf(pkg, cycles: int = 20, train_windows: int = 20, epochs: int = 4
                      f"sd={sd}; surrogate 4x32ch, 16 modes, 64^2 grid"}
     sur, out["training"] = train_surrogate(pkg, n, dt, H, train_windows, epochs, (5e-4, 2e-3),
                                            (0.1, 0.1), sigma, SEED)
+    progress("surrogate trained")
     rows, truths, truth0 = twin(pkg, n, dt, H, cycles, nobs, sd, nu, amp, sigma, SEED + 30)
     out["truth_rms"] = round(float(np.mean([np.sqrt(np.sum(np.abs(t) ** 2)) for t in truths])), 5)
     # modes of the 128^2 state outside the network's band |k| <= 31

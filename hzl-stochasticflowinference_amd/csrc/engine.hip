@@ -105,13 +105,19 @@ template <typename R> __global__ void __launch_bounds__(256) k_pf_loglik(PfArgs<
 // one workgroup per filter
 __global__ void __launch_bounds__(256) k_pf_normalize(PfControl a) {
     __shared__ double red[2 * 256];
-    pf_normalize_body(DevCtx{}, a, red, 256, (int)blockIdx.x);
+    __shared__ double cw[kPfLocalWeights];
+    pf_normalize_body(DevCtx{}, a, red, 256, (int)blockIdx.x, cw, kPfLocalWeights);
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_pf_mean(int B, long spec, const cx<R> *w0,
-                                                 const double *wn, double *mean) {
-    pf_mean_body<R>(B, spec, w0, wn, mean, (long)blockIdx.x * 256 + threadIdx.x);
+__global__ void __launch_bounds__(256) k_pf_mean_partial(int B, long spec, const cx<R> *w0,
+                                                         const double *wn, double *part) {
+    pf_mean_partial_body<R>(B, spec, w0, wn, part, (long)blockIdx.x * 256 + threadIdx.x,
+                            (int)blockIdx.y);
+}
+__global__ void __launch_bounds__(256) k_pf_mean_finish(long spec, int chunks,
+                                                        const double *part, double *mean) {
+    pf_mean_finish_body(spec, chunks, part, mean, (long)blockIdx.x * 256 + threadIdx.x);
 }
 
 // Resampling copies, a device-side no-op for every filter of P particles that
@@ -1366,6 +1372,7 @@ template <typename R> class Engine final : public EngineBase {
         pf_stats_ = DeviceBuffer<double>(8 * filters);
         pf_anc_ = DeviceBuffer<int>(B_);
         pf_mean_ = DeviceBuffer<double>(2 * spec_);
+        pf_mean_part_ = DeviceBuffer<double>(2 * spec_ * ((B_ + kPfMeanChunk - 1) / kPfMeanChunk));
         std::vector<double> w(B_, 1.0 / (double)P), st(8 * filters, 0.0), cs(n_), sn(n_);
         upload(pf_wstate_.p, w.data(), B_);
         upload(pf_stats_.p, st.data(), 8 * filters);
@@ -1991,9 +1998,12 @@ template <typename R> class Engine final : public EngineBase {
         hipLaunchKernelGGL(k_pf_normalize, dim3((unsigned)pf_filters_), dim3(256), 0, stream_, c);
         if (with_mean) {
             const long blocks = ((long)spec_ + 255) / 256;
-            hipLaunchKernelGGL(k_pf_mean<R>, dim3((unsigned)blocks), dim3(256), 0, stream_,
-                               (int)B_, (long)spec_, (const cx<R> *)w0_.p, pf_wnorm_.p,
-                               pf_mean_.p);
+            const int chunks = (int)((B_ + kPfMeanChunk - 1) / kPfMeanChunk);
+            hipLaunchKernelGGL(k_pf_mean_partial<R>, dim3((unsigned)blocks, (unsigned)chunks),
+                               dim3(256), 0, stream_, (int)B_, (long)spec_, (const cx<R> *)w0_.p,
+                               (const double *)pf_wnorm_.p, pf_mean_part_.p);
+            hipLaunchKernelGGL(k_pf_mean_finish, dim3((unsigned)blocks), dim3(256), 0, stream_,
+                               (long)spec_, chunks, (const double *)pf_mean_part_.p, pf_mean_.p);
         }
         const long total = (long)B_ * (long)spec_;
         const dim3 grid((unsigned)((total + 255) / 256));
@@ -2250,8 +2260,8 @@ template <typename R> class Engine final : public EngineBase {
     hipEvent_t obs_event_ = nullptr;
     const double *obs_v_ = nullptr, *obs_sd_ = nullptr;
     const int *obs_x_ = nullptr, *obs_y_ = nullptr;
-    DeviceBuffer<double> pf_wstate_, pf_wnorm_, pf_logw_, pf_stats_, pf_mean_, pf_uniforms_,
-        pf_cs_, pf_sn_;
+    DeviceBuffer<double> pf_wstate_, pf_wnorm_, pf_logw_, pf_stats_, pf_mean_, pf_mean_part_,
+        pf_uniforms_, pf_cs_, pf_sn_;
     DeviceBuffer<int> pf_anc_;
     int npairs_ = 0;
     DeviceBuffer<double> cfl_, cfl_diag_;

@@ -1428,10 +1428,19 @@ struct PfControl {
                                      // fields above (graph replays)
 };
 
+// Particles per filter whose resampling sums fit in a normalising
+// workgroup's LDS (32 KB).
+constexpr int kPfLocalWeights = 4096;
+
 // One workgroup per filter: log-sum-exp normalisation, ESS, systematic
 // resampling below P/2 unless this is the last group (src/filter.cpp:76-120).
+// cw: workgroup-local room for `cap` doubles; when P <= cap the weights, their
+// running sums and the ancestor search stay there (the running sum is one
+// thread's chain of P dependent additions, in the reference's order, so its
+// operands should not wait on device memory), else in wstate.
 template <class Ctx>
-HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, int nt, int f) {
+HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, int nt, int f,
+                              double *cw = nullptr, int cap = 0) {
     const int tid = ctx.tid(), P = a.P;
     const long base = (long)f * P;
     const double *logw = a.logw + base;
@@ -1467,10 +1476,13 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
     }
     const double inc = finite ? mx + log(red[0]) : -__builtin_huge_val();
     ctx.sync();
+    double *const local = cw && P <= cap ? cw : nullptr;
     double sq = 0, mw = 0;
     for (int i = tid; i < P; i += nt) {
         const double w = finite ? exp(logw[i] - inc) : 0.0;
         wnorm[i] = w;
+        if (local)
+            local[i] = w;
         sq += w * w;
         mw = w > mw ? w : mw;
     }
@@ -1502,9 +1514,16 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
             stats[6] += 1;
             stats[5] += 1;
             double cum = 0;
-            for (int i = 0; i < P; ++i) {
-                cum += wnorm[i];
-                wstate[i] = cum;
+            if (local) {
+                for (int i = 0; i < P; ++i) {
+                    cum += local[i];
+                    local[i] = cum;
+                }
+            } else {
+                for (int i = 0; i < P; ++i) {
+                    cum += wnorm[i];
+                    wstate[i] = cum;
+                }
             }
         }
     }
@@ -1514,12 +1533,13 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
         // u/P + i/P: the strict `position > cumulative` scan of
         // src/filter.cpp:102-120, by binary search
         const double start = red[0] / (double)P;
+        const double *sums = local ? local : wstate;
         for (int i = tid; i < P; i += nt) {
             const double pos = start + (double)i / (double)P;
             int lo = 0, hi = P - 1;
             while (lo < hi) {
                 const int mid = (lo + hi) / 2;
-                if (pos > wstate[mid])
+                if (pos > sums[mid])
                     lo = mid + 1;
                 else
                     hi = mid;
@@ -1538,18 +1558,36 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
     ctx.sync();
 }
 
-// Weighted mean of the retained states with the group's normalised weights,
-// accumulated over particles in order (src/filter.cpp:94-100).
+// Weighted mean of the retained states with the group's normalised weights
+// (src/filter.cpp:94-100), in two launches: sums over the particles
+// [c per, (c + 1) per) of chunk c in order, then the chunks in order (the
+// reference's running sum regrouped: results agree to FP64 rounding).  part
+// holds 2 doubles per (chunk, mode).
+constexpr int kPfMeanChunk = 32;
+
 template <typename R>
-HZL_HD void pf_mean_body(int B, long spec, const cx<R> *w0, const double *wn, double *mean,
-                         long e) {
+HZL_HD void pf_mean_partial_body(int B, long spec, const cx<R> *w0, const double *wn,
+                                 double *part, long e, int c) {
+    if (e >= spec)
+        return;
+    const int lo = c * kPfMeanChunk, hi = lo + kPfMeanChunk < B ? lo + kPfMeanChunk : B;
+    double re = 0, im = 0;
+    for (int m = lo; m < hi; ++m) {
+        const cx<R> v = w0[(long)m * spec + e];
+        re += wn[m] * (double)v.x;
+        im += wn[m] * (double)v.y;
+    }
+    part[2 * ((long)c * spec + e)] = re;
+    part[2 * ((long)c * spec + e) + 1] = im;
+}
+
+HZL_HD void pf_mean_finish_body(long spec, int chunks, const double *part, double *mean, long e) {
     if (e >= spec)
         return;
     double re = 0, im = 0;
-    for (int m = 0; m < B; ++m) {
-        const cx<R> v = w0[m * spec + e];
-        re += wn[m] * (double)v.x;
-        im += wn[m] * (double)v.y;
+    for (int c = 0; c < chunks; ++c) {
+        re += part[2 * ((long)c * spec + e)];
+        im += part[2 * ((long)c * spec + e) + 1];
     }
     mean[2 * e] = re;
     mean[2 * e + 1] = im;

@@ -45,7 +45,8 @@ template <typename R> struct MultiSources {
 
 __global__ void __launch_bounds__(256) k_pf_multi_normalize(PfControl a) {
     __shared__ double red[2 * 256];
-    pf_normalize_body(DevCtx{}, a, red, 256, 0);
+    __shared__ double cw[kPfLocalWeights];
+    pf_normalize_body(DevCtx{}, a, red, 256, 0, cw, kPfLocalWeights);
 }
 
 // dst member i of ensemble `self` (grid K) <- ancestor anc[first_self + i],

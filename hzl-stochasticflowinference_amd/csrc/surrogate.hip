@@ -204,8 +204,12 @@ __global__ void k_gather_nu(const double *nu, const int *idx, double *out, int S
 __global__ void k_dsum(const double *x, long n, int chunks, double *out) {
     dsum_body(x, n, chunks, out, HZL_GID);
 }
-__global__ void k_spread(const cx<float> *w0, int B, int K, double *out) {
-    spread_body(w0, B, K, out, HZL_GID);
+__global__ void k_spread_partial(const cx<float> *w0, int B, long spec, int per, double *part) {
+    spread_partial_body(w0, B, spec, per, part, (long)blockIdx.x * blockDim.x + threadIdx.x,
+                        (int)blockIdx.y);
+}
+__global__ void k_spread_finish(int B, int K, int chunks, const double *part, double *out) {
+    spread_finish_body(B, K, chunks, part, out, HZL_GID);
 }
 __global__ void k_fill_f32(float *p, long n, float v) {
     const long i = HZL_GID;
@@ -638,9 +642,17 @@ class Surrogate {
     double spread(EngineBase &eng) {
         const EngineViews ev = check_engine(eng);
         const long spec = s_.spec();
+        // members in chunks of 32 over the second grid dimension: enough
+        // workgroups to fill the device at a few hundred members
+        constexpr int per = 32;
+        const int chunks = (ev.B + per - 1) / per;
         spread_.alloc((size_t)spec);
-        hipLaunchKernelGGL(k_spread, grid1d(spec), dim3(256), 0, ev.stream,
-                           reinterpret_cast<const cx<float> *>(ev.w0), ev.B, s_.K, spread_.p);
+        spread_part_.alloc(3 * (size_t)chunks * (size_t)spec);
+        hipLaunchKernelGGL(k_spread_partial, dim3((unsigned)((spec + 255) / 256), (unsigned)chunks),
+                           dim3(256), 0, ev.stream, reinterpret_cast<const cx<float> *>(ev.w0),
+                           ev.B, (long)spec, per, spread_part_.p);
+        hipLaunchKernelGGL(k_spread_finish, grid1d(spec), dim3(256), 0, ev.stream, ev.B, s_.K,
+                           chunks, (const double *)spread_part_.p, spread_.p);
         std::vector<double> h((size_t)spec);
         hcheck(hipMemcpyAsync(h.data(), spread_.p, spec * sizeof(double), hipMemcpyDeviceToHost,
                               ev.stream),
@@ -1600,7 +1612,7 @@ class Surrogate {
     Buf<float> master_, grad_, m_, v_;
     Buf<float> g1_, g2_, f1_, f2_, bm_, dgrid_, incr_, target_, part_, rgrad_;
     Buf<cx<float>> diff_, hx_, hy_, ds_x_, ds_y_, bx_, by_, backup_, judge_x_, judge_y_;
-    Buf<double> hnu_, ds_nu_, bnu_, hamp_, ds_amp_, bamp_, lsum_, spread_;
+    Buf<double> hnu_, ds_nu_, bnu_, hamp_, ds_amp_, bamp_, lsum_, spread_, spread_part_;
     double error_rms_ = -1;
     Buf<uint64_t> counter_dev_;
     bool use_graph_ = graphs_by_default();

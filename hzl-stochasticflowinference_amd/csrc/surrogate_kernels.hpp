@@ -213,25 +213,49 @@ HZL_HD void apply_increment_body(cx<float> *w0, const float *incr, int S, int K,
 // Ensemble spread of one retained mode e: (kx > 0 ? 2 : 1) x the member
 // variance of w_e, so that the sum over modes is the grid mean square
 // deviation from the ensemble mean (Parseval).
-HZL_HD void spread_body(const cx<float> *w0, int B, int K, double *out, long e) {
+// Ensemble spread per retained mode in two launches.  The first sums, for
+// mode e over the members [c per, (c + 1) per) of chunk c, the deviations
+// d_m = w_m - w_0 from member 0 and their squares; the second combines the
+// chunks: sum |w_m - mean|^2 = sum |d|^2 - |sum d|^2 / B, shifted by member 0
+// so that a mode whose mean is large against its spread loses no digits to
+// the cancellation of raw moments.  part holds 3 doubles per (chunk, mode).
+HZL_HD void spread_partial_body(const cx<float> *w0, int B, long spec, int per, double *part,
+                                long e, int c) {
+    if (e >= spec)
+        return;
+    const int lo = c * per, hi = lo + per < B ? lo + per : B;
+    const cx<float> ref = w0[e];
+    double sr = 0, si = 0, ss = 0;
+    for (int m = lo; m < hi; ++m) {
+        const cx<float> v = w0[(long)m * spec + e];
+        const double dr = (double)v.x - (double)ref.x, di = (double)v.y - (double)ref.y;
+        sr += dr;
+        si += di;
+        ss += dr * dr + di * di;
+    }
+    double *p = part + 3 * ((long)c * spec + e);
+    p[0] = sr;
+    p[1] = si;
+    p[2] = ss;
+}
+
+// Mode e's share of the squared spread: the conjugate-symmetric modes with
+// kx > 0 count twice (the grid RMS over the full plane).
+HZL_HD void spread_finish_body(int B, int K, int chunks, const double *part, double *out,
+                               long e) {
     const int J = 2 * K + 1;
     const long spec = (long)(K + 1) * J;
     if (e >= spec)
         return;
-    double mr = 0, mi = 0;
-    for (int m = 0; m < B; ++m) {
-        mr += w0[m * spec + e].x;
-        mi += w0[m * spec + e].y;
+    double sr = 0, si = 0, ss = 0;
+    for (int c = 0; c < chunks; ++c) {
+        const double *p = part + 3 * ((long)c * spec + e);
+        sr += p[0];
+        si += p[1];
+        ss += p[2];
     }
-    mr /= B;
-    mi /= B;
-    double s = 0;
-    for (int m = 0; m < B; ++m) {
-        const double dr = w0[m * spec + e].x - mr, di = w0[m * spec + e].y - mi;
-        s += dr * dr + di * di;
-    }
-    const int c = (int)(e / J);
-    out[e] = (c > 0 ? 2.0 : 1.0) * s / B;
+    const double s = ss - (sr * sr + si * si) / B;
+    out[e] = (e / J > 0 ? 2.0 : 1.0) * (s > 0 ? s : 0.0) / B;
 }
 
 // Weighted column sums: out[c][n] = sum over rows m of chunk c of w[m] x[m][n].

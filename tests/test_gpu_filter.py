@@ -90,6 +90,32 @@ def test_pf_matches_oracle_with_stochastic_forcing(pkg, orc, precision):
         assert rel < (1e-12 if precision == "fp64" else 1e-5)
 
 
+@pytest.mark.parametrize("P", [4096, 4100], ids=["sums_in_lds", "sums_in_memory"])
+def test_pf_matches_oracle_at_thousands_of_particles(pkg, orc, P):
+    """Up to 4096 particles per filter the resampling sums and the ancestor
+    search stay in the normalising workgroup's LDS, above that in device
+    memory: both give the oracle's resampling (one event, in the second of
+    three groups) and log-likelihood to 1e-10 relative (FP64, 16^2)."""
+    n, nu, dt, seed = 16, 2e-3, 2e-3, 78
+    rows, _ = synthetic(orc, n, nu, dt, groups=3, every=4, nobs=24, truth_key=998, seed=4,
+                        sd=1.0)
+    base = orc.grf(n, 998)
+    init = np.stack([base + 0.05 * orc.grf(n, orc.stream_seed(6, i)) for i in range(P)])
+    want = oracle_bootstrap_filter(orc, init, rows, dt, seed, nu, sigma=0.2)
+    with pkg.Ensemble(n, P, "fp64", nu=nu, sigma=0.2, amp=0.1, kf=4, seed=seed) as ens:
+        ens.set_state(init)
+        got = pkg.bootstrap_filter(ens, rows, dt, seed)
+    print(P, got["log_likelihood"], want["log_likelihood"])
+    assert [h["resampled"] for h in want["history"]] == [False, True, False]
+    assert [h["resampled"] for h in got["history"]] == [False, True, False]
+    assert abs(got["log_likelihood"] - want["log_likelihood"]) < 1e-10 * abs(
+        want["log_likelihood"])
+    for a, b in zip(got["history"], want["history"]):
+        assert abs(a["ess"] - b["ess"]) < 1e-8 * b["ess"]
+        rel = np.linalg.norm(a["mean_field"] - b["mean_field"]) / np.linalg.norm(b["mean_field"])
+        assert rel < 1e-12
+
+
 def test_pf_all_minus_infinity_gives_minus_infinity(pkg):
     """All -inf weights end the filter with log-likelihood -inf (src/filter.cpp:76-80)."""
     n, P = 16, 4
