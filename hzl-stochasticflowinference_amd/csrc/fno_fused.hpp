@@ -112,28 +112,30 @@ HZL_HD bf16x8_t load8(const __bf16 *p) {
     return *reinterpret_cast<const bf16x8_t *>(p);
 }
 
-// GELU with erf by Abramowitz & Stegun 7.1.26 (|erf error| <= 1.5e-7, one
-// exponential and one reciprocal): the inference passes' activation, far
-// inside the BF16 rounding of their outputs.  Training keeps gelu() (erff).
-// Written as gelu(x) = x Phi(x), Phi(x) = 1 - q (x >= 0) or q (x < 0) with
-// q = erfc(|x| / sqrt 2) / 2 = t (a1 + t (a2 + ...)) / 2 e^{-x^2 / 2},
-// t = 1 / (1 + p |x| / sqrt 2).
-HZL_HD float gelu_as(float x) {
-    constexpr float P = 0.3275911f * 0.70710678118654752f;
-    constexpr float A1 = 0.5f * 0.254829592f, A2 = 0.5f * -0.284496736f,
-                    A3 = 0.5f * 1.421413741f, A4 = 0.5f * -1.453152027f,
-                    A5 = 0.5f * 1.061405429f;
-    constexpr float HALF_LOG2E = 0.72134752044448170f; // log2(e) / 2
-    const float ax = x < 0.f ? -x : x;
+// GELU as x sigma(z(x)): x sigma(z) is GELU exactly for z = logit Phi(x) =
+// log(Phi / (1 - Phi)); here z(x) = x (c1 + c3 x^2 + c5 x^4), fitted to that
+// logit by least squares weighted with d GELU / dz = x Phi (1 - Phi) and
+// reweighted to minimax (Lawson) on [-8, 8], with x clamped to [-8, 8] inside
+// z (beyond it Phi is 0 or 1 to 1e-15).  |GELU error| <= 2.6e-5 evaluated in
+// FP32 over the real line, far inside the BF16 rounding of the outputs, for
+// one exponential, one reciprocal and seven other operations: the inference
+// passes' activation.  Training keeps gelu() (erff).  Q1, Q3, Q5 are
+// -log2(e) (c1, c3, c5), so that exp(-z) = exp2(x (Q1 + Q3 x^2 + Q5 x^4)).
+HZL_HD float gelu_sg(float x) {
+    constexpr float L2E = 1.44269504088896341f;
+    constexpr float Q1 = -1.59501517f * L2E, Q3 = -7.40119890e-2f * L2E,
+                    Q5 = 7.03185240e-4f * L2E;
 #if __HIP_DEVICE_COMPILE__
-    const float t = __builtin_amdgcn_rcpf(1.0f + P * ax);
-    const float e = __builtin_amdgcn_exp2f(-HALF_LOG2E * (x * x));
+    const float xc = __builtin_amdgcn_fmed3f(x, -8.f, 8.f);
+    const float x2 = xc * xc;
+    const float t = xc * (Q1 + x2 * (Q3 + x2 * Q5));
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 #else
-    const float t = 1.0f / (1.0f + P * ax);
-    const float e = std::exp2(-HALF_LOG2E * (x * x));
+    const float xc = x < -8.f ? -8.f : (x > 8.f ? 8.f : x);
+    const float x2 = xc * xc;
+    const float t = xc * (Q1 + x2 * (Q3 + x2 * Q5));
+    return x / (1.0f + std::exp2(t));
 #endif
-    const float q = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * e;
-    return x * (x < 0.f ? q : 1.0f - q);
 }
 
 template <class Ctx>
@@ -335,7 +337,7 @@ HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *ld
                     for (int u = 0; u < 4; ++u) {
                         float x = d[4 * g + u] + bias[4 * g + u];
                         if (a.act == ACT_GELU)
-                            x = gelu_as(x);
+                            x = gelu_sg(x);
                         q4.v[u] = (__bf16)x;
                     }
                     *reinterpret_cast<Q *>(o + 8 * g + 4 * h) = q4;
@@ -382,7 +384,7 @@ HZL_HD void fno_head_wave(const Ctx &ctx, const FnoHeadArgs &a, long wave, int l
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int j = 32 * nt + 8 * (i / 4) + 4 * h + i % 4;
-            rs += hq[FNO_H + j] * gelu_as(d[i] + hq[j]);
+            rs += hq[FNO_H + j] * gelu_sg(d[i] + hq[j]);
         }
     }
     rs += ctx.shfl_xor(rs, 32, lane);

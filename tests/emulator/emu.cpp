@@ -699,9 +699,9 @@ int emu_fno_mix(long S, int modes, const float *V, const float *R, float *U) {
     return 0;
 }
 
-int emu_gelu_as(long n, const float *x, float *y) {
+int emu_gelu_sg(long n, const float *x, float *y) {
     for (long i = 0; i < n; ++i)
-        y[i] = gelu_as(x[i]);
+        y[i] = gelu_sg(x[i]);
     return 0;
 }
 

@@ -37,8 +37,8 @@ class Emulator:
         L.emu_grid_to_increment.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
         L.emu_fno_mix.restype = C.c_int
         L.emu_fno_mix.argtypes = [C.c_long, C.c_int, _fp, _fp, _fp]
-        L.emu_gelu_as.restype = C.c_int
-        L.emu_gelu_as.argtypes = [C.c_long, _fp, _fp]
+        L.emu_gelu_sg.restype = C.c_int
+        L.emu_gelu_sg.argtypes = [C.c_long, _fp, _fp]
         L.emu_fno_lift.restype = C.c_int
         L.emu_fno_lift.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
         L.emu_proj_head.restype = C.c_int
@@ -141,11 +141,11 @@ class Emulator:
                                     U.ctypes.data_as(fp)) == 0
         return U
 
-    def gelu_as(self, x):
+    def gelu_sg(self, x):
         x = self._f32(x)
         y = np.zeros_like(x)
         fp = C.POINTER(C.c_float)
-        assert self.lib.emu_gelu_as(x.size, x.ctypes.data_as(fp), y.ctypes.data_as(fp)) == 0
+        assert self.lib.emu_gelu_sg(x.size, x.ctypes.data_as(fp), y.ctypes.data_as(fp)) == 0
         return y
 
     def fno_inverse(self, U, Gy2, Gx, v, W, bias, act):

@@ -114,20 +114,30 @@ def test_head_matches_numpy(emu):
     q2 = np.float32(0.3)
     got = emu.fno_head(X, Q1, q1, Q2, q2)
     want = gelu(X @ Q1.T + q1.astype(np.float64)) @ Q2.astype(np.float64) + float(q2)
-    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
-    print("head max rel err", err)
-    assert err < 1e-5
+    # the inference GELU is within 2.6e-5 of the exact one: each output within
+    # sum |Q2| 2.6e-5 plus FP32 rounding of the 128-term sum
+    err = np.abs(got - want)
+    bound = 2.6e-5 * np.sum(np.abs(Q2)) + 1e-5 * np.max(np.abs(want))
+    print("head max err", float(np.max(err)), "bound", bound)
+    assert np.all(err <= bound)
 
 
-def test_gelu_as_matches_exact_erf(emu):
-    """Abramowitz & Stegun 7.1.26: |erf error| <= 1.5e-7, so GELU within
-    0.5 |x| 1.5e-7 plus FP32 rounding."""
-    x = np.linspace(-9.0, 9.0, 20001).astype(np.float32)
-    got = emu.gelu_as(x).astype(np.float64)
+def test_gelu_sg_matches_exact_erf(emu):
+    """The inference GELU x sigma(z(x)) (fitted odd quintic z, argument clamped
+    to [-8, 8]) within 2.6e-5 of x Phi(x) over [-9, 9] in FP32 and beyond the
+    clamp (|x| up to 1e4, where it is x or 0); exact at 0 and odd-symmetric
+    in the sense gelu(x) - gelu(-x) = x."""
+    x = np.concatenate([np.linspace(-9.0, 9.0, 200001),
+                        [-1e4, -300.0, -20.0, 20.0, 300.0, 1e4]]).astype(np.float32)
+    got = emu.gelu_sg(x).astype(np.float64)
     want = gelu(x.astype(np.float64))
-    bound = 0.5 * np.abs(x) * 1.6e-7 + 2e-7 * np.maximum(1.0, np.abs(want))
-    print("max gelu err", float(np.max(np.abs(got - want))))
-    assert np.all(np.abs(got - want) <= bound)
+    err = np.abs(got - want)
+    print("max gelu err", float(np.max(err)))
+    assert np.all(err <= 2.6e-5 + 1e-7 * np.abs(want))
+    assert emu.gelu_sg(np.zeros(1, np.float32))[0] == 0.0
+    xs = np.linspace(0.0, 6.0, 61).astype(np.float32)
+    d = emu.gelu_sg(xs).astype(np.float64) - emu.gelu_sg(-xs).astype(np.float64)
+    assert np.all(np.abs(d - xs) <= 1e-6 * np.maximum(1.0, xs))
 
 
 @pytest.mark.parametrize("K", [42, 85], ids=["state128", "state256"])
