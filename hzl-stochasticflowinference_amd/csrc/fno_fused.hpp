@@ -57,6 +57,11 @@ constexpr int BI_ELEMS = YBI * C * BI_LD;
 constexpr int UT_LD = 2 * KY + 8;             // a wavefront's U slice [c][k'] pitch
 constexpr int UT_ELEMS = C * UT_LD;
 constexpr int INV_LDS = BI_ELEMS + (THREADS / 64) * UT_ELEMS;
+// layer 0 reads its input lifted on the fly (FnoLift): LDS for the row
+// constants k[y][c] of the pass's rows and the weights Lw[c][0] (floats)
+constexpr int FWD_LIFT_F32 = G * C + C, INV_LIFT_F32 = YBI * C + C;
+constexpr int FWD_LDS_LIFT = FWD_LDS + 2 * FWD_LIFT_F32; // bf16 elements
+constexpr int INV_LDS_LIFT = INV_LDS + 2 * INV_LIFT_F32;
 constexpr long SAMPLE = (long)G * G * C;      // pixel activations per sample
 constexpr long SPECTRAL = (long)KX * 2 * KY * C;
 
@@ -66,15 +71,38 @@ static_assert(2 * KX == 32 && C == 32 && 2 * KY == 64, "tiles are 32 x 32");
 static_assert(2 * KX == 4 * (THREADS / 64), "four y-transform tiles per wavefront");
 static_assert(KX == 2 * (THREADS / 64) && YBI == 2 * (THREADS / 64),
               "two kx and two rows per wavefront in the inverse pass");
+static_assert(CIN == 3, "the lift reads the grid, viscosity and forcing features");
+static_assert(FWD_LDS % 8 == 0 && INV_LDS % 8 == 0, "lift tables start 16-byte aligned");
 
 } // namespace fno
 
+// Layer 0's input is the lift of the three input features,
+//   v[y][x][c] = bf16(f0[y][x] Lw[c][0] + k[y][c]),
+//   k[y][c] = f1 Lw[c][1] + f2[y] Lw[c][2] + Lb[c]  (two fma),
+// with f0 the scaled state grid, f1 the sample's viscosity feature and f2 the
+// row's forcing feature (constant along a row): the passes that read it
+// recompute it from the 6-byte features instead of reading 64 bytes of
+// activations per point.
+struct FnoLift {
+    const __bf16 *feat = nullptr; // [S][G][G][CIN]
+    const __bf16 *Lw = nullptr;   // [C][CIN]
+    const float *Lb = nullptr;    // [C]
+};
+
+HZL_HD float lift_row_constant(float f1, float f2, float w1, float w2, float b) {
+    return __builtin_fmaf(f2, w2, __builtin_fmaf(f1, w1, b));
+}
+HZL_HD __bf16 lift_value(float f0, float w0, float k) {
+    return (__bf16)__builtin_fmaf(f0, w0, k);
+}
+
 struct FnoForwardArgs {
-    const __bf16 *v;   // [S][G][G][C] layer input
+    const __bf16 *v;   // [S][G][G][C] layer input (unused when lifted)
     const __bf16 *Fx;  // [2KX][G]
     const __bf16 *Fy2; // [2KY][2G]
     __bf16 *V;         // [S][KX][2KY][C] = [s][kx][kyi][ri][c]
     long S;
+    FnoLift lift;      // layer 0: the input's features and lift weights
 };
 
 struct FnoInverseArgs {
@@ -87,14 +115,7 @@ struct FnoInverseArgs {
     __bf16 *out;       // [S][G][G][C]
     long S;
     int act;           // ACT_GELU or ACT_NONE
-};
-
-struct FnoLiftArgs {
-    const __bf16 *feat; // [S G G][CIN]
-    const __bf16 *Lw;   // [C][CIN]
-    const float *Lb;    // [C]
-    __bf16 *v;          // [S G G][C]
-    long points;
+    FnoLift lift;      // layer 0: v lifted from the features
 };
 
 struct FnoHeadArgs {
@@ -152,14 +173,38 @@ HZL_HD void fno_mfma(const Ctx &ctx, const bf16x8_t &fa, const bf16x8_t &fb, f32
 
 // Forward x and y transforms of sample s (workgroup of fno::THREADS lanes,
 // lds: fno::FWD_LDS bf16 elements).
+// The row constants k[y][c] of rows y0 .. y0 + rows - 1 of sample s and the
+// weights w0[c] = Lw[c][0] into LDS (f1, f2 read at each row's x = 0).
 template <class Ctx>
+HZL_HD void lift_stage(const Ctx &ctx, const FnoLift &L, long s, int y0, int rows, float *kt,
+                       float *w0) {
+    using namespace fno;
+    const __bf16 *fs = L.feat + s * (long)G * G * CIN;
+    for (int e = ctx.tid(); e < rows * C; e += THREADS) {
+        const int y = e / C, c = e % C;
+        const __bf16 *f = fs + (long)(y0 + y) * G * CIN;
+        kt[e] = lift_row_constant((float)f[1], (float)f[2], (float)L.Lw[c * CIN + 1],
+                                  (float)L.Lw[c * CIN + 2], L.Lb[c]);
+    }
+    for (int c = ctx.tid(); c < C; c += THREADS)
+        w0[c] = (float)L.Lw[c * CIN];
+    ctx.sync();
+}
+
+template <bool LIFT = false, class Ctx>
 HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *lds, long s) {
     using namespace fno;
     if (s >= a.S)
         return; // whole workgroup
     const int tid = ctx.tid(), wave = tid / 64, lane = tid % 64, r = lane % 32, h = lane / 32;
     __bf16 *vT = lds, *Ab = lds + VT_ELEMS;
-    const __bf16 *vs = a.v + s * SAMPLE;
+    const __bf16 *vs = LIFT ? nullptr : a.v + s * SAMPLE;
+    float *kt = nullptr, *lw0 = nullptr;
+    if constexpr (LIFT) {
+        kt = reinterpret_cast<float *>(lds + FWD_LDS);
+        lw0 = kt + G * C;
+        lift_stage(ctx, a.lift, s, 0, G, kt, lw0);
+    }
     // x-transform operand A: Fx[r][16 kb + 8 h + j]
     bf16x8_t fx[G / 16];
 #pragma unroll
@@ -176,12 +221,16 @@ HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *ld
     constexpr int PER_ROW = G * C / 8, CH = YB * PER_ROW / THREADS;
     static_assert(YB * PER_ROW % THREADS == 0, "block chunks must divide over the lanes");
     bf16x8_t pre[CH];
+    __bf16 f0pre[CH]; // lifted: the grid feature of each chunk's point
     auto load_block = [&](int b) {
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
             const int idx = tid + THREADS * q, y = idx / PER_ROW, rem = idx % PER_ROW;
             const int x = rem / (C / 8), c0 = (rem % (C / 8)) * 8;
-            pre[q] = load8(vs + ((long)(b * YB + y) * G + x) * C + c0);
+            if constexpr (LIFT)
+                f0pre[q] = a.lift.feat[((s * G + b * YB + y) * (long)G + x) * CIN];
+            else
+                pre[q] = load8(vs + ((long)(b * YB + y) * G + x) * C + c0);
         }
     };
     load_block(0);
@@ -191,9 +240,17 @@ HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *ld
         for (int q = 0; q < CH; ++q) {
             const int idx = tid + THREADS * q, y = idx / PER_ROW, rem = idx % PER_ROW;
             const int x = rem / (C / 8), c0 = (rem % (C / 8)) * 8;
+            if constexpr (LIFT) {
+                const float f0 = (float)f0pre[q];
+                const float *kr = kt + (b * YB + y) * C + c0;
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                vT[(y * C + c0 + j) * VT_LD + x] = pre[q][j];
+                for (int j = 0; j < 8; ++j)
+                    vT[(y * C + c0 + j) * VT_LD + x] = lift_value(f0, lw0[c0 + j], kr[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    vT[(y * C + c0 + j) * VT_LD + x] = pre[q][j];
+            }
         }
         ctx.sync();
         if (b + 1 < NBLK)
@@ -241,7 +298,7 @@ HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *ld
 // block `block % NBLKI` of sample `block / NBLKI` (workgroup of fno::THREADS
 // lanes, lds: fno::INV_LDS bf16 elements).  Row blocks are independent, so
 // a sample spreads over NBLKI workgroups.
-template <class Ctx>
+template <bool LIFT = false, class Ctx>
 HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *lds, long block) {
     using namespace fno;
     const long s = block / NBLKI;
@@ -250,6 +307,12 @@ HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *ld
         return; // whole workgroup
     const int tid = ctx.tid(), wave = tid / 64, lane = tid % 64, r = lane % 32, h = lane / 32;
     __bf16 *Bi = lds;
+    float *kt = nullptr, *lw0 = nullptr;
+    if constexpr (LIFT) {
+        kt = reinterpret_cast<float *>(lds + INV_LDS);
+        lw0 = kt + YBI * C;
+        lift_stage(ctx, a.lift, s, b * YBI, YBI, kt, lw0);
+    }
     // inverse-x operand B: Gx[32 xt + r][16 kb + 8 h + j]; pointwise operand A:
     // W[r][16 kb + 8 h + j]; bias of the accumulator rows c = 8 (i / 4) + 4 h + i % 4
     bf16x8_t gx[2][2], wf[2];
@@ -310,7 +373,7 @@ HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *ld
         for (int p = 0; p < 2; ++p) {
             const int yl = 2 * wave + p;
             const long row = (s * G + y0 + yl) * (long)G;
-            const __bf16 *vrow = a.v + row * C;
+            const __bf16 *vrow = LIFT ? nullptr : a.v + row * C;
 #pragma unroll
             for (int xt = 0; xt < 2; ++xt) {
                 f32x16_t d;
@@ -322,9 +385,20 @@ HZL_HD void fno_inverse_body(const Ctx &ctx, const FnoInverseArgs &a, __bf16 *ld
                     const bf16x8_t fa = load8(Bi + (yl * C + r) * BI_LD + 16 * kb + 8 * h);
                     fno_mfma(ctx, fa, gx[xt][kb], d, lane);
                 }
+                float f0 = 0.f;
+                if constexpr (LIFT)
+                    f0 = (float)a.lift.feat[(row + 32 * xt + r) * CIN];
 #pragma unroll
                 for (int kb = 0; kb < 2; ++kb) {
-                    const bf16x8_t fb = load8(vrow + (32 * xt + r) * C + 16 * kb + 8 * h);
+                    bf16x8_t fb;
+                    if constexpr (LIFT) {
+                        const int c0 = 16 * kb + 8 * h;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            fb[j] = lift_value(f0, lw0[c0 + j], kt[yl * C + c0 + j]);
+                    } else {
+                        fb = load8(vrow + (32 * xt + r) * C + 16 * kb + 8 * h);
+                    }
                     fno_mfma(ctx, wf[kb], fb, d, lane);
                 }
                 __bf16 *o = a.out + (row + 32 * xt + r) * C;
@@ -691,30 +765,5 @@ HZL_HD void fno_grid_to_increment_body(const Ctx &ctx, const GridIncrementArgs &
     }
 }
 
-// Lift of the input features to the first layer's channels: thread i writes
-// channels 8 (i % (C / 8)) .. + 8 of point i / (C / 8).
-HZL_HD void fno_lift_body(const FnoLiftArgs &a, long i) {
-    using namespace fno;
-    const long p = i / (C / 8);
-    const int c0 = (int)(i % (C / 8)) * 8;
-    if (p >= a.points)
-        return;
-    float f[CIN];
-#pragma unroll
-    for (int k = 0; k < CIN; ++k)
-        f[k] = (float)a.feat[p * CIN + k];
-    struct alignas(16) O {
-        __bf16 v[8];
-    } o;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        float x = 0.f;
-#pragma unroll
-        for (int k = 0; k < CIN; ++k)
-            x += f[k] * (float)a.Lw[(c0 + j) * CIN + k];
-        o.v[j] = (__bf16)(x + a.Lb[c0 + j]);
-    }
-    *reinterpret_cast<O *>(a.v + p * C + c0) = o;
-}
 
 } // namespace hzl_amd

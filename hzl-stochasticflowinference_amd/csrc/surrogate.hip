@@ -57,16 +57,16 @@ __global__ void k_feature(T *f, const double *nu, const double *amp, int S, int 
 }
 // Fused BF16 inference passes of a spectral layer (fno_fused.hpp): one
 // workgroup per sample.
+// LIFT: layer 0, whose input is lifted from the features inside the pass.
+template <bool LIFT>
 __global__ void __launch_bounds__(fno::THREADS) k_fno_forward(FnoForwardArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 lds[fno::FWD_LDS];
-    fno_forward_body(DevCtx{}, a, lds, (long)blockIdx.x);
+    __shared__ __attribute__((aligned(16))) __bf16 lds[LIFT ? fno::FWD_LDS_LIFT : fno::FWD_LDS];
+    fno_forward_body<LIFT>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
+template <bool LIFT>
 __global__ void __launch_bounds__(fno::THREADS) k_fno_inverse(FnoInverseArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 lds[fno::INV_LDS];
-    fno_inverse_body(DevCtx{}, a, lds, (long)blockIdx.x);
-}
-__global__ void k_fno_lift(FnoLiftArgs a) {
-    fno_lift_body(a, HZL_GID);
+    __shared__ __attribute__((aligned(16))) __bf16 lds[LIFT ? fno::INV_LDS_LIFT : fno::INV_LDS];
+    fno_inverse_body<LIFT>(DevCtx{}, a, lds, (long)blockIdx.x);
 }
 __global__ void __launch_bounds__(SG_THREADS) k_state_to_grid(StateGridArgs a) {
     __shared__ float lds[SG_LDS];
@@ -1289,11 +1289,17 @@ class Surrogate {
             w.w_T.n < (size_t)prm_.total || w.V[0]->n < (size_t)S * SV ||
             w.U[0]->n < (size_t)S * SV)
             throw std::logic_error("fused surrogate operands out of bounds");
-        FnoLiftArgs la{w.feat.p, w.w_T.p + prm_.Lw, master_.p + prm_.Lb, w.v[0]->p, SP};
-        hipLaunchKernelGGL(k_fno_lift, grid1d(SP * (C / 8)), dim3(256), 0, st_, la);
+        // layer 0 lifts its input from the features inside both of its passes
+        const FnoLift lift{w.feat.p, w.w_T.p + prm_.Lw, master_.p + prm_.Lb};
         for (int l = 0; l < s.L; ++l) {
-            FnoForwardArgs fa{w.v[l]->p, w.cst.Fx.p, w.cst.Fy2.p, w.V[l]->p, (long)S};
-            hipLaunchKernelGGL(k_fno_forward, dim3((unsigned)S), dim3(THREADS), 0, st_, fa);
+            const __bf16 *vin = l ? w.v[l]->p : nullptr;
+            FnoForwardArgs fa{vin, w.cst.Fx.p, w.cst.Fy2.p, w.V[l]->p, (long)S, lift};
+            if (l == 0)
+                hipLaunchKernelGGL(k_fno_forward<true>, dim3((unsigned)S), dim3(THREADS), 0, st_,
+                                   fa);
+            else
+                hipLaunchKernelGGL(k_fno_forward<false>, dim3((unsigned)S), dim3(THREADS), 0, st_,
+                                   fa);
             if (w.rcat.n < (size_t)((l + 1) * rper))
                 throw std::logic_error("fused surrogate mixing weights out of bounds");
             FnoMixArgs ma{w.V[l]->p, w.rcat.p + l * rper, w.U[l]->p, (long)S, s.modes()};
@@ -1301,14 +1307,19 @@ class Surrogate {
             FnoInverseArgs ia{w.U[l]->p,
                               w.cst.Gy2.p,
                               w.cst.Gx.p,
-                              w.v[l]->p,
+                              vin,
                               w.w_T.p + prm_.W[l],
                               master_.p + prm_.b[l],
                               w.v[l + 1]->p,
                               (long)S,
-                              l + 1 < s.L ? ACT_GELU : ACT_NONE};
-            hipLaunchKernelGGL(k_fno_inverse, dim3((unsigned)(S * NBLKI)), dim3(THREADS), 0, st_,
-                               ia);
+                              l + 1 < s.L ? ACT_GELU : ACT_NONE,
+                              lift};
+            if (l == 0)
+                hipLaunchKernelGGL(k_fno_inverse<true>, dim3((unsigned)(S * NBLKI)), dim3(THREADS),
+                                   0, st_, ia);
+            else
+                hipLaunchKernelGGL(k_fno_inverse<false>, dim3((unsigned)(S * NBLKI)),
+                                   dim3(THREADS), 0, st_, ia);
         }
         if (s.H != FNO_H || w.v[s.L]->n < (size_t)SP * C || w.out.n < (size_t)SP)
             throw std::logic_error("fused surrogate head operands out of bounds");

@@ -596,42 +596,56 @@ int emu_proj_head(int bf16, long M, int C, int H, const float *X, const float *Q
 
 // Fused surrogate layer passes (fno_fused.hpp) over S samples; operands
 // arrive as FP32 and are rounded to BF16, outputs return as FP32.
-int emu_fno_forward(long S, const float *v, const float *Fx, const float *Fy2, float *V) {
+static std::vector<__bf16> to_bf16(const float *x, size_t n) {
+    std::vector<__bf16> o(x ? n : 0);
+    for (size_t i = 0; i < o.size(); ++i)
+        o[i] = (__bf16)x[i];
+    return o;
+}
+
+// feat (with Lw, Lb) non-null: layer 0's pass, v lifted from the features
+// [S][G][G][CIN] inside the pass (v is then not read).
+int emu_fno_forward(long S, const float *v, const float *Fx, const float *Fy2, const float *feat,
+                    const float *Lw, const float *Lb, float *V) {
     using namespace fno;
-    auto bf = [](const float *x, size_t n) {
-        std::vector<__bf16> o(n);
-        for (size_t i = 0; i < n; ++i)
-            o[i] = (__bf16)x[i];
-        return o;
-    };
-    const auto bv = bf(v, (size_t)S * SAMPLE), bx = bf(Fx, 2 * KX * G), by = bf(Fy2, 2 * KY * 2 * G);
+    const bool lifted = feat != nullptr;
+    const auto bv = to_bf16(lifted ? nullptr : v, (size_t)S * SAMPLE), bx = to_bf16(Fx, 2 * KX * G),
+               by = to_bf16(Fy2, 2 * KY * 2 * G), bf = to_bf16(feat, (size_t)S * G * G * CIN),
+               bw = to_bf16(Lw, (size_t)C * CIN);
     std::vector<__bf16> out((size_t)S * SPECTRAL);
-    FnoForwardArgs a{bv.data(), bx.data(), by.data(), out.data(), S};
-    launch(S, THREADS, FWD_LDS * sizeof(__bf16), [&](const CpuCtx &ctx, long b, unsigned char *l) {
-        fno_forward_body(ctx, a, reinterpret_cast<__bf16 *>(l), b);
-    });
+    FnoForwardArgs a{lifted ? nullptr : bv.data(), bx.data(), by.data(), out.data(), S,
+                     FnoLift{lifted ? bf.data() : nullptr, lifted ? bw.data() : nullptr, Lb}};
+    launch(S, THREADS, (lifted ? FWD_LDS_LIFT : FWD_LDS) * sizeof(__bf16),
+           [&](const CpuCtx &ctx, long b, unsigned char *l) {
+               if (lifted)
+                   fno_forward_body<true>(ctx, a, reinterpret_cast<__bf16 *>(l), b);
+               else
+                   fno_forward_body<false>(ctx, a, reinterpret_cast<__bf16 *>(l), b);
+           });
     for (size_t i = 0; i < out.size(); ++i)
         V[i] = (float)out[i];
     return 0;
 }
 
 int emu_fno_inverse(long S, const float *U, const float *Gy2, const float *Gx, const float *v,
-                    const float *W, const float *bias, int act, float *out) {
+                    const float *W, const float *bias, int act, const float *feat,
+                    const float *Lw, const float *Lb, float *out) {
     using namespace fno;
-    auto bf = [](const float *x, size_t n) {
-        std::vector<__bf16> o(n);
-        for (size_t i = 0; i < n; ++i)
-            o[i] = (__bf16)x[i];
-        return o;
-    };
-    const auto bu = bf(U, (size_t)S * SPECTRAL), by = bf(Gy2, 2 * G * 2 * KY),
-               bx = bf(Gx, G * 2 * KX), bv = bf(v, (size_t)S * SAMPLE), bw = bf(W, C * C);
+    const bool lifted = feat != nullptr;
+    const auto bu = to_bf16(U, (size_t)S * SPECTRAL), by = to_bf16(Gy2, 2 * G * 2 * KY),
+               bx = to_bf16(Gx, G * 2 * KX), bv = to_bf16(lifted ? nullptr : v, (size_t)S * SAMPLE),
+               bw = to_bf16(W, C * C), bf = to_bf16(feat, (size_t)S * G * G * CIN),
+               bl = to_bf16(Lw, (size_t)C * CIN);
     std::vector<__bf16> o((size_t)S * SAMPLE);
-    FnoInverseArgs a{bu.data(), by.data(), bx.data(), bv.data(), bw.data(), bias, o.data(), S,
-                     act};
-    launch(S * NBLKI, THREADS, INV_LDS * sizeof(__bf16),
+    FnoInverseArgs a{bu.data(), by.data(), bx.data(), lifted ? nullptr : bv.data(), bw.data(),
+                     bias, o.data(), S, act,
+                     FnoLift{lifted ? bf.data() : nullptr, lifted ? bl.data() : nullptr, Lb}};
+    launch(S * NBLKI, THREADS, (lifted ? INV_LDS_LIFT : INV_LDS) * sizeof(__bf16),
            [&](const CpuCtx &ctx, long b, unsigned char *l) {
-               fno_inverse_body(ctx, a, reinterpret_cast<__bf16 *>(l), b);
+               if (lifted)
+                   fno_inverse_body<true>(ctx, a, reinterpret_cast<__bf16 *>(l), b);
+               else
+                   fno_inverse_body<false>(ctx, a, reinterpret_cast<__bf16 *>(l), b);
            });
     for (size_t i = 0; i < o.size(); ++i)
         out[i] = (float)o[i];
@@ -702,21 +716,6 @@ int emu_fno_mix(long S, int modes, const float *V, const float *R, float *U) {
 int emu_gelu_sg(long n, const float *x, float *y) {
     for (long i = 0; i < n; ++i)
         y[i] = gelu_sg(x[i]);
-    return 0;
-}
-
-int emu_fno_lift(long points, const float *feat, const float *Lw, const float *Lb, float *v) {
-    using namespace fno;
-    std::vector<__bf16> f((size_t)points * CIN), w(C * CIN), o((size_t)points * C);
-    for (size_t i = 0; i < f.size(); ++i)
-        f[i] = (__bf16)feat[i];
-    for (size_t i = 0; i < w.size(); ++i)
-        w[i] = (__bf16)Lw[i];
-    FnoLiftArgs a{f.data(), w.data(), Lb, o.data(), points};
-    for (long i = 0; i < points * (C / 8); ++i)
-        fno_lift_body(a, i);
-    for (size_t i = 0; i < o.size(); ++i)
-        v[i] = (float)o[i];
     return 0;
 }
 

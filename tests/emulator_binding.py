@@ -26,9 +26,10 @@ class Emulator:
                                C.c_size_t, _fp, _fp, C.c_size_t, _fp, C.c_float, C.c_float,
                                C.c_int, C.c_int]
         L.emu_fno_forward.restype = C.c_int
-        L.emu_fno_forward.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
+        L.emu_fno_forward.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp, _fp]
         L.emu_fno_inverse.restype = C.c_int
-        L.emu_fno_inverse.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int, _fp]
+        L.emu_fno_inverse.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int, _fp,
+                                      _fp, _fp, _fp]
         L.emu_fno_head.restype = C.c_int
         L.emu_fno_head.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp]
         L.emu_state_to_grid.restype = C.c_int
@@ -39,8 +40,6 @@ class Emulator:
         L.emu_fno_mix.argtypes = [C.c_long, C.c_int, _fp, _fp, _fp]
         L.emu_gelu_sg.restype = C.c_int
         L.emu_gelu_sg.argtypes = [C.c_long, _fp, _fp]
-        L.emu_fno_lift.restype = C.c_int
-        L.emu_fno_lift.argtypes = [C.c_long, _fp, _fp, _fp, _fp]
         L.emu_proj_head.restype = C.c_int
         L.emu_proj_head.argtypes = [C.c_int, C.c_long, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp,
                                     _fp]
@@ -88,14 +87,26 @@ class Emulator:
     def _f32(x):
         return np.ascontiguousarray(x, dtype=np.float32)
 
-    def fno_forward(self, v, Fx, Fy2):
-        """fno_forward_body over v [S][64][64][32]: V [S][16][64][32]."""
+    def _lift(self, lift):
+        """(feat, Lw, Lb) as float32 arrays and their pointers, or Nones."""
+        fp = C.POINTER(C.c_float)
+        if lift is None:
+            return (), (None, None, None)
+        arrs = tuple(self._f32(x) for x in lift)
+        return arrs, tuple(x.ctypes.data_as(fp) for x in arrs)
+
+    def fno_forward(self, v, Fx, Fy2, lift=None):
+        """fno_forward_body over v [S][64][64][32]: V [S][16][64][32]; with
+        lift = (feat [S][64][64][3], Lw [32][3], Lb [32]) the lifted layer-0
+        pass (v only gives S)."""
         v, Fx, Fy2 = self._f32(v), self._f32(Fx), self._f32(Fy2)
         S = v.shape[0]
         out = np.zeros((S, 16, 64, 32), dtype=np.float32)
         fp = C.POINTER(C.c_float)
+        keep, lp = self._lift(lift)
         assert self.lib.emu_fno_forward(S, v.ctypes.data_as(fp), Fx.ctypes.data_as(fp),
-                                        Fy2.ctypes.data_as(fp), out.ctypes.data_as(fp)) == 0
+                                        Fy2.ctypes.data_as(fp), *lp,
+                                        out.ctypes.data_as(fp)) == 0
         return out
 
     def fno_head(self, X, Q1, q1, Q2, q2):
@@ -148,26 +159,19 @@ class Emulator:
         assert self.lib.emu_gelu_sg(x.size, x.ctypes.data_as(fp), y.ctypes.data_as(fp)) == 0
         return y
 
-    def fno_inverse(self, U, Gy2, Gx, v, W, bias, act):
-        """fno_inverse_body: U [S][16][64][32], v [S][64][64][32] -> [S][64][64][32]."""
+    def fno_inverse(self, U, Gy2, Gx, v, W, bias, act, lift=None):
+        """fno_inverse_body: U [S][16][64][32], v [S][64][64][32] -> [S][64][64][32];
+        lift as in fno_forward."""
         U, Gy2, Gx, v = self._f32(U), self._f32(Gy2), self._f32(Gx), self._f32(v)
         W, bias = self._f32(W), self._f32(bias)
         S = v.shape[0]
         out = np.zeros_like(v)
         fp = C.POINTER(C.c_float)
+        keep, lp = self._lift(lift)
         assert self.lib.emu_fno_inverse(S, U.ctypes.data_as(fp), Gy2.ctypes.data_as(fp),
                                         Gx.ctypes.data_as(fp), v.ctypes.data_as(fp),
                                         W.ctypes.data_as(fp), bias.ctypes.data_as(fp), int(act),
-                                        out.ctypes.data_as(fp)) == 0
-        return out
-
-    def fno_lift(self, feat, Lw, Lb):
-        feat, Lw, Lb = self._f32(feat), self._f32(Lw), self._f32(Lb)
-        points = feat.shape[0]
-        out = np.zeros((points, 32), dtype=np.float32)
-        fp = C.POINTER(C.c_float)
-        assert self.lib.emu_fno_lift(points, feat.ctypes.data_as(fp), Lw.ctypes.data_as(fp),
-                                     Lb.ctypes.data_as(fp), out.ctypes.data_as(fp)) == 0
+                                        *lp, out.ctypes.data_as(fp)) == 0
         return out
 
     def proj_head(self, bf16, X, Q1, q1, Q2, q2):

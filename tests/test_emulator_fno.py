@@ -92,15 +92,54 @@ def test_inverse_pass_matches_numpy(emu, act):
     close(got, want)
 
 
-def test_lift_matches_numpy(emu):
-    rng = np.random.default_rng(13)
-    points = 300
-    feat = bf16(rng.standard_normal((points, 3))).astype(np.float64)
+def lift_case(rng, S):
+    """Features as the surrogate writes them (f0 per point, f1 per sample, f2
+    per row; BF16), lift weights, and the lifted input v in numpy:
+    v = bf16(f0 Lw0 + (f2 Lw2 + (f1 Lw1 + Lb))), each fma rounded to FP32."""
+    f0 = bf16(rng.standard_normal((S, G, G))).astype(np.float64)
+    f1 = bf16(rng.standard_normal(S)).astype(np.float64)
+    f2 = bf16(rng.standard_normal((S, G))).astype(np.float64)
+    feat = np.stack([f0, np.broadcast_to(f1[:, None, None], f0.shape),
+                     np.broadcast_to(f2[:, :, None], f0.shape)], -1)
     Lw = bf16(rng.standard_normal((C, 3))).astype(np.float64)
     Lb = rng.standard_normal(C).astype(np.float32)
-    got = emu.fno_lift(feat, Lw, Lb)
-    want = bf16(feat @ Lw.T + Lb.astype(np.float64))
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    k = f32(f2[:, :, None] * Lw[:, 2] + f32(f1[:, None, None] * Lw[:, 1] + Lb))  # [S][y][c]
+    v = bf16(f32(f0[..., None] * Lw[:, 0] + k[:, :, None, :])).astype(np.float64)
+    return feat, Lw, Lb, v
+
+
+def test_lifted_forward_pass_matches_numpy(emu):
+    """Layer 0's forward pass lifts its input from the features inside the
+    pass (the lift weights and the row constants in LDS): the DFTs of the
+    numpy-lifted input."""
+    rng = np.random.default_rng(21)
+    S = 2
+    fx, fy, _, _ = constants()
+    feat, Lw, Lb, v = lift_case(rng, S)
+    got = emu.fno_forward(np.zeros((S, G, G, C)), fx, fy, lift=(feat, Lw, Lb))
+    A = bf16(np.einsum("kx,syxc->skyc", fx, v)).astype(np.float64)
+    A2 = A.reshape(S, KX, 2, G, C).reshape(S, KX, 2 * G, C)
+    want = bf16(np.einsum("pm,skmc->skpc", fy, A2))
     close(got, want)
+
+
+def test_lifted_inverse_pass_matches_numpy(emu):
+    """Layer 0's inverse pass with the pointwise term on the lifted input:
+    rows of all four row blocks, GELU."""
+    rng = np.random.default_rng(22)
+    S = 2
+    _, _, gy, gx = constants()
+    feat, Lw, Lb, v = lift_case(rng, S)
+    U = bf16(rng.standard_normal((S, KX, 2 * KY, C))).astype(np.float64)
+    W = bf16(rng.standard_normal((C, C)) / math.sqrt(C)).astype(np.float64)
+    bias = rng.standard_normal(C).astype(np.float32)
+    got = emu.fno_inverse(U, gy, gx, np.zeros((S, G, G, C)), W, bias, 1, lift=(feat, Lw, Lb))
+    bi = np.einsum("mk,sxkc->sxmc", gy, U)
+    bi = bf16(bi.reshape(S, KX, 2, G, C).reshape(S, 2 * KX, G, C)).astype(np.float64)
+    pre = (np.einsum("xk,skyc->syxc", gx, bi) + np.einsum("syxd,cd->syxc", v, W)
+           + bias.astype(np.float64))
+    close(got, bf16(gelu(pre)))
 
 
 def test_head_matches_numpy(emu):
