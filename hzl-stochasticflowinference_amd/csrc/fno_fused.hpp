@@ -34,6 +34,7 @@
 // register i holds D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
 #pragma once
 #include "gemm_mfma.hpp"
+#include "surrogate_kernels.hpp"
 
 #include <cmath>
 
@@ -80,13 +81,16 @@ static_assert(FWD_LDS % 8 == 0 && INV_LDS % 8 == 0, "lift tables start 16-byte a
 //   v[y][x][c] = bf16(f0[y][x] Lw[c][0] + k[y][c]),
 //   k[y][c] = f1 Lw[c][1] + f2[y] Lw[c][2] + Lb[c]  (two fma),
 // with f0 the scaled state grid, f1 the sample's viscosity feature and f2 the
-// row's forcing feature (constant along a row): the passes that read it
-// recompute it from the 6-byte features instead of reading 64 bytes of
-// activations per point.
+// row's forcing feature (constant along a row), f1 and f2 rounded to BF16 as
+// the feature pass rounds them: the passes that read v recompute it from the
+// grid value instead of reading 64 bytes of activations per point.
 struct FnoLift {
-    const __bf16 *feat = nullptr; // [S][G][G][CIN]
+    const __bf16 *feat = nullptr; // [S][G][G][CIN], channel 0 (the grid) read
     const __bf16 *Lw = nullptr;   // [C][CIN]
     const float *Lb = nullptr;    // [C]
+    const double *nu = nullptr;   // [S] viscosities (feature 1)
+    const double *amp = nullptr;  // [S] forcing amplitudes, or null (feature 2)
+    int kf = 0;                   // forcing wavenumber
 };
 
 HZL_HD float lift_row_constant(float f1, float f2, float w1, float w2, float b) {
@@ -174,17 +178,17 @@ HZL_HD void fno_mfma(const Ctx &ctx, const bf16x8_t &fa, const bf16x8_t &fb, f32
 // Forward x and y transforms of sample s (workgroup of fno::THREADS lanes,
 // lds: fno::FWD_LDS bf16 elements).
 // The row constants k[y][c] of rows y0 .. y0 + rows - 1 of sample s and the
-// weights w0[c] = Lw[c][0] into LDS (f1, f2 read at each row's x = 0).
+// weights w0[c] = Lw[c][0] into LDS.
 template <class Ctx>
 HZL_HD void lift_stage(const Ctx &ctx, const FnoLift &L, long s, int y0, int rows, float *kt,
                        float *w0) {
     using namespace fno;
-    const __bf16 *fs = L.feat + s * (long)G * G * CIN;
+    const float f1 = (float)(__bf16)feature_viscosity(L.nu[s]);
     for (int e = ctx.tid(); e < rows * C; e += THREADS) {
         const int y = e / C, c = e % C;
-        const __bf16 *f = fs + (long)(y0 + y) * G * CIN;
-        kt[e] = lift_row_constant((float)f[1], (float)f[2], (float)L.Lw[c * CIN + 1],
-                                  (float)L.Lw[c * CIN + 2], L.Lb[c]);
+        const float f2 = (float)(__bf16)feature_forcing(L.amp, s, L.kf, y0 + y, G);
+        kt[e] = lift_row_constant(f1, f2, (float)L.Lw[c * CIN + 1], (float)L.Lw[c * CIN + 2],
+                                  L.Lb[c]);
     }
     for (int c = ctx.tid(); c < C; c += THREADS)
         w0[c] = (float)L.Lw[c * CIN];

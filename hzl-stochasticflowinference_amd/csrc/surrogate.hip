@@ -1184,11 +1184,10 @@ class Surrogate {
             if (!train && fused_ && fused_shape()) {
                 if (w.feat.n < (size_t)SP * s.CIN)
                     throw std::logic_error("fused surrogate features out of bounds");
+                // features 1 and 2 are computed where layer 0 lifts its input
                 state_to_grid_fused(states, S, w.feat.p, (long)G * s.CIN, s.CIN, P * s.CIN,
                                     (float)in_scale_);
-                hipLaunchKernelGGL(k_feature<T>, grid1d(SP), dim3(256), 0, st_, w.feat.p, nu, amp,
-                                   S, G, s.CIN, s.kf);
-                layers_fused(w, S);
+                layers_fused(w, S, nu, amp);
                 return;
             }
         }
@@ -1280,8 +1279,9 @@ class Surrogate {
     // fused passes (fno_fused.hpp): per layer one forward-transform pass, the
     // per-mode mixing pass and one inverse-transform pass with the
     // pointwise term, bias and activation; then the projection and head.
-    // w.feat holds the input features; the output lands in w.out.
-    void layers_fused(FnoWork<__bf16> &w, int S) {
+    // w.feat holds the grid feature (channel 0), nu and amp the samples'
+    // viscosities and forcing amplitudes; the output lands in w.out.
+    void layers_fused(FnoWork<__bf16> &w, int S, const double *nu, const double *amp) {
         using namespace fno;
         const FnoShape &s = s_;
         const long SP = (long)S * s.P(), SV = s.SV(), rper = 4L * C * C * s.modes();
@@ -1290,7 +1290,7 @@ class Surrogate {
             w.U[0]->n < (size_t)S * SV)
             throw std::logic_error("fused surrogate operands out of bounds");
         // layer 0 lifts its input from the features inside both of its passes
-        const FnoLift lift{w.feat.p, w.w_T.p + prm_.Lw, master_.p + prm_.Lb};
+        const FnoLift lift{w.feat.p, w.w_T.p + prm_.Lw, master_.p + prm_.Lb, nu, amp, s.kf};
         for (int l = 0; l < s.L; ++l) {
             const __bf16 *vin = l ? w.v[l]->p : nullptr;
             FnoForwardArgs fa{vin, w.cst.Fx.p, w.cst.Fy2.p, w.V[l]->p, (long)S, lift};

@@ -13,6 +13,16 @@ namespace hzl_amd {
 //   2: (amp_s / FNO_AMP_REF) sin(kf y), the sample's Kolmogorov forcing
 //      (amp null: FNO_AMP_REF for every sample)
 constexpr double FNO_AMP_REF = 0.1;
+
+// Features 1 and 2 of sample s, row y, before their rounding to the working type.
+HZL_HD float feature_viscosity(double nu) {
+    return (float)(log(nu) - log(1e-3));
+}
+HZL_HD float feature_forcing(const double *amp, long s, int kf, int y, int G) {
+    const double a = amp ? amp[s] / FNO_AMP_REF : 1.0;
+    return (float)(a * sin(2.0 * PI * kf * y / G));
+}
+
 template <typename T>
 HZL_HD void feature_body(T *feat, const double *nu, const double *amp, int S, int G, int CIN,
                          int kf, long i) {
@@ -21,9 +31,8 @@ HZL_HD void feature_body(T *feat, const double *nu, const double *amp, int S, in
         return;
     const long s = i / P, p = i % P;
     const int y = (int)(p / G);
-    const double a = amp ? amp[s] / FNO_AMP_REF : 1.0;
-    feat[i * CIN + 1] = (T)(float)(log(nu[s]) - log(1e-3));
-    feat[i * CIN + 2] = (T)(float)(a * sin(2.0 * PI * kf * y / G));
+    feat[i * CIN + 1] = (T)feature_viscosity(nu[s]);
+    feat[i * CIN + 2] = (T)feature_forcing(amp, s, kf, y, G);
 }
 
 // g = g_out * gelu'(pre) elementwise

@@ -603,10 +603,12 @@ static std::vector<__bf16> to_bf16(const float *x, size_t n) {
     return o;
 }
 
-// feat (with Lw, Lb) non-null: layer 0's pass, v lifted from the features
-// [S][G][G][CIN] inside the pass (v is then not read).
+// feat (with Lw, Lb, nu, amp, kf) non-null: layer 0's pass, v lifted inside
+// the pass from the grid feature (channel 0 of [S][G][G][CIN]) and the
+// samples' viscosity and forcing features (v is then not read).
 int emu_fno_forward(long S, const float *v, const float *Fx, const float *Fy2, const float *feat,
-                    const float *Lw, const float *Lb, float *V) {
+                    const float *Lw, const float *Lb, const double *nu, const double *amp,
+                    int kf, float *V) {
     using namespace fno;
     const bool lifted = feat != nullptr;
     const auto bv = to_bf16(lifted ? nullptr : v, (size_t)S * SAMPLE), bx = to_bf16(Fx, 2 * KX * G),
@@ -614,7 +616,8 @@ int emu_fno_forward(long S, const float *v, const float *Fx, const float *Fy2, c
                bw = to_bf16(Lw, (size_t)C * CIN);
     std::vector<__bf16> out((size_t)S * SPECTRAL);
     FnoForwardArgs a{lifted ? nullptr : bv.data(), bx.data(), by.data(), out.data(), S,
-                     FnoLift{lifted ? bf.data() : nullptr, lifted ? bw.data() : nullptr, Lb}};
+                     FnoLift{lifted ? bf.data() : nullptr, lifted ? bw.data() : nullptr, Lb, nu,
+                             amp, kf}};
     launch(S, THREADS, (lifted ? FWD_LDS_LIFT : FWD_LDS) * sizeof(__bf16),
            [&](const CpuCtx &ctx, long b, unsigned char *l) {
                if (lifted)
@@ -629,7 +632,8 @@ int emu_fno_forward(long S, const float *v, const float *Fx, const float *Fy2, c
 
 int emu_fno_inverse(long S, const float *U, const float *Gy2, const float *Gx, const float *v,
                     const float *W, const float *bias, int act, const float *feat,
-                    const float *Lw, const float *Lb, float *out) {
+                    const float *Lw, const float *Lb, const double *nu, const double *amp,
+                    int kf, float *out) {
     using namespace fno;
     const bool lifted = feat != nullptr;
     const auto bu = to_bf16(U, (size_t)S * SPECTRAL), by = to_bf16(Gy2, 2 * G * 2 * KY),
@@ -639,7 +643,8 @@ int emu_fno_inverse(long S, const float *U, const float *Gy2, const float *Gx, c
     std::vector<__bf16> o((size_t)S * SAMPLE);
     FnoInverseArgs a{bu.data(), by.data(), bx.data(), lifted ? nullptr : bv.data(), bw.data(),
                      bias, o.data(), S, act,
-                     FnoLift{lifted ? bf.data() : nullptr, lifted ? bl.data() : nullptr, Lb}};
+                     FnoLift{lifted ? bf.data() : nullptr, lifted ? bl.data() : nullptr, Lb, nu,
+                             amp, kf}};
     launch(S * NBLKI, THREADS, (lifted ? INV_LDS_LIFT : INV_LDS) * sizeof(__bf16),
            [&](const CpuCtx &ctx, long b, unsigned char *l) {
                if (lifted)

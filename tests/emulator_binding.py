@@ -26,10 +26,11 @@ class Emulator:
                                C.c_size_t, _fp, _fp, C.c_size_t, _fp, C.c_float, C.c_float,
                                C.c_int, C.c_int]
         L.emu_fno_forward.restype = C.c_int
-        L.emu_fno_forward.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp, _fp]
+        L.emu_fno_forward.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp, _dp, _dp,
+                                      C.c_int, _fp]
         L.emu_fno_inverse.restype = C.c_int
         L.emu_fno_inverse.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int, _fp,
-                                      _fp, _fp, _fp]
+                                      _fp, _fp, _dp, _dp, C.c_int, _fp]
         L.emu_fno_head.restype = C.c_int
         L.emu_fno_head.argtypes = [C.c_long, _fp, _fp, _fp, _fp, _fp, _fp]
         L.emu_state_to_grid.restype = C.c_int
@@ -88,17 +89,21 @@ class Emulator:
         return np.ascontiguousarray(x, dtype=np.float32)
 
     def _lift(self, lift):
-        """(feat, Lw, Lb) as float32 arrays and their pointers, or Nones."""
-        fp = C.POINTER(C.c_float)
+        """lift = (feat, Lw, Lb, nu, amp, kf): the arrays kept alive and the
+        argument list (float32 / float64 pointers, kf), or Nones."""
+        fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
         if lift is None:
-            return (), (None, None, None)
-        arrs = tuple(self._f32(x) for x in lift)
-        return arrs, tuple(x.ctypes.data_as(fp) for x in arrs)
+            return (), (None, None, None, None, None, 0)
+        feat, Lw, Lb, nu, amp, kf = lift
+        f = tuple(self._f32(x) for x in (feat, Lw, Lb))
+        d = tuple(np.ascontiguousarray(x, dtype=np.float64) for x in (nu, amp))
+        return f + d, tuple(x.ctypes.data_as(fp) for x in f) + tuple(
+            x.ctypes.data_as(dp) for x in d) + (int(kf),)
 
     def fno_forward(self, v, Fx, Fy2, lift=None):
         """fno_forward_body over v [S][64][64][32]: V [S][16][64][32]; with
         lift = (feat [S][64][64][3], Lw [32][3], Lb [32]) the lifted layer-0
-        pass (v only gives S)."""
+        pass (v only gives S; nu, amp [S] and kf give features 1 and 2)."""
         v, Fx, Fy2 = self._f32(v), self._f32(Fx), self._f32(Fy2)
         S = v.shape[0]
         out = np.zeros((S, 16, 64, 32), dtype=np.float32)

@@ -92,13 +92,20 @@ def test_inverse_pass_matches_numpy(emu, act):
     close(got, want)
 
 
-def lift_case(rng, S):
-    """Features as the surrogate writes them (f0 per point, f1 per sample, f2
-    per row; BF16), lift weights, and the lifted input v in numpy:
-    v = bf16(f0 Lw0 + (f2 Lw2 + (f1 Lw1 + Lb))), each fma rounded to FP32."""
+def lift_case(rng, S, kf=4):
+    """Inputs of layer 0's lifted passes and the lifted input v in numpy: the
+    grid feature f0 per point (BF16), per-sample viscosities and forcing
+    amplitudes whose features f1 = log nu - log 1e-3 and f2 = (amp / 0.1)
+    sin(2 pi kf y / G) are rounded to FP32 then BF16 (the feature pass's
+    rounding), lift weights, and v = bf16(f0 Lw0 + (f2 Lw2 + (f1 Lw1 + Lb))),
+    each fma rounded to FP32."""
     f0 = bf16(rng.standard_normal((S, G, G))).astype(np.float64)
-    f1 = bf16(rng.standard_normal(S)).astype(np.float64)
-    f2 = bf16(rng.standard_normal((S, G))).astype(np.float64)
+    nu = np.exp(rng.uniform(np.log(5e-4), np.log(2e-3), S))
+    amp = rng.uniform(0.05, 0.4, S)
+    f1 = bf16(np.float32(np.log(nu) - np.log(1e-3))).astype(np.float64)
+    y = np.arange(G)
+    f2 = bf16((amp[:, None] / 0.1 * np.sin(2.0 * np.pi * kf * y[None, :] / G)).astype(
+        np.float32)).astype(np.float64)
     feat = np.stack([f0, np.broadcast_to(f1[:, None, None], f0.shape),
                      np.broadcast_to(f2[:, :, None], f0.shape)], -1)
     Lw = bf16(rng.standard_normal((C, 3))).astype(np.float64)
@@ -106,7 +113,10 @@ def lift_case(rng, S):
     f32 = lambda x: x.astype(np.float32).astype(np.float64)
     k = f32(f2[:, :, None] * Lw[:, 2] + f32(f1[:, None, None] * Lw[:, 1] + Lb))  # [S][y][c]
     v = bf16(f32(f0[..., None] * Lw[:, 0] + k[:, :, None, :])).astype(np.float64)
-    return feat, Lw, Lb, v
+    # channels 1 and 2 of feat are not read by the passes: poison them
+    feat = feat.copy()
+    feat[..., 1:] = np.nan
+    return (feat, Lw, Lb, nu, amp, kf), v
 
 
 def test_lifted_forward_pass_matches_numpy(emu):
@@ -116,8 +126,8 @@ def test_lifted_forward_pass_matches_numpy(emu):
     rng = np.random.default_rng(21)
     S = 2
     fx, fy, _, _ = constants()
-    feat, Lw, Lb, v = lift_case(rng, S)
-    got = emu.fno_forward(np.zeros((S, G, G, C)), fx, fy, lift=(feat, Lw, Lb))
+    lift, v = lift_case(rng, S)
+    got = emu.fno_forward(np.zeros((S, G, G, C)), fx, fy, lift=lift)
     A = bf16(np.einsum("kx,syxc->skyc", fx, v)).astype(np.float64)
     A2 = A.reshape(S, KX, 2, G, C).reshape(S, KX, 2 * G, C)
     want = bf16(np.einsum("pm,skmc->skpc", fy, A2))
@@ -130,11 +140,11 @@ def test_lifted_inverse_pass_matches_numpy(emu):
     rng = np.random.default_rng(22)
     S = 2
     _, _, gy, gx = constants()
-    feat, Lw, Lb, v = lift_case(rng, S)
+    lift, v = lift_case(rng, S)
     U = bf16(rng.standard_normal((S, KX, 2 * KY, C))).astype(np.float64)
     W = bf16(rng.standard_normal((C, C)) / math.sqrt(C)).astype(np.float64)
     bias = rng.standard_normal(C).astype(np.float32)
-    got = emu.fno_inverse(U, gy, gx, np.zeros((S, G, G, C)), W, bias, 1, lift=(feat, Lw, Lb))
+    got = emu.fno_inverse(U, gy, gx, np.zeros((S, G, G, C)), W, bias, 1, lift=lift)
     bi = np.einsum("mk,sxkc->sxmc", gy, U)
     bi = bf16(bi.reshape(S, KX, 2, G, C).reshape(S, 2 * KX, G, C)).astype(np.float64)
     pre = (np.einsum("xk,skyc->syxc", gx, bi) + np.einsum("syxd,cd->syxc", v, W)
