@@ -1471,22 +1471,16 @@ template <typename R> class Engine final : public EngineBase {
                   "filter mean");
         }
         check(hipStreamSynchronize(stream_), "particle filter");
-        bool any_resampled = false;
-        for (size_t f = 0; f < pf_filters_; ++f) {
+        for (size_t f = 0; f < pf_filters_; ++f)
             if (record)
                 for (int k = 0; k < 4; ++k)
                     record[4 * f + k] = st[8 * f + k];
-            any_resampled = any_resampled || st[8 * f + 3] != 0;
-        }
         if (mean_field)
             half_to_full<double>(n_, reinterpret_cast<const cx<double> *>(mean.data()),
                                  mean_field);
         ++pf_group_;
-        // refresh the host copy of the member keys (children were reseeded)
-        if (any_resampled)
-            check(hipMemcpy(keys_.data(), d_keys_.p, B_ * sizeof(uint64_t),
-                            hipMemcpyDeviceToHost),
-                  "keys download");
+        // the children's keys were reseeded on the device; d_keys_ is the
+        // authoritative copy (keys_ only stages whole or single-member uploads)
     }
 
     void pf_result(double *log_likelihood, size_t *resampling_events) override {
@@ -2269,6 +2263,8 @@ template <typename R> class Engine final : public EngineBase {
     DeviceBuffer<double> d_nu_, d_alpha_, d_sigma_, d_tau_, d_amp_;
     DeviceBuffer<uint64_t> d_keys_;
     std::vector<double> nu_, alpha_, sigma_, tau_, amp_;
+    // staging for key uploads (creation, pf_begin with new keys, reseed_future);
+    // resampling reseeds children on the device only, so d_keys_ is authoritative
     std::vector<uint64_t> keys_;
     // OU forcing process (set_forcing_process): stream words [B][MT_N + 1],
     // coefficients [B][P] in the reference's pair order, their resampling
