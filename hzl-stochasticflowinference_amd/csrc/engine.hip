@@ -110,9 +110,10 @@ __global__ void __launch_bounds__(256) k_pf_normalize(PfControl a) {
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_pf_mean_partial(int B, long spec, const cx<R> *w0,
-                                                         const double *wn, double *part) {
-    pf_mean_partial_body<R>(B, spec, w0, wn, part, (long)blockIdx.x * 256 + threadIdx.x,
+__global__ void __launch_bounds__(256) k_pf_mean_partial(int B, long spec, int per,
+                                                         const cx<R> *w0, const double *wn,
+                                                         double *part) {
+    pf_mean_partial_body<R>(B, spec, per, w0, wn, part, (long)blockIdx.x * 256 + threadIdx.x,
                             (int)blockIdx.y);
 }
 __global__ void __launch_bounds__(256) k_pf_mean_finish(long spec, int chunks,
@@ -1372,7 +1373,8 @@ template <typename R> class Engine final : public EngineBase {
         pf_stats_ = DeviceBuffer<double>(8 * filters);
         pf_anc_ = DeviceBuffer<int>(B_);
         pf_mean_ = DeviceBuffer<double>(2 * spec_);
-        pf_mean_part_ = DeviceBuffer<double>(2 * spec_ * ((B_ + kPfMeanChunk - 1) / kPfMeanChunk));
+        pf_mean_part_ = DeviceBuffer<double>(
+            2 * spec_ * ((B_ + pf_mean_chunk((long)B_) - 1) / pf_mean_chunk((long)B_)));
         std::vector<double> w(B_, 1.0 / (double)P), st(8 * filters, 0.0), cs(n_), sn(n_);
         upload(pf_wstate_.p, w.data(), B_);
         upload(pf_stats_.p, st.data(), 8 * filters);
@@ -1992,10 +1994,12 @@ template <typename R> class Engine final : public EngineBase {
         hipLaunchKernelGGL(k_pf_normalize, dim3((unsigned)pf_filters_), dim3(256), 0, stream_, c);
         if (with_mean) {
             const long blocks = ((long)spec_ + 255) / 256;
-            const int chunks = (int)((B_ + kPfMeanChunk - 1) / kPfMeanChunk);
+            const int per = pf_mean_chunk((long)B_);
+            const int chunks = (int)((B_ + per - 1) / per);
             hipLaunchKernelGGL(k_pf_mean_partial<R>, dim3((unsigned)blocks, (unsigned)chunks),
-                               dim3(256), 0, stream_, (int)B_, (long)spec_, (const cx<R> *)w0_.p,
-                               (const double *)pf_wnorm_.p, pf_mean_part_.p);
+                               dim3(256), 0, stream_, (int)B_, (long)spec_, per,
+                               (const cx<R> *)w0_.p, (const double *)pf_wnorm_.p,
+                               pf_mean_part_.p);
             hipLaunchKernelGGL(k_pf_mean_finish, dim3((unsigned)blocks), dim3(256), 0, stream_,
                                (long)spec_, chunks, (const double *)pf_mean_part_.p, pf_mean_.p);
         }

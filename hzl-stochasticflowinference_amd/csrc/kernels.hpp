@@ -1562,15 +1562,19 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
 // (src/filter.cpp:94-100), in two launches: sums over the particles
 // [c per, (c + 1) per) of chunk c in order, then the chunks in order (the
 // reference's running sum regrouped: results agree to FP64 rounding).  part
-// holds 2 doubles per (chunk, mode).
-constexpr int kPfMeanChunk = 32;
+// holds 2 doubles per (chunk, mode).  per: 32, or more when that would give
+// more than 65535 chunks (the grid's second dimension).
+HZL_HD int pf_mean_chunk(long B) {
+    const long per = (B + 65534) / 65535;
+    return per > 32 ? (int)per : 32;
+}
 
 template <typename R>
-HZL_HD void pf_mean_partial_body(int B, long spec, const cx<R> *w0, const double *wn,
+HZL_HD void pf_mean_partial_body(int B, long spec, int per, const cx<R> *w0, const double *wn,
                                  double *part, long e, int c) {
     if (e >= spec)
         return;
-    const int lo = c * kPfMeanChunk, hi = lo + kPfMeanChunk < B ? lo + kPfMeanChunk : B;
+    const int lo = c * per, hi = lo + per < B ? lo + per : B;
     double re = 0, im = 0;
     for (int m = lo; m < hi; ++m) {
         const cx<R> v = w0[(long)m * spec + e];

@@ -642,9 +642,10 @@ class Surrogate {
     double spread(EngineBase &eng) {
         const EngineViews ev = check_engine(eng);
         const long spec = s_.spec();
-        // members in chunks of 32 over the second grid dimension: enough
-        // workgroups to fill the device at a few hundred members
-        constexpr int per = 32;
+        // members in chunks of 32 (more beyond 2M members: the second grid
+        // dimension holds at most 65535) over the second grid dimension:
+        // enough workgroups to fill the device at a few hundred members
+        const int per = std::max(32, (ev.B + 65534) / 65535);
         const int chunks = (ev.B + per - 1) / per;
         spread_.alloc((size_t)spec);
         spread_part_.alloc(3 * (size_t)chunks * (size_t)spec);
