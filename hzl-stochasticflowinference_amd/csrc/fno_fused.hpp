@@ -69,6 +69,7 @@ constexpr long SPECTRAL = (long)KX * 2 * KY * C;
 static_assert(G % YB == 0 && G % YBI == 0, "row blocks must tile the grid");
 static_assert(YB == THREADS / 64, "one row per wavefront in the forward x transform");
 static_assert(2 * KX == 32 && C == 32 && 2 * KY == 64, "tiles are 32 x 32");
+static_assert(G == 64, "the vT swizzle permutes x within 64 columns");
 static_assert(2 * KX == 4 * (THREADS / 64), "four y-transform tiles per wavefront");
 static_assert(KX == 2 * (THREADS / 64) && YBI == 2 * (THREADS / 64),
               "two kx and two rows per wavefront in the inverse pass");
@@ -195,6 +196,15 @@ HZL_HD void lift_stage(const Ctx &ctx, const FnoLift &L, long s, int y0, int row
     ctx.sync();
 }
 
+// Position of column x of channel c in a transposed row of vT: x XOR 16 (c / 8).
+// The staging stores of a wavefront write 8 consecutive x pairs for the four
+// channel groups c / 8 at once; the row pitch (a multiple of 16 bytes, for the
+// 16-byte operand loads) maps the groups to the same LDS banks, and the XOR
+// moves them to four disjoint sets of banks.  Runs of 8 x stay contiguous.
+HZL_HD int vt_swizzle(int x, int c) {
+    return x ^ (16 * ((c / 8) % 4));
+}
+
 template <bool LIFT = false, class Ctx>
 HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *lds, long s) {
     using namespace fno;
@@ -249,11 +259,12 @@ HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *ld
                 const float *kr = kt + (b * YB + y) * C + c0;
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    vT[(y * C + c0 + j) * VT_LD + x] = lift_value(f0, lw0[c0 + j], kr[j]);
+                    vT[(y * C + c0 + j) * VT_LD + vt_swizzle(x, c0)] =
+                        lift_value(f0, lw0[c0 + j], kr[j]);
             } else {
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    vT[(y * C + c0 + j) * VT_LD + x] = pre[q][j];
+                    vT[(y * C + c0 + j) * VT_LD + vt_swizzle(x, c0)] = pre[q][j];
             }
         }
         ctx.sync();
@@ -267,7 +278,7 @@ HZL_HD void fno_forward_body(const Ctx &ctx, const FnoForwardArgs &a, __bf16 *ld
                 d[i] = 0.f;
 #pragma unroll
             for (int kb = 0; kb < G / 16; ++kb) {
-                const bf16x8_t fb = load8(vT + (wave * C + r) * VT_LD + 16 * kb + 8 * h);
+                const bf16x8_t fb = load8(vT + (wave * C + r) * VT_LD + vt_swizzle(16 * kb + 8 * h, r));
                 fno_mfma(ctx, fx[kb], fb, d, lane);
             }
 #pragma unroll
