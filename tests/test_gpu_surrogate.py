@@ -344,6 +344,26 @@ def test_spread_and_solver_fallback(pkg, orc):
         assert a.step_index == 2 * steps
 
 
+def test_async_judge_rejects_unusable_judge_ensembles(pkg, orc):
+    """judge_start checks before it enqueues anything: a judge ensemble with
+    more members than the ensemble, the ensemble itself, an FP64 one and one
+    running the OU forcing process are refused, and nothing is left in flight."""
+    cfg = SMALL
+    s = shape_of(cfg)
+    with pkg.Surrogate(**cfg, seed=9) as sur, \
+            pkg.Ensemble(s.n, 4, "fp32", sigma=0.05, seed=6) as ens, \
+            pkg.Ensemble(s.n, 8, "fp32", sigma=0.0, seed=7) as big, \
+            pkg.Ensemble(s.n, 2, "fp64", sigma=0.0, seed=8) as f64, \
+            pkg.Ensemble(s.n, 2, "fp32", sigma=0.0, seed=9) as ou:
+        sur.set_scales(1.0, 1.0)
+        ens.init_grf()
+        ou.set_forcing_process("ou")
+        for judge in (big, ens, f64, ou):
+            with pytest.raises(pkg.HzlInvalidArgument):
+                sur.judge_start(ens, judge, 2e-3, 5, "fp32")
+        assert sur.judge_poll(wait=True) == (False, -1.0)
+
+
 @pytest.mark.parametrize("B", [1, 100, 1024])
 def test_spread_over_member_chunks(pkg, orc, B):
     """The spread sums members in chunks of 32 (a ragged last chunk at 100)
