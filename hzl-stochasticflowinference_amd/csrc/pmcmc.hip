@@ -12,8 +12,9 @@
 //   accept iff the proposed density is finite and
 //   log(max(u, DBL_MIN)) < min(0, density' - density); on rejection the
 //   current likelihood estimate and its seed are kept (pseudo-marginal).
-// The parameter vector is described by a Model: BASELINE's q = (log nu, log
-// Kolmogorov amplitude) with a log-uniform prior, or the reference's
+// The parameter vector is described by a Model: the viscosity and forcing
+// amplitude posterior's q = (log nu, log Kolmogorov amplitude) with a
+// log-uniform prior, or the reference's
 // q = (log nu, log alpha, log sigma, log tau) with its bounded Gaussian prior
 // (stochastic_log_prior, src/filter.cpp:127-140; decode, :15-24).  The
 // explicit-step guard dt (2 nu K^2 + alpha) > 2.5 gives a -inf likelihood

@@ -7,7 +7,7 @@
 // file only calls its public C++ API (include/hzl/*.hpp) so that the Python
 // tests and the golden-vector script can drive it through ctypes.
 //
-// The BASELINE physics (Kolmogorov forcing, shell white noise, Gaussian random
+// The engine's physics (Kolmogorov forcing, shell white noise, Gaussian random
 // field) is injected through the reference's public API: a frozen OU process
 // (sigma = 0, tau = 1e300, so transition_ou leaves the forcing unchanged) and
 // Solver::update_latent_state(coefficients(), F_step) before every step

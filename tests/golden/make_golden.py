@@ -5,12 +5,12 @@ reference's own sources).  Run in the container that holds the reference:
     python tests/golden/make_golden.py
 
 Each fixture stores inputs and reference outputs only (numpy .npz, no pickles)
-plus a JSON sidecar with the recipe and sha256.  The BASELINE physics is driven
+plus a JSON sidecar with the recipe and sha256.  The engine's physics is driven
 through the reference's public API (frozen OU forcing + update_latent_state
 before every step; see oracle/ref_shim.cpp).
 
 Fixtures
-  config0_64.npz   BASELINE config 0: 64x64, nu 1e-3, dt 1e-3, alpha 0,
+  trajectory_64.npz 64x64, nu 1e-3, dt 1e-3, alpha 0,
                    0.1 sin(4y) + white noise sigma 0.05 on 1<=|k|<=4, GRF start
                    (member 0 of seed 20261203); reference states at steps
                    0 / 100 / 1000 as retained half planes [kx 0..K][ky -K..K],
@@ -28,7 +28,7 @@ Fixtures
                    pf_deterministic_16 dataset at four parameter vectors, strides 7, 4,
                    and its likelihood_gradient (forward sensitivities) at the same vectors.
 
-Usage: python tests/golden/make_golden.py [config0 drift32 pf16 checkpoint16 adjoint16]
+Usage: python tests/golden/make_golden.py [trajectory64 drift32 pf16 checkpoint16 adjoint16]
 """
 from __future__ import annotations
 
@@ -65,7 +65,7 @@ def save(name: str, recipe: dict, **arrays) -> None:
     print("wrote", path, digest[:16])
 
 
-def config0(orc: Oracle, ref: Reference) -> None:
+def trajectory64(orc: Oracle, ref: Reference) -> None:
     n, nu, dt, seed = 64, 1e-3, 1e-3, 20261203
     key = orc.stream_seed(seed, 0)
     w0 = orc.grf(n, key)
@@ -74,7 +74,7 @@ def config0(orc: Oracle, ref: Reference) -> None:
     w1000 = ref.run(w100, nu, 0.0, dt, 900, F[100:], per_step=True)
     d = [ref.diagnostics(w, nu, 0.0) for w in (w0, w100, w1000)]
     diag = np.array([[x["energy"], x["enstrophy"], x["dissipation"], x["cfl_rate"]] for x in d])
-    save("config0_64",
+    save("trajectory_64",
          dict(recipe="reference Solver, frozen OU + update_latent_state(F_step) per step",
               n=n, nu=nu, alpha=0.0, dt=dt, seed=seed, member=0, amp=0.1, kf=4, sigma=0.05,
               kmin=1, kmax=4, steps=[0, 100, 1000],
@@ -261,10 +261,10 @@ def adjoint16(ref: Reference) -> None:
 
 def main() -> None:
     orc, ref = Oracle(), Reference()
-    which = sys.argv[1:] or ["config0", "drift32", "pf16", "checkpoint16", "adjoint16", "ou16",
+    which = sys.argv[1:] or ["trajectory64", "drift32", "pf16", "checkpoint16", "adjoint16", "ou16",
                              "cli16"]
-    if "config0" in which:
-        config0(orc, ref)
+    if "trajectory64" in which:
+        trajectory64(orc, ref)
     if "drift32" in which:
         drift32(ref)
     if "pf16" in which:

@@ -268,7 +268,7 @@ def oracle_pmcmc(orc, initial, obs, dt, q0, samples, scale, seed, lower, upper, 
     """Reference particle_metropolis (src/filter.cpp:166-203) per chain.
 
     prior "log_uniform": q = (log nu, log amplitude), uniform in [lower, upper],
-    alpha / sigma / tau fixed, Kolmogorov amplitude exp(q1) (BASELINE's chains);
+    alpha / sigma / tau fixed, Kolmogorov amplitude exp(q1) (the viscosity / amplitude chains);
     prior "stochastic": q = (log nu, log alpha, log sigma, log tau) with the
     reference's stochastic_log_prior and no Kolmogorov term (the reference's own
     sampler; use forcing="ou").  Chain c's stream seed s is chain_seeds[c] when

@@ -166,9 +166,10 @@ def test_nonsymmetric_forcing_rejected(pkg):
             e.step_forced(0.01, F)
 
 
-def test_golden_config0_reference_trajectory(pkg):
-    """BASELINE config 0 at 64^2 against the reference solver's own trajectory."""
-    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "config0_64.npz"))
+def test_golden_reference_trajectory_64(pkg):
+    """The 64^2 trajectory (nu = dt = 1e-3, Kolmogorov 0.1) against the reference
+    solver's own."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "trajectory_64.npz"))
     n, nu, dt = 64, 1e-3, 1e-3
     errs = {}
     for precision in ("fp64", "fp32"):

@@ -1,13 +1,13 @@
-"""The benchmark configurations' shapes checked on the GPU, not only timed.
+"""The benchmark workloads' shapes checked on the GPU, not only timed.
 
-* config 1: one batch (MFMA) EnKF analysis of 1024 FP32 members at 128^2 with
+* the EnKF workload: one batch (MFMA) analysis of 1024 FP32 members at 128^2 with
   64 observations against a dense FP64 closed form computed on the host from
   the downloaded forecast states and the same perturbations (the split-K
   products over the 7310-long state axis at their real size);
-* config 2: the joint particle filter over 256^2 and 512^2 particles (FP32,
+* the two-grid workload: the joint particle filter over 256^2 and 512^2 particles (FP32,
   the three-pass transforms at 512^2, the cross-grid gather on resampling)
   against the FP64 oracle's joint filter;
-* config 4: batched pMCMC at 256^2 (FP32) against the oracle's chains.
+* the pMCMC workload: batched chains at 256^2 (FP32) against the oracle's chains.
 """
 from __future__ import annotations
 

@@ -193,9 +193,10 @@ def test_inviscid_invariants(orc, ref):
     assert np.abs(phys.imag).max() < 1e-12
 
 
-def test_trajectory_matches_reference_golden_config0(orc):
-    """BASELINE config 0 (64^2, 1000 FP64 steps) against the reference's trajectory."""
-    g = np.load(os.path.join(GOLDEN, "config0_64.npz"))
+def test_trajectory_matches_reference_golden_64(orc):
+    """The 64^2 trajectory (nu = dt = 1e-3, Kolmogorov 0.1, 1000 FP64 steps) against
+    the reference's own."""
+    g = np.load(os.path.join(GOLDEN, "trajectory_64.npz"))
     w0 = full_from_half(g["w0"], 64)
     key = orc.stream_seed(20261203, 0)
     assert rel(half(orc.grf(64, key)), g["w0"]) < 1e-15
