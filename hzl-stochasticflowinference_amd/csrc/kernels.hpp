@@ -1372,6 +1372,9 @@ template <typename R>
 HZL_HD double point_value(const PfArgs<R> &a, const cx<R> *w, int x, int y) {
     const int KP = a.K + 1;
     double s = 0;
+    // unrolled so that several terms' loads are in flight at once; the sum
+    // keeps its order
+#pragma unroll 8
     for (int kx = 0; kx <= a.K; ++kx) {
         const cx<R> c = w[tiled(KP, kx, y)];
         const int j = (int)(((long)kx * x) % a.N);
