@@ -1579,6 +1579,7 @@ HZL_HD void pf_mean_partial_body(int B, long spec, int per, const cx<R> *w0, con
         return;
     const int lo = c * per, hi = lo + per < B ? lo + per : B;
     double re = 0, im = 0;
+#pragma unroll 8
     for (int m = lo; m < hi; ++m) {
         const cx<R> v = w0[(long)m * spec + e];
         re += wn[m] * (double)v.x;
@@ -1592,6 +1593,7 @@ HZL_HD void pf_mean_finish_body(long spec, int chunks, const double *part, doubl
     if (e >= spec)
         return;
     double re = 0, im = 0;
+#pragma unroll 8
     for (int c = 0; c < chunks; ++c) {
         re += part[2 * ((long)c * spec + e)];
         im += part[2 * ((long)c * spec + e) + 1];

@@ -235,6 +235,7 @@ HZL_HD void spread_partial_body(const cx<float> *w0, int B, long spec, int per, 
     const int lo = c * per, hi = lo + per < B ? lo + per : B;
     const cx<float> ref = w0[e];
     double sr = 0, si = 0, ss = 0;
+#pragma unroll 8
     for (int m = lo; m < hi; ++m) {
         const cx<float> v = w0[(long)m * spec + e];
         const double dr = (double)v.x - (double)ref.x, di = (double)v.y - (double)ref.y;
@@ -257,6 +258,7 @@ HZL_HD void spread_finish_body(int B, int K, int chunks, const double *part, dou
     if (e >= spec)
         return;
     double sr = 0, si = 0, ss = 0;
+#pragma unroll 8
     for (int c = 0; c < chunks; ++c) {
         const double *p = part + 3 * ((long)c * spec + e);
         sr += p[0];
