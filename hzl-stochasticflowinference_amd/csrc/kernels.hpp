@@ -1516,13 +1516,17 @@ HZL_HD void pf_normalize_body(const Ctx &ctx, const PfControl &a, double *red, i
             red[0] = uniforms[(int)stats[6]];
             stats[6] += 1;
             stats[5] += 1;
+            // unrolled so that the loads of several weights are in flight
+            // ahead of the chain of additions (whose order is the reference's)
             double cum = 0;
             if (local) {
+#pragma unroll 8
                 for (int i = 0; i < P; ++i) {
                     cum += local[i];
                     local[i] = cum;
                 }
             } else {
+#pragma unroll 8
                 for (int i = 0; i < P; ++i) {
                     cum += wnorm[i];
                     wstate[i] = cum;
