@@ -26,7 +26,9 @@
 //
 // Intermediates are rounded to BF16 where the generic path stores them (the
 // x-transformed rows A, the spectral V and U, the inverse-y rows Bi), so both
-// paths compute the same quantities to FP32 summation order.  Shapes are
+// paths compute the same quantities to FP32 summation order, except that the
+// activation here is gelu_sg (within 2.6e-5 of the generic path's erf GELU)
+// and layer 0's input is lifted where it is read (FnoLift).  Shapes are
 // compile-time: the surrogate's 64^2 grid, 32 channels, 16 x 32 retained modes.
 //
 // MFMA v_mfma_f32_32x32x16_bf16 lane layout (gemm_mfma.hpp): lane l supplies

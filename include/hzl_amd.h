@@ -578,8 +578,9 @@ int hzl_surrogate_set_graph(hzl_surrogate_t h, int enabled);
  * layer runs as one fused forward-transform pass and one fused inverse-
  * transform pass (MFMA, one workgroup per sample, intermediates in LDS) around
  * the per-mode mixing product; disabled, as the five batched products of the
- * training path.  Both round the same intermediates to BF16.  No counterpart
- * in the reference. */
+ * training path.  Both round the same intermediates to BF16; the fused passes
+ * evaluate GELU by a sigmoid form within 2.6e-5 of the exact erf form.  No
+ * counterpart in the reference. */
 int hzl_surrogate_set_fused(hzl_surrogate_t h, int enabled);
 
 /* The error estimate (negative until evaluated or set). */
