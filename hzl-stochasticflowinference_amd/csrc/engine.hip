@@ -567,6 +567,8 @@ template <typename R> class Engine final : public EngineBase {
             (void)hipEventDestroy(obs_event_);
         if (obs_host_)
             (void)hipHostFree(obs_host_);
+        if (pf_host_)
+            (void)hipHostFree(pf_host_);
     }
 
     // One observation group into device memory with a single asynchronous
@@ -1373,6 +1375,12 @@ template <typename R> class Engine final : public EngineBase {
         pf_stats_ = DeviceBuffer<double>(8 * filters);
         pf_anc_ = DeviceBuffer<int>(B_);
         pf_mean_ = DeviceBuffer<double>(2 * spec_);
+        if (pf_host_)
+            (void)hipHostFree(pf_host_);
+        pf_host_ = nullptr;
+        check(hipHostMalloc(reinterpret_cast<void **>(&pf_host_),
+                            (8 * filters + 2 * spec_) * sizeof(double)),
+              "hipHostMalloc");
         pf_mean_part_ = DeviceBuffer<double>(
             2 * spec_ * ((B_ + pf_mean_chunk((long)B_) - 1) / pf_mean_chunk((long)B_)));
         std::vector<double> w(B_, 1.0 / (double)P), st(8 * filters, 0.0), cs(n_), sn(n_);
@@ -1461,25 +1469,23 @@ template <typename R> class Engine final : public EngineBase {
         }
         check(hipGetLastError(), "particle filter kernels");
         mix_valid_ = false;
-        std::vector<double> st(8 * pf_filters_);
-        check(hipMemcpyAsync(st.data(), pf_stats_.p, st.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, stream_),
+        // statistics and mean land in pinned staging (pf_begin): direct copies
+        const size_t nst = 8 * pf_filters_;
+        double *st = pf_host_, *mean = pf_host_ + nst;
+        check(hipMemcpyAsync(st, pf_stats_.p, nst * sizeof(double), hipMemcpyDeviceToHost,
+                             stream_),
               "filter statistics");
-        std::vector<double> mean;
-        if (mean_field) {
-            mean.resize(2 * spec_);
-            check(hipMemcpyAsync(mean.data(), pf_mean_.p, mean.size() * sizeof(double),
+        if (mean_field)
+            check(hipMemcpyAsync(mean, pf_mean_.p, 2 * spec_ * sizeof(double),
                                  hipMemcpyDeviceToHost, stream_),
                   "filter mean");
-        }
         check(hipStreamSynchronize(stream_), "particle filter");
         for (size_t f = 0; f < pf_filters_; ++f)
             if (record)
                 for (int k = 0; k < 4; ++k)
                     record[4 * f + k] = st[8 * f + k];
         if (mean_field)
-            half_to_full<double>(n_, reinterpret_cast<const cx<double> *>(mean.data()),
-                                 mean_field);
+            half_to_full<double>(n_, reinterpret_cast<const cx<double> *>(mean), mean_field);
         ++pf_group_;
         // the children's keys were reseeded on the device; d_keys_ is the
         // authoritative copy (keys_ only stages whole or single-member uploads)
@@ -2254,6 +2260,7 @@ template <typename R> class Engine final : public EngineBase {
     DeviceBuffer<uint64_t> pf_seeds_;
     size_t obs_cap_ = 0;
     double *obs_host_ = nullptr; // pinned staging of one observation group
+    double *pf_host_ = nullptr;  // pinned staging of the filter statistics and mean
     DeviceBuffer<double> obs_dev_;
     hipEvent_t obs_event_ = nullptr;
     const double *obs_v_ = nullptr, *obs_sd_ = nullptr;

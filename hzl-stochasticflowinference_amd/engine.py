@@ -471,7 +471,8 @@ class Ensemble:
         value = np.ascontiguousarray(value, dtype=np.float64)
         sd = np.ascontiguousarray(sd, dtype=np.float64)
         rec = np.zeros(4 * getattr(self, "_filters", 1))
-        mf = np.zeros((self.n, self.n), dtype=np.complex128) if mean else None
+        # every entry is written (retained modes, their conjugates, zeros elsewhere)
+        mf = np.empty((self.n, self.n), dtype=np.complex128) if mean else None
         ip = C.POINTER(C.c_int)
         _check(_lib.hzl_pf_assimilate(self._h, dt, steps, len(x), x.ctypes.data_as(ip),
                                       y.ctypes.data_as(ip), value.ctypes.data_as(_dp),
