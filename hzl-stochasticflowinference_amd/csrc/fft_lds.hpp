@@ -114,6 +114,38 @@ template <int L, bool INV, typename R> HZL_HD cx<R> twiddle_const(int k, cx<R> a
     return {a.x * c + a.y * s, a.y * c - a.x * s};
 }
 
+// hi = e + a W_L^k, lo = e - a W_L^k.  A general twiddle is factored as
+// f (1, g) with f = cos, g = tan (or f = sin, g = cot when |sin| > |cos|), so
+// the twiddle and the butterfly take six fused multiply-adds, u = a (1 -/+ i g)
+// then e +/- f u, instead of a complex product and four additions.
+template <int L, bool INV, typename R>
+HZL_HD void butterfly_const(int k, cx<R> e, cx<R> a, cx<R> &hi, cx<R> &lo) {
+    const int m = k * (64 / L);
+    const bool general = k != 0 && 4 * k != L && 8 * k != L && 8 * k != 3 * L;
+    if (!general) {
+        const cx<R> t = twiddle_const<L, INV>(k, a);
+        hi = cadd(e, t);
+        lo = csub(e, t);
+        return;
+    }
+    const double c = cos64(m), s = cos64(m - 16); // W = c -/+ i s
+    cx<R> u;
+    R f;
+    if (c * c >= s * s) {
+        // a W = c a (1 -/+ i tan): forward (a.x + t a.y, a.y - t a.x)
+        const R t = static_cast<R>(s / c);
+        f = static_cast<R>(c);
+        u = INV ? cx<R>{a.x - t * a.y, a.y + t * a.x} : cx<R>{a.x + t * a.y, a.y - t * a.x};
+    } else {
+        // a W = s a (cot -/+ i): forward (cot a.x + a.y, cot a.y - a.x)
+        const R t = static_cast<R>(c / s);
+        f = static_cast<R>(s);
+        u = INV ? cx<R>{t * a.x - a.y, t * a.y + a.x} : cx<R>{t * a.x + a.y, t * a.y - a.x};
+    }
+    hi = {e.x + f * u.x, e.y + f * u.y};
+    lo = {e.x - f * u.x, e.y - f * u.y};
+}
+
 // In-register DFT of L values, natural order in and out.
 template <int L, bool INV, typename R> HZL_HD void dft(cx<R> *v) {
     if constexpr (L == 1) {
@@ -139,11 +171,8 @@ template <int L, bool INV, typename R> HZL_HD void dft(cx<R> *v) {
         dft<L / 2, INV>(e);
         dft<L / 2, INV>(o);
 #pragma unroll
-        for (int k = 0; k < L / 2; ++k) {
-            const cx<R> t = twiddle_const<L, INV>(k, o[k]);
-            v[k] = cadd(e[k], t);
-            v[k + L / 2] = csub(e[k], t);
-        }
+        for (int k = 0; k < L / 2; ++k)
+            butterfly_const<L, INV>(k, e[k], o[k], v[k], v[k + L / 2]);
     }
 }
 
