@@ -364,6 +364,34 @@ def test_async_judge_rejects_unusable_judge_ensembles(pkg, orc):
         assert sur.judge_poll(wait=True) == (False, -1.0)
 
 
+def test_async_judge_outlives_closed_handles(pkg, orc):
+    """A judge in flight when its judge ensemble is closed still folds in (the
+    ensemble's close waits for its stream; the sums live in the surrogate's
+    host memory), and closing the surrogate with a judge in flight waits for
+    it: the ensembles stay usable and step on."""
+    cfg = SMALL
+    s = shape_of(cfg)
+    B, J, dt, steps = 16, 6, 2e-3, 25
+    f = grf_fields(orc, s.n, B, 75)
+    with pkg.Ensemble(s.n, B, "fp32", sigma=0.05, amp=0.1, seed=6) as ens:
+        ens.set_state(f)
+        sur = pkg.Surrogate(**cfg, seed=9)
+        sur.set_scales(1.5, 4.0)
+        judge = pkg.Ensemble(s.n, J, "fp32", sigma=0.0, amp=0.1, seed=7)
+        sur.judge_start(ens, judge, dt, steps, "fp32")
+        judge.close()
+        finished, v = sur.judge_poll(wait=True)
+        assert finished and np.isfinite(v) and v > 0
+        with pkg.Ensemble(s.n, J, "fp32", sigma=0.0, amp=0.1, seed=8) as judge2:
+            sur.judge_start(ens, judge2, dt, steps, "fp32")
+            sur.close()
+            judge2.step(dt, 2)
+            judge2.synchronize()
+        ens.step(dt, 2)
+        ens.synchronize()
+        assert np.all(np.isfinite(ens.get_state()))
+
+
 @pytest.mark.parametrize("B", [1, 100, 1024])
 def test_spread_over_member_chunks(pkg, orc, B):
     """The spread sums members in chunks of 32 (a ragged last chunk at 100)
