@@ -123,26 +123,25 @@ __global__ void __launch_bounds__(256) k_pf_mean_finish(long spec, int chunks,
 
 // Resampling copies, a device-side no-op for every filter of P particles that
 // kept its particles this group (stats[8 f + 3] == 0): gather the ancestors'
-// states into dst, then commit dst back to the live states.
+// states into dst, then commit dst back to the live states.  Grid (members,
+// ceil(spec / 256)): member m = blockIdx.x, so the filter's resampling flag
+// is one value per workgroup.
 template <typename R>
 __global__ void __launch_bounds__(256) k_pf_gather(int B, long spec, const cx<R> *src, cx<R> *dst,
                                                    const int *anc, const double *stats, int P) {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (long)B * spec)
+    const long m = blockIdx.x, e = (long)blockIdx.y * 256 + threadIdx.x;
+    if (m >= B || e >= spec || stats[8 * (m / P) + 3] == 0)
         return;
-    const long m = i / spec;
-    if (stats[8 * (m / P) + 3] != 0)
-        dst[i] = src[(long)anc[m] * spec + i % spec];
+    dst[m * spec + e] = src[(long)anc[m] * spec + e];
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_pf_commit(int B, long spec, const cx<R> *src, cx<R> *dst,
                                                    const double *stats, int P) {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (long)B * spec)
+    const long m = blockIdx.x, e = (long)blockIdx.y * 256 + threadIdx.x;
+    if (m >= B || e >= spec || stats[8 * (m / P) + 3] == 0)
         return;
-    if (stats[8 * ((i / spec) / P) + 3] != 0)
-        dst[i] = src[i];
+    dst[m * spec + e] = src[m * spec + e];
 }
 
 // ----------------------------------------------- adjoint likelihood ----
@@ -2009,19 +2008,17 @@ template <typename R> class Engine final : public EngineBase {
             hipLaunchKernelGGL(k_pf_mean_finish, dim3((unsigned)blocks), dim3(256), 0, stream_,
                                (long)spec_, chunks, (const double *)pf_mean_part_.p, pf_mean_.p);
         }
-        const long total = (long)B_ * (long)spec_;
-        const dim3 grid((unsigned)((total + 255) / 256));
-        hipLaunchKernelGGL(k_pf_gather<R>, grid, dim3(256), 0, stream_, (int)B_, (long)spec_,
+        const dim3 mgrid((unsigned)B_, (unsigned)((spec_ + 255) / 256));
+        hipLaunchKernelGGL(k_pf_gather<R>, mgrid, dim3(256), 0, stream_, (int)B_, (long)spec_,
                            (const cx<R> *)w0_.p, w1_.p, (const int *)pf_anc_.p,
                            (const double *)pf_stats_.p, P);
-        hipLaunchKernelGGL(k_pf_commit<R>, grid, dim3(256), 0, stream_, (int)B_, (long)spec_,
+        hipLaunchKernelGGL(k_pf_commit<R>, mgrid, dim3(256), 0, stream_, (int)B_, (long)spec_,
                            (const cx<R> *)w1_.p, w0_.p, (const double *)pf_stats_.p, P);
         if (ou_) {
             // children keep their ancestor's OU coefficients and restart their
             // stream from the key k_pf_normalize gave them (Solver copy +
             // reseed_future, src/filter.cpp:112-115)
-            const long fc = (long)B_ * npairs_;
-            const dim3 fgrid((unsigned)((fc + 255) / 256));
+            const dim3 fgrid((unsigned)B_, (unsigned)((npairs_ + 255) / 256));
             hipLaunchKernelGGL(k_pf_gather<double>, fgrid, dim3(256), 0, stream_, (int)B_,
                                (long)npairs_, (const cx<double> *)ou_F_.p, ou_tmp_.p,
                                (const int *)pf_anc_.p, (const double *)pf_stats_.p, P);
