@@ -1368,16 +1368,18 @@ template <typename R> struct PfArgs {
 
 // Point value of member m's physical vorticity at grid (x, y) from the mixed
 // field: w(x, y) = Re w~(0, y) + 2 sum_{kx=1..K} Re(w~(kx, y) e^{2 pi i kx x / N}).
+// N is a power of two and 0 <= x < N (the observations are checked when
+// staged), so the table index kx x mod N is a mask of kx x < N^2.
 template <typename R>
 HZL_HD double point_value(const PfArgs<R> &a, const cx<R> *w, int x, int y) {
-    const int KP = a.K + 1;
+    const int KP = a.K + 1, mask = a.N - 1;
     double s = 0;
     // unrolled so that several terms' loads are in flight at once; the sum
     // keeps its order
 #pragma unroll 8
     for (int kx = 0; kx <= a.K; ++kx) {
         const cx<R> c = w[tiled(KP, kx, y)];
-        const int j = (int)(((long)kx * x) % a.N);
+        const int j = (kx * x) & mask;
         const double t = (double)c.x * a.cs[j] - (double)c.y * a.sn[j];
         s += kx == 0 ? t : 2 * t;
     }
