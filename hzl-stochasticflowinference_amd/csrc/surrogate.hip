@@ -217,13 +217,17 @@ __global__ void k_fill_f32(float *p, long n, float v) {
         p[i] = v;
 }
 // The noise counter is read from device memory so that a captured forecast
-// graph can be replayed with a new counter.
-__global__ void k_apply_increment(cx<float> *w0, const float *incr, int S, int K, int KS,
-                                  float scale, const double *nu, const double *alpha,
-                                  const double *sigma, const uint64_t *keys, const int *pair_of,
-                                  int kmin, int kmax, double horizon, const uint64_t *counter) {
+// graph can be replayed with a new counter.  Grid (samples, ceil(modes /
+// 256)): the sample is the workgroup's x index.
+__global__ void __launch_bounds__(256) k_apply_increment(cx<float> *w0, const float *incr, int S,
+                                                         int K, int KS, float scale,
+                                                         const double *nu, const double *alpha,
+                                                         const double *sigma, const uint64_t *keys,
+                                                         const int *pair_of, int kmin, int kmax,
+                                                         double horizon, const uint64_t *counter) {
     apply_increment_body(w0, incr, S, K, KS, scale, nu, alpha, sigma, keys, pair_of, kmin, kmax,
-                         horizon, counter[0], HZL_GID);
+                         horizon, counter[0], (long)blockIdx.x,
+                         (int)(blockIdx.y * 256 + threadIdx.x));
 }
 __global__ void k_set_u64(uint64_t *p, uint64_t v) {
     if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -907,8 +911,8 @@ class Surrogate {
                 forward(wf_, w0, ev.nu + s0, ev.amp + s0, S, false);
                 grid_to_increment(wf_.out.p, S);
             }
-            const long cnt = (long)S * s_.spec();
-            hipLaunchKernelGGL(k_apply_increment, grid1d(cnt), dim3(256), 0, st_, w0, incr_.p, S,
+            const dim3 agrid((unsigned)S, (unsigned)((s_.spec() + 255) / 256));
+            hipLaunchKernelGGL(k_apply_increment, agrid, dim3(256), 0, st_, w0, incr_.p, S,
                                s_.K, s_.KS, (float)(1.0 / out_scale_), ev.nu + s0, ev.alpha + s0,
                                ev.sigma + s0, ev.keys + s0, ev.pair_of, ev.kmin, ev.kmax,
                                (double)steps * dt, counter_dev_.p);

@@ -184,16 +184,15 @@ HZL_HD void diff_states_body(const cx<float> *a, const cx<float> *b, cx<float> *
 HZL_HD void apply_increment_body(cx<float> *w0, const float *incr, int S, int K, int KS,
                                  float scale, const double *nu, const double *alpha,
                                  const double *sigma, const uint64_t *keys, const int *pair_of,
-                                 int kmin, int kmax, double horizon, uint64_t counter, long i) {
-    const int J = 2 * K + 1, JS = 2 * KS + 1;
-    const long per = (long)(K + 1) * J;
-    if (i >= (long)S * per)
+                                 int kmin, int kmax, double horizon, uint64_t counter, long s,
+                                 int r) {
+    const int J = 2 * K + 1, JS = 2 * KS + 1, per = (K + 1) * J;
+    if (s >= S || r >= per)
         return;
-    const long s = i / per, r = i % per;
-    const int c = (int)(r / J), ky = (int)(r % J) - K;
+    const int c = r / J, ky = r % J - K;
     if (c == 0 && ky == 0)
         return;
-    cx<float> &w = w0[i];
+    cx<float> &w = w0[s * per + r];
     const int k2 = c * c + ky * ky;
     if (c > KS || ky > KS || ky < -KS) {
         const float d = (float)exp(-(nu[s] * k2 + alpha[s]) * horizon);
